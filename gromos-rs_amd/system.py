@@ -1,0 +1,133 @@
+"""Host-side description of one MD system: the topology essentials the nonbonded
+path reads, plus coordinates, box and run parameters.
+
+The fields mirror what gromos-rs keeps on `Topology` for this path
+(crates/gromos-core/src/topology.rs:619-712): flat per-atom arrays (charge,
+iac, mass), contiguous charge groups (`chargegroups`, `num_solute_chargegroups`),
+sorted symmetric exclusion lists (`exclusions`), the separate 1-4 lists
+(`one_four_pairs`), the LJ matrix (`lj_parameters`, stored flat as
+[ti * n_types + tj] x (c6, c12, cs6, cs12) like `LJParamMatrix`,
+crates/gromos-forces/src/nonbonded/params.rs:26-58), the solvent template size
+and its constraint lengths.  Lists are stored as CSR (offsets + indices) so they
+can be copied to the device as two flat arrays.
+"""
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+FOUR_PI_EPS_I = 138.9354          # crates/gromos-core/src/units.rs:47-53
+K_BOLTZMANN = 0.00831441          # crates/gromos-core/src/units.rs:42-46
+
+
+def csr_from_lists(lists, n):
+    off = np.zeros(n + 1, dtype=np.int32)
+    for i in range(min(n, len(lists))):
+        off[i + 1] = len(lists[i])
+    off = np.cumsum(off).astype(np.int32)
+    idx = np.zeros(int(off[-1]), dtype=np.int32)
+    for i in range(min(n, len(lists))):
+        if lists[i]:
+            idx[off[i]:off[i + 1]] = lists[i]
+    return off, idx
+
+
+@dataclass
+class Perturbation:
+    """Per-atom dual-topology data (PertAtomInfo, nonbonded/perturbed.rs:271-280);
+    alphas already multiplied by the global ALPHLJ / ALPHC
+    (crates/gromos-integrators/src/algorithms/forcefield.rs:175-199)."""
+    flag: np.ndarray          # u8 [n]
+    a_iac: np.ndarray         # i32 [n]
+    b_iac: np.ndarray
+    a_q: np.ndarray           # f64 [n]
+    b_q: np.ndarray
+    alpha_lj: np.ndarray
+    alpha_crf: np.ndarray
+    lam: float = 0.0
+    nlam: int = 1
+    atom_pairs: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
+
+    @staticmethod
+    def empty(n):
+        z = np.zeros(n)
+        return Perturbation(np.zeros(n, np.uint8), np.zeros(n, np.int32), np.zeros(n, np.int32),
+                            z.copy(), z.copy(), z.copy(), z.copy())
+
+
+@dataclass
+class System:
+    n_atoms: int
+    n_solute_atoms: int
+    cg_start: np.ndarray        # i32 [n_cg + 1]
+    n_solute_cg: int
+    charge: np.ndarray          # f64 [n]
+    iac: np.ndarray             # i32 [n], 0-based
+    mass: np.ndarray            # f64 [n]
+    n_types: int
+    lj: np.ndarray              # f64 [n_types * n_types * 4]
+    excl_off: np.ndarray        # i32 [n + 1]
+    excl: np.ndarray            # i32
+    o14_off: np.ndarray         # i32 [n + 1]
+    o14: np.ndarray             # i32
+    atoms_per_solvent: int
+    pos: np.ndarray             # f64 [n, 3]
+    vel: np.ndarray             # f64 [n, 3]
+    box: np.ndarray             # f64 [3]; zeros for vacuum
+    boundary: int = 1           # 0 vacuum, 1 rectangular
+    # run parameters (IMD PAIRLIST / NONBONDED / STEP blocks, crates/gromos-io/src/imd.rs:53-176)
+    rcutp: float = 1.4
+    rcutl: float = 1.4
+    skin: float = 0.0
+    nsnb: int = 5
+    algorithm: int = 0
+    epsrf: float = 61.0
+    kappa: float = 0.0
+    fpepsi: float = FOUR_PI_EPS_I
+    dt: float = 0.002
+    dist_oh: float = 0.1
+    dist_hh: float = 0.1632990
+    pert: Optional[Perturbation] = None
+    name: str = "system"
+
+    @property
+    def n_cg(self):
+        return len(self.cg_start) - 1
+
+    @property
+    def nsm(self):
+        if self.atoms_per_solvent == 0:
+            return 0
+        return (self.n_atoms - self.n_solute_atoms) // self.atoms_per_solvent
+
+    def copy(self):
+        import copy
+        return copy.deepcopy(self)
+
+    def validate(self):
+        n = self.n_atoms
+        assert self.pos.shape == (n, 3) and self.vel.shape == (n, 3)
+        assert self.charge.shape == (n,) and self.iac.shape == (n,) and self.mass.shape == (n,)
+        assert self.cg_start[0] == 0 and self.cg_start[-1] == n
+        assert np.all(np.diff(self.cg_start) > 0)
+        assert self.lj.shape == (self.n_types * self.n_types * 4,)
+        assert self.iac.min(initial=0) >= 0 and self.iac.max(initial=0) < self.n_types
+        assert len(self.excl_off) == n + 1 and len(self.o14_off) == n + 1
+        if self.boundary == 1:
+            assert np.all(self.box > 0)
+        if self.pert is not None:
+            p = self.pert
+            assert p.a_iac.max(initial=0) < self.n_types and p.b_iac.max(initial=0) < self.n_types
+        return self
+
+
+def lj_table_from_types(c6, c12, cs6=None, cs12=None):
+    """Flat (c6, c12, cs6, cs12) table from full per-pair matrices."""
+    c6 = np.asarray(c6, float)
+    c12 = np.asarray(c12, float)
+    cs6 = c6 if cs6 is None else np.asarray(cs6, float)
+    cs12 = c12 if cs12 is None else np.asarray(cs12, float)
+    nt = c6.shape[0]
+    t = np.zeros((nt, nt, 4))
+    t[..., 0], t[..., 1], t[..., 2], t[..., 3] = c6, c12, cs6, cs12
+    return t.reshape(-1)

@@ -1,0 +1,348 @@
+"""Extract small golden vectors from the gromosXX double-precision fixtures that
+gromos-rs ships (crates/gromos-md/tests/gromosXX_references/), as JSON.
+
+Run once in a container that has the reference checked out:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Only data is written: topology essentials (charges, types, charge groups,
+exclusions, 1-4 lists, the LJ table restricted to the types in use),
+positions / velocities / box, run parameters, and the expected gromosXX
+energies, forces and positions.  The parsing follows the reference's own
+readers so that the topology built here is the one gromos-rs builds:
+
+  * SOLUTEATOM: exclusions only from the atom line, 1-4 partners from the next
+    non-comment line (crates/gromos-io/src/topology.rs:270-370);
+  * IAC / type indices are stored 0-based (topology.rs:300-306, :738-745);
+  * bonds and angles add 1-2 / 1-3 exclusions, the 1-4 list stays separate
+    and is not mirrored (topology.rs:1141-1183); exclusion lists are sorted and
+    de-duplicated (topology.rs:1294-1297);
+  * charge groups close at CGC == 1 (topology.rs:1093-1110); every solvent
+    molecule is its own charge group and excludes all of its own atom pairs
+    (crates/gromos-core/src/topology.rs:1224-1263);
+  * perturbed alphas are multiplied by the global ALPHLJ / ALPHC
+    (crates/gromos-integrators/src/algorithms/forcefield.rs:175-199).
+
+The fixtures are gromosXX/gromos-rs data (GPL-2.0-or-later).
+"""
+import json
+import os
+import sys
+
+FIXDIR = "crates/gromos-md/tests/gromosXX_references"
+
+SYSTEMS = [
+    # name, topology, configuration, input, pttopo, frames of forces to keep
+    ("pair_lj", None, None, None, None, "all"),
+    ("pair_lj_mixed", None, None, None, None, "all"),
+    ("nacl_pair", None, None, None, None, "all"),
+    ("nacl_pair_box", None, None, None, None, "all"),
+    ("water_3_box", None, None, None, None, "all"),
+    ("water_3_box_twinrange", None, None, None, None, "all"),
+    ("water_10_box", None, None, None, None, "all"),
+    ("nacl_3water_box", None, None, None, None, "all"),
+    ("nacl_3water_cutoff", None, None, None, None, "all"),
+    ("nacl_water_box", None, None, None, None, "all"),
+    ("nacl_water_box_shifted", None, None, None, None, "all"),
+    ("nacl_1water_settle", None, None, None, None, "all"),
+    ("aladip_solvated", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
+    ("water_216_box", None, None, None, None, "first"),
+    ("water_1000_spc_gridcell", None, None, None, None, "first"),
+    ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
+     "../shared/ch4_spc_dummy.ptp", "first"),
+]
+
+
+def blocks(path):
+    """GROMOS block file -> {NAME: [lines]} (first occurrence) plus an ordered list."""
+    out, order = {}, []
+    name, buf = None, []
+    with open(path) as fh:
+        for raw in fh:
+            line = raw.rstrip("\n")
+            s = line.strip()
+            if name is None:
+                if s and not s.startswith("#"):
+                    name, buf = s, []
+                continue
+            if s == "END":
+                order.append((name, buf))
+                out.setdefault(name, buf)
+                name = None
+                continue
+            buf.append(line)
+    return out, order
+
+
+def data_lines(lines):
+    return [l for l in lines if l.strip() and not l.strip().startswith("#")]
+
+
+def parse_topology(path):
+    b, _ = blocks(path)
+    t = {}
+    fp = [float(x.split()[0]) for x in data_lines(b.get("PHYSICALCONSTANTS", []))]
+    t["fpepsi"] = fp[0] if fp else 138.9354
+    # SOLUTEATOM (topology.rs:270-370)
+    lines = data_lines(b.get("SOLUTEATOM", []))
+    n = int(lines[0].split()[0])
+    mass, q, iac, cgc, excl, o14 = [], [], [], [], [], []
+    k = 1
+    while len(mass) < n:
+        p = lines[k].split()
+        k += 1
+        iac.append(int(p[3]) - 1)
+        mass.append(float(p[4]))
+        q.append(float(p[5]))
+        cgc.append(int(p[6]))
+        ne = int(p[7])
+        excl.append([int(x) - 1 for x in p[8:8 + ne]])
+        p14 = lines[k].split()
+        k += 1
+        n14 = int(p14[0])
+        o14.append([int(x) - 1 for x in p14[1:1 + n14]])
+    t.update(n_solute=n, mass=mass, charge=q, iac=iac, cgc=cgc, excl=excl, o14=o14)
+    bonds, angles = [], []
+    for key in ("BONDH", "BOND"):
+        for l in data_lines(b.get(key, []))[1:]:
+            p = l.split()
+            bonds.append((int(p[0]) - 1, int(p[1]) - 1))
+    for key in ("BONDANGLEH", "BONDANGLE"):
+        for l in data_lines(b.get(key, []))[1:]:
+            p = l.split()
+            angles.append((int(p[0]) - 1, int(p[1]) - 1, int(p[2]) - 1))
+    t["bonds"], t["angles"] = bonds, angles
+    lj = {}
+    key = "LJPARAMETERS" if "LJPARAMETERS" in b else "CGPARAMETERS"
+    for l in data_lines(b.get(key, []))[1:]:
+        p = l.split()
+        i, j = int(p[0]) - 1, int(p[1]) - 1
+        c12, c6 = float(p[2]), float(p[3])
+        cs12, cs6 = (float(p[4]), float(p[5])) if len(p) >= 6 else (c12, c6)
+        lj[(i, j)] = lj[(j, i)] = (c6, c12, cs6, cs12)
+    t["lj"] = lj
+    sv = data_lines(b.get("SOLVENTATOM", []))
+    solv = []
+    if sv:
+        for l in sv[1:1 + int(sv[0].split()[0])]:
+            p = l.split()
+            solv.append((int(p[2]) - 1, float(p[3]), float(p[4])))
+    t["solvent"] = solv
+    sc = data_lines(b.get("SOLVENTCONSTR", []))
+    cons = []
+    if sc:
+        for l in sc[1:1 + int(sc[0].split()[0])]:
+            p = l.split()
+            cons.append((int(p[0]) - 1, int(p[1]) - 1, float(p[2])))
+    t["solvent_constr"] = cons
+    return t
+
+
+def parse_conf(path):
+    b, _ = blocks(path)
+    pos = [[float(x) for x in l.split()[-3:]] for l in data_lines(b["POSITION"])]
+    vel = None
+    if "VELOCITY" in b:
+        vel = [[float(x) for x in l.split()[-3:]] for l in data_lines(b["VELOCITY"])]
+    box = [0.0, 0.0, 0.0]
+    ntb = 0
+    if "GENBOX" in b:
+        g = data_lines(b["GENBOX"])
+        ntb = int(g[0].split()[0])
+        box = [float(x) for x in g[1].split()[:3]]
+    elif "BOX" in b:
+        box = [float(x) for x in data_lines(b["BOX"])[0].split()[:3]]
+        ntb = 1
+    return pos, vel, box, ntb
+
+
+def parse_imd(path):
+    b, _ = blocks(path)
+    d = {}
+    s = data_lines(b["SYSTEM"])[0].split()
+    d["npm"], d["nsm"] = int(s[0]), int(s[1])
+    st = data_lines(b["STEP"])[0].split()
+    d["nstlim"], d["dt"] = int(st[0]), float(st[2])
+    bc = data_lines(b["BOUNDCOND"])[0].split()
+    d["ntb"] = int(bc[0])
+    pl = data_lines(b["PAIRLIST"])[0].split()
+    alg = pl[0]
+    d["algorithm"] = {"standard": 0, "0": 0, "grid": 1, "1": 1, "grid_cell": 2, "2": 2}.get(alg, 0)
+    d["nsnb"], d["rcutp"], d["rcutl"] = int(pl[1]), float(pl[2]), float(pl[3])
+    nb = data_lines(b["NONBONDED"])
+    p = nb[1].split()
+    d["appak"], d["rcrf"], d["epsrf"] = float(p[0]), float(p[1]), float(p[2])
+    cons = data_lines(b.get("CONSTRAINT", []))
+    d["ntcs"] = cons[3].split()[0] if len(cons) > 3 else "shake"
+    if "PERTURBATION" in b:
+        pr = data_lines(b["PERTURBATION"])
+        a = pr[0].split()
+        c = pr[1].split()
+        d["ntg"], d["rlam"] = int(a[0]), float(a[2])
+        d["alphlj"], d["alphc"], d["nlam"] = float(c[0]), float(c[1]), int(c[2])
+    return d
+
+
+def parse_ptp(path):
+    b, _ = blocks(path)
+    lines = data_lines(b["PERTATOMPARAM"])
+    n = int(lines[0].split()[0])
+    atoms = []
+    for l in lines[1:1 + n]:
+        p = l.split()
+        atoms.append(dict(seq=int(p[0]) - 1, a_iac=int(p[3]) - 1, a_q=float(p[5]),
+                          b_iac=int(p[6]) - 1, b_q=float(p[8]), alj=float(p[9]), acrf=float(p[10])))
+    pairs = []
+    pl = data_lines(b.get("PERTATOMPAIR", []))
+    if pl:
+        for l in pl[1:1 + int(pl[0].split()[0])]:
+            p = l.split()
+            bt = int(p[3]) - 1
+            pairs.append([int(p[0]) - 1, int(p[1]) - 1, int(p[2]) - 1, bt if bt >= 0 else -1])
+    return atoms, pairs
+
+
+def parse_frames(path, block):
+    out = []
+    _, order = blocks(path)
+    step = None
+    for name, lines in order:
+        if name == "TIMESTEP":
+            step = int(data_lines(lines)[0].split()[0])
+        elif name == block:
+            out.append((step, lines))
+    return out
+
+
+def energies(path):
+    frames = []
+    for step, lines in parse_frames(path, "ENERGY03"):
+        vals = []
+        for l in lines:
+            s = l.strip()
+            if s.startswith("# baths"):
+                break
+            if s and not s.startswith("#"):
+                vals.append(float(s.split()[0]))
+        frames.append(dict(step=step, e_tot=vals[0], e_kin=vals[1], e_pot=vals[2],
+                           e_nonbonded=vals[9], e_lj=vals[10], e_crf=vals[11]))
+    return frames
+
+
+def forces(path):
+    out = []
+    for step, lines in parse_frames(path, "FREEFORCERED"):
+        f = [float(x) for l in data_lines(lines) for x in l.split()[:3]]
+        out.append(dict(step=step, f=f))
+    return out
+
+
+def positions(path):
+    out = []
+    for step, lines in parse_frames(path, "POSITIONRED"):
+        out.append(dict(step=step, x=[float(x) for l in data_lines(lines) for x in l.split()[:3]]))
+    return out
+
+
+def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
+    d = os.path.join(ref, FIXDIR, name)
+    files = os.listdir(d)
+    topo_p = os.path.join(d, topo_rel) if topo_rel else os.path.join(
+        d, [f for f in files if f.endswith((".topo", ".top"))][0])
+    conf_p = os.path.join(d, conf_rel) if conf_rel else os.path.join(
+        d, [f for f in files if f.endswith((".conf", ".cnf"))][0])
+    imd_p = os.path.join(d, imd_rel) if imd_rel else os.path.join(
+        d, [f for f in files if f.endswith(".in")][0])
+    t = parse_topology(topo_p)
+    imd = parse_imd(imd_p)
+    pos, vel, box, ntb_conf = parse_conf(conf_p)
+    nsm = imd["nsm"]
+    ns = t["n_solute"]
+    aps = len(t["solvent"])
+    n = ns + nsm * aps
+    assert len(pos) == n, (name, len(pos), n)
+    # charge groups (topology.rs:1093-1110) + solvent molecules
+    cg_start, cur = [0], 0
+    for i in range(ns):
+        if t["cgc"][i] == 1:
+            cg_start.append(i + 1)
+    if cg_start[-1] != ns:
+        cg_start.append(ns)
+    n_solute_cg = len(cg_start) - 1
+    for m in range(nsm):
+        cg_start.append(ns + (m + 1) * aps)
+    # exclusions (topology.rs:1141-1183, 1294-1297; core topology.rs:1242-1263)
+    ex = [set() for _ in range(n)]
+    for i, lst in enumerate(t["excl"]):
+        for j in lst:
+            ex[i].add(j)
+            ex[j].add(i)
+    for i, j in t["bonds"]:
+        ex[i].add(j); ex[j].add(i)
+    for i, _, k in t["angles"]:
+        ex[i].add(k); ex[k].add(i)
+    for m in range(nsm):
+        base = ns + m * aps
+        for a in range(aps):
+            for b2 in range(a + 1, aps):
+                ex[base + a].add(base + b2); ex[base + b2].add(base + a)
+    excl = [sorted(s) for s in ex]
+    o14 = [list(l) for l in t["o14"]] + [[] for _ in range(nsm * aps)]
+    iac = list(t["iac"]) + [s[0] for _ in range(nsm) for s in t["solvent"]]
+    mass = list(t["mass"]) + [s[1] for _ in range(nsm) for s in t["solvent"]]
+    charge = list(t["charge"]) + [s[2] for _ in range(nsm) for s in t["solvent"]]
+    pert = None
+    if ptp_rel:
+        atoms, apairs = parse_ptp(os.path.join(d, ptp_rel))
+        pert = dict(atoms=atoms, atom_pairs=apairs, rlam=imd.get("rlam", 0.0),
+                    nlam=imd.get("nlam", 1), alphlj=imd.get("alphlj", 1.0), alphc=imd.get("alphc", 1.0))
+    used = sorted(set(iac) | ({a["a_iac"] for a in pert["atoms"]} | {a["b_iac"] for a in pert["atoms"]}
+                              if pert else set()))
+    remap = {old: k for k, old in enumerate(used)}
+    nt = len(used)
+    lj = []
+    for a in used:
+        for b2 in used:
+            lj.extend(t["lj"].get((a, b2), (0.0, 0.0, 0.0, 0.0)))
+    if pert:
+        for a in pert["atoms"]:
+            a["a_iac"], a["b_iac"] = remap[a["a_iac"]], remap[a["b_iac"]]
+    ntb = imd["ntb"]
+    fx = dict(
+        name=name, source=os.path.join(FIXDIR, name),
+        n_atoms=n, n_solute_atoms=ns, n_cg=len(cg_start) - 1, n_solute_cg=n_solute_cg,
+        atoms_per_solvent=aps, nsm=nsm, cg_start=cg_start,
+        charge=charge, iac=[remap[x] for x in iac], mass=mass,
+        n_types=nt, lj=lj, excl=excl, o14=o14,
+        pos=[x for p in pos for x in p], vel=[x for v in vel for x in v] if vel else None,
+        box=box if ntb != 0 else [0.0, 0.0, 0.0], ntb=1 if ntb != 0 else 0,
+        fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
+        algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
+        dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"],
+        solvent_constr=t["solvent_constr"], pert=pert,
+        has_bonded_terms=bool(t["bonds"] or t["angles"]),
+    )
+    ex_dir = os.path.join(d, "expected")
+    fx["expected_energies"] = energies(os.path.join(ex_dir, "energies.tre"))
+    fr = forces(os.path.join(ex_dir, "forces.trf"))
+    fx["expected_forces"] = fr if keep == "all" else fr[:1]
+    trc = os.path.join(ex_dir, "trajectory.trc")
+    if keep == "all" and os.path.exists(trc):
+        fx["expected_positions"] = positions(trc)
+    return fx
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    here = os.path.dirname(os.path.abspath(__file__))
+    for spec in SYSTEMS:
+        fx = build(ref, *spec)
+        out = os.path.join(here, spec[0] + ".json")
+        with open(out, "w") as fh:
+            json.dump(fx, fh, separators=(",", ":"))
+        print(f"{spec[0]}: {fx['n_atoms']} atoms, {len(fx['expected_energies'])} energy frames "
+              f"-> {os.path.getsize(out) // 1024} KiB")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,294 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+* gromosXX double-precision fixtures shipped by gromos-rs (tests/golden/*.json,
+  written by tests/golden/make_fixtures.py): single-point LJ / CRF energies and
+  step-0 forces; a 10-step SETTLE run frame by frame.
+* closed forms and known answers from the reference's own unit tests.
+* an independent pure-Python restatement of the charge-group pair set on
+  small inputs, and the reference's standard == cell-list set invariant.
+* invariants: Newton's third law, translation by a box vector, finite
+  differences of energy vs force and of energy vs lambda (the soft-core CRF /
+  dH/dlambda terms have no fixture: those checks are diagnostic only).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import golden_io
+import oracle
+
+GOLDEN = golden_io.names()
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_fixture_single_point(name):
+    s, d = golden_io.load(name)
+    r = oracle.single_point(oracle.OracleSystem(s))
+    e0 = d["expected_energies"][0]
+    # energies are printed with 10 significant figures (.tre ENERGY03)
+    for key in ("e_lj", "e_crf"):
+        if e0[key] == 0.0:
+            assert abs(r[key]) < 1e-12
+        else:
+            assert rel(r[key], e0[key]) < 6e-10, (key, r[key], e0[key])
+    if not d["has_bonded_terms"]:
+        # forces are printed with 9 decimals (.trf FREEFORCERED)
+        f0 = np.array(d["expected_forces"][0]["f"]).reshape(-1, 3)
+        assert np.abs(r["forces"] - f0).max() < 1.0e-9
+
+
+def test_settle_md_matches_gromosxx_frames():
+    s, d = golden_io.load("nacl_1water_settle")
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=True)
+    fr = d["expected_energies"]
+    e = md.run(len(fr))
+    for k, f in enumerate(fr):
+        assert rel(e[k, 2], f["e_pot"]) < 6e-10
+        assert rel(e[k, 3], f["e_kin"]) < 6e-10
+
+
+def test_settle_md_positions_match_gromosxx():
+    s, d = golden_io.load("nacl_1water_settle")
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=True)
+    frames = d["expected_positions"]
+    for k in range(1, len(frames)):
+        md.run(1)
+        x, _, _ = md.state()
+        assert np.abs(x.reshape(-1) - np.array(frames[k]["x"])).max() < 1e-9
+
+
+def test_lj_kat_unit_distance():
+    # innerloops.rs:874-892: at r = 1 nm, E_LJ = c12 - c6, force attractive
+    crf = oracle.Crf(1.4, 0.0, 0.0, 1.96)
+    f, elj, _ = oracle.lj_crf_interaction([1.0, 0.0, 0.0], 0.001, 0.0001, 0.0, crf)
+    assert elj == 0.0001 - 0.001
+    assert f < 0
+
+
+def test_crf_constants_closed_form():
+    rc = 1.4
+    c = oracle.crf_new(rc, 1.0, 61.0, 0.0)
+    crf = -120.0 / 123.0
+    assert math.isclose(c.crf_cut3i, crf / rc ** 3, rel_tol=1e-15)
+    assert math.isclose(c.crf_2cut3i, crf / (2 * rc ** 3), rel_tol=1e-15)
+    assert math.isclose(c.crf_cut, (1 - crf / 2) / rc, rel_tol=1e-15)
+    # epsilon_rf = 1 gives crf = 0 (plain Coulomb shifted by 1/rc)
+    c1 = oracle.crf_new(rc, 1.0, 1.0, 0.0)
+    assert c1.crf_cut3i == 0.0 and math.isclose(c1.crf_cut, 1 / rc)
+
+
+def test_argon_pair_closed_form():
+    # forcefield.rs:1210-1306: two Ar atoms at 0.35 nm against the analytic LJ
+    c6, c12, r = 0.0062647225, 9.847044e-06, 0.35
+    crf = oracle.Crf(0.0, 0.0, 0.0, 1.96)
+    f, elj, ecrf = oracle.lj_crf_interaction([r, 0.0, 0.0], c6, c12, 0.0, crf)
+    assert math.isclose(elj, c12 / r ** 12 - c6 / r ** 6, rel_tol=1e-13)
+    assert math.isclose(f * r, 12 * c12 / r ** 13 - 6 * c6 / r ** 7, rel_tol=1e-13)
+    assert ecrf == 0.0
+
+
+def test_pair_energy_force_finite_difference():
+    crf = oracle.crf_new(1.4, 1.0, 61.0, 0.0)
+    r0 = np.array([0.31, -0.12, 0.2])
+    qp = -0.82 * 0.41 * 138.9354
+    f, _, _ = oracle.lj_crf_interaction(r0, 2.6e-3, 2.6e-6, qp, crf)
+    h = 1e-6
+    for a in range(3):
+        rp, rm = r0.copy(), r0.copy()
+        rp[a] += h
+        rm[a] -= h
+        ep = sum(oracle.lj_crf_interaction(rp, 2.6e-3, 2.6e-6, qp, crf)[1:])
+        em = sum(oracle.lj_crf_interaction(rm, 2.6e-3, 2.6e-6, qp, crf)[1:])
+        assert math.isclose(-(ep - em) / (2 * h), f * r0[a], rel_tol=1e-6)
+
+
+def test_nearest_image_kat():
+    # math.rs:286-294: 9.5 and 0.5 in a 10 nm box are 1 nm apart through the wall
+    p = oracle.make_pbc(1, [10.0, 10.0, 10.0])
+    r = oracle.nearest_image(p, [9.5, 0, 0], [0.5, 0, 0])
+    assert r[0] == -1.0
+    # exact half: f64::round rounds half away from zero
+    r = oracle.nearest_image(p, [5.0, 0, 0], [0.0, 0, 0])
+    assert r[0] == -5.0
+    r = oracle.nearest_image(p, [0.0, 0, 0], [5.0, 0, 0])
+    assert r[0] == 5.0
+
+
+# ---------------------------------------------------------------------------
+# pair set: independent Python restatement on small inputs
+# ---------------------------------------------------------------------------
+def _round_half_away(x):
+    t = math.trunc(x)
+    if abs(x - t) >= 0.5:
+        t += math.copysign(1.0, x)
+    return t
+
+
+def _nim(box, a, b):
+    r = []
+    for k in range(3):
+        d = a[k] - b[k]
+        if box is not None:
+            d = d - box[k] * _round_half_away(d * (1.0 / box[k]))
+        r.append(d)
+    return r
+
+
+def python_pairlist(s):
+    """pairlist.rs:183-442 written again in plain Python (no shared code)."""
+    box = list(s.box) if s.boundary == 1 else None
+    cs = [int(x) for x in s.cg_start]
+    ncg, nsc = len(cs) - 1, s.n_solute_cg
+    pos = [list(map(float, p)) for p in s.pos]
+    excl = [set(s.excl[s.excl_off[i]:s.excl_off[i + 1]].tolist()) for i in range(s.n_atoms)]
+    o14 = [set(s.o14[s.o14_off[i]:s.o14_off[i + 1]].tolist()) for i in range(s.n_atoms)]
+
+    def exc(i, j):
+        return j in excl[i] or i in excl[j] or j in o14[i] or i in o14[j]
+
+    def ref(cg):
+        if cg < nsc:
+            acc = [0.0, 0.0, 0.0]
+            for a in range(cs[cg], cs[cg + 1]):
+                acc = [acc[k] + pos[a][k] for k in range(3)]
+            n = float(cs[cg + 1] - cs[cg])
+            return [acc[k] / n for k in range(3)]
+        return pos[cs[cg]]
+
+    c2s = (s.rcutp + s.skin) * (s.rcutp + s.skin)
+    c2l = (s.rcutl + s.skin) * (s.rcutl + s.skin)
+    out = {k: set() for k in ("solute_short", "solute_long", "solvent_short", "solvent_long")}
+    for cg in range(nsc):
+        for a in range(cs[cg], cs[cg + 1]):
+            for b in range(a + 1, cs[cg + 1]):
+                if not exc(a, b):
+                    out["solute_short"].add((a, b))
+    for c1 in range(ncg):
+        for c2 in range(c1 + 1, ncg):
+            r = _nim(box, ref(c1), ref(c2))
+            d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2]
+            if d2 > c2l:
+                continue
+            atoms = [(a, b) for a in range(cs[c1], cs[c1 + 1]) for b in range(cs[c2], cs[c2 + 1])]
+            if c2 < nsc:
+                if d2 > c2s:
+                    out["solute_long"].update(atoms)
+                else:
+                    out["solute_short"].update(p for p in atoms if not exc(*p))
+            elif c1 < nsc:
+                out["solute_long" if d2 > c2s else "solute_short"].update(atoms)
+            else:
+                if d2 > c2s:
+                    out["solvent_long"].update(atoms)
+                else:
+                    out["solvent_short"].add((cs[c1], cs[c2]))
+    return out
+
+
+def as_set(a):
+    return set(map(tuple, np.asarray(a).tolist()))
+
+
+@pytest.mark.parametrize("name", ["water_3_box_twinrange", "nacl_3water_cutoff", "aladip_solvated",
+                                  "nacl_water_box", "water_216_box"])
+def test_pairlist_matches_python_restatement(name):
+    s, _ = golden_io.load(name)
+    pl = oracle.pairlist(oracle.OracleSystem(s))
+    ref = python_pairlist(s)
+    for k in ref:
+        assert as_set(pl[k]) == ref[k], k
+        assert len(pl[k]) == len(ref[k]), k  # no duplicates
+
+
+def test_pairlist_twin_range_nonempty_long():
+    s, _ = golden_io.load("aladip_solvated")
+    pl = oracle.pairlist(oracle.OracleSystem(s))
+    assert len(pl["solute_long"]) > 0 and len(pl["solvent_long"]) > 0 and len(pl["solvent_short"]) > 0
+
+
+@pytest.mark.parametrize("name", ["water_1000_spc_gridcell", "ch4_water_fep", "aladip_solvated",
+                                  "water_216_box"])
+def test_pairlist_standard_equals_cell(name):
+    # the reference's own invariant (pairlist.rs:1121-1152)
+    s, _ = golden_io.load(name)
+    o = oracle.OracleSystem(s)
+    a = oracle.pairlist(o, algorithm=0)
+    b = oracle.pairlist(o, algorithm=2)
+    for k in a:
+        sa = set(map(tuple, np.sort(a[k], axis=1).tolist()))
+        sb = set(map(tuple, np.sort(b[k], axis=1).tolist()))
+        assert sa == sb, k
+
+
+def test_newton_third_law_and_translation():
+    s, _ = golden_io.load("water_1000_spc_gridcell")
+    o = oracle.OracleSystem(s)
+    r = oracle.single_point(o)
+    assert np.abs(r["forces"].sum(0)).max() < 1e-8
+    r2 = oracle.single_point(o, pos=s.pos + np.array(s.box) * np.array([1.0, -2.0, 3.0]))
+    assert rel(r2["e_lj"], r["e_lj"]) < 1e-9 and rel(r2["e_crf"], r["e_crf"]) < 1e-9
+
+
+def test_parallel_equals_serial():
+    s, _ = golden_io.load("water_1000_spc_gridcell")
+    o = oracle.OracleSystem(s)
+    a = oracle.single_point(o, parallel=False)
+    b = oracle.single_point(o, parallel=True)
+    assert rel(b["e_lj"], a["e_lj"]) < 1e-12 and rel(b["e_crf"], a["e_crf"]) < 1e-12
+    assert np.abs(a["forces"] - b["forces"]).max() < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# soft-core: limits and finite differences (diagnostic; parity unpinned)
+# ---------------------------------------------------------------------------
+def _sc(r, lam, aq=0.2, bq=-0.3, alj=0.5, acrf=0.5, n=1):
+    crf = oracle.crf_new(1.4, 1.0, 61.0, 0.0)
+    lp = oracle.lambda_params(lam, n)
+    return oracle.perturbed_interaction(r, 2e-3, 2e-6, 1e-3, 3e-6, aq, bq, alj, acrf, lp, crf, 138.9354)
+
+
+def test_softcore_lambda0_reduces_to_state_a_lj():
+    crf = oracle.crf_new(1.4, 1.0, 61.0, 0.0)
+    r = [0.3, 0.1, -0.2]
+    o = _sc(r, 0.0, aq=0.0, bq=0.0)
+    _, elj, _ = oracle.lj_crf_interaction(r, 2e-3, 2e-6, 0.0, crf)
+    # 1/r^6 is formed differently (1/(r2^3) vs (1/r2)^3): equal to rounding
+    assert rel(o[1], elj) < 1e-14
+
+
+def test_softcore_ab_symmetry():
+    crf = oracle.crf_new(1.4, 1.0, 61.0, 0.0)
+    r = [0.3, 0.1, -0.2]
+    for lam in (0.0, 0.25, 0.5, 0.8):
+        lp = oracle.lambda_params(lam)
+        lq = oracle.lambda_params(1 - lam)
+        a = oracle.perturbed_interaction(r, 2e-3, 2e-6, 1e-3, 3e-6, 0.2, -0.3, 0.5, 0.5, lp, crf, 138.9354)
+        b = oracle.perturbed_interaction(r, 1e-3, 3e-6, 2e-3, 2e-6, -0.3, 0.2, 0.5, 0.5, lq, crf, 138.9354)
+        assert np.allclose(a[:3], b[:3], rtol=1e-12)
+
+
+def test_softcore_force_finite_difference():
+    r0 = np.array([0.27, 0.05, -0.11])
+    f = _sc(r0, 0.4)[0]
+    h = 1e-7
+    for a in range(3):
+        rp, rm = r0.copy(), r0.copy()
+        rp[a] += h
+        rm[a] -= h
+        dE = (_sc(rp, 0.4)[1:3].sum() - _sc(rm, 0.4)[1:3].sum()) / (2 * h)
+        assert math.isclose(-dE, f * r0[a], rel_tol=1e-5)
+
+
+def test_softcore_dhdl_lj_finite_difference():
+    # dH/dlambda of the LJ part matches dE_LJ/dlambda (soft-core lambda moves too)
+    r0 = np.array([0.27, 0.05, -0.11])
+    h = 1e-6
+    lam = 0.4
+    d = _sc(r0, lam, aq=0.0, bq=0.0)[3]
+    fd = (_sc(r0, lam + h, aq=0.0, bq=0.0)[1] - _sc(r0, lam - h, aq=0.0, bq=0.0)[1]) / (2 * h)
+    assert math.isclose(d, fd, rel_tol=1e-5)

@@ -1,4 +1,13 @@
 """MI355X-native nonbonded hot path of gromos-rs: charge-group pairlist,
 LJ + reaction-field pair loops and perturbed soft-core pairs, behind a C ABI
-(include/gromos_amd.h) with HIP kernels for gfx950."""
+(include/gromos_amd.h) with HIP kernels for gfx950.
+
+Modules:
+  system      host description of a topology + coordinates (System)
+  generators  seeded synthetic systems (SPC water boxes, heterogeneous solute)
+  runtime     ctypes binding of libgromos_amd.so, DeviceSystem
+  nonbonded   mirror of the gromos-forces nonbonded functions (device tensors)
+  build       hipcc recipe of libgromos_amd.so
+"""
 from . import system  # noqa: F401
+from . import runtime  # noqa: F401

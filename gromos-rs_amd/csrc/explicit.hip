@@ -1,0 +1,260 @@
+// Pair-list kernels: GPU drop-ins for the gromos-forces free functions that
+// take an explicit pair list (crates/gromos-forces/src/nonbonded/
+// innerloops.rs:318-733, excluded.rs:25-206, perturbed.rs:309-386).
+//
+// One lane per listed pair (or per group / sentinel / atom).  Forces go to the
+// caller's f64 buffer with hardware FP64 atomics; energies and the virial are
+// summed per block in a fixed order and added once per block.
+#include "device_common.hpp"
+#include "explicit.hpp"
+
+namespace gamd {
+
+constexpr int EXPL_BLOCK = 256;
+
+__device__ __forceinline__ void atomic_add3(double* f, int64_t i, double x, double y, double z) {
+    atomicAdd(f + 3 * i, x);
+    atomicAdd(f + 3 * i + 1, y);
+    atomicAdd(f + 3 * i + 2, z);
+}
+
+// sums v[0..1] into energies and v[2..10] into virial (if non-null)
+__device__ __forceinline__ void flush_block(double (&v)[11], double* energies, double* virial) {
+    __shared__ double lds[17 * 11];
+    block_sum<11>(v, lds);
+    if (threadIdx.x < 2) atomicAdd(energies + threadIdx.x, lds[16 * 11 + threadIdx.x]);
+    if (virial && threadIdx.x >= 2 && threadIdx.x < 11)
+        atomicAdd(virial + threadIdx.x - 2, lds[16 * 11 + threadIdx.x]);
+}
+
+__device__ __forceinline__ void acc_pair(double (&v)[11], d3 r, double fm, double elj, double ecrf,
+                                         bool vir, double* forces, int64_t i, int64_t j) {
+    double fx = r.x * fm, fy = r.y * fm, fz = r.z * fm;
+    atomic_add3(forces, i, fx, fy, fz);
+    atomic_add3(forces, j, -fx, -fy, -fz);
+    v[0] += elj;
+    v[1] += ecrf;
+    if (vir) {
+        double rr[3] = {r.x, r.y, r.z}, ff[3] = {fx, fy, fz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) v[2 + 3 * a + b] += rr[a] * ff[b];
+    }
+}
+
+// process_pairs (innerloops.rs:169-210)
+__global__ void __launch_bounds__(EXPL_BLOCK) k_plain(ExplArgs a) {
+    double v[11] = {0};
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.n) {
+        int64_t i = a.pairs[2 * k], j = a.pairs[2 * k + 1];
+        d3 r = nim(a.box, ld3(a.pos, i), ld3(a.pos, j));
+        const double* p = a.lj + 4 * ((int64_t)a.iac[i] * a.n_types + a.iac[j]);
+        double qp = a.q[i] * a.q[j] * a.fpepsi;
+        double fm, elj, ecrf;
+        lj_crf64(r.x, r.y, r.z, p[0], p[1], qp, a.crf, fm, elj, ecrf);
+        acc_pair(v, r, fm, elj, ecrf, a.virial != nullptr, a.forces, i, j);
+    }
+    flush_block(v, a.energies, a.virial);
+}
+
+// process_pairs_cg_grouped (innerloops.rs:105-162): one lane per group
+__global__ void __launch_bounds__(EXPL_BLOCK) k_cg_grouped(ExplArgs a) {
+    double v[11] = {0};
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < a.n) {
+        const uint32_t* G = a.groups + 4 * g;
+        d3 ri = ld3(a.pos, G[0]), rj = ld3(a.pos, G[1]);
+        d3 rr = nim(a.box, ri, rj);
+        double tx = rr.x - ri.x + rj.x, ty = rr.y - ri.y + rj.y, tz = rr.z - ri.z + rj.z;
+        for (uint32_t k = G[2]; k < G[3]; ++k) {
+            int64_t i = a.pairs[2 * k], j = a.pairs[2 * k + 1];
+            d3 pi = ld3(a.pos, i), pj = ld3(a.pos, j);
+            d3 r{pi.x + tx - pj.x, pi.y + ty - pj.y, pi.z + tz - pj.z};
+            const double* p = a.lj + 4 * ((int64_t)a.iac[i] * a.n_types + a.iac[j]);
+            double qp = a.q[i] * a.q[j] * a.fpepsi;
+            double fm, elj, ecrf;
+            lj_crf64(r.x, r.y, r.z, p[0], p[1], qp, a.crf, fm, elj, ecrf);
+            acc_pair(v, r, fm, elj, ecrf, a.virial != nullptr, a.forces, i, j);
+        }
+    }
+    flush_block(v, a.energies, a.virial);
+}
+
+// process_solvent_pairs (innerloops.rs:218-309): tables from the first pair's molecule
+__global__ void __launch_bounds__(EXPL_BLOCK) k_solvent(ExplArgs a) {
+    double v[11] = {0};
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.aps;
+    if (k < a.n) {
+        const int64_t first = a.pairs[0];
+        int64_t i0 = a.pairs[2 * k], j0 = a.pairs[2 * k + 1];
+        d3 pi0 = ld3(a.pos, i0), pj0 = ld3(a.pos, j0);
+        d3 rf = nim(a.box, pi0, pj0);
+        double tx = rf.x - pi0.x + pj0.x, ty = rf.y - pi0.y + pj0.y, tz = rf.z - pi0.z + pj0.z;
+        for (int ai = 0; ai < n; ++ai) {
+            d3 pi = ld3(a.pos, i0 + ai);
+            double xi = pi.x + tx, yi = pi.y + ty, zi = pi.z + tz;
+            double qi = a.q[first + ai];
+            for (int aj = 0; aj < n; ++aj) {
+                d3 pj = ld3(a.pos, j0 + aj);
+                d3 r{xi - pj.x, yi - pj.y, zi - pj.z};
+                const double* p = a.lj + 4 * ((int64_t)a.iac[first + ai] * a.n_types + a.iac[first + aj]);
+                double qp = qi * a.q[first + aj] * a.fpepsi;
+                double fm, elj, ecrf;
+                lj_crf64(r.x, r.y, r.z, p[0], p[1], qp, a.crf, fm, elj, ecrf);
+                acc_pair(v, r, fm, elj, ecrf, a.virial != nullptr, a.forces, i0 + ai, j0 + aj);
+            }
+        }
+    }
+    flush_block(v, a.energies, a.virial);
+}
+
+// rf_excluded_interactions (excluded.rs:25-112): one lane per atom i, pairs j > i
+__global__ void __launch_bounds__(EXPL_BLOCK) k_rf_excluded(ExclArgs a) {
+    double v[11] = {0};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_atoms) {
+        double qi = a.q[i];
+        d3 pi = ld3(a.pos, i);
+        if (i < a.n_solute) {
+            v[1] += -0.5 * qi * qi * a.fpepsi * a.crf.crf_cut;
+            for (int k = a.off[i]; k < a.off[i + 1]; ++k) {
+                int64_t j = a.idx[k];
+                if (j <= i) continue;
+                d3 r = nim(a.box, pi, ld3(a.pos, j));
+                double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                double qp = qi * a.q[j] * a.fpepsi;
+                v[1] += qp * (-a.crf.crf_2cut3i * r2 - a.crf.crf_cut);
+                double fs = qp * a.crf.crf_cut3i;
+                double fx = r.x * fs, fy = r.y * fs, fz = r.z * fs;
+                atomic_add3(a.forces, i, fx, fy, fz);
+                atomic_add3(a.forces, j, -fx, -fy, -fz);
+                double rr[3] = {r.x, r.y, r.z}, ff[3] = {fx, fy, fz};
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) v[2 + 3 * x + y] += rr[x] * ff[y];
+            }
+        } else {
+            for (int k = a.off[i]; k < a.off[i + 1]; ++k) {
+                int64_t j = a.idx[k];
+                if (j <= i) continue;
+                d3 r = nim(a.box, pi, ld3(a.pos, j));
+                double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                v[1] += -qi * a.q[j] * a.fpepsi * a.crf.crf_2cut3i * r2;
+            }
+        }
+    }
+    flush_block(v, a.energies, a.virial);
+}
+
+// one_four_interaction_loop (excluded.rs:131-206)
+__global__ void __launch_bounds__(EXPL_BLOCK) k_one_four(ExclArgs a) {
+    double v[11] = {0};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_atoms) {
+        d3 pi = ld3(a.pos, i);
+        for (int k = a.off[i]; k < a.off[i + 1]; ++k) {
+            int64_t j = a.idx[k];
+            if (j <= i) continue;
+            d3 r = nim(a.box, pi, ld3(a.pos, j));
+            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            if (r2 < 1e-10) continue;
+            const double* p = a.lj + 4 * ((int64_t)a.iac[i] * a.n_types + a.iac[j]);
+            double inv_r2 = 1.0 / r2;
+            double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+            double elj = (p[3] * inv_r6 - p[2]) * inv_r6;
+            double flj = (12.0 * p[3] * inv_r6 - 6.0 * p[2]) * inv_r6 * inv_r2;
+            double qp = a.q[i] * a.q[j] * a.fpepsi;
+            double inv_r = sqrt(inv_r2);
+            double ecrf = qp * (inv_r * a.coulomb_scaling - a.crf.crf_2cut3i * r2 - a.crf.crf_cut);
+            double fcrf = qp * (inv_r * a.coulomb_scaling * inv_r2 + a.crf.crf_cut3i);
+            double ft = flj + fcrf;
+            double fx = r.x * ft, fy = r.y * ft, fz = r.z * ft;
+            atomic_add3(a.forces, i, fx, fy, fz);
+            atomic_add3(a.forces, j, -fx, -fy, -fz);
+            v[0] += elj;
+            v[1] += ecrf;
+            double rr[3] = {r.x, r.y, r.z}, ff[3] = {fx, fy, fz};
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) v[2 + 3 * x + y] += rr[x] * ff[y];
+        }
+    }
+    flush_block(v, a.energies, a.virial);
+}
+
+// perturbed_pairlist_correction (perturbed.rs:309-386)
+__global__ void __launch_bounds__(EXPL_BLOCK) k_pert_pairs(PertArgs a) {
+    __shared__ double lds[17 * 3];
+    double v[3] = {0, 0, 0};
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.n) {
+        int64_t i = a.pairs[2 * k], j = a.pairs[2 * k + 1];
+        if (a.flag[i]) {
+            bool pj = a.flag[j] != 0;
+            int a_iac_i = a.a_iac[i], b_iac_i = a.b_iac[i];
+            int a_iac_j = pj ? a.a_iac[j] : (int)a.iac[j], b_iac_j = pj ? a.b_iac[j] : (int)a.iac[j];
+            double a_q = a.a_q[i] * (pj ? a.a_q[j] : a.q[j]);
+            double b_q = a.b_q[i] * (pj ? a.b_q[j] : a.q[j]);
+            double alj, acrf;
+            if (pj) {
+                alj = (a.alpha_lj[i] + a.alpha_lj[j]) * 0.5;
+                acrf = (a.alpha_crf[i] + a.alpha_crf[j]) * 0.5;
+            } else {
+                alj = a.alpha_lj[i];
+                acrf = a.alpha_crf[i];
+            }
+            const double* la = a.lj + 4 * ((int64_t)a_iac_i * a.n_types + a_iac_j);
+            const double* lb = a.lj + 4 * ((int64_t)b_iac_i * a.n_types + b_iac_j);
+            d3 r = nim(a.box, ld3(a.pos, i), ld3(a.pos, j));
+            double fa, elja, ecrfa, o[5];
+            lj_crf64(r.x, r.y, r.z, la[0], la[1], a_q * a.fpepsi, a.crf, fa, elja, ecrfa);
+            pert_lj_crf64(r.x, r.y, r.z, la[0], la[1], lb[0], lb[1], a_q, b_q, alj, acrf, a.lp,
+                          a.crf, a.fpepsi, o);
+            double d = o[0] - fa;
+            double fx = r.x * d, fy = r.y * d, fz = r.z * d;
+            atomic_add3(a.forces, i, fx, fy, fz);
+            atomic_add3(a.forces, j, -fx, -fy, -fz);
+            v[0] = o[1] - elja;
+            v[1] = o[2] - ecrfa;
+            v[2] = o[3] + o[4];
+        }
+    }
+    block_sum<3>(v, lds);
+    if (threadIdx.x < 3) atomicAdd(a.out + threadIdx.x, lds[16 * 3 + threadIdx.x]);
+}
+
+static int blocks_for(int64_t n) { return (int)((n + EXPL_BLOCK - 1) / EXPL_BLOCK); }
+
+hipError_t launch_plain(const ExplArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plain, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_cg_grouped(const ExplArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cg_grouped, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_solvent(const ExplArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_solvent, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_rf_excluded(const ExclArgs& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rf_excluded, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_one_four(const ExclArgs& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_one_four, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_pert_pairs(const PertArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pert_pairs, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gamd

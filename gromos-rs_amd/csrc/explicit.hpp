@@ -1,0 +1,74 @@
+// Argument blocks of the pair-list kernels (explicit.hip).
+#pragma once
+#include "device_common.hpp"
+
+namespace gamd {
+
+struct ExplArgs {
+    const double* pos;
+    const double* q;
+    const uint32_t* iac;
+    const uint32_t* pairs;   // 2 per pair (plain, solvent) or the flat pair array (cg_grouped)
+    const uint32_t* groups;  // cg_grouped only
+    int64_t n;               // pairs, sentinels or groups
+    const double* lj;
+    int32_t n_types;
+    int32_t aps;
+    Crf crf;
+    Box box;
+    double fpepsi;
+    double* forces;
+    double* energies;
+    double* virial;          // null: no virial
+};
+
+struct ExclArgs {
+    const int32_t* off;
+    const int32_t* idx;
+    int32_t n_atoms;
+    int32_t n_solute;
+    const double* pos;
+    const double* q;
+    const uint32_t* iac;
+    const double* lj;
+    int32_t n_types;
+    Crf crf;
+    Box box;
+    double fpepsi;
+    double coulomb_scaling;
+    double* forces;
+    double* energies;
+    double* virial;
+};
+
+struct PertArgs {
+    const double* pos;
+    const uint32_t* pairs;
+    int64_t n;
+    const uint8_t* flag;
+    const int32_t* a_iac;
+    const int32_t* b_iac;
+    const double* a_q;
+    const double* b_q;
+    const double* alpha_lj;
+    const double* alpha_crf;
+    const double* q;
+    const uint32_t* iac;
+    const double* lj;
+    int32_t n_types;
+    Crf crf;
+    Lambda lp;
+    Box box;
+    double fpepsi;
+    double* forces;
+    double* out;  // delta_e_lj, delta_e_crf, dhdl
+};
+
+hipError_t launch_plain(const ExplArgs& a, hipStream_t s);
+hipError_t launch_cg_grouped(const ExplArgs& a, hipStream_t s);
+hipError_t launch_solvent(const ExplArgs& a, hipStream_t s);
+hipError_t launch_rf_excluded(const ExclArgs& a, hipStream_t s);
+hipError_t launch_one_four(const ExclArgs& a, hipStream_t s);
+hipError_t launch_pert_pairs(const PertArgs& a, hipStream_t s);
+
+}  // namespace gamd

@@ -1,0 +1,62 @@
+// Device-resident MD jobs: data layout shared by the system kernels and the
+// host driver.
+//
+// Layout in HBM (one "batch" = one or more independent jobs, concatenated):
+//   * original atom space: job j owns [atom_off, atom_off + n_atoms); every job
+//     starts on a BLOCK-aligned offset so that each block of a per-atom kernel
+//     belongs to exactly one job (energies reduce per job without atomics);
+//   * charge groups are contiguous atom ranges (topology.rs:1093-1110);
+//   * sorted space: at every pair-list rebuild the charge groups of each job are
+//     sorted by cell (stable radix sort on the cell key, so ties keep the
+//     original order) and their atoms laid out contiguously inside the job's
+//     range ("cell-sorted coordinates"); per step the positions are gathered
+//     into xq (x, y, z, q as one 32-byte record per atom) in that order;
+//   * neighbour lists are FULL charge-group lists in sorted space (every pair is
+//     stored for both partners): one lane owns one atom and accumulates only
+//     its own force, so no atomics are needed and the result does not depend on
+//     scheduling.  Each entry carries the "canonical" bit (owner has the lower
+//     original charge-group index): the pair is then evaluated in the
+//     reference's orientation (lower group first) and only the canonical lane
+//     adds energy and virial.
+#pragma once
+#include "device_common.hpp"
+
+namespace gamd {
+
+constexpr int BLOCK = 256;
+
+// neighbour entry: partner sorted CG index | flags
+constexpr uint32_t ENT_IDX = 0x0FFFFFFFu;
+constexpr uint32_t ENT_VVLONG = 1u << 28;  // solvent-solvent long pair (expanded, per-atom PBC)
+constexpr uint32_t ENT_SS = 1u << 29;      // both solute: exclusion / 1-4 check on short pairs
+constexpr uint32_t ENT_LONG = 1u << 30;    // (RCUTP, RCUTL]
+constexpr uint32_t ENT_CANON = 1u << 31;   // owner is the lower original charge group
+
+struct JobDev {
+    Box box;
+    Crf crf;
+    double fpepsi;
+    double c2s, c2l;                       // (rc + skin)^2 short / long
+    double sv_c6[16], sv_c12[16], sv_qq[16];  // solvent template tables (innerloops.rs:241-255)
+    Lambda lp;
+    int32_t n_types, lj_off;
+    int32_t use_cg_grouped;                // forcefield.rs:271-283
+    int32_t aps;                           // atoms per solvent molecule (<= 4)
+    int32_t atom_off, n_atoms, n_solute_atoms;
+    int32_t cg_off, n_cg, n_solute_cg;
+    int32_t ncell[3], cell_off;
+    int32_t twin;                          // rcutp < rcutl - 1e-10 (forcefield.rs:126)
+    int32_t has_pert;
+    double dt;
+    double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
+};
+
+// energy partial rows: per kernel slot and block, 12 doubles
+// (e_lj, e_crf, vir[9], extra)
+constexpr int ROW = 12;
+enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 4, NSLOT = 5 };
+
+// per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]
+constexpr int REC = 14;
+
+}  // namespace gamd

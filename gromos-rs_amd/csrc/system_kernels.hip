@@ -1,0 +1,752 @@
+// Kernels of the device-resident path (plain FP64 form).
+//
+//  pair list      k_cg_ref / k_cell_hist / k_sorted_cg / k_sorted_atoms /
+//                 k_nbl<FILL>        StandardPairlistAlgorithm semantics
+//                                    (pairlist.rs:183-442) on a cell grid
+//  forces         k_gather, k_vv<APS>, k_gen<LONG>, k_excl14, k_assemble
+//                                    (innerloops.rs:105-309, excluded.rs:25-206,
+//                                     forcefield.rs:420-756)
+//  step           k_leapfrog, k_settle, k_ekin, k_reduce
+//                                    (leap_frog.rs:37-148, constraints.rs:815-1036,
+//                                     temperature.rs:63-87)
+#include "system_kernels.hpp"
+
+namespace gamd {
+
+// ---------------------------------------------------------------------------
+// pair list
+// ---------------------------------------------------------------------------
+
+// Reference point of each charge group: COG for solute groups (left-to-right sum
+// from zero, then division, topology.rs:408-416), first atom for solvent
+// (pairlist.rs:304-307, 359-367).  Cell key from the wrapped point.
+__global__ void k_cg_ref(SysPtrs p, int32_t ncg) {
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncg) return;
+    const JobDev& J = p.jobs[p.cg_job[c]];
+    int32_t s = p.cg_start[c], e = p.cg_start[c + 1];
+    const double* x = p.x_cur;
+    double rx, ry, rz;
+    if (c - J.cg_off < J.n_solute_cg) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int32_t a = s; a < e; ++a) {
+            sx += x[3 * a];
+            sy += x[3 * a + 1];
+            sz += x[3 * a + 2];
+        }
+        double n = (double)(e - s);
+        rx = sx / n;
+        ry = sy / n;
+        rz = sz / n;
+    } else {
+        rx = x[3 * s];
+        ry = x[3 * s + 1];
+        rz = x[3 * s + 2];
+    }
+    p.ref[3 * c] = rx;
+    p.ref[3 * c + 1] = ry;
+    p.ref[3 * c + 2] = rz;
+    int32_t cc[3] = {0, 0, 0};
+    if (J.box.kind) {
+        double r[3] = {rx, ry, rz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double u = r[a] * J.box.invL[a];
+            u = u - floor(u);
+            int32_t k = (int32_t)(u * (double)J.ncell[a]);
+            cc[a] = k < 0 ? 0 : (k >= J.ncell[a] ? J.ncell[a] - 1 : k);
+        }
+    }
+    p.key_in[c] = (uint32_t)(J.cell_off + cc[0] + J.ncell[0] * (cc[1] + J.ncell[1] * cc[2]));
+    p.val_in[c] = (uint32_t)c;
+}
+
+__global__ void k_cell_hist(SysPtrs p, int32_t ncg) {
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncg) return;
+    atomicAdd(p.cell_count + p.key_in[c], 1);
+}
+
+// sorted position k -> original group, atom count, sorted copy of the reference point
+__global__ void k_sorted_cg(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t c = (int32_t)p.val_out[k];
+    p.s_cg_orig[k] = c;
+    p.s_cg_n[k] = p.cg_start[c + 1] - p.cg_start[c];
+    p.s_ref[3 * k] = p.ref[3 * c];
+    p.s_ref[3 * k + 1] = p.ref[3 * c + 1];
+    p.s_ref[3 * k + 2] = p.ref[3 * c + 2];
+}
+
+// Sorted atom layout: job j's groups fill [atom_off, atom_off + n_atoms) in
+// sorted order.  s_cg_start comes from an exclusive scan of s_cg_n, shifted per
+// job to the job's aligned offset.
+__global__ void k_sorted_atoms(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t c = p.s_cg_orig[k];
+    const JobDev& J = p.jobs[p.cg_job[c]];
+    // groups of one job are contiguous in sorted space (the key carries cell_off)
+    int32_t start = p.s_cg_scan[k] - p.s_cg_scan[J.cg_off] + J.atom_off;
+    p.s_cg_start[k] = start;
+    int32_t n = p.s_cg_n[k];
+    int32_t o = p.cg_start[c];
+    for (int32_t a = 0; a < n; ++a) {
+        p.s2o[start + a] = o + a;
+        p.s_atom_cg[start + a] = k;
+        p.s_type[start + a] = p.iac[o + a];
+    }
+}
+
+// canonical squared distance of two reference points: nearest_image(lower, higher)
+__device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
+    d3 r = nim(b, lo, hi);
+    return r.x * r.x + r.y * r.y + r.z * r.z;
+}
+
+// Neighbour search for one sorted group: 27 cells (all cells along an axis with
+// fewer than 3), the reference's classification (pairlist.rs:252-415).
+template <bool FILL>
+__global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t ck = p.s_cg_orig[k];
+    const JobDev& J = p.jobs[p.cg_job[ck]];
+    bool sol_k = ck - J.cg_off < J.n_solute_cg;
+    d3 rk{p.s_ref[3 * k], p.s_ref[3 * k + 1], p.s_ref[3 * k + 2]};
+    int32_t local = (int32_t)p.key_out[k] - J.cell_off;
+    int32_t cx = local % J.ncell[0];
+    int32_t cy = (local / J.ncell[0]) % J.ncell[1];
+    int32_t cz = local / (J.ncell[0] * J.ncell[1]);
+    int32_t cc[3] = {cx, cy, cz};
+    int32_t lst[3][3], nl[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int32_t n = J.ncell[a];
+        if (n >= 3) {
+            nl[a] = 3;
+            lst[a][0] = (cc[a] + n - 1) % n;
+            lst[a][1] = cc[a];
+            lst[a][2] = (cc[a] + 1) % n;
+        } else {
+            nl[a] = n;
+            lst[a][0] = 0;
+            lst[a][1] = 1;
+            lst[a][2] = 2;
+        }
+    }
+    int32_t nvv = 0, ngen = 0;
+    int64_t vv_base = FILL ? p.vv_off[k] : 0, gen_base = FILL ? p.gen_off[k] : 0;
+    uint64_t pairs_short = 0, pairs_long = 0;
+    int32_t nk = p.s_cg_n[k];
+    for (int32_t iz = 0; iz < nl[2]; ++iz)
+        for (int32_t iy = 0; iy < nl[1]; ++iy)
+            for (int32_t ix = 0; ix < nl[0]; ++ix) {
+                int32_t cell = J.cell_off + lst[0][ix] + J.ncell[0] * (lst[1][iy] + J.ncell[1] * lst[2][iz]);
+                int32_t m0 = p.cell_start[cell], m1 = p.cell_start[cell + 1];
+                for (int32_t m = m0; m < m1; ++m) {
+                    if (m == k) continue;
+                    int32_t cm = p.s_cg_orig[m];
+                    d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
+                    bool canon = ck < cm;
+                    double d2 = canon ? ref_d2(J.box, rk, rm) : ref_d2(J.box, rm, rk);
+                    if (d2 > J.c2l) continue;
+                    bool lng = d2 > J.c2s;
+                    bool sol_m = cm - J.cg_off < J.n_solute_cg;
+                    uint32_t flags = canon ? ENT_CANON : 0u;
+                    if (!sol_k && !sol_m) {
+                        if (!lng) {
+                            if (FILL) p.vv_ent[vv_base + nvv] = (uint32_t)m | flags;
+                            ++nvv;
+                            if (canon) pairs_short += (uint64_t)J.aps * J.aps;
+                            continue;
+                        }
+                        flags |= ENT_VVLONG;
+                    }
+                    if (lng) flags |= ENT_LONG;
+                    if (sol_k && sol_m) flags |= ENT_SS;
+                    if (FILL) p.gen_ent[gen_base + ngen] = (uint32_t)m | flags;
+                    ++ngen;
+                    if (canon) {
+                        uint64_t np = (uint64_t)nk * p.s_cg_n[m];
+                        if (lng) pairs_long += np; else pairs_short += np;
+                    }
+                }
+            }
+    if (!FILL) {
+        p.vv_cnt[k] = nvv;
+        p.gen_cnt[k] = ngen;
+        if (pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
+        if (pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forces
+// ---------------------------------------------------------------------------
+
+__global__ void k_gather(SysPtrs p, int32_t npad) {
+    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= npad) return;
+    int32_t o = p.s2o[a];
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (o >= 0) v = make_double4(p.x_cur[3 * o], p.x_cur[3 * o + 1], p.x_cur[3 * o + 2], p.q[o]);
+    p.xq[a] = v;
+}
+
+__device__ __forceinline__ d3 xyz(double4 v) { return d3{v.x, v.y, v.z}; }
+
+__device__ __forceinline__ void add_virial(double (&v)[ROW], d3 r, double fx, double fy, double fz) {
+    v[2] += r.x * fx; v[3] += r.x * fy; v[4] += r.x * fz;
+    v[5] += r.y * fx; v[6] += r.y * fy; v[7] += r.y * fz;
+    v[8] += r.z * fx; v[9] += r.z * fy; v[10] += r.z * fz;
+}
+
+__device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int slot, int nblocks) {
+    __shared__ double lds[17 * ROW];
+    block_sum<ROW>(v, lds);
+    if (threadIdx.x < ROW)
+        rows[((int64_t)slot * nblocks + blockIdx.x) * ROW + threadIdx.x] = lds[16 * ROW + threadIdx.x];
+}
+
+// Solvent-solvent short pairs (process_solvent_pairs, innerloops.rs:218-309):
+// shift from the atom-0 pair, then n x n atom pairs with the template tables.
+template <int APS>
+__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad, int32_t virial) {
+    double v[ROW] = {0};
+    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t o = a < npad ? p.s2o[a] : -1;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (o >= 0) {
+        int32_t k = p.s_atom_cg[a];
+        const JobDev& J = p.jobs[p.atom_job[o]];
+        int32_t e0 = (int32_t)p.vv_off[k], e1 = (int32_t)p.vv_off[k + 1];
+        if (e1 > e0) {
+            int32_t sk = p.s_cg_start[k];
+            int32_t ia = a - sk;
+            d3 pa = xyz(p.xq[a]);
+            d3 pk0 = xyz(p.xq[sk]);
+            double c6[APS], c12[APS], qq[APS];
+#pragma unroll
+            for (int b = 0; b < APS; ++b) {
+                c6[b] = J.sv_c6[ia * 4 + b];
+                c12[b] = J.sv_c12[ia * 4 + b];
+                qq[b] = J.sv_qq[ia * 4 + b];
+            }
+            for (int32_t e = e0; e < e1; ++e) {
+                uint32_t ent = p.vv_ent[e];
+                int32_t m = (int32_t)(ent & ENT_IDX);
+                bool canon = (ent & ENT_CANON) != 0;
+                int32_t sm = p.s_cg_start[m];
+                d3 pm0 = xyz(p.xq[sm]);
+                d3 A0 = canon ? pk0 : pm0, B0 = canon ? pm0 : pk0;
+                d3 rf = nim(J.box, A0, B0);
+                double tx = rf.x - A0.x + B0.x, ty = rf.y - A0.y + B0.y, tz = rf.z - A0.z + B0.z;
+#pragma unroll
+                for (int b = 0; b < APS; ++b) {
+                    d3 pb = xyz(p.xq[sm + b]);
+                    d3 u = canon ? pa : pb, w = canon ? pb : pa;
+                    d3 r{(u.x + tx) - w.x, (u.y + ty) - w.y, (u.z + tz) - w.z};
+                    double fm, elj, ecrf;
+                    lj_crf64(r.x, r.y, r.z, c6[b], c12[b], qq[b], J.crf, fm, elj, ecrf);
+                    double gx = r.x * fm, gy = r.y * fm, gz = r.z * fm;
+                    if (canon) {
+                        fx += gx; fy += gy; fz += gz;
+                        v[0] += elj;
+                        v[1] += ecrf;
+                        if (virial) add_virial(v, r, gx, gy, gz);
+                    } else {
+                        fx -= gx; fy -= gy; fz -= gz;
+                    }
+                }
+            }
+        }
+    }
+    if (a < npad) {
+        p.f_s[3 * a] = fx;
+        p.f_s[3 * a + 1] = fy;
+        p.f_s[3 * a + 2] = fz;
+    }
+    write_row(v, p.rows, SLOT_VV, npad / BLOCK);
+}
+
+// Topology::is_excluded_or_14 (topology.rs:853-865) on original atom indices
+__device__ __forceinline__ bool bsearch32(const int32_t* a, int32_t n, int32_t key) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == key;
+}
+__device__ __forceinline__ bool excluded_or_14(const SysPtrs& p, int32_t i, int32_t j) {
+    if (bsearch32(p.excl + p.excl_off[i], p.excl_off[i + 1] - p.excl_off[i], j)) return true;
+    if (bsearch32(p.excl + p.excl_off[j], p.excl_off[j + 1] - p.excl_off[j], i)) return true;
+    for (int32_t k = p.o14_off[i]; k < p.o14_off[i + 1]; ++k) if (p.o14[k] == j) return true;
+    for (int32_t k = p.o14_off[j]; k < p.o14_off[j + 1]; ++k) if (p.o14[k] == i) return true;
+    return false;
+}
+
+// One pair (owner atom a, partner atom b) in the reference's orientation.
+// cg path: shift from the two groups' first atoms (process_pairs_cg_grouped,
+// innerloops.rs:117-135); otherwise per-atom nearest image (innerloops.rs:180-186).
+__device__ __forceinline__ void gen_pair(const SysPtrs& p, const JobDev& J, double (&v)[ROW],
+                                         double& fx, double& fy, double& fz, bool canon, bool cgpath,
+                                         d3 pa, d3 pb, d3 fa0, d3 fb0, int32_t ta, int32_t tb,
+                                         double qa, double qb, bool virial) {
+    d3 r;
+    if (cgpath) {
+        d3 A0 = canon ? fa0 : fb0, B0 = canon ? fb0 : fa0;
+        d3 rf = nim(J.box, A0, B0);
+        double tx = rf.x - A0.x + B0.x, ty = rf.y - A0.y + B0.y, tz = rf.z - A0.z + B0.z;
+        d3 u = canon ? pa : pb, w = canon ? pb : pa;
+        r = d3{u.x + tx - w.x, u.y + ty - w.y, u.z + tz - w.z};
+    } else {
+        r = canon ? nim(J.box, pa, pb) : nim(J.box, pb, pa);
+    }
+    int32_t ti = canon ? ta : tb, tj = canon ? tb : ta;
+    const double* lj = p.lj + J.lj_off + 4 * ((int64_t)ti * J.n_types + tj);
+    double qi = canon ? qa : qb, qj = canon ? qb : qa;
+    double qp = qi * qj * J.fpepsi;
+    double fm, elj, ecrf;
+    lj_crf64(r.x, r.y, r.z, lj[0], lj[1], qp, J.crf, fm, elj, ecrf);
+    double gx = r.x * fm, gy = r.y * fm, gz = r.z * fm;
+    if (canon) {
+        fx += gx; fy += gy; fz += gz;
+        v[0] += elj;
+        v[1] += ecrf;
+        if (virial) add_virial(v, r, gx, gy, gz);
+    } else {
+        fx -= gx; fy -= gy; fz -= gz;
+    }
+}
+
+// Every non-solvent-solvent-short pair: solute-solute (with exclusions), solute-
+// solvent, intra-group, and (LONG) the twin-range long pairs.
+template <bool LONG>
+__global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t virial) {
+    double v[ROW] = {0};
+    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t o = a < npad ? p.s2o[a] : -1;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (o >= 0) {
+        int32_t k = p.s_atom_cg[a];
+        const JobDev& J = p.jobs[p.atom_job[o]];
+        int32_t sk = p.s_cg_start[k];
+        int32_t nk = p.s_cg_n[k];
+        double4 xa = p.xq[a];
+        d3 pa = xyz(xa);
+        d3 pk0 = xyz(p.xq[sk]);
+        int32_t ta = p.s_type[a];
+        bool cgp = J.use_cg_grouped != 0;
+        int32_t e0 = (int32_t)p.gen_off[k], e1 = (int32_t)p.gen_off[k + 1];
+        for (int32_t e = e0; e < e1; ++e) {
+            uint32_t ent = p.gen_ent[e];
+            bool lng = (ent & ENT_LONG) != 0;
+            if (lng != LONG) continue;
+            int32_t m = (int32_t)(ent & ENT_IDX);
+            bool canon = (ent & ENT_CANON) != 0;
+            bool ss = (ent & ENT_SS) != 0;
+            int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+            d3 pm0 = xyz(p.xq[sm]);
+            for (int32_t b = 0; b < nm; ++b) {
+                if (!LONG && ss) {
+                    int32_t ob = p.s2o[sm + b];
+                    if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+                }
+                double4 xb = p.xq[sm + b];
+                gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
+                         p.s_type[sm + b], xa.w, xb.w, LONG || virial);
+            }
+        }
+        // intra-group pairs of solute groups (pairlist.rs:238-249), always short
+        if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
+            int32_t ia = a - sk;
+            for (int32_t b = 0; b < nk; ++b) {
+                if (b == ia) continue;
+                int32_t ob = p.s2o[sk + b];
+                bool canon = ia < b;
+                if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+                double4 xb = p.xq[sk + b];
+                gen_pair(p, J, v, fx, fy, fz, canon, cgp, pa, xyz(xb), pk0, pk0, ta, p.s_type[sk + b],
+                         xa.w, xb.w, virial);
+            }
+        }
+    }
+    if (a < npad) {
+        double* f = LONG ? p.f_long_s : p.f_s;
+        if (LONG) {
+            f[3 * a] = fx;
+            f[3 * a + 1] = fy;
+            f[3 * a + 2] = fz;
+        } else {
+            f[3 * a] += fx;
+            f[3 * a + 1] += fy;
+            f[3 * a + 2] += fz;
+        }
+    }
+    write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, npad / BLOCK);
+}
+
+// RF self / excluded pairs (excluded.rs:25-112) and 1-4 pairs (excluded.rs:131-206),
+// one lane per original atom.  Partners below the owner contribute only their
+// force, evaluated in the reference's orientation.
+__global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
+    double v[ROW] = {0};
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    int32_t jb = i < npad ? p.atom_job[i] : -1;
+    if (jb >= 0) {
+        const JobDev& J = p.jobs[jb];
+        const double* x = p.x_cur;
+        d3 pi = ld3(x, i);
+        double qi = p.q[i];
+        bool sol_i = i - J.atom_off < J.n_solute_atoms;
+        if (sol_i) v[1] += -0.5 * qi * qi * J.fpepsi * J.crf.crf_cut;
+        for (int32_t k = p.excl_off[i]; k < p.excl_off[i + 1]; ++k) {
+            int32_t j = p.excl[k];
+            d3 pj = ld3(x, j);
+            double qj = p.q[j];
+            if (j > i) {
+                d3 r = nim(J.box, pi, pj);
+                double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                if (sol_i) {
+                    double qp = qi * qj * J.fpepsi;
+                    v[1] += qp * (-J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
+                    double fs = qp * J.crf.crf_cut3i;
+                    double gx = r.x * fs, gy = r.y * fs, gz = r.z * fs;
+                    fx += gx; fy += gy; fz += gz;
+                    add_virial(v, r, gx, gy, gz);
+                } else {
+                    v[1] += -qi * qj * J.fpepsi * J.crf.crf_2cut3i * r2;
+                }
+            } else if (j < i && j - J.atom_off < J.n_solute_atoms) {
+                d3 r = nim(J.box, pj, pi);
+                double fs = qj * qi * J.fpepsi * J.crf.crf_cut3i;
+                fx -= r.x * fs; fy -= r.y * fs; fz -= r.z * fs;
+            }
+        }
+        for (int32_t k = p.o14s_off[i]; k < p.o14s_off[i + 1]; ++k) {
+            uint32_t ent = (uint32_t)p.o14s[k];
+            int32_t j = (int32_t)(ent & 0x7FFFFFFFu);
+            bool canon = (ent & 0x80000000u) != 0;
+            d3 pj = ld3(x, j);
+            d3 r = canon ? nim(J.box, pi, pj) : nim(J.box, pj, pi);
+            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            if (r2 < 1e-10) continue;
+            int32_t lo = canon ? i : j, hi = canon ? j : i;
+            const double* lj = p.lj + J.lj_off + 4 * ((int64_t)p.iac[lo] * J.n_types + p.iac[hi]);
+            double inv_r2 = 1.0 / r2;
+            double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+            double elj = (lj[3] * inv_r6 - lj[2]) * inv_r6;
+            double flj = (12.0 * lj[3] * inv_r6 - 6.0 * lj[2]) * inv_r6 * inv_r2;
+            double qp = p.q[lo] * p.q[hi] * J.fpepsi;
+            double inv_r = sqrt(inv_r2);
+            double ecrf = qp * (inv_r * 1.0 - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
+            double fcrf = qp * (inv_r * 1.0 * inv_r2 + J.crf.crf_cut3i);
+            double ft = flj + fcrf;
+            double gx = r.x * ft, gy = r.y * ft, gz = r.z * ft;
+            if (canon) {
+                fx += gx; fy += gy; fz += gz;
+                v[0] += elj;
+                v[1] += ecrf;
+                add_virial(v, r, gx, gy, gz);
+            } else {
+                fx -= gx; fy -= gy; fz -= gz;
+            }
+        }
+    }
+    if (i < npad) {
+        p.f_x[3 * i] = fx;
+        p.f_x[3 * i + 1] = fy;
+        p.f_x[3 * i + 2] = fz;
+    }
+    write_row(v, p.rows, SLOT_EXCL, npad / BLOCK);
+}
+
+// total force on each original atom
+__global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t with_pert) {
+    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= npad) return;
+    int32_t o = p.s2o[a];
+    if (o < 0) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double f = p.f_s[3 * a + c];
+        if (with_long) f += p.f_long_s[3 * a + c];
+        f += p.f_x[3 * o + c];
+        if (with_pert) f += p.f_p[3 * o + c];
+        p.f[3 * o + c] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// step
+// ---------------------------------------------------------------------------
+
+// LeapFrogVelocity then LeapFrogPosition (leap_frog.rs:37-68, 116-140)
+__global__ void k_leapfrog(SysPtrs p, int32_t npad) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad || p.atom_job[i] < 0) return;
+    double dt = p.jobs[p.atom_job[i]].dt;
+    double im = p.inv_mass[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double acc = p.f[3 * i + c] * im;
+        double vn = p.v_cur[3 * i + c] + acc * dt;
+        p.v_new[3 * i + c] = vn;
+        p.x_new[3 * i + c] = p.x_cur[3 * i + c] + vn * dt;
+    }
+}
+
+struct v3 {
+    double x, y, z;
+};
+__device__ __forceinline__ v3 mk(double x, double y, double z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 vmul(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 vdiv(v3 a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 vcross(v3 a, v3 b) {  // glam DVec3::cross
+    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ v3 ldv(const double* p, int64_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void stv(double* p, int64_t i, v3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// settle (constraints.rs:815-1036), one lane per water; FP64 throughout
+__global__ void k_settle(SysPtrs p, int32_t nmol, int32_t step) {
+    int32_t mi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mi >= nmol) return;
+    int32_t i = p.mol_first[mi];
+    int32_t jb = p.atom_job[i];
+    const JobDev& J = p.jobs[jb];
+    double m_o = J.m_o, m_h = J.m_h;
+    double half_mo_div_mh = 0.5 * m_o / m_h;
+    double rc = 0.5 * J.dist_hh;
+    double ra = sqrt(J.dist_oh * J.dist_oh - rc * rc) / (1.0 + half_mo_div_mh);
+    double rb = half_mo_div_mh * ra;
+    double dt_i = 1.0 / J.dt;
+    v3 po0 = ldv(p.x_cur, i), po1 = ldv(p.x_cur, i + 1), po2 = ldv(p.x_cur, i + 2);
+    v3 pn0 = ldv(p.x_new, i), pn1 = ldv(p.x_new, i + 1), pn2 = ldv(p.x_new, i + 2);
+    v3 b0 = vsub(po1, po0), c0 = vsub(po2, po0);
+    v3 d0 = vdiv(vadd(vadd(vmul(pn0, m_o), vmul(pn1, m_h)), vmul(pn2, m_h)), m_o + m_h + m_h);
+    v3 a1 = vsub(pn0, d0), b1 = vsub(pn1, d0), c1 = vsub(pn2, d0);
+    v3 n0 = vcross(b0, c0);
+    v3 n1 = vcross(a1, n0);
+    v3 n2 = vcross(n0, n1);
+    n0 = vdiv(n0, sqrt(vdot(n0, n0)));
+    n1 = vdiv(n1, sqrt(vdot(n1, n1)));
+    n2 = vdiv(n2, sqrt(vdot(n2, n2)));
+    v3 m1 = mk(n1.x, n2.x, n0.x), m2 = mk(n1.y, n2.y, n0.y), m0 = mk(n1.z, n2.z, n0.z);
+    b0 = mk(vdot(n1, b0), vdot(n2, b0), vdot(n0, b0));
+    c0 = mk(vdot(n1, c0), vdot(n2, c0), vdot(n0, c0));
+    a1 = mk(vdot(n1, a1), vdot(n2, a1), vdot(n0, a1));
+    b1 = mk(vdot(n1, b1), vdot(n2, b1), vdot(n0, b1));
+    c1 = mk(vdot(n1, c1), vdot(n2, c1), vdot(n0, c1));
+    double sinphi = a1.z / ra;
+    double omsp2 = 1.0 - sinphi * sinphi;
+    if (omsp2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
+    double cosphi = sqrt(omsp2);
+    double sinpsi = (b1.z - c1.z) / (2.0 * rc * cosphi);
+    double omss2 = 1.0 - sinpsi * sinpsi;
+    if (omss2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
+    double cospsi = sqrt(omss2);
+    double minus_rb_cosphi = -rb * cosphi;
+    double rc_cospsi = rc * cospsi;
+    double rc_sinpsi_sinphi = rc * sinpsi * sinphi;
+    double rc_sinpsi_cosphi = rc * sinpsi * cosphi;
+    v3 a2 = mk(0.0, ra * cosphi, ra * sinphi);
+    v3 b2 = mk(-rc_cospsi, minus_rb_cosphi - rc_sinpsi_sinphi, -rb * sinphi + rc_sinpsi_cosphi);
+    v3 c2 = mk(rc_cospsi, minus_rb_cosphi + rc_sinpsi_sinphi, -rb * sinphi - rc_sinpsi_cosphi);
+    double alpha = b2.x * (b0.x - c0.x) + b0.y * b2.y + c0.y * c2.y;
+    double beta = b2.x * (c0.y - b0.y) + b0.x * b2.y + c0.x * c2.y;
+    double gamma = b0.x * b1.y - b1.x * b0.y + c0.x * c1.y - c1.x * c0.y;
+    double a2b2 = alpha * alpha + beta * beta;
+    double sintheta = (alpha * gamma - beta * sqrt(a2b2 - gamma * gamma)) / a2b2;
+    double omst2 = 1.0 - sintheta * sintheta;
+    if (omst2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
+    double costheta = sqrt(omst2);
+    v3 a3 = mk(-a2.y * sintheta, a2.y * costheta, a1.z);
+    v3 b3 = mk(b2.x * costheta - b2.y * sintheta, b2.x * sintheta + b2.y * costheta, b1.z);
+    v3 c3 = mk(-b2.x * costheta - c2.y * sintheta, -b2.x * sintheta + c2.y * costheta, c1.z);
+    v3 pa = vadd(mk(vdot(a3, m1), vdot(a3, m2), vdot(a3, m0)), d0);
+    v3 pb = vadd(mk(vdot(b3, m1), vdot(b3, m2), vdot(b3, m0)), d0);
+    v3 pc = vadd(mk(vdot(c3, m1), vdot(c3, m2), vdot(c3, m0)), d0);
+    v3 da = vsub(pa, pn0), db = vsub(pb, pn1), dc = vsub(pc, pn2);
+    stv(p.x_new, i, pa);
+    stv(p.x_new, i + 1, pb);
+    stv(p.x_new, i + 2, pc);
+    stv(p.v_new, i, vadd(ldv(p.v_new, i), vmul(da, dt_i)));
+    stv(p.v_new, i + 1, vadd(ldv(p.v_new, i + 1), vmul(db, dt_i)));
+    stv(p.v_new, i + 2, vadd(ldv(p.v_new, i + 2), vmul(dc, dt_i)));
+}
+
+// TemperatureCalculation (temperature.rs:63-87): 0.5 m (|v_new|^2 + |v_old|^2) / 2
+__global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
+    double v[ROW] = {0};
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npad && p.atom_job[i] >= 0) {
+        const double* vn = p.v_new;
+        const double* vo = p.v_cur;
+        double a = vn[3 * i] * vn[3 * i] + vn[3 * i + 1] * vn[3 * i + 1] + vn[3 * i + 2] * vn[3 * i + 2];
+        double b = vo[3 * i] * vo[3 * i] + vo[3 * i + 1] * vo[3 * i + 1] + vo[3 * i + 2] * vo[3 * i + 2];
+        v[11] = 0.5 * p.mass[i] * (a + b) / 2.0;
+    }
+    write_row(v, p.rows, SLOT_KIN, npad / BLOCK);
+}
+
+// One block per job: sums the job's partial rows in a fixed order and writes
+// the step record (e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]).
+__global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32_t with_long,
+                                                  int32_t with_kin, int32_t step, int32_t njobs) {
+    __shared__ double lds[17 * ROW];
+    int32_t jb = blockIdx.x;
+    const JobDev& J = p.jobs[jb];
+    int32_t nblocks = npad / BLOCK;
+    int32_t b0 = J.atom_off / BLOCK;
+    int32_t b1 = (J.atom_off + J.n_atoms + BLOCK - 1) / BLOCK;
+    double v[ROW] = {0};
+    for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        for (int s = 0; s < NSLOT; ++s) {
+            if (s == SLOT_LONG && !with_long) continue;
+            if (s == SLOT_KIN && !with_kin) continue;
+            const double* r = p.rows + ((int64_t)s * nblocks + b) * ROW;
+#pragma unroll
+            for (int c = 0; c < ROW; ++c) v[c] += r[c];
+        }
+    }
+    block_sum<ROW>(v, lds);
+    if (threadIdx.x == 0) {
+        double* rec = p.records + ((int64_t)step * njobs + jb) * REC;
+        double elj = lds[16 * ROW + 0], ecrf = lds[16 * ROW + 1];
+        double dh = 0.0;
+        if (J.has_pert) {
+            elj += p.pert_e[3 * jb];
+            ecrf += p.pert_e[3 * jb + 1];
+            dh = p.pert_e[3 * jb + 2];
+        }
+        rec[0] = elj;
+        rec[1] = ecrf;
+        rec[2] = elj + ecrf;
+        rec[3] = lds[16 * ROW + 11];
+        rec[4] = dh;
+        for (int c = 0; c < 9; ++c) rec[5 + c] = lds[16 * ROW + 2 + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// test-mode materializer: the reference's four atom-pair lists
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void k_materialize(SysPtrs p, int32_t ncg, unsigned long long* counts, const int64_t* caps,
+                              uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t ck = p.s_cg_orig[k];
+    const JobDev& J = p.jobs[p.cg_job[ck]];
+    int32_t sk = p.s_cg_start[k], nk = p.s_cg_n[k];
+    uint32_t* outs[4] = {out0, out1, out2, out3};
+    auto emit = [&](int list, int32_t i, int32_t j) {
+        unsigned long long slot = atomicAdd(counts + list, 1ull);
+        if (FILL && (int64_t)slot < caps[list]) {
+            outs[list][2 * slot] = (uint32_t)(i - J.atom_off);
+            outs[list][2 * slot + 1] = (uint32_t)(j - J.atom_off);
+        }
+    };
+    if (ck - J.cg_off < J.n_solute_cg) {
+        for (int32_t x = 0; x < nk; ++x)
+            for (int32_t y = x + 1; y < nk; ++y) {
+                int32_t i = p.s2o[sk + x], j = p.s2o[sk + y];
+                if (!excluded_or_14(p, i, j)) emit(0, i, j);
+            }
+    }
+    for (int32_t e = (int32_t)p.vv_off[k]; e < (int32_t)p.vv_off[k + 1]; ++e) {
+        uint32_t ent = p.vv_ent[e];
+        if (!(ent & ENT_CANON)) continue;
+        int32_t m = (int32_t)(ent & ENT_IDX);
+        emit(2, p.s2o[sk], p.s2o[p.s_cg_start[m]]);
+    }
+    for (int32_t e = (int32_t)p.gen_off[k]; e < (int32_t)p.gen_off[k + 1]; ++e) {
+        uint32_t ent = p.gen_ent[e];
+        if (!(ent & ENT_CANON)) continue;
+        int32_t m = (int32_t)(ent & ENT_IDX);
+        int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+        bool lng = ent & ENT_LONG, ss = ent & ENT_SS, vvl = ent & ENT_VVLONG;
+        int list = vvl ? 3 : (lng ? 1 : 0);
+        for (int32_t x = 0; x < nk; ++x)
+            for (int32_t y = 0; y < nm; ++y) {
+                int32_t i = p.s2o[sk + x], j = p.s2o[sm + y];
+                if (ss && !lng && excluded_or_14(p, i, j)) continue;
+                emit(list, i, j);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_ref, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_hist, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s) {
+    hipLaunchKernelGGL(k_sorted_cg, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s) {
+    hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
+    if (fill) hipLaunchKernelGGL(k_nbl<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    else hipLaunchKernelGGL(k_nbl<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_gather(const SysPtrs& p, int32_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+}
+void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, hipStream_t s) {
+    switch (aps) {
+        case 1: hipLaunchKernelGGL(k_vv<1>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
+        case 2: hipLaunchKernelGGL(k_vv<2>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
+        case 4: hipLaunchKernelGGL(k_vv<4>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
+        default: hipLaunchKernelGGL(k_vv<3>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
+    }
+}
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, hipStream_t s) {
+    if (lng) hipLaunchKernelGGL(k_gen<true>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
+    else hipLaunchKernelGGL(k_gen<false>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
+}
+void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+}
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, with_pert);
+}
+void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+}
+void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s) {
+    if (nmol > 0) hipLaunchKernelGGL(k_settle, grid_for(nmol), dim3(BLOCK), 0, s, p, nmol, step);
+}
+void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(k_ekin, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+}
+void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
+                   int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, step, njobs);
+}
+void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,
+                        uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3, hipStream_t s) {
+    if (fill) hipLaunchKernelGGL(k_materialize<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, counts, caps, o0, o1, o2, o3);
+    else hipLaunchKernelGGL(k_materialize<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, counts, caps, o0, o1, o2, o3);
+}
+
+}  // namespace gamd

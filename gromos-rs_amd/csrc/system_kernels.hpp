@@ -1,0 +1,84 @@
+// Device pointer bundle and launchers of the device-resident kernels.
+#pragma once
+#include "system.hpp"
+
+namespace gamd {
+
+struct SysPtrs {
+    const JobDev* jobs;
+    // original-order topology (global over the batch)
+    const int32_t* cg_start;   // ncg + 1, global atom offsets
+    const int32_t* cg_job;     // ncg
+    const int32_t* atom_job;   // npad, -1 on padding
+    const double* q;
+    const int32_t* iac;
+    const double* mass;
+    const double* inv_mass;
+    const double* lj;          // all jobs' tables, job.lj_off
+    const int32_t* excl_off;   // npad + 1
+    const int32_t* excl;
+    const int32_t* o14_off;    // one_four_pairs as read (exclusion semantics)
+    const int32_t* o14;
+    const int32_t* o14s_off;   // symmetric 1-4 entries, bit 31 = owner is the lower atom
+    const int32_t* o14s;
+    const int32_t* mol_first;  // SETTLE: first atom of each solvent water
+    // state (original order)
+    const double* x_cur;
+    double* x_new;
+    const double* v_cur;
+    double* v_new;
+    double* f;
+    // pair list
+    double* ref;
+    uint32_t* key_in;
+    uint32_t* val_in;
+    uint32_t* key_out;
+    uint32_t* val_out;
+    int32_t* cell_count;
+    const int32_t* cell_start;
+    int32_t* s_cg_orig;
+    int32_t* s_cg_n;
+    const int32_t* s_cg_scan;
+    int32_t* s_cg_start;
+    double* s_ref;
+    int32_t* s2o;
+    int32_t* s_atom_cg;
+    int32_t* s_type;
+    int64_t* vv_cnt;
+    int64_t* gen_cnt;
+    const int64_t* vv_off;
+    const int64_t* gen_off;
+    uint32_t* vv_ent;
+    uint32_t* gen_ent;
+    uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild
+    // per step
+    double4* xq;
+    double* f_s;
+    double* f_long_s;
+    double* f_x;
+    double* f_p;
+    double* rows;
+    double* records;
+    double* pert_e;            // 3 per job: delta e_lj, delta e_crf, dhdl
+    int32_t* settle_fail;      // per job: first failing step
+};
+
+void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s);
+void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
+void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
+void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
+void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
+void launch_gather(const SysPtrs& p, int32_t npad, hipStream_t s);
+void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, hipStream_t s);
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, hipStream_t s);
+void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, hipStream_t s);
+void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s);
+void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
+void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);
+void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
+                   int32_t njobs, hipStream_t s);
+void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,
+                        uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3, hipStream_t s);
+
+}  // namespace gamd

@@ -1,0 +1,266 @@
+"""ctypes binding of libgromos_amd.so (include/gromos_amd.h).
+
+The library is the only compute path: nothing here falls back to the CPU.  If
+the shared library has not been built, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgromos_amd.so")
+
+P = C.c_void_p
+D = C.c_double
+I32 = C.c_int32
+I64 = C.c_int64
+
+FP64 = 0
+MIXED = 1
+
+_lib = None
+
+
+class GromosAmdError(RuntimeError):
+    pass
+
+
+class Crf(C.Structure):
+    """CRFParameters (crates/gromos-forces/src/nonbonded/params.rs:60-68)."""
+    _fields_ = [("crf_cut", D), ("crf_2cut3i", D), ("crf_cut3i", D), ("cutoff_sq", D)]
+
+
+class Lambda(C.Structure):
+    """PerturbedLambdaParams (crates/gromos-forces/src/nonbonded/perturbed.rs:11-50)."""
+    _fields_ = [(k, D) for k in (
+        "a_lj_lambda_n", "b_lj_lambda_n", "a_crf_lambda_n", "b_crf_lambda_n",
+        "a_lj_lambda_n_1", "b_lj_lambda_n_1", "a_crf_lambda_n_1", "b_crf_lambda_n_1",
+        "a_ljs_lambda", "b_ljs_lambda", "a_ljs_lambda2", "b_ljs_lambda2",
+        "a_crfs_lambda", "b_crfs_lambda", "a_crfs_lambda2", "b_crfs_lambda2")] + [("lambda_exp", I32)]
+
+
+class Pert(C.Structure):
+    _fields_ = [("flag", P), ("a_iac", P), ("b_iac", P), ("a_q", P), ("b_q", P), ("alpha_lj", P),
+                ("alpha_crf", P)]
+
+
+class Topology(C.Structure):
+    _fields_ = [("n_atoms", I32), ("n_solute_atoms", I32), ("n_cg", I32), ("n_solute_cg", I32),
+                ("cg_start", P), ("charge", P), ("iac", P), ("mass", P), ("n_types", I32), ("lj", P),
+                ("excl_off", P), ("excl", P), ("o14_off", P), ("o14", P), ("atoms_per_solvent", I32),
+                ("dist_oh", D), ("dist_hh", D),
+                ("pert_flag", P), ("pert_a_iac", P), ("pert_b_iac", P), ("pert_a_q", P), ("pert_b_q", P),
+                ("pert_alpha_lj", P), ("pert_alpha_crf", P), ("n_pert_atom_pairs", I32),
+                ("pert_atom_pairs", P)]
+
+
+class Params(C.Structure):
+    _fields_ = [("boundary", I32), ("box", D * 3), ("rcutp", D), ("rcutl", D), ("skin", D),
+                ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
+                ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32)]
+
+
+def built():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not built():
+            raise GromosAmdError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                                 "(the HIP library is required; there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    L.gromos_amd_last_error.restype = C.c_char_p
+    L.gromos_amd_version.restype = C.c_char_p
+    L.gromos_amd_device_count.restype = C.c_int
+    L.gromos_amd_crf_new.argtypes = [D, D, D, D, C.POINTER(Crf)]
+    L.gromos_amd_lambda_from.argtypes = [D] * 8 + [I32, C.POINTER(Lambda)]
+    base = [P, P, P]
+    L.gromos_amd_lj_crf_innerloop.argtypes = base + [P, I64, P, I32, C.POINTER(Crf), I32, P, D, I32, P, P, P, P]
+    L.gromos_amd_lj_crf_innerloop_cg_grouped.argtypes = base + [P, P, I64, P, I32, C.POINTER(Crf), I32, P, D,
+                                                                I32, P, P, P, P]
+    L.gromos_amd_solvent_innerloop.argtypes = base + [P, I64, P, I32, C.POINTER(Crf), I32, P, I32, D, I32, P, P,
+                                                      P, P]
+    L.gromos_amd_rf_excluded_interactions.argtypes = [P, P, P, I32, I32, P, C.POINTER(Crf), I32, P, D, P, P, P, P]
+    L.gromos_amd_one_four_interaction_loop.argtypes = [P, P, I32, P, P, P, P, I32, C.POINTER(Crf), I32, P, D, P,
+                                                       P, P, D, P]
+    L.gromos_amd_perturbed_pairlist_correction.argtypes = [P, P, I64, C.POINTER(Pert), P, P, P, I32,
+                                                           C.POINTER(Crf), C.POINTER(Lambda), I32, P, D, P, P, P]
+    L.gromos_amd_system_create.argtypes = [C.POINTER(Topology), C.POINTER(Params), I32, C.POINTER(P)]
+    L.gromos_amd_system_destroy.argtypes = [P]
+    L.gromos_amd_system_set_state.argtypes = [P, P, P]
+    L.gromos_amd_system_get_state.argtypes = [P, P, P]
+    L.gromos_amd_system_pairlist_update.argtypes = [P]
+    L.gromos_amd_system_forcefield_apply.argtypes = [P, P, P, P]
+    L.gromos_amd_system_md.argtypes = [P, I32, P]
+    L.gromos_amd_system_pairlist_get.argtypes = [P, P, P, P, P, P]
+    L.gromos_amd_system_pair_count.argtypes = [P]
+    L.gromos_amd_system_pair_count.restype = I64
+    L.gromos_amd_system_kernel_timing.argtypes = [P, I32, P, P]
+    L.gromos_amd_system_device_bytes.argtypes = [P]
+    L.gromos_amd_system_device_bytes.restype = I64
+
+
+def check(rc):
+    if rc != 0:
+        raise GromosAmdError(f"gromos_amd error {rc}: {lib().gromos_amd_last_error().decode()}")
+    return rc
+
+
+def ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count():
+    return lib().gromos_amd_device_count()
+
+
+def crf_new(cutoff, epsilon=1.0, rf_epsilon=61.0, rf_kappa=0.0):
+    c = Crf()
+    check(lib().gromos_amd_crf_new(cutoff, epsilon, rf_epsilon, rf_kappa, C.byref(c)))
+    return c
+
+
+def lambda_from(lam, n=1, ljs=None, crfs=None, deriv=1.0):
+    lp = Lambda()
+    ljs = lam if ljs is None else ljs
+    crfs = lam if crfs is None else crfs
+    check(lib().gromos_amd_lambda_from(lam, ljs, lam, crfs, deriv, deriv, deriv, deriv, n, C.byref(lp)))
+    return lp
+
+
+class _Keep:
+    """Holds numpy arrays alive while C structs point into them."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def __call__(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.arrays.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+def topology_struct(s, keep):
+    t = Topology()
+    t.n_atoms, t.n_solute_atoms = s.n_atoms, s.n_solute_atoms
+    t.n_cg, t.n_solute_cg = len(s.cg_start) - 1, s.n_solute_cg
+    t.cg_start = keep(s.cg_start, np.int32)
+    t.charge = keep(s.charge, np.float64)
+    t.iac = keep(s.iac, np.int32)
+    t.mass = keep(s.mass, np.float64)
+    t.n_types = s.n_types
+    t.lj = keep(s.lj, np.float64)
+    t.excl_off, t.excl = keep(s.excl_off, np.int32), keep(s.excl, np.int32)
+    t.o14_off, t.o14 = keep(s.o14_off, np.int32), keep(s.o14, np.int32)
+    t.atoms_per_solvent = s.atoms_per_solvent
+    t.dist_oh, t.dist_hh = s.dist_oh, s.dist_hh
+    if s.pert is not None and np.any(s.pert.flag):
+        p = s.pert
+        t.pert_flag = keep(p.flag, np.uint8)
+        t.pert_a_iac, t.pert_b_iac = keep(p.a_iac, np.int32), keep(p.b_iac, np.int32)
+        t.pert_a_q, t.pert_b_q = keep(p.a_q, np.float64), keep(p.b_q, np.float64)
+        t.pert_alpha_lj, t.pert_alpha_crf = keep(p.alpha_lj, np.float64), keep(p.alpha_crf, np.float64)
+        ap = np.asarray(p.atom_pairs, np.int32).reshape(-1)
+        t.n_pert_atom_pairs = len(ap) // 4
+        t.pert_atom_pairs = keep(ap, np.int32)
+    return t
+
+
+def params_struct(s, use_settle=True, do_virial=False):
+    p = Params()
+    p.boundary = s.boundary
+    for a in range(3):
+        p.box[a] = float(s.box[a])
+    p.rcutp, p.rcutl, p.skin = s.rcutp, s.rcutl, s.skin
+    p.nsnb = s.nsnb
+    p.epsrf, p.kappa, p.four_pi_eps_i, p.dt = s.epsrf, s.kappa, s.fpepsi, s.dt
+    p.use_settle, p.do_virial = int(use_settle), int(do_virial)
+    p.lam = s.pert.lam if s.pert is not None else 0.0
+    p.nlam = s.pert.nlam if s.pert is not None else 1
+    return p
+
+
+class DeviceSystem:
+    """One MD job resident in HBM (gromos_amd_system)."""
+
+    def __init__(self, s, precision=FP64, use_settle=True, do_virial=False):
+        self.s = s
+        keep = _Keep()
+        self._topo = topology_struct(s, keep)
+        self._keep = keep
+        self._prm = params_struct(s, use_settle and s.nsm > 0 and s.atoms_per_solvent == 3, do_virial)
+        h = P()
+        check(lib().gromos_amd_system_create(C.byref(self._topo), C.byref(self._prm), precision, C.byref(h)))
+        self.h = h
+        self.n = s.n_atoms
+        self.set_state(s.pos, s.vel)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gromos_amd_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, pos, vel=None):
+        x = np.ascontiguousarray(pos, np.float64).reshape(-1)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float64).reshape(-1)
+        check(lib().gromos_amd_system_set_state(self.h, ptr(x), ptr(v)))
+
+    def get_state(self):
+        x, v = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        check(lib().gromos_amd_system_get_state(self.h, ptr(x), ptr(v)))
+        return x, v
+
+    def pairlist_update(self):
+        check(lib().gromos_amd_system_pairlist_update(self.h))
+
+    def forcefield_apply(self):
+        f = np.zeros((self.n, 3))
+        e = np.zeros(3)
+        vir = np.zeros(9)
+        check(lib().gromos_amd_system_forcefield_apply(self.h, ptr(f), ptr(e), ptr(vir)))
+        return dict(forces=f, e_lj=e[0], e_crf=e[1], dhdl=e[2], virial=vir.reshape(3, 3))
+
+    def md(self, nsteps):
+        e = np.zeros((nsteps, 5))
+        check(lib().gromos_amd_system_md(self.h, int(nsteps), ptr(e)))
+        return e
+
+    def pairlist(self):
+        n = np.zeros(4, np.int64)
+        check(lib().gromos_amd_system_pairlist_get(self.h, ptr(n), None, None, None, None))
+        bufs = [np.zeros((max(int(k), 1), 2), np.uint32) for k in n]
+        check(lib().gromos_amd_system_pairlist_get(self.h, ptr(n), *[ptr(b) for b in bufs]))
+        names = ("solute_short", "solute_long", "solvent_short", "solvent_long")
+        return {k: b[:int(c)] for k, b, c in zip(names, bufs, n)}
+
+    @property
+    def pair_count(self):
+        return lib().gromos_amd_system_pair_count(self.h)
+
+    @property
+    def device_bytes(self):
+        return lib().gromos_amd_system_device_bytes(self.h)
+
+    def kernel_timing(self, enable=-1):
+        ms, n = D(), I64()
+        check(lib().gromos_amd_system_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
+        return ms.value, n.value

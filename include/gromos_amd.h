@@ -1,0 +1,239 @@
+/*
+ * gromos_amd: MI355X (gfx950) nonbonded hot path for gromos-rs.
+ *
+ * C ABI: plain pointers and sizes.  Two levels:
+ *
+ *  1. Pair-list kernels.  Drop-ins for the gromos-forces free functions
+ *     (crates/gromos-forces/src/nonbonded/{innerloops,excluded,perturbed}.rs).
+ *     Every pointer is DEVICE memory; results ACCUMULATE into the caller's
+ *     force / energy / virial buffers like the reference's `&mut ForceStorage`
+ *     (callers clear first, forcefield.rs:418).  `stream` is a hipStream_t
+ *     (NULL = default stream); calls are asynchronous on it.
+ *
+ *  2. Device-resident systems.  One or more MD jobs kept in HBM (positions,
+ *     velocities, pair lists); the GPU builds the charge-group pair list
+ *     (pairlist.rs:183-442 semantics) and runs Forcefield::apply
+ *     (forcefield.rs:226-1060, nonbonded part), leap-frog, SETTLE and E_kin.
+ *     Host pointers here are HOST memory; the library copies.
+ *
+ * Conventions (both levels):
+ *   - return 0 on success, a negative gromos_amd_status on error; the message
+ *     of the last error on the calling thread is gromos_amd_last_error();
+ *   - positions / forces are (x, y, z) per atom, atom-major, f64;
+ *   - the virial is row-major: vir[3*a + b] = sum r[a] * f[b]
+ *     (innerloops.rs:199-207);
+ *   - the LJ table is flat [ti * n_types + tj] x (c6, c12, cs6, cs12)
+ *     (LJParamMatrix / LJParameters, params.rs:6-58); type indices are 0-based;
+ *   - boundary: 0 = vacuum, 1 = rectangular (math.rs:20-74).
+ * A system handle must be used from one host thread at a time.
+ */
+#ifndef GROMOS_AMD_H
+#define GROMOS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gromos_amd_status {
+    GROMOS_AMD_OK = 0,
+    GROMOS_AMD_EINVAL = -1,     /* bad argument / shape */
+    GROMOS_AMD_EHIP = -2,       /* HIP runtime error */
+    GROMOS_AMD_ENOMEM = -3,     /* device allocation failed / over budget */
+    GROMOS_AMD_ESETTLE = -4,    /* SETTLE failed (constraints.rs:896-951) */
+    GROMOS_AMD_EBUSY = -5,      /* no admission (server full) */
+    GROMOS_AMD_ENOTFOUND = -6
+};
+
+/* CRFParameters (params.rs:60-68). */
+typedef struct {
+    double crf_cut;
+    double crf_2cut3i;
+    double crf_cut3i;
+    double cutoff_sq;
+} gromos_amd_crf;
+
+const char *gromos_amd_last_error(void);
+const char *gromos_amd_version(void);
+/* number of visible HIP devices (0 when none) */
+int gromos_amd_device_count(void);
+
+/* CRFParameters::new (params.rs:78-97); pure host arithmetic. */
+int gromos_amd_crf_new(double cutoff, double epsilon, double rf_epsilon, double rf_kappa,
+                       gromos_amd_crf *out);
+
+/* ------------------------------------------------------------------ */
+/* 1. pair-list kernels (device pointers, accumulate semantics)         */
+/* ------------------------------------------------------------------ */
+
+/* lj_crf_innerloop[_novirial] (innerloops.rs:318-368): per-pair nearest image.
+ * pairs: 2*n_pairs u32 (i, j).  energies: 2 doubles (e_lj, e_crf).
+ * virial may be NULL when do_virial == 0. */
+int gromos_amd_lj_crf_innerloop(const double *pos, const double *charges, const uint32_t *iac,
+                                const uint32_t *pairs, int64_t n_pairs, const double *lj,
+                                int32_t n_types, const gromos_amd_crf *crf, int32_t boundary,
+                                const double box[3], double four_pi_eps_i, int32_t do_virial,
+                                double *forces, double *energies, double *virial, void *stream);
+
+/* lj_crf_innerloop_cg_grouped[_novirial] (innerloops.rs:374-428).
+ * groups: 4*n_groups u32 (ref_atom_i, ref_atom_j, start, end) (CGPairGroup, params.rs:145-154). */
+int gromos_amd_lj_crf_innerloop_cg_grouped(const double *pos, const double *charges,
+                                           const uint32_t *iac, const uint32_t *pairs,
+                                           const uint32_t *groups, int64_t n_groups,
+                                           const double *lj, int32_t n_types,
+                                           const gromos_amd_crf *crf, int32_t boundary,
+                                           const double box[3], double four_pi_eps_i,
+                                           int32_t do_virial, double *forces, double *energies,
+                                           double *virial, void *stream);
+
+/* solvent_innerloop[_novirial] (innerloops.rs:679-733): sentinel pairs of
+ * first atoms, atoms_per_solvent <= 4, tables from the first pair's molecule. */
+int gromos_amd_solvent_innerloop(const double *pos, const double *charges, const uint32_t *iac,
+                                 const uint32_t *pairs, int64_t n_pairs, const double *lj,
+                                 int32_t n_types, const gromos_amd_crf *crf, int32_t boundary,
+                                 const double box[3], int32_t atoms_per_solvent,
+                                 double four_pi_eps_i, int32_t do_virial, double *forces,
+                                 double *energies, double *virial, void *stream);
+
+/* rf_excluded_interactions (excluded.rs:25-112).  excl_off/excl: CSR over all
+ * atoms, sorted symmetric lists (Topology::exclusions). */
+int gromos_amd_rf_excluded_interactions(const double *charges, const int32_t *excl_off,
+                                        const int32_t *excl, int32_t n_atoms,
+                                        int32_t num_solute_atoms, const double *pos,
+                                        const gromos_amd_crf *crf, int32_t boundary,
+                                        const double box[3], double four_pi_eps_i,
+                                        double *forces, double *energies, double *virial,
+                                        void *stream);
+
+/* one_four_interaction_loop (excluded.rs:131-206).  o14_off/o14: CSR of
+ * Topology::one_four_pairs (pairs with j > i are evaluated). */
+int gromos_amd_one_four_interaction_loop(const int32_t *o14_off, const int32_t *o14,
+                                         int32_t n_atoms, const double *pos,
+                                         const double *charges, const uint32_t *iac,
+                                         const double *lj, int32_t n_types,
+                                         const gromos_amd_crf *crf, int32_t boundary,
+                                         const double box[3], double four_pi_eps_i,
+                                         double *forces, double *energies, double *virial,
+                                         double coulomb_scaling, void *stream);
+
+/* PerturbedLambdaParams (perturbed.rs:11-50), from_lambda (perturbed.rs:57-101). */
+typedef struct {
+    double a_lj_lambda_n, b_lj_lambda_n, a_crf_lambda_n, b_crf_lambda_n;
+    double a_lj_lambda_n_1, b_lj_lambda_n_1, a_crf_lambda_n_1, b_crf_lambda_n_1;
+    double a_ljs_lambda, b_ljs_lambda, a_ljs_lambda2, b_ljs_lambda2;
+    double a_crfs_lambda, b_crfs_lambda, a_crfs_lambda2, b_crfs_lambda2;
+    int32_t lambda_exp;
+} gromos_amd_lambda;
+
+int gromos_amd_lambda_from(double lj_lambda, double ljs_lambda, double crf_lambda,
+                           double crfs_lambda, double lj_deriv, double ljs_deriv,
+                           double crf_deriv, double crfs_deriv, int32_t n,
+                           gromos_amd_lambda *out);
+
+/* Per-atom perturbation info (PertAtomInfo, perturbed.rs:271-280), device arrays;
+ * flag[i] != 0 marks a perturbed atom; alphas already include ALPHLJ / ALPHC. */
+typedef struct {
+    const uint8_t *flag;
+    const int32_t *a_iac, *b_iac;
+    const double *a_q, *b_q, *alpha_lj, *alpha_crf;
+} gromos_amd_pert;
+
+/* perturbed_pairlist_correction (perturbed.rs:309-386).  out_energies: 3 doubles
+ * accumulated (delta_e_lj, delta_e_crf, dhdl); forces accumulate the delta. */
+int gromos_amd_perturbed_pairlist_correction(const double *pos, const uint32_t *pairs,
+                                             int64_t n_pairs, const gromos_amd_pert *pert,
+                                             const double *charges, const uint32_t *iac,
+                                             const double *lj, int32_t n_types,
+                                             const gromos_amd_crf *crf,
+                                             const gromos_amd_lambda *lp, int32_t boundary,
+                                             const double box[3], double four_pi_eps_i,
+                                             double *forces, double *out_energies, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 2. device-resident systems                                          */
+/* ------------------------------------------------------------------ */
+
+/* Topology essentials of one job (host memory). */
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_solute_atoms;
+    int32_t n_cg;
+    int32_t n_solute_cg;
+    const int32_t *cg_start;        /* n_cg + 1; contiguous charge groups */
+    const double *charge;
+    const int32_t *iac;
+    const double *mass;
+    int32_t n_types;
+    const double *lj;               /* n_types^2 * 4 */
+    const int32_t *excl_off, *excl; /* sorted symmetric exclusions */
+    const int32_t *o14_off, *o14;   /* one_four_pairs as read */
+    int32_t atoms_per_solvent;
+    /* SETTLE template (constraints.rs:826-835): O-H and H-H lengths */
+    double dist_oh, dist_hh;
+    /* optional perturbation (NULL flag = none) */
+    const uint8_t *pert_flag;
+    const int32_t *pert_a_iac, *pert_b_iac;
+    const double *pert_a_q, *pert_b_q, *pert_alpha_lj, *pert_alpha_crf;
+    int32_t n_pert_atom_pairs;
+    const int32_t *pert_atom_pairs; /* i, j, a_type, b_type (-1 = absent in B) */
+} gromos_amd_topology;
+
+/* Run parameters (IMD PAIRLIST / NONBONDED / STEP / PERTURBATION blocks). */
+typedef struct {
+    int32_t boundary;
+    double box[3];
+    double rcutp, rcutl, skin;
+    int32_t nsnb;
+    double epsrf, kappa;            /* CRF from rcutl, epsilon = 1 (md.rs:326-329) */
+    double four_pi_eps_i;
+    double dt;
+    int32_t use_settle;
+    int32_t do_virial;
+    double lambda;
+    int32_t nlam;
+} gromos_amd_params;
+
+enum gromos_amd_precision {
+    GROMOS_AMD_FP64 = 0,   /* plain: FP64 throughout, reference-exact pair list */
+    GROMOS_AMD_MIXED = 1   /* FP32 pair math, int64 fixed-point forces, FP64 sums/SETTLE */
+};
+
+typedef struct gromos_amd_system gromos_amd_system;
+
+int gromos_amd_system_create(const gromos_amd_topology *topo, const gromos_amd_params *prm,
+                             int32_t precision, gromos_amd_system **out);
+void gromos_amd_system_destroy(gromos_amd_system *sys);
+/* positions, velocities: host (n_atoms * 3) f64; vel may be NULL (zeros). */
+int gromos_amd_system_set_state(gromos_amd_system *sys, const double *pos, const double *vel);
+int gromos_amd_system_get_state(gromos_amd_system *sys, double *pos, double *vel);
+/* Initial pair list before the step loop (md.rs:866-868); resets the rebuild counter. */
+int gromos_amd_system_pairlist_update(gromos_amd_system *sys);
+/* One Forcefield::apply on the current positions (rebuilds on its cadence).
+ * Host outputs, any may be NULL: forces (3n), energies (e_lj, e_crf, dhdl), virial (9). */
+int gromos_amd_system_forcefield_apply(gromos_amd_system *sys, double *forces, double *energies,
+                                       double *virial);
+/* nsteps of Forcefield -> LeapFrogVelocity -> LeapFrogPosition -> SETTLE ->
+ * TemperatureCalculation (md.rs:1143-1286).  energies (optional, host): per step
+ * 5 doubles (e_lj, e_crf, e_pot, e_kin, dhdl). */
+int gromos_amd_system_md(gromos_amd_system *sys, int32_t nsteps, double *energies);
+/* Pair lists materialized as the reference's four atom-pair lists (test mode).
+ * Pass NULL buffers to query counts (n_out[4]: solute_short, solute_long,
+ * solvent_short, solvent_long); then call again with buffers of 2*n u32. */
+int gromos_amd_system_pairlist_get(gromos_amd_system *sys, int64_t n_out[4],
+                                   uint32_t *solute_short, uint32_t *solute_long,
+                                   uint32_t *solvent_short, uint32_t *solvent_long);
+/* Pair interactions evaluated (counted as in the reference: n_i * n_j per CG pair,
+ * long pairs only when recomputed) since creation. */
+int64_t gromos_amd_system_pair_count(const gromos_amd_system *sys);
+/* Device timing of the dominant nonbonded kernel: hipEvents recorded around each
+ * launch while enabled.  total_ms / launches since enable. */
+int gromos_amd_system_kernel_timing(gromos_amd_system *sys, int32_t enable, double *total_ms,
+                                    int64_t *launches);
+/* HBM bytes held by the system's device buffers. */
+int64_t gromos_amd_system_device_bytes(const gromos_amd_system *sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -307,7 +307,7 @@ struct gromos_amd_system {
 
     // device arrays
     JobDev* d_jobs = nullptr;
-    int32_t *d_cg_start = nullptr, *d_cg_job = nullptr, *d_atom_job = nullptr, *d_iac = nullptr;
+    int32_t *d_cg_start = nullptr, *d_cg_n = nullptr, *d_cg_job = nullptr, *d_atom_job = nullptr, *d_iac = nullptr;
     double *d_q = nullptr, *d_mass = nullptr, *d_inv_mass = nullptr, *d_lj = nullptr;
     int32_t *d_excl_off = nullptr, *d_excl = nullptr, *d_o14_off = nullptr, *d_o14 = nullptr;
     int32_t *d_o14s_off = nullptr, *d_o14s = nullptr, *d_mol_first = nullptr;
@@ -390,6 +390,7 @@ struct gromos_amd_system {
         SysPtrs p{};
         p.jobs = d_jobs;
         p.cg_start = d_cg_start;
+        p.cg_n = d_cg_n;
         p.cg_job = d_cg_job;
         p.atom_job = d_atom_job;
         p.q = d_q;
@@ -490,7 +491,7 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     S->precision = precision;
     S->njobs = njobs;
     int64_t atom_off = 0, cg_off = 0, lj_off = 0, cell_off = 0;
-    std::vector<int32_t> cg_start, cg_job, atom_job, iac, excl_off, excl, o14_off, o14, o14s_off, o14s,
+    std::vector<int32_t> cg_start, cg_n, cg_job, atom_job, iac, excl_off, excl, o14_off, o14, o14s_off, o14s,
         mol_first;
     std::vector<double> q, mass, inv_mass, lj;
     excl_off.push_back(0);
@@ -583,6 +584,7 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         // arrays
         for (int c = 0; c < T.n_cg; ++c) {
             cg_start.push_back((int32_t)atom_off + T.cg_start[c]);
+            cg_n.push_back(T.cg_start[c + 1] - T.cg_start[c]);
             cg_job.push_back(j);
         }
         int64_t npad_j = align_up(n, BLOCK);
@@ -620,7 +622,6 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         atom_off += npad_j;
         cg_off += T.n_cg;
     }
-    cg_start.push_back((int32_t)atom_off);
     S->npad = (int32_t)atom_off;
     S->ncg = (int32_t)cg_off;
     S->ncell = (int32_t)cell_off;
@@ -637,6 +638,7 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     std::vector<JobDev> jobs = S->jobs;
     rc = rc ? rc : S->upload(S->d_jobs, jobs);
     rc = rc ? rc : S->upload(S->d_cg_start, cg_start);
+    rc = rc ? rc : S->upload(S->d_cg_n, cg_n);
     rc = rc ? rc : S->upload(S->d_cg_job, cg_job);
     rc = rc ? rc : S->upload(S->d_atom_job, atom_job);
     rc = rc ? rc : S->upload(S->d_q, q);

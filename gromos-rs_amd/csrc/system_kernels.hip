@@ -24,7 +24,7 @@ __global__ void k_cg_ref(SysPtrs p, int32_t ncg) {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncg) return;
     const JobDev& J = p.jobs[p.cg_job[c]];
-    int32_t s = p.cg_start[c], e = p.cg_start[c + 1];
+    int32_t s = p.cg_start[c], e = s + p.cg_n[c];
     const double* x = p.x_cur;
     double rx, ry, rz;
     if (c - J.cg_off < J.n_solute_cg) {
@@ -73,7 +73,7 @@ __global__ void k_sorted_cg(SysPtrs p, int32_t ncg) {
     if (k >= ncg) return;
     int32_t c = (int32_t)p.val_out[k];
     p.s_cg_orig[k] = c;
-    p.s_cg_n[k] = p.cg_start[c + 1] - p.cg_start[c];
+    p.s_cg_n[k] = p.cg_n[c];
     p.s_ref[3 * k] = p.ref[3 * c];
     p.s_ref[3 * k + 1] = p.ref[3 * c + 1];
     p.s_ref[3 * k + 2] = p.ref[3 * c + 2];

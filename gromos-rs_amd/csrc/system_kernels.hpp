@@ -7,7 +7,8 @@ namespace gamd {
 struct SysPtrs {
     const JobDev* jobs;
     // original-order topology (global over the batch)
-    const int32_t* cg_start;   // ncg + 1, global atom offsets
+    const int32_t* cg_start;   // ncg, first (global) atom of each charge group
+    const int32_t* cg_n;       // ncg, atoms per charge group
     const int32_t* cg_job;     // ncg
     const int32_t* atom_job;   // npad, -1 on padding
     const double* q;

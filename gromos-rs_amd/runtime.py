@@ -66,8 +66,16 @@ def built():
 
 
 def lib():
+    """Load libgromos_amd.so.  torch is imported first so that the process has a
+    single HIP runtime: torch's bundled libamdhip64.so.7 then also serves this
+    library (both carry the same SONAME, and a second copy loaded first would
+    leave torch without a device)."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not built():
             raise GromosAmdError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
                                  "(the HIP library is required; there is no CPU fallback)")

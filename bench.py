@@ -35,7 +35,22 @@ def parse():
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """One torch device copy of known size: FETCH_SIZE / WRITE_SIZE of this
+    dispatch calibrate the counters against algorithmic bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "read": n * 4, "written": n * 4}))
 
 
 def cpu_baseline(s, nsteps):
@@ -62,6 +77,9 @@ def cpu_baseline(s, nsteps):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); independent jobs shard across GPUs "
               "without per-step communication, see the design notes", file=sys.stderr)
@@ -92,7 +110,7 @@ def main():
         t = time.perf_counter()
         ds.md(1)
         lat.append((time.perf_counter() - t) * 1e3)
-    lat = np.array(lat)
+    lat = np.array(lat) if lat else np.array([float("nan")])
     # dominant kernel: the solvent-solvent pair kernel, one launch per step
     vv_pairs_per_step = ds_vv_pairs(ds, s)
     avg_ms = kern_ms / max(launches, 1)

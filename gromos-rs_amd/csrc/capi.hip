@@ -322,6 +322,9 @@ struct gromos_amd_system {
     int64_t vv_cap = 0, gen_cap = 0;
     uint64_t* d_counters = nullptr;
     double4* d_xq = nullptr;
+    float4* d_xf = nullptr;
+    long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
+    int32_t nrow_blocks = 0;
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr, *d_f_p = nullptr;
     double *d_rows = nullptr, *d_records = nullptr, *d_pert_e = nullptr;
     int64_t rec_cap = 0;
@@ -433,6 +436,10 @@ struct gromos_amd_system {
         p.gen_ent = d_gen_ent;
         p.counters = d_counters;
         p.xq = d_xq;
+        p.xf = d_xf;
+        p.fi_s = d_fi_s;
+        p.fi_long_s = d_fi_long_s;
+        p.nrow_blocks = nrow_blocks;
         p.f_s = d_f_s;
         p.f_long_s = d_f_long_s;
         p.f_x = d_f_x;
@@ -626,6 +633,8 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     S->ncg = (int32_t)cg_off;
     S->ncell = (int32_t)cell_off;
     S->nmol = (int32_t)mol_first.size();
+    S->nrow_blocks = (int32_t)(align_up((int64_t)S->npad * 4, BLOCK) / BLOCK);   // 4 lanes per atom in the pair kernel
+    if (S->npad >= (int32_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many atoms");
     S->nsnb = std::max(1, P0.nsnb);
     S->use_settle = P0.use_settle;
     S->do_virial = P0.do_virial;
@@ -672,13 +681,16 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         (rc = S->alloc(S->d_vv_off, ncg + 1)) || (rc = S->alloc(S->d_gen_off, ncg + 1)) ||
         (rc = S->alloc(S->d_counters, 2)) || (rc = S->alloc(S->d_xq, (size_t)S->npad)) ||
         (rc = S->alloc(S->d_f_s, n3)) || (rc = S->alloc(S->d_f_long_s, n3)) || (rc = S->alloc(S->d_f_x, n3)) ||
-        (rc = S->alloc(S->d_f_p, n3)) || (rc = S->alloc(S->d_rows, (size_t)NSLOT * (S->npad / BLOCK) * ROW)) ||
+        (rc = S->alloc(S->d_f_p, n3)) || (rc = S->alloc(S->d_xf, (size_t)S->npad)) ||
+        (rc = S->alloc(S->d_fi_s, n3)) || (rc = S->alloc(S->d_fi_long_s, n3)) ||
+        (rc = S->alloc(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW)) ||
         (rc = S->alloc(S->d_pert_e, (size_t)3 * njobs)) || (rc = S->alloc(S->d_settle_fail, (size_t)njobs)))
         return rc;
     HIPCHK(hipMemset(S->d_f_long_s, 0, n3 * sizeof(double)));
     HIPCHK(hipMemset(S->d_f_p, 0, n3 * sizeof(double)));
     HIPCHK(hipMemset(S->d_pert_e, 0, 3 * njobs * sizeof(double)));
-    HIPCHK(hipMemset(S->d_rows, 0, (size_t)NSLOT * (S->npad / BLOCK) * ROW * sizeof(double)));
+    HIPCHK(hipMemset(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double)));
+    HIPCHK(hipMemset(S->d_fi_long_s, 0, n3 * sizeof(long long)));
     HIPCHK(hipMemset(S->d_s_cg_n, 0, (ncg + 1) * sizeof(int32_t)));
     HIPCHK(hipMemset(S->d_vv_cnt, 0, (ncg + 1) * sizeof(int64_t)));
     HIPCHK(hipMemset(S->d_gen_cnt, 0, (ncg + 1) * sizeof(int64_t)));
@@ -796,7 +808,8 @@ static int forcefield(gromos_amd_system* S, int32_t rec_step, bool with_kin_rows
     S->steps_since_update++;
     SysPtrs p = S->ptrs();
     int32_t npad = S->npad;
-    launch_gather(p, npad, st);
+    const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
+    launch_gather(p, npad, mixed, st);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (S->timing) {
         e0 = S->next_event();
@@ -804,17 +817,17 @@ static int forcefield(gromos_amd_system* S, int32_t rec_step, bool with_kin_rows
         HIPCHK(hipEventRecord(e0, st));
     }
     int32_t aps = S->jobs[0].aps > 0 ? S->jobs[0].aps : 3;
-    launch_vv(p, npad, aps, S->do_virial, st);
+    launch_vv(p, npad, aps, S->do_virial, mixed, st);
     if (S->timing) HIPCHK(hipEventRecord(e1, st));
-    launch_gen(p, npad, false, S->do_virial, st);
+    launch_gen(p, npad, false, S->do_virial, mixed, st);
     S->pair_count += (int64_t)S->host_counters[0];
     if (S->twin && (rebuild || !S->long_computed)) {
-        launch_gen(p, npad, true, 1, st);
+        launch_gen(p, npad, true, 1, mixed, st);
         S->long_computed = true;
         S->pair_count += (int64_t)S->host_counters[1];
     }
     launch_excl14(p, npad, st);
-    launch_assemble(p, npad, S->twin ? 1 : 0, 0, st);
+    launch_assemble(p, npad, S->twin ? 1 : 0, 0, mixed, st);
     (void)with_kin_rows;
     (void)rec_step;
     HIPCHK(hipGetLastError());

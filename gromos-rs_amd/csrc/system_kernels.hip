@@ -105,25 +105,33 @@ __device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
     return r.x * r.x + r.y * r.y + r.z * r.z;
 }
 
-// Neighbour search for one sorted group: 27 cells (all cells along an axis with
-// fewer than 3), the reference's classification (pairlist.rs:252-415).
+// Neighbour search, one wave per sorted group k: the 64 lanes test 64
+// candidates of a cell at once (coalesced loads of the sorted reference points)
+// and the survivors are compacted with a ballot, so each group's entries are
+// written contiguously and in a fixed order (cells in z-y-x order, candidates
+// in sorted order).  27 cells, or all cells along an axis with fewer than 3.
+// Classification as the reference (pairlist.rs:252-415); the distance test is
+// nearest_image(lower group, higher group) in the reference's arithmetic.
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncg) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int32_t k = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x / WAVE);
+    if (k >= ncg) return;   // wave-uniform
     int32_t ck = p.s_cg_orig[k];
     const JobDev& J = p.jobs[p.cg_job[ck]];
-    bool sol_k = ck - J.cg_off < J.n_solute_cg;
+    const Box box = J.box;
+    const double c2s = J.c2s, c2l = J.c2l;
+    const int32_t cg_off = J.cg_off, nsol = J.n_solute_cg, aps = J.aps;
+    bool sol_k = ck - cg_off < nsol;
     d3 rk{p.s_ref[3 * k], p.s_ref[3 * k + 1], p.s_ref[3 * k + 2]};
+    int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
     int32_t local = (int32_t)p.key_out[k] - J.cell_off;
-    int32_t cx = local % J.ncell[0];
-    int32_t cy = (local / J.ncell[0]) % J.ncell[1];
-    int32_t cz = local / (J.ncell[0] * J.ncell[1]);
-    int32_t cc[3] = {cx, cy, cz};
+    int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
+    int32_t nd[3] = {nx, ny, nz};
     int32_t lst[3][3], nl[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int32_t n = J.ncell[a];
+        int32_t n = nd[a];
         if (n >= 3) {
             nl[a] = 3;
             lst[a][0] = (cc[a] + n - 1) % n;
@@ -136,49 +144,71 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
             lst[a][2] = 2;
         }
     }
-    int32_t nvv = 0, ngen = 0;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t nvv = 0, ngen = 0;
     int64_t vv_base = FILL ? p.vv_off[k] : 0, gen_base = FILL ? p.gen_off[k] : 0;
     uint64_t pairs_short = 0, pairs_long = 0;
     int32_t nk = p.s_cg_n[k];
     for (int32_t iz = 0; iz < nl[2]; ++iz)
         for (int32_t iy = 0; iy < nl[1]; ++iy)
             for (int32_t ix = 0; ix < nl[0]; ++ix) {
-                int32_t cell = J.cell_off + lst[0][ix] + J.ncell[0] * (lst[1][iy] + J.ncell[1] * lst[2][iz]);
+                int32_t cell = J.cell_off + lst[0][ix] + nx * (lst[1][iy] + ny * lst[2][iz]);
                 int32_t m0 = p.cell_start[cell], m1 = p.cell_start[cell + 1];
-                for (int32_t m = m0; m < m1; ++m) {
-                    if (m == k) continue;
-                    int32_t cm = p.s_cg_orig[m];
-                    d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
-                    bool canon = ck < cm;
-                    double d2 = canon ? ref_d2(J.box, rk, rm) : ref_d2(J.box, rm, rk);
-                    if (d2 > J.c2l) continue;
-                    bool lng = d2 > J.c2s;
-                    bool sol_m = cm - J.cg_off < J.n_solute_cg;
-                    uint32_t flags = canon ? ENT_CANON : 0u;
-                    if (!sol_k && !sol_m) {
-                        if (!lng) {
-                            if (FILL) p.vv_ent[vv_base + nvv] = (uint32_t)m | flags;
-                            ++nvv;
-                            if (canon) pairs_short += (uint64_t)J.aps * J.aps;
-                            continue;
+                for (int32_t base = m0; base < m1; base += WAVE) {
+                    int32_t m = base + lane;
+                    bool is_vv = false, is_gen = false;
+                    uint32_t ent = 0;
+                    if (m < m1 && m != k) {
+                        int32_t cm = p.s_cg_orig[m];
+                        d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
+                        bool canon = ck < cm;
+                        d3 lo = canon ? rk : rm, hi = canon ? rm : rk;
+                        d3 r = nim(box, lo, hi);
+                        double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                        if (!(d2 > c2l)) {
+                            bool lng = d2 > c2s;
+                            bool sol_m = cm - cg_off < nsol;
+                            uint32_t flags = canon ? ENT_CANON : 0u;
+                            if (!sol_k && !sol_m && !lng) {
+                                is_vv = true;
+                                // solvent partners: store the sorted first-atom index
+                                ent = (uint32_t)p.s_cg_start[m] | flags;
+                                if (canon) pairs_short += (uint64_t)aps * aps;
+                            } else {
+                                is_gen = true;
+                                if (!sol_k && !sol_m) flags |= ENT_VVLONG;
+                                if (lng) flags |= ENT_LONG;
+                                if (sol_k && sol_m) flags |= ENT_SS;
+                                ent = (uint32_t)m | flags;
+                                if (canon) {
+                                    uint64_t np = (uint64_t)nk * p.s_cg_n[m];
+                                    if (lng) pairs_long += np; else pairs_short += np;
+                                }
+                            }
                         }
-                        flags |= ENT_VVLONG;
                     }
-                    if (lng) flags |= ENT_LONG;
-                    if (sol_k && sol_m) flags |= ENT_SS;
-                    if (FILL) p.gen_ent[gen_base + ngen] = (uint32_t)m | flags;
-                    ++ngen;
-                    if (canon) {
-                        uint64_t np = (uint64_t)nk * p.s_cg_n[m];
-                        if (lng) pairs_long += np; else pairs_short += np;
+                    uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
+                    if (FILL) {
+                        if (is_vv) p.vv_ent[vv_base + nvv + __popcll(bv & below)] = ent;
+                        if (is_gen) p.gen_ent[gen_base + ngen + __popcll(bg & below)] = ent;
                     }
+                    nvv += __popcll(bv);
+                    ngen += __popcll(bg);
                 }
             }
     if (!FILL) {
-        p.vv_cnt[k] = nvv;
-        p.gen_cnt[k] = ngen;
-        if (pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
-        if (pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
+        if (lane == 0) {
+            p.vv_cnt[k] = nvv;
+            p.gen_cnt[k] = ngen;
+        }
+        // per-wave sums, one atomic per wave
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            pairs_short += __shfl_xor(pairs_short, o, WAVE);
+            pairs_long += __shfl_xor(pairs_long, o, WAVE);
+        }
+        if (lane == 0 && pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
+        if (lane == 0 && pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
     }
 }
 
@@ -186,13 +216,23 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
 // forces
 // ---------------------------------------------------------------------------
 
-__global__ void k_gather(SysPtrs p, int32_t npad) {
+__global__ void k_gather(SysPtrs p, int32_t npad, int32_t mixed) {
     int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= npad) return;
     int32_t o = p.s2o[a];
     double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
     if (o >= 0) v = make_double4(p.x_cur[3 * o], p.x_cur[3 * o + 1], p.x_cur[3 * o + 2], p.q[o]);
     p.xq[a] = v;
+    if (mixed) {
+        // FP32 offset from the group's first atom (difference taken in FP64)
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (o >= 0) {
+            int32_t f = p.cg_start[p.s_cg_orig[p.s_atom_cg[a]]];
+            d = make_float4((float)(v.x - p.x_cur[3 * f]), (float)(v.y - p.x_cur[3 * f + 1]),
+                            (float)(v.z - p.x_cur[3 * f + 2]), (float)v.w);
+        }
+        p.xf[a] = d;
+    }
 }
 
 __device__ __forceinline__ d3 xyz(double4 v) { return d3{v.x, v.y, v.z}; }
@@ -212,17 +252,28 @@ __device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int sl
 
 // Solvent-solvent short pairs (process_solvent_pairs, innerloops.rs:218-309):
 // shift from the atom-0 pair, then n x n atom pairs with the template tables.
-template <int APS>
-__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad, int32_t virial) {
+// NCH lanes share one atom (lane = atom * NCH + chunk) and take every NCH-th
+// partner; their partial forces are summed with a fixed xor butterfly.  Both
+// partners of a pair evaluate it in the reference's orientation (lower group
+// first), so the two evaluations are bit-identical: energies and virial are
+// accumulated by every lane and halved at the end (exact).
+constexpr int NCH = 4;
+
+template <int APS, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad) {
     double v[ROW] = {0};
-    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t o = a < npad ? p.s2o[a] : -1;
+    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t a = gid / NCH;
+    const int chunk = gid % NCH;
+    int32_t o = p.s2o[a];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
         int32_t k = p.s_atom_cg[a];
-        const JobDev& J = p.jobs[p.atom_job[o]];
         int32_t e0 = (int32_t)p.vv_off[k], e1 = (int32_t)p.vv_off[k + 1];
         if (e1 > e0) {
+            const JobDev& J = p.jobs[p.atom_job[o]];
+            const Box box = J.box;
+            const double c2 = J.crf.crf_2cut3i, cc = J.crf.crf_cut, c3 = J.crf.crf_cut3i;
             int32_t sk = p.s_cg_start[k];
             int32_t ia = a - sk;
             d3 pa = xyz(p.xq[a]);
@@ -234,41 +285,170 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad, int32_t v
                 c12[b] = J.sv_c12[ia * 4 + b];
                 qq[b] = J.sv_qq[ia * 4 + b];
             }
-            for (int32_t e = e0; e < e1; ++e) {
+            for (int32_t e = e0 + chunk; e < e1; e += NCH) {
                 uint32_t ent = p.vv_ent[e];
-                int32_t m = (int32_t)(ent & ENT_IDX);
+                int32_t sm = (int32_t)(ent & ENT_IDX);
                 bool canon = (ent & ENT_CANON) != 0;
-                int32_t sm = p.s_cg_start[m];
                 d3 pm0 = xyz(p.xq[sm]);
                 d3 A0 = canon ? pk0 : pm0, B0 = canon ? pm0 : pk0;
-                d3 rf = nim(J.box, A0, B0);
+                d3 rf = nim(box, A0, B0);
                 double tx = rf.x - A0.x + B0.x, ty = rf.y - A0.y + B0.y, tz = rf.z - A0.z + B0.z;
+                double sg = canon ? 1.0 : -1.0;
 #pragma unroll
                 for (int b = 0; b < APS; ++b) {
                     d3 pb = xyz(p.xq[sm + b]);
                     d3 u = canon ? pa : pb, w = canon ? pb : pa;
-                    d3 r{(u.x + tx) - w.x, (u.y + ty) - w.y, (u.z + tz) - w.z};
-                    double fm, elj, ecrf;
-                    lj_crf64(r.x, r.y, r.z, c6[b], c12[b], qq[b], J.crf, fm, elj, ecrf);
-                    double gx = r.x * fm, gy = r.y * fm, gz = r.z * fm;
-                    if (canon) {
-                        fx += gx; fy += gy; fz += gz;
-                        v[0] += elj;
-                        v[1] += ecrf;
-                        if (virial) add_virial(v, r, gx, gy, gz);
-                    } else {
-                        fx -= gx; fy -= gy; fz -= gz;
-                    }
+                    double rx = (u.x + tx) - w.x, ry = (u.y + ty) - w.y, rz = (u.z + tz) - w.z;
+                    // lj_crf_interaction (innerloops.rs:22-55), branch-free guard
+                    double r2 = rx * rx + ry * ry + rz * rz;
+                    bool ok = !(r2 < 1e-10);
+                    double inv_r2 = 1.0 / r2;
+                    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                    double elj = (c12[b] * inv_r6 - c6[b]) * inv_r6;
+                    double flj = (12.0 * c12[b] * inv_r6 - 6.0 * c6[b]) * inv_r6 * inv_r2;
+                    double inv_r = sqrt(inv_r2);
+                    double ecrf = qq[b] * (inv_r - c2 * r2 - cc);
+                    double fcrf = qq[b] * (inv_r * inv_r2 + c3);
+                    double fm = ok ? flj + fcrf : 0.0;
+                    elj = ok ? elj : 0.0;
+                    ecrf = ok ? ecrf : 0.0;
+                    double gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                    fx += sg * gx;
+                    fy += sg * gy;
+                    fz += sg * gz;
+                    v[0] += elj;
+                    v[1] += ecrf;
+                    if (VIR) add_virial(v, d3{rx, ry, rz}, gx, gy, gz);
                 }
             }
         }
     }
-    if (a < npad) {
+    // fixed-order butterfly over the NCH lanes of one atom
+#pragma unroll
+    for (int m = 1; m < NCH; m <<= 1) {
+        fx += __shfl_xor(fx, m, WAVE);
+        fy += __shfl_xor(fy, m, WAVE);
+        fz += __shfl_xor(fz, m, WAVE);
+    }
+    if (chunk == 0) {
         p.f_s[3 * a] = fx;
         p.f_s[3 * a + 1] = fy;
         p.f_s[3 * a + 2] = fz;
     }
-    write_row(v, p.rows, SLOT_VV, npad / BLOCK);
+#pragma unroll
+    for (int c = 0; c < 11; ++c) v[c] *= 0.5;
+    write_row(v, p.rows, SLOT_VV, p.nrow_blocks);
+}
+
+// ---- mixed precision: FP32 pair arithmetic, FP64 per-lane partial sums,
+// ---- int64 fixed-point per-atom totals (exact and order-independent)
+constexpr double FX_SCALE = 16777216.0;          // 2^24 per kJ mol^-1 nm^-1
+constexpr double FX_INV = 1.0 / 16777216.0;
+
+__device__ __forceinline__ long long to_fx(double v) { return (long long)rint(v * FX_SCALE); }
+
+__device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6, float c12, float qp, float c2,
+                                         float cc, float c3, float& f, float& elj, float& ecrf) {
+    float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    bool ok = !(r2 < 1e-10f);
+    float inv_r = __builtin_amdgcn_rsqf(r2);
+    float inv_r2 = inv_r * inv_r;
+    float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    float e = fmaf(c12, inv_r6, -c6) * inv_r6;
+    float fl = fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6 * inv_r2;
+    float ec = qp * (inv_r - fmaf(c2, r2, cc));
+    float fc = qp * fmaf(inv_r, inv_r2, c3);
+    f = ok ? fl + fc : 0.0f;
+    elj = ok ? e : 0.0f;
+    ecrf = ok ? ec : 0.0f;
+}
+
+// nearest-image vector between two group first atoms, FP64, as float
+__device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, float& y, float& z) {
+    d3 r = nim(b, A, B);
+    x = (float)r.x;
+    y = (float)r.y;
+    z = (float)r.z;
+}
+
+template <int APS, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t npad) {
+    double v[ROW] = {0};
+    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t a = gid / NCH;
+    const int chunk = gid % NCH;
+    int32_t o = p.s2o[a];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (o >= 0) {
+        int32_t k = p.s_atom_cg[a];
+        int32_t e0 = (int32_t)p.vv_off[k], e1 = (int32_t)p.vv_off[k + 1];
+        if (e1 > e0) {
+            const JobDev& J = p.jobs[p.atom_job[o]];
+            const Box box = J.box;
+            const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
+            int32_t sk = p.s_cg_start[k];
+            int32_t ia = a - sk;
+            float4 da = p.xf[a];
+            d3 pk0 = xyz(p.xq[sk]);
+            float c6[APS], c12[APS], qq[APS];
+#pragma unroll
+            for (int b = 0; b < APS; ++b) {
+                c6[b] = (float)J.sv_c6[ia * 4 + b];
+                c12[b] = (float)J.sv_c12[ia * 4 + b];
+                qq[b] = (float)J.sv_qq[ia * 4 + b];
+            }
+            for (int32_t e = e0 + chunk; e < e1; e += NCH) {
+                uint32_t ent = p.vv_ent[e];
+                int32_t sm = (int32_t)(ent & ENT_IDX);
+                float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
+                float sx, sy, sz;
+                shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+                float bx = sx + da.x, by = sy + da.y, bz = sz + da.z;
+                float gxs = 0.f, gys = 0.f, gzs = 0.f, el = 0.f, ec = 0.f;
+                float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int b = 0; b < APS; ++b) {
+                    float4 db = p.xf[sm + b];
+                    float rx = bx - db.x, ry = by - db.y, rz = bz - db.z;
+                    float fm, elj, ecrf;
+                    lj_crf32(rx, ry, rz, c6[b], c12[b], qq[b], c2, cc, c3, fm, elj, ecrf);
+                    float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                    gxs += gx;
+                    gys += gy;
+                    gzs += gz;
+                    el += elj;
+                    ec += ecrf;
+                    if (VIR) {
+                        w[0] = fmaf(rx, gx, w[0]); w[1] = fmaf(rx, gy, w[1]); w[2] = fmaf(rx, gz, w[2]);
+                        w[3] = fmaf(ry, gx, w[3]); w[4] = fmaf(ry, gy, w[4]); w[5] = fmaf(ry, gz, w[5]);
+                        w[6] = fmaf(rz, gx, w[6]); w[7] = fmaf(rz, gy, w[7]); w[8] = fmaf(rz, gz, w[8]);
+                    }
+                }
+                if (VIR) {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) v[2 + c] += (double)(canon * w[c]);
+                }
+                fx += (double)gxs;
+                fy += (double)gys;
+                fz += (double)gzs;
+                v[0] += (double)(canon * el);
+                v[1] += (double)(canon * ec);
+            }
+        }
+    }
+    long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
+#pragma unroll
+    for (int m = 1; m < NCH; m <<= 1) {
+        ix += __shfl_xor(ix, m, WAVE);
+        iy += __shfl_xor(iy, m, WAVE);
+        iz += __shfl_xor(iz, m, WAVE);
+    }
+    if (chunk == 0) {
+        p.fi_s[3 * a] = ix;
+        p.fi_s[3 * a + 1] = iy;
+        p.fi_s[3 * a + 2] = iz;
+    }
+    write_row(v, p.rows, SLOT_VV, p.nrow_blocks);
 }
 
 // Topology::is_excluded_or_14 (topology.rs:853-865) on original atom indices
@@ -386,7 +566,111 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
             f[3 * a + 2] += fz;
         }
     }
-    write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, npad / BLOCK);
+    write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
+}
+
+// Mixed-precision counterpart of k_gen: FP32 pair arithmetic on offsets from
+// the groups' first atoms (their nearest-image vector taken in FP64), FP64
+// per-lane sums, int64 fixed-point per-atom result.  Energies from the
+// canonical partner only.
+template <bool LONG, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
+    double v[ROW] = {0};
+    int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t o = a < npad ? p.s2o[a] : -1;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (o >= 0) {
+        int32_t k = p.s_atom_cg[a];
+        const JobDev& J = p.jobs[p.atom_job[o]];
+        const Box box = J.box;
+        const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
+        const float fpe = (float)J.fpepsi;
+        const int32_t nt = J.n_types;
+        const double* ljb = p.lj + J.lj_off;
+        int32_t sk = p.s_cg_start[k], nk = p.s_cg_n[k];
+        float4 da = p.xf[a];
+        double4 xa = p.xq[a];
+        d3 pk0 = xyz(p.xq[sk]);
+        int32_t ta = p.s_type[a];
+        bool cgp = J.use_cg_grouped != 0 && !LONG;
+        int32_t e0 = (int32_t)p.gen_off[k], e1 = (int32_t)p.gen_off[k + 1];
+        for (int32_t e = e0; e < e1; ++e) {
+            uint32_t ent = p.gen_ent[e];
+            bool lng = (ent & ENT_LONG) != 0;
+            if (lng != LONG) continue;
+            int32_t m = (int32_t)(ent & ENT_IDX);
+            float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
+            bool ss = (ent & ENT_SS) != 0;
+            int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            if (cgp) shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+            for (int32_t b = 0; b < nm; ++b) {
+                if (!LONG && ss) {
+                    int32_t ob = p.s2o[sm + b];
+                    if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+                }
+                double4 xb = p.xq[sm + b];
+                float rx, ry, rz;
+                if (cgp) {
+                    float4 db = p.xf[sm + b];
+                    rx = (sx + da.x) - db.x;
+                    ry = (sy + da.y) - db.y;
+                    rz = (sz + da.z) - db.z;
+                } else {
+                    d3 r = nim(box, xyz(xa), xyz(xb));
+                    rx = (float)r.x; ry = (float)r.y; rz = (float)r.z;
+                }
+                const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sm + b]);
+                float fm, elj, ecrf;
+                lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
+                         ecrf);
+                float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                fx += (double)gx;
+                fy += (double)gy;
+                fz += (double)gz;
+                v[0] += (double)(canon * elj);
+                v[1] += (double)(canon * ecrf);
+                if (VIR || LONG)
+                    add_virial(v, d3{(double)(canon * rx), (double)(canon * ry), (double)(canon * rz)}, gx, gy, gz);
+            }
+        }
+        if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
+            int32_t ia = a - sk;
+            for (int32_t b = 0; b < nk; ++b) {
+                if (b == ia) continue;
+                int32_t ob = p.s2o[sk + b];
+                bool canon = ia < b;
+                if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+                double4 xb = p.xq[sk + b];
+                float rx = (float)(xa.x - xb.x), ry = (float)(xa.y - xb.y), rz = (float)(xa.z - xb.z);
+                const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sk + b]);
+                float fm, elj, ecrf;
+                lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
+                         ecrf);
+                float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                fx += (double)gx;
+                fy += (double)gy;
+                fz += (double)gz;
+                float cf = canon ? 1.0f : 0.0f;
+                v[0] += (double)(cf * elj);
+                v[1] += (double)(cf * ecrf);
+                if (VIR) add_virial(v, d3{(double)(cf * rx), (double)(cf * ry), (double)(cf * rz)}, gx, gy, gz);
+            }
+        }
+    }
+    if (a < npad) {
+        long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
+        if (LONG) {
+            p.fi_long_s[3 * a] = ix;
+            p.fi_long_s[3 * a + 1] = iy;
+            p.fi_long_s[3 * a + 2] = iz;
+        } else {
+            p.fi_s[3 * a] += ix;
+            p.fi_s[3 * a + 1] += iy;
+            p.fi_s[3 * a + 2] += iz;
+        }
+    }
+    write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
 // RF self / excluded pairs (excluded.rs:25-112) and 1-4 pairs (excluded.rs:131-206),
@@ -462,19 +746,26 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
         p.f_x[3 * i + 1] = fy;
         p.f_x[3 * i + 2] = fz;
     }
-    write_row(v, p.rows, SLOT_EXCL, npad / BLOCK);
+    write_row(v, p.rows, SLOT_EXCL, p.nrow_blocks);
 }
 
 // total force on each original atom
-__global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t with_pert) {
+__global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed) {
     int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= npad) return;
     int32_t o = p.s2o[a];
     if (o < 0) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double f = p.f_s[3 * a + c];
-        if (with_long) f += p.f_long_s[3 * a + c];
+        double f;
+        if (mixed) {
+            long long t = p.fi_s[3 * a + c];
+            if (with_long) t += p.fi_long_s[3 * a + c];
+            f = (double)t * FX_INV;
+        } else {
+            f = p.f_s[3 * a + c];
+            if (with_long) f += p.f_long_s[3 * a + c];
+        }
         f += p.f_x[3 * o + c];
         if (with_pert) f += p.f_p[3 * o + c];
         p.f[3 * o + c] = f;
@@ -598,7 +889,7 @@ __global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
         double b = vo[3 * i] * vo[3 * i] + vo[3 * i + 1] * vo[3 * i + 1] + vo[3 * i + 2] * vo[3 * i + 2];
         v[11] = 0.5 * p.mass[i] * (a + b) / 2.0;
     }
-    write_row(v, p.rows, SLOT_KIN, npad / BLOCK);
+    write_row(v, p.rows, SLOT_KIN, p.nrow_blocks);
 }
 
 // One block per job: sums the job's partial rows in a fixed order and writes
@@ -608,15 +899,17 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
     __shared__ double lds[17 * ROW];
     int32_t jb = blockIdx.x;
     const JobDev& J = p.jobs[jb];
-    int32_t nblocks = npad / BLOCK;
-    int32_t b0 = J.atom_off / BLOCK;
-    int32_t b1 = (J.atom_off + J.n_atoms + BLOCK - 1) / BLOCK;
+    const int32_t stride = p.nrow_blocks;
     double v[ROW] = {0};
-    for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-        for (int s = 0; s < NSLOT; ++s) {
-            if (s == SLOT_LONG && !with_long) continue;
-            if (s == SLOT_KIN && !with_kin) continue;
-            const double* r = p.rows + ((int64_t)s * nblocks + b) * ROW;
+    for (int s = 0; s < NSLOT; ++s) {
+        if (s == SLOT_LONG && !with_long) continue;
+        if (s == SLOT_KIN && !with_kin) continue;
+        // the pair kernel runs NCH lanes per atom, the others one
+        int32_t per = (s == SLOT_VV) ? BLOCK / NCH : BLOCK;
+        int32_t b0 = J.atom_off / per;
+        int32_t b1 = (J.atom_off + J.n_atoms + per - 1) / per;
+        for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+            const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
 #pragma unroll
             for (int c = 0; c < ROW; ++c) v[c] += r[c];
         }
@@ -670,7 +963,7 @@ __global__ void k_materialize(SysPtrs p, int32_t ncg, unsigned long long* counts
         uint32_t ent = p.vv_ent[e];
         if (!(ent & ENT_CANON)) continue;
         int32_t m = (int32_t)(ent & ENT_IDX);
-        emit(2, p.s2o[sk], p.s2o[p.s_cg_start[m]]);
+        emit(2, p.s2o[sk], p.s2o[m]);   // solvent entries hold the partner's first sorted atom
     }
     for (int32_t e = (int32_t)p.gen_off[k]; e < (int32_t)p.gen_off[k + 1]; ++e) {
         uint32_t ent = p.gen_ent[e];
@@ -706,29 +999,49 @@ void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s) {
     hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
-    if (fill) hipLaunchKernelGGL(k_nbl<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
-    else hipLaunchKernelGGL(k_nbl<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    // one wave per group
+    dim3 g((unsigned)((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE)));
+    if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncg);
+    else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncg);
 }
-void launch_gather(const SysPtrs& p, int32_t npad, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
 }
-void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, hipStream_t s) {
-    switch (aps) {
-        case 1: hipLaunchKernelGGL(k_vv<1>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
-        case 2: hipLaunchKernelGGL(k_vv<2>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
-        case 4: hipLaunchKernelGGL(k_vv<4>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
-        default: hipLaunchKernelGGL(k_vv<3>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial); break;
+template <int APS>
+static void launch_vv_aps(const SysPtrs& p, int32_t npad, int32_t virial, int32_t mixed, hipStream_t s) {
+    dim3 g(grid_for((int64_t)npad * NCH));
+    if (mixed) {
+        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, true>), g, dim3(BLOCK), 0, s, p, npad);
+        else hipLaunchKernelGGL((k_vv_mx<APS, false>), g, dim3(BLOCK), 0, s, p, npad);
+    } else {
+        if (virial) hipLaunchKernelGGL((k_vv<APS, true>), g, dim3(BLOCK), 0, s, p, npad);
+        else hipLaunchKernelGGL((k_vv<APS, false>), g, dim3(BLOCK), 0, s, p, npad);
     }
 }
-void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, hipStream_t s) {
+void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s) {
+    switch (aps) {
+        case 1: launch_vv_aps<1>(p, npad, virial, mixed, s); break;
+        case 2: launch_vv_aps<2>(p, npad, virial, mixed, s); break;
+        case 4: launch_vv_aps<4>(p, npad, virial, mixed, s); break;
+        default: launch_vv_aps<3>(p, npad, virial, mixed, s); break;
+    }
+}
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {
+    if (mixed) {
+        if (lng) hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+        else if (virial) hipLaunchKernelGGL((k_gen_mx<false, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+        else hipLaunchKernelGGL((k_gen_mx<false, false>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+        return;
+    }
     if (lng) hipLaunchKernelGGL(k_gen<true>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
     else hipLaunchKernelGGL(k_gen<false>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
 }
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
     hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
 }
-void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, with_pert);
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, with_pert, mixed);
 }
 void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s) {
     hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad);

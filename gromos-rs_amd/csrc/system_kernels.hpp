@@ -54,6 +54,10 @@ struct SysPtrs {
     uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild
     // per step
     double4* xq;
+    float4* xf;                // mixed: FP32 offset from the group's first atom, charge
+    long long* fi_s;           // mixed: int64 fixed-point forces (sorted)
+    long long* fi_long_s;      // mixed: twin-range cache
+    int32_t nrow_blocks;       // row stride of the energy partials per slot
     double* f_s;
     double* f_long_s;
     double* f_x;
@@ -69,11 +73,11 @@ void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
-void launch_gather(const SysPtrs& p, int32_t npad, hipStream_t s);
-void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, hipStream_t s);
-void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, hipStream_t s);
+void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
+void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s);
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
-void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, hipStream_t s);
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed, hipStream_t s);
 void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);

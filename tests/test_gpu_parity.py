@@ -232,3 +232,60 @@ def test_water32768_full_size():
     assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
     assert force_err(g["forces"], r["forces"]) < F_TOL
     assert np.abs(g["forces"].sum(0)).max() < 1e-6
+
+
+# --------------------------------------------------------------------------
+# mixed precision (FP32 pair math, int64 fixed-point forces, FP64 sums)
+# tolerance: max per-atom force error 1e-5 relative (F_floor 1 kJ/mol/nm),
+# energies 1e-6 relative
+# --------------------------------------------------------------------------
+MX_F_TOL = 1e-5
+MX_E_TOL = 1e-6
+
+
+@pytest.mark.parametrize("name", ["water_10_box", "nacl_water_box", "aladip_solvated", "water_1000_spc_gridcell",
+                                  "water_3_box_twinrange"])
+def test_mixed_fixture_forces(name):
+    s, _ = golden_io.load(name)
+    ds = rt.DeviceSystem(s, precision=rt.MIXED)
+    ds.pairlist_update()
+    assert_same_lists(ds.pairlist(), oracle.pairlist(oracle.OracleSystem(s)))
+    g = ds.forcefield_apply()
+    r = oracle.single_point(oracle.OracleSystem(s))
+    assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+    for k in ("e_lj", "e_crf"):
+        assert abs(g[k] - r[k]) <= MX_E_TOL * max(abs(r[k]), 1.0), (k, g[k], r[k])
+
+
+def test_mixed_water32768_error_and_determinism():
+    s = gen.water_32768()
+    r = oracle.single_point(oracle.OracleSystem(s), parallel=True)
+    out = []
+    for _ in range(2):
+        ds = rt.DeviceSystem(s, precision=rt.MIXED)
+        ds.pairlist_update()
+        out.append(ds.forcefield_apply())
+        ds.close()
+    g = out[0]
+    assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+    assert rel(g["e_lj"], r["e_lj"]) < MX_E_TOL and rel(g["e_crf"], r["e_crf"]) < MX_E_TOL
+    # fixed order everywhere: bit-identical run to run
+    assert np.array_equal(out[0]["forces"], out[1]["forces"])
+    assert out[0]["e_lj"] == out[1]["e_lj"] and out[0]["e_crf"] == out[1]["e_crf"]
+
+
+def test_mixed_md_tracks_fp64():
+    s = gen.spc_water_box(4096, seed=11)
+    res = {}
+    for prec in (rt.FP64, rt.MIXED):
+        ds = rt.DeviceSystem(s, precision=prec)
+        ds.pairlist_update()
+        res[prec] = ds.md(200)
+        ds.close()
+    e64, emx = res[rt.FP64], res[rt.MIXED]
+    tot64, totmx = e64[:, 2] + e64[:, 3], emx[:, 2] + emx[:, 3]
+    # same physics step by step at the start
+    assert np.all(np.abs(emx[:10, 2] - e64[:10, 2]) <= 1e-5 * np.abs(e64[:10, 2]))
+    # energy drift over the run agrees within 2% of the kinetic energy scale
+    d64, dmx = tot64[-1] - tot64[0], totmx[-1] - totmx[0]
+    assert abs(dmx - d64) <= 0.02 * abs(e64[-1, 3]), (d64, dmx)

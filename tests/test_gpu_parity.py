@@ -237,10 +237,11 @@ def test_water32768_full_size():
 # --------------------------------------------------------------------------
 # mixed precision (FP32 pair math, int64 fixed-point forces, FP64 sums)
 # tolerance: max per-atom force error 1e-5 relative (F_floor 1 kJ/mol/nm),
-# energies 1e-6 relative
+# energies 1e-5 relative (FP32 pair energies summed in FP64: the error grows
+# like sqrt(#pairs) x 6e-8 x |pair energy|, ~3e-6 for ion-water systems)
 # --------------------------------------------------------------------------
 MX_F_TOL = 1e-5
-MX_E_TOL = 1e-6
+MX_E_TOL = 1e-5
 
 
 @pytest.mark.parametrize("name", ["water_10_box", "nacl_water_box", "aladip_solvated", "water_1000_spc_gridcell",

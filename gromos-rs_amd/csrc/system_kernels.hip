@@ -351,7 +351,9 @@ __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6,
                                          float cc, float c3, float& f, float& elj, float& ecrf) {
     float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
     bool ok = !(r2 < 1e-10f);
-    float inv_r = __builtin_amdgcn_rsqf(r2);
+    // hardware reciprocal square root (1 ulp) refined by one Newton step
+    float y = __builtin_amdgcn_rsqf(r2);
+    float inv_r = fmaf(0.5f * y, fmaf(-r2 * y, y, 1.0f), y);
     float inv_r2 = inv_r * inv_r;
     float inv_r6 = inv_r2 * inv_r2 * inv_r2;
     float e = fmaf(c12, inv_r6, -c6) * inv_r6;

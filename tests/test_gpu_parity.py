@@ -236,12 +236,22 @@ def test_water32768_full_size():
 
 # --------------------------------------------------------------------------
 # mixed precision (FP32 pair math, int64 fixed-point forces, FP64 sums)
-# tolerance: max per-atom force error 1e-5 relative (F_floor 1 kJ/mol/nm),
-# energies 1e-5 relative (FP32 pair energies summed in FP64: the error grows
-# like sqrt(#pairs) x 6e-8 x |pair energy|, ~3e-6 for ion-water systems)
+# tolerance (per atom |dF_i| / max(|F_i|, 1 kJ/mol/nm)): median <= 2e-6, max
+# <= 5e-5.  FP32 pair forces carry ~2-3 ulp (1.5e-7) each; an atom's net force
+# cancels ~1000 terms up to 10x larger than itself, so the worst atoms of a
+# 98k-atom water box sit near 2e-5 (a float64 emulation of the same FP32
+# operation sequence gives 1.4e-5 even with a correctly rounded rsqrt).
+# Energies: 1e-5 relative (FP32 pair energies summed in FP64).
 # --------------------------------------------------------------------------
-MX_F_TOL = 1e-5
+MX_F_TOL = 5e-5
+MX_F_MED = 2e-6
 MX_E_TOL = 1e-5
+
+
+def force_err_median(f, ref):
+    d = np.linalg.norm(f - ref, axis=1)
+    s = np.maximum(np.linalg.norm(ref, axis=1), F_FLOOR)
+    return float(np.median(d / s))
 
 
 @pytest.mark.parametrize("name", ["water_10_box", "nacl_water_box", "aladip_solvated", "water_1000_spc_gridcell",
@@ -269,6 +279,7 @@ def test_mixed_water32768_error_and_determinism():
         ds.close()
     g = out[0]
     assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+    assert force_err_median(g["forces"], r["forces"]) < MX_F_MED
     assert rel(g["e_lj"], r["e_lj"]) < MX_E_TOL and rel(g["e_crf"], r["e_crf"]) < MX_E_TOL
     # fixed order everywhere: bit-identical run to run
     assert np.array_equal(out[0]["forces"], out[1]["forces"])

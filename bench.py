@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 60          # algorithmic count per pair interaction (r, r2, 1/r2, r^-6,
                             # E_LJ, F_LJ, 1/r, q, E_CRF, F_CRF, F, accumulate, energies, virial)
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same figure)
+PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 (packed; this build keeps FP32 un-packed: 78.6 effective)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -31,7 +32,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -75,6 +76,68 @@ def cpu_baseline(s, nsteps):
                        f"{dt:.2f} s wall, {dt / nsteps * 1e3:.1f} ms/step")
 
 
+def traffic_from_profile(kernel, precision):
+    """HBM bytes per dispatch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE scaled by
+    a copy of known size), read from the committed summary of the same kernel."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_water32768_{precision}.md")
+    try:
+        for line in open(path):
+            if line.startswith("machine-readable: traffic_bytes="):
+                return float(line.split("=", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def run_path(rt, s, prec, steps, warmup, latency_steps):
+    import numpy as np
+    import torch
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    if warmup:
+        ds.md(warmup)
+    ds.kernel_timing(enable=1)
+    p0 = ds.pair_count
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = ds.md(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms, launches = ds.kernel_timing(enable=0)
+    pairs = ds.pair_count - p0
+    lat = []
+    for _ in range(latency_steps):
+        t = time.perf_counter()
+        ds.md(1)
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat) if lat else np.array([float("nan")])
+    vv_pairs = ds_vv_pairs(ds, s)
+    ds.close()
+    return dict(pairs_per_s=pairs / wall, ms_per_step=wall / steps * 1e3, wall=wall,
+                ns_per_day=steps * s.dt * 1e-3 * 86400.0 / wall,
+                p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                kern_ms=kern_ms / max(launches, 1), launches=launches, vv_pairs=vv_pairs,
+                pairs_per_step=pairs / steps, e_first=float(e[0, 2] + e[0, 3]), e_last=float(e[-1, 2] + e[-1, 3]))
+
+
+def force_error(rt, s):
+    """mixed vs plain FP64 GPU forces on the same configuration (the plain path
+    matches the CPU oracle to 1e-10 in the parity tests)"""
+    import numpy as np
+    out = {}
+    for prec in (rt.FP64, rt.MIXED):
+        ds = rt.DeviceSystem(s, precision=prec)
+        ds.pairlist_update()
+        out[prec] = ds.forcefield_apply()
+        ds.close()
+    f64, fmx = out[rt.FP64]["forces"], out[rt.MIXED]["forces"]
+    d = np.linalg.norm(fmx - f64, axis=1) / np.maximum(np.linalg.norm(f64, axis=1), 1.0)
+    e64 = out[rt.FP64]["e_lj"] + out[rt.FP64]["e_crf"]
+    emx = out[rt.MIXED]["e_lj"] + out[rt.MIXED]["e_crf"]
+    return dict(max=float(d.max()), median=float(np.median(d)), f_floor=1.0,
+                energy_rel=float(abs(emx - e64) / abs(e64)))
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -84,66 +147,60 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); independent jobs shard across GPUs "
               "without per-step communication, see the design notes", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
     assert torch.cuda.is_available(), "no GPU visible"
     gen = importlib.import_module("gromos-rs_amd.generators")
     rt = importlib.import_module("gromos-rs_amd.runtime")
     s = gen.water_32768()
-    prec = rt.FP64 if a.precision == "fp64" else rt.MIXED
-    ds = rt.DeviceSystem(s, precision=prec)
-    ds.pairlist_update()
-    if a.warmup:
-        ds.md(a.warmup)
-    ds.kernel_timing(enable=1)
-    p0 = ds.pair_count
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e = ds.md(a.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms, launches = ds.kernel_timing(enable=0)
-    pairs = ds.pair_count - p0
-    # per-step latency (host clock around single steps, synchronised each step)
-    lat = []
-    for _ in range(a.latency_steps):
-        t = time.perf_counter()
-        ds.md(1)
-        lat.append((time.perf_counter() - t) * 1e3)
-    lat = np.array(lat) if lat else np.array([float("nan")])
-    # dominant kernel: the solvent-solvent pair kernel, one launch per step
-    vv_pairs_per_step = ds_vv_pairs(ds, s)
-    avg_ms = kern_ms / max(launches, 1)
-    achieved = vv_pairs_per_step * FLOP_PER_PAIR / (avg_ms * 1e-3) / 1e12
-    ms_per_step = wall / a.steps * 1e3
-    ns_day = a.steps * s.dt * 1e-3 * 86400.0 / wall
+    paths = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    res = {}
+    for name in paths:
+        prec = rt.FP64 if name == "fp64" else rt.MIXED
+        res[name] = run_path(rt, s, prec, a.steps, a.warmup, a.latency_steps)
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
+    kernel = "k_vv_mx" if head == "mixed" else "k_vv"
+    flop = h["vv_pairs"] * FLOP_PER_PAIR
+    achieved = flop / (h["kern_ms"] * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
+    traffic, traffic_src = traffic_from_profile(kernel, head)
     out = {
         "metric": "pair interactions/s + aggregate ns/day; p50/p99 step latency vs plain FP64 GPU",
-        "value": pairs / wall,
+        "value": h["pairs_per_s"],
         "unit": "pair interactions/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == rt.FP64 else "f32+int64",
+        "dtype": "f32+int64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"water_32768: 32,768 SPC waters (98,304 atoms), cubic box "
                                f"{s.box[0]:.4f} nm (0.997 g/cm3), single-range cutoff 1.4 nm, RF eps_rf 61, "
                                f"pair list every 5 steps, SETTLE, dt 2 fs, device-resident MD",
-                   "atoms": s.n_atoms, "precision": a.precision},
-        "ns_per_day": ns_day,
-        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "pairs_per_step": pairs / a.steps,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "k_vv<3> (solvent-solvent pair kernel)", "avg_launch_ms": avg_ms,
-                     "launches": launches,
-                     "note": "FP64 vector-ALU bound; the fp64 peak is the same 78.6 TFLOP/s for VALU and MFMA"},
-        "energy_first_last": [float(e[0, 2] + e[0, 3]), float(e[-1, 2] + e[-1, 3])],
+                   "atoms": s.n_atoms, "precision": head},
+        "ns_per_day": h["ns_per_day"],
+        "step_latency_ms": {"p50": h["p50"], "p99": h["p99"]},
+        "pairs_per_step": h["pairs_per_step"],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": f"{kernel} (solvent-solvent pair kernel)", "avg_launch_ms": h["kern_ms"],
+                     "launches": h["launches"], "flop_per_launch": flop, "traffic_source": traffic_src,
+                     "note": "vector-ALU bound pair kernel (no dense contraction, no MFMA); peak = MI355X "
+                             + ("FP32 157.3 TFLOP/s (packed; 78.6 without packed FP32)" if head == "mixed"
+                                else "FP64 78.6 TFLOP/s")},
     }
+    if "fp64" in res and head != "fp64":
+        p = res["fp64"]
+        out["plain_fp64_gpu"] = {"pairs_per_s": p["pairs_per_s"], "ms_per_step": p["ms_per_step"],
+                                 "ns_per_day": p["ns_per_day"], "step_latency_ms": {"p50": p["p50"], "p99": p["p99"]},
+                                 "vv_kernel_ms": p["kern_ms"],
+                                 "vv_kernel_tflops": p["vv_pairs"] * FLOP_PER_PAIR / (p["kern_ms"] * 1e-3) / 1e12}
+        out["speedup_vs_plain_fp64_gpu"] = h["pairs_per_s"] / p["pairs_per_s"]
+        out["mixed_force_error_vs_fp64"] = force_error(rt, s)
+    out["energy_first_last"] = [h["e_first"], h["e_last"]]
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s, a.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -321,6 +321,11 @@ struct gromos_amd_system {
     uint32_t *d_vv_ent = nullptr, *d_gen_ent = nullptr;
     int64_t vv_cap = 0, gen_cap = 0;
     uint64_t* d_counters = nullptr;
+    int64_t *d_vv_cap = nullptr, *d_gen_cap = nullptr;
+    int32_t *d_prev_vv = nullptr, *d_prev_gen = nullptr, *d_overflow = nullptr;
+    bool have_prev = false;
+    int64_t prev_tot_vv = 0, prev_tot_gen = 0;
+    int64_t rebuilds = 0, refills = 0;
     double4* d_xq = nullptr;
     float4* d_xf = nullptr;
     long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
@@ -341,7 +346,7 @@ struct gromos_amd_system {
     bool twin = false, long_computed = false, has_pert = false;
     int32_t use_settle = 0, do_virial = 0;
     int64_t pair_count = 0;
-    uint64_t host_counters[2] = {0, 0};
+    uint64_t host_counters[4] = {0, 0, 0, 0};
     // timing of the dominant kernel
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -435,6 +440,11 @@ struct gromos_amd_system {
         p.vv_ent = d_vv_ent;
         p.gen_ent = d_gen_ent;
         p.counters = d_counters;
+        p.vv_cap = d_vv_cap;
+        p.gen_cap = d_gen_cap;
+        p.prev_vv = d_prev_vv;
+        p.prev_gen = d_prev_gen;
+        p.overflow = d_overflow;
         p.xq = d_xq;
         p.xf = d_xf;
         p.fi_s = d_fi_s;
@@ -679,7 +689,10 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         (rc = S->alloc(S->d_s_atom_cg, (size_t)S->npad)) || (rc = S->alloc(S->d_s_type, (size_t)S->npad)) ||
         (rc = S->alloc(S->d_vv_cnt, ncg + 1)) || (rc = S->alloc(S->d_gen_cnt, ncg + 1)) ||
         (rc = S->alloc(S->d_vv_off, ncg + 1)) || (rc = S->alloc(S->d_gen_off, ncg + 1)) ||
-        (rc = S->alloc(S->d_counters, 2)) || (rc = S->alloc(S->d_xq, (size_t)S->npad)) ||
+        (rc = S->alloc(S->d_counters, 4)) || (rc = S->alloc(S->d_xq, (size_t)S->npad)) ||
+        (rc = S->alloc(S->d_vv_cap, ncg + 1)) || (rc = S->alloc(S->d_gen_cap, ncg + 1)) ||
+        (rc = S->alloc(S->d_prev_vv, ncg)) || (rc = S->alloc(S->d_prev_gen, ncg)) ||
+        (rc = S->alloc(S->d_overflow, 1)) ||
         (rc = S->alloc(S->d_f_s, n3)) || (rc = S->alloc(S->d_f_long_s, n3)) || (rc = S->alloc(S->d_f_x, n3)) ||
         (rc = S->alloc(S->d_f_p, n3)) || (rc = S->alloc(S->d_xf, (size_t)S->npad)) ||
         (rc = S->alloc(S->d_fi_s, n3)) || (rc = S->alloc(S->d_fi_long_s, n3)) ||
@@ -694,6 +707,8 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     HIPCHK(hipMemset(S->d_s_cg_n, 0, (ncg + 1) * sizeof(int32_t)));
     HIPCHK(hipMemset(S->d_vv_cnt, 0, (ncg + 1) * sizeof(int64_t)));
     HIPCHK(hipMemset(S->d_gen_cnt, 0, (ncg + 1) * sizeof(int64_t)));
+    HIPCHK(hipMemset(S->d_vv_cap, 0, (ncg + 1) * sizeof(int64_t)));
+    HIPCHK(hipMemset(S->d_gen_cap, 0, (ncg + 1) * sizeof(int64_t)));
     // scratch for the library sort / scans
     size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
@@ -749,7 +764,19 @@ extern "C" int gromos_amd_system_get_state(gromos_amd_system* S, double* pos, do
     return 0;
 }
 
-// StandardPairlistAlgorithm::update (pairlist.rs:155-174) on the device
+// StandardPairlistAlgorithm::update (pairlist.rs:155-174) on the device.
+// Cell sort, then the neighbour lists.  After the first build the lists are
+// filled in one pass into slots sized from each group's previous count; if a
+// group outgrew its slot (flag) the lists are refilled with the exact counts
+// that pass produced.
+static int scan_offsets(gromos_amd_system* S, const int64_t* cnt_vv, const int64_t* cnt_gen) {
+    size_t tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, cnt_vv, S->d_vv_off, S->ncg + 1, S->stream));
+    tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, cnt_gen, S->d_gen_off, S->ncg + 1, S->stream));
+    return 0;
+}
+
 static int build_pairlist(gromos_amd_system* S) {
     hipStream_t st = S->stream;
     SysPtrs p = S->ptrs();
@@ -767,24 +794,52 @@ static int build_pairlist(gromos_amd_system* S) {
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
     HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, (size_t)S->npad * sizeof(int32_t), st));
     launch_sorted_atoms(p, ncg, st);
-    HIPCHK(hipMemsetAsync(S->d_counters, 0, 2 * sizeof(uint64_t), st));
-    launch_nbl(p, ncg, false, st);
-    tb = S->temp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_vv_cnt, S->d_vv_off, ncg + 1, st));
-    tb = S->temp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_gen_cnt, S->d_gen_off, ncg + 1, st));
+    HIPCHK(hipMemsetAsync(S->d_counters, 0, 4 * sizeof(uint64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_overflow, 0, sizeof(int32_t), st));
+    int32_t overflow = 0;
+    bool need_exact = !S->have_prev;
+    if (S->have_prev) {
+        launch_caps(p, ncg, 0, st);
+        if (int rc = scan_offsets(S, S->d_vv_cap, S->d_gen_cap)) return rc;
+        int64_t bv = S->prev_tot_vv + S->prev_tot_vv / 8 + 16 * (int64_t)ncg + 64;
+        int64_t bg = S->prev_tot_gen + S->prev_tot_gen / 8 + 16 * (int64_t)ncg + 64;
+        if (int rc = S->realloc_dev(S->d_vv_ent, (size_t)bv, S->vv_cap)) return rc;
+        if (int rc = S->realloc_dev(S->d_gen_ent, (size_t)bg, S->gen_cap)) return rc;
+        p = S->ptrs();
+        launch_nbl(p, ncg, true, st);
+        HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (overflow) {
+            S->refills++;
+            need_exact = true;   // the fill pass left exact counts in vv_cnt / gen_cnt
+        }
+    } else {
+        launch_nbl(p, ncg, false, st);
+    }
+    if (need_exact) {
+        if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
+        int64_t tot[2];
+        HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (int rc = S->realloc_dev(S->d_vv_ent, (size_t)tot[0], S->vv_cap)) return rc;
+        if (int rc = S->realloc_dev(S->d_gen_ent, (size_t)tot[1], S->gen_cap)) return rc;
+        p = S->ptrs();
+        launch_caps(p, ncg, 1, st);
+        uint64_t keep[4];
+        std::memcpy(keep, S->host_counters, sizeof(keep));
+        launch_nbl(p, ncg, true, st);
+        std::memcpy(S->host_counters, keep, sizeof(keep));
+    }
+    launch_caps(p, ncg, 2, st);
     HIPCHK(hipGetLastError());
-    int64_t tot[2];
-    HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (int rc = S->realloc_dev(S->d_vv_ent, (size_t)tot[0], S->vv_cap)) return rc;
-    if (int rc = S->realloc_dev(S->d_gen_ent, (size_t)tot[1], S->gen_cap)) return rc;
-    p = S->ptrs();
-    launch_nbl(p, ncg, true, st);
-    HIPCHK(hipGetLastError());
+    S->prev_tot_vv = (int64_t)S->host_counters[2];
+    S->prev_tot_gen = (int64_t)S->host_counters[3];
+    S->have_prev = true;
     S->built = true;
+    S->rebuilds++;
     S->steps_since_update = 0;
     return 0;
 }

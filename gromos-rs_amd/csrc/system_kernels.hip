@@ -110,8 +110,11 @@ __device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
 // and the survivors are compacted with a ballot, so each group's entries are
 // written contiguously and in a fixed order (cells in z-y-x order, candidates
 // in sorted order).  27 cells, or all cells along an axis with fewer than 3.
-// Classification as the reference (pairlist.rs:252-415); the distance test is
-// nearest_image(lower group, higher group) in the reference's arithmetic.
+// Classification as the reference (pairlist.rs:252-415).  nearest_image is
+// exactly antisymmetric (IEEE subtraction and half-away rounding are), so the
+// squared distance is the reference's whichever group comes first.
+// FILL = false: count only.  FILL = true: write at most cap[k] entries per
+// group, record the true counts and raise *overflow when a group had more.
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -146,7 +149,13 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t nvv = 0, ngen = 0;
-    int64_t vv_base = FILL ? p.vv_off[k] : 0, gen_base = FILL ? p.gen_off[k] : 0;
+    int64_t vv_base = 0, gen_base = 0, vv_cap = 0, gen_cap = 0;
+    if (FILL) {
+        vv_base = p.vv_off[k];
+        gen_base = p.gen_off[k];
+        vv_cap = p.vv_cap[k];
+        gen_cap = p.gen_cap[k];
+    }
     uint64_t pairs_short = 0, pairs_long = 0;
     int32_t nk = p.s_cg_n[k];
     for (int32_t iz = 0; iz < nl[2]; ++iz)
@@ -159,13 +168,12 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
                     bool is_vv = false, is_gen = false;
                     uint32_t ent = 0;
                     if (m < m1 && m != k) {
-                        int32_t cm = p.s_cg_orig[m];
                         d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
-                        bool canon = ck < cm;
-                        d3 lo = canon ? rk : rm, hi = canon ? rm : rk;
-                        d3 r = nim(box, lo, hi);
+                        d3 r = nim(box, rk, rm);
                         double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
                         if (!(d2 > c2l)) {
+                            int32_t cm = p.s_cg_orig[m];
+                            bool canon = ck < cm;
                             bool lng = d2 > c2s;
                             bool sol_m = cm - cg_off < nsol;
                             uint32_t flags = canon ? ENT_CANON : 0u;
@@ -189,27 +197,59 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
                     }
                     uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
                     if (FILL) {
-                        if (is_vv) p.vv_ent[vv_base + nvv + __popcll(bv & below)] = ent;
-                        if (is_gen) p.gen_ent[gen_base + ngen + __popcll(bg & below)] = ent;
+                        int64_t pv = nvv + __popcll(bv & below), pg = ngen + __popcll(bg & below);
+                        if (is_vv && pv < vv_cap) p.vv_ent[vv_base + pv] = ent;
+                        if (is_gen && pg < gen_cap) p.gen_ent[gen_base + pg] = ent;
                     }
                     nvv += __popcll(bv);
                     ngen += __popcll(bg);
                 }
             }
-    if (!FILL) {
-        if (lane == 0) {
-            p.vv_cnt[k] = nvv;
-            p.gen_cnt[k] = ngen;
-        }
-        // per-wave sums, one atomic per wave
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1) {
-            pairs_short += __shfl_xor(pairs_short, o, WAVE);
-            pairs_long += __shfl_xor(pairs_long, o, WAVE);
-        }
-        if (lane == 0 && pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
-        if (lane == 0 && pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
+    if (lane == 0) {
+        p.vv_cnt[k] = nvv;
+        p.gen_cnt[k] = ngen;
+        if (FILL && (nvv > vv_cap || ngen > gen_cap)) atomicOr(p.overflow, 1);
     }
+    // per-wave sums, one atomic per wave
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        pairs_short += __shfl_xor(pairs_short, o, WAVE);
+        pairs_long += __shfl_xor(pairs_long, o, WAVE);
+    }
+    if (lane == 0) {
+        if (pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
+        if (pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
+        atomicAdd((unsigned long long*)p.counters + 2, (unsigned long long)nvv);
+        atomicAdd((unsigned long long*)p.counters + 3, (unsigned long long)ngen);
+    }
+}
+
+// capacities for a single-pass fill: the previous build's count of the same
+// (original) group plus 1/8 and 16 entries
+__global__ void k_caps(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t c = p.s_cg_orig[k];
+    int64_t v = p.prev_vv[c], g = p.prev_gen[c];
+    p.vv_cap[k] = v + (v >> 3) + 16;
+    p.gen_cap[k] = g + (g >> 3) + 16;
+}
+
+// remember this build's counts per original group; capacities for an exact
+// refill are the counts themselves
+__global__ void k_keep_counts(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    int32_t c = p.s_cg_orig[k];
+    p.prev_vv[c] = (int32_t)p.vv_cnt[k];
+    p.prev_gen[c] = (int32_t)p.gen_cnt[k];
+}
+
+__global__ void k_caps_from_counts(SysPtrs p, int32_t ncg) {
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncg) return;
+    p.vv_cap[k] = p.vv_cnt[k];
+    p.gen_cap[k] = p.gen_cnt[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -253,10 +293,12 @@ __device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int sl
 // Solvent-solvent short pairs (process_solvent_pairs, innerloops.rs:218-309):
 // shift from the atom-0 pair, then n x n atom pairs with the template tables.
 // NCH lanes share one atom (lane = atom * NCH + chunk) and take every NCH-th
-// partner; their partial forces are summed with a fixed xor butterfly.  Both
-// partners of a pair evaluate it in the reference's orientation (lower group
-// first), so the two evaluations are bit-identical: energies and virial are
-// accumulated by every lane and halved at the end (exact).
+// partner; their partial forces are summed with a fixed xor butterfly.  Each
+// lane evaluates the pair from its own side: r = (x_own + t) - x_partner with
+// t from nearest_image(own first atom, partner first atom).  For the lane of
+// the lower group this is the reference's operation sequence exactly; for the
+// other lane it is the mirrored sequence, equal up to rounding.  Every pair is
+// seen twice, so energies and virial are halved at the end.
 constexpr int NCH = 4;
 
 template <int APS, bool VIR>
@@ -269,7 +311,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
         int32_t k = p.s_atom_cg[a];
-        int32_t e0 = (int32_t)p.vv_off[k], e1 = (int32_t)p.vv_off[k + 1];
+        int32_t e0 = (int32_t)p.vv_off[k], e1 = e0 + (int32_t)p.vv_cnt[k];
         if (e1 > e0) {
             const JobDev& J = p.jobs[p.atom_job[o]];
             const Box box = J.box;
@@ -286,19 +328,15 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad) {
                 qq[b] = J.sv_qq[ia * 4 + b];
             }
             for (int32_t e = e0 + chunk; e < e1; e += NCH) {
-                uint32_t ent = p.vv_ent[e];
-                int32_t sm = (int32_t)(ent & ENT_IDX);
-                bool canon = (ent & ENT_CANON) != 0;
+                int32_t sm = (int32_t)(p.vv_ent[e] & ENT_IDX);
                 d3 pm0 = xyz(p.xq[sm]);
-                d3 A0 = canon ? pk0 : pm0, B0 = canon ? pm0 : pk0;
-                d3 rf = nim(box, A0, B0);
-                double tx = rf.x - A0.x + B0.x, ty = rf.y - A0.y + B0.y, tz = rf.z - A0.z + B0.z;
-                double sg = canon ? 1.0 : -1.0;
+                d3 rf = nim(box, pk0, pm0);
+                double tx = rf.x - pk0.x + pm0.x, ty = rf.y - pk0.y + pm0.y, tz = rf.z - pk0.z + pm0.z;
+                double xi = pa.x + tx, yi = pa.y + ty, zi = pa.z + tz;
 #pragma unroll
                 for (int b = 0; b < APS; ++b) {
                     d3 pb = xyz(p.xq[sm + b]);
-                    d3 u = canon ? pa : pb, w = canon ? pb : pa;
-                    double rx = (u.x + tx) - w.x, ry = (u.y + ty) - w.y, rz = (u.z + tz) - w.z;
+                    double rx = xi - pb.x, ry = yi - pb.y, rz = zi - pb.z;
                     // lj_crf_interaction (innerloops.rs:22-55), branch-free guard
                     double r2 = rx * rx + ry * ry + rz * rz;
                     bool ok = !(r2 < 1e-10);
@@ -310,14 +348,12 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad) {
                     double ecrf = qq[b] * (inv_r - c2 * r2 - cc);
                     double fcrf = qq[b] * (inv_r * inv_r2 + c3);
                     double fm = ok ? flj + fcrf : 0.0;
-                    elj = ok ? elj : 0.0;
-                    ecrf = ok ? ecrf : 0.0;
                     double gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                    fx += sg * gx;
-                    fy += sg * gy;
-                    fz += sg * gz;
-                    v[0] += elj;
-                    v[1] += ecrf;
+                    fx += gx;
+                    fy += gy;
+                    fz += gz;
+                    v[0] += ok ? elj : 0.0;
+                    v[1] += ok ? ecrf : 0.0;
                     if (VIR) add_virial(v, d3{rx, ry, rz}, gx, gy, gz);
                 }
             }
@@ -365,12 +401,19 @@ __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6,
     ecrf = ok ? ec : 0.0f;
 }
 
-// nearest-image vector between two group first atoms, FP64, as float
+// nearest-image vector between two group first atoms in FP64, returned as
+// float; ties of the image rounding go to even (rint), which only differs from
+// the reference's half-away rounding at exactly L/2
 __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, float& y, float& z) {
-    d3 r = nim(b, A, B);
-    x = (float)r.x;
-    y = (float)r.y;
-    z = (float)r.z;
+    double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    if (b.kind) {
+        dx = dx - b.L[0] * rint(dx * b.invL[0]);
+        dy = dy - b.L[1] * rint(dy * b.invL[1]);
+        dz = dz - b.L[2] * rint(dz * b.invL[2]);
+    }
+    x = (float)dx;
+    y = (float)dy;
+    z = (float)dz;
 }
 
 template <int APS, bool VIR>
@@ -383,7 +426,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t npad) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
         int32_t k = p.s_atom_cg[a];
-        int32_t e0 = (int32_t)p.vv_off[k], e1 = (int32_t)p.vv_off[k + 1];
+        int32_t e0 = (int32_t)p.vv_off[k], e1 = e0 + (int32_t)p.vv_cnt[k];
         if (e1 > e0) {
             const JobDev& J = p.jobs[p.atom_job[o]];
             const Box box = J.box;
@@ -522,7 +565,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
         d3 pk0 = xyz(p.xq[sk]);
         int32_t ta = p.s_type[a];
         bool cgp = J.use_cg_grouped != 0;
-        int32_t e0 = (int32_t)p.gen_off[k], e1 = (int32_t)p.gen_off[k + 1];
+        int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
         for (int32_t e = e0; e < e1; ++e) {
             uint32_t ent = p.gen_ent[e];
             bool lng = (ent & ENT_LONG) != 0;
@@ -595,7 +638,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
         d3 pk0 = xyz(p.xq[sk]);
         int32_t ta = p.s_type[a];
         bool cgp = J.use_cg_grouped != 0 && !LONG;
-        int32_t e0 = (int32_t)p.gen_off[k], e1 = (int32_t)p.gen_off[k + 1];
+        int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
         for (int32_t e = e0; e < e1; ++e) {
             uint32_t ent = p.gen_ent[e];
             bool lng = (ent & ENT_LONG) != 0;
@@ -961,13 +1004,13 @@ __global__ void k_materialize(SysPtrs p, int32_t ncg, unsigned long long* counts
                 if (!excluded_or_14(p, i, j)) emit(0, i, j);
             }
     }
-    for (int32_t e = (int32_t)p.vv_off[k]; e < (int32_t)p.vv_off[k + 1]; ++e) {
+    for (int32_t e = (int32_t)p.vv_off[k]; e < (int32_t)(p.vv_off[k] + p.vv_cnt[k]); ++e) {
         uint32_t ent = p.vv_ent[e];
         if (!(ent & ENT_CANON)) continue;
         int32_t m = (int32_t)(ent & ENT_IDX);
         emit(2, p.s2o[sk], p.s2o[m]);   // solvent entries hold the partner's first sorted atom
     }
-    for (int32_t e = (int32_t)p.gen_off[k]; e < (int32_t)p.gen_off[k + 1]; ++e) {
+    for (int32_t e = (int32_t)p.gen_off[k]; e < (int32_t)(p.gen_off[k] + p.gen_cnt[k]); ++e) {
         uint32_t ent = p.gen_ent[e];
         if (!(ent & ENT_CANON)) continue;
         int32_t m = (int32_t)(ent & ENT_IDX);
@@ -999,6 +1042,11 @@ void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s) {
 }
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s) {
     hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+}
+void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
+    if (mode == 0) hipLaunchKernelGGL(k_caps, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    else if (mode == 1) hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    else hipLaunchKernelGGL(k_keep_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
     // one wave per group

@@ -52,6 +52,11 @@ struct SysPtrs {
     uint32_t* vv_ent;
     uint32_t* gen_ent;
     uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild
+    int64_t* vv_cap;           // single-pass fill: entries reserved per sorted group
+    int64_t* gen_cap;
+    int32_t* prev_vv;          // previous build's counts per original group
+    int32_t* prev_gen;
+    int32_t* overflow;         // set when a group had more entries than reserved
     // per step
     double4* xq;
     float4* xf;                // mixed: FP32 offset from the group's first atom, charge
@@ -73,6 +78,8 @@ void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
+// mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
+void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
 void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);

@@ -334,6 +334,10 @@ struct gromos_amd_system {
     double *d_rows = nullptr, *d_records = nullptr, *d_pert_e = nullptr;
     int64_t rec_cap = 0;
     int32_t* d_settle_fail = nullptr;
+    uint8_t* d_pflag = nullptr;
+    int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_plist = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
+    double *d_pa_q = nullptr, *d_pb_q = nullptr, *d_palj = nullptr, *d_pacrf = nullptr;
+    int32_t n_plist = 0, n_papairs = 0;
     char* d_temp = nullptr;
     size_t temp_bytes = 0;
     int cell_bits = 1;
@@ -458,6 +462,18 @@ struct gromos_amd_system {
         p.records = d_records;
         p.pert_e = d_pert_e;
         p.settle_fail = d_settle_fail;
+        p.pflag = d_pflag;
+        p.pa_iac = d_pa_iac;
+        p.pb_iac = d_pb_iac;
+        p.pa_q = d_pa_q;
+        p.pb_q = d_pb_q;
+        p.palj = d_palj;
+        p.pacrf = d_pacrf;
+        p.plist = d_plist;
+        p.n_plist = n_plist;
+        p.papairs = d_papairs;
+        p.n_papairs = n_papairs;
+        p.o2s = d_o2s;
         return p;
     }
 
@@ -511,6 +527,9 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     std::vector<int32_t> cg_start, cg_n, cg_job, atom_job, iac, excl_off, excl, o14_off, o14, o14s_off, o14s,
         mol_first;
     std::vector<double> q, mass, inv_mass, lj;
+    std::vector<uint8_t> pflag;
+    std::vector<int32_t> pa_iac, pb_iac, plist, papairs;
+    std::vector<double> pa_q, pb_q, palj, pacrf;
     excl_off.push_back(0);
     o14_off.push_back(0);
     o14s_off.push_back(0);
@@ -587,8 +606,20 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         }
         if (T.pert_flag) {
             for (int a = 0; a < n; ++a)
-                if (T.pert_flag[a]) J.has_pert = 1;
-            if (J.has_pert) return fail(GROMOS_AMD_EINVAL, "perturbed topologies: use the pair-list correction kernels (device-resident FEP not enabled in this build)");
+                if (T.pert_flag[a]) {
+                    J.has_pert = 1;
+                    if (a >= T.n_solute_atoms) return fail(GROMOS_AMD_EINVAL, "job %d: only solute atoms can be perturbed", j);
+                    if (!T.pert_a_iac || !T.pert_b_iac || !T.pert_a_q || !T.pert_b_q || !T.pert_alpha_lj || !T.pert_alpha_crf)
+                        return fail(GROMOS_AMD_EINVAL, "job %d: incomplete perturbation arrays", j);
+                    if (T.pert_a_iac[a] < 0 || T.pert_a_iac[a] >= T.n_types || T.pert_b_iac[a] < 0 || T.pert_b_iac[a] >= T.n_types)
+                        return fail(GROMOS_AMD_EINVAL, "job %d: perturbed type of atom %d out of range", j, a);
+                }
+        }
+        // from_lambda(l, l, l, l, 1, 1, 1, 1, NLAM) (forcefield.rs:767-777)
+        {
+            gromos_amd_lambda L;
+            gromos_amd_lambda_from(P.lambda, P.lambda, P.lambda, P.lambda, 1.0, 1.0, 1.0, 1.0, P.nlam, &L);
+            std::memcpy(&J.lp, &L, sizeof(L));
         }
         if (P.use_settle && nsolv_atoms > 0) {
             if (aps != 3) return fail(GROMOS_AMD_EINVAL, "SETTLE needs 3-atom solvent (settle_algorithm.rs:43-45)");
@@ -629,12 +660,31 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
                     }
                 }
             }
+            bool pa = live && T.pert_flag && T.pert_flag[a];
+            pflag.push_back(pa ? 1 : 0);
+            pa_iac.push_back(pa ? T.pert_a_iac[a] : 0);
+            pb_iac.push_back(pa ? T.pert_b_iac[a] : 0);
+            pa_q.push_back(pa ? T.pert_a_q[a] : 0.0);
+            pb_q.push_back(pa ? T.pert_b_q[a] : 0.0);
+            palj.push_back(pa ? T.pert_alpha_lj[a] : 0.0);
+            pacrf.push_back(pa ? T.pert_alpha_crf[a] : 0.0);
+            if (pa) plist.push_back((int32_t)atom_off + a);
             excl_off.push_back((int32_t)excl.size());
             o14_off.push_back((int32_t)o14.size());
             if (live) for (int32_t e : sym[a]) o14s.push_back(e);
             o14s_off.push_back((int32_t)o14s.size());
         }
         for (int64_t k = 0; k < (int64_t)T.n_types * T.n_types * 4; ++k) lj.push_back(T.lj[k]);
+        if (T.pert_flag && T.n_pert_atom_pairs > 0) {
+            for (int k = 0; k < T.n_pert_atom_pairs; ++k) {
+                const int32_t* e = T.pert_atom_pairs + 4 * k;
+                if (e[0] < 0 || e[0] >= n || e[1] < 0 || e[1] >= n) return fail(GROMOS_AMD_EINVAL, "job %d: atom pair out of range", j);
+                papairs.push_back((int32_t)atom_off + e[0]);
+                papairs.push_back((int32_t)atom_off + e[1]);
+                papairs.push_back(e[2]);
+                papairs.push_back(e[3]);
+            }
+        }
         lj_off += (int64_t)T.n_types * T.n_types * 4;
         atom_off += npad_j;
         cg_off += T.n_cg;
@@ -672,6 +722,18 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
     rc = rc ? rc : S->upload(S->d_o14s_off, o14s_off);
     rc = rc ? rc : S->upload(S->d_o14s, o14s);
     rc = rc ? rc : S->upload(S->d_mol_first, mol_first);
+    rc = rc ? rc : S->upload(S->d_pflag, pflag);
+    rc = rc ? rc : S->upload(S->d_pa_iac, pa_iac);
+    rc = rc ? rc : S->upload(S->d_pb_iac, pb_iac);
+    rc = rc ? rc : S->upload(S->d_pa_q, pa_q);
+    rc = rc ? rc : S->upload(S->d_pb_q, pb_q);
+    rc = rc ? rc : S->upload(S->d_palj, palj);
+    rc = rc ? rc : S->upload(S->d_pacrf, pacrf);
+    rc = rc ? rc : S->upload(S->d_plist, plist);
+    rc = rc ? rc : S->upload(S->d_papairs, papairs);
+    S->n_plist = (int32_t)plist.size();
+    S->n_papairs = (int32_t)papairs.size() / 4;
+    S->has_pert = S->n_plist > 0;
     if (rc) return rc;
     size_t n3 = (size_t)S->npad * 3, ncg = (size_t)S->ncg;
     for (int k = 0; k < 2; ++k) {
@@ -686,7 +748,7 @@ static int create_batch(const gromos_amd_topology* topos, const gromos_amd_param
         (rc = S->alloc(S->d_cell_start, (size_t)S->ncell + 1)) || (rc = S->alloc(S->d_s_cg_orig, ncg)) ||
         (rc = S->alloc(S->d_s_cg_n, ncg + 1)) || (rc = S->alloc(S->d_s_cg_scan, ncg + 1)) ||
         (rc = S->alloc(S->d_s_cg_start, ncg)) || (rc = S->alloc(S->d_s2o, (size_t)S->npad)) ||
-        (rc = S->alloc(S->d_s_atom_cg, (size_t)S->npad)) || (rc = S->alloc(S->d_s_type, (size_t)S->npad)) ||
+        (rc = S->alloc(S->d_s_atom_cg, (size_t)S->npad)) || (rc = S->alloc(S->d_o2s, (size_t)S->npad)) || (rc = S->alloc(S->d_s_type, (size_t)S->npad)) ||
         (rc = S->alloc(S->d_vv_cnt, ncg + 1)) || (rc = S->alloc(S->d_gen_cnt, ncg + 1)) ||
         (rc = S->alloc(S->d_vv_off, ncg + 1)) || (rc = S->alloc(S->d_gen_off, ncg + 1)) ||
         (rc = S->alloc(S->d_counters, 4)) || (rc = S->alloc(S->d_xq, (size_t)S->npad)) ||
@@ -882,7 +944,8 @@ static int forcefield(gromos_amd_system* S, int32_t rec_step, bool with_kin_rows
         S->pair_count += (int64_t)S->host_counters[1];
     }
     launch_excl14(p, npad, st);
-    launch_assemble(p, npad, S->twin ? 1 : 0, 0, mixed, st);
+    if (S->has_pert) launch_pert(p, st);
+    launch_assemble(p, npad, S->twin ? 1 : 0, S->has_pert ? 1 : 0, mixed, st);
     (void)with_kin_rows;
     (void)rec_step;
     HIPCHK(hipGetLastError());

@@ -71,6 +71,19 @@ struct SysPtrs {
     double* records;
     double* pert_e;            // 3 per job: delta e_lj, delta e_crf, dhdl
     int32_t* settle_fail;      // per job: first failing step
+    // perturbation (original order; flag 0 = not perturbed)
+    const uint8_t* pflag;
+    const int32_t* pa_iac;
+    const int32_t* pb_iac;
+    const double* pa_q;
+    const double* pb_q;
+    const double* palj;
+    const double* pacrf;
+    const int32_t* plist;      // perturbed atoms (original index)
+    int32_t n_plist;
+    const int32_t* papairs;    // PERTATOMPAIR: i, j, a_type, b_type (original indices)
+    int32_t n_papairs;
+    int32_t* o2s;              // original atom -> sorted atom
 };
 
 void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s);
@@ -87,6 +100,7 @@ void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed, hipStream_t s);
 void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
+void launch_pert(const SysPtrs& p, hipStream_t s);
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
                    int32_t njobs, hipStream_t s);

@@ -123,3 +123,238 @@ def water_replica(seed, temperature, n_waters=4000, rc=1.2):
     """12,000-atom SPC replica box (L ~ 4.93 nm), cutoff 1.2 nm."""
     return spc_water_box(n_waters, seed=seed, temperature=temperature, rcutp=rc, rcutl=rc, nsnb=5,
                          name=f"replica_{temperature:.1f}K")
+
+
+# ---------------------------------------------------------------------------
+# heterogeneous solute + water, FEP jobs
+# ---------------------------------------------------------------------------
+def lj_type_table(rng, n_types=20):
+    """C6 log-uniform in [1e-4, 1e-2], C12 log-uniform in [1e-7, 1e-4] per type;
+    geometric mixing (crates/gromos-core/src/topology.rs:1141-1152); 1-4 table
+    scaled by 1/2."""
+    c6 = np.exp(rng.uniform(np.log(1e-4), np.log(1e-2), n_types))
+    c12 = np.exp(rng.uniform(np.log(1e-7), np.log(1e-4), n_types))
+    return c6, c12
+
+
+def _mixed_table(c6s, c12s):
+    """types: solute types ..., water O, water H; returns flat (c6, c12, cs6, cs12)."""
+    c6 = np.concatenate([c6s, [SPC_C6, 0.0]])
+    c12 = np.concatenate([c12s, [SPC_C12, 0.0]])
+    nt = len(c6)
+    C6 = np.sqrt(np.outer(c6, c6))
+    C12 = np.sqrt(np.outer(c12, c12))
+    t = np.zeros((nt, nt, 4))
+    t[..., 0], t[..., 1] = C6, C12
+    t[..., 2], t[..., 3] = 0.5 * C6, 0.5 * C12
+    return t.reshape(-1), nt
+
+
+def _periodic_min_dist2(a, b, L):
+    d = a[:, None, :] - b[None, :, :]
+    d -= L * np.round(d / L)
+    return (d * d).sum(-1)
+
+
+def _random_walk(rng, n, bond=0.15, start=None, min_dist=0.22, L=None):
+    """Self-avoiding-ish chain: successive bonds at >= 70 degrees turning
+    freedom, new atoms at least min_dist from all but their two predecessors."""
+    pos = np.zeros((n, 3))
+    pos[0] = start if start is not None else rng.uniform(0, 1, 3)
+    prev_dir = None
+    for i in range(1, n):
+        for _ in range(200):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            if prev_dir is not None and np.dot(d, prev_dir) > 0.34:   # angle to previous bond > ~110 deg apart
+                continue
+            cand = pos[i - 1] + bond * d
+            if i > 2:
+                dd = pos[: i - 2] - cand
+                if L is not None:
+                    dd -= L * np.round(dd / L)
+                if (dd * dd).sum(1).min() < min_dist * min_dist:
+                    continue
+            break
+        pos[i] = cand
+        prev_dir = -d
+    return pos
+
+
+def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5, n_types=20, pert_every=0,
+                  lam=0.5, name="hetero"):
+    """Heterogeneous single-point system: a random-walk solute chain (bond 0.15 nm)
+    in charge groups of 1-4 atoms with N(0, 0.3^2) charges shifted to zero per
+    group, 20 LJ types, 1-2 / 1-3 exclusions, 1-4 pairs with the scaled table,
+    plus SPC water.  pert_every > 0 perturbs every pert_every-th chain atom
+    (types and charges of state B drawn the same way)."""
+    rng = np.random.default_rng(seed)
+    L = float(box)
+    chain = _random_walk(rng, n_chain, start=rng.uniform(0, L, 3), L=L)
+    # charge groups of 1-4 consecutive atoms, each moved by lattice vectors so that
+    # its first atom lies in the box (the reference takes raw COGs)
+    sizes = []
+    left = n_chain
+    while left > 0:
+        k = int(min(left, rng.integers(1, 5)))
+        sizes.append(k)
+        left -= k
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    for c in range(len(sizes)):
+        a, b = starts[c], starts[c + 1]
+        chain[a:b] -= L * np.floor(chain[a] / L)
+    q = np.zeros(n_chain)
+    for c in range(len(sizes)):
+        a, b = starts[c], starts[c + 1]
+        v = rng.normal(0, 0.3, b - a)
+        q[a:b] = v - v.mean()
+    c6s, c12s = lj_type_table(rng, n_types)
+    lj, nt = _mixed_table(c6s, c12s)
+    t_o, t_h = n_types, n_types + 1
+    iac_chain = rng.integers(0, n_types, n_chain).astype(np.int32)
+    # water on a lattice, sites too close to the chain removed
+    side = int(np.ceil((n_waters * 1.25) ** (1 / 3)))
+    g = (np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * (L / side)
+    g += rng.uniform(-0.02, 0.02, g.shape)
+    keep = np.ones(len(g), bool)
+    cw = chain - L * np.floor(chain / L)
+    cell = 0.6
+    nc = max(1, int(L // cell))
+    ci = np.floor(cw / (L / nc)).astype(int) % nc
+    buckets = {}
+    for idx, c in enumerate(map(tuple, ci)):
+        buckets.setdefault(c, []).append(idx)
+    gi = np.floor(g / (L / nc)).astype(int) % nc
+    for s_idx in range(len(g)):
+        cx, cy, cz = gi[s_idx]
+        near = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    near += buckets.get(((cx + dx) % nc, (cy + dy) % nc, (cz + dz) % nc), [])
+        if near:
+            d = cw[near] - g[s_idx]
+            d -= L * np.round(d / L)
+            if ((d * d).sum(1) < 0.28 ** 2).any():
+                keep[s_idx] = False
+    g = g[keep]
+    if len(g) > n_waters:
+        g = g[np.sort(rng.choice(len(g), n_waters, replace=False))]
+    nw = len(g)
+    R = random_rotations(rng, nw)
+    wpos = (g[:, None, :] + np.einsum("nij,aj->nai", R, water_template())).reshape(-1, 3)
+    n = n_chain + 3 * nw
+    pos = np.concatenate([chain, wpos])
+    mass = np.concatenate([np.full(n_chain, 13.019), np.tile([M_O, M_H, M_H], nw)])
+    charge = np.concatenate([q, np.tile([Q_O, Q_H, Q_H], nw)])
+    iac = np.concatenate([iac_chain, np.tile(np.array([t_o, t_h, t_h], np.int32), nw)]).astype(np.int32)
+    excl = [[] for _ in range(n)]
+    o14 = [[] for _ in range(n)]
+    for i in range(n_chain):
+        for d in (1, 2):
+            if i + d < n_chain:
+                excl[i].append(i + d)
+                excl[i + d].append(i)
+        if i + 3 < n_chain:
+            o14[i].append(i + 3)
+    for w in range(nw):
+        b = n_chain + 3 * w
+        excl[b] += [b + 1, b + 2]
+        excl[b + 1] += [b, b + 2]
+        excl[b + 2] += [b, b + 1]
+    excl = [sorted(e) for e in excl]
+    excl_off, excl_idx = csr_from_lists(excl, n)
+    o14_off, o14_idx = csr_from_lists(o14, n)
+    cg_start = np.concatenate([starts, n_chain + 3 * np.arange(1, nw + 1)]).astype(np.int32)
+    pert = None
+    if pert_every:
+        pert = Perturbation.empty(n)
+        for i in range(0, n_chain, pert_every):
+            pert.flag[i] = 1
+            pert.a_iac[i], pert.a_q[i] = iac[i], charge[i]
+            pert.b_iac[i] = rng.integers(0, n_types)
+            pert.b_q[i] = rng.uniform(-0.5, 0.5)
+            pert.alpha_lj[i], pert.alpha_crf[i] = 0.5, 0.5
+        pert.lam, pert.nlam = lam, 1
+    vel = np.zeros((n, 3))
+    return System(n_atoms=n, n_solute_atoms=n_chain, cg_start=cg_start, n_solute_cg=len(sizes), charge=charge,
+                  iac=iac, mass=mass, n_types=nt, lj=lj, excl_off=excl_off, excl=excl_idx, o14_off=o14_off,
+                  o14=o14_idx, atoms_per_solvent=3, pos=pos, vel=vel, box=np.array([L, L, L]), boundary=1,
+                  rcutp=rc, rcutl=rc, nsnb=nsnb, epsrf=61.0, kappa=0.0, fpepsi=FOUR_PI_EPS_I, dt=0.002,
+                  dist_oh=D_OH, dist_hh=D_HH, pert=pert, name=name).validate()
+
+
+def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=0.5, n_types=20,
+            type_seed=5, temperature=300.0, name=None):
+    """One FEP lambda-window job: n_pert perturbed solute atoms (each its own
+    charge group, no bonds, >= 0.4 nm apart) with A/B charges uniform in
+    [-0.5, 0.5] e and A/B types from the 20-type table, in SPC water filling the
+    rest of n_atoms at 0.997 g/cm^3; soft-core alphas 0.5 / 0.5 nm^2 (global
+    ALPHLJ = ALPHC = 1)."""
+    rng = np.random.default_rng(seed)
+    n_w = max(1, (n_atoms - n_pert) // 3)
+    L = box_for_density(n_w + n_pert // 3 + 1)
+    base = spc_water_box(n_w + 64, box=L, seed=seed, temperature=temperature, rcutp=rc, rcutl=rc, nsnb=nsnb)
+    # solute sites
+    sites = []
+    while len(sites) < n_pert:
+        c = rng.uniform(0, L, 3)
+        if all(((np.asarray(sv) - c - L * np.round((np.asarray(sv) - c) / L)) ** 2).sum() >= 0.16 for sv in sites):
+            sites.append(c)
+    sites = np.array(sites)
+    o = base.pos[0::3]
+    d2 = _periodic_min_dist2(o, sites, L).min(1)
+    order = np.argsort(-d2)        # farthest waters first
+    keepw = np.sort(order[:n_w])
+    wpos = base.pos.reshape(-1, 3, 3)[keepw].reshape(-1, 3)
+    wvel = base.vel.reshape(-1, 3, 3)[keepw].reshape(-1, 3)
+    trng = np.random.default_rng(type_seed)
+    c6s, c12s = lj_type_table(trng, n_types)
+    lj, nt = _mixed_table(c6s, c12s)
+    t_o, t_h = n_types, n_types + 1
+    n = n_pert + 3 * n_w
+    a_iac = rng.integers(0, n_types, n_pert).astype(np.int32)
+    b_iac = rng.integers(0, n_types, n_pert).astype(np.int32)
+    a_q = rng.uniform(-0.5, 0.5, n_pert)
+    b_q = rng.uniform(-0.5, 0.5, n_pert)
+    pos = np.concatenate([sites, wpos])
+    mass = np.concatenate([np.full(n_pert, 15.035), np.tile([M_O, M_H, M_H], n_w)])
+    vs = np.sqrt(K_BOLTZMANN * temperature / 15.035) * rng.normal(size=(n_pert, 3)) if temperature > 0 \
+        else np.zeros((n_pert, 3))
+    vel = np.concatenate([vs, wvel])
+    charge = np.concatenate([a_q, np.tile([Q_O, Q_H, Q_H], n_w)])
+    iac = np.concatenate([a_iac, np.tile(np.array([t_o, t_h, t_h], np.int32), n_w)]).astype(np.int32)
+    excl = [[] for _ in range(n)]
+    for w in range(n_w):
+        b = n_pert + 3 * w
+        excl[b] += [b + 1, b + 2]
+        excl[b + 1] += [b, b + 2]
+        excl[b + 2] += [b, b + 1]
+    excl_off, excl_idx = csr_from_lists(excl, n)
+    pert = Perturbation.empty(n)
+    pert.flag[:n_pert] = 1
+    pert.a_iac[:n_pert], pert.b_iac[:n_pert] = a_iac, b_iac
+    pert.a_q[:n_pert], pert.b_q[:n_pert] = a_q, b_q
+    pert.alpha_lj[:n_pert], pert.alpha_crf[:n_pert] = alpha_lj, alpha_crf
+    pert.lam, pert.nlam = float(lam), 1
+    cg_start = np.concatenate([np.arange(n_pert + 1), n_pert + 3 * np.arange(1, n_w + 1)]).astype(np.int32)
+    return System(n_atoms=n, n_solute_atoms=n_pert, cg_start=cg_start, n_solute_cg=n_pert, charge=charge, iac=iac,
+                  mass=mass, n_types=nt, lj=lj, excl_off=excl_off, excl=excl_idx,
+                  o14_off=np.zeros(n + 1, np.int32), o14=np.zeros(0, np.int32), atoms_per_solvent=3, pos=pos,
+                  vel=vel, box=np.array([L, L, L]), boundary=1, rcutp=rc, rcutl=rc, nsnb=nsnb, epsrf=61.0,
+                  kappa=0.0, fpepsi=FOUR_PI_EPS_I, dt=0.002, dist_oh=D_OH, dist_hh=D_HH, pert=pert,
+                  name=name or f"fep_{n_atoms}_l{lam:.2f}").validate()
+
+
+def fep_mix(n_jobs=64, seed=3, rate_per_step=0.05, lo=1000, hi=30000, n_windows=21):
+    """The FEP serving mix: n_jobs jobs with log-uniform atom counts in [lo, hi],
+    3-30 perturbed atoms, lambda from n_windows evenly spaced windows, and seeded
+    Poisson arrival steps (exponential gaps with the given rate per MD step)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_jobs)).astype(int)
+    npert = rng.integers(3, 31, n_jobs)
+    lams = rng.integers(0, n_windows, n_jobs) / (n_windows - 1)
+    gaps = rng.exponential(1.0 / rate_per_step, n_jobs)
+    arrivals = np.floor(np.cumsum(gaps) - gaps[0]).astype(int)
+    return [dict(n_atoms=int(sizes[k]), n_pert=int(npert[k]), lam=float(lams[k]), seed=int(seed * 1000 + k),
+                 arrival=int(arrivals[k])) for k in range(n_jobs)]

@@ -155,7 +155,7 @@ def test_cg_grouped_kernel_matches_oracle():
 # --------------------------------------------------------------------------
 # device-resident systems: GPU pair list + Forcefield::apply
 # --------------------------------------------------------------------------
-RESIDENT = [n for n in golden_io.names() if n != "ch4_water_fep"]
+RESIDENT = golden_io.names()
 
 
 @pytest.mark.parametrize("name", RESIDENT)
@@ -167,7 +167,7 @@ def test_resident_pairlist_and_forces_fixture(name):
     assert_same_lists(ds.pairlist(), oracle.pairlist(o))
     g = ds.forcefield_apply()
     r = oracle.single_point(o)
-    for k in ("e_lj", "e_crf"):
+    for k in ("e_lj", "e_crf", "dhdl"):
         assert rel(g[k], r[k]) < E_TOL or abs(g[k] - r[k]) < 1e-12, (k, g[k], r[k])
     assert force_err(g["forces"], r["forces"]) < F_TOL
     ds.close()
@@ -301,3 +301,41 @@ def test_mixed_md_tracks_fp64():
     # energy drift over the run agrees within 2% of the kinetic energy scale
     d64, dmx = tot64[-1] - tot64[0], totmx[-1] - totmx[0]
     assert abs(dmx - d64) <= 0.02 * abs(e64[-1, 3]), (d64, dmx)
+
+
+# --------------------------------------------------------------------------
+# free energy: device-resident soft-core correction passes against the oracle
+# (soft-core CRF with charges, dH/dlambda, excluded / 1-4 / atom-pair passes are
+# parity-unpinned against the reference: this checks GPU == oracle restatement)
+# --------------------------------------------------------------------------
+def _fep_case(s, prec=None, tol_e=E_TOL, tol_f=F_TOL):
+    o = oracle.OracleSystem(s)
+    ds = rt.DeviceSystem(s, precision=rt.FP64 if prec is None else prec)
+    ds.pairlist_update()
+    assert_same_lists(ds.pairlist(), oracle.pairlist(o))
+    g = ds.forcefield_apply()
+    r = oracle.single_point(o)
+    for k in ("e_lj", "e_crf", "dhdl"):
+        assert abs(g[k] - r[k]) <= tol_e * max(abs(r[k]), 1.0), (k, g[k], r[k])
+    assert force_err(g["forces"], r["forces"]) < tol_f
+    ds.close()
+    return g, r
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.35, 1.0])
+def test_resident_fep_job(lam):
+    s = gen.fep_job(3000, 12, lam, seed=21)
+    _fep_case(s)
+
+
+def test_resident_fep_bonded_solute_all_passes():
+    s = gen.hetero_system(n_chain=300, n_waters=700, box=3.2, seed=8, pert_every=5, lam=0.4)
+    # PERTATOMPAIR entries on some 1-4 pairs (a_type 0 / 1, b_type absent or 1-4)
+    s.pert.atom_pairs = np.array([[0, 3, 0, -1], [5, 8, 1, 0], [10, 13, 0, 1]], np.int32)
+    g, r = _fep_case(s)
+    assert g["dhdl"] != 0.0
+
+
+def test_mixed_fep_job():
+    s = gen.fep_job(3000, 12, 0.45, seed=22)
+    _fep_case(s, prec=rt.MIXED, tol_e=MX_E_TOL, tol_f=MX_F_TOL)

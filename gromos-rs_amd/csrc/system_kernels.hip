@@ -1054,11 +1054,21 @@ __global__ void __launch_bounds__(BLOCK) k_pert_atoms(SysPtrs p) {
 // step
 // ---------------------------------------------------------------------------
 
-// LeapFrogVelocity then LeapFrogPosition (leap_frog.rs:37-68, 116-140)
-__global__ void k_leapfrog(SysPtrs p, int32_t npad) {
+// LeapFrogVelocity then LeapFrogPosition (leap_frog.rs:37-68, 116-140).  A job
+// past its stop step keeps its state (copied to the new buffers).
+__global__ void k_leapfrog(SysPtrs p, int32_t npad, int32_t step) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad || p.atom_job[i] < 0) return;
-    double dt = p.jobs[p.atom_job[i]].dt;
+    const JobDev& J = p.jobs[p.atom_job[i]];
+    if (step >= J.stop) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p.v_new[3 * i + c] = p.v_cur[3 * i + c];
+            p.x_new[3 * i + c] = p.x_cur[3 * i + c];
+        }
+        return;
+    }
+    double dt = J.dt;
     double im = p.inv_mass[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1095,6 +1105,7 @@ __global__ void k_settle(SysPtrs p, int32_t nmol, int32_t step) {
     int32_t i = p.mol_first[mi];
     int32_t jb = p.atom_job[i];
     const JobDev& J = p.jobs[jb];
+    if (step >= J.stop) return;
     double m_o = J.m_o, m_h = J.m_h;
     double half_mo_div_mh = 0.5 * m_o / m_h;
     double rc = 0.5 * J.dist_hh;
@@ -1329,8 +1340,8 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
                      hipStream_t s) {
     hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, with_pert, mixed);
 }
-void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s) {
-    hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s) {
+    hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad, step);
 }
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s) {
     if (nmol > 0) hipLaunchKernelGGL(k_settle, grid_for(nmol), dim3(BLOCK), 0, s, p, nmol, step);

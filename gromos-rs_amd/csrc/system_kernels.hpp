@@ -98,7 +98,7 @@ void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int3
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed, hipStream_t s);
-void launch_leapfrog(const SysPtrs& p, int32_t npad, hipStream_t s);
+void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s);
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
 void launch_pert(const SysPtrs& p, hipStream_t s);
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);

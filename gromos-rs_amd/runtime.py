@@ -55,6 +55,16 @@ class Topology(C.Structure):
                 ("pert_atom_pairs", P)]
 
 
+class SessionParams(C.Structure):
+    _fields_ = [("device", I32), ("hbm_budget", I64), ("precision", I32), ("nsnb", I32), ("admit_interval", I32)]
+
+
+class SessionInfo(C.Structure):
+    _fields_ = [("queued", I32), ("running", I32), ("done", I32), ("resident_jobs", I32),
+                ("device_bytes", I64), ("reserved_bytes", I64), ("budget", I64), ("clock", I64),
+                ("layouts", I64), ("layout_ms", D), ("pair_count", I64)]
+
+
 class Params(C.Structure):
     _fields_ = [("boundary", I32), ("box", D * 3), ("rcutp", D), ("rcutl", D), ("skin", D),
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
@@ -115,6 +125,21 @@ def _declare(L):
     L.gromos_amd_system_kernel_timing.argtypes = [P, I32, P, P]
     L.gromos_amd_system_device_bytes.argtypes = [P]
     L.gromos_amd_system_device_bytes.restype = I64
+    L.gromos_amd_session_create.argtypes = [C.POINTER(SessionParams), C.POINTER(P)]
+    L.gromos_amd_session_destroy.argtypes = [P]
+    L.gromos_amd_job_footprint.argtypes = [C.POINTER(Topology), C.POINTER(Params)]
+    L.gromos_amd_job_footprint.restype = I64
+    L.gromos_amd_job_admit.argtypes = [P, C.POINTER(Topology), C.POINTER(Params), P, P, I64, C.POINTER(I64)]
+    L.gromos_amd_session_step.argtypes = [P, I32]
+    L.gromos_amd_job_query.argtypes = [P, I64, C.POINTER(I32), C.POINTER(I64)]
+    L.gromos_amd_job_fetch.argtypes = [P, I64, P, P]
+    L.gromos_amd_job_energies.argtypes = [P, I64, I64, I64, P]
+    L.gromos_amd_job_evict.argtypes = [P, I64]
+    L.gromos_amd_jobs_kinetic_energy.argtypes = [P, I32, P, P]
+    L.gromos_amd_jobs_scale_velocities.argtypes = [P, I32, P, P]
+    L.gromos_amd_jobs_swap_configurations.argtypes = [P, I64, I64]
+    L.gromos_amd_session_get_info.argtypes = [P, C.POINTER(SessionInfo)]
+    L.gromos_amd_session_step_times.argtypes = [P, I32, P, I64, C.POINTER(I64)]
 
 
 def check(rc):
@@ -272,3 +297,115 @@ class DeviceSystem:
         ms, n = D(), I64()
         check(lib().gromos_amd_system_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+EBUSY = -5
+QUEUED, RUNNING, DONE = 0, 1, 2
+
+
+class Busy(GromosAmdError):
+    """The job's footprint does not fit the session's remaining HBM budget now."""
+
+
+class Session:
+    """Continuous batching of independent jobs on one device (gromos_amd_session).
+
+    Jobs are admitted against an HBM budget and join / leave the resident batch
+    at clock multiples of ``admit_interval`` (a multiple of the shared pair-list
+    cadence ``nsnb``); every job computes exactly what it computes alone."""
+
+    def __init__(self, precision=FP64, nsnb=5, admit_interval=0, hbm_budget=0, device=0):
+        sp = SessionParams(device=device, hbm_budget=int(hbm_budget), precision=precision, nsnb=nsnb,
+                           admit_interval=admit_interval)
+        h = P()
+        check(lib().gromos_amd_session_create(C.byref(sp), C.byref(h)))
+        self.h = h
+        self.n_atoms = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gromos_amd_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def footprint(s, use_settle=True):
+        keep = _Keep()
+        t = topology_struct(s, keep)
+        p = params_struct(s, use_settle and s.nsm > 0 and s.atoms_per_solvent == 3)
+        b = lib().gromos_amd_job_footprint(C.byref(t), C.byref(p))
+        if b < 0:
+            check(int(b))
+        return b
+
+    def admit(self, s, nsteps, use_settle=True, do_virial=False):
+        """queue job `s` for `nsteps` md steps; raises Busy when the budget is full"""
+        keep = _Keep()
+        t = topology_struct(s, keep)
+        p = params_struct(s, use_settle and s.nsm > 0 and s.atoms_per_solvent == 3, do_virial)
+        x = np.ascontiguousarray(s.pos, np.float64).reshape(-1)
+        v = None if s.vel is None else np.ascontiguousarray(s.vel, np.float64).reshape(-1)
+        jid = I64()
+        rc = lib().gromos_amd_job_admit(self.h, C.byref(t), C.byref(p), ptr(x), ptr(v), int(nsteps), C.byref(jid))
+        if rc == EBUSY:
+            raise Busy(lib().gromos_amd_last_error().decode())
+        check(rc)
+        self.n_atoms[jid.value] = s.n_atoms
+        return jid.value
+
+    def step(self, nsteps):
+        check(lib().gromos_amd_session_step(self.h, int(nsteps)))
+
+    def query(self, job):
+        st, done = I32(), I64()
+        check(lib().gromos_amd_job_query(self.h, int(job), C.byref(st), C.byref(done)))
+        return st.value, done.value
+
+    def fetch(self, job):
+        n = self.n_atoms[job]
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        check(lib().gromos_amd_job_fetch(self.h, int(job), ptr(x), ptr(v)))
+        return x, v
+
+    def energies(self, job, first=0, count=None):
+        if count is None:
+            count = self.query(job)[1] - first
+        e = np.zeros((max(count, 0), 5))
+        if count > 0:
+            check(lib().gromos_amd_job_energies(self.h, int(job), int(first), int(count), ptr(e)))
+        return e
+
+    def evict(self, job):
+        check(lib().gromos_amd_job_evict(self.h, int(job)))
+        self.n_atoms.pop(job, None)
+
+    def kinetic_energy(self, jobs):
+        ids = np.ascontiguousarray(jobs, np.int64)
+        out = np.zeros(len(ids))
+        check(lib().gromos_amd_jobs_kinetic_energy(self.h, len(ids), ptr(ids), ptr(out)))
+        return out
+
+    def scale_velocities(self, jobs, factors):
+        ids = np.ascontiguousarray(jobs, np.int64)
+        f = np.ascontiguousarray(factors, np.float64)
+        check(lib().gromos_amd_jobs_scale_velocities(self.h, len(ids), ptr(ids), ptr(f)))
+
+    def swap_configurations(self, a, b):
+        check(lib().gromos_amd_jobs_swap_configurations(self.h, int(a), int(b)))
+
+    def info(self):
+        i = SessionInfo()
+        check(lib().gromos_amd_session_get_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in SessionInfo._fields_}
+
+    def step_times(self, enable=-1):
+        n = I64()
+        check(lib().gromos_amd_session_step_times(self.h, -1, None, 0, C.byref(n)))
+        ms = np.zeros(n.value)
+        check(lib().gromos_amd_session_step_times(self.h, int(enable), ptr(ms), n.value, C.byref(n)))
+        return ms

@@ -233,6 +233,75 @@ int gromos_amd_system_kernel_timing(gromos_amd_system *sys, int32_t enable, doub
 /* HBM bytes held by the system's device buffers. */
 int64_t gromos_amd_system_device_bytes(const gromos_amd_system *sys);
 
+/* ------------------------------------------------------------------ */
+/* 3. sessions: continuous batching of independent jobs on one device  */
+/*    (no counterpart in the reference, which runs one system per      */
+/*    process, md.rs, and steps replicas one after another, remd.rs)   */
+/* ------------------------------------------------------------------ */
+
+typedef struct gromos_amd_session gromos_amd_session;
+
+enum gromos_amd_job_state {
+    GROMOS_AMD_JOB_QUEUED = 0,   /* admitted, waiting for the next layout boundary */
+    GROMOS_AMD_JOB_RUNNING = 1,
+    GROMOS_AMD_JOB_DONE = 2      /* all requested steps done */
+};
+
+typedef struct {
+    int32_t device;          /* HIP device ordinal */
+    int64_t hbm_budget;      /* bytes of job footprint the session may reserve; 0 = 90 % of free memory */
+    int32_t precision;       /* gromos_amd_precision, for every job */
+    int32_t nsnb;            /* pair-list cadence shared by all jobs (params.nsnb must match) */
+    int32_t admit_interval;  /* steps between batch layout changes, a multiple of nsnb; 0 = nsnb */
+} gromos_amd_session_params;
+
+typedef struct {
+    int32_t queued, running, done, resident_jobs;
+    int64_t device_bytes;    /* held by the resident batch */
+    int64_t reserved_bytes;  /* footprints of queued and resident jobs */
+    int64_t budget;
+    int64_t clock;           /* session steps so far */
+    int64_t layouts;         /* batch layout changes */
+    double layout_ms;        /* host wall time spent in them */
+    int64_t pair_count;      /* pair interactions evaluated, all jobs */
+} gromos_amd_session_info;
+
+int gromos_amd_session_create(const gromos_amd_session_params *p, gromos_amd_session **out);
+void gromos_amd_session_destroy(gromos_amd_session *s);
+/* Device bytes a job adds to a batch (estimate used for admission); < 0 on error. */
+int64_t gromos_amd_job_footprint(const gromos_amd_topology *t, const gromos_amd_params *p);
+/* Copy topology and state (host) into the session and queue the job for nsteps
+ * md steps.  GROMOS_AMD_EBUSY when its footprint does not fit the remaining
+ * budget now (retry after jobs finish), GROMOS_AMD_ENOMEM when it never will. */
+int gromos_amd_job_admit(gromos_amd_session *s, const gromos_amd_topology *t, const gromos_amd_params *p,
+                         const double *pos, const double *vel, int64_t nsteps, int64_t *job_id);
+/* Advance the session clock by nsteps: every running job does one md step per
+ * clock step until its budget is used; queued jobs join and finished jobs leave
+ * at clock multiples of admit_interval (at once when nothing is running). */
+int gromos_amd_session_step(gromos_amd_session *s, int32_t nsteps);
+int gromos_amd_job_query(gromos_amd_session *s, int64_t job, int32_t *state, int64_t *steps_done);
+/* Current positions / velocities (n_atoms * 3, host); either may be NULL. */
+int gromos_amd_job_fetch(gromos_amd_session *s, int64_t job, double *pos, double *vel);
+/* Energy records of steps [first, first + count): 5 doubles per step
+ * (e_lj, e_crf, e_pot, e_kin, dhdl), as gromos_amd_system_md. */
+int gromos_amd_job_energies(gromos_amd_session *s, int64_t job, int64_t first, int64_t count, double *out);
+/* Remove a job (a resident one leaves at the next layout boundary). */
+int gromos_amd_job_evict(gromos_amd_session *s, int64_t job);
+/* Replica exchange support, resident jobs, between steps:
+ * 0.5 sum m v^2 of the current velocities (Replica::temperature, replica.rs:343-347). */
+int gromos_amd_jobs_kinetic_energy(gromos_amd_session *s, int32_t n, const int64_t *jobs, double *ekin);
+/* v *= factor per job (Replica::rescale_velocities, replica.rs:355-371). */
+int gromos_amd_jobs_scale_velocities(gromos_amd_session *s, int32_t n, const int64_t *jobs,
+                                     const double *factors);
+/* Exchange positions and velocities of two jobs with equal atom counts
+ * (Replica::exchange_configuration, replica.rs:350-352); only at clock
+ * multiples of nsnb, the next step rebuilds the pair lists. */
+int gromos_amd_jobs_swap_configurations(gromos_amd_session *s, int64_t a, int64_t b);
+int gromos_amd_session_get_info(gromos_amd_session *s, gromos_amd_session_info *info);
+/* Device time of each session step (hipEvents) while enabled: enable = 1 clears
+ * and starts, 0 stops, -1 leaves it; copies up to cap values, *n = how many. */
+int gromos_amd_session_step_times(gromos_amd_session *s, int32_t enable, double *ms, int64_t cap, int64_t *n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,154 @@
+"""GPU tests of the session (continuous batching of independent jobs).
+
+A job's results must not depend on the batch it runs in: each job, admitted at
+some clock time into a batch whose other members come and go, is compared with
+the same job run alone on a single-job system.  Jobs without perturbed atoms
+are bit-identical (per-job fixed-order reductions, jobs on separate 256-atom
+aligned ranges, pair lists rebuilt on the job's own step multiples of NSNB).
+The free-energy correction passes add with FP64 atomics, whose order varies,
+so jobs with perturbed atoms agree to 1e-11 relative over the short runs here.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+gamd = importlib.import_module("gromos-rs_amd")
+rt = gamd.runtime
+gen = importlib.import_module("gromos-rs_amd.generators")
+
+pytestmark = pytest.mark.gpu
+
+PERT_REL = 1e-11
+
+
+def alone(s, nsteps, prec):
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    e = ds.md(nsteps)
+    x, v = ds.get_state()
+    ds.close()
+    return e, x, v
+
+
+def job_specs():
+    # (system, nsteps, admit clock)
+    return [
+        (gen.water_216(seed=3), 23, 0),
+        (gen.spc_water_box(1200, seed=5), 31, 3),
+        (gen.hetero_system(n_chain=240, n_waters=600, box=3.0, seed=4), 17, 10),
+        (gen.fep_job(2000, 6, 0.4, seed=3), 26, 12),
+        (gen.spc_water_box(300, seed=9), 9, 12),
+    ]
+
+
+def close(a, b, exact):
+    if exact:
+        return np.array_equal(a, b)
+    return np.all(np.abs(a - b) <= PERT_REL * np.maximum(np.abs(b), 1.0))
+
+
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_session_jobs_match_alone(prec):
+    specs = job_specs()
+    ref = [alone(s, n, prec) for s, n, _ in specs]
+    sess = rt.Session(precision=prec, nsnb=5)
+    ids = [None] * len(specs)
+    chunks = [3, 4, 6, 1, 7, 2, 5, 8, 3, 11, 9, 4]
+    k = 0
+    while True:
+        clock = sess.info()["clock"]
+        for i, (s, n, at) in enumerate(specs):
+            if ids[i] is None and clock >= at:
+                ids[i] = sess.admit(s, n)
+        if all(j is not None for j in ids) and all(sess.query(j)[0] == rt.DONE for j in ids):
+            break
+        sess.step(chunks[k % len(chunks)])
+        k += 1
+        assert k < 200
+    info = sess.info()
+    assert info["layouts"] >= 3
+    for i, (s, n, _) in enumerate(specs):
+        st, done = sess.query(ids[i])
+        assert st == rt.DONE and done == n
+        e = sess.energies(ids[i])
+        x, v = sess.fetch(ids[i])
+        e0, x0, v0 = ref[i]
+        exact = s.pert is None or not np.any(s.pert.flag)
+        assert close(e, e0, exact), (i, np.max(np.abs(e - e0)))
+        assert close(x, x0, exact), (i, np.max(np.abs(x - x0)))
+        assert close(v, v0, exact), (i, np.max(np.abs(v - v0)))
+    sess.close()
+
+
+def test_session_admission_by_footprint():
+    a = gen.spc_water_box(900, seed=1)
+    b = gen.spc_water_box(1000, seed=2)
+    fa, fb = rt.Session.footprint(a), rt.Session.footprint(b)
+    sess = rt.Session(precision=rt.FP64, nsnb=5, hbm_budget=fa + fb // 2)
+    ja = sess.admit(a, 10)
+    with pytest.raises(rt.Busy):
+        sess.admit(b, 10)
+    info = sess.info()
+    assert info["reserved_bytes"] == fa and info["queued"] == 1
+    sess.step(10)
+    assert sess.query(ja)[0] == rt.DONE
+    # a finished job leaves at the boundary, releasing its reservation
+    assert sess.info()["reserved_bytes"] == 0
+    jb = sess.admit(b, 5)
+    sess.step(5)
+    assert sess.query(jb) == (rt.DONE, 5)
+    e_alone, x_alone, _ = alone(b, 5, rt.FP64)
+    assert np.array_equal(sess.energies(jb), e_alone)
+    assert np.array_equal(sess.fetch(jb)[0], x_alone)
+    # a job larger than the whole budget is refused outright
+    with pytest.raises(rt.GromosAmdError):
+        rt.Session(precision=rt.FP64, nsnb=5, hbm_budget=fa // 2).admit(a, 1)
+    sess.close()
+
+
+def test_session_evict_and_idle_clock():
+    sess = rt.Session(precision=rt.FP64, nsnb=5)
+    sess.step(3)                      # nothing resident: the clock advances
+    s = gen.spc_water_box(400, seed=7)
+    j = sess.admit(s, 20)
+    sess.step(2)                      # joins at clock 5
+    assert sess.query(j) == (rt.RUNNING, 0)
+    sess.step(6)
+    assert sess.query(j) == (rt.RUNNING, 6)
+    sess.evict(j)
+    sess.step(5)
+    assert sess.info()["resident_jobs"] == 0
+    with pytest.raises(rt.GromosAmdError):
+        sess.query(j)
+    sess.close()
+
+
+def test_replica_exchange_primitives():
+    a = gen.water_replica(seed=1, temperature=280.0, n_waters=500)
+    b = gen.water_replica(seed=2, temperature=320.0, n_waters=500)
+    sess = rt.Session(precision=rt.FP64, nsnb=5)
+    ja, jb = sess.admit(a, 100), sess.admit(b, 100)
+    sess.step(10)
+    xa, va = sess.fetch(ja)
+    xb, vb = sess.fetch(jb)
+    ke = sess.kinetic_energy([ja, jb])
+    m = a.mass[:, None]
+    assert np.allclose(ke, [0.5 * np.sum(m * va ** 2), 0.5 * np.sum(m * vb ** 2)], rtol=1e-12)
+    sess.swap_configurations(ja, jb)
+    xa2, va2 = sess.fetch(ja)
+    xb2, vb2 = sess.fetch(jb)
+    assert np.array_equal(xa2, xb) and np.array_equal(va2, vb)
+    assert np.array_equal(xb2, xa) and np.array_equal(vb2, va)
+    sess.scale_velocities([ja, jb], [0.5, 2.0])
+    ke2 = sess.kinetic_energy([ja, jb])
+    assert np.allclose(ke2, [0.25 * ke[1], 4.0 * ke[0]], rtol=1e-12)
+    # continuing after the exchange: job a now runs b's configuration; the same
+    # as a single system started from that state
+    sess.step(5)
+    s2 = gen.water_replica(seed=1, temperature=280.0, n_waters=500)
+    s2.pos, s2.vel = xb, vb * 0.5
+    e_alone, x_alone, _ = alone(s2, 5, rt.FP64)
+    assert np.array_equal(sess.energies(ja, first=10), e_alone)
+    assert np.array_equal(sess.fetch(ja)[0], x_alone)
+    sess.close()

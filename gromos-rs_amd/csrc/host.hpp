@@ -47,7 +47,7 @@ struct JobImage {
     std::vector<int32_t> mol_first;        // SETTLE waters
     bool has_pert = false;
     std::vector<uint8_t> pflag;
-    std::vector<int32_t> pa_iac, pb_iac, plist, papairs;
+    std::vector<int32_t> pa_iac, pb_iac, papairs;
     std::vector<double> pa_q, pb_q, palj, pacrf;
     int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
 };
@@ -91,13 +91,14 @@ struct gromos_amd_system {
     float4* d_xf = nullptr;
     long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
     int32_t nrow_blocks = 0;
-    double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr, *d_f_p = nullptr;
-    double *d_rows = nullptr, *d_records = nullptr, *d_pert_e = nullptr;
+    double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
+    double *d_rows = nullptr, *d_records = nullptr;
     int32_t* d_settle_fail = nullptr;
-    uint8_t* d_pflag = nullptr;
-    int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_plist = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
+    uint8_t *d_pflag = nullptr, *d_cg_pert = nullptr;
+    int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
+    int32_t *d_pap_off = nullptr, *d_pap = nullptr;
     double *d_pa_q = nullptr, *d_pb_q = nullptr, *d_palj = nullptr, *d_pacrf = nullptr;
-    int32_t n_plist = 0, n_papairs = 0;
+    int32_t n_papairs = 0;
     char* d_temp = nullptr;
     size_t temp_bytes = 0;
     int cell_bits = 1;

@@ -181,8 +181,6 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
         im.pb_q.assign(T.pert_b_q, T.pert_b_q + n);
         im.palj.assign(T.pert_alpha_lj, T.pert_alpha_lj + n);
         im.pacrf.assign(T.pert_alpha_crf, T.pert_alpha_crf + n);
-        for (int a = 0; a < n; ++a)
-            if (T.pert_flag[a]) im.plist.push_back(a);
         for (int k = 0; k < T.n_pert_atom_pairs; ++k) {
             const int32_t* e = T.pert_atom_pairs + 4 * k;
             if (e[0] < 0 || e[0] >= n || e[1] < 0 || e[1] >= n) return fail(GROMOS_AMD_EINVAL, "job %d: atom pair out of range", j);
@@ -240,8 +238,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     bool any_pert = false;
     for (const JobImage* im : imgs) any_pert = any_pert || im->has_pert;
     std::vector<uint8_t> pflag;
-    std::vector<int32_t> pa_iac, pb_iac, plist, papairs;
+    std::vector<int32_t> pa_iac, pb_iac, papairs;
     std::vector<double> pa_q, pb_q, palj, pacrf;
+    std::vector<uint8_t> cg_pert;
     if (any_pert) {
         pflag.assign(npad, 0);
         pa_iac.assign(npad, 0);
@@ -281,6 +280,12 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             cg_start.push_back(A + im.cg_start[c]);
             cg_n.push_back(im.cg_start[c + 1] - im.cg_start[c]);
             cg_job.push_back(j);
+            if (any_pert) {
+                uint8_t f = 0;
+                if (im.has_pert)
+                    for (int a = im.cg_start[c]; a < im.cg_start[c + 1]; ++a) f |= im.pflag[a];
+                cg_pert.push_back(f ? 1 : 0);
+            }
         }
         int64_t npad_j = align_up(im.n, BLOCK);
         for (int a = 0; a < im.n; ++a) {
@@ -315,7 +320,6 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         }
         if (im.P.use_settle)
             for (int32_t m : im.mol_first) mol_first.push_back(A + m);
-        for (int32_t a : im.plist) plist.push_back(A + a);
         for (size_t k = 0; k < im.papairs.size(); k += 4) {
             papairs.push_back(A + im.papairs[k]);
             papairs.push_back(A + im.papairs[k + 1]);
@@ -338,9 +342,25 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->twin = twin;
     S->cell_bits = 1;
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
-    S->n_plist = (int32_t)plist.size();
     S->n_papairs = (int32_t)papairs.size() / 4;
-    S->has_pert = S->n_plist > 0;
+    S->has_pert = any_pert;
+    // per-atom index of the PERTATOMPAIR entries (both atoms of an entry)
+    std::vector<int32_t> pap_off, pap;
+    if (any_pert) {
+        std::vector<std::vector<int32_t>> per(npad);
+        for (int32_t e = 0; e < S->n_papairs; ++e) {
+            int32_t i = papairs[4 * e], jj = papairs[4 * e + 1];
+            if (i == jj) continue;   // zero distance: skipped by the reference (perturbed.rs:668-671)
+            per[i].push_back((int32_t)((uint32_t)e | 0x80000000u));
+            per[jj].push_back(e);
+        }
+        pap_off.reserve(npad + 1);
+        pap_off.push_back(0);
+        for (int64_t a = 0; a < npad; ++a) {
+            for (int32_t x : per[a]) pap.push_back(x);
+            pap_off.push_back((int32_t)pap.size());
+        }
+    }
 
     hipStream_t st = S->stream;
     int rc = 0;
@@ -368,8 +388,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_pb_q, pb_q);
     rc = rc ? rc : S->upload(S->d_palj, palj);
     rc = rc ? rc : S->upload(S->d_pacrf, pacrf);
-    rc = rc ? rc : S->upload(S->d_plist, plist);
+    rc = rc ? rc : S->upload(S->d_cg_pert, cg_pert);
     rc = rc ? rc : S->upload(S->d_papairs, papairs);
+    rc = rc ? rc : S->upload(S->d_pap_off, pap_off);
+    rc = rc ? rc : S->upload(S->d_pap, pap);
     if (rc) return rc;
     size_t n3 = (size_t)npad * 3, nc = (size_t)ncg, ncl = (size_t)S->ncell + 1, npa = (size_t)npad;
     for (int k = 0; k < 2; ++k) {
@@ -393,16 +415,13 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_gen_cap, nc + 1, true)) || (rc = S->reserve(S->d_prev_vv, nc, true)) ||
         (rc = S->reserve(S->d_prev_gen, nc, true)) || (rc = S->reserve(S->d_overflow, 1)) ||
         (rc = S->reserve(S->d_f_s, n3, true)) || (rc = S->reserve(S->d_f_long_s, n3, true)) ||
-        (rc = S->reserve(S->d_f_x, n3, true)) || (rc = S->reserve(S->d_f_p, n3, true)) ||
+        (rc = S->reserve(S->d_f_x, n3, true)) ||
         (rc = S->reserve(S->d_xf, npa, true)) || (rc = S->reserve(S->d_fi_s, n3, true)) ||
         (rc = S->reserve(S->d_fi_long_s, n3, true)) ||
         (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
-        (rc = S->reserve(S->d_pert_e, (size_t)3 * njobs, true)) ||
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(S->d_f_p, 0, n3 * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(S->d_pert_e, 0, 3 * njobs * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_fi_long_s, 0, n3 * sizeof(long long), st));
     HIPCHK(hipMemsetAsync(S->d_s_cg_n, 0, (nc + 1) * sizeof(int32_t), st));
@@ -555,8 +574,7 @@ int forcefield(gromos_amd_system* S) {
         S->pair_count += (int64_t)S->host_counters[1];
     }
     launch_excl14(p, npad, st);
-    if (S->has_pert) launch_pert(p, st);
-    launch_assemble(p, npad, S->twin ? 1 : 0, S->has_pert ? 1 : 0, mixed, st);
+    launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -713,10 +731,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.f_s = d_f_s;
     p.f_long_s = d_f_long_s;
     p.f_x = d_f_x;
-    p.f_p = d_f_p;
     p.rows = d_rows;
     p.records = d_records;
-    p.pert_e = d_pert_e;
     p.settle_fail = d_settle_fail;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
@@ -725,10 +741,10 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.pb_q = d_pb_q;
     p.palj = d_palj;
     p.pacrf = d_pacrf;
-    p.plist = d_plist;
-    p.n_plist = n_plist;
+    p.cg_pert = d_cg_pert;
     p.papairs = d_papairs;
-    p.n_papairs = n_papairs;
+    p.pap_off = d_pap_off;
+    p.pap = d_pap;
     p.o2s = d_o2s;
     return p;
 }

@@ -514,6 +514,174 @@ __device__ __forceinline__ bool excluded_or_14(const SysPtrs& p, int32_t i, int3
     return false;
 }
 
+// ---------------------------------------------------------------------------
+// perturbation (free energy).  The reference evaluates every pair in state A
+// and then adds (perturbed - state A) for the listed pairs whose first atom is
+// perturbed, for the excluded and 1-4 pairs with a perturbed atom, the RF self
+// terms and the PERTATOMPAIR entries (forcefield.rs:764-951).  Here each lane
+// adds those deltas for its own atom in the pass that evaluates the regular
+// pair (k_gen / k_gen_mx / k_excl14), so no atomics are needed and the result
+// does not depend on scheduling; the pair's first lane adds the energies and
+// dH/dlambda (row column 11).
+// ---------------------------------------------------------------------------
+struct PertPair {
+    const double* la;   // state-A LJ entry (c6, c12, cs6, cs12)
+    const double* lb;   // state-B
+    double a_q, b_q, alj, acrf;
+};
+
+// dual-topology parameters of the pair (i first) (perturbed.rs:335-355, 531-562)
+__device__ __forceinline__ PertPair pert_params(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j) {
+    bool pi = p.pflag[i] != 0, pj = p.pflag[j] != 0;
+    int32_t a_iac_i = pi ? p.pa_iac[i] : p.iac[i], b_iac_i = pi ? p.pb_iac[i] : p.iac[i];
+    int32_t a_iac_j = pj ? p.pa_iac[j] : p.iac[j], b_iac_j = pj ? p.pb_iac[j] : p.iac[j];
+    PertPair q;
+    q.la = p.lj + J.lj_off + 4 * ((int64_t)a_iac_i * J.n_types + a_iac_j);
+    q.lb = p.lj + J.lj_off + 4 * ((int64_t)b_iac_i * J.n_types + b_iac_j);
+    q.a_q = (pi ? p.pa_q[i] : p.q[i]) * (pj ? p.pa_q[j] : p.q[j]);
+    q.b_q = (pi ? p.pb_q[i] : p.q[i]) * (pj ? p.pb_q[j] : p.q[j]);
+    if (pi && pj) {
+        q.alj = (p.palj[i] + p.palj[j]) * 0.5;
+        q.acrf = (p.pacrf[i] + p.pacrf[j]) * 0.5;
+    } else {
+        int32_t w = pi ? i : j;
+        q.alj = p.palj[w];
+        q.acrf = p.pacrf[w];
+    }
+    return q;
+}
+
+__device__ __forceinline__ void own_side(bool first, d3 r, double d, double& fx, double& fy, double& fz) {
+    double gx = r.x * d, gy = r.y * d, gz = r.z * d;
+    if (first) {
+        fx += gx; fy += gy; fz += gz;
+    } else {
+        fx -= gx; fy -= gy; fz -= gz;
+    }
+}
+
+// perturbed_pairlist_correction body for one listed pair, i first and perturbed
+// (perturbed.rs:327-384)
+__device__ __forceinline__ void pert_pair_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
+                                                double (&v)[ROW], double& fx, double& fy, double& fz) {
+    PertPair q = pert_params(p, J, i, j);
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    double fa, elja, ecrfa, o[5];
+    lj_crf64(r.x, r.y, r.z, q.la[0], q.la[1], q.a_q * J.fpepsi, J.crf, fa, elja, ecrfa);
+    pert_lj_crf64(r.x, r.y, r.z, q.la[0], q.la[1], q.lb[0], q.lb[1], q.a_q, q.b_q, q.alj, q.acrf, J.lp, J.crf,
+                  J.fpepsi, o);
+    own_side(first, r, o[0] - fa, fx, fy, fz);
+    if (first) {
+        v[0] += o[1] - elja;
+        v[1] += o[2] - ecrfa;
+        v[11] += o[3] + o[4];
+    }
+}
+
+// perturbed_excluded_correction for one excluded pair i < j (perturbed.rs:445-510)
+__device__ __forceinline__ void pert_excl_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
+                                                double (&v)[ROW], double& fx, double& fy, double& fz) {
+    const Lambda& lp = J.lp;
+    PertPair q = pert_params(p, J, i, j);
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    double cut2 = J.crf.cutoff_sq;
+    double a_cut2soft = cut2 + q.acrf * lp.b_crfs_lambda2;
+    double b_cut2soft = cut2 + q.acrf * lp.a_crfs_lambda2;
+    double ra = cut2 / a_cut2soft, rb = cut2 / b_cut2soft;
+    double a_crf_2cut3i = J.crf.crf_2cut3i * sqrt(ra * (ra * ra));
+    double b_crf_2cut3i = J.crf.crf_2cut3i * sqrt(rb * (rb * rb));
+    double a_crf_cut3i = 2.0 * a_crf_2cut3i, b_crf_cut3i = 2.0 * b_crf_2cut3i;
+    double a_crf_pert = 3.0 * a_crf_2cut3i / a_cut2soft, b_crf_pert = 3.0 * b_crf_2cut3i / b_cut2soft;
+    double a_e_crf = q.a_q * (-a_crf_2cut3i * r2 - J.crf.crf_cut);
+    double b_e_crf = q.b_q * (-b_crf_2cut3i * r2 - J.crf.crf_cut);
+    double e_crf_p = (lp.a_crf_lambda_n * a_e_crf + lp.b_crf_lambda_n * b_e_crf) * J.fpepsi;
+    double f_p = (lp.a_crf_lambda_n * q.a_q * a_crf_cut3i + lp.b_crf_lambda_n * q.b_q * b_crf_cut3i) * J.fpepsi;
+    double e_crf_a = q.a_q * J.fpepsi * (-J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
+    double f_a = q.a_q * J.fpepsi * J.crf.crf_cut3i;
+    double n = (double)lp.lambda_exp;
+    double dcrf = ((lp.a_crf_lambda_n * q.a_q * lp.b_crfs_lambda * a_crf_pert -
+                    lp.b_crf_lambda_n * q.b_q * lp.a_crfs_lambda * b_crf_pert) * r2 * q.acrf +
+                   n * (lp.b_crf_lambda_n_1 * b_e_crf - lp.a_crf_lambda_n_1 * a_e_crf)) * J.fpepsi;
+    own_side(first, r, f_p - f_a, fx, fy, fz);
+    if (first) {
+        v[1] += e_crf_p - e_crf_a;
+        v[11] += dcrf;
+    }
+}
+
+// perturbed_one_four_correction for one 1-4 pair i < j (perturbed.rs:531-604)
+__device__ __forceinline__ void pert_14_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
+                                              double (&v)[ROW], double& fx, double& fy, double& fz) {
+    PertPair q = pert_params(p, J, i, j);
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    if (r2 < 1e-10) return;
+    double inv_r2 = 1.0 / r2;
+    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    double inv_r = sqrt(inv_r2);
+    double e_lj_a = (q.la[3] * inv_r6 - q.la[2]) * inv_r6;
+    double f_lj_a = (12.0 * q.la[3] * inv_r6 - 6.0 * q.la[2]) * inv_r6 * inv_r2;
+    double e_crf_a = q.a_q * J.fpepsi * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
+    double f_crf_a = q.a_q * J.fpepsi * (inv_r * inv_r2 + J.crf.crf_cut3i);
+    double o[5];
+    pert_lj_crf64(r.x, r.y, r.z, q.la[2], q.la[3], q.lb[2], q.lb[3], q.a_q, q.b_q, q.alj, q.acrf, J.lp, J.crf,
+                  J.fpepsi, o);
+    own_side(first, r, o[0] - (f_lj_a + f_crf_a), fx, fy, fz);
+    if (first) {
+        v[0] += o[1] - e_lj_a;
+        v[1] += o[2] - e_crf_a;
+        v[11] += o[3] + o[4];
+    }
+}
+
+// perturbed_atom_pair_correction for one PERTATOMPAIR entry (perturbed.rs:624-707);
+// only the LJ energy delta is added (forcefield.rs:923-928)
+__device__ __forceinline__ void pert_atom_pair_delta(const SysPtrs& p, const JobDev& J, const int32_t* ap, bool first,
+                                                     double (&v)[ROW], double& fx, double& fy, double& fz) {
+    int32_t i = ap[0], j = ap[1], at = ap[2], bt = ap[3];
+    bool in14 = false;
+    for (int32_t kk = p.o14_off[i]; kk < p.o14_off[i + 1]; ++kk) in14 |= p.o14[kk] == j;
+    for (int32_t kk = p.o14_off[j]; kk < p.o14_off[j + 1]; ++kk) in14 |= p.o14[kk] == i;
+    if (!in14) return;
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    if (r2 < 1e-10) return;
+    const Lambda& lp = J.lp;
+    double inv_r2 = 1.0 / r2;
+    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    double inv_r = sqrt(inv_r2);
+    const double* lj = p.lj + J.lj_off + 4 * ((int64_t)p.iac[i] * J.n_types + p.iac[j]);
+    double e_lj_reg = (lj[3] * inv_r6 - lj[2]) * inv_r6;
+    double f_lj_reg = (12.0 * lj[3] * inv_r6 - 6.0 * lj[2]) * inv_r6 * inv_r2;
+    double qp = p.q[i] * p.q[j] * J.fpepsi;
+    double e_crf_reg = qp * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
+    double f_crf_reg = qp * (inv_r * inv_r2 + J.crf.crf_cut3i);
+    auto ljp = [&](int32_t ty, double& e, double& f) {
+        double c6 = ty == 0 ? lj[0] : lj[2], c12 = ty == 0 ? lj[1] : lj[3];
+        e = (c12 * inv_r6 - c6) * inv_r6;
+        f = (12.0 * c12 * inv_r6 - 6.0 * c6) * inv_r6 * inv_r2;
+    };
+    double e_lj_a, f_lj_a, e_lj_b = 0.0, f_lj_b = 0.0, e_crf_b = 0.0, f_crf_b = 0.0;
+    ljp(at, e_lj_a, f_lj_a);
+    if (bt >= 0) {
+        ljp(bt, e_lj_b, f_lj_b);
+        e_crf_b = e_crf_reg;
+        f_crf_b = f_crf_reg;
+    }
+    double n = (double)lp.lambda_exp;
+    double e_lj_p = lp.a_lj_lambda_n * e_lj_a + lp.b_lj_lambda_n * e_lj_b;
+    double f_lj_p = lp.a_lj_lambda_n * f_lj_a + lp.b_lj_lambda_n * f_lj_b;
+    double f_crf_p = lp.a_crf_lambda_n * f_crf_reg + lp.b_crf_lambda_n * f_crf_b;
+    double dlj = n * (lp.b_lj_lambda_n_1 * e_lj_b - lp.a_lj_lambda_n_1 * e_lj_a);
+    double dcrf = n * (lp.b_crf_lambda_n_1 * e_crf_b - lp.a_crf_lambda_n_1 * e_crf_reg);
+    own_side(first, r, (f_lj_p - f_lj_reg) + (f_crf_p - f_crf_reg), fx, fy, fz);
+    if (first) {
+        v[0] += e_lj_p - e_lj_reg;
+        v[11] += dlj + dcrf;
+    }
+}
+
 // One pair (owner atom a, partner atom b) in the reference's orientation.
 // cg path: shift from the two groups' first atoms (process_pairs_cg_grouped,
 // innerloops.rs:117-135); otherwise per-atom nearest image (innerloops.rs:180-186).
@@ -566,15 +734,29 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
         d3 pk0 = xyz(p.xq[sk]);
         int32_t ta = p.s_type[a];
         bool cgp = J.use_cg_grouped != 0;
+        const bool jp = !LONG && J.has_pert;   // perturbation deltas ride on the short pass
         int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
         for (int32_t e = e0; e < e1; ++e) {
             uint32_t ent = p.gen_ent[e];
             bool lng = (ent & ENT_LONG) != 0;
-            if (lng != LONG) continue;
             int32_t m = (int32_t)(ent & ENT_IDX);
             bool canon = (ent & ENT_CANON) != 0;
-            bool ss = (ent & ENT_SS) != 0;
+            // the pair's first atom lies in the lower group; only its group can
+            // hold the perturbed first atom (perturbed.rs:327-332)
+            bool pchk = jp && p.cg_pert[p.s_cg_orig[canon ? k : m]];
             int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+            if (lng != LONG) {
+                // long pairs are cached from the last rebuild; their correction is
+                // taken every step at the current positions (forcefield.rs:796-830)
+                if (!pchk) continue;
+                for (int32_t b = 0; b < nm; ++b) {
+                    int32_t ob = p.s2o[sm + b];
+                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                }
+                continue;
+            }
+            bool ss = (ent & ENT_SS) != 0;
             d3 pm0 = xyz(p.xq[sm]);
             for (int32_t b = 0; b < nm; ++b) {
                 if (!LONG && ss) {
@@ -584,11 +766,17 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
                 double4 xb = p.xq[sm + b];
                 gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
                          p.s_type[sm + b], xa.w, xb.w, LONG || virial);
+                if (pchk) {
+                    int32_t ob = p.s2o[sm + b];
+                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                }
             }
         }
         // intra-group pairs of solute groups (pairlist.rs:238-249), always short
         if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
             int32_t ia = a - sk;
+            bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
             for (int32_t b = 0; b < nk; ++b) {
                 if (b == ia) continue;
                 int32_t ob = p.s2o[sk + b];
@@ -597,6 +785,10 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
                 double4 xb = p.xq[sk + b];
                 gen_pair(p, J, v, fx, fy, fz, canon, cgp, pa, xyz(xb), pk0, pk0, ta, p.s_type[sk + b],
                          xa.w, xb.w, virial);
+                if (pown) {
+                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                }
             }
         }
     }
@@ -639,15 +831,25 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
         d3 pk0 = xyz(p.xq[sk]);
         int32_t ta = p.s_type[a];
         bool cgp = J.use_cg_grouped != 0 && !LONG;
+        const bool jp = !LONG && J.has_pert;
         int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
         for (int32_t e = e0; e < e1; ++e) {
             uint32_t ent = p.gen_ent[e];
             bool lng = (ent & ENT_LONG) != 0;
-            if (lng != LONG) continue;
             int32_t m = (int32_t)(ent & ENT_IDX);
             float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
-            bool ss = (ent & ENT_SS) != 0;
+            bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
             int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+            if (lng != LONG) {
+                if (!pchk) continue;
+                for (int32_t b = 0; b < nm; ++b) {
+                    int32_t ob = p.s2o[sm + b];
+                    int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+                }
+                continue;
+            }
+            bool ss = (ent & ENT_SS) != 0;
             float sx = 0.f, sy = 0.f, sz = 0.f;
             if (cgp) shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
             for (int32_t b = 0; b < nm; ++b) {
@@ -656,6 +858,18 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
                     if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
                 }
                 double4 xb = p.xq[sm + b];
+                if (pchk) {
+                    // a perturbed first atom: the state-A pair in FP64 (so that the
+                    // FP64 delta below cancels it exactly), then the delta
+                    int32_t ob = p.s2o[sm + b];
+                    int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                    if (p.pflag[fi]) {
+                        gen_pair(p, J, v, fx, fy, fz, canon != 0.f, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
+                                 p.s_type[sm + b], xa.w, xb.w, VIR);
+                        pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+                        continue;
+                    }
+                }
                 float rx, ry, rz;
                 if (cgp) {
                     float4 db = p.xf[sm + b];
@@ -682,12 +896,22 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
         }
         if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
             int32_t ia = a - sk;
+            bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
             for (int32_t b = 0; b < nk; ++b) {
                 if (b == ia) continue;
                 int32_t ob = p.s2o[sk + b];
                 bool canon = ia < b;
                 if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
                 double4 xb = p.xq[sk + b];
+                if (pown) {
+                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                    if (p.pflag[fi]) {
+                        gen_pair(p, J, v, fx, fy, fz, canon, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
+                                 p.s_type[sk + b], xa.w, xb.w, VIR);
+                        pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                        continue;
+                    }
+                }
                 float rx = (float)(xa.x - xb.x), ry = (float)(xa.y - xb.y), rz = (float)(xa.z - xb.z);
                 const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sk + b]);
                 float fm, elj, ecrf;
@@ -733,7 +957,19 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
         d3 pi = ld3(x, i);
         double qi = p.q[i];
         bool sol_i = i - J.atom_off < J.n_solute_atoms;
+        const bool jp = J.has_pert != 0;
+        bool pi_ = jp && p.pflag[i];
         if (sol_i) v[1] += -0.5 * qi * qi * J.fpepsi * J.crf.crf_cut;
+        if (pi_) {
+            // perturbed_self_energy_correction (perturbed.rs:395-425)
+            const Lambda& lp = J.lp;
+            double qa2 = p.pa_q[i] * p.pa_q[i], qb2 = p.pb_q[i] * p.pb_q[i];
+            double e_rf = (-(lp.a_crf_lambda_n * qa2) - lp.b_crf_lambda_n * qb2 + qa2) * J.crf.crf_cut * J.fpepsi;
+            double de_rf = (double)lp.lambda_exp * (lp.a_crf_lambda_n_1 * qa2 - lp.b_crf_lambda_n_1 * qb2) *
+                           J.crf.crf_cut * J.fpepsi;
+            v[1] += 0.5 * e_rf;
+            v[11] += 0.5 * de_rf;
+        }
         for (int32_t k = p.excl_off[i]; k < p.excl_off[i + 1]; ++k) {
             int32_t j = p.excl[k];
             d3 pj = ld3(x, j);
@@ -748,6 +984,7 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
                     double gx = r.x * fs, gy = r.y * fs, gz = r.z * fs;
                     fx += gx; fy += gy; fz += gz;
                     add_virial(v, r, gx, gy, gz);
+                    if (jp && (pi_ || p.pflag[j])) pert_excl_delta(p, J, i, j, true, v, fx, fy, fz);
                 } else {
                     v[1] += -qi * qj * J.fpepsi * J.crf.crf_2cut3i * r2;
                 }
@@ -755,6 +992,7 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
                 d3 r = nim(J.box, pj, pi);
                 double fs = qj * qi * J.fpepsi * J.crf.crf_cut3i;
                 fx -= r.x * fs; fy -= r.y * fs; fz -= r.z * fs;
+                if (jp && (pi_ || p.pflag[j])) pert_excl_delta(p, J, j, i, false, v, fx, fy, fz);
             }
         }
         for (int32_t k = p.o14s_off[i]; k < p.o14s_off[i + 1]; ++k) {
@@ -785,23 +1023,27 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
             } else {
                 fx -= gx; fy -= gy; fz -= gz;
             }
+            if (jp && (pi_ || p.pflag[j])) pert_14_delta(p, J, lo, hi, canon, v, fx, fy, fz);
+        }
+        if (jp) {
+            // PERTATOMPAIR entries of this atom (bit 31: this atom is the entry's i)
+            for (int32_t k = p.pap_off[i]; k < p.pap_off[i + 1]; ++k) {
+                uint32_t ent = (uint32_t)p.pap[k];
+                const int32_t* ap = p.papairs + 4 * (int64_t)(ent & 0x7FFFFFFFu);
+                pert_atom_pair_delta(p, J, ap, (ent & 0x80000000u) != 0, v, fx, fy, fz);
+            }
         }
     }
     if (i < npad) {
         p.f_x[3 * i] = fx;
         p.f_x[3 * i + 1] = fy;
         p.f_x[3 * i + 2] = fz;
-        if (p.n_plist) {   // cleared here, accumulated by the perturbation kernels
-            p.f_p[3 * i] = 0.0;
-            p.f_p[3 * i + 1] = 0.0;
-            p.f_p[3 * i + 2] = 0.0;
-        }
     }
     write_row(v, p.rows, SLOT_EXCL, p.nrow_blocks);
 }
 
 // total force on each original atom
-__global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed) {
+__global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t mixed) {
     int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= npad) return;
     int32_t o = p.s2o[a];
@@ -818,235 +1060,7 @@ __global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t w
             if (with_long) f += p.f_long_s[3 * a + c];
         }
         f += p.f_x[3 * o + c];
-        if (with_pert) f += p.f_p[3 * o + c];
         p.f[3 * o + c] = f;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// perturbation (free energy): the five correction passes of forcefield.rs:764-951
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void atomic_add3d(double* f, int64_t i, double x, double y, double z) {
-    atomicAdd(f + 3 * i, x);
-    atomicAdd(f + 3 * i + 1, y);
-    atomicAdd(f + 3 * i + 2, z);
-}
-
-// one perturbed pair (i first), perturbed_pairlist_correction body (perturbed.rs:327-384)
-__device__ __forceinline__ void pert_pair(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, double (&v)[3]) {
-    bool pj = p.pflag[j] != 0;
-    int32_t a_iac_i = p.pa_iac[i], b_iac_i = p.pb_iac[i];
-    int32_t a_iac_j = pj ? p.pa_iac[j] : p.iac[j], b_iac_j = pj ? p.pb_iac[j] : p.iac[j];
-    double a_q = p.pa_q[i] * (pj ? p.pa_q[j] : p.q[j]);
-    double b_q = p.pb_q[i] * (pj ? p.pb_q[j] : p.q[j]);
-    double alj, acrf;
-    if (pj) {
-        alj = (p.palj[i] + p.palj[j]) * 0.5;
-        acrf = (p.pacrf[i] + p.pacrf[j]) * 0.5;
-    } else {
-        alj = p.palj[i];
-        acrf = p.pacrf[i];
-    }
-    const double* la = p.lj + J.lj_off + 4 * ((int64_t)a_iac_i * J.n_types + a_iac_j);
-    const double* lb = p.lj + J.lj_off + 4 * ((int64_t)b_iac_i * J.n_types + b_iac_j);
-    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
-    double fa, elja, ecrfa, o[5];
-    lj_crf64(r.x, r.y, r.z, la[0], la[1], a_q * J.fpepsi, J.crf, fa, elja, ecrfa);
-    pert_lj_crf64(r.x, r.y, r.z, la[0], la[1], lb[0], lb[1], a_q, b_q, alj, acrf, J.lp, J.crf, J.fpepsi, o);
-    double d = o[0] - fa;
-    double gx = r.x * d, gy = r.y * d, gz = r.z * d;
-    atomic_add3d(p.f_p, i, gx, gy, gz);
-    atomic_add3d(p.f_p, j, -gx, -gy, -gz);
-    v[0] += o[1] - elja;
-    v[1] += o[2] - ecrfa;
-    v[2] += o[3] + o[4];
-}
-
-// Pair-list correction: one block per perturbed atom p.  p corrects every
-// listed pair in which it is the first atom (lower charge group, or lower
-// index inside its own group), short and long lists, every step; pairs whose
-// first atom is not perturbed keep state A (perturbed.rs:327-332).
-__global__ void __launch_bounds__(BLOCK) k_pert_pairs(SysPtrs p) {
-    __shared__ double lds[17 * 3];
-    double v[3] = {0.0, 0.0, 0.0};
-    int32_t ip = p.plist[blockIdx.x];
-    const JobDev& J = p.jobs[p.atom_job[ip]];
-    int32_t a = p.o2s[ip];
-    int32_t k = p.s_atom_cg[a];
-    int32_t sk = p.s_cg_start[k], nk = p.s_cg_n[k];
-    int32_t e0 = (int32_t)p.gen_off[k], ne = (int32_t)p.gen_cnt[k];
-    for (int32_t t = threadIdx.x; t < ne + nk; t += blockDim.x) {
-        if (t < ne) {
-            uint32_t ent = p.gen_ent[e0 + t];
-            if (!(ent & ENT_CANON)) continue;
-            int32_t m = (int32_t)(ent & ENT_IDX);
-            bool excl_check = (ent & ENT_SS) && !(ent & ENT_LONG);
-            int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
-            for (int32_t b = 0; b < nm; ++b) {
-                int32_t j = p.s2o[sm + b];
-                if (excl_check && excluded_or_14(p, ip, j)) continue;
-                pert_pair(p, J, ip, j, v);
-            }
-        } else {
-            int32_t j = p.s2o[sk + (t - ne)];
-            if (j <= ip) continue;
-            if (excluded_or_14(p, ip, j)) continue;
-            pert_pair(p, J, ip, j, v);
-        }
-    }
-    block_sum<3>(v, lds);
-    if (threadIdx.x < 3) atomicAdd(p.pert_e + 3 * p.atom_job[ip] + threadIdx.x, lds[16 * 3 + threadIdx.x]);
-}
-
-// Self (perturbed.rs:395-425), excluded (perturbed.rs:431-511) and 1-4
-// (perturbed.rs:517-605) corrections, one lane per perturbed atom; lanes past
-// the perturbed atoms take the PERTATOMPAIR entries (perturbed.rs:624-707).
-__global__ void __launch_bounds__(BLOCK) k_pert_atoms(SysPtrs p) {
-    int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    double de_lj = 0.0, de_crf = 0.0, dh = 0.0;
-    int32_t jb = -1;
-    if (t < p.n_plist) {
-        int32_t ip = p.plist[t];
-        jb = p.atom_job[ip];
-        const JobDev& J = p.jobs[jb];
-        const Lambda& lp = J.lp;
-        double n = (double)lp.lambda_exp;
-        double qa2 = p.pa_q[ip] * p.pa_q[ip], qb2 = p.pb_q[ip] * p.pb_q[ip];
-        double e_rf = (-(lp.a_crf_lambda_n * qa2) - lp.b_crf_lambda_n * qb2 + qa2) * J.crf.crf_cut * J.fpepsi;
-        de_crf += 0.5 * e_rf;
-        double de_rf = n * (lp.a_crf_lambda_n_1 * qa2 - lp.b_crf_lambda_n_1 * qb2) * J.crf.crf_cut * J.fpepsi;
-        dh += 0.5 * de_rf;
-        // excluded pairs: (ip, j > ip) always; (j < ip, ip) when j is a solute atom
-        // that is not perturbed itself (its own lane corrects it otherwise)
-        for (int32_t kk = p.excl_off[ip]; kk < p.excl_off[ip + 1]; ++kk) {
-            int32_t j = p.excl[kk];
-            int32_t i0, j0;
-            if (j > ip) { i0 = ip; j0 = j; }
-            else {
-                if (p.pflag[j] || j - J.atom_off >= J.n_solute_atoms) continue;
-                i0 = j; j0 = ip;
-            }
-            bool pi = p.pflag[i0] != 0, pj = p.pflag[j0] != 0;
-            double a_q = (pi ? p.pa_q[i0] : p.q[i0]) * (pj ? p.pa_q[j0] : p.q[j0]);
-            double b_q = (pi ? p.pb_q[i0] : p.q[i0]) * (pj ? p.pb_q[j0] : p.q[j0]);
-            double acrf = (pi && pj) ? (p.pacrf[i0] + p.pacrf[j0]) * 0.5 : (pi ? p.pacrf[i0] : p.pacrf[j0]);
-            d3 r = nim(J.box, ld3(p.x_cur, i0), ld3(p.x_cur, j0));
-            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-            double cut2 = J.crf.cutoff_sq;
-            double a_cut2soft = cut2 + acrf * lp.b_crfs_lambda2;
-            double b_cut2soft = cut2 + acrf * lp.a_crfs_lambda2;
-            double ra = cut2 / a_cut2soft, rb = cut2 / b_cut2soft;
-            double a_crf_2cut3i = J.crf.crf_2cut3i * sqrt(ra * (ra * ra));
-            double b_crf_2cut3i = J.crf.crf_2cut3i * sqrt(rb * (rb * rb));
-            double a_crf_cut3i = 2.0 * a_crf_2cut3i, b_crf_cut3i = 2.0 * b_crf_2cut3i;
-            double a_crf_pert = 3.0 * a_crf_2cut3i / a_cut2soft, b_crf_pert = 3.0 * b_crf_2cut3i / b_cut2soft;
-            double a_e_crf = a_q * (-a_crf_2cut3i * r2 - J.crf.crf_cut);
-            double b_e_crf = b_q * (-b_crf_2cut3i * r2 - J.crf.crf_cut);
-            double e_crf_p = (lp.a_crf_lambda_n * a_e_crf + lp.b_crf_lambda_n * b_e_crf) * J.fpepsi;
-            double f_p = (lp.a_crf_lambda_n * a_q * a_crf_cut3i + lp.b_crf_lambda_n * b_q * b_crf_cut3i) * J.fpepsi;
-            double e_crf_a = a_q * J.fpepsi * (-J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-            double f_a = a_q * J.fpepsi * J.crf.crf_cut3i;
-            double dcrf = ((lp.a_crf_lambda_n * a_q * lp.b_crfs_lambda * a_crf_pert -
-                            lp.b_crf_lambda_n * b_q * lp.a_crfs_lambda * b_crf_pert) * r2 * acrf +
-                           n * (lp.b_crf_lambda_n_1 * b_e_crf - lp.a_crf_lambda_n_1 * a_e_crf)) * J.fpepsi;
-            double d = f_p - f_a;
-            atomic_add3d(p.f_p, i0, r.x * d, r.y * d, r.z * d);
-            atomic_add3d(p.f_p, j0, -r.x * d, -r.y * d, -r.z * d);
-            de_crf += e_crf_p - e_crf_a;
-            dh += dcrf;
-        }
-        // 1-4 pairs (symmetric list, bit 31 = this atom is the lower one)
-        for (int32_t kk = p.o14s_off[ip]; kk < p.o14s_off[ip + 1]; ++kk) {
-            uint32_t ent = (uint32_t)p.o14s[kk];
-            int32_t j = (int32_t)(ent & 0x7FFFFFFFu);
-            bool canon = (ent & 0x80000000u) != 0;
-            if (!canon && p.pflag[j]) continue;
-            int32_t i0 = canon ? ip : j, j0 = canon ? j : ip;
-            bool pi = p.pflag[i0] != 0, pj = p.pflag[j0] != 0;
-            int32_t a_iac_i = pi ? p.pa_iac[i0] : p.iac[i0], b_iac_i = pi ? p.pb_iac[i0] : p.iac[i0];
-            int32_t a_iac_j = pj ? p.pa_iac[j0] : p.iac[j0], b_iac_j = pj ? p.pb_iac[j0] : p.iac[j0];
-            double a_q = (pi ? p.pa_q[i0] : p.q[i0]) * (pj ? p.pa_q[j0] : p.q[j0]);
-            double b_q = (pi ? p.pb_q[i0] : p.q[i0]) * (pj ? p.pb_q[j0] : p.q[j0]);
-            double alj, acrf;
-            if (pi && pj) {
-                alj = (p.palj[i0] + p.palj[j0]) * 0.5;
-                acrf = (p.pacrf[i0] + p.pacrf[j0]) * 0.5;
-            } else {
-                int32_t w = pi ? i0 : j0;
-                alj = p.palj[w];
-                acrf = p.pacrf[w];
-            }
-            const double* la = p.lj + J.lj_off + 4 * ((int64_t)a_iac_i * J.n_types + a_iac_j);
-            const double* lb = p.lj + J.lj_off + 4 * ((int64_t)b_iac_i * J.n_types + b_iac_j);
-            d3 r = nim(J.box, ld3(p.x_cur, i0), ld3(p.x_cur, j0));
-            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-            if (r2 < 1e-10) continue;
-            double inv_r2 = 1.0 / r2;
-            double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-            double inv_r = sqrt(inv_r2);
-            double e_lj_a = (la[3] * inv_r6 - la[2]) * inv_r6;
-            double f_lj_a = (12.0 * la[3] * inv_r6 - 6.0 * la[2]) * inv_r6 * inv_r2;
-            double e_crf_a = a_q * J.fpepsi * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-            double f_crf_a = a_q * J.fpepsi * (inv_r * inv_r2 + J.crf.crf_cut3i);
-            double o[5];
-            pert_lj_crf64(r.x, r.y, r.z, la[2], la[3], lb[2], lb[3], a_q, b_q, alj, acrf, lp, J.crf, J.fpepsi, o);
-            double d = o[0] - (f_lj_a + f_crf_a);
-            atomic_add3d(p.f_p, i0, r.x * d, r.y * d, r.z * d);
-            atomic_add3d(p.f_p, j0, -r.x * d, -r.y * d, -r.z * d);
-            de_lj += o[1] - e_lj_a;
-            de_crf += o[2] - e_crf_a;
-            dh += o[3] + o[4];
-        }
-    } else if (t < p.n_plist + p.n_papairs) {
-        const int32_t* ap = p.papairs + 4 * (t - p.n_plist);
-        int32_t i = ap[0], j = ap[1], at = ap[2], bt = ap[3];
-        jb = p.atom_job[i];
-        const JobDev& J = p.jobs[jb];
-        const Lambda& lp = J.lp;
-        bool in14 = false;
-        for (int32_t kk = p.o14_off[i]; kk < p.o14_off[i + 1]; ++kk) in14 |= p.o14[kk] == j;
-        for (int32_t kk = p.o14_off[j]; kk < p.o14_off[j + 1]; ++kk) in14 |= p.o14[kk] == i;
-        d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
-        double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-        if (in14 && !(r2 < 1e-10)) {
-            double inv_r2 = 1.0 / r2;
-            double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-            double inv_r = sqrt(inv_r2);
-            const double* lj = p.lj + J.lj_off + 4 * ((int64_t)p.iac[i] * J.n_types + p.iac[j]);
-            double e_lj_reg = (lj[3] * inv_r6 - lj[2]) * inv_r6;
-            double f_lj_reg = (12.0 * lj[3] * inv_r6 - 6.0 * lj[2]) * inv_r6 * inv_r2;
-            double qp = p.q[i] * p.q[j] * J.fpepsi;
-            double e_crf_reg = qp * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-            double f_crf_reg = qp * (inv_r * inv_r2 + J.crf.crf_cut3i);
-            auto ljp = [&](int32_t ty, double& e, double& f) {
-                double c6 = ty == 0 ? lj[0] : lj[2], c12 = ty == 0 ? lj[1] : lj[3];
-                e = (c12 * inv_r6 - c6) * inv_r6;
-                f = (12.0 * c12 * inv_r6 - 6.0 * c6) * inv_r6 * inv_r2;
-            };
-            double e_lj_a, f_lj_a, e_lj_b = 0.0, f_lj_b = 0.0, e_crf_b = 0.0, f_crf_b = 0.0;
-            ljp(at, e_lj_a, f_lj_a);
-            if (bt >= 0) {
-                ljp(bt, e_lj_b, f_lj_b);
-                e_crf_b = e_crf_reg;
-                f_crf_b = f_crf_reg;
-            }
-            double n = (double)lp.lambda_exp;
-            double e_lj_p = lp.a_lj_lambda_n * e_lj_a + lp.b_lj_lambda_n * e_lj_b;
-            double f_lj_p = lp.a_lj_lambda_n * f_lj_a + lp.b_lj_lambda_n * f_lj_b;
-            double f_crf_p = lp.a_crf_lambda_n * f_crf_reg + lp.b_crf_lambda_n * f_crf_b;
-            double dlj = n * (lp.b_lj_lambda_n_1 * e_lj_b - lp.a_lj_lambda_n_1 * e_lj_a);
-            double dcrf = n * (lp.b_crf_lambda_n_1 * e_crf_b - lp.a_crf_lambda_n_1 * e_crf_reg);
-            double d = (f_lj_p - f_lj_reg) + (f_crf_p - f_crf_reg);
-            atomic_add3d(p.f_p, i, r.x * d, r.y * d, r.z * d);
-            atomic_add3d(p.f_p, j, -r.x * d, -r.y * d, -r.z * d);
-            de_lj += e_lj_p - e_lj_reg;   // forcefield.rs:923-928 adds only the LJ delta
-            dh += dlj + dcrf;
-        }
-    }
-    if (jb >= 0) {
-        atomicAdd(p.pert_e + 3 * jb, de_lj);
-        atomicAdd(p.pert_e + 3 * jb + 1, de_crf);
-        atomicAdd(p.pert_e + 3 * jb + 2, dh);
     }
 }
 
@@ -1182,14 +1196,16 @@ __global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
 }
 
 // One block per job: sums the job's partial rows in a fixed order and writes
-// the step record (e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]).
+// the step record (e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]).  Column 11 of a
+// row is E_kin in the kinetic slot and dH/dlambda in the others.
 __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32_t with_long,
                                                   int32_t with_kin, int32_t step, int32_t njobs) {
-    __shared__ double lds[17 * ROW];
+    constexpr int W = ROW + 1;
+    __shared__ double lds[17 * W];
     int32_t jb = blockIdx.x;
     const JobDev& J = p.jobs[jb];
     const int32_t stride = p.nrow_blocks;
-    double v[ROW] = {0};
+    double v[W] = {0};
     for (int s = 0; s < NSLOT; ++s) {
         if (s == SLOT_LONG && !with_long) continue;
         if (s == SLOT_KIN && !with_kin) continue;
@@ -1199,29 +1215,25 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         int32_t b1 = (J.atom_off + J.n_atoms + per - 1) / per;
         for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
             const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
+            if (s == SLOT_KIN) {
+                v[11] += r[11];
+            } else {
 #pragma unroll
-            for (int c = 0; c < ROW; ++c) v[c] += r[c];
+                for (int c = 0; c < 11; ++c) v[c] += r[c];
+                v[12] += r[11];
+            }
         }
     }
-    block_sum<ROW>(v, lds);
+    block_sum<W>(v, lds);
     if (threadIdx.x == 0) {
         double* rec = p.records + ((int64_t)step * njobs + jb) * REC;
-        double elj = lds[16 * ROW + 0], ecrf = lds[16 * ROW + 1];
-        double dh = 0.0;
-        if (J.has_pert) {
-            elj += p.pert_e[3 * jb];
-            ecrf += p.pert_e[3 * jb + 1];
-            dh = p.pert_e[3 * jb + 2];
-            p.pert_e[3 * jb] = 0.0;
-            p.pert_e[3 * jb + 1] = 0.0;
-            p.pert_e[3 * jb + 2] = 0.0;
-        }
-        rec[0] = elj;
-        rec[1] = ecrf;
-        rec[2] = elj + ecrf;
-        rec[3] = lds[16 * ROW + 11];
-        rec[4] = dh;
-        for (int c = 0; c < 9; ++c) rec[5 + c] = lds[16 * ROW + 2 + c];
+        const double* t = lds + 16 * W;
+        rec[0] = t[0];
+        rec[1] = t[1];
+        rec[2] = t[0] + t[1];
+        rec[3] = t[11];
+        rec[4] = t[12];
+        for (int c = 0; c < 9; ++c) rec[5 + c] = t[2 + c];
     }
 }
 
@@ -1336,20 +1348,15 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
     hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
 }
-void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed,
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, with_pert, mixed);
+    hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, mixed);
 }
 void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s) {
     hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad, step);
 }
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s) {
     if (nmol > 0) hipLaunchKernelGGL(k_settle, grid_for(nmol), dim3(BLOCK), 0, s, p, nmol, step);
-}
-void launch_pert(const SysPtrs& p, hipStream_t s) {
-    if (p.n_plist <= 0) return;
-    hipLaunchKernelGGL(k_pert_pairs, dim3(p.n_plist), dim3(BLOCK), 0, s, p);
-    hipLaunchKernelGGL(k_pert_atoms, grid_for((int64_t)p.n_plist + p.n_papairs), dim3(BLOCK), 0, s, p);
 }
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s) {
     hipLaunchKernelGGL(k_ekin, grid_for(npad), dim3(BLOCK), 0, s, p, npad);

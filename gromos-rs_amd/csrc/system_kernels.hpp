@@ -66,10 +66,8 @@ struct SysPtrs {
     double* f_s;
     double* f_long_s;
     double* f_x;
-    double* f_p;
     double* rows;
     double* records;
-    double* pert_e;            // 3 per job: delta e_lj, delta e_crf, dhdl
     int32_t* settle_fail;      // per job: first failing step
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
@@ -79,10 +77,10 @@ struct SysPtrs {
     const double* pb_q;
     const double* palj;
     const double* pacrf;
-    const int32_t* plist;      // perturbed atoms (original index)
-    int32_t n_plist;
+    const uint8_t* cg_pert;    // per original charge group: holds a perturbed atom
     const int32_t* papairs;    // PERTATOMPAIR: i, j, a_type, b_type (original indices)
-    int32_t n_papairs;
+    const int32_t* pap_off;    // npad + 1: entries of each atom
+    const int32_t* pap;        // entry index | bit 31 when the atom is the entry's i
     int32_t* o2s;              // original atom -> sorted atom
 };
 
@@ -97,10 +95,9 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
-void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_pert, int32_t mixed, hipStream_t s);
+void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s);
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
-void launch_pert(const SysPtrs& p, hipStream_t s);
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
                    int32_t njobs, hipStream_t s);

@@ -2,11 +2,10 @@
 
 A job's results must not depend on the batch it runs in: each job, admitted at
 some clock time into a batch whose other members come and go, is compared with
-the same job run alone on a single-job system.  Jobs without perturbed atoms
-are bit-identical (per-job fixed-order reductions, jobs on separate 256-atom
-aligned ranges, pair lists rebuilt on the job's own step multiples of NSNB).
-The free-energy correction passes add with FP64 atomics, whose order varies,
-so jobs with perturbed atoms agree to 1e-11 relative over the short runs here.
+the same job run alone on a single-job system, and must be bit-identical
+(per-job fixed-order reductions, no atomics in the force or energy paths, jobs
+on separate 256-atom aligned ranges, pair lists rebuilt on the job's own step
+multiples of NSNB).
 """
 import importlib
 
@@ -18,9 +17,6 @@ rt = gamd.runtime
 gen = importlib.import_module("gromos-rs_amd.generators")
 
 pytestmark = pytest.mark.gpu
-
-PERT_REL = 1e-11
-
 
 def alone(s, nsteps, prec):
     ds = rt.DeviceSystem(s, precision=prec)
@@ -40,12 +36,6 @@ def job_specs():
         (gen.fep_job(2000, 6, 0.4, seed=3), 26, 12),
         (gen.spc_water_box(300, seed=9), 9, 12),
     ]
-
-
-def close(a, b, exact):
-    if exact:
-        return np.array_equal(a, b)
-    return np.all(np.abs(a - b) <= PERT_REL * np.maximum(np.abs(b), 1.0))
 
 
 @pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
@@ -74,10 +64,9 @@ def test_session_jobs_match_alone(prec):
         e = sess.energies(ids[i])
         x, v = sess.fetch(ids[i])
         e0, x0, v0 = ref[i]
-        exact = s.pert is None or not np.any(s.pert.flag)
-        assert close(e, e0, exact), (i, np.max(np.abs(e - e0)))
-        assert close(x, x0, exact), (i, np.max(np.abs(x - x0)))
-        assert close(v, v0, exact), (i, np.max(np.abs(v - v0)))
+        assert np.array_equal(e, e0), (i, np.max(np.abs(e - e0)))
+        assert np.array_equal(x, x0), (i, np.max(np.abs(x - x0)))
+        assert np.array_equal(v, v0), (i, np.max(np.abs(v - v0)))
     sess.close()
 
 

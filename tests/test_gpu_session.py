@@ -18,6 +18,7 @@ gen = importlib.import_module("gromos-rs_amd.generators")
 
 pytestmark = pytest.mark.gpu
 
+
 def alone(s, nsteps, prec):
     ds = rt.DeviceSystem(s, precision=prec)
     ds.pairlist_update()

@@ -38,6 +38,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
+    ap.add_argument("--fep-steps", type=int, default=2000, help="md steps per job of the 64-job FEP mix")
+    ap.add_argument("--fep-alone-steps", type=int, default=200,
+                    help="steps per job timed for the one-job-at-a-time plain FP64 comparison")
+    ap.add_argument("--remd-steps", type=int, default=1000, help="steps of the 128-replica exchange run")
+    ap.add_argument("--no-fep", action="store_true")
+    ap.add_argument("--no-remd", action="store_true")
     return ap.parse_args()
 
 
@@ -201,9 +207,87 @@ def main():
         out["speedup_vs_plain_fp64_gpu"] = h["pairs_per_s"] / p["pairs_per_s"]
         out["mixed_force_error_vs_fp64"] = force_error(rt, s)
     out["energy_first_last"] = [h["e_first"], h["e_last"]]
+    if not a.no_fep:
+        out["fep_mix"] = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps)
+    if not a.no_remd:
+        out["remd"] = remd_bench(rt, gen, a.remd_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s, a.cpu_steps)
     print(json.dumps(out), flush=True)
+
+
+def fep_mix_bench(rt, gen, steps, alone_steps):
+    """The 64-job FEP lambda-window mix: seeded Poisson arrivals, continuous
+    batching (mixed precision, then plain FP64), against the same jobs run one at
+    a time on the plain FP64 single-job path."""
+    import numpy as np
+    srv = importlib.import_module("gromos-rs_amd.server")
+    mix = gen.fep_mix()
+    systems = [gen.fep_job(m["n_atoms"], m["n_pert"], m["lam"], m["seed"]) for m in mix]
+    sched = [dict(key=k, system=systems[k], nsteps=steps, arrival=m["arrival"]) for k, m in enumerate(mix)]
+    out = dict(jobs=len(mix), atoms=int(sum(s.n_atoms for s in systems)), steps_per_job=steps,
+               arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5,
+               arrivals_last_step=int(max(m["arrival"] for m in mix)))
+    for name, prec in (("batched_mixed", rt.MIXED), ("batched_fp64", rt.FP64)):
+        server = srv.Server(precision=prec, nsnb=5)
+        r = srv.replay(server, sched)
+        sim_ns = len(mix) * steps * 0.002 * 1e-3
+        out[name] = dict(pairs_per_s=r["pairs"] / r["wall"], wall_s=r["wall"], ns_per_day=sim_ns * 86400.0 / r["wall"],
+                         step_ms_p50=float(np.percentile(r["step_ms"], 50)),
+                         step_ms_p99=float(np.percentile(r["step_ms"], 99)),
+                         job_latency_s_p50=float(np.percentile(r["job_latency_s"], 50)),
+                         job_latency_s_p99=float(np.percentile(r["job_latency_s"], 99)),
+                         layouts=r["layouts"], layout_ms_total=r["layout_ms"], session_steps=r["clock"])
+        server.sess.close()
+    # one at a time, plain FP64 (rate over alone_steps per job, pair-list cadence included)
+    pairs, wall = 0, 0.0
+    import torch
+    for s in systems:
+        ds = rt.DeviceSystem(s, precision=rt.FP64)
+        ds.pairlist_update()
+        p0 = ds.pair_count
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ds.md(alone_steps)
+        wall += time.perf_counter() - t0
+        pairs += ds.pair_count - p0
+        ds.close()
+    out["one_at_a_time_fp64"] = dict(pairs_per_s=pairs / wall, steps_per_job_timed=alone_steps, wall_s=wall)
+    out["speedup_batched_mixed_vs_one_at_a_time_fp64"] = out["batched_mixed"]["pairs_per_s"] / (pairs / wall)
+    out["speedup_batched_fp64_vs_one_at_a_time_fp64"] = out["batched_fp64"]["pairs_per_s"] / (pairs / wall)
+    return out
+
+
+def remd_bench(rt, gen, steps):
+    """128 replicas of 4,000 SPC waters (12,000 atoms, rc 1.2 nm), temperatures
+    280 * (450/280)^(k/127) K, NVE segments with temperature exchange every 500
+    steps (odd-even pairs, velocity rescaling), all replicas batched."""
+    import numpy as np
+    remd = importlib.import_module("gromos-rs_amd.remd")
+    n = 128
+    temps = [280.0 * (450.0 / 280.0) ** (k / (n - 1)) for k in range(n)]
+    systems = [gen.water_replica(seed=100 + k, temperature=temps[k], n_waters=4000, rc=1.2) for k in range(n)]
+    sess = rt.Session(precision=rt.MIXED, nsnb=5)
+    rx = remd.ReplicaExchange(sess, systems, temps, exchange_interval=500, scheme="odd_even", seed=7,
+                              total_steps=steps)
+    sess.step_times(enable=1)
+    i0 = sess.info()
+    t0 = time.perf_counter()
+    rx.run(steps)
+    wall = time.perf_counter() - t0
+    i1 = sess.info()
+    ms = sess.step_times(enable=0)
+    e = np.array([sess.energies(j) for j in rx.jobs])          # replicas x steps x 5
+    tot = e[:, :, 2] + e[:, :, 3]
+    seg = min(500, steps)
+    drift = (tot[:, seg - 1] - tot[:, 0]) / (seg * 0.002)      # kJ/mol/ps over the first NVE segment
+    sess.close()
+    return dict(replicas=n, atoms_per_replica=systems[0].n_atoms, box_nm=float(systems[0].box[0]), rc=1.2,
+                steps=steps, exchange_interval=500, pairs_per_s=(i1["pair_count"] - i0["pair_count"]) / wall,
+                ns_per_day=n * steps * 0.002 * 1e-3 * 86400.0 / wall, wall_s=wall,
+                step_ms_p50=float(np.percentile(ms, 50)), step_ms_p99=float(np.percentile(ms, 99)),
+                exchange_attempts=rx.attempts, exchange_accepted=rx.accepted,
+                nve_drift_kj_mol_ps_median_abs=float(np.median(np.abs(drift))), precision="mixed")
 
 
 def ds_vv_pairs(ds, s):

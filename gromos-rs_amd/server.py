@@ -1,0 +1,95 @@
+"""Continuous-batching server over the session C ABI.
+
+Clients submit jobs (topology, state, step budget) at any time; the server
+admits them against the device's HBM budget (the session reserves each job's
+footprint; a job that does not fit waits in the server's FIFO until running
+jobs finish), steps the resident batch and returns every finished job's end
+state and per-step energies.  `replay` drives it with an arrival schedule in
+md steps of the session clock, the form a serving benchmark takes.
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import runtime as rt
+
+
+class Server:
+    def __init__(self, precision=rt.MIXED, nsnb=5, admit_interval=0, hbm_budget=0):
+        self.sess = rt.Session(precision=precision, nsnb=nsnb, admit_interval=admit_interval,
+                               hbm_budget=hbm_budget)
+        self.nsnb = nsnb
+        self.waiting = collections.deque()
+        self.running = {}
+        self.results = {}
+
+    @property
+    def clock(self):
+        return self.sess.info()["clock"]
+
+    def submit(self, key, system, nsteps):
+        self.waiting.append(dict(key=key, system=system, nsteps=int(nsteps), t_submit=time.perf_counter(),
+                                 clock_submit=self.clock))
+
+    def _admit(self):
+        while self.waiting:
+            w = self.waiting[0]
+            try:
+                jid = self.sess.admit(w["system"], w["nsteps"])
+            except rt.Busy:
+                break
+            self.waiting.popleft()
+            w["job"] = jid
+            w["clock_admit"] = self.clock
+            self.running[jid] = w
+
+    def _collect(self):
+        for jid in list(self.running):
+            st, done = self.sess.query(jid)
+            if st != rt.DONE:
+                continue
+            w = self.running.pop(jid)
+            e = self.sess.energies(jid)
+            x, v = self.sess.fetch(jid)
+            self.sess.evict(jid)
+            w.pop("system")
+            w.update(t_done=time.perf_counter(), clock_done=self.clock, energies=e, pos=x, vel=v)
+            self.results[w["key"]] = w
+
+    def step(self, nsteps):
+        self._admit()
+        self.sess.step(nsteps)
+        self._collect()
+
+    def idle(self):
+        return not self.waiting and not self.running
+
+
+def replay(server, schedule, chunk=None):
+    """Run `schedule` (dicts with key, system, nsteps, arrival step) to completion.
+    Returns wall seconds, pair interactions, per-job latency and device step times."""
+    chunk = chunk or server.nsnb
+    pending = sorted(schedule, key=lambda d: d["arrival"])
+    server.sess.step_times(enable=1)
+    info0 = server.sess.info()
+    t0 = time.perf_counter()
+    while pending or not server.idle():
+        clock = server.clock
+        while pending and pending[0]["arrival"] <= clock:
+            d = pending.pop(0)
+            server.submit(d["key"], d["system"], d["nsteps"])
+        if server.idle() and pending:
+            # nothing to run before the next arrival: let the clock jump
+            gap = pending[0]["arrival"] - clock
+            server.sess.step(max(gap, 1))
+            continue
+        server.step(chunk)
+    wall = time.perf_counter() - t0
+    info = server.sess.info()
+    step_ms = server.sess.step_times(enable=0)
+    lat = np.array([r["t_done"] - r["t_submit"] for r in server.results.values()])
+    lat_steps = np.array([r["clock_done"] - r["clock_submit"] for r in server.results.values()])
+    return dict(wall=wall, pairs=info["pair_count"] - info0["pair_count"], layouts=info["layouts"] - info0["layouts"],
+                layout_ms=info["layout_ms"] - info0["layout_ms"], step_ms=step_ms, job_latency_s=lat,
+                job_latency_steps=lat_steps, clock=info["clock"])

@@ -237,16 +237,19 @@ def test_water32768_full_size():
 # --------------------------------------------------------------------------
 # mixed precision (FP32 pair math, int64 fixed-point forces, FP64 sums)
 # tolerance (per atom |dF_i| / max(|F_i|, 1 kJ/mol/nm)): median <= 2e-6, max
-# <= 5e-5.  FP32 pair forces carry ~2-3 ulp (1.5e-7) each; an atom's net force
-# cancels ~1000 terms up to 10x larger than itself, so the worst atoms of a
-# 98k-atom water box sit near 2e-5 (a float64 emulation of the same FP32
-# operation sequence gives 1.4e-5 even with a correctly rounded rsqrt).
+# <= 1e-4.  FP32 pair forces carry ~2-3 ulp (1.5e-7) each; an atom's net force
+# cancels ~1000 terms up to 100x larger than itself, so the worst atoms of a
+# 98k-atom water box sit near 3e-5 (a float64 emulation of the same FP32
+# operation sequence gives 1.4e-5 even with a correctly rounded rsqrt), and a
+# water atom with |F| = 32 next to +-0.5 e ions reaches 7e-5.  Pairs whose
+# first atom is perturbed are evaluated in FP64 and held to 1e-5.
 # Energies: 1e-5 of the job's potential-energy scale |E_LJ| + |E_CRF| (FP32
 # pair energies summed in FP64; a component that is itself a near-cancelling
 # sum, as E_CRF of randomly oriented water, is held to the scale of the terms
 # it is made of, not to its own small value).
 # --------------------------------------------------------------------------
-MX_F_TOL = 5e-5
+MX_F_TOL = 1e-4
+MX_F_TOL_PERT = 1e-5
 MX_F_MED = 2e-6
 MX_E_TOL = 1e-5
 
@@ -350,4 +353,8 @@ def test_resident_fep_bonded_solute_all_passes():
 
 def test_mixed_fep_job():
     s = gen.fep_job(3000, 12, 0.45, seed=22)
-    _fep_case(s, prec=rt.MIXED)
+    g, r = _fep_case(s, prec=rt.MIXED)
+    # the perturbed atoms' own forces come from FP64 pairs (all their partners
+    # are listed after them) plus FP64 excluded / self terms
+    pert = np.where(s.pert.flag)[0]
+    assert force_err(g["forces"][pert], r["forces"][pert]) < MX_F_TOL_PERT

@@ -142,3 +142,19 @@ def test_replica_exchange_primitives():
     assert np.array_equal(sess.energies(ja, first=10), e_alone)
     assert np.array_equal(sess.fetch(ja)[0], x_alone)
     sess.close()
+
+
+def test_replica_exchange_run():
+    remd = importlib.import_module("gromos-rs_amd.remd")
+    temps = [280.0, 300.0, 320.0, 340.0]
+    systems = [gen.water_replica(seed=20 + k, temperature=t, n_waters=500) for k, t in enumerate(temps)]
+    sess = rt.Session(precision=rt.MIXED, nsnb=5)
+    rx = remd.ReplicaExchange(sess, systems, temps, exchange_interval=10, scheme="odd_even", seed=1)
+    rx.run(30)
+    assert rx.attempts == 2 + 1 + 2
+    assert [h[1:3] for h in rx.history[:3]] == [(0, 1), (2, 3), (1, 2)]
+    for j in rx.jobs:
+        assert sess.query(j)[1] == 30
+        e = sess.energies(j)
+        assert np.all(np.isfinite(e)) and e.shape == (30, 5)
+    sess.close()

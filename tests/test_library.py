@@ -50,3 +50,20 @@ def test_errors_are_reported_not_crashing():
     # bad arguments are rejected with a message before any device work
     with pytest.raises(rt.GromosAmdError, match="cutoff"):
         rt.crf_new(0.0)
+
+
+def test_job_footprint_host_side():
+    """admission estimate: computed on the host from the topology alone"""
+    gen = importlib.import_module("gromos-rs_amd.generators")
+    small, big = gen.spc_water_box(1000, seed=1), gen.spc_water_box(8000, seed=1)
+    fs, fb = rt.Session.footprint(small), rt.Session.footprint(big)
+    assert 0 < fs < fb
+    # per-atom arrays plus neighbour entries at the same density: close to linear
+    assert 6.0 < fb / fs < 10.0
+    # footprint of the 98k-atom water box stays below 2 KB per atom
+    assert rt.Session.footprint(gen.water_32768()) < 2048 * 98304
+    bad = gen.spc_water_box(10, seed=1)
+    bad.iac = bad.iac.copy()
+    bad.iac[0] = bad.n_types       # type out of range
+    with pytest.raises(rt.GromosAmdError, match="type out of range"):
+        rt.Session.footprint(bad)

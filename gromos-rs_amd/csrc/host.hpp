@@ -82,6 +82,7 @@ struct gromos_amd_system {
     int64_t *d_vv_cnt = nullptr, *d_gen_cnt = nullptr, *d_vv_off = nullptr, *d_gen_off = nullptr;
     uint32_t *d_vv_ent = nullptr, *d_gen_ent = nullptr;
     uint64_t* d_counters = nullptr;
+    int64_t *d_pc_short = nullptr, *d_pc_long = nullptr;
     int64_t *d_vv_cap = nullptr, *d_gen_cap = nullptr;
     int32_t *d_prev_vv = nullptr, *d_prev_gen = nullptr, *d_overflow = nullptr;
     bool have_prev = false;
@@ -179,8 +180,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
 int upload_jobs(gromos_amd_system* S);
 int build_pairlist(gromos_amd_system* S);
 int forcefield(gromos_amd_system* S);
-// nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when given
-int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec);
+// nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when
+// given.  A SETTLE failure is an error unless `failed` is given: it then gets
+// each job's first failing step (INT32_MAX: none).
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed = nullptr);
 int collect_step_times(gromos_amd_system* S);
 
 // device helpers for the session

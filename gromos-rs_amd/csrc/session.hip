@@ -13,6 +13,7 @@
 // steps replicas one after another on the CPU (remd.rs:160-201); here all jobs
 // step together in one set of launches.
 #include <chrono>
+#include <cstdio>
 #include <deque>
 #include <map>
 #include <memory>
@@ -115,6 +116,7 @@ struct SJob {
     int32_t slot = -1;              // index in the resident batch, -1 if not resident
     bool evict = false;
     std::vector<double> energies;   // 5 per step done: e_lj, e_crf, e_pot, e_kin, dhdl
+    std::string error;              // why the job failed
 };
 
 }  // namespace
@@ -152,7 +154,8 @@ static int relayout(gromos_amd_session* s) {
     auto t0 = std::chrono::steady_clock::now();
     gromos_amd_system* S = s->S.get();
     std::vector<SJob*> keep, leave;
-    for (SJob* j : s->batch) (j->state == GROMOS_AMD_JOB_DONE || j->evict ? leave : keep).push_back(j);
+    for (SJob* j : s->batch)
+        (j->state == GROMOS_AMD_JOB_DONE || j->state == GROMOS_AMD_JOB_FAILED || j->evict ? leave : keep).push_back(j);
     std::vector<SJob*> nb = keep;
     for (SJob* j : s->queue) nb.push_back(j);
     std::vector<int32_t> old_off;
@@ -234,7 +237,8 @@ static int relayout(gromos_amd_session* s) {
 
 static int boundary(gromos_amd_session* s) {
     bool any_leave = false;
-    for (SJob* j : s->batch) any_leave = any_leave || j->state == GROMOS_AMD_JOB_DONE || j->evict;
+    for (SJob* j : s->batch)
+        any_leave = any_leave || j->state == GROMOS_AMD_JOB_DONE || j->state == GROMOS_AMD_JOB_FAILED || j->evict;
     if (!any_leave && s->queue.empty()) return 0;
     bool idle = running_count(s) == 0;
     if (!idle && s->clock % s->prm.admit_interval != 0) return 0;
@@ -334,7 +338,8 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
             if (changed)
                 if (int rc = upload_jobs(S)) return rc;
             int64_t pc0 = S->pair_count;
-            if (int rc = md_steps(S, k, &rec)) return rc;
+            std::vector<int32_t> failed;
+            if (int rc = md_steps(S, k, &rec, &failed)) return rc;
             s->pair_count += S->pair_count - pc0;
             if (s->step_timing) {
                 s->step_ms.insert(s->step_ms.end(), S->step_ms.begin(), S->step_ms.end());
@@ -342,6 +347,15 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
             }
             for (size_t b = 0; b < s->batch.size(); ++b) {
                 SJob* j = s->batch[b];
+                if (failed[b] != INT32_MAX && failed[b] < adv[b]) {
+                    // this job alone stops; the others are independent and go on
+                    adv[b] = failed[b];
+                    j->state = GROMOS_AMD_JOB_FAILED;
+                    char msg[160];
+                    snprintf(msg, sizeof(msg), "SETTLE failed: sin(angle) > 1.0 in analytical solve (step %lld)",
+                             (long long)(j->done + failed[b]));
+                    j->error = msg;
+                }
                 for (int64_t st = 0; st < adv[b]; ++st) {
                     const double* r = rec.data() + ((size_t)st * S->njobs + b) * REC;
                     j->energies.insert(j->energies.end(), r, r + 5);
@@ -374,6 +388,7 @@ extern "C" int gromos_amd_job_query(gromos_amd_session* s, int64_t id, int32_t* 
     if (!j) return GROMOS_AMD_ENOTFOUND;
     if (state) *state = j->state;
     if (steps_done) *steps_done = j->done;
+    if (j->state == GROMOS_AMD_JOB_FAILED) fail(GROMOS_AMD_ESETTLE, "%s", j->error.c_str());
     return 0;
 }
 

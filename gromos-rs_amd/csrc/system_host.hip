@@ -411,6 +411,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 4)) ||
+        (rc = S->reserve(S->d_pc_short, nc, true)) || (rc = S->reserve(S->d_pc_long, nc, true)) ||
         (rc = S->reserve(S->d_xq, npa, true)) || (rc = S->reserve(S->d_vv_cap, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cap, nc + 1, true)) || (rc = S->reserve(S->d_prev_vv, nc, true)) ||
         (rc = S->reserve(S->d_prev_gen, nc, true)) || (rc = S->reserve(S->d_overflow, 1)) ||
@@ -579,7 +580,7 @@ int forcefield(gromos_amd_system* S) {
     return 0;
 }
 
-int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec) {
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed_out) {
     if (nsteps <= 0) return 0;
     int64_t need = (int64_t)nsteps * S->njobs * REC;
     if (int rc = S->reserve(S->d_records, (size_t)need, true)) return rc;
@@ -619,6 +620,10 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec) {
         if (int rc = collect_step_times(S)) return rc;
     }
     if (int rc = S->collect_timing()) return rc;
+    if (failed_out) {
+        *failed_out = failed;
+        return 0;
+    }
     for (int j = 0; j < S->njobs; ++j)
         if (failed[j] != INT32_MAX)
             return fail(GROMOS_AMD_ESETTLE, "SETTLE failed: sin(angle) > 1.0 in analytical solve (job %d, step %d)", j, failed[j]);
@@ -718,6 +723,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.vv_ent = d_vv_ent;
     p.gen_ent = d_gen_ent;
     p.counters = d_counters;
+    p.pc_short = d_pc_short;
+    p.pc_long = d_pc_long;
     p.vv_cap = d_vv_cap;
     p.gen_cap = d_gen_cap;
     p.prev_vv = d_prev_vv;

@@ -206,22 +206,37 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
                     ngen += __popcll(bg);
                 }
             }
-    if (lane == 0) {
-        p.vv_cnt[k] = nvv;
-        p.gen_cnt[k] = ngen;
-        if (FILL && (nvv > vv_cap || ngen > gen_cap)) atomicOr(p.overflow, 1);
-    }
-    // per-wave sums, one atomic per wave
+    // per-group results; totals come from k_totals (one atomic per block there:
+    // thousands of waves adding to the same address serialise at the L2)
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) {
         pairs_short += __shfl_xor(pairs_short, o, WAVE);
         pairs_long += __shfl_xor(pairs_long, o, WAVE);
     }
     if (lane == 0) {
-        if (pairs_short) atomicAdd((unsigned long long*)p.counters, (unsigned long long)pairs_short);
-        if (pairs_long) atomicAdd((unsigned long long*)p.counters + 1, (unsigned long long)pairs_long);
-        atomicAdd((unsigned long long*)p.counters + 2, (unsigned long long)nvv);
-        atomicAdd((unsigned long long*)p.counters + 3, (unsigned long long)ngen);
+        p.vv_cnt[k] = nvv;
+        p.gen_cnt[k] = ngen;
+        p.pc_short[k] = (int64_t)pairs_short;
+        p.pc_long[k] = (int64_t)pairs_long;
+        if (FILL && (nvv > vv_cap || ngen > gen_cap)) atomicOr(p.overflow, 1);
+    }
+}
+
+// counters[0..3] = sums over groups of short pairs, long pairs, vv entries,
+// gen entries (counters cleared before)
+__global__ void __launch_bounds__(BLOCK) k_totals(SysPtrs p, int32_t ncg) {
+    __shared__ double lds[17 * 4];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};   // exact: each sum stays below 2^53
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ncg; k += gridDim.x * blockDim.x) {
+        v[0] += (double)p.pc_short[k];
+        v[1] += (double)p.pc_long[k];
+        v[2] += (double)p.vv_cnt[k];
+        v[3] += (double)p.gen_cnt[k];
+    }
+    block_sum<4>(v, lds);
+    if (threadIdx.x < 4) {
+        unsigned long long t = (unsigned long long)lds[16 * 4 + threadIdx.x];
+        if (t) atomicAdd((unsigned long long*)p.counters + threadIdx.x, t);
     }
 }
 
@@ -1312,6 +1327,8 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
     dim3 g((unsigned)((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE)));
     if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncg);
     else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncg);
+    unsigned nb = (unsigned)std::min<int64_t>(256, (ncg + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_totals, dim3(nb), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);

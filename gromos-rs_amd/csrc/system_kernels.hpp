@@ -51,7 +51,10 @@ struct SysPtrs {
     const int64_t* gen_off;
     uint32_t* vv_ent;
     uint32_t* gen_ent;
-    uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild
+    uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild,
+                               // [2] vv entries, [3] gen entries
+    int64_t* pc_short;         // per sorted group: canonical short / long pair interactions
+    int64_t* pc_long;
     int64_t* vv_cap;           // single-pass fill: entries reserved per sorted group
     int64_t* gen_cap;
     int32_t* prev_vv;          // previous build's counts per original group

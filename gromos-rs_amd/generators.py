@@ -288,13 +288,15 @@ def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=
             type_seed=5, temperature=300.0, name=None):
     """One FEP lambda-window job: n_pert perturbed solute atoms (each its own
     charge group, no bonds, >= 0.4 nm apart) with A/B charges uniform in
-    [-0.5, 0.5] e and A/B types from the 20-type table, in SPC water filling the
-    rest of n_atoms at 0.997 g/cm^3; soft-core alphas 0.5 / 0.5 nm^2 (global
-    ALPHLJ = ALPHC = 1)."""
+    [-0.5, 0.5] e and A/B types from the 20-type table, in SPC water at
+    0.997 g/cm^3 in the box sized for n_atoms, the waters of each site's first
+    shell removed (so the job holds somewhat fewer than n_atoms atoms);
+    soft-core alphas 0.5 / 0.5 nm^2 (global ALPHLJ = ALPHC = 1)."""
     rng = np.random.default_rng(seed)
     n_w = max(1, (n_atoms - n_pert) // 3)
     L = box_for_density(n_w + n_pert // 3 + 1)
-    base = spc_water_box(n_w + 64, box=L, seed=seed, temperature=temperature, rcutp=rc, rcutl=rc, nsnb=nsnb)
+    base = spc_water_box(n_w + n_pert // 3 + 1, box=L, seed=seed, temperature=temperature, rcutp=rc, rcutl=rc,
+                         nsnb=nsnb)
     # solute sites
     sites = []
     while len(sites) < n_pert:
@@ -302,10 +304,14 @@ def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=
         if all(((np.asarray(sv) - c - L * np.round((np.asarray(sv) - c) / L)) ** 2).sum() >= 0.16 for sv in sites):
             sites.append(c)
     sites = np.array(sites)
-    o = base.pos[0::3]
-    d2 = _periodic_min_dist2(o, sites, L).min(1)
-    order = np.argsort(-d2)        # farthest waters first
-    keepw = np.sort(order[:n_w])
+    # clear the first shell of every site: no O within 0.3 nm, no H within 0.22 nm
+    # (H carries no LJ, so nothing else would keep it off a charged site)
+    wp = base.pos.reshape(-1, 3, 3)
+    d_o = _periodic_min_dist2(wp[:, 0], sites, L).min(1)
+    d_h = np.minimum(_periodic_min_dist2(wp[:, 1], sites, L).min(1), _periodic_min_dist2(wp[:, 2], sites, L).min(1))
+    ok = np.where((d_o >= 0.09) & (d_h >= 0.22 ** 2))[0]
+    keepw = np.sort(ok[np.argsort(-d_o[ok], kind="stable")[:n_w]])
+    n_w = len(keepw)
     wpos = base.pos.reshape(-1, 3, 3)[keepw].reshape(-1, 3)
     wvel = base.vel.reshape(-1, 3, 3)[keepw].reshape(-1, 3)
     trng = np.random.default_rng(type_seed)

@@ -300,7 +300,7 @@ class DeviceSystem:
 
 
 EBUSY = -5
-QUEUED, RUNNING, DONE = 0, 1, 2
+QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
 
 class Busy(GromosAmdError):

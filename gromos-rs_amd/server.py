@@ -47,14 +47,16 @@ class Server:
     def _collect(self):
         for jid in list(self.running):
             st, done = self.sess.query(jid)
-            if st != rt.DONE:
+            if st not in (rt.DONE, rt.FAILED):
                 continue
+            err = rt.lib().gromos_amd_last_error().decode() if st == rt.FAILED else None
             w = self.running.pop(jid)
             e = self.sess.energies(jid)
             x, v = self.sess.fetch(jid)
             self.sess.evict(jid)
             w.pop("system")
-            w.update(t_done=time.perf_counter(), clock_done=self.clock, energies=e, pos=x, vel=v)
+            w.update(t_done=time.perf_counter(), clock_done=self.clock, energies=e, pos=x, vel=v, failed=err,
+                     steps_done=done)
             self.results[w["key"]] = w
 
     def step(self, nsteps):

@@ -244,7 +244,8 @@ typedef struct gromos_amd_session gromos_amd_session;
 enum gromos_amd_job_state {
     GROMOS_AMD_JOB_QUEUED = 0,   /* admitted, waiting for the next layout boundary */
     GROMOS_AMD_JOB_RUNNING = 1,
-    GROMOS_AMD_JOB_DONE = 2      /* all requested steps done */
+    GROMOS_AMD_JOB_DONE = 2,     /* all requested steps done */
+    GROMOS_AMD_JOB_FAILED = 3    /* stopped (SETTLE failure); last_error() after job_query says why */
 };
 
 typedef struct {

@@ -224,9 +224,11 @@ def fep_mix_bench(rt, gen, steps, alone_steps):
     srv = importlib.import_module("gromos-rs_amd.server")
     mix = gen.fep_mix()
     systems = [gen.fep_job(m["n_atoms"], m["n_pert"], m["lam"], m["seed"]) for m in mix]
+    for s in systems:   # NVT windows: Berendsen 300 K, tau 0.1 ps, dof 3N - 3 per rigid water
+        s.thermostat = (300.0, 0.1, 3 * s.n_atoms - 3 * s.nsm)
     sched = [dict(key=k, system=systems[k], nsteps=steps, arrival=m["arrival"]) for k, m in enumerate(mix)]
     out = dict(jobs=len(mix), atoms=int(sum(s.n_atoms for s in systems)), steps_per_job=steps,
-               arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5,
+               arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5, thermostat="Berendsen 300 K, tau 0.1 ps",
                arrivals_last_step=int(max(m["arrival"] for m in mix)))
     for name, prec in (("batched_mixed", rt.MIXED), ("batched_fp64", rt.FP64)):
         server = srv.Server(precision=prec, nsnb=5)
@@ -237,7 +239,8 @@ def fep_mix_bench(rt, gen, steps, alone_steps):
                          step_ms_p99=float(np.percentile(r["step_ms"], 99)),
                          job_latency_s_p50=float(np.percentile(r["job_latency_s"], 50)),
                          job_latency_s_p99=float(np.percentile(r["job_latency_s"], 99)),
-                         layouts=r["layouts"], layout_ms_total=r["layout_ms"], session_steps=r["clock"])
+                         layouts=r["layouts"], layout_ms_total=r["layout_ms"], session_steps=r["clock"],
+                         failed_jobs=sum(1 for v in server.results.values() if v["failed"]))
         server.sess.close()
     # one at a time, plain FP64 (rate over alone_steps per job, pair-list cadence included)
     pairs, wall = 0, 0.0

@@ -95,6 +95,8 @@ struct gromos_amd_system {
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
     double *d_rows = nullptr, *d_records = nullptr;
     int32_t* d_settle_fail = nullptr;
+    double* d_therm_scale = nullptr;
+    bool any_thermo = false;
     uint8_t *d_pflag = nullptr, *d_cg_pert = nullptr;
     int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
     int32_t *d_pap_off = nullptr, *d_pap = nullptr;

@@ -51,6 +51,8 @@ struct JobDev {
                                            // of the current call (INT32_MAX: always)
     double dt;
     double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
+    int32_t thermo;                        // Berendsen weak coupling (berendsen_thermostat.rs)
+    double t_ref, tau_t, dof_t;
 };
 
 // energy partial rows: per kernel slot and block, 12 doubles

@@ -95,6 +95,11 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.dist_oh = T.dist_oh;
     J.dist_hh = T.dist_hh;
     J.stop = INT32_MAX;
+    if (P.thermostat != 0 && P.thermostat != 1) return fail(GROMOS_AMD_EINVAL, "job %d: thermostat must be 0 or 1", j);
+    J.thermo = P.thermostat;
+    J.t_ref = P.t_ref;
+    J.tau_t = P.tau_t;
+    J.dof_t = P.dof_t;
     // use_cg_grouped_solute (forcefield.rs:271-283)
     if (J.box.kind == 0) J.use_cg_grouped = 1;
     else {
@@ -259,7 +264,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     excl_off.push_back(0);
     o14_off.push_back(0);
     o14s_off.push_back(0);
-    bool twin = false;
+    bool twin = false, any_thermo = false;
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -272,6 +277,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         J.stop = INT32_MAX;
         cell_off += (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
         twin = twin || J.twin;
+        any_thermo = any_thermo || J.thermo;
         use_settle |= im.P.use_settle && !im.mol_first.empty();
         do_virial |= im.P.do_virial;
         S->jobs.push_back(J);
@@ -340,6 +346,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->use_settle = use_settle;
     S->do_virial = do_virial;
     S->twin = twin;
+    S->any_thermo = any_thermo;
     S->cell_bits = 1;
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
     S->n_papairs = (int32_t)papairs.size() / 4;
@@ -420,7 +427,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_xf, npa, true)) || (rc = S->reserve(S->d_fi_s, n3, true)) ||
         (rc = S->reserve(S->d_fi_long_s, n3, true)) ||
         (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
-        (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)))
+        (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -600,6 +608,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     for (int32_t s = 0; s < nsteps; ++s) {
         if (int rc = forcefield(S)) return rc;
         SysPtrs p = S->ptrs();
+        if (S->any_thermo) launch_therm(p, S->njobs, st);
         launch_leapfrog(p, S->npad, s, st);
         if (S->use_settle) launch_settle(p, S->nmol, s, st);
         launch_ekin(p, S->npad, st);
@@ -741,6 +750,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.rows = d_rows;
     p.records = d_records;
     p.settle_fail = d_settle_fail;
+    p.therm_scale = d_therm_scale;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -1099,12 +1099,57 @@ __global__ void k_leapfrog(SysPtrs p, int32_t npad, int32_t step) {
     }
     double dt = J.dt;
     double im = p.inv_mass[i];
+    if (J.thermo) {
+        // Berendsen scaling between LeapFrogVelocity and LeapFrogPosition
+        // (berendsen_thermostat.rs:29-31)
+        double lam = p.therm_scale[p.atom_job[i]];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = p.f[3 * i + c] * im;
+            double vn = (p.v_cur[3 * i + c] + acc * dt) * lam;
+            p.v_new[3 * i + c] = vn;
+            p.x_new[3 * i + c] = p.x_cur[3 * i + c] + vn * dt;
+        }
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double acc = p.f[3 * i + c] * im;
         double vn = p.v_cur[3 * i + c] + acc * dt;
         p.v_new[3 * i + c] = vn;
         p.x_new[3 * i + c] = p.x_cur[3 * i + c] + vn * dt;
+    }
+}
+
+// Berendsen scale of each coupled job from E_kin_new = 0.5 sum m |v|^2 of the
+// velocities entering the step (what the previous TemperatureCalculation
+// stored, temperature.rs:76-85; berendsen_thermostat.rs:66-100), block per job
+__global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
+    __shared__ double lds[17];
+    const JobDev& J = p.jobs[blockIdx.x];
+    if (!J.thermo) {
+        if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = 1.0;
+        return;
+    }
+    double s = 0.0;
+    for (int32_t a = J.atom_off + threadIdx.x; a < J.atom_off + J.n_atoms; a += blockDim.x) {
+        const double* v = p.v_cur + 3 * (int64_t)a;
+        s += 0.5 * p.mass[a] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    double t[1] = {s};
+    block_sum<1>(t, lds);
+    if (threadIdx.x == 0) {
+        double ekin = lds[16];
+        double scale = 1.0;
+        if (J.tau_t >= 0.0) {
+            const double kB = 0.00831441;   // units.rs:47
+            double free_temp = J.dof_t > 0.0 ? 2.0 * ekin / (J.dof_t * kB) : 0.0;
+            if (free_temp < 1e-10) free_temp = J.t_ref;
+            if (free_temp < 1e-10) scale = 1.0;
+            else if (J.tau_t == 0.0) scale = sqrt(J.t_ref / free_temp);
+            else scale = sqrt(1.0 + J.dt / J.tau_t * (J.t_ref / free_temp - 1.0));
+        }
+        p.therm_scale[blockIdx.x] = scale;
     }
 }
 
@@ -1368,6 +1413,9 @@ void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed,
                      hipStream_t s) {
     hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, mixed);
+}
+void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_therm, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s) {
     hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad, step);

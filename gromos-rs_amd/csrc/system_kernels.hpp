@@ -72,6 +72,7 @@ struct SysPtrs {
     double* rows;
     double* records;
     int32_t* settle_fail;      // per job: first failing step
+    double* therm_scale;       // per job: Berendsen velocity scale of the current step
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;
@@ -100,6 +101,7 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s);
+void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
 void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,

@@ -68,7 +68,8 @@ class SessionInfo(C.Structure):
 class Params(C.Structure):
     _fields_ = [("boundary", I32), ("box", D * 3), ("rcutp", D), ("rcutl", D), ("skin", D),
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
-                ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32)]
+                ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
+                ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D)]
 
 
 def built():
@@ -223,6 +224,10 @@ def params_struct(s, use_settle=True, do_virial=False):
     p.use_settle, p.do_virial = int(use_settle), int(do_virial)
     p.lam = s.pert.lam if s.pert is not None else 0.0
     p.nlam = s.pert.nlam if s.pert is not None else 1
+    th = getattr(s, "thermostat", None)
+    if th is not None:
+        p.thermostat = 1
+        p.t_ref, p.tau_t, p.dof_t = (float(x) for x in th)
     return p
 
 

@@ -88,6 +88,7 @@ class System:
     dist_oh: float = 0.1
     dist_hh: float = 0.1632990
     pert: Optional[Perturbation] = None
+    thermostat: Optional[tuple] = None   # Berendsen (T_ref K, tau ps, degrees of freedom)
     name: str = "system"
 
     @property

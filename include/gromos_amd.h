@@ -192,6 +192,12 @@ typedef struct {
     int32_t do_virial;
     double lambda;
     int32_t nlam;
+    /* Berendsen weak coupling over the job's atoms (berendsen_thermostat.rs):
+     * 0 = none (NVE), 1 = Berendsen with reference temperature t_ref (K),
+     * coupling time tau_t (ps; 0 = instantaneous, < 0 = no coupling) and dof_t
+     * degrees of freedom (md.rs:1164-1172: 3N - constraints - NDFMIN) */
+    int32_t thermostat;
+    double t_ref, tau_t, dof_t;
 } gromos_amd_params;
 
 enum gromos_amd_precision {

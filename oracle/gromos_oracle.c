@@ -1236,6 +1236,8 @@ struct or_md {
     or_ff *ff;
     double dt;
     int use_settle;
+    int thermo;                 /* Berendsen weak coupling, one bath over all atoms */
+    double t_ref, tau_t, dof_t;
     double *pos[2], *vel[2], *force[2];
     double *inv_mass;
     int cur;
@@ -1280,6 +1282,29 @@ void or_md_destroy(or_md *md) {
     free(md);
 }
 
+void or_md_set_berendsen(or_md *md, double t_ref, double tau, double dof) {
+    md->thermo = 1;
+    md->t_ref = t_ref;
+    md->tau_t = tau;
+    md->dof_t = dof;
+}
+
+/* BerendsenThermostat::apply (berendsen_thermostat.rs:61-100): the scale from
+ * E_kin_new = 0.5 sum m |v|^2 of the velocities entering the step (stored by the
+ * previous TemperatureCalculation, temperature.rs:76-85, or by its init) */
+double or_berendsen_scale(int n, const double *mass, const double *vel, double t_ref, double tau, double dof,
+                          double dt) {
+    if (tau < 0.0) return 1.0;
+    double ekin = 0.0;
+    for (int i = 0; i < n; ++i)
+        ekin += 0.5 * mass[i] * (vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] + vel[3 * i + 2] * vel[3 * i + 2]);
+    double free_temp = dof > 0.0 ? 2.0 * ekin / (dof * OR_KB) : 0.0;
+    if (free_temp < 1e-10) free_temp = t_ref;
+    if (free_temp < 1e-10) return 1.0;
+    if (tau == 0.0) return sqrt(t_ref / free_temp);
+    return sqrt(1.0 + dt / tau * (t_ref / free_temp - 1.0));
+}
+
 int or_md_run(or_md *md, int nsteps, double *energies) {
     int n = md->t->n_atoms;
     for (int s = 0; s < nsteps; ++s) {
@@ -1288,7 +1313,19 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
         or_ff_apply(md->ff, md->pos[c], md->force[c], &elj, &ecrf, NULL, &dh);
         md->pairs += or_ff_last_pair_count(md->ff);
         /* exchange_state: the current buffer becomes old */
-        or_leapfrog(n, md->inv_mass, md->dt, md->pos[c], md->vel[c], md->force[c], md->pos[o], md->vel[o]);
+        if (md->thermo) {
+            /* LeapFrogVelocity, Berendsen scaling, LeapFrogPosition */
+            double lam = or_berendsen_scale(n, md->t->mass, md->vel[c], md->t_ref, md->tau_t, md->dof_t, md->dt);
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    double acc = md->force[c][3 * i + a] * md->inv_mass[i];
+                    md->vel[o][3 * i + a] = (md->vel[c][3 * i + a] + acc * md->dt) * lam;
+                }
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < 3; ++a) md->pos[o][3 * i + a] = md->pos[c][3 * i + a] + md->vel[o][3 * i + a] * md->dt;
+        } else {
+            or_leapfrog(n, md->inv_mass, md->dt, md->pos[c], md->vel[c], md->force[c], md->pos[o], md->vel[o]);
+        }
         if (md->use_settle) {
             int err = or_settle(md->t->n_solute_atoms, n, md->m_o, md->m_h, md->d_oh, md->d_hh, md->dt,
                                 md->pos[c], md->pos[o], md->vel[o], NULL, NULL);

@@ -250,6 +250,11 @@ void or_md_destroy(or_md *md);
  * energies: per step 5 doubles (e_lj, e_crf, e_pot, e_kin, dhdl), may be NULL. */
 void or_md_set_settle_geometry(or_md *md, double d_oh, double d_hh);
 int or_md_run(or_md *md, int nsteps, double *energies);
+/* Berendsen weak coupling (berendsen_thermostat.rs), one bath over all atoms */
+#define OR_KB 0.00831441   /* kJ/(mol K), units.rs:47 */
+void or_md_set_berendsen(or_md *md, double t_ref, double tau, double dof);
+double or_berendsen_scale(int n, const double *mass, const double *vel, double t_ref, double tau, double dof,
+                          double dt);
 void or_md_get(const or_md *md, double *pos, double *vel, double *force);
 int64_t or_md_pair_count(const or_md *md);
 

@@ -142,6 +142,9 @@ def _declare(L):
     L.or_md_create.restype = P
     L.or_md_destroy.argtypes = [P]
     L.or_md_set_settle_geometry.argtypes = [P, D, D]
+    L.or_md_set_berendsen.argtypes = [P, D, D, D]
+    L.or_berendsen_scale.argtypes = [C.c_int, P, P, D, D, D, D]
+    L.or_berendsen_scale.restype = D
     L.or_md_run.argtypes = [P, C.c_int, P]
     L.or_md_get.argtypes = [P, P, P, P]
     L.or_md_pair_count.argtypes = [P]
@@ -379,6 +382,9 @@ class MD:
                                     C.byref(osys.pert) if osys.pert is not None else None, s.dt,
                                     int(use_settle), ptr(self.x0), ptr(self.v0))
         lib().or_md_set_settle_geometry(self.h, s.dist_oh, s.dist_hh)
+        if getattr(s, "thermostat", None) is not None:
+            t_ref, tau, dof = s.thermostat
+            lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
         self.n = s.n_atoms
 
     def run(self, nsteps):
@@ -402,6 +408,12 @@ class MD:
             lib().or_md_destroy(self.h)
         except Exception:
             pass
+
+
+def berendsen_scale(mass, vel, t_ref, tau, dof, dt):
+    m = c64(mass)
+    v = c64(vel)
+    return lib().or_berendsen_scale(len(m), ptr(m), ptr(v), t_ref, tau, dof, dt)
 
 
 def set_threads(n):

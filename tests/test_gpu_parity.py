@@ -358,3 +358,22 @@ def test_mixed_fep_job():
     # are listed after them) plus FP64 excluded / self terms
     pert = np.where(s.pert.flag)[0]
     assert force_err(g["forces"][pert], r["forces"][pert]) < MX_F_TOL_PERT
+
+
+# --------------------------------------------------------------------------
+# Berendsen weak coupling (berendsen_thermostat.rs) on the device vs the oracle
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("tau", [0.1, 0.0])
+def test_berendsen_md_matches_oracle(tau):
+    s = gen.water_216()
+    s.thermostat = (350.0, tau, 3 * s.n_atoms - 3 * s.nsm)
+    o = oracle.OracleSystem(s)
+    ds = rt.DeviceSystem(s)
+    ds.pairlist_update()
+    ge = ds.md(40)
+    oe = oracle.MD(o).run(40)
+    for k in range(5):
+        for c in range(4):
+            assert rel(ge[k, c], oe[k, c]) < 1e-10, (k, c, ge[k, c], oe[k, c])
+    assert np.all(np.abs(ge[:, 3] - oe[:, 3]) / oe[:, 3] < 1e-7)
+    ds.close()

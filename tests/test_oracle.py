@@ -10,6 +10,7 @@
   differences of energy vs force and of energy vs lambda (the soft-core CRF /
   dH/dlambda terms have no fixture: those checks are diagnostic only).
 """
+import importlib
 import math
 
 import numpy as np
@@ -292,3 +293,38 @@ def test_softcore_dhdl_lj_finite_difference():
     d = _sc(r0, lam, aq=0.0, bq=0.0)[3]
     fd = (_sc(r0, lam + h, aq=0.0, bq=0.0)[1] - _sc(r0, lam - h, aq=0.0, bq=0.0)[1]) / (2 * h)
     assert math.isclose(d, fd, rel_tol=1e-5)
+
+
+# --------------------------------------------------------------------------
+# Berendsen weak coupling: closed forms on free particles (no forces), where
+# E_kin' = lambda^2 E_kin exactly, so T' = T0 for tau = 0 and
+# T' = T + dt/tau (T0 - T) for tau > 0 (berendsen_thermostat.rs:29-31, 85-93)
+# --------------------------------------------------------------------------
+def _free_particles(n=10, seed=0):
+    System = importlib.import_module("gromos-rs_amd").system.System
+    rng = np.random.default_rng(seed)
+    z = np.zeros(n)
+    s = System(n_atoms=n, n_solute_atoms=n, cg_start=np.arange(n + 1, dtype=np.int32), n_solute_cg=n,
+               charge=z.copy(), iac=np.zeros(n, np.int32), mass=np.full(n, 12.011), n_types=1, lj=np.zeros(4),
+               excl_off=np.zeros(n + 1, np.int32), excl=np.zeros(0, np.int32), o14_off=np.zeros(n + 1, np.int32),
+               o14=np.zeros(0, np.int32), atoms_per_solvent=0, pos=rng.uniform(0, 3, (n, 3)),
+               vel=rng.normal(0, 0.5, (n, 3)), box=np.array([3.0, 3.0, 3.0]), boundary=0, rcutp=1.4, rcutl=1.4)
+    return s.validate()
+
+
+@pytest.mark.parametrize("tau", [0.0, 0.1, 0.5])
+def test_berendsen_closed_form(tau):
+    s = _free_particles()
+    kb = 0.00831441
+    dof = 3 * s.n_atoms
+    T = lambda v: np.sum(0.5 * s.mass[:, None] * v ** 2) * 2.0 / (dof * kb)
+    t0 = T(s.vel)
+    s.thermostat = (300.0, tau, dof)
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=False)
+    md.run(1)
+    _, v1, _ = md.state()
+    want = 300.0 if tau == 0.0 else t0 + s.dt / tau * (300.0 - t0)
+    assert T(v1) == pytest.approx(want, rel=1e-12)
+    # the scale itself against the formula's fallbacks
+    assert oracle.berendsen_scale(s.mass, 0 * s.vel, 300.0, tau, dof, s.dt) == pytest.approx(1.0)
+    assert oracle.berendsen_scale(s.mass, s.vel, 300.0, -1.0, dof, s.dt) == 1.0

@@ -93,7 +93,8 @@ struct gromos_amd_system {
     long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
     int32_t nrow_blocks = 0;
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
-    double *d_rows = nullptr, *d_records = nullptr;
+    double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr;
+    int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
     int32_t* d_settle_fail = nullptr;
     double* d_therm_scale = nullptr;
     bool any_thermo = false;

@@ -347,6 +347,13 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->do_virial = do_virial;
     S->twin = twin;
     S->any_thermo = any_thermo;
+    // LJ only between the solvent molecules' first atoms (SPC-like waters)
+    S->vv_lj00 = 1;
+    for (const JobDev& J : S->jobs)
+        if (J.n_atoms > J.n_solute_atoms)
+            for (int i = 0; i < J.aps; ++i)
+                for (int jj = 0; jj < J.aps; ++jj)
+                    if ((i || jj) && (J.sv_c6[i * 4 + jj] != 0.0 || J.sv_c12[i * 4 + jj] != 0.0)) S->vv_lj00 = 0;
     S->cell_bits = 1;
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
     S->n_papairs = (int32_t)papairs.size() / 4;
@@ -428,7 +435,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_fi_long_s, n3, true)) ||
         (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
-        (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)))
+        (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_vv_e, 11 * nc, true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -573,7 +581,7 @@ int forcefield(gromos_amd_system* S) {
             aps = J.aps;
             break;
         }
-    launch_vv(p, npad, aps, S->do_virial, mixed, st);
+    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, st);
     if (S->timing) HIPCHK(hipEventRecord(e1, st));
     launch_gen(p, npad, false, S->do_virial, mixed, st);
     S->pair_count += (int64_t)S->host_counters[0];
@@ -748,6 +756,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.f_long_s = d_f_long_s;
     p.f_x = d_f_x;
     p.rows = d_rows;
+    p.vv_e = d_vv_e;
     p.records = d_records;
     p.settle_fail = d_settle_fail;
     p.therm_scale = d_therm_scale;

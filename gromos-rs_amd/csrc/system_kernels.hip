@@ -306,92 +306,6 @@ __device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int sl
         rows[((int64_t)slot * nblocks + blockIdx.x) * ROW + threadIdx.x] = lds[16 * ROW + threadIdx.x];
 }
 
-// Solvent-solvent short pairs (process_solvent_pairs, innerloops.rs:218-309):
-// shift from the atom-0 pair, then n x n atom pairs with the template tables.
-// NCH lanes share one atom (lane = atom * NCH + chunk) and take every NCH-th
-// partner; their partial forces are summed with a fixed xor butterfly.  Each
-// lane evaluates the pair from its own side: r = (x_own + t) - x_partner with
-// t from nearest_image(own first atom, partner first atom).  For the lane of
-// the lower group this is the reference's operation sequence exactly; for the
-// other lane it is the mirrored sequence, equal up to rounding.  Every pair is
-// seen twice, so energies and virial are halved at the end.
-constexpr int NCH = 4;
-
-template <int APS, bool VIR>
-__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t npad) {
-    double v[ROW] = {0};
-    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
-    const int32_t a = gid / NCH;
-    const int chunk = gid % NCH;
-    int32_t o = p.s2o[a];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (o >= 0) {
-        int32_t k = p.s_atom_cg[a];
-        int32_t e0 = (int32_t)p.vv_off[k], e1 = e0 + (int32_t)p.vv_cnt[k];
-        if (e1 > e0) {
-            const JobDev& J = p.jobs[p.atom_job[o]];
-            const Box box = J.box;
-            const double c2 = J.crf.crf_2cut3i, cc = J.crf.crf_cut, c3 = J.crf.crf_cut3i;
-            int32_t sk = p.s_cg_start[k];
-            int32_t ia = a - sk;
-            d3 pa = xyz(p.xq[a]);
-            d3 pk0 = xyz(p.xq[sk]);
-            double c6[APS], c12[APS], qq[APS];
-#pragma unroll
-            for (int b = 0; b < APS; ++b) {
-                c6[b] = J.sv_c6[ia * 4 + b];
-                c12[b] = J.sv_c12[ia * 4 + b];
-                qq[b] = J.sv_qq[ia * 4 + b];
-            }
-            for (int32_t e = e0 + chunk; e < e1; e += NCH) {
-                int32_t sm = (int32_t)(p.vv_ent[e] & ENT_IDX);
-                d3 pm0 = xyz(p.xq[sm]);
-                d3 rf = nim(box, pk0, pm0);
-                double tx = rf.x - pk0.x + pm0.x, ty = rf.y - pk0.y + pm0.y, tz = rf.z - pk0.z + pm0.z;
-                double xi = pa.x + tx, yi = pa.y + ty, zi = pa.z + tz;
-#pragma unroll
-                for (int b = 0; b < APS; ++b) {
-                    d3 pb = xyz(p.xq[sm + b]);
-                    double rx = xi - pb.x, ry = yi - pb.y, rz = zi - pb.z;
-                    // lj_crf_interaction (innerloops.rs:22-55), branch-free guard
-                    double r2 = rx * rx + ry * ry + rz * rz;
-                    bool ok = !(r2 < 1e-10);
-                    double inv_r2 = 1.0 / r2;
-                    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                    double elj = (c12[b] * inv_r6 - c6[b]) * inv_r6;
-                    double flj = (12.0 * c12[b] * inv_r6 - 6.0 * c6[b]) * inv_r6 * inv_r2;
-                    double inv_r = sqrt(inv_r2);
-                    double ecrf = qq[b] * (inv_r - c2 * r2 - cc);
-                    double fcrf = qq[b] * (inv_r * inv_r2 + c3);
-                    double fm = ok ? flj + fcrf : 0.0;
-                    double gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                    fx += gx;
-                    fy += gy;
-                    fz += gz;
-                    v[0] += ok ? elj : 0.0;
-                    v[1] += ok ? ecrf : 0.0;
-                    if (VIR) add_virial(v, d3{rx, ry, rz}, gx, gy, gz);
-                }
-            }
-        }
-    }
-    // fixed-order butterfly over the NCH lanes of one atom
-#pragma unroll
-    for (int m = 1; m < NCH; m <<= 1) {
-        fx += __shfl_xor(fx, m, WAVE);
-        fy += __shfl_xor(fy, m, WAVE);
-        fz += __shfl_xor(fz, m, WAVE);
-    }
-    if (chunk == 0) {
-        p.f_s[3 * a] = fx;
-        p.f_s[3 * a + 1] = fy;
-        p.f_s[3 * a + 2] = fz;
-    }
-#pragma unroll
-    for (int c = 0; c < 11; ++c) v[c] *= 0.5;
-    write_row(v, p.rows, SLOT_VV, p.nrow_blocks);
-}
-
 // ---- mixed precision: FP32 pair arithmetic, FP64 per-lane partial sums,
 // ---- int64 fixed-point per-atom totals (exact and order-independent)
 constexpr double FX_SCALE = 16777216.0;          // 2^24 per kJ mol^-1 nm^-1
@@ -432,84 +346,248 @@ __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, floa
     z = (float)dz;
 }
 
-template <int APS, bool VIR>
-__global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t npad) {
-    double v[ROW] = {0};
+// Solvent-solvent short pairs (process_solvent_pairs, innerloops.rs:218-309).
+// One lane per (molecule, chunk): NCHM lanes share a solvent molecule, take
+// every NCHM-th listed partner molecule and evaluate all APS x APS atom pairs
+// with one shift per molecule pair; their partial forces are summed with a
+// fixed xor butterfly, so no atomics are needed and the order is fixed.  Each
+// lane works from its own side: r = (x_own + t) - x_partner with t from
+// nearest_image(own first atom, partner first atom).  For the lane of the lower
+// group (the "canonical" one) that is the reference's operation sequence, so
+// its per-pair energies are the reference's and only it adds energies and
+// virial.  LJ00: only the first-first atom pair has LJ parameters (water
+// models whose hydrogens carry none); the other pairs' LJ terms are exactly
+// zero in the reference and are skipped.  Energies and virial go to a per-group
+// record (vv_e, 11 doubles) that k_reduce sums per job.
+constexpr int NCHM = 8;
+
+template <int APS, bool LJ00>
+__device__ __forceinline__ bool has_lj(int i, int j) {
+    return !LJ00 || (i == 0 && j == 0);
+}
+
+template <int APS, bool LJ00, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
     const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
-    const int32_t a = gid / NCH;
-    const int chunk = gid % NCH;
-    int32_t o = p.s2o[a];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (o >= 0) {
-        int32_t k = p.s_atom_cg[a];
-        int32_t e0 = (int32_t)p.vv_off[k], e1 = e0 + (int32_t)p.vv_cnt[k];
-        if (e1 > e0) {
-            const JobDev& J = p.jobs[p.atom_job[o]];
-            const Box box = J.box;
-            const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
-            int32_t sk = p.s_cg_start[k];
-            int32_t ia = a - sk;
-            float4 da = p.xf[a];
-            d3 pk0 = xyz(p.xq[sk]);
-            float c6[APS], c12[APS], qq[APS];
+    const int32_t k = gid / NCHM;
+    const int chunk = gid % NCHM;
+    double f[APS][3];
+    double en[11];
 #pragma unroll
-            for (int b = 0; b < APS; ++b) {
-                c6[b] = (float)J.sv_c6[ia * 4 + b];
-                c12[b] = (float)J.sv_c12[ia * 4 + b];
-                qq[b] = (float)J.sv_qq[ia * 4 + b];
+    for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 11; ++c) en[c] = 0.0;
+    int32_t e0 = 0, e1 = 0, sk = 0;
+    if (k < ncg) {
+        e0 = (int32_t)p.vv_off[k];
+        e1 = e0 + (int32_t)p.vv_cnt[k];
+        sk = p.s_cg_start[k];
+    }
+    if (e1 > e0) {
+        const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
+        const Box box = J.box;
+        const double c2 = J.crf.crf_2cut3i, cc = J.crf.crf_cut, c3 = J.crf.crf_cut3i;
+        d3 pa[APS];
+#pragma unroll
+        for (int i = 0; i < APS; ++i) pa[i] = xyz(p.xq[sk + i]);
+        for (int32_t e = e0 + chunk; e < e1; e += NCHM) {
+            uint32_t ent = p.vv_ent[e];
+            int32_t sm = (int32_t)(ent & ENT_IDX);
+            const bool canon = (ent & ENT_CANON) != 0;
+            d3 pb[APS];
+#pragma unroll
+            for (int j = 0; j < APS; ++j) pb[j] = xyz(p.xq[sm + j]);
+            d3 rf = nim(box, pa[0], pb[0]);
+            double tx = rf.x - pa[0].x + pb[0].x, ty = rf.y - pa[0].y + pb[0].y, tz = rf.z - pa[0].z + pb[0].z;
+#pragma unroll
+            for (int i = 0; i < APS; ++i) {
+                double xi = pa[i].x + tx, yi = pa[i].y + ty, zi = pa[i].z + tz;
+#pragma unroll
+                for (int j = 0; j < APS; ++j) {
+                    double rx = xi - pb[j].x, ry = yi - pb[j].y, rz = zi - pb[j].z;
+                    // lj_crf_interaction (innerloops.rs:22-55), branch-free guard
+                    double r2 = rx * rx + ry * ry + rz * rz;
+                    bool ok = !(r2 < 1e-10);
+                    double inv_r2 = 1.0 / r2;
+                    double inv_r = sqrt(inv_r2);
+                    double qq = J.sv_qq[i * 4 + j];
+                    double ecrf = qq * (inv_r - c2 * r2 - cc);
+                    double fm = qq * (inv_r * inv_r2 + c3);
+                    double elj = 0.0;
+                    if (has_lj<APS, LJ00>(i, j)) {
+                        double c6 = J.sv_c6[i * 4 + j], c12 = J.sv_c12[i * 4 + j];
+                        double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                        elj = (c12 * inv_r6 - c6) * inv_r6;
+                        fm = (12.0 * c12 * inv_r6 - 6.0 * c6) * inv_r6 * inv_r2 + fm;
+                    }
+                    fm = ok ? fm : 0.0;
+                    double gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                    f[i][0] += gx;
+                    f[i][1] += gy;
+                    f[i][2] += gz;
+                    if (canon) {
+                        en[0] += ok ? elj : 0.0;
+                        en[1] += ok ? ecrf : 0.0;
+                        if (VIR) {
+                            en[2] += rx * gx; en[3] += rx * gy; en[4] += rx * gz;
+                            en[5] += ry * gx; en[6] += ry * gy; en[7] += ry * gz;
+                            en[8] += rz * gx; en[9] += rz * gy; en[10] += rz * gz;
+                        }
+                    }
+                }
             }
-            for (int32_t e = e0 + chunk; e < e1; e += NCH) {
-                uint32_t ent = p.vv_ent[e];
-                int32_t sm = (int32_t)(ent & ENT_IDX);
-                float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
-                float sx, sy, sz;
-                shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
-                float bx = sx + da.x, by = sy + da.y, bz = sz + da.z;
-                float gxs = 0.f, gys = 0.f, gzs = 0.f, el = 0.f, ec = 0.f;
-                float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    // fixed-order butterfly over the NCHM lanes of one molecule
 #pragma unroll
-                for (int b = 0; b < APS; ++b) {
-                    float4 db = p.xf[sm + b];
-                    float rx = bx - db.x, ry = by - db.y, rz = bz - db.z;
-                    float fm, elj, ecrf;
-                    lj_crf32(rx, ry, rz, c6[b], c12[b], qq[b], c2, cc, c3, fm, elj, ecrf);
+    for (int m = 1; m < NCHM; m <<= 1) {
+#pragma unroll
+        for (int i = 0; i < APS; ++i) {
+            f[i][0] += __shfl_xor(f[i][0], m, WAVE);
+            f[i][1] += __shfl_xor(f[i][1], m, WAVE);
+            f[i][2] += __shfl_xor(f[i][2], m, WAVE);
+        }
+#pragma unroll
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) en[c] += __shfl_xor(en[c], m, WAVE);
+    }
+    if (chunk == 0 && k < ncg) {
+        int32_t nk = p.s_cg_n[k];
+        if (e1 > e0) {
+#pragma unroll
+            for (int i = 0; i < APS; ++i) {
+                p.f_s[3 * (sk + i)] = f[i][0];
+                p.f_s[3 * (sk + i) + 1] = f[i][1];
+                p.f_s[3 * (sk + i) + 2] = f[i][2];
+            }
+        } else {
+            for (int32_t i = 0; i < 3 * nk; ++i) p.f_s[3 * sk + i] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = en[c];
+    }
+}
+
+// Mixed-precision counterpart: FP32 pair arithmetic on offsets from the
+// molecules' first atoms (their nearest-image vector taken in FP64), FP64 per-
+// lane sums, int64 fixed-point per-atom result.
+template <int APS, bool LJ00, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
+    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t k = gid / NCHM;
+    const int chunk = gid % NCHM;
+    double f[APS][3];
+    double en[11];
+#pragma unroll
+    for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 11; ++c) en[c] = 0.0;
+    int32_t e0 = 0, e1 = 0, sk = 0;
+    if (k < ncg) {
+        e0 = (int32_t)p.vv_off[k];
+        e1 = e0 + (int32_t)p.vv_cnt[k];
+        sk = p.s_cg_start[k];
+    }
+    if (e1 > e0) {
+        const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
+        const Box box = J.box;
+        const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
+        float qq[APS][APS];
+#pragma unroll
+        for (int i = 0; i < APS; ++i)
+#pragma unroll
+            for (int j = 0; j < APS; ++j) qq[i][j] = (float)J.sv_qq[i * 4 + j];
+        const float c6_00 = (float)J.sv_c6[0], c12_00 = (float)J.sv_c12[0];
+        float4 da[APS];
+#pragma unroll
+        for (int i = 0; i < APS; ++i) da[i] = p.xf[sk + i];
+        d3 pk0 = xyz(p.xq[sk]);
+        for (int32_t e = e0 + chunk; e < e1; e += NCHM) {
+            uint32_t ent = p.vv_ent[e];
+            int32_t sm = (int32_t)(ent & ENT_IDX);
+            const float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
+            float sx, sy, sz;
+            shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+            float4 db[APS];
+#pragma unroll
+            for (int j = 0; j < APS; ++j) db[j] = p.xf[sm + j];
+            float el = 0.f, ec = 0.f;
+            float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < APS; ++i) {
+                float bx = sx + da[i].x, by = sy + da[i].y, bz = sz + da[i].z;
+                float gxs = 0.f, gys = 0.f, gzs = 0.f;
+#pragma unroll
+                for (int j = 0; j < APS; ++j) {
+                    float rx = bx - db[j].x, ry = by - db[j].y, rz = bz - db[j].z;
+                    float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+                    bool ok = !(r2 < 1e-10f);
+                    // hardware reciprocal square root (1 ulp) refined by one Newton step
+                    float y = __builtin_amdgcn_rsqf(r2);
+                    float inv_r = fmaf(0.5f * y, fmaf(-r2 * y, y, 1.0f), y);
+                    float inv_r2 = inv_r * inv_r;
+                    float q = qq[i][j];
+                    float ecr = q * (inv_r - fmaf(c2, r2, cc));
+                    float fm = q * fmaf(inv_r, inv_r2, c3);
+                    float elj = 0.f;
+                    if (has_lj<APS, LJ00>(i, j)) {
+                        float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
+                        float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
+                        float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                        elj = fmaf(c12, inv_r6, -c6) * inv_r6;
+                        fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                    }
+                    fm = ok ? fm : 0.0f;
                     float gx = rx * fm, gy = ry * fm, gz = rz * fm;
                     gxs += gx;
                     gys += gy;
                     gzs += gz;
-                    el += elj;
-                    ec += ecrf;
+                    el += ok ? elj : 0.0f;
+                    ec += ok ? ecr : 0.0f;
                     if (VIR) {
                         w[0] = fmaf(rx, gx, w[0]); w[1] = fmaf(rx, gy, w[1]); w[2] = fmaf(rx, gz, w[2]);
                         w[3] = fmaf(ry, gx, w[3]); w[4] = fmaf(ry, gy, w[4]); w[5] = fmaf(ry, gz, w[5]);
                         w[6] = fmaf(rz, gx, w[6]); w[7] = fmaf(rz, gy, w[7]); w[8] = fmaf(rz, gz, w[8]);
                     }
                 }
-                if (VIR) {
+                f[i][0] += (double)gxs;
+                f[i][1] += (double)gys;
+                f[i][2] += (double)gzs;
+            }
+            en[0] += (double)(canon * el);
+            en[1] += (double)(canon * ec);
+            if (VIR) {
 #pragma unroll
-                    for (int c = 0; c < 9; ++c) v[2 + c] += (double)(canon * w[c]);
-                }
-                fx += (double)gxs;
-                fy += (double)gys;
-                fz += (double)gzs;
-                v[0] += (double)(canon * el);
-                v[1] += (double)(canon * ec);
+                for (int c = 0; c < 9; ++c) en[2 + c] += (double)(canon * w[c]);
             }
         }
     }
-    long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
+    long long fi[APS][3];
 #pragma unroll
-    for (int m = 1; m < NCH; m <<= 1) {
-        ix += __shfl_xor(ix, m, WAVE);
-        iy += __shfl_xor(iy, m, WAVE);
-        iz += __shfl_xor(iz, m, WAVE);
+    for (int i = 0; i < APS; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fi[i][c] = to_fx(f[i][c]);
+#pragma unroll
+    for (int m = 1; m < NCHM; m <<= 1) {
+#pragma unroll
+        for (int i = 0; i < APS; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fi[i][c] += __shfl_xor(fi[i][c], m, WAVE);
+#pragma unroll
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) en[c] += __shfl_xor(en[c], m, WAVE);
     }
-    if (chunk == 0) {
-        p.fi_s[3 * a] = ix;
-        p.fi_s[3 * a + 1] = iy;
-        p.fi_s[3 * a + 2] = iz;
+    if (chunk == 0 && k < ncg) {
+        int32_t nk = p.s_cg_n[k];
+        if (e1 > e0) {
+#pragma unroll
+            for (int i = 0; i < APS; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p.fi_s[3 * (sk + i) + c] = fi[i][c];
+        } else {
+            for (int32_t i = 0; i < 3 * nk; ++i) p.fi_s[3 * sk + i] = 0;
+        }
+#pragma unroll
+        for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = en[c];
     }
-    write_row(v, p.rows, SLOT_VV, p.nrow_blocks);
 }
 
 // Topology::is_excluded_or_14 (topology.rs:853-865) on original atom indices
@@ -1266,11 +1344,17 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
     const JobDev& J = p.jobs[jb];
     const int32_t stride = p.nrow_blocks;
     double v[W] = {0};
+    // solvent-solvent pairs: per-group records over the job's sorted groups
+    for (int32_t k = J.cg_off + threadIdx.x; k < J.cg_off + J.n_cg; k += blockDim.x) {
+        const double* r = p.vv_e + 11 * (int64_t)k;
+#pragma unroll
+        for (int c = 0; c < 11; ++c) v[c] += r[c];
+    }
     for (int s = 0; s < NSLOT; ++s) {
+        if (s == SLOT_VV) continue;
         if (s == SLOT_LONG && !with_long) continue;
         if (s == SLOT_KIN && !with_kin) continue;
-        // the pair kernel runs NCH lanes per atom, the others one
-        int32_t per = (s == SLOT_VV) ? BLOCK / NCH : BLOCK;
+        const int32_t per = BLOCK;
         int32_t b0 = J.atom_off / per;
         int32_t b1 = (J.atom_off + J.n_atoms + per - 1) / per;
         for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
@@ -1378,23 +1462,28 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
 }
-template <int APS>
-static void launch_vv_aps(const SysPtrs& p, int32_t npad, int32_t virial, int32_t mixed, hipStream_t s) {
-    dim3 g(grid_for((int64_t)npad * NCH));
+template <int APS, bool LJ00>
+static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipStream_t s) {
+    dim3 g(grid_for((int64_t)ncg * NCHM));
     if (mixed) {
-        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, true>), g, dim3(BLOCK), 0, s, p, npad);
-        else hipLaunchKernelGGL((k_vv_mx<APS, false>), g, dim3(BLOCK), 0, s, p, npad);
+        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, LJ00, true>), g, dim3(BLOCK), 0, s, p, ncg);
+        else hipLaunchKernelGGL((k_vv_mx<APS, LJ00, false>), g, dim3(BLOCK), 0, s, p, ncg);
     } else {
-        if (virial) hipLaunchKernelGGL((k_vv<APS, true>), g, dim3(BLOCK), 0, s, p, npad);
-        else hipLaunchKernelGGL((k_vv<APS, false>), g, dim3(BLOCK), 0, s, p, npad);
+        if (virial) hipLaunchKernelGGL((k_vv<APS, LJ00, true>), g, dim3(BLOCK), 0, s, p, ncg);
+        else hipLaunchKernelGGL((k_vv<APS, LJ00, false>), g, dim3(BLOCK), 0, s, p, ncg);
     }
 }
-void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s) {
-    switch (aps) {
-        case 1: launch_vv_aps<1>(p, npad, virial, mixed, s); break;
-        case 2: launch_vv_aps<2>(p, npad, virial, mixed, s); break;
-        case 4: launch_vv_aps<4>(p, npad, virial, mixed, s); break;
-        default: launch_vv_aps<3>(p, npad, virial, mixed, s); break;
+void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
+               hipStream_t s) {
+    switch (aps * 2 + (lj00 ? 1 : 0)) {
+        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, s); break;
+        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, s); break;
+        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, s); break;
+        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, s); break;
+        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, s); break;
+        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, s); break;
+        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, s); break;
+        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, s); break;
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {

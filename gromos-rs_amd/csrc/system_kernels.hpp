@@ -70,6 +70,7 @@ struct SysPtrs {
     double* f_long_s;
     double* f_x;
     double* rows;
+    double* vv_e;              // per sorted group: solvent-solvent e_lj, e_crf, virial[9]
     double* records;
     int32_t* settle_fail;      // per job: first failing step
     double* therm_scale;       // per job: Berendsen velocity scale of the current step
@@ -96,7 +97,8 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
 // mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
-void launch_vv(const SysPtrs& p, int32_t npad, int32_t aps, int32_t virial, int32_t mixed, hipStream_t s);
+void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
+               hipStream_t s);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);

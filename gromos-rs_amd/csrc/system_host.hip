@@ -30,12 +30,13 @@ int check_box(int32_t boundary, const double* box) {
     return 0;
 }
 
-// cells per axis: the largest count whose edge still exceeds the list cutoff
-// (with a relative margin so that rounding in the wrapped coordinates can never
-// push an in-range pair two cells apart); 1 when the box is smaller.
+// cells per axis: the largest count whose edge still exceeds half the list
+// cutoff (with a relative margin so that rounding in the wrapped coordinates can
+// never push an in-range pair three cells apart; the search takes two cells on
+// each side); 1 when the box is smaller.
 static int32_t cells_for(double L, double rl) {
     if (!(rl > 0.0)) return 1;
-    int64_t n = (int64_t)std::floor(L / (rl * (1.0 + 1e-9) + 1e-12));
+    int64_t n = (int64_t)std::floor(L / (0.5 * rl * (1.0 + 1e-9) + 1e-12));
     if (n < 1) n = 1;
     if (n > 1024) n = 1024;
     return (int32_t)n;

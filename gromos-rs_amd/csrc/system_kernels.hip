@@ -116,6 +116,8 @@ __device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
 // squared distance is the reference's whichever group comes first.
 // FILL = false: count only.  FILL = true: write at most cap[k] entries per
 // group, record the true counts and raise *overflow when a group had more.
+constexpr int CELL_R = 2;   // neighbour cells searched along each axis
+
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -131,21 +133,19 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
     int32_t local = (int32_t)p.key_out[k] - J.cell_off;
     int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
-    int32_t nd[3] = {nx, ny, nz};
-    int32_t lst[3][3], nl[3];
+    // cells are at least half the list cutoff wide: every partner lies within
+    // two cells along each axis (all cells along an axis with fewer than five)
+    int32_t lyz[2][2 * CELL_R + 1], nyz[2];
+    const int32_t nd[2] = {ny, nz};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 2; ++a) {
         int32_t n = nd[a];
-        if (n >= 3) {
-            nl[a] = 3;
-            lst[a][0] = (cc[a] + n - 1) % n;
-            lst[a][1] = cc[a];
-            lst[a][2] = (cc[a] + 1) % n;
+        if (n >= 2 * CELL_R + 1) {
+            nyz[a] = 2 * CELL_R + 1;
+            for (int d = -CELL_R; d <= CELL_R; ++d) lyz[a][d + CELL_R] = (cc[a + 1] + n + d) % n;
         } else {
-            nl[a] = n;
-            lst[a][0] = 0;
-            lst[a][1] = 1;
-            lst[a][2] = 2;
+            nyz[a] = n;
+            for (int d = 0; d < n; ++d) lyz[a][d] = d;
         }
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -159,53 +159,69 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     }
     uint64_t pairs_short = 0, pairs_long = 0;
     int32_t nk = p.s_cg_n[k];
-    for (int32_t iz = 0; iz < nl[2]; ++iz)
-        for (int32_t iy = 0; iy < nl[1]; ++iy)
-            for (int32_t ix = 0; ix < nl[0]; ++ix) {
-                int32_t cell = J.cell_off + lst[0][ix] + nx * (lst[1][iy] + ny * lst[2][iz]);
-                int32_t m0 = p.cell_start[cell], m1 = p.cell_start[cell + 1];
-                for (int32_t base = m0; base < m1; base += WAVE) {
-                    int32_t m = base + lane;
-                    bool is_vv = false, is_gen = false;
-                    uint32_t ent = 0;
-                    if (m < m1 && m != k) {
-                        d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
-                        d3 r = nim(box, rk, rm);
-                        double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
-                        if (!(d2 > c2l)) {
-                            int32_t cm = p.s_cg_orig[m];
-                            bool canon = ck < cm;
-                            bool lng = d2 > c2s;
-                            bool sol_m = cm - cg_off < nsol;
-                            uint32_t flags = canon ? ENT_CANON : 0u;
-                            if (!sol_k && !sol_m && !lng) {
-                                is_vv = true;
-                                // solvent partners: store the sorted first-atom index
-                                ent = (uint32_t)p.s_cg_start[m] | flags;
-                                if (canon) pairs_short += (uint64_t)aps * aps;
-                            } else {
-                                is_gen = true;
-                                if (!sol_k && !sol_m) flags |= ENT_VVLONG;
-                                if (lng) flags |= ENT_LONG;
-                                if (sol_k && sol_m) flags |= ENT_SS;
-                                ent = (uint32_t)m | flags;
-                                if (canon) {
-                                    uint64_t np = (uint64_t)nk * p.s_cg_n[m];
-                                    if (lng) pairs_long += np; else pairs_short += np;
-                                }
-                            }
+    // candidates of a run of sorted groups [m0, m1), 64 at a time
+    auto scan_run = [&](int32_t m0, int32_t m1) {
+        for (int32_t base = m0; base < m1; base += WAVE) {
+            int32_t m = base + lane;
+            bool is_vv = false, is_gen = false;
+            uint32_t ent = 0;
+            if (m < m1 && m != k) {
+                d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
+                d3 r = nim(box, rk, rm);
+                double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                if (!(d2 > c2l)) {
+                    int32_t cm = p.s_cg_orig[m];
+                    bool canon = ck < cm;
+                    bool lng = d2 > c2s;
+                    bool sol_m = cm - cg_off < nsol;
+                    uint32_t flags = canon ? ENT_CANON : 0u;
+                    if (!sol_k && !sol_m && !lng) {
+                        is_vv = true;
+                        // solvent partners: store the sorted first-atom index
+                        ent = (uint32_t)p.s_cg_start[m] | flags;
+                        if (canon) pairs_short += (uint64_t)aps * aps;
+                    } else {
+                        is_gen = true;
+                        if (!sol_k && !sol_m) flags |= ENT_VVLONG;
+                        if (lng) flags |= ENT_LONG;
+                        if (sol_k && sol_m) flags |= ENT_SS;
+                        ent = (uint32_t)m | flags;
+                        if (canon) {
+                            uint64_t np = (uint64_t)nk * p.s_cg_n[m];
+                            if (lng) pairs_long += np; else pairs_short += np;
                         }
                     }
-                    uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
-                    if (FILL) {
-                        int64_t pv = nvv + __popcll(bv & below), pg = ngen + __popcll(bg & below);
-                        if (is_vv && pv < vv_cap) p.vv_ent[vv_base + pv] = ent;
-                        if (is_gen && pg < gen_cap) p.gen_ent[gen_base + pg] = ent;
-                    }
-                    nvv += __popcll(bv);
-                    ngen += __popcll(bg);
                 }
             }
+            uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
+            if (FILL) {
+                int64_t pv = nvv + __popcll(bv & below), pg = ngen + __popcll(bg & below);
+                if (is_vv && pv < vv_cap) p.vv_ent[vv_base + pv] = ent;
+                if (is_gen && pg < gen_cap) p.gen_ent[gen_base + pg] = ent;
+            }
+            nvv += __popcll(bv);
+            ngen += __popcll(bg);
+        }
+    };
+    for (int32_t iz = 0; iz < nyz[1]; ++iz)
+        for (int32_t iy = 0; iy < nyz[0]; ++iy) {
+            // the x cells of one (y, z) row are contiguous in sorted order
+            int32_t row = J.cell_off + nx * (lyz[0][iy] + ny * lyz[1][iz]);
+            if (nx < 2 * CELL_R + 1) {
+                scan_run(p.cell_start[row], p.cell_start[row + nx]);
+            } else {
+                int32_t x0 = cc[0] - CELL_R, x1 = cc[0] + CELL_R;
+                if (x0 < 0) {
+                    scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
+                    scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
+                } else if (x1 >= nx) {
+                    scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
+                    scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
+                } else {
+                    scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
+                }
+            }
+        }
     // per-group results; totals come from k_totals (one atomic per block there:
     // thousands of waves adding to the same address serialise at the L2)
 #pragma unroll

@@ -95,6 +95,8 @@ struct gromos_amd_system {
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
     double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr;
     int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
+    int32_t vv_lanes = 0;  // lanes per solvent molecule in the pair kernel (0: not tuned yet)
+    int64_t tuned_key = -1;
     int32_t* d_settle_fail = nullptr;
     double* d_therm_scale = nullptr;
     bool any_thermo = false;

@@ -3,6 +3,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "host.hpp"
 
@@ -554,6 +557,64 @@ int build_pairlist(gromos_amd_system* S) {
     return 0;
 }
 
+// Autotuner of the solvent pair kernel: lanes per molecule (4, 8, 16) chosen by
+// timing one launch of each on the batch itself, the first time a batch shape
+// (precision, molecule size, LJ layout, virial, log2 of the group count) is met
+// in this process; GROMOS_AMD_VV_LANES forces a value.  The launches compute the
+// same forces, so the step that tunes is an ordinary step.
+static std::mutex g_tune_mu;
+static std::map<int64_t, int32_t> g_tuned;
+
+static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t mixed) {
+    int32_t bucket = 0;
+    while ((1 << bucket) < S->ncg) ++bucket;
+    int64_t key = (((((int64_t)bucket * 8 + aps) * 2 + mixed) * 2 + S->vv_lj00) * 2 + S->do_virial);
+    if (S->vv_lanes && S->tuned_key == key) return 0;
+    if (const char* env = std::getenv("GROMOS_AMD_VV_LANES")) {
+        int v = std::atoi(env);
+        S->vv_lanes = (v == 4 || v == 16) ? v : 8;
+        S->tuned_key = key;
+        return 0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            S->vv_lanes = it->second;
+            S->tuned_key = key;
+            return 0;
+        }
+    }
+    const int32_t cand[3] = {4, 8, 16};
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], S->stream));
+    for (int c = 0; c < 3; ++c) {
+        launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, cand[c], S->stream);
+        HIPCHK(hipEventRecord(ev[c + 1], S->stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[3]));
+    int32_t best = 8;
+    float best_ms = 1e30f;
+    for (int c = 0; c < 3; ++c) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[c], ev[c + 1]));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cand[c];
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    HIPCHK(hipGetLastError());
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = best;
+    }
+    S->vv_lanes = best;
+    S->tuned_key = key;
+    return 0;
+}
+
 // Forcefield::apply nonbonded part (forcefield.rs:226-1060) into d_f; energy
 // partials into the rows
 int forcefield(gromos_amd_system* S) {
@@ -582,7 +643,8 @@ int forcefield(gromos_amd_system* S) {
             aps = J.aps;
             break;
         }
-    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, st);
+    if (int rc = tune_vv(S, p, aps, mixed)) return rc;
+    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, S->vv_lanes, st);
     if (S->timing) HIPCHK(hipEventRecord(e1, st));
     launch_gen(p, npad, false, S->do_virial, mixed, st);
     S->pair_count += (int64_t)S->host_counters[0];

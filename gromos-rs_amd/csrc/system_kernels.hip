@@ -374,15 +374,15 @@ __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, floa
 // virial.  LJ00: only the first-first atom pair has LJ parameters (water
 // models whose hydrogens carry none); the other pairs' LJ terms are exactly
 // zero in the reference and are skipped.  Energies and virial go to a per-group
-// record (vv_e, 11 doubles) that k_reduce sums per job.
-constexpr int NCHM = 8;
+// record (vv_e, 11 doubles) that k_reduce sums per job.  NCHM (lanes per
+// molecule: 4, 8 or 16) is chosen per batch shape by timing (autotune).
 
 template <int APS, bool LJ00>
 __device__ __forceinline__ bool has_lj(int i, int j) {
     return !LJ00 || (i == 0 && j == 0);
 }
 
-template <int APS, bool LJ00, bool VIR>
+template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
     const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     const int32_t k = gid / NCHM;
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 // Mixed-precision counterpart: FP32 pair arithmetic on offsets from the
 // molecules' first atoms (their nearest-image vector taken in FP64), FP64 per-
 // lane sums, int64 fixed-point per-atom result.
-template <int APS, bool LJ00, bool VIR>
+template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
     const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     const int32_t k = gid / NCHM;
@@ -1478,28 +1478,35 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
 }
-template <int APS, bool LJ00>
-static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipStream_t s) {
-    dim3 g(grid_for((int64_t)ncg * NCHM));
+template <int APS, bool LJ00, int L>
+static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipStream_t s) {
+    dim3 g(grid_for((int64_t)ncg * L));
     if (mixed) {
-        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, LJ00, true>), g, dim3(BLOCK), 0, s, p, ncg);
-        else hipLaunchKernelGGL((k_vv_mx<APS, LJ00, false>), g, dim3(BLOCK), 0, s, p, ncg);
+        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, p, ncg);
+        else hipLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, p, ncg);
     } else {
-        if (virial) hipLaunchKernelGGL((k_vv<APS, LJ00, true>), g, dim3(BLOCK), 0, s, p, ncg);
-        else hipLaunchKernelGGL((k_vv<APS, LJ00, false>), g, dim3(BLOCK), 0, s, p, ncg);
+        if (virial) hipLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, p, ncg);
+        else hipLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, p, ncg);
     }
 }
+template <int APS, bool LJ00>
+static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t lanes,
+                          hipStream_t s) {
+    if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, s);
+    else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, s);
+    else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, s);
+}
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
-               hipStream_t s) {
+               int32_t lanes, hipStream_t s) {
     switch (aps * 2 + (lj00 ? 1 : 0)) {
-        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, s); break;
-        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, s); break;
-        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, s); break;
-        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, s); break;
-        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, s); break;
-        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, s); break;
-        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, s); break;
-        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, s); break;
+        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, lanes, s); break;
+        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, lanes, s); break;
+        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, lanes, s); break;
+        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, lanes, s); break;
+        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, lanes, s); break;
+        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, lanes, s); break;
+        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, lanes, s); break;
+        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, lanes, s); break;
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {

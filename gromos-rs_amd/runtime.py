@@ -126,6 +126,8 @@ def _declare(L):
     L.gromos_amd_system_kernel_timing.argtypes = [P, I32, P, P]
     L.gromos_amd_system_device_bytes.argtypes = [P]
     L.gromos_amd_system_device_bytes.restype = I64
+    L.gromos_amd_system_vv_lanes.argtypes = [P]
+    L.gromos_amd_system_vv_lanes.restype = I32
     L.gromos_amd_session_create.argtypes = [C.POINTER(SessionParams), C.POINTER(P)]
     L.gromos_amd_session_destroy.argtypes = [P]
     L.gromos_amd_job_footprint.argtypes = [C.POINTER(Topology), C.POINTER(Params)]
@@ -297,6 +299,11 @@ class DeviceSystem:
     @property
     def device_bytes(self):
         return lib().gromos_amd_system_device_bytes(self.h)
+
+    @property
+    def vv_lanes(self):
+        """lanes per solvent molecule chosen by the autotuner"""
+        return lib().gromos_amd_system_vv_lanes(self.h)
 
     def kernel_timing(self, enable=-1):
         ms, n = D(), I64()

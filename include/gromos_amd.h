@@ -238,6 +238,10 @@ int gromos_amd_system_kernel_timing(gromos_amd_system *sys, int32_t enable, doub
                                     int64_t *launches);
 /* HBM bytes held by the system's device buffers. */
 int64_t gromos_amd_system_device_bytes(const gromos_amd_system *sys);
+/* Lanes per solvent molecule the autotuner chose for the pair kernel (4, 8, 16;
+ * 0 before the first force evaluation).  GROMOS_AMD_VV_LANES in the environment
+ * forces a value. */
+int32_t gromos_amd_system_vv_lanes(const gromos_amd_system *sys);
 
 /* ------------------------------------------------------------------ */
 /* 3. sessions: continuous batching of independent jobs on one device  */

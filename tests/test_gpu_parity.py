@@ -377,3 +377,35 @@ def test_berendsen_md_matches_oracle(tau):
             assert rel(ge[k, c], oe[k, c]) < 1e-10, (k, c, ge[k, c], oe[k, c])
     assert np.all(np.abs(ge[:, 3] - oe[:, 3]) / oe[:, 3] < 1e-7)
     ds.close()
+
+
+# --------------------------------------------------------------------------
+# every lane count the autotuner may pick gives the oracle's result
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_vv_lane_variants(lanes, prec, monkeypatch):
+    monkeypatch.setenv("GROMOS_AMD_VV_LANES", str(lanes))
+    s = gen.spc_water_box(1500, seed=12)
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    g = ds.forcefield_apply()
+    assert ds.vv_lanes == lanes
+    ds.close()
+    r = oracle.single_point(oracle.OracleSystem(s))
+    if prec == rt.FP64:
+        assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+        assert force_err(g["forces"], r["forces"]) < F_TOL
+    else:
+        mixed_energy_ok(g, r)
+        assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+
+
+def test_autotuner_picks_a_variant():
+    s = gen.spc_water_box(3000, seed=13)
+    ds = rt.DeviceSystem(s, precision=rt.MIXED)
+    assert ds.vv_lanes == 0
+    ds.pairlist_update()
+    ds.forcefield_apply()
+    assert ds.vv_lanes in (4, 8, 16)
+    ds.close()

@@ -557,11 +557,12 @@ int build_pairlist(gromos_amd_system* S) {
     return 0;
 }
 
-// Autotuner of the solvent pair kernel: lanes per molecule (4, 8, 16) chosen by
-// timing one launch of each on the batch itself, the first time a batch shape
-// (precision, molecule size, LJ layout, virial, log2 of the group count) is met
-// in this process; GROMOS_AMD_VV_LANES forces a value.  The launches compute the
-// same forces, so the step that tunes is an ordinary step.
+// Autotuner of the mixed-precision solvent pair kernel: lanes per molecule (4, 8,
+// 16) chosen by timing one launch of each on the batch itself, the first time a
+// batch shape (molecule size, LJ layout, virial, log2 of the group count) is met
+// in this process; GROMOS_AMD_VV_LANES forces a value.  That kernel adds fixed-
+// point groups of entries, so every choice gives the same bits and the step
+// that tunes is an ordinary step.
 static std::mutex g_tune_mu;
 static std::map<int64_t, int32_t> g_tuned;
 
@@ -573,6 +574,13 @@ static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t 
     if (const char* env = std::getenv("GROMOS_AMD_VV_LANES")) {
         int v = std::atoi(env);
         S->vv_lanes = (v == 4 || v == 16) ? v : 8;
+        S->tuned_key = key;
+        return 0;
+    }
+    if (!mixed) {
+        // the FP64 path sums each lane's pairs in FP64, so its bits depend on the
+        // lane count: keep one fixed count (results independent of the batch)
+        S->vv_lanes = 8;
         S->tuned_key = key;
         return 0;
     }

@@ -326,6 +326,9 @@ __device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int sl
 // ---- int64 fixed-point per-atom totals (exact and order-independent)
 constexpr double FX_SCALE = 16777216.0;          // 2^24 per kJ mol^-1 nm^-1
 constexpr double FX_INV = 1.0 / 16777216.0;
+constexpr double EX_SCALE = 4294967296.0;        // 2^32 per kJ mol^-1 (energies, virial)
+constexpr double EX_INV = 1.0 / 4294967296.0;
+constexpr int VV_GROUP = 8;                      // entries per fixed-point group
 
 __device__ __forceinline__ long long to_fx(double v) { return (long long)rint(v * FX_SCALE); }
 
@@ -491,12 +494,12 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
     const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
     const int32_t k = gid / NCHM;
     const int chunk = gid % NCHM;
-    double f[APS][3];
-    double en[11];
+    long long fi[APS][3];
+    long long ei[11];
 #pragma unroll
-    for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
+    for (int i = 0; i < APS; ++i) fi[i][0] = fi[i][1] = fi[i][2] = 0;
 #pragma unroll
-    for (int c = 0; c < 11; ++c) en[c] = 0.0;
+    for (int c = 0; c < 11; ++c) ei[c] = 0;
     int32_t e0 = 0, e1 = 0, sk = 0;
     if (k < ncg) {
         e0 = (int32_t)p.vv_off[k];
@@ -517,71 +520,85 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
 #pragma unroll
         for (int i = 0; i < APS; ++i) da[i] = p.xf[sk + i];
         d3 pk0 = xyz(p.xq[sk]);
-        for (int32_t e = e0 + chunk; e < e1; e += NCHM) {
-            uint32_t ent = p.vv_ent[e];
-            int32_t sm = (int32_t)(ent & ENT_IDX);
-            const float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
-            float sx, sy, sz;
-            shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
-            float4 db[APS];
+        const int32_t n = e1 - e0;
+        // Groups of VV_GROUP consecutive entries are summed in FP64 and then
+        // added as fixed point: the result does not depend on how groups are
+        // spread over the lanes, so every lane count gives the same bits.
+        for (int32_t g0 = chunk * VV_GROUP; g0 < n; g0 += NCHM * VV_GROUP) {
+            double f[APS][3];
+            double en[11];
 #pragma unroll
-            for (int j = 0; j < APS; ++j) db[j] = p.xf[sm + j];
-            float el = 0.f, ec = 0.f;
-            float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
 #pragma unroll
-            for (int i = 0; i < APS; ++i) {
-                float bx = sx + da[i].x, by = sy + da[i].y, bz = sz + da[i].z;
-                float gxs = 0.f, gys = 0.f, gzs = 0.f;
+            for (int c = 0; c < 11; ++c) en[c] = 0.0;
+            const int32_t g1 = min(g0 + VV_GROUP, n);
+            for (int32_t e = e0 + g0; e < e0 + g1; ++e) {
+                uint32_t ent = p.vv_ent[e];
+                int32_t sm = (int32_t)(ent & ENT_IDX);
+                const float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
+                float sx, sy, sz;
+                shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+                float4 db[APS];
 #pragma unroll
-                for (int j = 0; j < APS; ++j) {
-                    float rx = bx - db[j].x, ry = by - db[j].y, rz = bz - db[j].z;
-                    float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-                    bool ok = !(r2 < 1e-10f);
-                    // hardware reciprocal square root (1 ulp) refined by one Newton step
-                    float y = __builtin_amdgcn_rsqf(r2);
-                    float inv_r = fmaf(0.5f * y, fmaf(-r2 * y, y, 1.0f), y);
-                    float inv_r2 = inv_r * inv_r;
-                    float q = qq[i][j];
-                    float ecr = q * (inv_r - fmaf(c2, r2, cc));
-                    float fm = q * fmaf(inv_r, inv_r2, c3);
-                    float elj = 0.f;
-                    if (has_lj<APS, LJ00>(i, j)) {
-                        float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
-                        float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
-                        float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                        elj = fmaf(c12, inv_r6, -c6) * inv_r6;
-                        fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                for (int j = 0; j < APS; ++j) db[j] = p.xf[sm + j];
+                float el = 0.f, ec = 0.f;
+                float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int i = 0; i < APS; ++i) {
+                    float bx = sx + da[i].x, by = sy + da[i].y, bz = sz + da[i].z;
+                    float gxs = 0.f, gys = 0.f, gzs = 0.f;
+#pragma unroll
+                    for (int j = 0; j < APS; ++j) {
+                        float rx = bx - db[j].x, ry = by - db[j].y, rz = bz - db[j].z;
+                        float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+                        bool ok = !(r2 < 1e-10f);
+                        // hardware reciprocal square root (1 ulp) refined by one Newton step
+                        float y = __builtin_amdgcn_rsqf(r2);
+                        float inv_r = fmaf(0.5f * y, fmaf(-r2 * y, y, 1.0f), y);
+                        float inv_r2 = inv_r * inv_r;
+                        float q = qq[i][j];
+                        float ecr = q * (inv_r - fmaf(c2, r2, cc));
+                        float fm = q * fmaf(inv_r, inv_r2, c3);
+                        float elj = 0.f;
+                        if (has_lj<APS, LJ00>(i, j)) {
+                            float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
+                            float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
+                            float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                            elj = fmaf(c12, inv_r6, -c6) * inv_r6;
+                            fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                        }
+                        fm = ok ? fm : 0.0f;
+                        float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                        gxs += gx;
+                        gys += gy;
+                        gzs += gz;
+                        el += ok ? elj : 0.0f;
+                        ec += ok ? ecr : 0.0f;
+                        if (VIR) {
+                            w[0] = fmaf(rx, gx, w[0]); w[1] = fmaf(rx, gy, w[1]); w[2] = fmaf(rx, gz, w[2]);
+                            w[3] = fmaf(ry, gx, w[3]); w[4] = fmaf(ry, gy, w[4]); w[5] = fmaf(ry, gz, w[5]);
+                            w[6] = fmaf(rz, gx, w[6]); w[7] = fmaf(rz, gy, w[7]); w[8] = fmaf(rz, gz, w[8]);
+                        }
                     }
-                    fm = ok ? fm : 0.0f;
-                    float gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                    gxs += gx;
-                    gys += gy;
-                    gzs += gz;
-                    el += ok ? elj : 0.0f;
-                    ec += ok ? ecr : 0.0f;
-                    if (VIR) {
-                        w[0] = fmaf(rx, gx, w[0]); w[1] = fmaf(rx, gy, w[1]); w[2] = fmaf(rx, gz, w[2]);
-                        w[3] = fmaf(ry, gx, w[3]); w[4] = fmaf(ry, gy, w[4]); w[5] = fmaf(ry, gz, w[5]);
-                        w[6] = fmaf(rz, gx, w[6]); w[7] = fmaf(rz, gy, w[7]); w[8] = fmaf(rz, gz, w[8]);
-                    }
+                    f[i][0] += (double)gxs;
+                    f[i][1] += (double)gys;
+                    f[i][2] += (double)gzs;
                 }
-                f[i][0] += (double)gxs;
-                f[i][1] += (double)gys;
-                f[i][2] += (double)gzs;
-            }
-            en[0] += (double)(canon * el);
-            en[1] += (double)(canon * ec);
-            if (VIR) {
+                en[0] += (double)(canon * el);
+                en[1] += (double)(canon * ec);
+                if (VIR) {
 #pragma unroll
-                for (int c = 0; c < 9; ++c) en[2 + c] += (double)(canon * w[c]);
+                    for (int c = 0; c < 9; ++c) en[2 + c] += (double)(canon * w[c]);
+                }
             }
+#pragma unroll
+            for (int i = 0; i < APS; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) fi[i][c] += to_fx(f[i][c]);
+#pragma unroll
+            for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += (long long)rint(en[c] * EX_SCALE);
         }
     }
-    long long fi[APS][3];
-#pragma unroll
-    for (int i = 0; i < APS; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) fi[i][c] = to_fx(f[i][c]);
 #pragma unroll
     for (int m = 1; m < NCHM; m <<= 1) {
 #pragma unroll
@@ -589,7 +606,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) fi[i][c] += __shfl_xor(fi[i][c], m, WAVE);
 #pragma unroll
-        for (int c = 0; c < (VIR ? 11 : 2); ++c) en[c] += __shfl_xor(en[c], m, WAVE);
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += __shfl_xor(ei[c], m, WAVE);
     }
     if (chunk == 0 && k < ncg) {
         int32_t nk = p.s_cg_n[k];
@@ -602,7 +619,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
             for (int32_t i = 0; i < 3 * nk; ++i) p.fi_s[3 * sk + i] = 0;
         }
 #pragma unroll
-        for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = en[c];
+        for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = (double)ei[c] * EX_INV;
     }
 }
 

@@ -409,3 +409,17 @@ def test_autotuner_picks_a_variant():
     ds.forcefield_apply()
     assert ds.vv_lanes in (4, 8, 16)
     ds.close()
+
+
+def test_mixed_bits_do_not_depend_on_lane_count(monkeypatch):
+    s = gen.spc_water_box(1500, seed=14)
+    out = []
+    for lanes in (4, 8, 16):
+        monkeypatch.setenv("GROMOS_AMD_VV_LANES", str(lanes))
+        ds = rt.DeviceSystem(s, precision=rt.MIXED)
+        ds.pairlist_update()
+        out.append(ds.forcefield_apply())
+        ds.close()
+    for g in out[1:]:
+        assert np.array_equal(g["forces"], out[0]["forces"])
+        assert g["e_lj"] == out[0]["e_lj"] and g["e_crf"] == out[0]["e_crf"]

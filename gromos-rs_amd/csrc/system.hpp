@@ -63,4 +63,7 @@ enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 
 // per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]
 constexpr int REC = 14;
 
+// parts per job of the solvent-solvent energy records (two-stage reduction)
+constexpr int VV_PARTS = 32;
+
 }  // namespace gamd

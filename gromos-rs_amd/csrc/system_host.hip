@@ -440,7 +440,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
-        (rc = S->reserve(S->d_vv_e, 11 * nc, true)))
+        (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
+        (rc = S->reserve(S->d_vv_part, (size_t)11 * VV_PARTS * njobs, true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -643,7 +644,6 @@ int forcefield(gromos_amd_system* S) {
     if (S->timing) {
         e0 = S->next_event();
         e1 = S->next_event();
-        HIPCHK(hipEventRecord(e0, st));
     }
     int32_t aps = S->jobs[0].aps > 0 ? S->jobs[0].aps : 3;
     for (const JobDev& J : S->jobs)
@@ -652,8 +652,7 @@ int forcefield(gromos_amd_system* S) {
             break;
         }
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
-    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, S->vv_lanes, st);
-    if (S->timing) HIPCHK(hipEventRecord(e1, st));
+    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
     launch_gen(p, npad, false, S->do_virial, mixed, st);
     S->pair_count += (int64_t)S->host_counters[0];
     if (S->twin && (rebuild || !S->long_computed)) {
@@ -828,6 +827,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.f_x = d_f_x;
     p.rows = d_rows;
     p.vv_e = d_vv_e;
+    p.vv_part = d_vv_part;
+    p.vv_virial = do_virial;
     p.records = d_records;
     p.settle_fail = d_settle_fail;
     p.therm_scale = d_therm_scale;

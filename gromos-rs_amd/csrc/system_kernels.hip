@@ -1366,6 +1366,23 @@ __global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
     write_row(v, p.rows, SLOT_KIN, p.nrow_blocks);
 }
 
+// Per-group solvent-solvent records summed in VV_PARTS contiguous parts per job
+// (grid VV_PARTS x njobs), so that the per-job reduction reads few values.
+__global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_virial) {
+    __shared__ double lds[17 * 11];
+    const JobDev& J = p.jobs[blockIdx.y];
+    int64_t n = J.n_cg;
+    int64_t k0 = J.cg_off + n * blockIdx.x / VV_PARTS, k1 = J.cg_off + n * (blockIdx.x + 1) / VV_PARTS;
+    double v[11] = {0};
+    const int nc = with_virial ? 11 : 2;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const double* r = p.vv_e + 11 * k;
+        for (int c = 0; c < nc; ++c) v[c] += r[c];
+    }
+    block_sum<11>(v, lds);
+    if (threadIdx.x < 11) p.vv_part[((int64_t)blockIdx.y * VV_PARTS + blockIdx.x) * 11 + threadIdx.x] = lds[16 * 11 + threadIdx.x];
+}
+
 // One block per job: sums the job's partial rows in a fixed order and writes
 // the step record (e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]).  Column 11 of a
 // row is E_kin in the kinetic slot and dH/dlambda in the others.
@@ -1377,9 +1394,9 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
     const JobDev& J = p.jobs[jb];
     const int32_t stride = p.nrow_blocks;
     double v[W] = {0};
-    // solvent-solvent pairs: per-group records over the job's sorted groups
-    for (int32_t k = J.cg_off + threadIdx.x; k < J.cg_off + J.n_cg; k += blockDim.x) {
-        const double* r = p.vv_e + 11 * (int64_t)k;
+    // solvent-solvent pairs: the job's VV_PARTS partial sums of the per-group records
+    if (threadIdx.x < VV_PARTS) {
+        const double* r = p.vv_part + ((int64_t)jb * VV_PARTS + threadIdx.x) * 11;
 #pragma unroll
         for (int c = 0; c < 11; ++c) v[c] += r[c];
     }
@@ -1495,35 +1512,38 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
 }
+// start / stop events (optional) are recorded by the dispatch itself
+// (hipExtLaunchKernelGGL), so timing adds no packets between kernels
 template <int APS, bool LJ00, int L>
-static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipStream_t s) {
+static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipEvent_t e0,
+                        hipEvent_t e1, hipStream_t s) {
     dim3 g(grid_for((int64_t)ncg * L));
     if (mixed) {
-        if (virial) hipLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, p, ncg);
-        else hipLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, p, ncg);
+        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
     } else {
-        if (virial) hipLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, p, ncg);
-        else hipLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, p, ncg);
+        if (virial) hipExtLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        else hipExtLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
     }
 }
 template <int APS, bool LJ00>
 static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t lanes,
-                          hipStream_t s) {
-    if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, s);
-    else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, s);
-    else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, s);
+                          hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+    if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, e0, e1, s);
+    else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, e0, e1, s);
+    else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, e0, e1, s);
 }
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
-               int32_t lanes, hipStream_t s) {
+               int32_t lanes, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     switch (aps * 2 + (lj00 ? 1 : 0)) {
-        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, lanes, s); break;
-        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, lanes, s); break;
-        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, lanes, s); break;
-        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, lanes, s); break;
-        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, lanes, s); break;
-        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, lanes, s); break;
-        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, lanes, s); break;
-        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, lanes, s); break;
+        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {
@@ -1557,6 +1577,7 @@ void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s) {
 }
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
                    int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(BLOCK), 0, s, p, p.vv_virial);
     hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, step, njobs);
 }
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,

@@ -1,5 +1,6 @@
 // Device pointer bundle and launchers of the device-resident kernels.
 #pragma once
+#include <hip/hip_ext.h>
 #include "system.hpp"
 
 namespace gamd {
@@ -71,6 +72,8 @@ struct SysPtrs {
     double* f_x;
     double* rows;
     double* vv_e;              // per sorted group: solvent-solvent e_lj, e_crf, virial[9]
+    double* vv_part;           // per job: VV_PARTS partial sums of vv_e
+    int32_t vv_virial;         // vv_e carries the virial
     double* records;
     int32_t* settle_fail;      // per job: first failing step
     double* therm_scale;       // per job: Berendsen velocity scale of the current step
@@ -99,7 +102,7 @@ void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
 // lanes: lanes per solvent molecule (4, 8 or 16)
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
-               int32_t lanes, hipStream_t s);
+               int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);

@@ -72,7 +72,7 @@ class ReplicaExchange:
         out = []
         for j in self.jobs:
             done = self.sess.query(j)[1]
-            out.append(self.sess.energies(j, first=done - 1, count=1)[0])
+            out.append(self.sess.energies(j, first=done - 1, count=1)[0] if done > 0 else np.zeros(5))
         return np.array(out)
 
     def attempt_all_exchanges(self):
@@ -81,7 +81,10 @@ class ReplicaExchange:
         pairs, self.odd_even_state = exchange_pairs(len(self.jobs), self.scheme, self.odd_even_state)
         factors = np.ones(len(self.jobs))
         changed = []
+        states = [self.sess.query(j)[0] for j in self.jobs]
         for i, k in pairs:
+            if states[i] != rt.RUNNING or states[k] != rt.RUNNING:
+                continue     # a replica that stopped (failed or done) takes no part
             p, _ = exchange_probability(self.T[i], self.T[k], epot[i], epot[k])
             r = self.rng.random()
             acc = bool(r < p)

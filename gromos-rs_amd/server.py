@@ -95,3 +95,28 @@ def replay(server, schedule, chunk=None):
     return dict(wall=wall, pairs=info["pair_count"] - info0["pair_count"], layouts=info["layouts"] - info0["layouts"],
                 layout_ms=info["layout_ms"] - info0["layout_ms"], step_ms=step_ms, job_latency_s=lat,
                 job_latency_steps=lat_steps, clock=info["clock"])
+
+
+def equilibrate(systems, temperatures, steps=300, tau=0.01, precision=rt.MIXED, nsnb=5):
+    """Relax synthetic starting structures (lattice waters with random orientations
+    release a lot of potential energy in their first picoseconds): all systems
+    run together in one session under strong Berendsen coupling to their
+    temperatures, then their end states replace the starting ones.  Returns new
+    System objects (thermostat setting of the originals kept)."""
+    sess = rt.Session(precision=precision, nsnb=nsnb)
+    jobs = []
+    for s, t in zip(systems, temperatures):
+        q = s.copy()
+        q.thermostat = (float(t), float(tau), 3 * q.n_atoms - 3 * q.nsm)
+        jobs.append(sess.admit(q, steps))
+    sess.step(steps)
+    out = []
+    for s, j in zip(systems, jobs):
+        st, _ = sess.query(j)
+        if st != rt.DONE:
+            raise rt.GromosAmdError(f"equilibration of {s.name} did not finish (state {st})")
+        q = s.copy()
+        q.pos, q.vel = sess.fetch(j)
+        out.append(q)
+    sess.close()
+    return out

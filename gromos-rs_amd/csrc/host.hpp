@@ -93,7 +93,9 @@ struct gromos_amd_system {
     long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
     int32_t nrow_blocks = 0;
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
-    double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr, *d_vv_part = nullptr;
+    double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr, *d_vv_part = nullptr, *d_sol_e = nullptr;
+    int32_t* d_sol_atom = nullptr;
+    int32_t nsol = 0;
     int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
     int32_t vv_lanes = 0;  // lanes per solvent molecule in the pair kernel (0: not tuned yet)
     int64_t tuned_key = -1;

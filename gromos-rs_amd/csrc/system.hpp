@@ -51,6 +51,7 @@ struct JobDev {
                                            // of the current call (INT32_MAX: always)
     double dt;
     double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
+    int32_t sol_off;                       // first index of the job's solute atoms in sol_atom
     int32_t thermo;                        // Berendsen weak coupling (berendsen_thermostat.rs)
     double t_ref, tau_t, dof_t;
 };

@@ -269,6 +269,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     o14_off.push_back(0);
     o14s_off.push_back(0);
     bool twin = false, any_thermo = false;
+    std::vector<int32_t> sol_atom;
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -279,6 +280,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         J.lj_off = (int32_t)lj_off;
         J.cell_off = (int32_t)cell_off;
         J.stop = INT32_MAX;
+        J.sol_off = (int32_t)sol_atom.size();
+        for (int32_t a = 0; a < J.n_solute_atoms; ++a) sol_atom.push_back(A + a);
         cell_off += (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
         twin = twin || J.twin;
         any_thermo = any_thermo || J.thermo;
@@ -361,6 +364,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->cell_bits = 1;
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
     S->n_papairs = (int32_t)papairs.size() / 4;
+    S->nsol = (int32_t)sol_atom.size();
     S->has_pert = any_pert;
     // per-atom index of the PERTATOMPAIR entries (both atoms of an entry)
     std::vector<int32_t> pap_off, pap;
@@ -407,6 +411,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_palj, palj);
     rc = rc ? rc : S->upload(S->d_pacrf, pacrf);
     rc = rc ? rc : S->upload(S->d_cg_pert, cg_pert);
+    rc = rc ? rc : S->upload(S->d_sol_atom, sol_atom);
     rc = rc ? rc : S->upload(S->d_papairs, papairs);
     rc = rc ? rc : S->upload(S->d_pap_off, pap_off);
     rc = rc ? rc : S->upload(S->d_pap, pap);
@@ -441,7 +446,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
-        (rc = S->reserve(S->d_vv_part, (size_t)11 * VV_PARTS * njobs, true)))
+        (rc = S->reserve(S->d_vv_part, (size_t)11 * VV_PARTS * njobs, true)) ||
+        (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -828,6 +834,9 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.rows = d_rows;
     p.vv_e = d_vv_e;
     p.vv_part = d_vv_part;
+    p.sol_atom = d_sol_atom;
+    p.nsol = nsol;
+    p.sol_e = d_sol_e;
     p.vv_virial = do_virial;
     p.records = d_records;
     p.settle_fail = d_settle_fail;

@@ -842,8 +842,84 @@ __device__ __forceinline__ void gen_pair(const SysPtrs& p, const JobDev& J, doub
     }
 }
 
+// The pairs of one sorted atom a (original o) from its group's general list
+// and, for solute groups, its own group: entries e0 + lane0, e0 + lane0 + stride,
+// ... (one lane: lane0 = 0, stride = 1; a wave: lane0 = lane, stride = 64).
+template <bool LONG>
+__device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t o, int32_t lane0, int32_t stride,
+                                           int32_t virial, double (&v)[ROW], double& fx, double& fy, double& fz) {
+    int32_t k = p.s_atom_cg[a];
+    const JobDev& J = p.jobs[p.atom_job[o]];
+    int32_t sk = p.s_cg_start[k];
+    int32_t nk = p.s_cg_n[k];
+    double4 xa = p.xq[a];
+    d3 pa = xyz(xa);
+    d3 pk0 = xyz(p.xq[sk]);
+    int32_t ta = p.s_type[a];
+    bool cgp = J.use_cg_grouped != 0;
+    const bool jp = !LONG && J.has_pert;   // perturbation deltas ride on the short pass
+    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
+    for (int32_t e = e0 + lane0; e < e1; e += stride) {
+        uint32_t ent = p.gen_ent[e];
+        bool lng = (ent & ENT_LONG) != 0;
+        int32_t m = (int32_t)(ent & ENT_IDX);
+        bool canon = (ent & ENT_CANON) != 0;
+        // the pair's first atom lies in the lower group; only its group can
+        // hold the perturbed first atom (perturbed.rs:327-332)
+        bool pchk = jp && p.cg_pert[p.s_cg_orig[canon ? k : m]];
+        int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+        if (lng != LONG) {
+            // long pairs are cached from the last rebuild; their correction is
+            // taken every step at the current positions (forcefield.rs:796-830)
+            if (!pchk) continue;
+            for (int32_t b = 0; b < nm; ++b) {
+                int32_t ob = p.s2o[sm + b];
+                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+            }
+            continue;
+        }
+        bool ss = (ent & ENT_SS) != 0;
+        d3 pm0 = xyz(p.xq[sm]);
+        for (int32_t b = 0; b < nm; ++b) {
+            if (!LONG && ss) {
+                int32_t ob = p.s2o[sm + b];
+                if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            }
+            double4 xb = p.xq[sm + b];
+            gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
+                     p.s_type[sm + b], xa.w, xb.w, LONG || virial);
+            if (pchk) {
+                int32_t ob = p.s2o[sm + b];
+                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+            }
+        }
+    }
+    // intra-group pairs of solute groups (pairlist.rs:238-249), always short
+    if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
+        int32_t ia = a - sk;
+        bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
+        for (int32_t b = lane0; b < nk; b += stride) {
+            if (b == ia) continue;
+            int32_t ob = p.s2o[sk + b];
+            bool canon = ia < b;
+            if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            double4 xb = p.xq[sk + b];
+            gen_pair(p, J, v, fx, fy, fz, canon, cgp, pa, xyz(xb), pk0, pk0, ta, p.s_type[sk + b],
+                     xa.w, xb.w, virial);
+            if (pown) {
+                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+            }
+        }
+    }
+}
+
 // Every non-solvent-solvent-short pair: solute-solute (with exclusions), solute-
-// solvent, intra-group, and (LONG) the twin-range long pairs.
+// solvent, intra-group, and (LONG) the twin-range long pairs; one lane per
+// solvent atom (a solvent atom has few such partners).  Solute atoms, whose
+// lists are long, are done a wave each by k_gen_sol.
 template <bool LONG>
 __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t virial) {
     double v[ROW] = {0};
@@ -851,72 +927,8 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     int32_t o = a < npad ? p.s2o[a] : -1;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
-        int32_t k = p.s_atom_cg[a];
         const JobDev& J = p.jobs[p.atom_job[o]];
-        int32_t sk = p.s_cg_start[k];
-        int32_t nk = p.s_cg_n[k];
-        double4 xa = p.xq[a];
-        d3 pa = xyz(xa);
-        d3 pk0 = xyz(p.xq[sk]);
-        int32_t ta = p.s_type[a];
-        bool cgp = J.use_cg_grouped != 0;
-        const bool jp = !LONG && J.has_pert;   // perturbation deltas ride on the short pass
-        int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
-        for (int32_t e = e0; e < e1; ++e) {
-            uint32_t ent = p.gen_ent[e];
-            bool lng = (ent & ENT_LONG) != 0;
-            int32_t m = (int32_t)(ent & ENT_IDX);
-            bool canon = (ent & ENT_CANON) != 0;
-            // the pair's first atom lies in the lower group; only its group can
-            // hold the perturbed first atom (perturbed.rs:327-332)
-            bool pchk = jp && p.cg_pert[p.s_cg_orig[canon ? k : m]];
-            int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
-            if (lng != LONG) {
-                // long pairs are cached from the last rebuild; their correction is
-                // taken every step at the current positions (forcefield.rs:796-830)
-                if (!pchk) continue;
-                for (int32_t b = 0; b < nm; ++b) {
-                    int32_t ob = p.s2o[sm + b];
-                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
-                }
-                continue;
-            }
-            bool ss = (ent & ENT_SS) != 0;
-            d3 pm0 = xyz(p.xq[sm]);
-            for (int32_t b = 0; b < nm; ++b) {
-                if (!LONG && ss) {
-                    int32_t ob = p.s2o[sm + b];
-                    if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
-                }
-                double4 xb = p.xq[sm + b];
-                gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
-                         p.s_type[sm + b], xa.w, xb.w, LONG || virial);
-                if (pchk) {
-                    int32_t ob = p.s2o[sm + b];
-                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
-                }
-            }
-        }
-        // intra-group pairs of solute groups (pairlist.rs:238-249), always short
-        if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
-            int32_t ia = a - sk;
-            bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
-            for (int32_t b = 0; b < nk; ++b) {
-                if (b == ia) continue;
-                int32_t ob = p.s2o[sk + b];
-                bool canon = ia < b;
-                if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
-                double4 xb = p.xq[sk + b];
-                gen_pair(p, J, v, fx, fy, fz, canon, cgp, pa, xyz(xb), pk0, pk0, ta, p.s_type[sk + b],
-                         xa.w, xb.w, virial);
-                if (pown) {
-                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
-                }
-            }
-        }
+        if (o - J.atom_off >= J.n_solute_atoms) gen_atom64<LONG>(p, a, o, 0, 1, virial, v, fx, fy, fz);
     }
     if (a < npad) {
         double* f = LONG ? p.f_long_s : p.f_s;
@@ -933,10 +945,124 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
-// Mixed-precision counterpart of k_gen: FP32 pair arithmetic on offsets from
-// the groups' first atoms (their nearest-image vector taken in FP64), FP64
-// per-lane sums, int64 fixed-point per-atom result.  Energies from the
+// Mixed-precision counterpart of gen_atom64: FP32 pair arithmetic on offsets
+// from the groups' first atoms (their nearest-image vector taken in FP64), FP64
+// sums; pairs with a perturbed first atom entirely in FP64.  Energies from the
 // canonical partner only.
+template <bool LONG, bool VIR>
+__device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t o, int32_t lane0, int32_t stride,
+                                           double (&v)[ROW], double& fx, double& fy, double& fz) {
+    int32_t k = p.s_atom_cg[a];
+    const JobDev& J = p.jobs[p.atom_job[o]];
+    const Box box = J.box;
+    const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
+    const float fpe = (float)J.fpepsi;
+    const int32_t nt = J.n_types;
+    const double* ljb = p.lj + J.lj_off;
+    int32_t sk = p.s_cg_start[k], nk = p.s_cg_n[k];
+    float4 da = p.xf[a];
+    double4 xa = p.xq[a];
+    d3 pk0 = xyz(p.xq[sk]);
+    int32_t ta = p.s_type[a];
+    bool cgp = J.use_cg_grouped != 0 && !LONG;
+    const bool jp = !LONG && J.has_pert;
+    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
+    for (int32_t e = e0 + lane0; e < e1; e += stride) {
+        uint32_t ent = p.gen_ent[e];
+        bool lng = (ent & ENT_LONG) != 0;
+        int32_t m = (int32_t)(ent & ENT_IDX);
+        float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
+        bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
+        int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
+        if (lng != LONG) {
+            if (!pchk) continue;
+            for (int32_t b = 0; b < nm; ++b) {
+                int32_t ob = p.s2o[sm + b];
+                int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+            }
+            continue;
+        }
+        bool ss = (ent & ENT_SS) != 0;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        if (cgp) shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+        for (int32_t b = 0; b < nm; ++b) {
+            if (!LONG && ss) {
+                int32_t ob = p.s2o[sm + b];
+                if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            }
+            double4 xb = p.xq[sm + b];
+            if (pchk) {
+                // a perturbed first atom: the state-A pair in FP64 (so that the
+                // FP64 delta below cancels it exactly), then the delta
+                int32_t ob = p.s2o[sm + b];
+                int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                if (p.pflag[fi]) {
+                    gen_pair(p, J, v, fx, fy, fz, canon != 0.f, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
+                             p.s_type[sm + b], xa.w, xb.w, VIR);
+                    pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+                    continue;
+                }
+            }
+            float rx, ry, rz;
+            if (cgp) {
+                float4 db = p.xf[sm + b];
+                rx = (sx + da.x) - db.x;
+                ry = (sy + da.y) - db.y;
+                rz = (sz + da.z) - db.z;
+            } else {
+                d3 r = nim(box, xyz(xa), xyz(xb));
+                rx = (float)r.x; ry = (float)r.y; rz = (float)r.z;
+            }
+            const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sm + b]);
+            float fm, elj, ecrf;
+            lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
+                     ecrf);
+            float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+            fx += (double)gx;
+            fy += (double)gy;
+            fz += (double)gz;
+            v[0] += (double)(canon * elj);
+            v[1] += (double)(canon * ecrf);
+            if (VIR || LONG)
+                add_virial(v, d3{(double)(canon * rx), (double)(canon * ry), (double)(canon * rz)}, gx, gy, gz);
+        }
+    }
+    if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
+        int32_t ia = a - sk;
+        bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
+        for (int32_t b = lane0; b < nk; b += stride) {
+            if (b == ia) continue;
+            int32_t ob = p.s2o[sk + b];
+            bool canon = ia < b;
+            if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            double4 xb = p.xq[sk + b];
+            if (pown) {
+                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                if (p.pflag[fi]) {
+                    gen_pair(p, J, v, fx, fy, fz, canon, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
+                             p.s_type[sk + b], xa.w, xb.w, VIR);
+                    pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                    continue;
+                }
+            }
+            float rx = (float)(xa.x - xb.x), ry = (float)(xa.y - xb.y), rz = (float)(xa.z - xb.z);
+            const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sk + b]);
+            float fm, elj, ecrf;
+            lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
+                     ecrf);
+            float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+            fx += (double)gx;
+            fy += (double)gy;
+            fz += (double)gz;
+            float cf = canon ? 1.0f : 0.0f;
+            v[0] += (double)(cf * elj);
+            v[1] += (double)(cf * ecrf);
+            if (VIR) add_virial(v, d3{(double)(cf * rx), (double)(cf * ry), (double)(cf * rz)}, gx, gy, gz);
+        }
+    }
+}
+
 template <bool LONG, bool VIR>
 __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     double v[ROW] = {0};
@@ -944,115 +1070,8 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     int32_t o = a < npad ? p.s2o[a] : -1;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
-        int32_t k = p.s_atom_cg[a];
         const JobDev& J = p.jobs[p.atom_job[o]];
-        const Box box = J.box;
-        const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
-        const float fpe = (float)J.fpepsi;
-        const int32_t nt = J.n_types;
-        const double* ljb = p.lj + J.lj_off;
-        int32_t sk = p.s_cg_start[k], nk = p.s_cg_n[k];
-        float4 da = p.xf[a];
-        double4 xa = p.xq[a];
-        d3 pk0 = xyz(p.xq[sk]);
-        int32_t ta = p.s_type[a];
-        bool cgp = J.use_cg_grouped != 0 && !LONG;
-        const bool jp = !LONG && J.has_pert;
-        int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
-        for (int32_t e = e0; e < e1; ++e) {
-            uint32_t ent = p.gen_ent[e];
-            bool lng = (ent & ENT_LONG) != 0;
-            int32_t m = (int32_t)(ent & ENT_IDX);
-            float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
-            bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
-            int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
-            if (lng != LONG) {
-                if (!pchk) continue;
-                for (int32_t b = 0; b < nm; ++b) {
-                    int32_t ob = p.s2o[sm + b];
-                    int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
-                    if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
-                }
-                continue;
-            }
-            bool ss = (ent & ENT_SS) != 0;
-            float sx = 0.f, sy = 0.f, sz = 0.f;
-            if (cgp) shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
-            for (int32_t b = 0; b < nm; ++b) {
-                if (!LONG && ss) {
-                    int32_t ob = p.s2o[sm + b];
-                    if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
-                }
-                double4 xb = p.xq[sm + b];
-                if (pchk) {
-                    // a perturbed first atom: the state-A pair in FP64 (so that the
-                    // FP64 delta below cancels it exactly), then the delta
-                    int32_t ob = p.s2o[sm + b];
-                    int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
-                    if (p.pflag[fi]) {
-                        gen_pair(p, J, v, fx, fy, fz, canon != 0.f, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
-                                 p.s_type[sm + b], xa.w, xb.w, VIR);
-                        pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
-                        continue;
-                    }
-                }
-                float rx, ry, rz;
-                if (cgp) {
-                    float4 db = p.xf[sm + b];
-                    rx = (sx + da.x) - db.x;
-                    ry = (sy + da.y) - db.y;
-                    rz = (sz + da.z) - db.z;
-                } else {
-                    d3 r = nim(box, xyz(xa), xyz(xb));
-                    rx = (float)r.x; ry = (float)r.y; rz = (float)r.z;
-                }
-                const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sm + b]);
-                float fm, elj, ecrf;
-                lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
-                         ecrf);
-                float gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                fx += (double)gx;
-                fy += (double)gy;
-                fz += (double)gz;
-                v[0] += (double)(canon * elj);
-                v[1] += (double)(canon * ecrf);
-                if (VIR || LONG)
-                    add_virial(v, d3{(double)(canon * rx), (double)(canon * ry), (double)(canon * rz)}, gx, gy, gz);
-            }
-        }
-        if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
-            int32_t ia = a - sk;
-            bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
-            for (int32_t b = 0; b < nk; ++b) {
-                if (b == ia) continue;
-                int32_t ob = p.s2o[sk + b];
-                bool canon = ia < b;
-                if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
-                double4 xb = p.xq[sk + b];
-                if (pown) {
-                    int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                    if (p.pflag[fi]) {
-                        gen_pair(p, J, v, fx, fy, fz, canon, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
-                                 p.s_type[sk + b], xa.w, xb.w, VIR);
-                        pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
-                        continue;
-                    }
-                }
-                float rx = (float)(xa.x - xb.x), ry = (float)(xa.y - xb.y), rz = (float)(xa.z - xb.z);
-                const double* lj = ljb + 4 * ((int64_t)ta * nt + p.s_type[sk + b]);
-                float fm, elj, ecrf;
-                lj_crf32(rx, ry, rz, (float)lj[0], (float)lj[1], (float)xa.w * (float)xb.w * fpe, c2, cc, c3, fm, elj,
-                         ecrf);
-                float gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                fx += (double)gx;
-                fy += (double)gy;
-                fz += (double)gz;
-                float cf = canon ? 1.0f : 0.0f;
-                v[0] += (double)(cf * elj);
-                v[1] += (double)(cf * ecrf);
-                if (VIR) add_virial(v, d3{(double)(cf * rx), (double)(cf * ry), (double)(cf * rz)}, gx, gy, gz);
-            }
-        }
+        if (o - J.atom_off >= J.n_solute_atoms) gen_atom32<LONG, VIR>(p, a, o, 0, 1, v, fx, fy, fz);
     }
     if (a < npad) {
         long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
@@ -1067,6 +1086,58 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
         }
     }
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
+}
+
+// Solute atoms: one wave per atom (global solute index w, original atom
+// sol_atom[w]), the 64 lanes taking every 64th entry; forces and energies summed
+// over the wave with a fixed xor butterfly.  Energies go to the per-solute-atom
+// record sol_e[(LONG ? nsol : 0) + w] that k_reduce adds per job.
+template <bool LONG, bool MIXED, bool VIR>
+__global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int32_t virial) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int32_t w = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x / WAVE);
+    if (w >= nsol) return;   // wave-uniform
+    double v[ROW] = {0};
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int32_t o = p.sol_atom[w];
+    const int32_t a = p.o2s[o];
+    if (MIXED) gen_atom32<LONG, VIR>(p, a, o, lane, WAVE, v, fx, fy, fz);
+    else gen_atom64<LONG>(p, a, o, lane, WAVE, virial, v, fx, fy, fz);
+#pragma unroll
+    for (int m = 1; m < WAVE; m <<= 1) {
+        fx += __shfl_xor(fx, m, WAVE);
+        fy += __shfl_xor(fy, m, WAVE);
+        fz += __shfl_xor(fz, m, WAVE);
+#pragma unroll
+        for (int c = 0; c < ROW; ++c) v[c] += __shfl_xor(v[c], m, WAVE);
+    }
+    if (lane == 0) {
+        if (MIXED) {
+            long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
+            if (LONG) {
+                p.fi_long_s[3 * a] = ix;
+                p.fi_long_s[3 * a + 1] = iy;
+                p.fi_long_s[3 * a + 2] = iz;
+            } else {
+                p.fi_s[3 * a] += ix;
+                p.fi_s[3 * a + 1] += iy;
+                p.fi_s[3 * a + 2] += iz;
+            }
+        } else {
+            if (LONG) {
+                p.f_long_s[3 * a] = fx;
+                p.f_long_s[3 * a + 1] = fy;
+                p.f_long_s[3 * a + 2] = fz;
+            } else {
+                p.f_s[3 * a] += fx;
+                p.f_s[3 * a + 1] += fy;
+                p.f_s[3 * a + 2] += fz;
+            }
+        }
+        double* r = p.sol_e + ((int64_t)(LONG ? nsol : 0) + w) * ROW;
+#pragma unroll
+        for (int c = 0; c < ROW; ++c) r[c] = v[c];
+    }
 }
 
 // RF self / excluded pairs (excluded.rs:25-112) and 1-4 pairs (excluded.rs:131-206),
@@ -1394,6 +1465,19 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
     const JobDev& J = p.jobs[jb];
     const int32_t stride = p.nrow_blocks;
     double v[W] = {0};
+    // solute atoms' general pairs: per-atom records (short every step, long cached)
+    for (int32_t w = J.sol_off + threadIdx.x; w < J.sol_off + J.n_solute_atoms; w += blockDim.x) {
+        const double* r = p.sol_e + (int64_t)w * ROW;
+#pragma unroll
+        for (int c = 0; c < 11; ++c) v[c] += r[c];
+        v[12] += r[11];
+        if (with_long) {
+            const double* q = p.sol_e + ((int64_t)p.nsol + w) * ROW;
+#pragma unroll
+            for (int c = 0; c < 11; ++c) v[c] += q[c];
+            v[12] += q[11];
+        }
+    }
     // solvent-solvent pairs: the job's VV_PARTS partial sums of the per-group records
     if (threadIdx.x < VV_PARTS) {
         const double* r = p.vv_part + ((int64_t)jb * VV_PARTS + threadIdx.x) * 11;
@@ -1547,14 +1631,30 @@ void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {
+    // solvent atoms a lane each, then solute atoms a wave each (after: the long
+    // pass of k_gen writes zeros for the solute atoms)
+    const int32_t nsol = p.nsol;
+    dim3 gs((unsigned)((nsol + BLOCK / WAVE - 1) / (BLOCK / WAVE)));
     if (mixed) {
-        if (lng) hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-        else if (virial) hipLaunchKernelGGL((k_gen_mx<false, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-        else hipLaunchKernelGGL((k_gen_mx<false, false>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+        if (lng) {
+            hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+            if (nsol) hipLaunchKernelGGL((k_gen_sol<true, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+        } else if (virial) {
+            hipLaunchKernelGGL((k_gen_mx<false, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+            if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+        } else {
+            hipLaunchKernelGGL((k_gen_mx<false, false>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+            if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, false>), gs, dim3(BLOCK), 0, s, p, nsol, 0);
+        }
         return;
     }
-    if (lng) hipLaunchKernelGGL(k_gen<true>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
-    else hipLaunchKernelGGL(k_gen<false>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
+    if (lng) {
+        hipLaunchKernelGGL(k_gen<true>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
+        if (nsol) hipLaunchKernelGGL((k_gen_sol<true, false, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+    } else {
+        hipLaunchKernelGGL(k_gen<false>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
+        if (nsol) hipLaunchKernelGGL((k_gen_sol<false, false, false>), gs, dim3(BLOCK), 0, s, p, nsol, virial);
+    }
 }
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
     hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad);

@@ -73,6 +73,9 @@ struct SysPtrs {
     double* rows;
     double* vv_e;              // per sorted group: solvent-solvent e_lj, e_crf, virial[9]
     double* vv_part;           // per job: VV_PARTS partial sums of vv_e
+    const int32_t* sol_atom;   // nsol: original index of every solute atom, job by job
+    int32_t nsol;
+    double* sol_e;             // 2 x nsol x ROW: per solute atom, short then long pairs
     int32_t vv_virial;         // vv_e carries the virial
     double* records;
     int32_t* settle_fail;      // per job: first failing step

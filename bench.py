@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--fep-alone-steps", type=int, default=200,
                     help="steps per job timed for the one-job-at-a-time plain FP64 comparison")
     ap.add_argument("--remd-steps", type=int, default=1000, help="steps of the 128-replica exchange run")
+    ap.add_argument("--fep-admit-interval", type=int, default=25,
+                    help="steps between batch layout changes of the FEP server (a multiple of 5)")
     ap.add_argument("--no-fep", action="store_true")
     ap.add_argument("--no-remd", action="store_true")
     return ap.parse_args()
@@ -208,7 +210,7 @@ def main():
         out["mixed_force_error_vs_fp64"] = force_error(rt, s)
     out["energy_first_last"] = [h["e_first"], h["e_last"]]
     if not a.no_fep:
-        out["fep_mix"] = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps)
+        out["fep_mix"] = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps, a.fep_admit_interval)
     if not a.no_remd:
         out["remd"] = remd_bench(rt, gen, a.remd_steps)
     if not a.no_cpu_baseline:
@@ -216,7 +218,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def fep_mix_bench(rt, gen, steps, alone_steps):
+def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
     """The 64-job FEP lambda-window mix: seeded Poisson arrivals, continuous
     batching (mixed precision, then plain FP64), against the same jobs run one at
     a time on the plain FP64 single-job path."""
@@ -232,9 +234,10 @@ def fep_mix_bench(rt, gen, steps, alone_steps):
     sched = [dict(key=k, system=systems[k], nsteps=steps, arrival=m["arrival"]) for k, m in enumerate(mix)]
     out = dict(jobs=len(mix), atoms=int(sum(s.n_atoms for s in systems)), steps_per_job=steps,
                arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5, thermostat="Berendsen 300 K, tau 0.1 ps",
+               admit_interval=admit_interval,
                arrivals_last_step=int(max(m["arrival"] for m in mix)))
     for name, prec in (("batched_mixed", rt.MIXED), ("batched_fp64", rt.FP64)):
-        server = srv.Server(precision=prec, nsnb=5)
+        server = srv.Server(precision=prec, nsnb=5, admit_interval=admit_interval)
         r = srv.replay(server, sched)
         sim_ns = len(mix) * steps * 0.002 * 1e-3
         out[name] = dict(pairs_per_s=r["pairs"] / r["wall"], wall_s=r["wall"], ns_per_day=sim_ns * 86400.0 / r["wall"],

@@ -304,6 +304,7 @@ extern "C" int gromos_amd_system_set_state(gromos_amd_system* S, const double* p
     HIPCHK(hipStreamSynchronize(S->stream));
     S->built = false;
     S->long_computed = false;
+    S->therm_dirty = true;
     return 0;
 }
 

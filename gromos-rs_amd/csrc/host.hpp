@@ -102,6 +102,7 @@ struct gromos_amd_system {
     int32_t* d_settle_fail = nullptr;
     double* d_therm_scale = nullptr;
     bool any_thermo = false;
+    bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
     uint8_t *d_pflag = nullptr, *d_cg_pert = nullptr;
     int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
     int32_t *d_pap_off = nullptr, *d_pap = nullptr;

@@ -485,6 +485,7 @@ extern "C" int gromos_amd_jobs_scale_velocities(gromos_amd_session* s, int32_t n
                        S->d_atom_job, (const double*)s->small.ptr, S->npad);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(S->stream));
+    S->therm_dirty = true;
     return 0;
 }
 
@@ -508,6 +509,7 @@ extern "C" int gromos_amd_jobs_swap_configurations(gromos_amd_session* s, int64_
     HIPCHK(hipStreamSynchronize(S->stream));
     S->built = false;   // the swapped jobs' lists belong to the other configuration
     S->long_computed = false;
+    S->therm_dirty = true;
     return 0;
 }
 

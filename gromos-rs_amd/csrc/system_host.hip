@@ -469,6 +469,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->temp_bytes = S->bufs[(void*)&S->d_temp].bytes;
     HIPCHK(hipStreamSynchronize(st));
     S->cur = 0;
+    S->therm_dirty = true;
     S->built = false;
     S->have_prev = false;
     S->long_computed = false;
@@ -692,7 +693,10 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     for (int32_t s = 0; s < nsteps; ++s) {
         if (int rc = forcefield(S)) return rc;
         SysPtrs p = S->ptrs();
-        if (S->any_thermo) launch_therm(p, S->njobs, st);
+        if (S->any_thermo && S->therm_dirty) {
+            launch_therm(p, S->njobs, st);
+            S->therm_dirty = false;
+        }
         launch_leapfrog(p, S->npad, s, st);
         if (S->use_settle) launch_settle(p, S->nmol, s, st);
         launch_ekin(p, S->npad, st);

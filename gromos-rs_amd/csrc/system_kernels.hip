@@ -13,6 +13,21 @@
 
 namespace gamd {
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// With the blocks of one XCD renumbered to a contiguous range of the (cell-
+// sorted, hence spatially ordered) work, each L2 holds one slab of positions
+// and its halo instead of the whole system.  The grid is a multiple of NXCD;
+// returns -1 for padding blocks (whole blocks exit, no block barrier is hit).
+constexpr int NXCD = 8;
+__device__ __forceinline__ int32_t xcd_block(int32_t nreal) {
+    const int32_t per = (int32_t)gridDim.x / NXCD;
+    int32_t b = ((int32_t)blockIdx.x % NXCD) * per + (int32_t)blockIdx.x / NXCD;
+    return b < nreal ? b : -1;
+}
+static inline dim3 xcd_grid(int64_t nblocks) {
+    return dim3((unsigned)((nblocks + NXCD - 1) / NXCD * NXCD));
+}
+
 // ---------------------------------------------------------------------------
 // pair list
 // ---------------------------------------------------------------------------
@@ -121,7 +136,9 @@ constexpr int CELL_R = 2;   // neighbour cells searched along each axis
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int32_t k = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x / WAVE);
+    const int32_t lb = xcd_block((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE));
+    if (lb < 0) return;
+    const int32_t k = lb * (BLOCK / WAVE) + (threadIdx.x / WAVE);
     if (k >= ncg) return;   // wave-uniform
     int32_t ck = p.s_cg_orig[k];
     const JobDev& J = p.jobs[p.cg_job[ck]];
@@ -387,7 +404,9 @@ __device__ __forceinline__ bool has_lj(int i, int j) {
 
 template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
-    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t lb = xcd_block((int32_t)(((int64_t)ncg * NCHM + BLOCK - 1) / BLOCK));
+    if (lb < 0) return;
+    const int32_t gid = lb * BLOCK + threadIdx.x;
     const int32_t k = gid / NCHM;
     const int chunk = gid % NCHM;
     double f[APS][3];
@@ -491,7 +510,9 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 // lane sums, int64 fixed-point per-atom result.
 template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
-    const int32_t gid = blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t lb = xcd_block((int32_t)(((int64_t)ncg * NCHM + BLOCK - 1) / BLOCK));
+    if (lb < 0) return;
+    const int32_t gid = lb * BLOCK + threadIdx.x;
     const int32_t k = gid / NCHM;
     const int chunk = gid % NCHM;
     long long fi[APS][3];
@@ -1265,6 +1286,16 @@ __global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t m
 // step
 // ---------------------------------------------------------------------------
 
+__device__ __forceinline__ double berendsen_scale(const JobDev& J, double ekin) {
+    if (J.tau_t < 0.0) return 1.0;   // no coupling
+    const double kB = 0.00831441;    // units.rs:47
+    double free_temp = J.dof_t > 0.0 ? 2.0 * ekin / (J.dof_t * kB) : 0.0;
+    if (free_temp < 1e-10) free_temp = J.t_ref;
+    if (free_temp < 1e-10) return 1.0;
+    if (J.tau_t == 0.0) return sqrt(J.t_ref / free_temp);
+    return sqrt(1.0 + J.dt / J.tau_t * (J.t_ref / free_temp - 1.0));
+}
+
 // LeapFrogVelocity then LeapFrogPosition (leap_frog.rs:37-68, 116-140).  A job
 // past its stop step keeps its state (copied to the new buffers).
 __global__ void k_leapfrog(SysPtrs p, int32_t npad, int32_t step) {
@@ -1305,7 +1336,8 @@ __global__ void k_leapfrog(SysPtrs p, int32_t npad, int32_t step) {
 
 // Berendsen scale of each coupled job from E_kin_new = 0.5 sum m |v|^2 of the
 // velocities entering the step (what the previous TemperatureCalculation
-// stored, temperature.rs:76-85; berendsen_thermostat.rs:66-100), block per job
+// stored, temperature.rs:76-85; berendsen_thermostat.rs:66-100).  k_reduce sets
+// it after every step; k_therm, block per job, after velocities changed outside.
 __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     __shared__ double lds[17];
     const JobDev& J = p.jobs[blockIdx.x];
@@ -1320,19 +1352,7 @@ __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     }
     double t[1] = {s};
     block_sum<1>(t, lds);
-    if (threadIdx.x == 0) {
-        double ekin = lds[16];
-        double scale = 1.0;
-        if (J.tau_t >= 0.0) {
-            const double kB = 0.00831441;   // units.rs:47
-            double free_temp = J.dof_t > 0.0 ? 2.0 * ekin / (J.dof_t * kB) : 0.0;
-            if (free_temp < 1e-10) free_temp = J.t_ref;
-            if (free_temp < 1e-10) scale = 1.0;
-            else if (J.tau_t == 0.0) scale = sqrt(J.t_ref / free_temp);
-            else scale = sqrt(1.0 + J.dt / J.tau_t * (J.t_ref / free_temp - 1.0));
-        }
-        p.therm_scale[blockIdx.x] = scale;
-    }
+    if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = berendsen_scale(J, lds[16]);
 }
 
 struct v3 {
@@ -1433,6 +1453,7 @@ __global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
         double a = vn[3 * i] * vn[3 * i] + vn[3 * i + 1] * vn[3 * i + 1] + vn[3 * i + 2] * vn[3 * i + 2];
         double b = vo[3 * i] * vo[3 * i] + vo[3 * i + 1] * vo[3 * i + 1] + vo[3 * i + 2] * vo[3 * i + 2];
         v[11] = 0.5 * p.mass[i] * (a + b) / 2.0;
+        v[10] = 0.5 * p.mass[i] * a;   // E_kin_new (temperature.rs:81), for the thermostat
     }
     write_row(v, p.rows, SLOT_KIN, p.nrow_blocks);
 }
@@ -1459,7 +1480,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_viri
 // row is E_kin in the kinetic slot and dH/dlambda in the others.
 __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32_t with_long,
                                                   int32_t with_kin, int32_t step, int32_t njobs) {
-    constexpr int W = ROW + 1;
+    constexpr int W = ROW + 2;   // + dH/dlambda, E_kin_new
     __shared__ double lds[17 * W];
     int32_t jb = blockIdx.x;
     const JobDev& J = p.jobs[jb];
@@ -1495,6 +1516,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
             const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
             if (s == SLOT_KIN) {
                 v[11] += r[11];
+                v[13] += r[10];
             } else {
 #pragma unroll
                 for (int c = 0; c < 11; ++c) v[c] += r[c];
@@ -1512,6 +1534,9 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         rec[3] = t[11];
         rec[4] = t[12];
         for (int c = 0; c < 9; ++c) rec[5 + c] = t[2 + c];
+        // the next step's Berendsen scale from this step's E_kin_new: what the
+        // reference reads after TemperatureCalculation stored it
+        if (with_kin && J.thermo) p.therm_scale[jb] = berendsen_scale(J, t[13]);
     }
 }
 
@@ -1587,7 +1612,7 @@ void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
 }
 void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
     // one wave per group
-    dim3 g((unsigned)((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE)));
+    dim3 g = xcd_grid((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE));
     if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncg);
     else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncg);
     unsigned nb = (unsigned)std::min<int64_t>(256, (ncg + BLOCK - 1) / BLOCK);
@@ -1601,7 +1626,7 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 template <int APS, bool LJ00, int L>
 static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipEvent_t e0,
                         hipEvent_t e1, hipStream_t s) {
-    dim3 g(grid_for((int64_t)ncg * L));
+    dim3 g = xcd_grid(((int64_t)ncg * L + BLOCK - 1) / BLOCK);
     if (mixed) {
         if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
         else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);

@@ -305,6 +305,7 @@ extern "C" int gromos_amd_system_set_state(gromos_amd_system* S, const double* p
     S->built = false;
     S->long_computed = false;
     S->therm_dirty = true;
+    S->xq_valid = false;
     return 0;
 }
 
@@ -330,7 +331,7 @@ extern "C" int gromos_amd_system_forcefield_apply(gromos_amd_system* S, double* 
                                                   double* virial) {
     if (!S) return fail(GROMOS_AMD_EINVAL, "null system");
     if (int rc = S->reserve(S->d_records, (size_t)S->njobs * REC, true)) return rc;
-    if (int rc = forcefield(S)) return rc;
+    if (int rc = forcefield(S, true)) return rc;
     SysPtrs p = S->ptrs();
     launch_reduce(p, S->npad, S->twin ? 1 : 0, 0, 0, S->njobs, S->stream);
     HIPCHK(hipGetLastError());

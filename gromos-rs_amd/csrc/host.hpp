@@ -72,7 +72,7 @@ struct gromos_amd_system {
     int32_t *d_cg_start = nullptr, *d_cg_n = nullptr, *d_cg_job = nullptr, *d_atom_job = nullptr, *d_iac = nullptr;
     double *d_q = nullptr, *d_mass = nullptr, *d_inv_mass = nullptr, *d_lj = nullptr;
     int32_t *d_excl_off = nullptr, *d_excl = nullptr, *d_o14_off = nullptr, *d_o14 = nullptr;
-    int32_t *d_o14s_off = nullptr, *d_o14s = nullptr, *d_mol_first = nullptr;
+    int32_t *d_o14s_off = nullptr, *d_o14s = nullptr;
     double *d_x[2] = {nullptr, nullptr}, *d_v[2] = {nullptr, nullptr}, *d_f = nullptr;
     double *d_ref = nullptr, *d_s_ref = nullptr;
     uint32_t *d_key_in = nullptr, *d_val_in = nullptr, *d_key_out = nullptr, *d_val_out = nullptr;
@@ -103,6 +103,8 @@ struct gromos_amd_system {
     double* d_therm_scale = nullptr;
     bool any_thermo = false;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
+    bool xq_valid = false;     // xq / xf hold the current positions in the current sorted order
+    double* d_cg_ek = nullptr;
     uint8_t *d_pflag = nullptr, *d_cg_pert = nullptr;
     int32_t *d_pa_iac = nullptr, *d_pb_iac = nullptr, *d_papairs = nullptr, *d_o2s = nullptr;
     int32_t *d_pap_off = nullptr, *d_pap = nullptr;
@@ -187,7 +189,7 @@ namespace gamd {
 int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs, int32_t precision);
 int upload_jobs(gromos_amd_system* S);
 int build_pairlist(gromos_amd_system* S);
-int forcefield(gromos_amd_system* S);
+int forcefield(gromos_amd_system* S, bool assemble);
 // nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when
 // given.  A SETTLE failure is an error unless `failed` is given: it then gets
 // each job's first failing step (INT32_MAX: none).

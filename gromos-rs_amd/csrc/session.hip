@@ -510,6 +510,7 @@ extern "C" int gromos_amd_jobs_swap_configurations(gromos_amd_session* s, int64_
     S->built = false;   // the swapped jobs' lists belong to the other configuration
     S->long_computed = false;
     S->therm_dirty = true;
+    S->xq_valid = false;
     return 0;
 }
 

@@ -52,6 +52,7 @@ struct JobDev {
     double dt;
     double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
     int32_t sol_off;                       // first index of the job's solute atoms in sol_atom
+    int32_t settle;                        // SETTLE the solvent molecules
     int32_t thermo;                        // Berendsen weak coupling (berendsen_thermostat.rs)
     double t_ref, tau_t, dof_t;
 };
@@ -64,7 +65,9 @@ enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 
 // per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]
 constexpr int REC = 14;
 
-// parts per job of the solvent-solvent energy records (two-stage reduction)
+// parts per job of the solvent-solvent energy and E_kin records (two-stage
+// reduction); PART_W columns: e_lj, e_crf, virial[9], E_kin, E_kin_new
 constexpr int VV_PARTS = 32;
+constexpr int PART_W = 13;
 
 }  // namespace gamd

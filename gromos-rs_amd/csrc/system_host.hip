@@ -136,6 +136,7 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
             }
     }
     J.has_pert = im.has_pert ? 1 : 0;
+    J.settle = (P.use_settle && nsolv_atoms > 0 && aps == 3) ? 1 : 0;
     // from_lambda(l, l, l, l, 1, 1, 1, 1, NLAM) (forcefield.rs:767-777)
     {
         gromos_amd_lambda L;
@@ -402,7 +403,6 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_o14, o14);
     rc = rc ? rc : S->upload(S->d_o14s_off, o14s_off);
     rc = rc ? rc : S->upload(S->d_o14s, o14s);
-    rc = rc ? rc : S->upload(S->d_mol_first, mol_first);
     rc = rc ? rc : S->upload(S->d_pflag, pflag);
     rc = rc ? rc : S->upload(S->d_pa_iac, pa_iac);
     rc = rc ? rc : S->upload(S->d_pb_iac, pb_iac);
@@ -446,7 +446,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
-        (rc = S->reserve(S->d_vv_part, (size_t)11 * VV_PARTS * njobs, true)) ||
+        (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
+        (rc = S->reserve(S->d_cg_ek, 2 * nc, true)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
@@ -470,6 +471,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipStreamSynchronize(st));
     S->cur = 0;
     S->therm_dirty = true;
+    S->xq_valid = false;
     S->built = false;
     S->have_prev = false;
     S->long_computed = false;
@@ -631,9 +633,10 @@ static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t 
     return 0;
 }
 
-// Forcefield::apply nonbonded part (forcefield.rs:226-1060) into d_f; energy
-// partials into the rows
-int forcefield(gromos_amd_system* S) {
+// Forcefield::apply nonbonded part (forcefield.rs:226-1060); energy partials
+// into the rows and records.  assemble: total forces into d_f (the md step
+// assembles them itself).
+int forcefield(gromos_amd_system* S, bool assemble) {
     hipStream_t st = S->stream;
     if (!S->built) {
         if (int rc = build_pairlist(S)) return rc;
@@ -646,7 +649,10 @@ int forcefield(gromos_amd_system* S) {
     SysPtrs p = S->ptrs();
     int32_t npad = S->npad;
     const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
-    launch_gather(p, npad, mixed, st);
+    // sorted position copies: written by the previous step's integrator unless
+    // the list was rebuilt (new order) or the state was set from outside
+    if (rebuild || !S->xq_valid) launch_gather(p, npad, mixed, st);
+    S->xq_valid = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (S->timing) {
         e0 = S->next_event();
@@ -668,7 +674,7 @@ int forcefield(gromos_amd_system* S) {
         S->pair_count += (int64_t)S->host_counters[1];
     }
     launch_excl14(p, npad, st);
-    launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
+    if (assemble) launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -690,18 +696,21 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     };
     if (S->step_timing)
         if (int rc = step_event()) return rc;
+    const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
     for (int32_t s = 0; s < nsteps; ++s) {
-        if (int rc = forcefield(S)) return rc;
+        if (int rc = forcefield(S, false)) return rc;
         SysPtrs p = S->ptrs();
         if (S->any_thermo && S->therm_dirty) {
             launch_therm(p, S->njobs, st);
             S->therm_dirty = false;
         }
-        launch_leapfrog(p, S->npad, s, st);
-        if (S->use_settle) launch_settle(p, S->nmol, s, st);
-        launch_ekin(p, S->npad, st);
+        // the integrator leaves the sorted copies for the next step unless that
+        // step rebuilds the list (pairlist.rs:105-107)
+        const int32_t write_xq = S->steps_since_update < S->nsnb ? 1 : 0;
+        launch_integrate(p, S->ncg, s, S->twin ? 1 : 0, write_xq, mixed, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, s, S->njobs, st);
         S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
+        S->xq_valid = write_xq != 0;
         if (S->step_timing)
             if (int rc = step_event()) return rc;
     }
@@ -792,7 +801,6 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.o14 = d_o14;
     p.o14s_off = d_o14s_off;
     p.o14s = d_o14s;
-    p.mol_first = d_mol_first;
     p.x_cur = d_x[cur];
     p.x_new = d_x[1 - cur];
     p.v_cur = d_v[cur];
@@ -838,6 +846,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.rows = d_rows;
     p.vv_e = d_vv_e;
     p.vv_part = d_vv_part;
+    p.cg_ek = d_cg_ek;
     p.sol_atom = d_sol_atom;
     p.nsol = nsol;
     p.sol_e = d_sol_e;

@@ -150,19 +150,35 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
     int32_t local = (int32_t)p.key_out[k] - J.cell_off;
     int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
-    // cells are at least half the list cutoff wide: every partner lies within
-    // two cells along each axis (all cells along an axis with fewer than five)
-    int32_t lyz[2][2 * CELL_R + 1], nyz[2];
-    const int32_t nd[2] = {ny, nz};
+    // Cells are at least half the list cutoff wide: every partner lies within
+    // two cells along each axis (all cells along an axis with fewer than five).
+    // Cells farther from the group's reference point than the cutoff (with a
+    // margin; the exact test below decides) are skipped: rows of cells in (y, z)
+    // whose distance alone exceeds it, and the x ends of the remaining rows.
+    // dist[a][d] = distance along axis a from the point to cell offset d - 2
+    // (0 for axes searched whole).
+    const int32_t nd3[3] = {nx, ny, nz};
+    double dist[3][2 * CELL_R + 1];
+    const double rcull = sqrt(c2l) * (1.0 + 1e-7) + 1e-9, rcull2 = rcull * rcull;
+    {
+        const double rr[3] = {rk.x, rk.y, rk.z};
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        int32_t n = nd[a];
-        if (n >= 2 * CELL_R + 1) {
-            nyz[a] = 2 * CELL_R + 1;
-            for (int d = -CELL_R; d <= CELL_R; ++d) lyz[a][d + CELL_R] = (cc[a + 1] + n + d) % n;
-        } else {
-            nyz[a] = n;
-            for (int d = 0; d < n; ++d) lyz[a][d] = d;
+        for (int a = 0; a < 3; ++a) {
+            const int32_t n = nd3[a];
+            double u = 0.0;
+            if (box.kind) {
+                u = rr[a] * box.invL[a];   // wrapped as k_cg_ref wraps it
+                u = u - floor(u);
+            }
+#pragma unroll
+            for (int d = -CELL_R; d <= CELL_R; ++d) {
+                double dd = 0.0;
+                if (n >= 2 * CELL_R + 1) {
+                    double lo = (double)(cc[a] + d) / n, hi = (double)(cc[a] + d + 1) / n;
+                    dd = fmax(0.0, fmax(lo - u, u - hi)) * box.L[a];
+                }
+                dist[a][d + CELL_R] = dd;
+            }
         }
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -220,25 +236,40 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
             ngen += __popcll(bg);
         }
     };
-    for (int32_t iz = 0; iz < nyz[1]; ++iz)
-        for (int32_t iy = 0; iy < nyz[0]; ++iy) {
+    const int32_t nz_l = nz >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : nz;
+    const int32_t ny_l = ny >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : ny;
+    for (int32_t iz = 0; iz < nz_l; ++iz) {
+        const bool fz = nz >= 2 * CELL_R + 1;
+        const int32_t cz = fz ? (cc[2] + nz + iz - CELL_R) % nz : iz;
+        const double dz = fz ? dist[2][iz] : 0.0;
+        for (int32_t iy = 0; iy < ny_l; ++iy) {
+            const bool fy = ny >= 2 * CELL_R + 1;
+            const int32_t cy = fy ? (cc[1] + ny + iy - CELL_R) % ny : iy;
+            const double dy = fy ? dist[1][iy] : 0.0;
+            const double dyz2 = dy * dy + dz * dz;
+            if (dyz2 > rcull2) continue;
             // the x cells of one (y, z) row are contiguous in sorted order
-            int32_t row = J.cell_off + nx * (lyz[0][iy] + ny * lyz[1][iz]);
+            int32_t row = J.cell_off + nx * (cy + ny * cz);
             if (nx < 2 * CELL_R + 1) {
                 scan_run(p.cell_start[row], p.cell_start[row + nx]);
+                continue;
+            }
+            const double rx2 = rcull2 - dyz2;
+            int32_t dlo = -CELL_R, dhi = CELL_R;
+            while (dlo < 0 && dist[0][dlo + CELL_R] * dist[0][dlo + CELL_R] > rx2) ++dlo;
+            while (dhi > 0 && dist[0][dhi + CELL_R] * dist[0][dhi + CELL_R] > rx2) --dhi;
+            int32_t x0 = cc[0] + dlo, x1 = cc[0] + dhi;
+            if (x0 < 0) {
+                scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
+                scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
+            } else if (x1 >= nx) {
+                scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
+                scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
             } else {
-                int32_t x0 = cc[0] - CELL_R, x1 = cc[0] + CELL_R;
-                if (x0 < 0) {
-                    scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
-                    scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
-                } else if (x1 >= nx) {
-                    scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
-                    scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
-                } else {
-                    scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
-                }
+                scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
             }
         }
+    }
     // per-group results; totals come from k_totals (one atomic per block there:
     // thousands of waves adding to the same address serialise at the L2)
 #pragma unroll
@@ -1296,42 +1327,89 @@ __device__ __forceinline__ double berendsen_scale(const JobDev& J, double ekin) 
     return sqrt(1.0 + J.dt / J.tau_t * (J.t_ref / free_temp - 1.0));
 }
 
-// LeapFrogVelocity then LeapFrogPosition (leap_frog.rs:37-68, 116-140).  A job
-// past its stop step keeps its state (copied to the new buffers).
-__global__ void k_leapfrog(SysPtrs p, int32_t npad, int32_t step) {
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npad || p.atom_job[i] < 0) return;
-    const JobDev& J = p.jobs[p.atom_job[i]];
-    if (step >= J.stop) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            p.v_new[3 * i + c] = p.v_cur[3 * i + c];
-            p.x_new[3 * i + c] = p.x_cur[3 * i + c];
-        }
-        return;
-    }
-    double dt = J.dt;
-    double im = p.inv_mass[i];
-    if (J.thermo) {
-        // Berendsen scaling between LeapFrogVelocity and LeapFrogPosition
-        // (berendsen_thermostat.rs:29-31)
-        double lam = p.therm_scale[p.atom_job[i]];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double acc = p.f[3 * i + c] * im;
-            double vn = (p.v_cur[3 * i + c] + acc * dt) * lam;
-            p.v_new[3 * i + c] = vn;
-            p.x_new[3 * i + c] = p.x_cur[3 * i + c] + vn * dt;
-        }
-        return;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double acc = p.f[3 * i + c] * im;
-        double vn = p.v_cur[3 * i + c] + acc * dt;
-        p.v_new[3 * i + c] = vn;
-        p.x_new[3 * i + c] = p.x_cur[3 * i + c] + vn * dt;
-    }
+struct v3 {
+    double x, y, z;
+};
+__device__ __forceinline__ v3 mk(double x, double y, double z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 vmul(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 vdiv(v3 a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 vcross(v3 a, v3 b) {  // glam DVec3::cross
+    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ v3 ldv(const double* p, int64_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void stv(double* p, int64_t i, v3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// settle (constraints.rs:815-1036) of one water: old positions po, new
+// positions pn and velocities vn updated in place; false when the analytical
+// solve fails (sin > 1), pn / vn then unchanged.  FP64 throughout.
+__device__ __forceinline__ bool settle_one(const JobDev& J, const v3 (&po)[3], v3 (&pn)[3], v3 (&vn)[3]) {
+    double m_o = J.m_o, m_h = J.m_h;
+    double half_mo_div_mh = 0.5 * m_o / m_h;
+    double rc = 0.5 * J.dist_hh;
+    double ra = sqrt(J.dist_oh * J.dist_oh - rc * rc) / (1.0 + half_mo_div_mh);
+    double rb = half_mo_div_mh * ra;
+    double dt_i = 1.0 / J.dt;
+    v3 po0 = po[0], po1 = po[1], po2 = po[2];
+    v3 pn0 = pn[0], pn1 = pn[1], pn2 = pn[2];
+    v3 b0 = vsub(po1, po0), c0 = vsub(po2, po0);
+    v3 d0 = vdiv(vadd(vadd(vmul(pn0, m_o), vmul(pn1, m_h)), vmul(pn2, m_h)), m_o + m_h + m_h);
+    v3 a1 = vsub(pn0, d0), b1 = vsub(pn1, d0), c1 = vsub(pn2, d0);
+    v3 n0 = vcross(b0, c0);
+    v3 n1 = vcross(a1, n0);
+    v3 n2 = vcross(n0, n1);
+    n0 = vdiv(n0, sqrt(vdot(n0, n0)));
+    n1 = vdiv(n1, sqrt(vdot(n1, n1)));
+    n2 = vdiv(n2, sqrt(vdot(n2, n2)));
+    v3 m1 = mk(n1.x, n2.x, n0.x), m2 = mk(n1.y, n2.y, n0.y), m0 = mk(n1.z, n2.z, n0.z);
+    b0 = mk(vdot(n1, b0), vdot(n2, b0), vdot(n0, b0));
+    c0 = mk(vdot(n1, c0), vdot(n2, c0), vdot(n0, c0));
+    a1 = mk(vdot(n1, a1), vdot(n2, a1), vdot(n0, a1));
+    b1 = mk(vdot(n1, b1), vdot(n2, b1), vdot(n0, b1));
+    c1 = mk(vdot(n1, c1), vdot(n2, c1), vdot(n0, c1));
+    double sinphi = a1.z / ra;
+    double omsp2 = 1.0 - sinphi * sinphi;
+    if (omsp2 < 0.0) return false;
+    double cosphi = sqrt(omsp2);
+    double sinpsi = (b1.z - c1.z) / (2.0 * rc * cosphi);
+    double omss2 = 1.0 - sinpsi * sinpsi;
+    if (omss2 < 0.0) return false;
+    double cospsi = sqrt(omss2);
+    double minus_rb_cosphi = -rb * cosphi;
+    double rc_cospsi = rc * cospsi;
+    double rc_sinpsi_sinphi = rc * sinpsi * sinphi;
+    double rc_sinpsi_cosphi = rc * sinpsi * cosphi;
+    v3 a2 = mk(0.0, ra * cosphi, ra * sinphi);
+    v3 b2 = mk(-rc_cospsi, minus_rb_cosphi - rc_sinpsi_sinphi, -rb * sinphi + rc_sinpsi_cosphi);
+    v3 c2 = mk(rc_cospsi, minus_rb_cosphi + rc_sinpsi_sinphi, -rb * sinphi - rc_sinpsi_cosphi);
+    double alpha = b2.x * (b0.x - c0.x) + b0.y * b2.y + c0.y * c2.y;
+    double beta = b2.x * (c0.y - b0.y) + b0.x * b2.y + c0.x * c2.y;
+    double gamma = b0.x * b1.y - b1.x * b0.y + c0.x * c1.y - c1.x * c0.y;
+    double a2b2 = alpha * alpha + beta * beta;
+    double sintheta = (alpha * gamma - beta * sqrt(a2b2 - gamma * gamma)) / a2b2;
+    double omst2 = 1.0 - sintheta * sintheta;
+    if (omst2 < 0.0) return false;
+    double costheta = sqrt(omst2);
+    v3 a3 = mk(-a2.y * sintheta, a2.y * costheta, a1.z);
+    v3 b3 = mk(b2.x * costheta - b2.y * sintheta, b2.x * sintheta + b2.y * costheta, b1.z);
+    v3 c3 = mk(-b2.x * costheta - c2.y * sintheta, -b2.x * sintheta + c2.y * costheta, c1.z);
+    v3 pa = vadd(mk(vdot(a3, m1), vdot(a3, m2), vdot(a3, m0)), d0);
+    v3 pb = vadd(mk(vdot(b3, m1), vdot(b3, m2), vdot(b3, m0)), d0);
+    v3 pc = vadd(mk(vdot(c3, m1), vdot(c3, m2), vdot(c3, m0)), d0);
+    v3 da = vsub(pa, pn0), db = vsub(pb, pn1), dc = vsub(pc, pn2);
+    pn[0] = pa;
+    pn[1] = pb;
+    pn[2] = pc;
+    vn[0] = vadd(vn[0], vmul(da, dt_i));
+    vn[1] = vadd(vn[1], vmul(db, dt_i));
+    vn[2] = vadd(vn[2], vmul(dc, dt_i));
+    return true;
 }
 
 // Berendsen scale of each coupled job from E_kin_new = 0.5 sum m |v|^2 of the
@@ -1355,124 +1433,113 @@ __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = berendsen_scale(J, lds[16]);
 }
 
-struct v3 {
-    double x, y, z;
-};
-__device__ __forceinline__ v3 mk(double x, double y, double z) { return v3{x, y, z}; }
-__device__ __forceinline__ v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ v3 vmul(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ v3 vdiv(v3 a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ v3 vcross(v3 a, v3 b) {  // glam DVec3::cross
-    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
-}
-__device__ __forceinline__ v3 ldv(const double* p, int64_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
-__device__ __forceinline__ void stv(double* p, int64_t i, v3 v) {
-    p[3 * i] = v.x;
-    p[3 * i + 1] = v.y;
-    p[3 * i + 2] = v.z;
-}
-
-// settle (constraints.rs:815-1036), one lane per water; FP64 throughout
-__global__ void k_settle(SysPtrs p, int32_t nmol, int32_t step) {
-    int32_t mi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mi >= nmol) return;
-    int32_t i = p.mol_first[mi];
-    int32_t jb = p.atom_job[i];
-    const JobDev& J = p.jobs[jb];
-    if (step >= J.stop) return;
-    double m_o = J.m_o, m_h = J.m_h;
-    double half_mo_div_mh = 0.5 * m_o / m_h;
-    double rc = 0.5 * J.dist_hh;
-    double ra = sqrt(J.dist_oh * J.dist_oh - rc * rc) / (1.0 + half_mo_div_mh);
-    double rb = half_mo_div_mh * ra;
-    double dt_i = 1.0 / J.dt;
-    v3 po0 = ldv(p.x_cur, i), po1 = ldv(p.x_cur, i + 1), po2 = ldv(p.x_cur, i + 2);
-    v3 pn0 = ldv(p.x_new, i), pn1 = ldv(p.x_new, i + 1), pn2 = ldv(p.x_new, i + 2);
-    v3 b0 = vsub(po1, po0), c0 = vsub(po2, po0);
-    v3 d0 = vdiv(vadd(vadd(vmul(pn0, m_o), vmul(pn1, m_h)), vmul(pn2, m_h)), m_o + m_h + m_h);
-    v3 a1 = vsub(pn0, d0), b1 = vsub(pn1, d0), c1 = vsub(pn2, d0);
-    v3 n0 = vcross(b0, c0);
-    v3 n1 = vcross(a1, n0);
-    v3 n2 = vcross(n0, n1);
-    n0 = vdiv(n0, sqrt(vdot(n0, n0)));
-    n1 = vdiv(n1, sqrt(vdot(n1, n1)));
-    n2 = vdiv(n2, sqrt(vdot(n2, n2)));
-    v3 m1 = mk(n1.x, n2.x, n0.x), m2 = mk(n1.y, n2.y, n0.y), m0 = mk(n1.z, n2.z, n0.z);
-    b0 = mk(vdot(n1, b0), vdot(n2, b0), vdot(n0, b0));
-    c0 = mk(vdot(n1, c0), vdot(n2, c0), vdot(n0, c0));
-    a1 = mk(vdot(n1, a1), vdot(n2, a1), vdot(n0, a1));
-    b1 = mk(vdot(n1, b1), vdot(n2, b1), vdot(n0, b1));
-    c1 = mk(vdot(n1, c1), vdot(n2, c1), vdot(n0, c1));
-    double sinphi = a1.z / ra;
-    double omsp2 = 1.0 - sinphi * sinphi;
-    if (omsp2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
-    double cosphi = sqrt(omsp2);
-    double sinpsi = (b1.z - c1.z) / (2.0 * rc * cosphi);
-    double omss2 = 1.0 - sinpsi * sinpsi;
-    if (omss2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
-    double cospsi = sqrt(omss2);
-    double minus_rb_cosphi = -rb * cosphi;
-    double rc_cospsi = rc * cospsi;
-    double rc_sinpsi_sinphi = rc * sinpsi * sinphi;
-    double rc_sinpsi_cosphi = rc * sinpsi * cosphi;
-    v3 a2 = mk(0.0, ra * cosphi, ra * sinphi);
-    v3 b2 = mk(-rc_cospsi, minus_rb_cosphi - rc_sinpsi_sinphi, -rb * sinphi + rc_sinpsi_cosphi);
-    v3 c2 = mk(rc_cospsi, minus_rb_cosphi + rc_sinpsi_sinphi, -rb * sinphi - rc_sinpsi_cosphi);
-    double alpha = b2.x * (b0.x - c0.x) + b0.y * b2.y + c0.y * c2.y;
-    double beta = b2.x * (c0.y - b0.y) + b0.x * b2.y + c0.x * c2.y;
-    double gamma = b0.x * b1.y - b1.x * b0.y + c0.x * c1.y - c1.x * c0.y;
-    double a2b2 = alpha * alpha + beta * beta;
-    double sintheta = (alpha * gamma - beta * sqrt(a2b2 - gamma * gamma)) / a2b2;
-    double omst2 = 1.0 - sintheta * sintheta;
-    if (omst2 < 0.0) { atomicMin(p.settle_fail + jb, step); return; }
-    double costheta = sqrt(omst2);
-    v3 a3 = mk(-a2.y * sintheta, a2.y * costheta, a1.z);
-    v3 b3 = mk(b2.x * costheta - b2.y * sintheta, b2.x * sintheta + b2.y * costheta, b1.z);
-    v3 c3 = mk(-b2.x * costheta - c2.y * sintheta, -b2.x * sintheta + c2.y * costheta, c1.z);
-    v3 pa = vadd(mk(vdot(a3, m1), vdot(a3, m2), vdot(a3, m0)), d0);
-    v3 pb = vadd(mk(vdot(b3, m1), vdot(b3, m2), vdot(b3, m0)), d0);
-    v3 pc = vadd(mk(vdot(c3, m1), vdot(c3, m2), vdot(c3, m0)), d0);
-    v3 da = vsub(pa, pn0), db = vsub(pb, pn1), dc = vsub(pc, pn2);
-    stv(p.x_new, i, pa);
-    stv(p.x_new, i + 1, pb);
-    stv(p.x_new, i + 2, pc);
-    stv(p.v_new, i, vadd(ldv(p.v_new, i), vmul(da, dt_i)));
-    stv(p.v_new, i + 1, vadd(ldv(p.v_new, i + 1), vmul(db, dt_i)));
-    stv(p.v_new, i + 2, vadd(ldv(p.v_new, i + 2), vmul(dc, dt_i)));
-}
-
-// TemperatureCalculation (temperature.rs:63-87): 0.5 m (|v_new|^2 + |v_old|^2) / 2
-__global__ void __launch_bounds__(BLOCK) k_ekin(SysPtrs p, int32_t npad) {
-    double v[ROW] = {0};
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < npad && p.atom_job[i] >= 0) {
-        const double* vn = p.v_new;
-        const double* vo = p.v_cur;
-        double a = vn[3 * i] * vn[3 * i] + vn[3 * i + 1] * vn[3 * i + 1] + vn[3 * i + 2] * vn[3 * i + 2];
-        double b = vo[3 * i] * vo[3 * i] + vo[3 * i + 1] * vo[3 * i + 1] + vo[3 * i + 2] * vo[3 * i + 2];
-        v[11] = 0.5 * p.mass[i] * (a + b) / 2.0;
-        v[10] = 0.5 * p.mass[i] * a;   // E_kin_new (temperature.rs:81), for the thermostat
+// One md step's update of one original charge group (thread per group): the
+// total force (k_assemble), LeapFrogVelocity with the Berendsen scale and
+// LeapFrogPosition (k_leapfrog), SETTLE for rigid waters (k_settle), the E_kin
+// terms per group (k_ekin), and - when the next step keeps the pair list - the
+// sorted copies xq / xf it reads (k_gather).  A frozen job keeps its state.
+template <bool MIXED>
+__device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t with_long) {
+    int32_t a = p.o2s[i];
+    double f[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (MIXED) {
+            long long t = p.fi_s[3 * a + c];
+            if (with_long) t += p.fi_long_s[3 * a + c];
+            f[c] = (double)t * FX_INV;
+        } else {
+            f[c] = p.f_s[3 * a + c];
+            if (with_long) f[c] += p.f_long_s[3 * a + c];
+        }
+        f[c] += p.f_x[3 * i + c];
     }
-    write_row(v, p.rows, SLOT_KIN, p.nrow_blocks);
+    return mk(f[0], f[1], f[2]);
 }
+
+template <bool MIXED>
+__global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int32_t step, int32_t with_long,
+                                                     int32_t write_xq) {
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncg) return;
+    int32_t jb = p.cg_job[c];
+    const JobDev& J = p.jobs[jb];
+    const int32_t s = p.cg_start[c], n = p.cg_n[c];
+    const bool frozen = step >= J.stop;
+    const double dt = J.dt;
+    const double lam = J.thermo ? p.therm_scale[jb] : 1.0;
+    double ek = 0.0, ekn = 0.0;
+    auto step_atom = [&](int32_t i, v3& xo, v3& xn, v3& vo, v3& vn) {
+        xo = ldv(p.x_cur, i);
+        vo = ldv(p.v_cur, i);
+        if (frozen) {
+            xn = xo;
+            vn = vo;
+            return;
+        }
+        v3 f = total_force<MIXED>(p, i, with_long);
+        double im = p.inv_mass[i];
+        if (J.thermo) vn = mk((vo.x + f.x * im * dt) * lam, (vo.y + f.y * im * dt) * lam, (vo.z + f.z * im * dt) * lam);
+        else vn = mk(vo.x + f.x * im * dt, vo.y + f.y * im * dt, vo.z + f.z * im * dt);
+        xn = mk(xo.x + vn.x * dt, xo.y + vn.y * dt, xo.z + vn.z * dt);
+    };
+    auto finish_atom = [&](int32_t i, v3 xn, v3 vo, v3 vn, v3 x_first) {
+        stv(p.x_new, i, xn);
+        stv(p.v_new, i, vn);
+        double a2 = vn.x * vn.x + vn.y * vn.y + vn.z * vn.z;
+        double b2 = vo.x * vo.x + vo.y * vo.y + vo.z * vo.z;
+        ek += 0.5 * p.mass[i] * (a2 + b2) / 2.0;   // temperature.rs:76-80
+        ekn += 0.5 * p.mass[i] * a2;
+        if (write_xq) {
+            int32_t a = p.o2s[i];
+            double q = p.q[i];
+            p.xq[a] = make_double4(xn.x, xn.y, xn.z, q);
+            if (MIXED)
+                p.xf[a] = make_float4((float)(xn.x - x_first.x), (float)(xn.y - x_first.y), (float)(xn.z - x_first.z),
+                                      (float)q);
+        }
+    };
+    if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
+        v3 xo[3], xn[3], vo[3], vn[3];
+        for (int k = 0; k < 3; ++k) step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
+        if (!frozen && !settle_one(J, xo, xn, vn)) atomicMin(p.settle_fail + jb, step);
+        for (int k = 0; k < 3; ++k) finish_atom(s + k, xn[k], vo[k], vn[k], xn[0]);
+    } else {
+        v3 x_first = mk(0.0, 0.0, 0.0);
+        for (int32_t k = 0; k < n; ++k) {
+            v3 xo, xn, vo, vn;
+            step_atom(s + k, xo, xn, vo, vn);
+            if (k == 0) x_first = xn;
+            finish_atom(s + k, xn, vo, vn, x_first);
+        }
+    }
+    p.cg_ek[2 * (int64_t)c] = ek;
+    p.cg_ek[2 * (int64_t)c + 1] = ekn;
+}
+
 
 // Per-group solvent-solvent records summed in VV_PARTS contiguous parts per job
 // (grid VV_PARTS x njobs), so that the per-job reduction reads few values.
-__global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_virial) {
-    __shared__ double lds[17 * 11];
+// Per job and part: columns 0-10 from the solvent-solvent records (sorted
+// groups), 11-12 the E_kin terms of k_integrate (original groups).
+__global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_virial, int32_t with_kin) {
+    __shared__ double lds[17 * PART_W];
     const JobDev& J = p.jobs[blockIdx.y];
     int64_t n = J.n_cg;
     int64_t k0 = J.cg_off + n * blockIdx.x / VV_PARTS, k1 = J.cg_off + n * (blockIdx.x + 1) / VV_PARTS;
-    double v[11] = {0};
+    double v[PART_W] = {0};
     const int nc = with_virial ? 11 : 2;
     for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         const double* r = p.vv_e + 11 * k;
         for (int c = 0; c < nc; ++c) v[c] += r[c];
+        if (with_kin) {
+            v[11] += p.cg_ek[2 * k];
+            v[12] += p.cg_ek[2 * k + 1];
+        }
     }
-    block_sum<11>(v, lds);
-    if (threadIdx.x < 11) p.vv_part[((int64_t)blockIdx.y * VV_PARTS + blockIdx.x) * 11 + threadIdx.x] = lds[16 * 11 + threadIdx.x];
+    block_sum<PART_W>(v, lds);
+    if (threadIdx.x < PART_W)
+        p.vv_part[((int64_t)blockIdx.y * VV_PARTS + blockIdx.x) * PART_W + threadIdx.x] = lds[16 * PART_W + threadIdx.x];
 }
 
 // One block per job: sums the job's partial rows in a fixed order and writes
@@ -1499,29 +1566,27 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
             v[12] += q[11];
         }
     }
-    // solvent-solvent pairs: the job's VV_PARTS partial sums of the per-group records
+    // solvent-solvent pairs and E_kin: the job's VV_PARTS partial sums
     if (threadIdx.x < VV_PARTS) {
-        const double* r = p.vv_part + ((int64_t)jb * VV_PARTS + threadIdx.x) * 11;
+        const double* r = p.vv_part + ((int64_t)jb * VV_PARTS + threadIdx.x) * PART_W;
 #pragma unroll
         for (int c = 0; c < 11; ++c) v[c] += r[c];
+        if (with_kin) {
+            v[11] += r[11];
+            v[13] += r[12];
+        }
     }
     for (int s = 0; s < NSLOT; ++s) {
-        if (s == SLOT_VV) continue;
+        if (s == SLOT_VV || s == SLOT_KIN) continue;
         if (s == SLOT_LONG && !with_long) continue;
-        if (s == SLOT_KIN && !with_kin) continue;
         const int32_t per = BLOCK;
         int32_t b0 = J.atom_off / per;
         int32_t b1 = (J.atom_off + J.n_atoms + per - 1) / per;
         for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
             const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
-            if (s == SLOT_KIN) {
-                v[11] += r[11];
-                v[13] += r[10];
-            } else {
 #pragma unroll
-                for (int c = 0; c < 11; ++c) v[c] += r[c];
-                v[12] += r[11];
-            }
+            for (int c = 0; c < 11; ++c) v[c] += r[c];
+            v[12] += r[11];
         }
     }
     block_sum<W>(v, lds);
@@ -1688,21 +1753,17 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
                      hipStream_t s) {
     hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, mixed);
 }
+void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t step, int32_t with_long, int32_t write_xq,
+                      int32_t mixed, hipStream_t s) {
+    if (mixed) hipLaunchKernelGGL(k_integrate<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, step, with_long, write_xq);
+    else hipLaunchKernelGGL(k_integrate<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, step, with_long, write_xq);
+}
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_therm, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
-void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s) {
-    hipLaunchKernelGGL(k_leapfrog, grid_for(npad), dim3(BLOCK), 0, s, p, npad, step);
-}
-void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s) {
-    if (nmol > 0) hipLaunchKernelGGL(k_settle, grid_for(nmol), dim3(BLOCK), 0, s, p, nmol, step);
-}
-void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s) {
-    hipLaunchKernelGGL(k_ekin, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-}
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
                    int32_t njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(BLOCK), 0, s, p, p.vv_virial);
+    hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(BLOCK), 0, s, p, p.vv_virial, with_kin);
     hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, step, njobs);
 }
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,

@@ -23,7 +23,6 @@ struct SysPtrs {
     const int32_t* o14;
     const int32_t* o14s_off;   // symmetric 1-4 entries, bit 31 = owner is the lower atom
     const int32_t* o14s;
-    const int32_t* mol_first;  // SETTLE: first atom of each solvent water
     // state (original order)
     const double* x_cur;
     double* x_new;
@@ -72,7 +71,8 @@ struct SysPtrs {
     double* f_x;
     double* rows;
     double* vv_e;              // per sorted group: solvent-solvent e_lj, e_crf, virial[9]
-    double* vv_part;           // per job: VV_PARTS partial sums of vv_e
+    double* vv_part;           // per job: VV_PARTS partial sums of vv_e and cg_ek
+    double* cg_ek;             // per original group: E_kin, E_kin_new of its atoms
     const int32_t* sol_atom;   // nsol: original index of every solute atom, job by job
     int32_t nsol;
     double* sol_e;             // 2 x nsol x ROW: per solute atom, short then long pairs
@@ -109,10 +109,10 @@ void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
-void launch_leapfrog(const SysPtrs& p, int32_t npad, int32_t step, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
-void launch_settle(const SysPtrs& p, int32_t nmol, int32_t step, hipStream_t s);
-void launch_ekin(const SysPtrs& p, int32_t npad, hipStream_t s);
+// leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
+void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t step, int32_t with_long, int32_t write_xq,
+                      int32_t mixed, hipStream_t s);
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
                    int32_t njobs, hipStream_t s);
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,

@@ -150,35 +150,19 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
     int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
     int32_t local = (int32_t)p.key_out[k] - J.cell_off;
     int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
-    // Cells are at least half the list cutoff wide: every partner lies within
-    // two cells along each axis (all cells along an axis with fewer than five).
-    // Cells farther from the group's reference point than the cutoff (with a
-    // margin; the exact test below decides) are skipped: rows of cells in (y, z)
-    // whose distance alone exceeds it, and the x ends of the remaining rows.
-    // dist[a][d] = distance along axis a from the point to cell offset d - 2
-    // (0 for axes searched whole).
-    const int32_t nd3[3] = {nx, ny, nz};
-    double dist[3][2 * CELL_R + 1];
-    const double rcull = sqrt(c2l) * (1.0 + 1e-7) + 1e-9, rcull2 = rcull * rcull;
-    {
-        const double rr[3] = {rk.x, rk.y, rk.z};
+    // cells are at least half the list cutoff wide: every partner lies within
+    // two cells along each axis (all cells along an axis with fewer than five)
+    int32_t lyz[2][2 * CELL_R + 1], nyz[2];
+    const int32_t nd[2] = {ny, nz};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int32_t n = nd3[a];
-            double u = 0.0;
-            if (box.kind) {
-                u = rr[a] * box.invL[a];   // wrapped as k_cg_ref wraps it
-                u = u - floor(u);
-            }
-#pragma unroll
-            for (int d = -CELL_R; d <= CELL_R; ++d) {
-                double dd = 0.0;
-                if (n >= 2 * CELL_R + 1) {
-                    double lo = (double)(cc[a] + d) / n, hi = (double)(cc[a] + d + 1) / n;
-                    dd = fmax(0.0, fmax(lo - u, u - hi)) * box.L[a];
-                }
-                dist[a][d + CELL_R] = dd;
-            }
+    for (int a = 0; a < 2; ++a) {
+        int32_t n = nd[a];
+        if (n >= 2 * CELL_R + 1) {
+            nyz[a] = 2 * CELL_R + 1;
+            for (int d = -CELL_R; d <= CELL_R; ++d) lyz[a][d + CELL_R] = (cc[a + 1] + n + d) % n;
+        } else {
+            nyz[a] = n;
+            for (int d = 0; d < n; ++d) lyz[a][d] = d;
         }
     }
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -236,40 +220,25 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
             ngen += __popcll(bg);
         }
     };
-    const int32_t nz_l = nz >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : nz;
-    const int32_t ny_l = ny >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : ny;
-    for (int32_t iz = 0; iz < nz_l; ++iz) {
-        const bool fz = nz >= 2 * CELL_R + 1;
-        const int32_t cz = fz ? (cc[2] + nz + iz - CELL_R) % nz : iz;
-        const double dz = fz ? dist[2][iz] : 0.0;
-        for (int32_t iy = 0; iy < ny_l; ++iy) {
-            const bool fy = ny >= 2 * CELL_R + 1;
-            const int32_t cy = fy ? (cc[1] + ny + iy - CELL_R) % ny : iy;
-            const double dy = fy ? dist[1][iy] : 0.0;
-            const double dyz2 = dy * dy + dz * dz;
-            if (dyz2 > rcull2) continue;
+    for (int32_t iz = 0; iz < nyz[1]; ++iz)
+        for (int32_t iy = 0; iy < nyz[0]; ++iy) {
             // the x cells of one (y, z) row are contiguous in sorted order
-            int32_t row = J.cell_off + nx * (cy + ny * cz);
+            int32_t row = J.cell_off + nx * (lyz[0][iy] + ny * lyz[1][iz]);
             if (nx < 2 * CELL_R + 1) {
                 scan_run(p.cell_start[row], p.cell_start[row + nx]);
-                continue;
-            }
-            const double rx2 = rcull2 - dyz2;
-            int32_t dlo = -CELL_R, dhi = CELL_R;
-            while (dlo < 0 && dist[0][dlo + CELL_R] * dist[0][dlo + CELL_R] > rx2) ++dlo;
-            while (dhi > 0 && dist[0][dhi + CELL_R] * dist[0][dhi + CELL_R] > rx2) --dhi;
-            int32_t x0 = cc[0] + dlo, x1 = cc[0] + dhi;
-            if (x0 < 0) {
-                scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
-                scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
-            } else if (x1 >= nx) {
-                scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
-                scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
             } else {
-                scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
+                int32_t x0 = cc[0] - CELL_R, x1 = cc[0] + CELL_R;
+                if (x0 < 0) {
+                    scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
+                    scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
+                } else if (x1 >= nx) {
+                    scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
+                    scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
+                } else {
+                    scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
+                }
             }
         }
-    }
     // per-group results; totals come from k_totals (one atomic per block there:
     // thousands of waves adding to the same address serialise at the L2)
 #pragma unroll
@@ -997,6 +966,15 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
+// The partner's share (minus the given pair force) of a pair evaluated by its
+// perturbed first atom, added to the partner's int64 fixed-point force: integer
+// additions commute, so the result does not depend on their order.
+__device__ __forceinline__ void partner_fx(const SysPtrs& p, int32_t sb, double fx, double fy, double fz) {
+    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb), (unsigned long long)to_fx(-fx));
+    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb + 1), (unsigned long long)to_fx(-fy));
+    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb + 2), (unsigned long long)to_fx(-fz));
+}
+
 // Mixed-precision counterpart of gen_atom64: FP32 pair arithmetic on offsets
 // from the groups' first atoms (their nearest-image vector taken in FP64), FP64
 // sums; pairs with a perturbed first atom entirely in FP64.  Energies from the
@@ -1027,11 +1005,17 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
         bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
         int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
         if (lng != LONG) {
-            if (!pchk) continue;
+            // only the pair's first atom, when perturbed, takes the delta, for both
+            // atoms (the partner's share as an exact fixed-point atomic add)
+            if (!pchk || canon == 0.f || !p.pflag[o]) continue;
             for (int32_t b = 0; b < nm; ++b) {
                 int32_t ob = p.s2o[sm + b];
-                int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
-                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+                double lx = 0.0, ly = 0.0, lz = 0.0;
+                pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
+                fx += lx;
+                fy += ly;
+                fz += lz;
+                partner_fx(p, sm + b, lx, ly, lz);
             }
             continue;
         }
@@ -1046,13 +1030,20 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
             double4 xb = p.xq[sm + b];
             if (pchk) {
                 // a perturbed first atom: the state-A pair in FP64 (so that the
-                // FP64 delta below cancels it exactly), then the delta
+                // FP64 delta below cancels it exactly) and the delta, both atoms'
+                // shares, by the first atom's lanes only
                 int32_t ob = p.s2o[sm + b];
-                int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                int32_t fi = canon != 0.f ? o : ob;
                 if (p.pflag[fi]) {
-                    gen_pair(p, J, v, fx, fy, fz, canon != 0.f, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
+                    if (canon == 0.f) continue;
+                    double lx = 0.0, ly = 0.0, lz = 0.0;
+                    gen_pair(p, J, v, lx, ly, lz, true, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
                              p.s_type[sm + b], xa.w, xb.w, VIR);
-                    pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
+                    pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
+                    fx += lx;
+                    fy += ly;
+                    fz += lz;
+                    partner_fx(p, sm + b, lx, ly, lz);
                     continue;
                 }
             }
@@ -1090,11 +1081,17 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
             if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             double4 xb = p.xq[sk + b];
             if (pown) {
-                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                int32_t fi = canon ? o : ob;
                 if (p.pflag[fi]) {
-                    gen_pair(p, J, v, fx, fy, fz, canon, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
+                    if (!canon) continue;
+                    double lx = 0.0, ly = 0.0, lz = 0.0;
+                    gen_pair(p, J, v, lx, ly, lz, true, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
                              p.s_type[sk + b], xa.w, xb.w, VIR);
-                    pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                    pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
+                    fx += lx;
+                    fy += ly;
+                    fz += lz;
+                    partner_fx(p, sk + b, lx, ly, lz);
                     continue;
                 }
             }
@@ -1140,21 +1137,23 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
-// Solute atoms: one wave per atom (global solute index w, original atom
-// sol_atom[w]), the 64 lanes taking every 64th entry; forces and energies summed
-// over the wave with a fixed xor butterfly.  Energies go to the per-solute-atom
-// record sol_e[(LONG ? nsol : 0) + w] that k_reduce adds per job.
+// Solute atoms: one block (four waves) per atom (global solute index w, original
+// atom sol_atom[w]), the 256 lanes taking every 256th entry; forces and energies
+// summed over each wave with a fixed xor butterfly and over the waves in order.
+// Energies go to the per-solute-atom record sol_e[(LONG ? nsol : 0) + w] that
+// k_reduce adds per job.
 template <bool LONG, bool MIXED, bool VIR>
 __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int32_t virial) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int32_t w = blockIdx.x * (BLOCK / WAVE) + (threadIdx.x / WAVE);
-    if (w >= nsol) return;   // wave-uniform
+    constexpr int NV = ROW + 3;
+    __shared__ double lds[BLOCK / WAVE][NV];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const int32_t w = blockIdx.x;
     double v[ROW] = {0};
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int32_t o = p.sol_atom[w];
     const int32_t a = p.o2s[o];
-    if (MIXED) gen_atom32<LONG, VIR>(p, a, o, lane, WAVE, v, fx, fy, fz);
-    else gen_atom64<LONG>(p, a, o, lane, WAVE, virial, v, fx, fy, fz);
+    if (MIXED) gen_atom32<LONG, VIR>(p, a, o, threadIdx.x, BLOCK, v, fx, fy, fz);
+    else gen_atom64<LONG>(p, a, o, threadIdx.x, BLOCK, virial, v, fx, fy, fz);
 #pragma unroll
     for (int m = 1; m < WAVE; m <<= 1) {
         fx += __shfl_xor(fx, m, WAVE);
@@ -1164,31 +1163,47 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
         for (int c = 0; c < ROW; ++c) v[c] += __shfl_xor(v[c], m, WAVE);
     }
     if (lane == 0) {
+        lds[wv][0] = fx;
+        lds[wv][1] = fy;
+        lds[wv][2] = fz;
+#pragma unroll
+        for (int c = 0; c < ROW; ++c) lds[wv][3 + c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) t[c] = lds[0][c];
+        for (int q = 1; q < BLOCK / WAVE; ++q)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) t[c] += lds[q][c];
         if (MIXED) {
-            long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
+            long long ix = to_fx(t[0]), iy = to_fx(t[1]), iz = to_fx(t[2]);
             if (LONG) {
                 p.fi_long_s[3 * a] = ix;
                 p.fi_long_s[3 * a + 1] = iy;
                 p.fi_long_s[3 * a + 2] = iz;
             } else {
-                p.fi_s[3 * a] += ix;
-                p.fi_s[3 * a + 1] += iy;
-                p.fi_s[3 * a + 2] += iz;
+                // other solute blocks may add partner shares to this atom at the
+                // same time: atomic
+                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a), (unsigned long long)ix);
+                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a + 1), (unsigned long long)iy);
+                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a + 2), (unsigned long long)iz);
             }
         } else {
             if (LONG) {
-                p.f_long_s[3 * a] = fx;
-                p.f_long_s[3 * a + 1] = fy;
-                p.f_long_s[3 * a + 2] = fz;
+                p.f_long_s[3 * a] = t[0];
+                p.f_long_s[3 * a + 1] = t[1];
+                p.f_long_s[3 * a + 2] = t[2];
             } else {
-                p.f_s[3 * a] += fx;
-                p.f_s[3 * a + 1] += fy;
-                p.f_s[3 * a + 2] += fz;
+                p.f_s[3 * a] += t[0];
+                p.f_s[3 * a + 1] += t[1];
+                p.f_s[3 * a + 2] += t[2];
             }
         }
         double* r = p.sol_e + ((int64_t)(LONG ? nsol : 0) + w) * ROW;
 #pragma unroll
-        for (int c = 0; c < ROW; ++c) r[c] = v[c];
+        for (int c = 0; c < ROW; ++c) r[c] = t[3 + c];
     }
 }
 
@@ -1724,7 +1739,7 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
     // solvent atoms a lane each, then solute atoms a wave each (after: the long
     // pass of k_gen writes zeros for the solute atoms)
     const int32_t nsol = p.nsol;
-    dim3 gs((unsigned)((nsol + BLOCK / WAVE - 1) / (BLOCK / WAVE)));
+    dim3 gs((unsigned)nsol);   // a block per solute atom
     if (mixed) {
         if (lng) {
             hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);

@@ -97,6 +97,7 @@ struct gromos_amd_system {
     int32_t* d_sol_atom = nullptr;
     int32_t nsol = 0;
     int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
+    int32_t vv_neutral = 0;  // every job's solvent molecule has zero net charge
     int32_t vv_lanes = 0;  // lanes per solvent molecule in the pair kernel (0: not tuned yet)
     int64_t tuned_key = -1;
     int32_t* d_settle_fail = nullptr;

@@ -362,6 +362,18 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             for (int i = 0; i < J.aps; ++i)
                 for (int jj = 0; jj < J.aps; ++jj)
                     if ((i || jj) && (J.sv_c6[i * 4 + jj] != 0.0 || J.sv_c12[i * 4 + jj] != 0.0)) S->vv_lj00 = 0;
+    // neutral solvent molecules (the charges of one molecule sum to zero exactly)
+    S->vv_neutral = 1;
+    for (const JobImage* im : imgs)
+        if (im->solvent) {
+            const int32_t f = im->J.n_solute_atoms, a = im->J.aps;
+            double sum = 0.0;
+            for (int i = 0; i < a; ++i) sum += im->q[f + i];
+            if (sum != 0.0) S->vv_neutral = 0;
+        }
+    // GROMOS_AMD_VV_PAIRWISE_RF=1: the per-atom-pair RF form everywhere (checks)
+    if (const char* env = std::getenv("GROMOS_AMD_VV_PAIRWISE_RF"))
+        if (std::atoi(env) == 1) S->vv_neutral = 0;
     S->cell_bits = 1;
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
     S->n_papairs = (int32_t)papairs.size() / 4;
@@ -579,7 +591,8 @@ static std::map<int64_t, int32_t> g_tuned;
 static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t mixed) {
     int32_t bucket = 0;
     while ((1 << bucket) < S->ncg) ++bucket;
-    int64_t key = (((((int64_t)bucket * 8 + aps) * 2 + mixed) * 2 + S->vv_lj00) * 2 + S->do_virial);
+    const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
+    int64_t key = (((((int64_t)bucket * 8 + aps) * 2 + mixed) * 4 + form) * 2 + S->do_virial);
     if (S->vv_lanes && S->tuned_key == key) return 0;
     if (const char* env = std::getenv("GROMOS_AMD_VV_LANES")) {
         int v = std::atoi(env);
@@ -608,7 +621,7 @@ static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t 
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], S->stream));
     for (int c = 0; c < 3; ++c) {
-        launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, cand[c], S->stream);
+        launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, cand[c], S->stream);
         HIPCHK(hipEventRecord(ev[c + 1], S->stream));
     }
     HIPCHK(hipEventSynchronize(ev[3]));
@@ -665,7 +678,8 @@ int forcefield(gromos_amd_system* S, bool assemble) {
             break;
         }
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
-    launch_vv(p, S->ncg, aps, S->vv_lj00, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
+    const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
+    launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
     launch_gen(p, npad, false, S->do_virial, mixed, st);
     S->pair_count += (int64_t)S->host_counters[0];
     if (S->twin && (rebuild || !S->long_computed)) {

@@ -507,8 +507,21 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 
 // Mixed-precision counterpart: FP32 pair arithmetic on offsets from the
 // molecules' first atoms (their nearest-image vector taken in FP64), FP64 per-
-// lane sums, int64 fixed-point per-atom result.
-template <int APS, bool LJ00, bool VIR, int NCHM>
+// group sums, int64 fixed-point per-atom result.
+//
+// NEUT (every solvent molecule has zero net charge, as SPC water): the reaction-
+// field polynomial is summed per molecule pair instead of per atom pair.  With
+// r_ij = u_i - w_j and sum_i q_i = sum_j q_j = 0,
+//   sum_ij q_i q_j (-crf_2cut3i r_ij^2 - crf_cut) =  2 crf_2cut3i  mu_A . mu_B
+//   sum_j  q_i q_j  crf_cut3i r_ij               = -q_i crf_cut3i  mu_B
+//   sum_ij q_i q_j  crf_cut3i r_ij (x) r_ij       = -crf_cut3i (mu_A (x) mu_B + mu_B (x) mu_A)
+// where mu = sum q x is the molecule's dipole (origin independent for a neutral
+// molecule), so per atom pair only the Coulomb 1/r and LJ terms remain.  This
+// is the same sum as the reference's n x n loop (innerloops.rs:256-299) in
+// exact arithmetic, and in floating point it drops the cancelling constant
+// terms.  Atom pairs closer than 1e-5 nm (never two molecules) keep zero 1/r
+// terms but, unlike the reference, their RF polynomial is not dropped.
+template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
 __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
     const int32_t lb = xcd_block((int32_t)(((int64_t)ncg * NCHM + BLOCK - 1) / BLOCK));
     if (lb < 0) return;
@@ -540,6 +553,19 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
         float4 da[APS];
 #pragma unroll
         for (int i = 0; i < APS; ++i) da[i] = p.xf[sk + i];
+        // NEUT: own dipole and the per-atom RF force factor -q_i fpepsi crf_cut3i
+        double mua[3] = {0.0, 0.0, 0.0};
+        double kf[APS];
+        if (NEUT) {
+#pragma unroll
+            for (int i = 1; i < APS; ++i) {
+                mua[0] += (double)da[i].w * (double)da[i].x;
+                mua[1] += (double)da[i].w * (double)da[i].y;
+                mua[2] += (double)da[i].w * (double)da[i].z;
+            }
+#pragma unroll
+            for (int i = 0; i < APS; ++i) kf[i] = -(double)da[i].w * J.fpepsi * J.crf.crf_cut3i;
+        }
         d3 pk0 = xyz(p.xq[sk]);
         const int32_t n = e1 - e0;
         // Groups of VV_GROUP consecutive entries are summed in FP64 and then
@@ -552,6 +578,8 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
             for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
 #pragma unroll
             for (int c = 0; c < 11; ++c) en[c] = 0.0;
+            float msx = 0.f, msy = 0.f, msz = 0.f;   // NEUT: sum of partner dipoles
+            float mcx = 0.f, mcy = 0.f, mcz = 0.f;   //       over canonical entries
             const int32_t g1 = min(g0 + VV_GROUP, n);
             for (int32_t e = e0 + g0; e < e0 + g1; ++e) {
                 uint32_t ent = p.vv_ent[e];
@@ -562,6 +590,21 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
                 float4 db[APS];
 #pragma unroll
                 for (int j = 0; j < APS; ++j) db[j] = p.xf[sm + j];
+                if (NEUT) {
+                    float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+                    for (int j = 1; j < APS; ++j) {
+                        ux = fmaf(db[j].w, db[j].x, ux);
+                        uy = fmaf(db[j].w, db[j].y, uy);
+                        uz = fmaf(db[j].w, db[j].z, uz);
+                    }
+                    msx += ux;
+                    msy += uy;
+                    msz += uz;
+                    mcx = fmaf(canon, ux, mcx);
+                    mcy = fmaf(canon, uy, mcy);
+                    mcz = fmaf(canon, uz, mcz);
+                }
                 float el = 0.f, ec = 0.f;
                 float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -576,29 +619,48 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
                         // hardware reciprocal square root (1 ulp) refined by one Newton step
                         float y = __builtin_amdgcn_rsqf(r2);
                         float inv_r = fmaf(0.5f * y, fmaf(-r2 * y, y, 1.0f), y);
-                        float inv_r2 = inv_r * inv_r;
                         float q = qq[i][j];
-                        float ecr = q * (inv_r - fmaf(c2, r2, cc));
-                        float fm = q * fmaf(inv_r, inv_r2, c3);
-                        float elj = 0.f;
-                        if (has_lj<APS, LJ00>(i, j)) {
-                            float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
-                            float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
-                            float inv_r6 = inv_r2 * inv_r2 * inv_r2;
-                            elj = fmaf(c12, inv_r6, -c6) * inv_r6;
-                            fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                        float fm;
+                        if (NEUT) {
+                            inv_r = ok ? inv_r : 0.0f;
+                            float inv_r2 = inv_r * inv_r;
+                            ec = fmaf(q, inv_r, ec);
+                            fm = q * (inv_r * inv_r2);
+                            if (has_lj<APS, LJ00>(i, j)) {
+                                float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
+                                float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
+                                float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                                el = fmaf(fmaf(c12, inv_r6, -c6), inv_r6, el);
+                                fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                            }
+                        } else {
+                            float inv_r2 = inv_r * inv_r;
+                            float ecr = q * (inv_r - fmaf(c2, r2, cc));
+                            fm = q * fmaf(inv_r, inv_r2, c3);
+                            float elj = 0.f;
+                            if (has_lj<APS, LJ00>(i, j)) {
+                                float c6 = LJ00 ? c6_00 : (float)J.sv_c6[i * 4 + j];
+                                float c12 = LJ00 ? c12_00 : (float)J.sv_c12[i * 4 + j];
+                                float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                                elj = fmaf(c12, inv_r6, -c6) * inv_r6;
+                                fm = fmaf(fmaf(12.0f * c12, inv_r6, -6.0f * c6) * inv_r6, inv_r2, fm);
+                            }
+                            fm = ok ? fm : 0.0f;
+                            el += ok ? elj : 0.0f;
+                            ec += ok ? ecr : 0.0f;
                         }
-                        fm = ok ? fm : 0.0f;
-                        float gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                        gxs += gx;
-                        gys += gy;
-                        gzs += gz;
-                        el += ok ? elj : 0.0f;
-                        ec += ok ? ecr : 0.0f;
                         if (VIR) {
+                            float gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                            gxs += gx;
+                            gys += gy;
+                            gzs += gz;
                             w[0] = fmaf(rx, gx, w[0]); w[1] = fmaf(rx, gy, w[1]); w[2] = fmaf(rx, gz, w[2]);
                             w[3] = fmaf(ry, gx, w[3]); w[4] = fmaf(ry, gy, w[4]); w[5] = fmaf(ry, gz, w[5]);
                             w[6] = fmaf(rz, gx, w[6]); w[7] = fmaf(rz, gy, w[7]); w[8] = fmaf(rz, gz, w[8]);
+                        } else {
+                            gxs = fmaf(rx, fm, gxs);
+                            gys = fmaf(ry, fm, gys);
+                            gzs = fmaf(rz, fm, gzs);
                         }
                     }
                     f[i][0] += (double)gxs;
@@ -610,6 +672,23 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
                 if (VIR) {
 #pragma unroll
                     for (int c = 0; c < 9; ++c) en[2 + c] += (double)(canon * w[c]);
+                }
+            }
+            if (NEUT) {
+                const double ms[3] = {(double)msx, (double)msy, (double)msz};
+                const double mc[3] = {(double)mcx, (double)mcy, (double)mcz};
+#pragma unroll
+                for (int i = 0; i < APS; ++i)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) f[i][c] += kf[i] * ms[c];
+                const double fe = J.fpepsi;
+                en[1] += 2.0 * J.crf.crf_2cut3i * fe * (mua[0] * mc[0] + mua[1] * mc[1] + mua[2] * mc[2]);
+                if (VIR) {
+                    const double kv = -J.crf.crf_cut3i * fe;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) en[2 + 3 * a + b] += kv * (mua[a] * mc[b] + mc[a] * mua[b]);
                 }
             }
 #pragma unroll
@@ -1704,35 +1783,39 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 // start / stop events (optional) are recorded by the dispatch itself
 // (hipExtLaunchKernelGGL), so timing adds no packets between kernels
 template <int APS, bool LJ00, int L>
-static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, hipEvent_t e0,
-                        hipEvent_t e1, hipStream_t s) {
+static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
+                        hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
     dim3 g = xcd_grid(((int64_t)ncg * L + BLOCK - 1) / BLOCK);
-    if (mixed) {
-        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
-        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+    if (mixed && neutral) {
+        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+    } else if (mixed) {
+        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
     } else {
         if (virial) hipExtLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
         else hipExtLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
     }
 }
 template <int APS, bool LJ00>
-static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t lanes,
-                          hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
-    if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, e0, e1, s);
-    else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, e0, e1, s);
-    else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, e0, e1, s);
+static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
+                          int32_t lanes, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+    if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, neutral, e0, e1, s);
+    else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, neutral, e0, e1, s);
+    else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, neutral, e0, e1, s);
 }
-void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
+void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    switch (aps * 2 + (lj00 ? 1 : 0)) {
-        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
-        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, lanes, e0, e1, s); break;
+    const int32_t neutral = (form & VV_NEUTRAL) ? 1 : 0;
+    switch (aps * 2 + ((form & VV_LJ00) ? 1 : 0)) {
+        case 2: launch_vv_aps<1, false>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 3: launch_vv_aps<1, true>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 4: launch_vv_aps<2, false>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 5: launch_vv_aps<2, true>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 8: launch_vv_aps<4, false>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 9: launch_vv_aps<4, true>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        case 7: launch_vv_aps<3, true>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
+        default: launch_vv_aps<3, false>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {

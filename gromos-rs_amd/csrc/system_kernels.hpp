@@ -103,8 +103,11 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
 // mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
+// form: VV_LJ00 (LJ on the first-first atom pair only) | VV_NEUTRAL (neutral
+// solvent molecules: the mixed kernel sums the RF polynomial per molecule pair);
 // lanes: lanes per solvent molecule (4, 8 or 16)
-void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t lj00, int32_t virial, int32_t mixed,
+constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
+void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
 void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);

@@ -314,6 +314,32 @@ def test_mixed_md_tracks_fp64():
     assert abs(dmx - d64) <= 0.02 * abs(e64[-1, 3]), (d64, dmx)
 
 
+@pytest.mark.parametrize("prec,pairwise_rf", [(rt.FP64, False), (rt.MIXED, False), (rt.MIXED, True)])
+def test_resident_virial_and_rf_forms(prec, pairwise_rf, monkeypatch):
+    """Virial of the device-resident path against the oracle.  The mixed kernel
+    sums the RF polynomial per neutral molecule pair (dipole form); the per-atom-
+    pair form (GROMOS_AMD_VV_PAIRWISE_RF=1) must give the same results within
+    the mixed tolerance."""
+    if pairwise_rf:
+        monkeypatch.setenv("GROMOS_AMD_VV_PAIRWISE_RF", "1")
+    s = gen.spc_water_box(1500, seed=15)
+    ds = rt.DeviceSystem(s, precision=prec, do_virial=True)
+    ds.pairlist_update()
+    g = ds.forcefield_apply()
+    ds.close()
+    r = oracle.single_point(oracle.OracleSystem(s))
+    vscale = float(np.abs(r["virial"]).max())
+    if prec == rt.FP64:
+        assert np.abs(g["virial"] - r["virial"]).max() <= 1e-10 * vscale
+        assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+        assert force_err(g["forces"], r["forces"]) < F_TOL
+    else:
+        assert np.abs(g["virial"] - r["virial"]).max() <= 1e-5 * vscale, (g["virial"], r["virial"])
+        mixed_energy_ok(g, r)
+        assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+        assert force_err_median(g["forces"], r["forces"]) < MX_F_MED
+
+
 # --------------------------------------------------------------------------
 # free energy: device-resident soft-core correction passes against the oracle
 # (soft-core CRF with charges, dH/dlambda, excluded / 1-4 / atom-pair passes are

@@ -541,7 +541,7 @@ int build_pairlist(gromos_amd_system* S) {
         if (int rc = S->reserve(S->d_vv_ent, (size_t)bv, true)) return rc;
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
-        launch_nbl(p, ncg, true, st);
+        launch_nbl(p, ncg, S->ncell, true, st);
         HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -550,7 +550,7 @@ int build_pairlist(gromos_amd_system* S) {
             need_exact = true;   // the fill pass left exact counts in vv_cnt / gen_cnt
         }
     } else {
-        launch_nbl(p, ncg, false, st);
+        launch_nbl(p, ncg, S->ncell, false, st);
     }
     if (need_exact) {
         if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
@@ -565,7 +565,7 @@ int build_pairlist(gromos_amd_system* S) {
         launch_caps(p, ncg, 1, st);
         uint64_t keep[4];
         std::memcpy(keep, S->host_counters, sizeof(keep));
-        launch_nbl(p, ncg, true, st);
+        launch_nbl(p, ncg, S->ncell, true, st);
         std::memcpy(S->host_counters, keep, sizeof(keep));
     }
     launch_caps(p, ncg, 2, st);

@@ -121,137 +121,222 @@ __device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
     return r.x * r.x + r.y * r.y + r.z * r.z;
 }
 
-// Neighbour search, one wave per sorted group k: the 64 lanes test 64
-// candidates of a cell at once (coalesced loads of the sorted reference points)
-// and the survivors are compacted with a ballot, so each group's entries are
-// written contiguously and in a fixed order (cells in z-y-x order, candidates
-// in sorted order).  27 cells, or all cells along an axis with fewer than 3.
-// Classification as the reference (pairlist.rs:252-415).  nearest_image is
-// exactly antisymmetric (IEEE subtraction and half-away rounding are), so the
-// squared distance is the reference's whichever group comes first.
+// Neighbour search, one block per cell.  The groups of a cell share its
+// stencil (the cells within two along each axis, all cells along an axis with
+// fewer than five), so the block stages the stencil's candidate groups in LDS,
+// NBL_PIECE at a time (coalesced loads by all four waves), and each wave tests
+// them against its share of the cell's groups (group s0 + w + 4 t in lane t of
+// wave w), the group's data broadcast from the lane that holds it.  Survivors
+// are compacted per group with a ballot, so each group's entries are written
+// contiguously and in a fixed order (cells in z-y-x order, candidates in
+// sorted order).  Classification as the reference (pairlist.rs:252-415).
+// nearest_image is exactly antisymmetric (IEEE subtraction and half-away
+// rounding are), so the squared distance is the reference's whichever group
+// comes first.
 // FILL = false: count only.  FILL = true: write at most cap[k] entries per
 // group, record the true counts and raise *overflow when a group had more.
-constexpr int CELL_R = 2;   // neighbour cells searched along each axis
+// Pair counts: the cell's total goes to the cell's first group, zeros to the
+// others (only their sum is used).
+constexpr int CELL_R = 2;        // neighbour cells searched along each axis
+constexpr int NBL_RUNS = 2 * (2 * CELL_R + 1) * (2 * CELL_R + 1);   // x runs of the stencil
+constexpr int NBL_PIECE = 512;   // candidates staged in LDS at a time
+constexpr int NBL_WAVES = BLOCK / WAVE;
+
+__device__ __forceinline__ int32_t bcast(int32_t v, int32_t t) { return __builtin_amdgcn_readlane(v, t); }
+__device__ __forceinline__ double bcast(double v, int32_t t) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)b, t);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(b >> 32), t);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 template <bool FILL>
-__global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncg) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int32_t lb = xcd_block((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE));
-    if (lb < 0) return;
-    const int32_t k = lb * (BLOCK / WAVE) + (threadIdx.x / WAVE);
-    if (k >= ncg) return;   // wave-uniform
-    int32_t ck = p.s_cg_orig[k];
-    const JobDev& J = p.jobs[p.cg_job[ck]];
+__global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
+    __shared__ double sx[NBL_PIECE], sy[NBL_PIECE], sz[NBL_PIECE];
+    __shared__ int32_t sm[NBL_PIECE], scm[NBL_PIECE], sst[NBL_PIECE], snn[NBL_PIECE];
+    __shared__ int32_t run_m0[NBL_RUNS], run_off[NBL_RUNS + 1];
+    const int32_t c = xcd_block(ncell);
+    if (c < 0) return;
+    const int32_t s0 = p.cell_start[c], s1 = p.cell_start[c + 1];
+    if (s0 >= s1) return;     // empty cell: the whole block leaves before any barrier
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[s0]]];
     const Box box = J.box;
     const double c2s = J.c2s, c2l = J.c2l;
     const int32_t cg_off = J.cg_off, nsol = J.n_solute_cg, aps = J.aps;
-    bool sol_k = ck - cg_off < nsol;
-    d3 rk{p.s_ref[3 * k], p.s_ref[3 * k + 1], p.s_ref[3 * k + 2]};
-    int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
-    int32_t local = (int32_t)p.key_out[k] - J.cell_off;
-    int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
-    // cells are at least half the list cutoff wide: every partner lies within
-    // two cells along each axis (all cells along an axis with fewer than five)
-    int32_t lyz[2][2 * CELL_R + 1], nyz[2];
-    const int32_t nd[2] = {ny, nz};
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        int32_t n = nd[a];
-        if (n >= 2 * CELL_R + 1) {
-            nyz[a] = 2 * CELL_R + 1;
-            for (int d = -CELL_R; d <= CELL_R; ++d) lyz[a][d + CELL_R] = (cc[a + 1] + n + d) % n;
-        } else {
-            nyz[a] = n;
-            for (int d = 0; d < n; ++d) lyz[a][d] = d;
+    const int32_t nx = J.ncell[0], ny = J.ncell[1], nz = J.ncell[2];
+    const int32_t local = c - J.cell_off;
+    const int32_t cc[3] = {local % nx, (local / nx) % ny, local / (nx * ny)};
+    // the stencil as x runs of sorted groups, in (z, y, x-segment) order
+    if (wv == 0) {
+        const int32_t nyy = ny >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : ny;
+        const int32_t nzz = nz >= 2 * CELL_R + 1 ? 2 * CELL_R + 1 : nz;
+        int32_t m0 = 0, cnt = 0;
+        const int32_t r = lane, rowi = r / 2, seg = r % 2;
+        if (r < NBL_RUNS && rowi < nyy * nzz) {
+            const int32_t iy = rowi % nyy, iz = rowi / nyy;
+            const int32_t ly = ny >= 2 * CELL_R + 1 ? (cc[1] + ny + iy - CELL_R) % ny : iy;
+            const int32_t lz = nz >= 2 * CELL_R + 1 ? (cc[2] + nz + iz - CELL_R) % nz : iz;
+            const int32_t row = J.cell_off + nx * (ly + ny * lz);
+            int32_t a0 = 0, a1 = 0;   // cell range [a0, a1) of this segment
+            if (nx < 2 * CELL_R + 1) {
+                if (seg == 0) { a0 = 0; a1 = nx; }
+            } else {
+                const int32_t x0 = cc[0] - CELL_R, x1 = cc[0] + CELL_R;
+                if (x0 < 0) {
+                    if (seg == 0) { a0 = x0 + nx; a1 = nx; } else { a0 = 0; a1 = x1 + 1; }
+                } else if (x1 >= nx) {
+                    if (seg == 0) { a0 = x0; a1 = nx; } else { a0 = 0; a1 = x1 - nx + 1; }
+                } else if (seg == 0) {
+                    a0 = x0;
+                    a1 = x1 + 1;
+                }
+            }
+            if (a1 > a0) {
+                m0 = p.cell_start[row + a0];
+                cnt = p.cell_start[row + a1] - m0;
+            }
         }
+        // exclusive scan of the run lengths over the wave
+        int32_t inc = cnt;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int32_t v = __shfl_up(inc, o, WAVE);
+            if (lane >= o) inc += v;
+        }
+        if (r < NBL_RUNS) {
+            run_m0[r] = m0;
+            run_off[r] = inc - cnt;
+        }
+        if (r == NBL_RUNS - 1) run_off[NBL_RUNS] = inc;
     }
+    __syncthreads();
+    const int32_t total = run_off[NBL_RUNS];
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int64_t nvv = 0, ngen = 0;
-    int64_t vv_base = 0, gen_base = 0, vv_cap = 0, gen_cap = 0;
-    if (FILL) {
-        vv_base = p.vv_off[k];
-        gen_base = p.gen_off[k];
-        vv_cap = p.vv_cap[k];
-        gen_cap = p.gen_cap[k];
-    }
-    uint64_t pairs_short = 0, pairs_long = 0;
-    int32_t nk = p.s_cg_n[k];
-    // candidates of a run of sorted groups [m0, m1), 64 at a time
-    auto scan_run = [&](int32_t m0, int32_t m1) {
-        for (int32_t base = m0; base < m1; base += WAVE) {
-            int32_t m = base + lane;
-            bool is_vv = false, is_gen = false;
-            uint32_t ent = 0;
-            if (m < m1 && m != k) {
-                d3 rm{p.s_ref[3 * m], p.s_ref[3 * m + 1], p.s_ref[3 * m + 2]};
-                d3 r = nim(box, rk, rm);
-                double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
-                if (!(d2 > c2l)) {
-                    int32_t cm = p.s_cg_orig[m];
-                    bool canon = ck < cm;
-                    bool lng = d2 > c2s;
-                    bool sol_m = cm - cg_off < nsol;
-                    uint32_t flags = canon ? ENT_CANON : 0u;
-                    if (!sol_k && !sol_m && !lng) {
-                        is_vv = true;
-                        // solvent partners: store the sorted first-atom index
-                        ent = (uint32_t)p.s_cg_start[m] | flags;
-                        if (canon) pairs_short += (uint64_t)aps * aps;
-                    } else {
-                        is_gen = true;
+    uint64_t pairs_short = 0;                        // wave-uniform: solvent pairs
+    uint64_t pairs_short_l = 0, pairs_long_l = 0;    // lane sums: solute pairs
+    for (int32_t rb = s0; rb < s1; rb += BLOCK) {
+        // this round's groups: lane t of wave w holds group rb + w + NBL_WAVES t
+        const int32_t nr = min(BLOCK, s1 - rb);
+        const int32_t gi = wv + NBL_WAVES * lane;
+        const bool own = gi < nr;
+        const int32_t k = own ? rb + gi : rb;
+        const double kx = p.s_ref[3 * k], ky = p.s_ref[3 * k + 1], kz = p.s_ref[3 * k + 2];
+        const int32_t ck = p.s_cg_orig[k], nk = p.s_cg_n[k];
+        const uint64_t own_mask = __ballot(own);
+        const uint64_t solk_mask = __ballot(own && ck - cg_off < nsol);
+        int32_t vv_base = 0, gen_base = 0, vv_end = 0, gen_end = 0;
+        if (FILL && own) {
+            vv_base = (int32_t)p.vv_off[k];
+            gen_base = (int32_t)p.gen_off[k];
+            vv_end = vv_base + (int32_t)p.vv_cap[k];
+            gen_end = gen_base + (int32_t)p.gen_cap[k];
+        }
+        int32_t vv_pos = vv_base, gen_pos = gen_base;
+        for (int32_t P = 0; P < total; P += NBL_PIECE) {
+            const int32_t plen = min(NBL_PIECE, total - P);
+            __syncthreads();   // the previous piece is consumed
+            for (int32_t q = threadIdx.x; q < plen; q += BLOCK) {
+                const int32_t g = P + q;
+                // run holding candidate g: the last run starting at or before g
+                int32_t lo = 0, hi = NBL_RUNS - 1;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (run_off[mid] <= g) lo = mid; else hi = mid - 1;
+                }
+                const int32_t m = run_m0[lo] + (g - run_off[lo]);
+                sx[q] = p.s_ref[3 * m];
+                sy[q] = p.s_ref[3 * m + 1];
+                sz[q] = p.s_ref[3 * m + 2];
+                sm[q] = m;
+                scm[q] = p.s_cg_orig[m];
+                sst[q] = p.s_cg_start[m];
+                snn[q] = p.s_cg_n[m];
+            }
+            __syncthreads();
+            if (!own_mask) continue;   // wave-uniform; the barriers above are reached by all
+            for (int32_t cb = 0; cb < plen; cb += WAVE) {
+                const int32_t q = cb + lane;
+                const bool valid = q < plen;
+                const int32_t qq = valid ? q : 0;
+                const d3 rm{sx[qq], sy[qq], sz[qq]};
+                const int32_t m = sm[qq], cm = scm[qq];
+                const bool sol_m = cm - cg_off < nsol;
+                const uint32_t vv_ent = (uint32_t)sst[qq];
+                const int32_t n_m = snn[qq];
+                uint64_t rest = own_mask;
+                while (rest) {
+                    const int32_t t = (int32_t)__builtin_ctzll(rest);
+                    rest &= rest - 1;
+                    const int32_t i = rb + wv + NBL_WAVES * t;
+                    const d3 rk{bcast(kx, t), bcast(ky, t), bcast(kz, t)};
+                    const int32_t ck_t = bcast(ck, t);
+                    const bool sol_k = (solk_mask >> t) & 1ull;
+                    const d3 r = nim(box, rk, rm);
+                    const double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                    const bool in = valid && m != i && !(d2 > c2l);
+                    const bool lng = d2 > c2s;
+                    const bool canon = ck_t < cm;
+                    const bool is_vv = in && !lng && !sol_k && !sol_m;
+                    const bool is_gen = in && !is_vv;
+                    const uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
+                    if (bv) {
+                        if (FILL) {
+                            const int32_t pv = bcast(vv_pos, t) + (int32_t)__popcll(bv & below);
+                            if (is_vv && pv < bcast(vv_end, t)) p.vv_ent[pv] = vv_ent | (canon ? ENT_CANON : 0u);
+                        }
+                        if (lane == t) vv_pos += (int32_t)__popcll(bv);
+                        pairs_short += (uint64_t)__popcll(bv & __ballot(canon)) * (uint64_t)(aps * aps);
+                    }
+                    if (bg) {
+                        // solute partners, and solvent pairs beyond the short cutoff
+                        uint32_t flags = canon ? ENT_CANON : 0u;
                         if (!sol_k && !sol_m) flags |= ENT_VVLONG;
                         if (lng) flags |= ENT_LONG;
                         if (sol_k && sol_m) flags |= ENT_SS;
-                        ent = (uint32_t)m | flags;
-                        if (canon) {
-                            uint64_t np = (uint64_t)nk * p.s_cg_n[m];
-                            if (lng) pairs_long += np; else pairs_short += np;
+                        if (FILL) {
+                            const int32_t pg = bcast(gen_pos, t) + (int32_t)__popcll(bg & below);
+                            if (is_gen && pg < bcast(gen_end, t)) p.gen_ent[pg] = (uint32_t)m | flags;
+                        }
+                        if (lane == t) gen_pos += (int32_t)__popcll(bg);
+                        if (is_gen && canon) {
+                            const uint64_t np = (uint64_t)bcast(nk, t) * (uint64_t)n_m;
+                            if (lng) pairs_long_l += np; else pairs_short_l += np;
                         }
                     }
                 }
             }
-            uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
-            if (FILL) {
-                int64_t pv = nvv + __popcll(bv & below), pg = ngen + __popcll(bg & below);
-                if (is_vv && pv < vv_cap) p.vv_ent[vv_base + pv] = ent;
-                if (is_gen && pg < gen_cap) p.gen_ent[gen_base + pg] = ent;
-            }
-            nvv += __popcll(bv);
-            ngen += __popcll(bg);
         }
-    };
-    for (int32_t iz = 0; iz < nyz[1]; ++iz)
-        for (int32_t iy = 0; iy < nyz[0]; ++iy) {
-            // the x cells of one (y, z) row are contiguous in sorted order
-            int32_t row = J.cell_off + nx * (lyz[0][iy] + ny * lyz[1][iz]);
-            if (nx < 2 * CELL_R + 1) {
-                scan_run(p.cell_start[row], p.cell_start[row + nx]);
-            } else {
-                int32_t x0 = cc[0] - CELL_R, x1 = cc[0] + CELL_R;
-                if (x0 < 0) {
-                    scan_run(p.cell_start[row + x0 + nx], p.cell_start[row + nx]);
-                    scan_run(p.cell_start[row], p.cell_start[row + x1 + 1]);
-                } else if (x1 >= nx) {
-                    scan_run(p.cell_start[row + x0], p.cell_start[row + nx]);
-                    scan_run(p.cell_start[row], p.cell_start[row + x1 - nx + 1]);
-                } else {
-                    scan_run(p.cell_start[row + x0], p.cell_start[row + x1 + 1]);
-                }
-            }
+        if (own) {
+            const int32_t nvv = vv_pos - vv_base, ngen = gen_pos - gen_base;
+            p.vv_cnt[k] = nvv;
+            p.gen_cnt[k] = ngen;
+            p.pc_short[k] = 0;
+            p.pc_long[k] = 0;
+            if (FILL && (nvv > vv_end - vv_base || ngen > gen_end - gen_base)) atomicOr(p.overflow, 1);
         }
-    // per-group results; totals come from k_totals (one atomic per block there:
-    // thousands of waves adding to the same address serialise at the L2)
+    }
+    // per-cell pair counts; totals come from k_totals (one atomic per block
+    // there: thousands of blocks adding to the same address serialise at the L2)
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) {
-        pairs_short += __shfl_xor(pairs_short, o, WAVE);
-        pairs_long += __shfl_xor(pairs_long, o, WAVE);
+        pairs_short_l += __shfl_xor(pairs_short_l, o, WAVE);
+        pairs_long_l += __shfl_xor(pairs_long_l, o, WAVE);
     }
+    __shared__ uint64_t red[2 * NBL_WAVES];
     if (lane == 0) {
-        p.vv_cnt[k] = nvv;
-        p.gen_cnt[k] = ngen;
-        p.pc_short[k] = (int64_t)pairs_short;
-        p.pc_long[k] = (int64_t)pairs_long;
-        if (FILL && (nvv > vv_cap || ngen > gen_cap)) atomicOr(p.overflow, 1);
+        red[2 * wv] = pairs_short + pairs_short_l;
+        red[2 * wv + 1] = pairs_long_l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b2 = 0;
+        for (int w = 0; w < NBL_WAVES; ++w) {
+            a += red[2 * w];
+            b2 += red[2 * w + 1];
+        }
+        p.pc_short[s0] = (int64_t)a;
+        p.pc_long[s0] = (int64_t)b2;
     }
 }
 
@@ -1769,11 +1854,11 @@ void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
     else if (mode == 1) hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
     else hipLaunchKernelGGL(k_keep_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
-void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s) {
-    // one wave per group
-    dim3 g = xcd_grid((ncg + BLOCK / WAVE - 1) / (BLOCK / WAVE));
-    if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncg);
-    else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncg);
+void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s) {
+    // one block per cell
+    dim3 g = xcd_grid(ncell);
+    if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncell);
+    else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncell);
     unsigned nb = (unsigned)std::min<int64_t>(256, (ncg + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_totals, dim3(nb), dim3(BLOCK), 0, s, p, ncg);
 }

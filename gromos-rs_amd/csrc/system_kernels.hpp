@@ -99,7 +99,7 @@ void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
-void launch_nbl(const SysPtrs& p, int32_t ncg, bool fill, hipStream_t s);
+void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
 // mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);

@@ -46,6 +46,7 @@ def parse():
                     help="steps between batch layout changes of the FEP server (a multiple of 5)")
     ap.add_argument("--no-fep", action="store_true")
     ap.add_argument("--no-remd", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=2000, help="NVE steps of the equilibrated drift comparison")
     return ap.parse_args()
 
 
@@ -121,7 +122,10 @@ def run_path(rt, s, prec, steps, warmup, latency_steps):
     lat = np.array(lat) if lat else np.array([float("nan")])
     vv_pairs = ds_vv_pairs(ds, s)
     ds.close()
-    return dict(pairs_per_s=pairs / wall, ms_per_step=wall / steps * 1e3, wall=wall,
+    tot = e[:, 2] + e[:, 3]
+    t_ps = np.arange(len(tot)) * s.dt
+    drift = float(np.polyfit(t_ps, tot, 1)[0]) if len(tot) > 1 else float("nan")
+    return dict(drift=drift, pairs_per_s=pairs / wall, ms_per_step=wall / steps * 1e3, wall=wall,
                 ns_per_day=steps * s.dt * 1e-3 * 86400.0 / wall,
                 p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
                 kern_ms=kern_ms / max(launches, 1), launches=launches, vv_pairs=vv_pairs,
@@ -207,6 +211,13 @@ def main():
                                  "vv_kernel_ms": p["kern_ms"],
                                  "vv_kernel_tflops": p["vv_pairs"] * FLOP_PER_PAIR / (p["kern_ms"] * 1e-3) / 1e12}
         out["speedup_vs_plain_fp64_gpu"] = h["pairs_per_s"] / p["pairs_per_s"]
+        # total-energy drift (least-squares slope over the timed NVE run from
+        # the lattice start, same start for both precisions)
+        out["energy_drift_kj_mol_ps"] = {"fp64": p["drift"], "mixed": h["drift"],
+                                         "rel_diff": abs(h["drift"] - p["drift"]) / abs(p["drift"]),
+                                         "steps": a.steps, "atoms": s.n_atoms, "start": "lattice"}
+        if a.drift_steps > 0:
+            out["energy_drift_equilibrated"] = drift_bench(rt, gen, s, a.drift_steps)
         out["mixed_force_error_vs_fp64"] = force_error(rt, s)
     out["energy_first_last"] = [h["e_first"], h["e_last"]]
     if not a.no_fep:
@@ -245,7 +256,8 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
                          step_ms_p99=float(np.percentile(r["step_ms"], 99)),
                          job_latency_s_p50=float(np.percentile(r["job_latency_s"], 50)),
                          job_latency_s_p99=float(np.percentile(r["job_latency_s"], 99)),
-                         layouts=r["layouts"], layout_ms_total=r["layout_ms"], session_steps=r["clock"],
+                         layouts=r["layouts"], layout_ms_total=r["layout_ms"],
+                         layout_concat_ms_total=r["layout_concat_ms"], session_steps=r["clock"],
                          failed_jobs=sum(1 for v in server.results.values() if v["failed"]))
         server.sess.close()
     # one at a time, plain FP64 (rate over alone_steps per job, pair-list cadence included)
@@ -306,6 +318,26 @@ def remd_bench(rt, gen, steps):
                 failed_replicas=int(failed),
                 nve_drift_kj_mol_ps_median_abs=float(np.median(np.abs(drift))) if drift else None,
                 precision="mixed", start="lattice waters relaxed 300 steps under Berendsen tau 0.01 ps (untimed)")
+
+
+def drift_bench(rt, gen, s, steps):
+    """NVE total-energy drift of the same system after 300 untimed Berendsen
+    steps (tau 0.01 ps, 300 K) in FP64 and mixed precision from the same state:
+    least-squares slope of E_pot + E_kin over `steps` steps."""
+    import numpy as np
+    srv = importlib.import_module("gromos-rs_amd.server")
+    eq = srv.equilibrate([s], [300.0])[0]
+    out = {"steps": steps, "atoms": eq.n_atoms, "start": "lattice relaxed 300 steps under Berendsen tau 0.01 ps"}
+    for name, prec in (("fp64", rt.FP64), ("mixed", rt.MIXED)):
+        ds = rt.DeviceSystem(eq, precision=prec)
+        ds.pairlist_update()
+        e = ds.md(steps)
+        ds.close()
+        tot = e[:, 2] + e[:, 3]
+        out[name] = float(np.polyfit(np.arange(steps) * eq.dt, tot, 1)[0])
+        out[name + "_per_atom"] = out[name] / eq.n_atoms
+    out["rel_diff"] = abs(out["mixed"] - out["fp64"]) / abs(out["fp64"])
+    return out
 
 
 def ds_vv_pairs(ds, s):

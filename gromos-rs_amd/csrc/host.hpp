@@ -88,6 +88,7 @@ struct gromos_amd_system {
     bool have_prev = false;
     int64_t prev_tot_vv = 0, prev_tot_gen = 0;
     int64_t rebuilds = 0, refills = 0;
+    double concat_ms = 0.0;   // host time concatenating job arrays in layouts
     double4* d_xq = nullptr;
     float4* d_xf = nullptr;
     long long *d_fi_s = nullptr, *d_fi_long_s = nullptr;
@@ -96,6 +97,8 @@ struct gromos_amd_system {
     double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr, *d_vv_part = nullptr, *d_sol_e = nullptr;
     int32_t* d_sol_atom = nullptr;
     int32_t nsol = 0;
+    int4* d_vv_chunk = nullptr;   // solvent pair kernel chunks (first sorted group, end, job, 0)
+    int32_t nvv_chunk = 0;
     int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
     int32_t vv_neutral = 0;  // every job's solvent molecule has zero net charge
     int32_t vv_lanes = 0;  // lanes per solvent molecule in the pair kernel (0: not tuned yet)

@@ -130,6 +130,7 @@ struct gromos_amd_session {
     std::deque<SJob*> queue;
     int64_t next_id = 1, reserved = 0, clock = 0, layouts = 0, pair_count = 0;
     double layout_ms = 0.0;
+    double concat_ms_gone = 0.0;   // concatenation time of batches already released
     DevScratch stage, map, small;
     bool step_timing = false;
     std::vector<double> step_ms;
@@ -188,6 +189,7 @@ static int relayout(gromos_amd_session* s) {
     s->queue.clear();
     s->batch.clear();
     if (nb.empty()) {
+        if (S) s->concat_ms_gone += S->concat_ms;
         s->S.reset();
         s->layouts++;
         s->layout_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -530,6 +532,7 @@ extern "C" int gromos_amd_session_get_info(gromos_amd_session* s, gromos_amd_ses
     o->clock = s->clock;
     o->layouts = s->layouts;
     o->layout_ms = s->layout_ms;
+    o->layout_concat_ms = s->concat_ms_gone + (s->S ? s->S->concat_ms : 0.0);
     o->pair_count = s->pair_count;
     return 0;
 }

@@ -2,10 +2,12 @@
 // pair-list rebuild cadence, Forcefield::apply and the md step sequence.
 #include <hipcub/hipcub.hpp>
 
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "host.hpp"
 
@@ -235,22 +237,57 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->njobs = njobs;
     S->jobs.clear();
     S->job_n_atoms.clear();
-    int64_t atom_off = 0, cg_off = 0, lj_off = 0, cell_off = 0, npad = 0, ncg = 0;
-    for (const JobImage* im : imgs) {
-        npad += align_up(im->n, BLOCK);
-        ncg += im->n_cg;
+    const auto t_concat = std::chrono::steady_clock::now();
+    // pass 1 (serial): each job's offsets in every concatenated array
+    struct Slice {
+        int64_t atom, cg, lj, excl, o14, o14s, molf, pap, sol;
+    };
+    std::vector<Slice> sl(njobs + 1);
+    int64_t cell_off = 0;
+    bool any_pert = false, twin = false, any_thermo = false;
+    int32_t use_settle = 0, do_virial = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const JobImage& im = *imgs[j];
+        const Slice& o = sl[j];
+        Slice& n = sl[j + 1];
+        n.atom = o.atom + align_up(im.n, BLOCK);
+        n.cg = o.cg + im.n_cg;
+        n.lj = o.lj + (int64_t)im.lj.size();
+        n.excl = o.excl + (int64_t)im.excl.size();
+        n.o14 = o.o14 + (int64_t)im.o14.size();
+        n.o14s = o.o14s + (int64_t)im.o14s.size();
+        n.molf = o.molf + (im.P.use_settle ? (int64_t)im.mol_first.size() : 0);
+        n.pap = o.pap + (int64_t)im.papairs.size();
+        n.sol = o.sol + im.J.n_solute_atoms;
+        JobDev J = im.J;
+        J.atom_off = (int32_t)o.atom;
+        J.cg_off = (int32_t)o.cg;
+        J.lj_off = (int32_t)o.lj;
+        J.cell_off = (int32_t)cell_off;
+        J.stop = INT32_MAX;
+        J.sol_off = (int32_t)o.sol;
+        cell_off += (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
+        any_pert = any_pert || im.has_pert;
+        twin = twin || J.twin;
+        any_thermo = any_thermo || J.thermo;
+        use_settle |= im.P.use_settle && !im.mol_first.empty();
+        do_virial |= im.P.do_virial;
+        S->jobs.push_back(J);
+        S->job_n_atoms.push_back(im.n);
     }
+    const Slice& T = sl[njobs];
+    const int64_t npad = T.atom, ncg = T.cg;
     if (npad >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many atoms");
     if (ncg >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many charge groups");
-    std::vector<int32_t> cg_start, cg_n, cg_job, atom_job(npad, -1), iac(npad, 0), excl_off, excl, o14_off, o14,
-        o14s_off, o14s, mol_first;
-    std::vector<double> q(npad, 0.0), mass(npad, 0.0), inv_mass(npad, 0.0), lj;
-    bool any_pert = false;
-    for (const JobImage* im : imgs) any_pert = any_pert || im->has_pert;
-    std::vector<uint8_t> pflag;
-    std::vector<int32_t> pa_iac, pb_iac, papairs;
+    if (T.excl >= INT32_MAX || T.o14 >= INT32_MAX || T.o14s >= INT32_MAX || cell_off >= INT32_MAX)
+        return fail(GROMOS_AMD_EINVAL, "batch too large");
+    std::vector<int32_t> cg_start(ncg), cg_n(ncg), cg_job(ncg), atom_job(npad, -1), iac(npad, 0);
+    std::vector<int32_t> excl_off(npad + 1), excl(T.excl), o14_off(npad + 1), o14(T.o14), o14s_off(npad + 1),
+        o14s(T.o14s), mol_first(T.molf), papairs(T.pap), sol_atom(T.sol);
+    std::vector<double> q(npad, 0.0), mass(npad, 0.0), inv_mass(npad, 0.0), lj(T.lj);
+    std::vector<uint8_t> pflag, cg_pert;
+    std::vector<int32_t> pa_iac, pb_iac;
     std::vector<double> pa_q, pb_q, palj, pacrf;
-    std::vector<uint8_t> cg_pert;
     if (any_pert) {
         pflag.assign(npad, 0);
         pa_iac.assign(npad, 0);
@@ -259,92 +296,83 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         pb_q.assign(npad, 0.0);
         palj.assign(npad, 0.0);
         pacrf.assign(npad, 0.0);
+        cg_pert.assign(ncg, 0);
     }
-    cg_start.reserve(ncg);
-    cg_n.reserve(ncg);
-    cg_job.reserve(ncg);
-    excl_off.reserve(npad + 1);
-    o14_off.reserve(npad + 1);
-    o14s_off.reserve(npad + 1);
-    excl_off.push_back(0);
-    o14_off.push_back(0);
-    o14s_off.push_back(0);
-    bool twin = false, any_thermo = false;
-    std::vector<int32_t> sol_atom;
-    int32_t use_settle = 0, do_virial = 0;
-    for (int j = 0; j < njobs; ++j) {
+    // pass 2 (parallel over jobs): each job fills its own slices, indices
+    // shifted by its atom offset
+    auto fill = [&](int j) {
         const JobImage& im = *imgs[j];
-        JobDev J = im.J;
-        const int32_t A = (int32_t)atom_off;
-        J.atom_off = A;
-        J.cg_off = (int32_t)cg_off;
-        J.lj_off = (int32_t)lj_off;
-        J.cell_off = (int32_t)cell_off;
-        J.stop = INT32_MAX;
-        J.sol_off = (int32_t)sol_atom.size();
-        for (int32_t a = 0; a < J.n_solute_atoms; ++a) sol_atom.push_back(A + a);
-        cell_off += (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
-        twin = twin || J.twin;
-        any_thermo = any_thermo || J.thermo;
-        use_settle |= im.P.use_settle && !im.mol_first.empty();
-        do_virial |= im.P.do_virial;
-        S->jobs.push_back(J);
-        S->job_n_atoms.push_back(im.n);
+        const Slice& o = sl[j];
+        const int32_t A = (int32_t)o.atom;
         for (int c = 0; c < im.n_cg; ++c) {
-            cg_start.push_back(A + im.cg_start[c]);
-            cg_n.push_back(im.cg_start[c + 1] - im.cg_start[c]);
-            cg_job.push_back(j);
-            if (any_pert) {
+            const int64_t g = o.cg + c;
+            cg_start[g] = A + im.cg_start[c];
+            cg_n[g] = im.cg_start[c + 1] - im.cg_start[c];
+            cg_job[g] = j;
+            if (im.has_pert) {
                 uint8_t f = 0;
-                if (im.has_pert)
-                    for (int a = im.cg_start[c]; a < im.cg_start[c + 1]; ++a) f |= im.pflag[a];
-                cg_pert.push_back(f ? 1 : 0);
+                for (int a = im.cg_start[c]; a < im.cg_start[c + 1]; ++a) f |= im.pflag[a];
+                cg_pert[g] = f ? 1 : 0;
             }
         }
-        int64_t npad_j = align_up(im.n, BLOCK);
+        const int64_t npad_j = sl[j + 1].atom - o.atom;
         for (int a = 0; a < im.n; ++a) {
             atom_job[A + a] = j;
             q[A + a] = im.q[a];
             iac[A + a] = im.iac[a];
             mass[A + a] = im.mass[a];
             inv_mass[A + a] = im.mass[a] > 0.0 ? 1.0 / im.mass[a] : 0.0;
-            for (int k = im.excl_off[a]; k < im.excl_off[a + 1]; ++k) excl.push_back(A + im.excl[k]);
-            for (int k = im.o14_off[a]; k < im.o14_off[a + 1]; ++k) o14.push_back(A + im.o14[k]);
-            for (int k = im.o14s_off[a]; k < im.o14s_off[a + 1]; ++k) {
-                int32_t e = im.o14s[k];
-                o14s.push_back((int32_t)((uint32_t)(A + (e & 0x7FFFFFFF)) | ((uint32_t)e & 0x80000000u)));
-            }
-            excl_off.push_back((int32_t)excl.size());
-            o14_off.push_back((int32_t)o14.size());
-            o14s_off.push_back((int32_t)o14s.size());
-            if (im.has_pert && im.pflag[a]) {
-                pflag[A + a] = 1;
-                pa_iac[A + a] = im.pa_iac[a];
-                pb_iac[A + a] = im.pb_iac[a];
-                pa_q[A + a] = im.pa_q[a];
-                pb_q[A + a] = im.pb_q[a];
-                palj[A + a] = im.palj[a];
-                pacrf[A + a] = im.pacrf[a];
-            }
         }
-        for (int64_t a = im.n; a < npad_j; ++a) {
-            excl_off.push_back((int32_t)excl.size());
-            o14_off.push_back((int32_t)o14.size());
-            o14s_off.push_back((int32_t)o14s.size());
+        for (int64_t a = 0; a < npad_j; ++a) {
+            const int32_t la = (int32_t)std::min<int64_t>(a, im.n);
+            excl_off[A + a] = (int32_t)(o.excl + im.excl_off[la]);
+            o14_off[A + a] = (int32_t)(o.o14 + im.o14_off[la]);
+            o14s_off[A + a] = (int32_t)(o.o14s + im.o14s_off[la]);
         }
+        for (size_t k = 0; k < im.excl.size(); ++k) excl[o.excl + k] = A + im.excl[k];
+        for (size_t k = 0; k < im.o14.size(); ++k) o14[o.o14 + k] = A + im.o14[k];
+        for (size_t k = 0; k < im.o14s.size(); ++k) {
+            const int32_t e = im.o14s[k];
+            o14s[o.o14s + k] = (int32_t)((uint32_t)(A + (e & 0x7FFFFFFF)) | ((uint32_t)e & 0x80000000u));
+        }
+        if (im.has_pert)
+            for (int a = 0; a < im.n; ++a)
+                if (im.pflag[a]) {
+                    pflag[A + a] = 1;
+                    pa_iac[A + a] = im.pa_iac[a];
+                    pb_iac[A + a] = im.pb_iac[a];
+                    pa_q[A + a] = im.pa_q[a];
+                    pb_q[A + a] = im.pb_q[a];
+                    palj[A + a] = im.palj[a];
+                    pacrf[A + a] = im.pacrf[a];
+                }
         if (im.P.use_settle)
-            for (int32_t m : im.mol_first) mol_first.push_back(A + m);
+            for (size_t k = 0; k < im.mol_first.size(); ++k) mol_first[o.molf + k] = A + im.mol_first[k];
         for (size_t k = 0; k < im.papairs.size(); k += 4) {
-            papairs.push_back(A + im.papairs[k]);
-            papairs.push_back(A + im.papairs[k + 1]);
-            papairs.push_back(im.papairs[k + 2]);
-            papairs.push_back(im.papairs[k + 3]);
+            papairs[o.pap + k] = A + im.papairs[k];
+            papairs[o.pap + k + 1] = A + im.papairs[k + 1];
+            papairs[o.pap + k + 2] = im.papairs[k + 2];
+            papairs[o.pap + k + 3] = im.papairs[k + 3];
         }
-        lj.insert(lj.end(), im.lj.begin(), im.lj.end());
-        lj_off += (int64_t)im.lj.size();
-        atom_off += npad_j;
-        cg_off += im.n_cg;
+        std::copy(im.lj.begin(), im.lj.end(), lj.begin() + o.lj);
+        for (int32_t a = 0; a < im.J.n_solute_atoms; ++a) sol_atom[o.sol + a] = A + a;
+    };
+    {
+        const int nt = (int)std::min<int64_t>(std::min(njobs, 8), std::max<int64_t>(1, npad / 65536));
+        if (nt <= 1) {
+            for (int j = 0; j < njobs; ++j) fill(j);
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t]() {
+                    for (int j = t; j < njobs; j += nt) fill(j);
+                });
+            for (auto& x : th) x.join();
+        }
     }
+    excl_off[npad] = (int32_t)T.excl;
+    o14_off[npad] = (int32_t)T.o14;
+    o14s_off[npad] = (int32_t)T.o14s;
     S->npad = (int32_t)npad;
     S->ncg = (int32_t)ncg;
     S->ncell = (int32_t)cell_off;
@@ -379,24 +407,37 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->n_papairs = (int32_t)papairs.size() / 4;
     S->nsol = (int32_t)sol_atom.size();
     S->has_pert = any_pert;
-    // per-atom index of the PERTATOMPAIR entries (both atoms of an entry)
+    // chunks of at most VV_CHUNK sorted groups, each within one job (a job's
+    // groups keep its own range [cg_off, cg_off + n_cg) in sorted space)
+    std::vector<int4> vv_chunk;
+    for (int j = 0; j < njobs; ++j) {
+        const int32_t g0 = S->jobs[j].cg_off, g1 = g0 + imgs[j]->n_cg;
+        for (int32_t c = g0; c < g1; c += VV_CHUNK) vv_chunk.push_back(make_int4(c, std::min(c + VV_CHUNK, g1), j, 0));
+    }
+    S->nvv_chunk = (int32_t)vv_chunk.size();
+    // per-atom index of the PERTATOMPAIR entries (both atoms of an entry),
+    // entries of an atom in entry order
     std::vector<int32_t> pap_off, pap;
     if (any_pert) {
-        std::vector<std::vector<int32_t>> per(npad);
+        pap_off.assign(npad + 1, 0);
         for (int32_t e = 0; e < S->n_papairs; ++e) {
             int32_t i = papairs[4 * e], jj = papairs[4 * e + 1];
             if (i == jj) continue;   // zero distance: skipped by the reference (perturbed.rs:668-671)
-            per[i].push_back((int32_t)((uint32_t)e | 0x80000000u));
-            per[jj].push_back(e);
+            pap_off[i + 1]++;
+            pap_off[jj + 1]++;
         }
-        pap_off.reserve(npad + 1);
-        pap_off.push_back(0);
-        for (int64_t a = 0; a < npad; ++a) {
-            for (int32_t x : per[a]) pap.push_back(x);
-            pap_off.push_back((int32_t)pap.size());
+        for (int64_t a = 0; a < npad; ++a) pap_off[a + 1] += pap_off[a];
+        pap.resize(pap_off[npad]);
+        std::vector<int32_t> fillp(pap_off.begin(), pap_off.end() - 1);
+        for (int32_t e = 0; e < S->n_papairs; ++e) {
+            int32_t i = papairs[4 * e], jj = papairs[4 * e + 1];
+            if (i == jj) continue;
+            pap[fillp[i]++] = (int32_t)((uint32_t)e | 0x80000000u);
+            pap[fillp[jj]++] = e;
         }
     }
 
+    S->concat_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_concat).count();
     hipStream_t st = S->stream;
     int rc = 0;
     rc = rc ? rc : S->upload(S->d_jobs, S->jobs);
@@ -427,6 +468,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_papairs, papairs);
     rc = rc ? rc : S->upload(S->d_pap_off, pap_off);
     rc = rc ? rc : S->upload(S->d_pap, pap);
+    rc = rc ? rc : S->upload(S->d_vv_chunk, vv_chunk);
     if (rc) return rc;
     size_t n3 = (size_t)npad * 3, nc = (size_t)ncg, ncl = (size_t)S->ncell + 1, npa = (size_t)npad;
     for (int k = 0; k < 2; ++k) {
@@ -580,7 +622,7 @@ int build_pairlist(gromos_amd_system* S) {
 }
 
 // Autotuner of the mixed-precision solvent pair kernel: lanes per molecule (4, 8,
-// 16) chosen by timing one launch of each on the batch itself, the first time a
+// 16) chosen by timing two launches of each on the batch itself, the first time a
 // batch shape (molecule size, LJ layout, virial, log2 of the group count) is met
 // in this process; GROMOS_AMD_VV_LANES forces a value.  That kernel adds fixed-
 // point groups of entries, so every choice gives the same bits and the step
@@ -616,25 +658,31 @@ static int tune_vv(gromos_amd_system* S, const SysPtrs& p, int32_t aps, int32_t 
             return 0;
         }
     }
+    // each candidate timed twice (order 4 8 16 16 8 4), the faster launch counts:
+    // a single cold launch can be slowed by cache and clock state
     const int32_t cand[3] = {4, 8, 16};
-    hipEvent_t ev[4];
+    const int order[6] = {0, 1, 2, 2, 1, 0};
+    hipEvent_t ev[7];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], S->stream));
-    for (int c = 0; c < 3; ++c) {
-        launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, cand[c], S->stream);
+    for (int c = 0; c < 6; ++c) {
+        launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, cand[order[c]], S->stream);
         HIPCHK(hipEventRecord(ev[c + 1], S->stream));
     }
-    HIPCHK(hipEventSynchronize(ev[3]));
-    int32_t best = 8;
-    float best_ms = 1e30f;
-    for (int c = 0; c < 3; ++c) {
+    HIPCHK(hipEventSynchronize(ev[6]));
+    float best_of[3] = {1e30f, 1e30f, 1e30f};
+    for (int c = 0; c < 6; ++c) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ev[c], ev[c + 1]));
-        if (ms < best_ms) {
-            best_ms = ms;
+        best_of[order[c]] = std::min(best_of[order[c]], ms);
+    }
+    int32_t best = 8;
+    float best_ms = 1e30f;
+    for (int c = 0; c < 3; ++c)
+        if (best_of[c] < best_ms) {
+            best_ms = best_of[c];
             best = cand[c];
         }
-    }
     for (auto& e : ev) (void)hipEventDestroy(e);
     HIPCHK(hipGetLastError());
     {
@@ -863,6 +911,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.cg_ek = d_cg_ek;
     p.sol_atom = d_sol_atom;
     p.nsol = nsol;
+    p.vv_chunk = d_vv_chunk;
+    p.nvv_chunk = nvv_chunk;
     p.sol_e = d_sol_e;
     p.vv_virial = do_virial;
     p.records = d_records;

@@ -482,18 +482,41 @@ __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, floa
 // record (vv_e, 11 doubles) that k_reduce sums per job.  NCHM (lanes per
 // molecule: 4, 8 or 16) is chosen per batch shape by timing (autotune).
 
+// Wave -> groups of the solvent pair kernels: the batch's sorted groups in
+// chunks of VV_CHUNK that never straddle two jobs (vv_chunk = first group, end,
+// job); a wave holds WAVE / NCHM consecutive groups of one chunk, so the job's
+// constants are wave-uniform (scalar registers, fewer vector registers).
+// Returns false for waves past the last chunk (the whole wave leaves).
+template <int NCHM>
+__device__ __forceinline__ bool vv_wave(const SysPtrs& p, int32_t nwaves, int32_t& k, bool& valid, int& chunk,
+                                        int32_t& job) {
+    constexpr int G = WAVE / NCHM;
+    constexpr int SUB = VV_CHUNK / G;
+    const int32_t lb = xcd_block((nwaves + BLOCK / WAVE - 1) / (BLOCK / WAVE));
+    if (lb < 0) return false;
+    const int32_t wvi = lb * (BLOCK / WAVE) + (int32_t)(threadIdx.x / WAVE);
+    if (wvi >= nwaves) return false;
+    const int4 ch = p.vv_chunk[wvi / SUB];
+    const int32_t first = __builtin_amdgcn_readfirstlane(ch.x), end = __builtin_amdgcn_readfirstlane(ch.y);
+    job = __builtin_amdgcn_readfirstlane(ch.z);
+    const int lane = threadIdx.x & (WAVE - 1);
+    k = first + (wvi % SUB) * G + lane / NCHM;
+    valid = k < end;
+    chunk = lane % NCHM;
+    return true;
+}
+
 template <int APS, bool LJ00>
 __device__ __forceinline__ bool has_lj(int i, int j) {
     return !LJ00 || (i == 0 && j == 0);
 }
 
 template <int APS, bool LJ00, bool VIR, int NCHM>
-__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
-    const int32_t lb = xcd_block((int32_t)(((int64_t)ncg * NCHM + BLOCK - 1) / BLOCK));
-    if (lb < 0) return;
-    const int32_t gid = lb * BLOCK + threadIdx.x;
-    const int32_t k = gid / NCHM;
-    const int chunk = gid % NCHM;
+__global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
+    int32_t k = 0, job = 0;
+    bool valid = false;
+    int chunk = 0;
+    if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
     double f[APS][3];
     double en[11];
 #pragma unroll
@@ -501,13 +524,13 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 #pragma unroll
     for (int c = 0; c < 11; ++c) en[c] = 0.0;
     int32_t e0 = 0, e1 = 0, sk = 0;
-    if (k < ncg) {
+    if (valid) {
         e0 = (int32_t)p.vv_off[k];
         e1 = e0 + (int32_t)p.vv_cnt[k];
         sk = p.s_cg_start[k];
     }
     if (e1 > e0) {
-        const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
+        const JobDev& J = p.jobs[job];
         const Box box = J.box;
         const double c2 = J.crf.crf_2cut3i, cc = J.crf.crf_cut, c3 = J.crf.crf_cut3i;
         d3 pa[APS];
@@ -573,7 +596,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 #pragma unroll
         for (int c = 0; c < (VIR ? 11 : 2); ++c) en[c] += __shfl_xor(en[c], m, WAVE);
     }
-    if (chunk == 0 && k < ncg) {
+    if (chunk == 0 && valid) {
         int32_t nk = p.s_cg_n[k];
         if (e1 > e0) {
 #pragma unroll
@@ -607,12 +630,13 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t ncg) {
 // terms.  Atom pairs closer than 1e-5 nm (never two molecules) keep zero 1/r
 // terms but, unlike the reference, their RF polynomial is not dropped.
 template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
-__global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
-    const int32_t lb = xcd_block((int32_t)(((int64_t)ncg * NCHM + BLOCK - 1) / BLOCK));
-    if (lb < 0) return;
-    const int32_t gid = lb * BLOCK + threadIdx.x;
-    const int32_t k = gid / NCHM;
-    const int chunk = gid % NCHM;
+// four waves per SIMD (at most 128 vector registers; a few values spill to
+// scratch outside the pair loop): 8 % faster than three waves with no spills
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) k_vv_mx(SysPtrs p, int32_t nwaves) {
+    int32_t k = 0, job = 0;
+    bool valid = false;
+    int chunk = 0;
+    if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
     long long fi[APS][3];
     long long ei[11];
 #pragma unroll
@@ -620,13 +644,13 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
 #pragma unroll
     for (int c = 0; c < 11; ++c) ei[c] = 0;
     int32_t e0 = 0, e1 = 0, sk = 0;
-    if (k < ncg) {
+    if (valid) {
         e0 = (int32_t)p.vv_off[k];
         e1 = e0 + (int32_t)p.vv_cnt[k];
         sk = p.s_cg_start[k];
     }
     if (e1 > e0) {
-        const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
+        const JobDev& J = p.jobs[job];
         const Box box = J.box;
         const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
         float qq[APS][APS];
@@ -793,7 +817,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_mx(SysPtrs p, int32_t ncg) {
 #pragma unroll
         for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += __shfl_xor(ei[c], m, WAVE);
     }
-    if (chunk == 0 && k < ncg) {
+    if (chunk == 0 && valid) {
         int32_t nk = p.s_cg_n[k];
         if (e1 > e0) {
 #pragma unroll
@@ -886,6 +910,36 @@ __device__ __forceinline__ void pert_pair_delta(const SysPtrs& p, const JobDev& 
         v[0] += o[1] - elja;
         v[1] += o[2] - ecrfa;
         v[11] += o[3] + o[4];
+    }
+}
+
+// A listed short pair whose first atom i is perturbed, evaluated directly: the
+// soft-core result of perturbed_lj_crf_interaction (perturbed.rs:124-266) is the
+// pair's force and energy.  The reference takes the state-A pair in its regular
+// loop and adds (perturbed - state A) at the nearest-image vector
+// (perturbed.rs:356-380); both are the same sum, but where a soft-core atom
+// overlaps another the unsoftened state-A r^-12 terms (1e15 kJ mol^-1 nm^-1 at
+// 0.03 nm) leave rounding noise of thousands after the subtraction, because the
+// regular loop forms r from the charge groups' shift: here nothing unsoftened
+// is formed.  The virial keeps the reference's state-A pair term (the
+// correction does not touch the virial).
+__device__ __forceinline__ void pert_pair_direct(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
+                                                 bool vir, double (&v)[ROW], double& fx, double& fy, double& fz) {
+    PertPair q = pert_params(p, J, i, j);
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    double o[5];
+    pert_lj_crf64(r.x, r.y, r.z, q.la[0], q.la[1], q.lb[0], q.lb[1], q.a_q, q.b_q, q.alj, q.acrf, J.lp, J.crf,
+                  J.fpepsi, o);
+    own_side(first, r, o[0], fx, fy, fz);
+    if (first) {
+        v[0] += o[1];
+        v[1] += o[2];
+        v[11] += o[3] + o[4];
+        if (vir) {
+            double fa, elja, ecrfa;
+            lj_crf64(r.x, r.y, r.z, q.la[0], q.la[1], q.a_q * J.fpepsi, J.crf, fa, elja, ecrfa);
+            add_virial(v, r, r.x * fa, r.y * fa, r.z * fa);
+        }
     }
 }
 
@@ -1071,14 +1125,17 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
                 int32_t ob = p.s2o[sm + b];
                 if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             }
-            double4 xb = p.xq[sm + b];
-            gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
-                     p.s_type[sm + b], xa.w, xb.w, LONG || virial);
             if (pchk) {
                 int32_t ob = p.s2o[sm + b];
                 int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
+                if (p.pflag[fi]) {
+                    pert_pair_direct(p, J, fi, fj, canon, virial != 0, v, fx, fy, fz);
+                    continue;
+                }
             }
+            double4 xb = p.xq[sm + b];
+            gen_pair(p, J, v, fx, fy, fz, canon, !LONG && cgp, pa, xyz(xb), pk0, pm0, ta,
+                     p.s_type[sm + b], xa.w, xb.w, LONG || virial);
         }
     }
     // intra-group pairs of solute groups (pairlist.rs:238-249), always short
@@ -1090,13 +1147,16 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
             int32_t ob = p.s2o[sk + b];
             bool canon = ia < b;
             if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            if (pown) {
+                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
+                if (p.pflag[fi]) {
+                    pert_pair_direct(p, J, fi, fj, canon, virial != 0, v, fx, fy, fz);
+                    continue;
+                }
+            }
             double4 xb = p.xq[sk + b];
             gen_pair(p, J, v, fx, fy, fz, canon, cgp, pa, xyz(xb), pk0, pk0, ta, p.s_type[sk + b],
                      xa.w, xb.w, virial);
-            if (pown) {
-                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon, v, fx, fy, fz);
-            }
         }
     }
 }
@@ -1193,17 +1253,14 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
             }
             double4 xb = p.xq[sm + b];
             if (pchk) {
-                // a perturbed first atom: the state-A pair in FP64 (so that the
-                // FP64 delta below cancels it exactly) and the delta, both atoms'
+                // a perturbed first atom: the soft-core pair in FP64, both atoms'
                 // shares, by the first atom's lanes only
                 int32_t ob = p.s2o[sm + b];
                 int32_t fi = canon != 0.f ? o : ob;
                 if (p.pflag[fi]) {
                     if (canon == 0.f) continue;
                     double lx = 0.0, ly = 0.0, lz = 0.0;
-                    gen_pair(p, J, v, lx, ly, lz, true, cgp, xyz(xa), xyz(xb), pk0, xyz(p.xq[sm]), ta,
-                             p.s_type[sm + b], xa.w, xb.w, VIR);
-                    pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
+                    pert_pair_direct(p, J, o, ob, true, VIR, v, lx, ly, lz);
                     fx += lx;
                     fy += ly;
                     fz += lz;
@@ -1249,9 +1306,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 if (p.pflag[fi]) {
                     if (!canon) continue;
                     double lx = 0.0, ly = 0.0, lz = 0.0;
-                    gen_pair(p, J, v, lx, ly, lz, true, J.use_cg_grouped != 0, xyz(xa), xyz(xb), pk0, pk0, ta,
-                             p.s_type[sk + b], xa.w, xb.w, VIR);
-                    pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
+                    pert_pair_direct(p, J, o, ob, true, VIR, v, lx, ly, lz);
                     fx += lx;
                     fy += ly;
                     fz += lz;
@@ -1870,16 +1925,18 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 template <int APS, bool LJ00, int L>
 static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
                         hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
-    dim3 g = xcd_grid(((int64_t)ncg * L + BLOCK - 1) / BLOCK);
+    // waves: VV_CHUNK / (WAVE / L) per chunk of groups
+    const int32_t nw = p.nvv_chunk * (VV_CHUNK / (WAVE / L));
+    dim3 g = xcd_grid((nw + BLOCK / WAVE - 1) / (BLOCK / WAVE));
     if (mixed && neutral) {
-        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
-        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
     } else if (mixed) {
-        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
-        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
     } else {
-        if (virial) hipExtLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
-        else hipExtLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, ncg);
+        if (virial) hipExtLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        else hipExtLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
     }
 }
 template <int APS, bool LJ00>

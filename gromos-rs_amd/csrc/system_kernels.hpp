@@ -47,6 +47,8 @@ struct SysPtrs {
     int32_t* s_type;
     int64_t* vv_cnt;
     int64_t* gen_cnt;
+    const int4* vv_chunk;      // nvv_chunk: solvent pair kernel chunks (first sorted group, end, job, 0)
+    int32_t nvv_chunk;
     const int64_t* vv_off;
     const int64_t* gen_off;
     uint32_t* vv_ent;

@@ -285,10 +285,11 @@ def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5,
 
 
 def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=0.5, n_types=20,
-            type_seed=5, temperature=300.0, name=None):
+            type_seed=5, temperature=300.0, name=None, min_sigma_ox=0.28):
     """One FEP lambda-window job: n_pert perturbed solute atoms (each its own
     charge group, no bonds, >= 0.4 nm apart) with A/B charges uniform in
-    [-0.5, 0.5] e and A/B types from the 20-type table, in SPC water at
+    [-0.5, 0.5] e and A/B types from the 20-type table (those whose LJ diameter
+    against the water oxygen is at least min_sigma_ox nm), in SPC water at
     0.997 g/cm^3 in the box sized for n_atoms, the waters of each site's first
     shell removed (so the job holds somewhat fewer than n_atoms atoms);
     soft-core alphas 0.5 / 0.5 nm^2 (global ALPHLJ = ALPHC = 1)."""
@@ -319,8 +320,16 @@ def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=
     lj, nt = _mixed_table(c6s, c12s)
     t_o, t_h = n_types, n_types + 1
     n = n_pert + 3 * n_w
-    a_iac = rng.integers(0, n_types, n_pert).astype(np.int32)
-    b_iac = rng.integers(0, n_types, n_pert).astype(np.int32)
+    # A/B types from the table's types with an oxygen-site LJ diameter
+    # (C12/C6)^(1/6) >= min_sigma_ox: with charges up to 0.5 e, a smaller site lets
+    # a water hydrogen (no LJ of its own) collapse onto it at the unsoftened end
+    # states and the job fails (SETTLE)
+    ljt = np.asarray(lj).reshape(nt, nt, 4)
+    c6o, c12o = ljt[t_o, :n_types, 0], ljt[t_o, :n_types, 1]
+    sig = np.where(c6o > 0, (c12o / np.where(c6o > 0, c6o, 1.0)) ** (1.0 / 6.0), 0.0)
+    allowed = np.where(sig >= min_sigma_ox)[0]
+    a_iac = allowed[rng.integers(0, len(allowed), n_pert)].astype(np.int32)
+    b_iac = allowed[rng.integers(0, len(allowed), n_pert)].astype(np.int32)
     a_q = rng.uniform(-0.5, 0.5, n_pert)
     b_q = rng.uniform(-0.5, 0.5, n_pert)
     pos = np.concatenate([sites, wpos])

@@ -62,7 +62,7 @@ class SessionParams(C.Structure):
 class SessionInfo(C.Structure):
     _fields_ = [("queued", I32), ("running", I32), ("done", I32), ("resident_jobs", I32),
                 ("device_bytes", I64), ("reserved_bytes", I64), ("budget", I64), ("clock", I64),
-                ("layouts", I64), ("layout_ms", D), ("pair_count", I64)]
+                ("layouts", I64), ("layout_ms", D), ("pair_count", I64), ("layout_concat_ms", D)]
 
 
 class Params(C.Structure):

@@ -93,7 +93,8 @@ def replay(server, schedule, chunk=None):
     lat = np.array([r["t_done"] - r["t_submit"] for r in server.results.values()])
     lat_steps = np.array([r["clock_done"] - r["clock_submit"] for r in server.results.values()])
     return dict(wall=wall, pairs=info["pair_count"] - info0["pair_count"], layouts=info["layouts"] - info0["layouts"],
-                layout_ms=info["layout_ms"] - info0["layout_ms"], step_ms=step_ms, job_latency_s=lat,
+                layout_ms=info["layout_ms"] - info0["layout_ms"],
+                layout_concat_ms=info["layout_concat_ms"] - info0["layout_concat_ms"], step_ms=step_ms, job_latency_s=lat,
                 job_latency_steps=lat_steps, clock=info["clock"])
 
 

@@ -275,6 +275,7 @@ typedef struct {
     int64_t layouts;         /* batch layout changes */
     double layout_ms;        /* host wall time spent in them */
     int64_t pair_count;      /* pair interactions evaluated, all jobs */
+    double layout_concat_ms; /* part of layout_ms: concatenating the jobs' arrays on the host */
 } gromos_amd_session_info;
 
 int gromos_amd_session_create(const gromos_amd_session_params *p, gromos_amd_session **out);

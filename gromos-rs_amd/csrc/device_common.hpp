@@ -82,12 +82,35 @@ struct Lambda {
     int32_t pad;
 };
 
-// perturbed_lj_crf_interaction: o = {f, e_lj, e_crf, de_lj, de_crf}
-__device__ __forceinline__ void pert_lj_crf64(double x, double y, double z, double a_c6,
-                                              double a_c12, double b_c6, double b_c12,
-                                              double a_q, double b_q, double alpha_lj,
-                                              double alpha_crf, const Lambda& lp,
-                                              const Crf& crf, double fpepsi, double o[5]) {
+// Reaction-field constants of the soft-core pair that depend only on the pair's
+// alpha_crf and lambda (perturbed.rs:160-190): hoisted where one perturbed atom
+// meets many partners (same expressions, same rounding).
+struct SoftRF {
+    double a_crf_2cut3i, b_crf_2cut3i, a_crf_cut3i, b_crf_cut3i, a_crf_pert, b_crf_pert;
+};
+__device__ __forceinline__ SoftRF soft_rf(double alpha_crf, const Lambda& lp, const Crf& crf) {
+    SoftRF c;
+    double cut2 = crf.cutoff_sq;
+    double a_cut2soft = cut2 + alpha_crf * lp.b_crfs_lambda2;
+    double b_cut2soft = cut2 + alpha_crf * lp.a_crfs_lambda2;
+    double a_cut2soft3 = a_cut2soft * a_cut2soft * a_cut2soft;
+    double b_cut2soft3 = b_cut2soft * b_cut2soft * b_cut2soft;
+    c.a_crf_2cut3i = crf.crf_2cut3i / sqrt(a_cut2soft3);
+    c.b_crf_2cut3i = crf.crf_2cut3i / sqrt(b_cut2soft3);
+    c.a_crf_cut3i = 2.0 * c.a_crf_2cut3i;
+    c.b_crf_cut3i = 2.0 * c.b_crf_2cut3i;
+    c.a_crf_pert = 3.0 * c.a_crf_2cut3i / a_cut2soft;
+    c.b_crf_pert = 3.0 * c.b_crf_2cut3i / b_cut2soft;
+    return c;
+}
+
+// perturbed_lj_crf_interaction: o = {f, e_lj, e_crf, de_lj, de_crf}, with the
+// pair's soft_rf constants
+__device__ __forceinline__ void pert_lj_crf64_rf(double x, double y, double z, double a_c6,
+                                                 double a_c12, double b_c6, double b_c12,
+                                                 double a_q, double b_q, double alpha_lj,
+                                                 double alpha_crf, const Lambda& lp,
+                                                 const Crf& crf, const SoftRF& sc, double fpepsi, double o[5]) {
     double r2 = x * x + y * y + z * z;
     if (r2 < 1e-10) {
         o[0] = o[1] = o[2] = o[3] = o[4] = 0.0;
@@ -107,17 +130,9 @@ __device__ __forceinline__ void pert_lj_crf64(double x, double y, double z, doub
     double b_dist3isoft = b_distisoft / b_dist2soft;
     double b_dist6soft = r6 + alpha_lj * lp.a_ljs_lambda2 * b_c126;
     double b_dist6isoft = 1.0 / b_dist6soft;
-    double cut2 = crf.cutoff_sq;
-    double a_cut2soft = cut2 + alpha_crf * lp.b_crfs_lambda2;
-    double b_cut2soft = cut2 + alpha_crf * lp.a_crfs_lambda2;
-    double a_cut2soft3 = a_cut2soft * a_cut2soft * a_cut2soft;
-    double b_cut2soft3 = b_cut2soft * b_cut2soft * b_cut2soft;
-    double a_crf_2cut3i = crf.crf_2cut3i / sqrt(a_cut2soft3);
-    double b_crf_2cut3i = crf.crf_2cut3i / sqrt(b_cut2soft3);
-    double a_crf_cut3i = 2.0 * a_crf_2cut3i;
-    double b_crf_cut3i = 2.0 * b_crf_2cut3i;
-    double a_crf_pert = 3.0 * a_crf_2cut3i / a_cut2soft;
-    double b_crf_pert = 3.0 * b_crf_2cut3i / b_cut2soft;
+    const double a_crf_2cut3i = sc.a_crf_2cut3i, b_crf_2cut3i = sc.b_crf_2cut3i;
+    const double a_crf_cut3i = sc.a_crf_cut3i, b_crf_cut3i = sc.b_crf_cut3i;
+    const double a_crf_pert = sc.a_crf_pert, b_crf_pert = sc.b_crf_pert;
     double force = (lp.a_crf_lambda_n * a_q * (a_dist3isoft + a_crf_cut3i) +
                     lp.b_crf_lambda_n * b_q * (b_dist3isoft + b_crf_cut3i)) *
                    fpepsi;
@@ -152,6 +167,15 @@ __device__ __forceinline__ void pert_lj_crf64(double x, double y, double z, doub
     o[2] = e_crf;
     o[3] = de_lj;
     o[4] = de_crf;
+}
+
+__device__ __forceinline__ void pert_lj_crf64(double x, double y, double z, double a_c6,
+                                              double a_c12, double b_c6, double b_c12,
+                                              double a_q, double b_q, double alpha_lj,
+                                              double alpha_crf, const Lambda& lp,
+                                              const Crf& crf, double fpepsi, double o[5]) {
+    pert_lj_crf64_rf(x, y, z, a_c6, a_c12, b_c6, b_c12, a_q, b_q, alpha_lj, alpha_crf, lp, crf,
+                     soft_rf(alpha_crf, lp, crf), fpepsi, o);
 }
 
 // ---- wave / block reductions (deterministic order) ----

@@ -56,6 +56,53 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int job
 
 }  // namespace gamd
 
+// Pinned host staging for a batch layout's arrays (hipHostMalloc: DMA uploads and
+// no page faults when reused); carved afresh by every layout, grown between layouts.
+struct PinnedArena {
+    template <class T>
+    struct Span {
+        T* p = nullptr;
+        size_t n = 0;
+        T& operator[](size_t i) const { return p[i]; }
+        size_t size() const { return n; }
+        T* data() const { return p; }
+        T* begin() const { return p; }
+        T* end() const { return p + n; }
+    };
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    ~PinnedArena() {
+        if (base) (void)hipHostFree(base);
+    }
+    PinnedArena() = default;
+    PinnedArena(const PinnedArena&) = delete;
+    PinnedArena& operator=(const PinnedArena&) = delete;
+    // room for `bytes` in `parts` arrays; the caller has synchronised with every
+    // copy still reading the old contents
+    int reset(size_t bytes, size_t parts) {
+        const size_t need = bytes + (parts + 1) * 256;
+        if (need > cap) {
+            if (base) HIPCHK(hipHostFree(base));
+            base = nullptr;
+            cap = 0;
+            const size_t c = need + need / 4;
+            if (hipHostMalloc((void**)&base, c, hipHostMallocDefault) != hipSuccess) {
+                base = nullptr;
+                return ::gamd::fail(GROMOS_AMD_ENOMEM, "hipHostMalloc of %zu bytes failed", c);
+            }
+            cap = c;
+        }
+        used = 0;
+        return 0;
+    }
+    template <class T>
+    Span<T> take(size_t n) {
+        size_t off = (used + 255) & ~(size_t)255;
+        used = off + n * sizeof(T);
+        return Span<T>{(T*)(base + off), n};
+    }
+};
+
 // A batch of independent jobs resident on one device (one job for the
 // single-system API).  Device buffers are kept across layouts and grow only.
 struct gromos_amd_system {
@@ -175,11 +222,20 @@ struct gromos_amd_system {
         return 0;
     }
     template <class T>
-    int upload(T*& p, const std::vector<T>& h) {
-        if (int rc = reserve(p, h.size(), true)) return rc;
-        if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    int upload(T*& p, const T* h, size_t n) {
+        if (int rc = reserve(p, n, true)) return rc;
+        if (n) HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, stream));
         return 0;
     }
+    template <class T>
+    int upload(T*& p, const std::vector<T>& h) {
+        return upload(p, h.data(), h.size());
+    }
+    template <class T>
+    int upload(T*& p, const PinnedArena::Span<T>& h) {
+        return upload(p, h.data(), h.size());
+    }
+    PinnedArena host;   // staging of the layout's arrays
 
     gamd::SysPtrs ptrs() const;
     hipEvent_t next_event();

@@ -281,23 +281,28 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     if (ncg >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many charge groups");
     if (T.excl >= INT32_MAX || T.o14 >= INT32_MAX || T.o14s >= INT32_MAX || cell_off >= INT32_MAX)
         return fail(GROMOS_AMD_EINVAL, "batch too large");
-    std::vector<int32_t> cg_start(ncg), cg_n(ncg), cg_job(ncg), atom_job(npad, -1), iac(npad, 0);
-    std::vector<int32_t> excl_off(npad + 1), excl(T.excl), o14_off(npad + 1), o14(T.o14), o14s_off(npad + 1),
-        o14s(T.o14s), mol_first(T.molf), papairs(T.pap), sol_atom(T.sol);
-    std::vector<double> q(npad, 0.0), mass(npad, 0.0), inv_mass(npad, 0.0), lj(T.lj);
-    std::vector<uint8_t> pflag, cg_pert;
-    std::vector<int32_t> pa_iac, pb_iac;
-    std::vector<double> pa_q, pb_q, palj, pacrf;
-    if (any_pert) {
-        pflag.assign(npad, 0);
-        pa_iac.assign(npad, 0);
-        pb_iac.assign(npad, 0);
-        pa_q.assign(npad, 0.0);
-        pb_q.assign(npad, 0.0);
-        palj.assign(npad, 0.0);
-        pacrf.assign(npad, 0.0);
-        cg_pert.assign(ncg, 0);
-    }
+    // the arrays live in the batch's pinned staging arena (all written below)
+    const size_t np1 = (size_t)npad + 1, pn = any_pert ? (size_t)npad : 0;
+    const size_t n_pap = (size_t)T.pap / 2;   // two index entries per PERTATOMPAIR entry at most
+    const size_t bytes = 4 * (3 * (size_t)ncg + 2 * (size_t)npad + 3 * np1 + T.excl + T.o14 + T.o14s + T.molf +
+                              T.pap + T.sol) +
+                         8 * (3 * (size_t)npad + T.lj) + pn * (1 + 4 * 2 + 8 * 4) +
+                         (any_pert ? (size_t)ncg + 4 * (np1 + n_pap) : 0);
+    if (int rc = S->host.reset(bytes, 40)) return rc;
+    auto cg_start = S->host.take<int32_t>(ncg), cg_n = S->host.take<int32_t>(ncg), cg_job = S->host.take<int32_t>(ncg);
+    auto atom_job = S->host.take<int32_t>(npad), iac = S->host.take<int32_t>(npad);
+    auto excl_off = S->host.take<int32_t>(np1), excl = S->host.take<int32_t>(T.excl);
+    auto o14_off = S->host.take<int32_t>(np1), o14 = S->host.take<int32_t>(T.o14);
+    auto o14s_off = S->host.take<int32_t>(np1), o14s = S->host.take<int32_t>(T.o14s);
+    auto mol_first = S->host.take<int32_t>(T.molf), papairs = S->host.take<int32_t>(T.pap);
+    auto sol_atom = S->host.take<int32_t>(T.sol);
+    auto q = S->host.take<double>(npad), mass = S->host.take<double>(npad), inv_mass = S->host.take<double>(npad);
+    auto lj = S->host.take<double>(T.lj);
+    auto pflag = S->host.take<uint8_t>(pn), cg_pert = S->host.take<uint8_t>(any_pert ? ncg : 0);
+    auto pa_iac = S->host.take<int32_t>(pn), pb_iac = S->host.take<int32_t>(pn);
+    auto pa_q = S->host.take<double>(pn), pb_q = S->host.take<double>(pn), palj = S->host.take<double>(pn),
+         pacrf = S->host.take<double>(pn);
+    if (S->host.used > S->host.cap) return fail(GROMOS_AMD_EHIP, "layout staging overflow (internal)");
     // pass 2 (parallel over jobs): each job fills its own slices, indices
     // shifted by its atom offset
     auto fill = [&](int j) {
@@ -309,9 +314,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             cg_start[g] = A + im.cg_start[c];
             cg_n[g] = im.cg_start[c + 1] - im.cg_start[c];
             cg_job[g] = j;
-            if (im.has_pert) {
+            if (any_pert) {
                 uint8_t f = 0;
-                for (int a = im.cg_start[c]; a < im.cg_start[c + 1]; ++a) f |= im.pflag[a];
+                if (im.has_pert)
+                    for (int a = im.cg_start[c]; a < im.cg_start[c + 1]; ++a) f |= im.pflag[a];
                 cg_pert[g] = f ? 1 : 0;
             }
         }
@@ -322,6 +328,13 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             iac[A + a] = im.iac[a];
             mass[A + a] = im.mass[a];
             inv_mass[A + a] = im.mass[a] > 0.0 ? 1.0 / im.mass[a] : 0.0;
+        }
+        for (int64_t a = im.n; a < npad_j; ++a) {
+            atom_job[A + a] = -1;
+            iac[A + a] = 0;
+            q[A + a] = 0.0;
+            mass[A + a] = 0.0;
+            inv_mass[A + a] = 0.0;
         }
         for (int64_t a = 0; a < npad_j; ++a) {
             const int32_t la = (int32_t)std::min<int64_t>(a, im.n);
@@ -335,17 +348,17 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             const int32_t e = im.o14s[k];
             o14s[o.o14s + k] = (int32_t)((uint32_t)(A + (e & 0x7FFFFFFF)) | ((uint32_t)e & 0x80000000u));
         }
-        if (im.has_pert)
-            for (int a = 0; a < im.n; ++a)
-                if (im.pflag[a]) {
-                    pflag[A + a] = 1;
-                    pa_iac[A + a] = im.pa_iac[a];
-                    pb_iac[A + a] = im.pb_iac[a];
-                    pa_q[A + a] = im.pa_q[a];
-                    pb_q[A + a] = im.pb_q[a];
-                    palj[A + a] = im.palj[a];
-                    pacrf[A + a] = im.pacrf[a];
-                }
+        if (any_pert)
+            for (int64_t a = 0; a < npad_j; ++a) {
+                const bool pa = im.has_pert && a < im.n && im.pflag[a];
+                pflag[A + a] = pa ? 1 : 0;
+                pa_iac[A + a] = pa ? im.pa_iac[a] : 0;
+                pb_iac[A + a] = pa ? im.pb_iac[a] : 0;
+                pa_q[A + a] = pa ? im.pa_q[a] : 0.0;
+                pb_q[A + a] = pa ? im.pb_q[a] : 0.0;
+                palj[A + a] = pa ? im.palj[a] : 0.0;
+                pacrf[A + a] = pa ? im.pacrf[a] : 0.0;
+            }
         if (im.P.use_settle)
             for (size_t k = 0; k < im.mol_first.size(); ++k) mol_first[o.molf + k] = A + im.mol_first[k];
         for (size_t k = 0; k < im.papairs.size(); k += 4) {
@@ -354,7 +367,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             papairs[o.pap + k + 2] = im.papairs[k + 2];
             papairs[o.pap + k + 3] = im.papairs[k + 3];
         }
-        std::copy(im.lj.begin(), im.lj.end(), lj.begin() + o.lj);
+        std::copy(im.lj.begin(), im.lj.end(), lj.data() + o.lj);
         for (int32_t a = 0; a < im.J.n_solute_atoms; ++a) sol_atom[o.sol + a] = A + a;
     };
     {
@@ -417,9 +430,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->nvv_chunk = (int32_t)vv_chunk.size();
     // per-atom index of the PERTATOMPAIR entries (both atoms of an entry),
     // entries of an atom in entry order
-    std::vector<int32_t> pap_off, pap;
+    PinnedArena::Span<int32_t> pap_off, pap;
     if (any_pert) {
-        pap_off.assign(npad + 1, 0);
+        pap_off = S->host.take<int32_t>(np1);
+        std::fill(pap_off.begin(), pap_off.end(), 0);
         for (int32_t e = 0; e < S->n_papairs; ++e) {
             int32_t i = papairs[4 * e], jj = papairs[4 * e + 1];
             if (i == jj) continue;   // zero distance: skipped by the reference (perturbed.rs:668-671)
@@ -427,7 +441,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             pap_off[jj + 1]++;
         }
         for (int64_t a = 0; a < npad; ++a) pap_off[a + 1] += pap_off[a];
-        pap.resize(pap_off[npad]);
+        pap = S->host.take<int32_t>(pap_off[npad]);
+        if (S->host.used > S->host.cap) return fail(GROMOS_AMD_EHIP, "layout staging overflow (internal)");
         std::vector<int32_t> fillp(pap_off.begin(), pap_off.end() - 1);
         for (int32_t e = 0; e < S->n_papairs; ++e) {
             int32_t i = papairs[4 * e], jj = papairs[4 * e + 1];

@@ -943,6 +943,58 @@ __device__ __forceinline__ void pert_pair_direct(const SysPtrs& p, const JobDev&
     }
 }
 
+// One perturbed atom's dual-topology parameters and position, loaded once where
+// it is the first atom of many pairs (a solute atom's lanes), with the soft-core
+// RF constants of its pairs with unperturbed partners (alpha = its own).
+struct PertAtom {
+    int32_t a_iac, b_iac;
+    double a_q, b_q, alj, acrf;
+    d3 x;
+    SoftRF rf;
+};
+__device__ __forceinline__ PertAtom pert_atom(const SysPtrs& p, const JobDev& J, int32_t i) {
+    PertAtom a;
+    a.a_iac = p.pa_iac[i];
+    a.b_iac = p.pb_iac[i];
+    a.a_q = p.pa_q[i];
+    a.b_q = p.pb_q[i];
+    a.alj = p.palj[i];
+    a.acrf = p.pacrf[i];
+    a.x = ld3(p.x_cur, i);
+    a.rf = soft_rf(a.acrf, J.lp, J.crf);
+    return a;
+}
+
+// pert_pair_direct with the first atom i given as a PertAtom (same operations)
+__device__ __forceinline__ void pert_pair_direct_i(const SysPtrs& p, const JobDev& J, const PertAtom& A, int32_t j,
+                                                   bool vir, double (&v)[ROW], double& fx, double& fy, double& fz) {
+    const bool pj = p.pflag[j] != 0;
+    const int32_t a_iac_j = pj ? p.pa_iac[j] : p.iac[j], b_iac_j = pj ? p.pb_iac[j] : p.iac[j];
+    const double* la = p.lj + J.lj_off + 4 * ((int64_t)A.a_iac * J.n_types + a_iac_j);
+    const double* lb = p.lj + J.lj_off + 4 * ((int64_t)A.b_iac * J.n_types + b_iac_j);
+    const double a_q = A.a_q * (pj ? p.pa_q[j] : p.q[j]);
+    const double b_q = A.b_q * (pj ? p.pb_q[j] : p.q[j]);
+    double alj = A.alj, acrf = A.acrf;
+    SoftRF rf = A.rf;
+    if (pj) {
+        alj = (A.alj + p.palj[j]) * 0.5;
+        acrf = (A.acrf + p.pacrf[j]) * 0.5;
+        rf = soft_rf(acrf, J.lp, J.crf);
+    }
+    d3 r = nim(J.box, A.x, ld3(p.x_cur, j));
+    double o[5];
+    pert_lj_crf64_rf(r.x, r.y, r.z, la[0], la[1], lb[0], lb[1], a_q, b_q, alj, acrf, J.lp, J.crf, rf, J.fpepsi, o);
+    own_side(true, r, o[0], fx, fy, fz);
+    v[0] += o[1];
+    v[1] += o[2];
+    v[11] += o[3] + o[4];
+    if (vir) {
+        double fa, elja, ecrfa;
+        lj_crf64(r.x, r.y, r.z, la[0], la[1], a_q * J.fpepsi, J.crf, fa, elja, ecrfa);
+        add_virial(v, r, r.x * fa, r.y * fa, r.z * fa);
+    }
+}
+
 // perturbed_excluded_correction for one excluded pair i < j (perturbed.rs:445-510)
 __device__ __forceinline__ void pert_excl_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
                                                 double (&v)[ROW], double& fx, double& fy, double& fz) {
@@ -1084,7 +1136,7 @@ __device__ __forceinline__ void gen_pair(const SysPtrs& p, const JobDev& J, doub
 // The pairs of one sorted atom a (original o) from its group's general list
 // and, for solute groups, its own group: entries e0 + lane0, e0 + lane0 + stride,
 // ... (one lane: lane0 = 0, stride = 1; a wave: lane0 = lane, stride = 64).
-template <bool LONG>
+template <bool LONG, bool SOLV>
 __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t o, int32_t lane0, int32_t stride,
                                            int32_t virial, double (&v)[ROW], double& fx, double& fy, double& fz) {
     int32_t k = p.s_atom_cg[a];
@@ -1097,6 +1149,9 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
     int32_t ta = p.s_type[a];
     bool cgp = J.use_cg_grouped != 0;
     const bool jp = !LONG && J.has_pert;   // perturbation deltas ride on the short pass
+    const bool po = !SOLV && jp && p.pflag[o];   // solvent atoms are never perturbed
+    PertAtom PA{};
+    if (po) PA = pert_atom(p, J, o);
     int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
     for (int32_t e = e0 + lane0; e < e1; e += stride) {
         uint32_t ent = p.gen_ent[e];
@@ -1127,9 +1182,12 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
             }
             if (pchk) {
                 int32_t ob = p.s2o[sm + b];
-                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                if (p.pflag[fi]) {
-                    pert_pair_direct(p, J, fi, fj, canon, virial != 0, v, fx, fy, fz);
+                if (canon && po) {
+                    pert_pair_direct_i(p, J, PA, ob, virial != 0, v, fx, fy, fz);
+                    continue;
+                }
+                if (!canon && p.pflag[ob]) {
+                    pert_pair_direct(p, J, ob, o, false, virial != 0, v, fx, fy, fz);
                     continue;
                 }
             }
@@ -1148,9 +1206,12 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
             bool canon = ia < b;
             if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             if (pown) {
-                int32_t fi = canon ? o : ob, fj = canon ? ob : o;
-                if (p.pflag[fi]) {
-                    pert_pair_direct(p, J, fi, fj, canon, virial != 0, v, fx, fy, fz);
+                if (canon && po) {
+                    pert_pair_direct_i(p, J, PA, ob, virial != 0, v, fx, fy, fz);
+                    continue;
+                }
+                if (!canon && p.pflag[ob]) {
+                    pert_pair_direct(p, J, ob, o, false, virial != 0, v, fx, fy, fz);
                     continue;
                 }
             }
@@ -1173,7 +1234,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
         const JobDev& J = p.jobs[p.atom_job[o]];
-        if (o - J.atom_off >= J.n_solute_atoms) gen_atom64<LONG>(p, a, o, 0, 1, virial, v, fx, fy, fz);
+        if (o - J.atom_off >= J.n_solute_atoms) gen_atom64<LONG, true>(p, a, o, 0, 1, virial, v, fx, fy, fz);
     }
     if (a < npad) {
         double* f = LONG ? p.f_long_s : p.f_s;
@@ -1203,7 +1264,7 @@ __device__ __forceinline__ void partner_fx(const SysPtrs& p, int32_t sb, double 
 // from the groups' first atoms (their nearest-image vector taken in FP64), FP64
 // sums; pairs with a perturbed first atom entirely in FP64.  Energies from the
 // canonical partner only.
-template <bool LONG, bool VIR>
+template <bool LONG, bool VIR, bool SOLV>
 __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t o, int32_t lane0, int32_t stride,
                                            double (&v)[ROW], double& fx, double& fy, double& fz) {
     int32_t k = p.s_atom_cg[a];
@@ -1220,6 +1281,12 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
     int32_t ta = p.s_type[a];
     bool cgp = J.use_cg_grouped != 0 && !LONG;
     const bool jp = !LONG && J.has_pert;
+    // SOLV (a solvent atom's lane): solvent atoms are never perturbed, so the
+    // lane never evaluates a perturbed pair (its first atom's lanes do both
+    // shares) and carries no FP64 soft-core code
+    const bool po = !SOLV && jp && p.pflag[o];
+    PertAtom PA{};
+    if (po) PA = pert_atom(p, J, o);
     int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
     for (int32_t e = e0 + lane0; e < e1; e += stride) {
         uint32_t ent = p.gen_ent[e];
@@ -1231,7 +1298,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
         if (lng != LONG) {
             // only the pair's first atom, when perturbed, takes the delta, for both
             // atoms (the partner's share as an exact fixed-point atomic add)
-            if (!pchk || canon == 0.f || !p.pflag[o]) continue;
+            if (SOLV || !pchk || canon == 0.f || !po) continue;
             for (int32_t b = 0; b < nm; ++b) {
                 int32_t ob = p.s2o[sm + b];
                 double lx = 0.0, ly = 0.0, lz = 0.0;
@@ -1256,11 +1323,11 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 // a perturbed first atom: the soft-core pair in FP64, both atoms'
                 // shares, by the first atom's lanes only
                 int32_t ob = p.s2o[sm + b];
-                int32_t fi = canon != 0.f ? o : ob;
-                if (p.pflag[fi]) {
-                    if (canon == 0.f) continue;
+                if (canon == 0.f) {
+                    if (p.pflag[ob]) continue;
+                } else if (po) {
                     double lx = 0.0, ly = 0.0, lz = 0.0;
-                    pert_pair_direct(p, J, o, ob, true, VIR, v, lx, ly, lz);
+                    pert_pair_direct_i(p, J, PA, ob, VIR, v, lx, ly, lz);
                     fx += lx;
                     fy += ly;
                     fz += lz;
@@ -1292,7 +1359,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 add_virial(v, d3{(double)(canon * rx), (double)(canon * ry), (double)(canon * rz)}, gx, gy, gz);
         }
     }
-    if (!LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
+    if (!SOLV && !LONG && p.s_cg_orig[k] - J.cg_off < J.n_solute_cg) {
         int32_t ia = a - sk;
         bool pown = jp && p.cg_pert[p.s_cg_orig[k]];
         for (int32_t b = lane0; b < nk; b += stride) {
@@ -1306,7 +1373,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 if (p.pflag[fi]) {
                     if (!canon) continue;
                     double lx = 0.0, ly = 0.0, lz = 0.0;
-                    pert_pair_direct(p, J, o, ob, true, VIR, v, lx, ly, lz);
+                    pert_pair_direct_i(p, J, PA, ob, VIR, v, lx, ly, lz);
                     fx += lx;
                     fy += ly;
                     fz += lz;
@@ -1339,7 +1406,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (o >= 0) {
         const JobDev& J = p.jobs[p.atom_job[o]];
-        if (o - J.atom_off >= J.n_solute_atoms) gen_atom32<LONG, VIR>(p, a, o, 0, 1, v, fx, fy, fz);
+        if (o - J.atom_off >= J.n_solute_atoms) gen_atom32<LONG, VIR, true>(p, a, o, 0, 1, v, fx, fy, fz);
     }
     if (a < npad) {
         long long ix = to_fx(fx), iy = to_fx(fy), iz = to_fx(fz);
@@ -1371,8 +1438,8 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int32_t o = p.sol_atom[w];
     const int32_t a = p.o2s[o];
-    if (MIXED) gen_atom32<LONG, VIR>(p, a, o, threadIdx.x, BLOCK, v, fx, fy, fz);
-    else gen_atom64<LONG>(p, a, o, threadIdx.x, BLOCK, virial, v, fx, fy, fz);
+    if (MIXED) gen_atom32<LONG, VIR, false>(p, a, o, threadIdx.x, BLOCK, v, fx, fy, fz);
+    else gen_atom64<LONG, false>(p, a, o, threadIdx.x, BLOCK, virial, v, fx, fy, fz);
 #pragma unroll
     for (int m = 1; m < WAVE; m <<= 1) {
         fx += __shfl_xor(fx, m, WAVE);

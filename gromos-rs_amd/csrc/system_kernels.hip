@@ -995,6 +995,76 @@ __device__ __forceinline__ void pert_pair_direct_i(const SysPtrs& p, const JobDe
     }
 }
 
+// pert_pair_direct_i with an unperturbed partner read from the sorted copies
+// (xq = position and charge, s_type = type; the same values as the original
+// arrays): one dependent load less per pair
+__device__ __forceinline__ void pert_pair_direct_s(const SysPtrs& p, const JobDev& J, const PertAtom& A, double4 xb,
+                                                   int32_t tb, bool vir, double (&v)[ROW], double& fx, double& fy,
+                                                   double& fz) {
+    const double* la = p.lj + J.lj_off + 4 * ((int64_t)A.a_iac * J.n_types + tb);
+    const double* lb = p.lj + J.lj_off + 4 * ((int64_t)A.b_iac * J.n_types + tb);
+    const double a_q = A.a_q * xb.w;
+    const double b_q = A.b_q * xb.w;
+    d3 r = nim(J.box, A.x, xyz(xb));
+    double o[5];
+    pert_lj_crf64_rf(r.x, r.y, r.z, la[0], la[1], lb[0], lb[1], a_q, b_q, A.alj, A.acrf, J.lp, J.crf, A.rf, J.fpepsi,
+                     o);
+    own_side(true, r, o[0], fx, fy, fz);
+    v[0] += o[1];
+    v[1] += o[2];
+    v[11] += o[3] + o[4];
+    if (vir) {
+        double fa, elja, ecrfa;
+        lj_crf64(r.x, r.y, r.z, la[0], la[1], a_q * J.fpepsi, J.crf, fa, elja, ecrfa);
+        add_virial(v, r, r.x * fa, r.y * fa, r.z * fa);
+    }
+}
+
+// FP32 counterpart of perturbed_lj_crf_interaction's force (perturbed.rs:124-266),
+// for the unperturbed partner's share in the mixed path: returns f/r (same
+// expression as pert_lj_crf64_rf, single precision, no energies).
+__device__ __forceinline__ float pert_force32(float r2, float a_c6, float a_c12, float b_c6, float b_c12, float a_q,
+                                              float b_q, float alpha_lj, float alpha_crf, const Lambda& lp,
+                                              const Crf& crf, float fpepsi) {
+    if (r2 < 1e-10f) return 0.0f;
+    const float r4 = r2 * r2, r6 = r4 * r2;
+    const float a_c126 = fabsf(a_c6) > 1e-10f ? a_c12 / a_c6 : 0.0f;
+    const float b_c126 = fabsf(b_c6) > 1e-10f ? b_c12 / b_c6 : 0.0f;
+    const float a_d2 = r2 + alpha_crf * (float)lp.b_crfs_lambda2;
+    const float b_d2 = r2 + alpha_crf * (float)lp.a_crfs_lambda2;
+    const float a_di = 1.0f / sqrtf(a_d2), b_di = 1.0f / sqrtf(b_d2);
+    const float a_d3i = a_di / a_d2, b_d3i = b_di / b_d2;
+    const float a_d6i = 1.0f / (r6 + alpha_lj * (float)lp.b_ljs_lambda2 * a_c126);
+    const float b_d6i = 1.0f / (r6 + alpha_lj * (float)lp.a_ljs_lambda2 * b_c126);
+    const float cut2 = (float)crf.cutoff_sq;
+    const float a_c2s = cut2 + alpha_crf * (float)lp.b_crfs_lambda2;
+    const float b_c2s = cut2 + alpha_crf * (float)lp.a_crfs_lambda2;
+    const float a_crf_cut3i = 2.0f * (float)crf.crf_2cut3i / sqrtf(a_c2s * a_c2s * a_c2s);
+    const float b_crf_cut3i = 2.0f * (float)crf.crf_2cut3i / sqrtf(b_c2s * b_c2s * b_c2s);
+    const float aln = (float)lp.a_lj_lambda_n, bln = (float)lp.b_lj_lambda_n;
+    float f = ((float)lp.a_crf_lambda_n * a_q * (a_d3i + a_crf_cut3i) +
+               (float)lp.b_crf_lambda_n * b_q * (b_d3i + b_crf_cut3i)) * fpepsi;
+    f += -6.0f * (aln * a_c6 * a_d6i * a_d6i + bln * b_c6 * b_d6i * b_d6i) * r4;
+    f += 12.0f * (aln * a_c12 * a_d6i * a_d6i * a_d6i + bln * b_c12 * b_d6i * b_d6i * b_d6i) * r4;
+    return f;
+}
+
+// The unperturbed second atom j's share of a listed short pair whose first atom
+// i is perturbed, in FP32 (mixed path): force only; the first atom's lanes add
+// the energies and their own share in FP64.
+__device__ __forceinline__ void pert_share32(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, double& fx,
+                                             double& fy, double& fz) {
+    PertPair q = pert_params(p, J, i, j);
+    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
+    const float rx = (float)r.x, ry = (float)r.y, rz = (float)r.z;
+    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    const float f = pert_force32(r2, (float)q.la[0], (float)q.la[1], (float)q.lb[0], (float)q.lb[1], (float)q.a_q,
+                                 (float)q.b_q, (float)q.alj, (float)q.acrf, J.lp, J.crf, (float)J.fpepsi);
+    fx -= (double)(rx * f);
+    fy -= (double)(ry * f);
+    fz -= (double)(rz * f);
+}
+
 // perturbed_excluded_correction for one excluded pair i < j (perturbed.rs:445-510)
 __device__ __forceinline__ void pert_excl_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
                                                 double (&v)[ROW], double& fx, double& fy, double& fz) {
@@ -1175,10 +1245,17 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
         }
         bool ss = (ent & ENT_SS) != 0;
         d3 pm0 = xyz(p.xq[sm]);
+        // a perturbed first atom with an unperturbed partner group: partner data
+        // from the sorted copies
+        const bool pfast = po && canon && !p.cg_pert[p.s_cg_orig[m]];
         for (int32_t b = 0; b < nm; ++b) {
             if (!LONG && ss) {
                 int32_t ob = p.s2o[sm + b];
                 if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
+            }
+            if (pfast) {
+                pert_pair_direct_s(p, J, PA, p.xq[sm + b], p.s_type[sm + b], virial != 0, v, fx, fy, fz);
+                continue;
             }
             if (pchk) {
                 int32_t ob = p.s2o[sm + b];
@@ -1251,14 +1328,6 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
-// The partner's share (minus the given pair force) of a pair evaluated by its
-// perturbed first atom, added to the partner's int64 fixed-point force: integer
-// additions commute, so the result does not depend on their order.
-__device__ __forceinline__ void partner_fx(const SysPtrs& p, int32_t sb, double fx, double fy, double fz) {
-    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb), (unsigned long long)to_fx(-fx));
-    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb + 1), (unsigned long long)to_fx(-fy));
-    atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)sb + 2), (unsigned long long)to_fx(-fz));
-}
 
 // Mixed-precision counterpart of gen_atom64: FP32 pair arithmetic on offsets
 // from the groups' first atoms (their nearest-image vector taken in FP64), FP64
@@ -1296,42 +1365,45 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
         bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
         int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
         if (lng != LONG) {
-            // only the pair's first atom, when perturbed, takes the delta, for both
-            // atoms (the partner's share as an exact fixed-point atomic add)
-            if (SOLV || !pchk || canon == 0.f || !po) continue;
+            // cached long pairs: the correction is taken every step at the current
+            // positions (forcefield.rs:796-830), each atom its own share (FP64)
+            if (SOLV || !pchk) continue;
             for (int32_t b = 0; b < nm; ++b) {
                 int32_t ob = p.s2o[sm + b];
-                double lx = 0.0, ly = 0.0, lz = 0.0;
-                pert_pair_delta(p, J, o, ob, true, v, lx, ly, lz);
-                fx += lx;
-                fy += ly;
-                fz += lz;
-                partner_fx(p, sm + b, lx, ly, lz);
+                int32_t fi = canon != 0.f ? o : ob, fj = canon != 0.f ? ob : o;
+                if (p.pflag[fi]) pert_pair_delta(p, J, fi, fj, canon != 0.f, v, fx, fy, fz);
             }
             continue;
         }
         bool ss = (ent & ENT_SS) != 0;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         if (cgp) shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+        // a perturbed first atom with an unperturbed partner group: partner data
+        // from the sorted copies
+        const bool pfast = po && canon != 0.f && !p.cg_pert[p.s_cg_orig[m]];
         for (int32_t b = 0; b < nm; ++b) {
             if (!LONG && ss) {
                 int32_t ob = p.s2o[sm + b];
                 if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             }
             double4 xb = p.xq[sm + b];
+            if (pfast) {
+                pert_pair_direct_s(p, J, PA, xb, p.s_type[sm + b], VIR, v, fx, fy, fz);
+                continue;
+            }
             if (pchk) {
-                // a perturbed first atom: the soft-core pair in FP64, both atoms'
-                // shares, by the first atom's lanes only
+                // a perturbed first atom: its lanes take the soft-core pair in FP64
+                // with the energies; the second atom's lanes take their own share
+                // (a solvent atom in FP32, a solute atom in FP64)
                 int32_t ob = p.s2o[sm + b];
                 if (canon == 0.f) {
-                    if (p.pflag[ob]) continue;
+                    if (p.pflag[ob]) {
+                        if (SOLV) pert_share32(p, J, ob, o, fx, fy, fz);
+                        else pert_pair_direct(p, J, ob, o, false, VIR, v, fx, fy, fz);
+                        continue;
+                    }
                 } else if (po) {
-                    double lx = 0.0, ly = 0.0, lz = 0.0;
-                    pert_pair_direct_i(p, J, PA, ob, VIR, v, lx, ly, lz);
-                    fx += lx;
-                    fy += ly;
-                    fz += lz;
-                    partner_fx(p, sm + b, lx, ly, lz);
+                    pert_pair_direct_i(p, J, PA, ob, VIR, v, fx, fy, fz);
                     continue;
                 }
             }
@@ -1369,15 +1441,12 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
             if (canon ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             double4 xb = p.xq[sk + b];
             if (pown) {
-                int32_t fi = canon ? o : ob;
-                if (p.pflag[fi]) {
-                    if (!canon) continue;
-                    double lx = 0.0, ly = 0.0, lz = 0.0;
-                    pert_pair_direct_i(p, J, PA, ob, VIR, v, lx, ly, lz);
-                    fx += lx;
-                    fy += ly;
-                    fz += lz;
-                    partner_fx(p, sk + b, lx, ly, lz);
+                if (canon && po) {
+                    pert_pair_direct_i(p, J, PA, ob, VIR, v, fx, fy, fz);
+                    continue;
+                }
+                if (!canon && p.pflag[ob]) {
+                    pert_pair_direct(p, J, ob, o, false, VIR, v, fx, fy, fz);
                     continue;
                 }
             }
@@ -1470,11 +1539,10 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
                 p.fi_long_s[3 * a + 1] = iy;
                 p.fi_long_s[3 * a + 2] = iz;
             } else {
-                // other solute blocks may add partner shares to this atom at the
-                // same time: atomic
-                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a), (unsigned long long)ix);
-                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a + 1), (unsigned long long)iy);
-                atomicAdd((unsigned long long*)(p.fi_s + 3 * (int64_t)a + 2), (unsigned long long)iz);
+                // one block per atom: nothing else writes this atom meanwhile
+                p.fi_s[3 * a] += ix;
+                p.fi_s[3 * a + 1] += iy;
+                p.fi_s[3 * a + 2] += iz;
             }
         } else {
             if (LONG) {

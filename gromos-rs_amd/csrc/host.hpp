@@ -145,6 +145,18 @@ struct gromos_amd_system {
     int32_t* d_sol_atom = nullptr;
     int32_t nsol = 0;
     int4* d_vv_chunk = nullptr;   // solvent pair kernel chunks (first sorted group, end, job, 0)
+    int32_t* d_step = nullptr;    // md step index within the current md call
+    uint32_t* d_step_done = nullptr;
+    // captured plain steps (no rebuild, no outside change), one per state buffer
+    // parity; replayed while the kernels' arguments are unchanged
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        gamd::SysPtrs key{};
+        int64_t shape[10] = {};
+    };
+    StepGraph graphs[2];
+    bool use_graphs = true;
+    int64_t graph_launches = 0, graph_captures = 0;
     int32_t nvv_chunk = 0;
     int32_t vv_lj00 = 0;   // every job's solvent has LJ on the first-first atom pair only
     int32_t vv_neutral = 0;  // every job's solvent molecule has zero net charge

@@ -1827,10 +1827,10 @@ __device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t w
 }
 
 template <bool MIXED>
-__global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int32_t step, int32_t with_long,
-                                                     int32_t write_xq) {
+__global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int32_t with_long, int32_t write_xq) {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncg) return;
+    const int32_t step = *p.step;   // advanced by the last block of this step's k_reduce
     int32_t jb = p.cg_job[c];
     const JobDev& J = p.jobs[jb];
     const int32_t s = p.cg_start[c], n = p.cg_n[c];
@@ -1914,10 +1914,13 @@ __global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_viri
 // One block per job: sums the job's partial rows in a fixed order and writes
 // the step record (e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]).  Column 11 of a
 // row is E_kin in the kinetic slot and dH/dlambda in the others.
+// advance: an md step (records row = the device step counter, which the last
+// block to finish advances); otherwise a single point (row 0, counter untouched)
 __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32_t with_long,
-                                                  int32_t with_kin, int32_t step, int32_t njobs) {
+                                                  int32_t with_kin, int32_t advance, int32_t njobs) {
     constexpr int W = ROW + 2;   // + dH/dlambda, E_kin_new
     __shared__ double lds[17 * W];
+    const int32_t step = advance ? *p.step : 0;
     int32_t jb = blockIdx.x;
     const JobDev& J = p.jobs[jb];
     const int32_t stride = p.nrow_blocks;
@@ -1971,6 +1974,16 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         // the next step's Berendsen scale from this step's E_kin_new: what the
         // reference reads after TemperatureCalculation stored it
         if (with_kin && J.thermo) p.therm_scale[jb] = berendsen_scale(J, t[13]);
+        if (advance) {
+            // every block read the counter at its start: the last one to finish
+            // advances it for the next step's kernels and rearms the count
+            __threadfence();
+            if (atomicAdd(p.step_done, 1u) == gridDim.x - 1) {
+                *p.step_done = 0u;
+                *p.step = step + 1;
+                __threadfence();
+            }
+        }
     }
 }
 
@@ -2063,16 +2076,23 @@ static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t m
     // waves: VV_CHUNK / (WAVE / L) per chunk of groups
     const int32_t nw = p.nvv_chunk * (VV_CHUNK / (WAVE / L));
     dim3 g = xcd_grid((nw + BLOCK / WAVE - 1) / (BLOCK / WAVE));
+    // without timing events: a plain launch (also inside a captured step graph)
+#define GAMD_VV_LAUNCH(KER)                                                                      \
+    do {                                                                                         \
+        if (e0 || e1) hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);       \
+        else hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                               \
+    } while (0)
     if (mixed && neutral) {
-        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
-        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, true>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, true>));
+        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, true>));
     } else if (mixed) {
-        if (virial) hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, true, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
-        else hipExtLaunchKernelGGL((k_vv_mx<APS, LJ00, false, L, false>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, false>));
+        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, false>));
     } else {
-        if (virial) hipExtLaunchKernelGGL((k_vv<APS, LJ00, true, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
-        else hipExtLaunchKernelGGL((k_vv<APS, LJ00, false, L>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        if (virial) GAMD_VV_LAUNCH((k_vv<APS, LJ00, true, L>));
+        else GAMD_VV_LAUNCH((k_vv<APS, LJ00, false, L>));
     }
+#undef GAMD_VV_LAUNCH
 }
 template <int APS, bool LJ00>
 static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
@@ -2128,18 +2148,18 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
                      hipStream_t s) {
     hipLaunchKernelGGL(k_assemble, grid_for(npad), dim3(BLOCK), 0, s, p, npad, with_long, mixed);
 }
-void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t step, int32_t with_long, int32_t write_xq,
+void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,
                       int32_t mixed, hipStream_t s) {
-    if (mixed) hipLaunchKernelGGL(k_integrate<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, step, with_long, write_xq);
-    else hipLaunchKernelGGL(k_integrate<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, step, with_long, write_xq);
+    if (mixed) hipLaunchKernelGGL(k_integrate<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, with_long, write_xq);
+    else hipLaunchKernelGGL(k_integrate<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, with_long, write_xq);
 }
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_therm, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
-void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
+void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t advance,
                    int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(BLOCK), 0, s, p, p.vv_virial, with_kin);
-    hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, step, njobs);
+    hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, advance, njobs);
 }
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,
                         uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3, hipStream_t s) {

@@ -49,6 +49,8 @@ struct SysPtrs {
     int64_t* gen_cnt;
     const int4* vv_chunk;      // nvv_chunk: solvent pair kernel chunks (first sorted group, end, job, 0)
     int32_t nvv_chunk;
+    int32_t* step;             // md step index within the current md call (device counter)
+    uint32_t* step_done;       // blocks of k_reduce finished (self-resetting)
     const int64_t* vv_off;
     const int64_t* gen_off;
     uint32_t* vv_ent;
@@ -116,9 +118,9 @@ void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
-void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t step, int32_t with_long, int32_t write_xq,
+void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,
                       int32_t mixed, hipStream_t s);
-void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t step,
+void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t advance,
                    int32_t njobs, hipStream_t s);
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,
                         uint32_t* o0, uint32_t* o1, uint32_t* o2, uint32_t* o3, hipStream_t s);

@@ -128,6 +128,8 @@ def _declare(L):
     L.gromos_amd_system_device_bytes.restype = I64
     L.gromos_amd_system_vv_lanes.argtypes = [P]
     L.gromos_amd_system_vv_lanes.restype = I32
+    L.gromos_amd_system_graph_launches.argtypes = [P]
+    L.gromos_amd_system_graph_launches.restype = I64
     L.gromos_amd_session_create.argtypes = [C.POINTER(SessionParams), C.POINTER(P)]
     L.gromos_amd_session_destroy.argtypes = [P]
     L.gromos_amd_job_footprint.argtypes = [C.POINTER(Topology), C.POINTER(Params)]
@@ -304,6 +306,11 @@ class DeviceSystem:
     def vv_lanes(self):
         """lanes per solvent molecule chosen by the autotuner"""
         return lib().gromos_amd_system_vv_lanes(self.h)
+
+    @property
+    def graph_launches(self):
+        """md steps run as a replayed graph so far"""
+        return lib().gromos_amd_system_graph_launches(self.h)
 
     def kernel_timing(self, enable=-1):
         ms, n = D(), I64()

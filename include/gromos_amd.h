@@ -242,6 +242,11 @@ int64_t gromos_amd_system_device_bytes(const gromos_amd_system *sys);
  * 0 before the first force evaluation).  GROMOS_AMD_VV_LANES in the environment
  * forces a value. */
 int32_t gromos_amd_system_vv_lanes(const gromos_amd_system *sys);
+/* md steps run as a replayed graph so far.  With GROMOS_AMD_GRAPHS=1 the plain
+ * steps (no pair-list rebuild now or next, no outside state change, no
+ * per-kernel timing) replay one captured graph each; by default every step is
+ * launched kernel by kernel (faster on ROCm 7.2).  No counterpart in the reference. */
+int64_t gromos_amd_system_graph_launches(const gromos_amd_system *sys);
 
 /* ------------------------------------------------------------------ */
 /* 3. sessions: continuous batching of independent jobs on one device  */

@@ -427,6 +427,27 @@ def test_vv_lane_variants(lanes, prec, monkeypatch):
         assert force_err(g["forces"], r["forces"]) < MX_F_TOL
 
 
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_step_graphs_match_separate_launches(prec, monkeypatch):
+    """Plain steps replayed from captured graphs (with the eager rebuild steps and
+    host synchronisations of every md call between replays) give the same bits
+    as every step launched separately."""
+    s = gen.spc_water_box(1500, seed=16)
+    s.thermostat = (300.0, 0.1, 3 * s.n_atoms - 3 * s.nsm)
+    out = {}
+    for graphs in ("1", "0"):   # opt-in replay vs the default separate launches
+        monkeypatch.setenv("GROMOS_AMD_GRAPHS", graphs)
+        ds = rt.DeviceSystem(s, precision=prec)
+        ds.pairlist_update()
+        e = np.concatenate([ds.md(7), ds.md(1), ds.md(13), ds.md(9)])
+        x, v = ds.get_state()
+        out[graphs] = (e, x, v, ds.graph_launches)
+        ds.close()
+    assert out["1"][3] > 0 and out["0"][3] == 0
+    for k in range(3):
+        assert np.array_equal(out["1"][k], out["0"][k])
+
+
 def test_autotuner_picks_a_variant():
     s = gen.spc_water_box(3000, seed=13)
     ds = rt.DeviceSystem(s, precision=rt.MIXED)

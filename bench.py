@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--fep-alone-steps", type=int, default=200,
                     help="steps per job timed for the one-job-at-a-time plain FP64 comparison")
     ap.add_argument("--remd-steps", type=int, default=1000, help="steps of the 128-replica exchange run")
-    ap.add_argument("--fep-admit-interval", type=int, default=25,
+    ap.add_argument("--fep-admit-interval", type=int, default=50,
                     help="steps between batch layout changes of the FEP server (a multiple of 5)")
     ap.add_argument("--no-fep", action="store_true")
     ap.add_argument("--no-remd", action="store_true")

@@ -322,7 +322,12 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
     while (nsteps > 0) {
         if (s->clock % nsnb == 0)
             if (int rc = boundary(s)) return rc;
-        int32_t k = std::min<int32_t>(nsteps, nsnb - (int32_t)(s->clock % nsnb));
+        // one md call up to the next admission boundary (a job that finishes on the
+        // way is held by its stop step), or to the next NSNB boundary while jobs
+        // wait in the queue (an idle batch takes them in at once)
+        const int32_t ai = s->prm.admit_interval;
+        const int32_t span = s->queue.empty() ? ai - (int32_t)(s->clock % ai) : nsnb - (int32_t)(s->clock % nsnb);
+        int32_t k = std::min<int32_t>(nsteps, span);
         gromos_amd_system* S = s->S.get();
         if (S && running_count(s) > 0) {
             bool changed = false;

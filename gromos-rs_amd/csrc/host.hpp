@@ -146,6 +146,8 @@ struct gromos_amd_system {
     int32_t nsol = 0;
     int4* d_vv_chunk = nullptr;   // solvent pair kernel chunks (first sorted group, end, job, 0)
     int32_t* d_step = nullptr;    // md step index within the current md call
+    float4* d_prec = nullptr;     // perturbed solute atoms' FP32 parameter records (2 per solute atom)
+    int32_t* d_s_prec = nullptr;  // sorted atom -> record index, -1 if not perturbed
     uint32_t* d_step_done = nullptr;
     // captured plain steps (no rebuild, no outside change), one per state buffer
     // parity; replayed while the kernels' arguments are unchanged

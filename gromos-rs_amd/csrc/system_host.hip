@@ -196,6 +196,10 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
         for (int k = 0; k < T.n_pert_atom_pairs; ++k) {
             const int32_t* e = T.pert_atom_pairs + 4 * k;
             if (e[0] < 0 || e[0] >= n || e[1] < 0 || e[1] >= n) return fail(GROMOS_AMD_EINVAL, "job %d: atom pair out of range", j);
+            // like the perturbed atoms themselves, PERTATOMPAIR entries are solute
+            // atoms (the solvent takes no excluded / 1-4 / atom-pair forces here)
+            if (e[0] >= T.n_solute_atoms || e[1] >= T.n_solute_atoms)
+                return fail(GROMOS_AMD_EINVAL, "job %d: PERTATOMPAIR atoms must be solute atoms", j);
             for (int c = 0; c < 4; ++c) im.papairs.push_back(e[c]);
         }
     }
@@ -419,6 +423,27 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     while ((1ll << S->cell_bits) < (int64_t)S->ncell + 1) S->cell_bits++;
     S->n_papairs = (int32_t)papairs.size() / 4;
     S->nsol = (int32_t)sol_atom.size();
+    // FP32 records of the perturbed solute atoms for the unperturbed partners'
+    // shares: charges, alphas, soft-core RF constants (perturbed.rs:160-190), types
+    std::vector<float4> prec(2 * std::max<int64_t>(T.sol, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int j = 0; j < njobs; ++j) {
+        const JobImage& im = *imgs[j];
+        if (!im.has_pert) continue;
+        const JobDev& J = S->jobs[j];
+        for (int32_t a = 0; a < J.n_solute_atoms; ++a) {
+            if (!im.pflag[a]) continue;
+            const int64_t w = sl[j].sol + a;
+            const double cut2 = J.crf.cutoff_sq, acrf = im.pacrf[a];
+            const double a_c2s = cut2 + acrf * J.lp.b_crfs_lambda2, b_c2s = cut2 + acrf * J.lp.a_crfs_lambda2;
+            const double a_cut3i = 2.0 * J.crf.crf_2cut3i / std::sqrt(a_c2s * a_c2s * a_c2s);
+            const double b_cut3i = 2.0 * J.crf.crf_2cut3i / std::sqrt(b_c2s * b_c2s * b_c2s);
+            float ai, bi;
+            std::memcpy(&ai, &im.pa_iac[a], 4);
+            std::memcpy(&bi, &im.pb_iac[a], 4);
+            prec[2 * w] = make_float4((float)im.pa_q[a], (float)im.pb_q[a], (float)im.palj[a], (float)acrf);
+            prec[2 * w + 1] = make_float4((float)a_cut3i, (float)b_cut3i, ai, bi);
+        }
+    }
     S->has_pert = any_pert;
     // chunks of at most VV_CHUNK sorted groups, each within one job (a job's
     // groups keep its own range [cg_off, cg_off + n_cg) in sorted space)
@@ -484,6 +509,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_pap_off, pap_off);
     rc = rc ? rc : S->upload(S->d_pap, pap);
     rc = rc ? rc : S->upload(S->d_vv_chunk, vv_chunk);
+    rc = rc ? rc : S->upload(S->d_prec, prec);
     if (rc) return rc;
     size_t n3 = (size_t)npad * 3, nc = (size_t)ncg, ncl = (size_t)S->ncell + 1, npa = (size_t)npad;
     for (int k = 0; k < 2; ++k) {
@@ -500,7 +526,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_s_cg_n, nc + 1, true)) || (rc = S->reserve(S->d_s_cg_scan, nc + 1, true)) ||
         (rc = S->reserve(S->d_s_cg_start, nc, true)) || (rc = S->reserve(S->d_s2o, npa, true)) ||
         (rc = S->reserve(S->d_s_atom_cg, npa, true)) || (rc = S->reserve(S->d_o2s, npa, true)) ||
-        (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
+        (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_s_prec, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 4)) ||
         (rc = S->reserve(S->d_pc_short, nc, true)) || (rc = S->reserve(S->d_pc_long, nc, true)) ||
@@ -516,7 +542,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
-        (rc = S->reserve(S->d_cg_ek, 2 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
+        (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
         (rc = S->reserve(S->d_step_done, 1)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
@@ -755,7 +781,7 @@ int forcefield(gromos_amd_system* S, bool assemble) {
         S->long_computed = true;
         S->pair_count += (int64_t)S->host_counters[1];
     }
-    launch_excl14(p, npad, st);
+    launch_excl14(p, npad, assemble ? 1 : 0, st);
     if (assemble) launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
     HIPCHK(hipGetLastError());
     return 0;
@@ -782,7 +808,7 @@ static int launch_step_graph(gromos_amd_system* S) {
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st);
         launch_gen(p, S->npad, false, S->do_virial, mixed, st);
-        launch_excl14(p, S->npad, st);
+        launch_excl14(p, S->npad, 0, st);
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, 1, mixed, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
         hipError_t e = hipStreamEndCapture(st, &graph);
@@ -996,6 +1022,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.vv_chunk = d_vv_chunk;
     p.nvv_chunk = nvv_chunk;
     p.step = d_step;
+    p.prec = d_prec;
+    p.s_prec = d_s_prec;
     p.step_done = d_step_done;
     p.sol_e = d_sol_e;
     p.vv_virial = do_virial;

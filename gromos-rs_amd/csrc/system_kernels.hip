@@ -112,6 +112,7 @@ __global__ void k_sorted_atoms(SysPtrs p, int32_t ncg) {
         p.o2s[o + a] = start + a;
         p.s_atom_cg[start + a] = k;
         p.s_type[start + a] = p.iac[o + a];
+        p.s_prec[start + a] = (J.has_pert && p.pflag[o + a]) ? J.sol_off + (o + a - J.atom_off) : -1;
     }
 }
 
@@ -1020,13 +1021,24 @@ __device__ __forceinline__ void pert_pair_direct_s(const SysPtrs& p, const JobDe
     }
 }
 
-// FP32 counterpart of perturbed_lj_crf_interaction's force (perturbed.rs:124-266),
-// for the unperturbed partner's share in the mixed path: returns f/r (same
-// expression as pert_lj_crf64_rf, single precision, no energies).
-__device__ __forceinline__ float pert_force32(float r2, float a_c6, float a_c12, float b_c6, float b_c12, float a_q,
-                                              float b_q, float alpha_lj, float alpha_crf, const Lambda& lp,
-                                              const Crf& crf, float fpepsi) {
-    if (r2 < 1e-10f) return 0.0f;
+// The unperturbed second atom j's share of a listed short pair whose first atom
+// i is perturbed, in FP32 (mixed path): force only; the first atom's lanes add
+// the energies and their own share in FP64.  i's parameters come from its FP32
+// record prec[2w..2w+1], its position from the sorted copy; j's type and charge
+// are the lane's own.
+__device__ __forceinline__ void pert_share32(const SysPtrs& p, const JobDev& J, int64_t w, d3 xi, d3 xj, int32_t tj,
+                                             float qj, double& fx, double& fy, double& fz) {
+    const float4 r0 = p.prec[2 * w], r1 = p.prec[2 * w + 1];
+    const int32_t a_iac = __float_as_int(r1.z), b_iac = __float_as_int(r1.w);
+    const double* la = p.lj + J.lj_off + 4 * ((int64_t)a_iac * J.n_types + tj);
+    const double* lb = p.lj + J.lj_off + 4 * ((int64_t)b_iac * J.n_types + tj);
+    d3 r = nim(J.box, xi, xj);
+    const float rx = (float)r.x, ry = (float)r.y, rz = (float)r.z;
+    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    if (r2 < 1e-10f) return;
+    const Lambda& lp = J.lp;
+    const float a_c6 = (float)la[0], a_c12 = (float)la[1], b_c6 = (float)lb[0], b_c12 = (float)lb[1];
+    const float a_q = r0.x * qj, b_q = r0.y * qj, alpha_lj = r0.z, alpha_crf = r0.w;
     const float r4 = r2 * r2, r6 = r4 * r2;
     const float a_c126 = fabsf(a_c6) > 1e-10f ? a_c12 / a_c6 : 0.0f;
     const float b_c126 = fabsf(b_c6) > 1e-10f ? b_c12 / b_c6 : 0.0f;
@@ -1036,30 +1048,11 @@ __device__ __forceinline__ float pert_force32(float r2, float a_c6, float a_c12,
     const float a_d3i = a_di / a_d2, b_d3i = b_di / b_d2;
     const float a_d6i = 1.0f / (r6 + alpha_lj * (float)lp.b_ljs_lambda2 * a_c126);
     const float b_d6i = 1.0f / (r6 + alpha_lj * (float)lp.a_ljs_lambda2 * b_c126);
-    const float cut2 = (float)crf.cutoff_sq;
-    const float a_c2s = cut2 + alpha_crf * (float)lp.b_crfs_lambda2;
-    const float b_c2s = cut2 + alpha_crf * (float)lp.a_crfs_lambda2;
-    const float a_crf_cut3i = 2.0f * (float)crf.crf_2cut3i / sqrtf(a_c2s * a_c2s * a_c2s);
-    const float b_crf_cut3i = 2.0f * (float)crf.crf_2cut3i / sqrtf(b_c2s * b_c2s * b_c2s);
     const float aln = (float)lp.a_lj_lambda_n, bln = (float)lp.b_lj_lambda_n;
-    float f = ((float)lp.a_crf_lambda_n * a_q * (a_d3i + a_crf_cut3i) +
-               (float)lp.b_crf_lambda_n * b_q * (b_d3i + b_crf_cut3i)) * fpepsi;
+    float f = ((float)lp.a_crf_lambda_n * a_q * (a_d3i + r1.x) + (float)lp.b_crf_lambda_n * b_q * (b_d3i + r1.y)) *
+              (float)J.fpepsi;
     f += -6.0f * (aln * a_c6 * a_d6i * a_d6i + bln * b_c6 * b_d6i * b_d6i) * r4;
     f += 12.0f * (aln * a_c12 * a_d6i * a_d6i * a_d6i + bln * b_c12 * b_d6i * b_d6i * b_d6i) * r4;
-    return f;
-}
-
-// The unperturbed second atom j's share of a listed short pair whose first atom
-// i is perturbed, in FP32 (mixed path): force only; the first atom's lanes add
-// the energies and their own share in FP64.
-__device__ __forceinline__ void pert_share32(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, double& fx,
-                                             double& fy, double& fz) {
-    PertPair q = pert_params(p, J, i, j);
-    d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
-    const float rx = (float)r.x, ry = (float)r.y, rz = (float)r.z;
-    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-    const float f = pert_force32(r2, (float)q.la[0], (float)q.la[1], (float)q.lb[0], (float)q.lb[1], (float)q.a_q,
-                                 (float)q.b_q, (float)q.alj, (float)q.acrf, J.lp, J.crf, (float)J.fpepsi);
     fx -= (double)(rx * f);
     fy -= (double)(ry * f);
     fz -= (double)(rz * f);
@@ -1362,7 +1355,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
         bool lng = (ent & ENT_LONG) != 0;
         int32_t m = (int32_t)(ent & ENT_IDX);
         float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
-        bool pchk = jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
+        bool pchk = !SOLV && jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
         int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
         if (lng != LONG) {
             // cached long pairs: the correction is taken every step at the current
@@ -1387,6 +1380,15 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 if (canon != 0.f ? excluded_or_14(p, o, ob) : excluded_or_14(p, ob, o)) continue;
             }
             double4 xb = p.xq[sm + b];
+            if (SOLV && jp && canon == 0.f) {
+                // a perturbed partner (always the pair's first atom): this lane's
+                // FP32 share
+                const int32_t w = p.s_prec[sm + b];
+                if (w >= 0) {
+                    pert_share32(p, J, w, xyz(xb), xyz(xa), ta, (float)xa.w, fx, fy, fz);
+                    continue;
+                }
+            }
             if (pfast) {
                 pert_pair_direct_s(p, J, PA, xb, p.s_type[sm + b], VIR, v, fx, fy, fz);
                 continue;
@@ -1398,8 +1400,7 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
                 int32_t ob = p.s2o[sm + b];
                 if (canon == 0.f) {
                     if (p.pflag[ob]) {
-                        if (SOLV) pert_share32(p, J, ob, o, fx, fy, fz);
-                        else pert_pair_direct(p, J, ob, o, false, VIR, v, fx, fy, fz);
+                        pert_pair_direct(p, J, ob, o, false, VIR, v, fx, fy, fz);
                         continue;
                     }
                 } else if (po) {
@@ -1564,11 +1565,15 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
 // RF self / excluded pairs (excluded.rs:25-112) and 1-4 pairs (excluded.rs:131-206),
 // one lane per original atom.  Partners below the owner contribute only their
 // force, evaluated in the reference's orientation.
-__global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
+// solv_energy = 0 (md steps): the solvent molecules' excluded RF energy is taken
+// by the integrator's group lanes (k_integrate), and solvent atoms have nothing
+// else here (no force, no 1-4, never perturbed).
+__global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad, int32_t solv_energy) {
     double v[ROW] = {0};
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int32_t jb = i < npad ? p.atom_job[i] : -1;
+    if (jb >= 0 && !solv_energy && i - p.jobs[jb].atom_off >= p.jobs[jb].n_solute_atoms) jb = -1;
     if (jb >= 0) {
         const JobDev& J = p.jobs[jb];
         const double* x = p.x_cur;
@@ -1652,7 +1657,7 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad) {
             }
         }
     }
-    if (i < npad) {
+    if (i < npad && (solv_energy || jb >= 0)) {
         p.f_x[3 * i] = fx;
         p.f_x[3 * i + 1] = fy;
         p.f_x[3 * i + 2] = fz;
@@ -1808,7 +1813,9 @@ __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
 // terms per group (k_ekin), and - when the next step keeps the pair list - the
 // sorted copies xq / xf it reads (k_gather).  A frozen job keeps its state.
 template <bool MIXED>
-__device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t with_long) {
+// with_x: add the excluded / 1-4 forces (solute atoms; a solvent atom has none in
+// md steps, where k_excl14 leaves its slot unwritten)
+__device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t with_long, bool with_x) {
     int32_t a = p.o2s[i];
     double f[3];
 #pragma unroll
@@ -1821,7 +1828,7 @@ __device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t w
             f[c] = p.f_s[3 * a + c];
             if (with_long) f[c] += p.f_long_s[3 * a + c];
         }
-        f[c] += p.f_x[3 * i + c];
+        if (with_x) f[c] += p.f_x[3 * i + c];
     }
     return mk(f[0], f[1], f[2]);
 }
@@ -1835,6 +1842,7 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
     const JobDev& J = p.jobs[jb];
     const int32_t s = p.cg_start[c], n = p.cg_n[c];
     const bool frozen = step >= J.stop;
+    const bool solute_group = c - J.cg_off < J.n_solute_cg;
     const double dt = J.dt;
     const double lam = J.thermo ? p.therm_scale[jb] : 1.0;
     double ek = 0.0, ekn = 0.0;
@@ -1846,7 +1854,7 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
             vn = vo;
             return;
         }
-        v3 f = total_force<MIXED>(p, i, with_long);
+        v3 f = total_force<MIXED>(p, i, with_long, solute_group);
         double im = p.inv_mass[i];
         if (J.thermo) vn = mk((vo.x + f.x * im * dt) * lam, (vo.y + f.y * im * dt) * lam, (vo.z + f.z * im * dt) * lam);
         else vn = mk(vo.x + f.x * im * dt, vo.y + f.y * im * dt, vo.z + f.z * im * dt);
@@ -1882,8 +1890,25 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
             finish_atom(s + k, xn, vo, vn, x_first);
         }
     }
-    p.cg_ek[2 * (int64_t)c] = ek;
-    p.cg_ek[2 * (int64_t)c + 1] = ekn;
+    // rf_excluded_interactions for a solvent molecule (excluded.rs:229-262): energy
+    // only, -q_i q_j fpepsi crf_2cut3i r^2 for its excluded pairs j > i at the
+    // step's starting positions
+    double ex = 0.0;
+    if (c - J.cg_off >= J.n_solute_cg)
+        for (int32_t i = s; i < s + n; ++i) {
+            const d3 pi = ld3(p.x_cur, i);
+            const double qi = p.q[i];
+            for (int32_t k = p.excl_off[i]; k < p.excl_off[i + 1]; ++k) {
+                const int32_t j = p.excl[k];
+                if (j <= i) continue;
+                const d3 r = nim(J.box, pi, ld3(p.x_cur, j));
+                const double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+                ex += -qi * p.q[j] * J.fpepsi * J.crf.crf_2cut3i * r2;
+            }
+        }
+    p.cg_ek[3 * (int64_t)c] = ek;
+    p.cg_ek[3 * (int64_t)c + 1] = ekn;
+    p.cg_ek[3 * (int64_t)c + 2] = ex;
 }
 
 
@@ -1902,8 +1927,9 @@ __global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_viri
         const double* r = p.vv_e + 11 * k;
         for (int c = 0; c < nc; ++c) v[c] += r[c];
         if (with_kin) {
-            v[11] += p.cg_ek[2 * k];
-            v[12] += p.cg_ek[2 * k + 1];
+            v[1] += p.cg_ek[3 * k + 2];   // solvent excluded RF energy (md steps)
+            v[11] += p.cg_ek[3 * k];
+            v[12] += p.cg_ek[3 * k + 1];
         }
     }
     block_sum<PART_W>(v, lds);
@@ -2141,8 +2167,8 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
         if (nsol) hipLaunchKernelGGL((k_gen_sol<false, false, false>), gs, dim3(BLOCK), 0, s, p, nsol, virial);
     }
 }
-void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s) {
-    hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad);
+void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, hipStream_t s) {
+    hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad, solv_energy);
 }
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed,
                      hipStream_t s) {

@@ -50,6 +50,8 @@ struct SysPtrs {
     const int4* vv_chunk;      // nvv_chunk: solvent pair kernel chunks (first sorted group, end, job, 0)
     int32_t nvv_chunk;
     int32_t* step;             // md step index within the current md call (device counter)
+    const float4* prec;        // 2 per solute atom: (a_q, b_q, alpha_lj, alpha_crf), (a/b soft crf_cut3i, a/b iac bits)
+    int32_t* s_prec;           // sorted atom -> its prec record (perturbed atoms), -1 otherwise
     uint32_t* step_done;       // blocks of k_reduce finished (self-resetting)
     const int64_t* vv_off;
     const int64_t* gen_off;
@@ -76,7 +78,7 @@ struct SysPtrs {
     double* rows;
     double* vv_e;              // per sorted group: solvent-solvent e_lj, e_crf, virial[9]
     double* vv_part;           // per job: VV_PARTS partial sums of vv_e and cg_ek
-    double* cg_ek;             // per original group: E_kin, E_kin_new of its atoms
+    double* cg_ek;             // per original group: E_kin, E_kin_new of its atoms, solvent excluded RF energy
     const int32_t* sol_atom;   // nsol: original index of every solute atom, job by job
     int32_t nsol;
     double* sol_e;             // 2 x nsol x ROW: per solute atom, short then long pairs
@@ -114,7 +116,9 @@ constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
-void launch_excl14(const SysPtrs& p, int32_t npad, hipStream_t s);
+// solv_energy: also the solvent molecules' excluded RF energy (single points;
+// md steps take it in the integrator)
+void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group

@@ -42,11 +42,14 @@ def parse():
     ap.add_argument("--fep-alone-steps", type=int, default=200,
                     help="steps per job timed for the one-job-at-a-time plain FP64 comparison")
     ap.add_argument("--remd-steps", type=int, default=1000, help="steps of the 128-replica exchange run")
-    ap.add_argument("--fep-admit-interval", type=int, default=50,
-                    help="steps between batch layout changes of the FEP server (a multiple of 5)")
+    ap.add_argument("--fep-admit-interval", type=int, default=200,
+                    help="steps between batch layout changes of the FEP server (a multiple of 5); the mixed "
+                         "server also runs at 50 for the latency side of the trade-off")
     ap.add_argument("--no-fep", action="store_true")
     ap.add_argument("--no-remd", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=2000, help="NVE steps of the equilibrated drift comparison")
+    ap.add_argument("--drift-replicas", type=int, default=8,
+                    help="independent copies in the ensemble drift comparison (0: skip)")
     return ap.parse_args()
 
 
@@ -218,6 +221,8 @@ def main():
                                          "steps": a.steps, "atoms": s.n_atoms, "start": "lattice"}
         if a.drift_steps > 0:
             out["energy_drift_equilibrated"] = drift_bench(rt, gen, s, a.drift_steps)
+        if a.drift_replicas > 0:
+            out["energy_drift_ensemble"] = drift_ensemble_bench(rt, gen, a.drift_steps, a.drift_replicas)
         out["mixed_force_error_vs_fp64"] = force_error(rt, s)
     out["energy_first_last"] = [h["e_first"], h["e_last"]]
     if not a.no_fep:
@@ -229,7 +234,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
+def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
     """The 64-job FEP lambda-window mix: seeded Poisson arrivals, continuous
     batching (mixed precision, then plain FP64), against the same jobs run one at
     a time on the plain FP64 single-job path."""
@@ -247,8 +252,11 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
                arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5, thermostat="Berendsen 300 K, tau 0.1 ps",
                admit_interval=admit_interval,
                arrivals_last_step=int(max(m["arrival"] for m in mix)))
-    for name, prec in (("batched_mixed", rt.MIXED), ("batched_fp64", rt.FP64)):
-        server = srv.Server(precision=prec, nsnb=5, admit_interval=admit_interval)
+    runs = [("batched_mixed", rt.MIXED, admit_interval), ("batched_fp64", rt.FP64, admit_interval)]
+    if admit_interval != 50:
+        runs.append(("batched_mixed_admit50", rt.MIXED, 50))
+    for name, prec, ai in runs:
+        server = srv.Server(precision=prec, nsnb=5, admit_interval=ai)
         r = srv.replay(server, sched)
         sim_ns = len(mix) * steps * 0.002 * 1e-3
         out[name] = dict(pairs_per_s=r["pairs"] / r["wall"], wall_s=r["wall"], ns_per_day=sim_ns * 86400.0 / r["wall"],
@@ -256,7 +264,7 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
                          step_ms_p99=float(np.percentile(r["step_ms"], 99)),
                          job_latency_s_p50=float(np.percentile(r["job_latency_s"], 50)),
                          job_latency_s_p99=float(np.percentile(r["job_latency_s"], 99)),
-                         layouts=r["layouts"], layout_ms_total=r["layout_ms"],
+                         admit_interval=ai, layouts=r["layouts"], layout_ms_total=r["layout_ms"],
                          layout_concat_ms_total=r["layout_concat_ms"], session_steps=r["clock"],
                          failed_jobs=sum(1 for v in server.results.values() if v["failed"]))
         server.sess.close()
@@ -276,6 +284,9 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=25):
     out["one_at_a_time_fp64"] = dict(pairs_per_s=pairs / wall, steps_per_job_timed=alone_steps, wall_s=wall)
     out["speedup_batched_mixed_vs_one_at_a_time_fp64"] = out["batched_mixed"]["pairs_per_s"] / (pairs / wall)
     out["speedup_batched_fp64_vs_one_at_a_time_fp64"] = out["batched_fp64"]["pairs_per_s"] / (pairs / wall)
+    if "batched_mixed_admit50" in out:
+        out["speedup_batched_mixed_admit50_vs_one_at_a_time_fp64"] = (out["batched_mixed_admit50"]["pairs_per_s"] /
+                                                                      (pairs / wall))
     return out
 
 
@@ -338,6 +349,43 @@ def drift_bench(rt, gen, s, steps):
         out[name + "_per_atom"] = out[name] / eq.n_atoms
     out["rel_diff"] = abs(out["mixed"] - out["fp64"]) / abs(out["fp64"])
     return out
+
+
+def drift_ensemble_bench(rt, gen, steps, replicas):
+    """Drift over an ensemble: `replicas` copies of the 32,768-water system
+    (lattice seeds 2, 3, ...) relaxed together (300 Berendsen steps, tau 0.01 ps,
+    300 K), then NVE for `steps` steps, all replicas batched in one session, in
+    FP64 and in mixed precision from the same states.  A single trajectory's
+    slope carries sampling noise (trajectories from nearby states separate
+    within a few hundred steps), so the comparison is the ensemble mean with its
+    standard error, and per replica the paired difference mixed - FP64."""
+    import numpy as np
+    srv = importlib.import_module("gromos-rs_amd.server")
+    systems = [gen.water_32768(seed=2 + k) for k in range(replicas)]
+    eq = srv.equilibrate(systems, [300.0] * replicas)
+    slopes = {}
+    for name, prec in (("fp64", rt.FP64), ("mixed", rt.MIXED)):
+        sess = rt.Session(precision=prec, nsnb=5)
+        jobs = [sess.admit(q, steps) for q in eq]
+        sess.step(steps)
+        sl = []
+        for j, q in zip(jobs, eq):
+            st, done = sess.query(j)
+            if st != rt.DONE or done != steps:
+                raise rt.GromosAmdError(f"drift replica {j} ended in state {st} after {done} steps")
+            e = sess.energies(j, 0, steps)
+            sl.append(float(np.polyfit(np.arange(steps) * q.dt, e[:, 2] + e[:, 3], 1)[0]))
+        sess.close()
+        slopes[name] = np.array(sl)
+    f, m = slopes["fp64"], slopes["mixed"]
+    sem = lambda v: float(v.std(ddof=1) / np.sqrt(len(v))) if len(v) > 1 else float("nan")
+    paired = (m - f) / np.abs(f)
+    return {"replicas": replicas, "steps": steps, "atoms_per_replica": eq[0].n_atoms,
+            "start": "lattice seeds 2..%d relaxed 300 steps under Berendsen tau 0.01 ps" % (1 + replicas),
+            "fp64_mean": float(f.mean()), "fp64_sem": sem(f), "mixed_mean": float(m.mean()), "mixed_sem": sem(m),
+            "rel_diff_of_means": float(abs(m.mean() - f.mean()) / abs(f.mean())),
+            "paired_rel_diff_mean": float(paired.mean()), "paired_rel_diff_sem": sem(paired),
+            "fp64": f.tolist(), "mixed": m.tolist()}
 
 
 def ds_vv_pairs(ds, s):

@@ -129,7 +129,9 @@ struct gromos_amd_system {
     int64_t *d_vv_cnt = nullptr, *d_gen_cnt = nullptr, *d_vv_off = nullptr, *d_gen_off = nullptr;
     uint32_t *d_vv_ent = nullptr, *d_gen_ent = nullptr;
     uint64_t* d_counters = nullptr;
+    int32_t* d_gsol = nullptr;
     int64_t *d_pc_short = nullptr, *d_pc_long = nullptr;
+    int64_t* d_job_pairs = nullptr;   // 2 per job: short, long pairs of the last build
     int64_t *d_vv_cap = nullptr, *d_gen_cap = nullptr;
     int32_t *d_prev_vv = nullptr, *d_prev_gen = nullptr, *d_overflow = nullptr;
     bool have_prev = false;
@@ -143,6 +145,9 @@ struct gromos_amd_system {
     double *d_f_s = nullptr, *d_f_long_s = nullptr, *d_f_x = nullptr;
     double *d_rows = nullptr, *d_records = nullptr, *d_vv_e = nullptr, *d_vv_part = nullptr, *d_sol_e = nullptr;
     int32_t* d_sol_atom = nullptr;
+    int32_t* d_excl_blocks = nullptr;   // atom blocks holding solute atoms (md-step k_excl14 grid)
+    int32_t n_excl_blocks = 0;
+    std::vector<int32_t> h_excl_blocks;   // host copy (the async upload's source)
     int32_t nsol = 0;
     int4* d_vv_chunk = nullptr;   // solvent pair kernel chunks (first sorted group, end, job, 0)
     int32_t* d_step = nullptr;    // md step index within the current md call
@@ -154,7 +159,7 @@ struct gromos_amd_system {
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         gamd::SysPtrs key{};
-        int64_t shape[10] = {};
+        int64_t shape[11] = {};
     };
     StepGraph graphs[2];
     bool use_graphs = true;
@@ -187,7 +192,9 @@ struct gromos_amd_system {
     bool twin = false, long_computed = false, has_pert = false;
     int32_t use_settle = 0, do_virial = 0;
     int64_t pair_count = 0;
-    uint64_t host_counters[4] = {0, 0, 0, 0};
+    uint64_t host_counters[5] = {0, 0, 0, 0, 0};
+    std::vector<int64_t> host_job_pairs;   // d_job_pairs of the last build
+    int32_t md_step = 0;                   // step index within the current md call
     // timing of the dominant kernel
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

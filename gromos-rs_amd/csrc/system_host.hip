@@ -239,6 +239,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         if (im->P.nsnb != I0.P.nsnb) return fail(GROMOS_AMD_EINVAL, "jobs of one batch share the rebuild cadence");
     S->precision = precision;
     S->njobs = njobs;
+    S->host_job_pairs.assign(2 * (size_t)njobs, 0);
     S->jobs.clear();
     S->job_n_atoms.clear();
     const auto t_concat = std::chrono::steady_clock::now();
@@ -510,6 +511,17 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_pap, pap);
     rc = rc ? rc : S->upload(S->d_vv_chunk, vv_chunk);
     rc = rc ? rc : S->upload(S->d_prec, prec);
+    {
+        // the atom blocks of BLOCK atoms that hold solute atoms: in md steps the
+        // excluded / 1-4 / self terms of solvent atoms are taken elsewhere
+        std::vector<int32_t>& eb = S->h_excl_blocks;
+        eb.clear();
+        for (const JobDev& J : S->jobs)
+            for (int32_t b = J.atom_off / BLOCK; b < (J.atom_off + J.n_solute_atoms + BLOCK - 1) / BLOCK; ++b)
+                eb.push_back(b);
+        S->n_excl_blocks = (int32_t)eb.size();
+        rc = rc ? rc : S->upload(S->d_excl_blocks, eb);
+    }
     if (rc) return rc;
     size_t n3 = (size_t)npad * 3, nc = (size_t)ncg, ncl = (size_t)S->ncell + 1, npa = (size_t)npad;
     for (int k = 0; k < 2; ++k) {
@@ -528,7 +540,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_s_atom_cg, npa, true)) || (rc = S->reserve(S->d_o2s, npa, true)) ||
         (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_s_prec, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
-        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 4)) ||
+        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 5)) ||
+        (rc = S->reserve(S->d_gsol, nc, true)) ||
         (rc = S->reserve(S->d_pc_short, nc, true)) || (rc = S->reserve(S->d_pc_long, nc, true)) ||
         (rc = S->reserve(S->d_xq, npa, true)) || (rc = S->reserve(S->d_vv_cap, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cap, nc + 1, true)) || (rc = S->reserve(S->d_prev_vv, nc, true)) ||
@@ -543,7 +556,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
-        (rc = S->reserve(S->d_step_done, 1)) ||
+        (rc = S->reserve(S->d_step_done, 1)) || (rc = S->reserve(S->d_job_pairs, 2 * (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
@@ -615,7 +628,7 @@ int build_pairlist(gromos_amd_system* S) {
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
     HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, (size_t)S->npad * sizeof(int32_t), st));
     launch_sorted_atoms(p, ncg, st);
-    HIPCHK(hipMemsetAsync(S->d_counters, 0, 4 * sizeof(uint64_t), st));
+    HIPCHK(hipMemsetAsync(S->d_counters, 0, 5 * sizeof(uint64_t), st));
     HIPCHK(hipMemsetAsync(S->d_overflow, 0, sizeof(int32_t), st));
     int32_t overflow = 0;
     bool need_exact = !S->have_prev;
@@ -628,8 +641,11 @@ int build_pairlist(gromos_amd_system* S) {
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
         launch_nbl(p, ncg, S->ncell, true, st);
+        launch_job_pairs(p, S->njobs, st);
         HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (overflow) {
             S->refills++;
@@ -637,20 +653,25 @@ int build_pairlist(gromos_amd_system* S) {
         }
     } else {
         launch_nbl(p, ncg, S->ncell, false, st);
+        launch_job_pairs(p, S->njobs, st);
+        HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, st));
     }
     if (need_exact) {
         if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
         int64_t tot[2];
         HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
         if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
         p = S->ptrs();
         launch_caps(p, ncg, 1, st);
-        uint64_t keep[4];
+        uint64_t keep[5];
         std::memcpy(keep, S->host_counters, sizeof(keep));
+        // the refill lists the solvent groups with general partners again
+        HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
         launch_nbl(p, ncg, S->ncell, true, st);
         std::memcpy(S->host_counters, keep, sizeof(keep));
     }
@@ -745,6 +766,19 @@ static int32_t solvent_aps(const gromos_amd_system* S) {
     return S->jobs[0].aps > 0 ? S->jobs[0].aps : 3;
 }
 
+// pair interactions of one step (which = 0: short range, 1: long range), only
+// over the jobs that advance in it: a job past its stop step is held until the
+// next batch change and its pairs are not evaluated
+static int64_t active_pairs(const gromos_amd_system* S, int which) {
+    bool all = true;
+    for (const JobDev& J : S->jobs) all = all && S->md_step < J.stop;
+    if (all) return (int64_t)S->host_counters[which];
+    int64_t n = 0;
+    for (int32_t b = 0; b < S->njobs; ++b)
+        if (S->md_step < S->jobs[b].stop) n += S->host_job_pairs[2 * (size_t)b + which];
+    return n;
+}
+
 // Forcefield::apply nonbonded part (forcefield.rs:226-1060); energy partials
 // into the rows and records.  assemble: total forces into d_f (the md step
 // assembles them itself).
@@ -774,14 +808,15 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
     launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
-    launch_gen(p, npad, false, S->do_virial, mixed, st);
-    S->pair_count += (int64_t)S->host_counters[0];
+    launch_gen(p, npad, false, S->do_virial, mixed, (int32_t)S->host_counters[4], st);
+    S->pair_count += active_pairs(S, 0);
     if (S->twin && (rebuild || !S->long_computed)) {
-        launch_gen(p, npad, true, 1, mixed, st);
+        launch_gen(p, npad, true, 1, mixed, (int32_t)S->host_counters[4], st);
         S->long_computed = true;
-        S->pair_count += (int64_t)S->host_counters[1];
+        S->pair_count += active_pairs(S, 1);
     }
-    launch_excl14(p, npad, assemble ? 1 : 0, st);
+    if (assemble) launch_excl14(p, npad, 1, nullptr, 0, st);
+    else launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
     if (assemble) launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
     HIPCHK(hipGetLastError());
     return 0;
@@ -798,8 +833,8 @@ static int launch_step_graph(gromos_amd_system* S) {
     const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
     const int32_t aps = solvent_aps(S);
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
-    const int64_t shape[10] = {S->ncg, S->npad, S->njobs, S->nsol, S->vv_lanes, form, S->do_virial, S->twin ? 1 : 0,
-                               mixed, aps};
+    const int64_t shape[11] = {S->ncg, S->npad, S->njobs, S->nsol, S->vv_lanes, form, S->do_virial, S->twin ? 1 : 0,
+                               mixed, aps, (int64_t)S->host_counters[4]};
     auto& g = S->graphs[S->cur];
     if (!g.exec || std::memcmp(&g.key, &p, sizeof p) != 0 || std::memcmp(g.shape, shape, sizeof shape) != 0) {
         if (g.exec) HIPCHK(hipGraphExecDestroy(g.exec));
@@ -807,8 +842,8 @@ static int launch_step_graph(gromos_amd_system* S) {
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st);
-        launch_gen(p, S->npad, false, S->do_virial, mixed, st);
-        launch_excl14(p, S->npad, 0, st);
+        launch_gen(p, S->npad, false, S->do_virial, mixed, (int32_t)S->host_counters[4], st);
+        launch_excl14(p, S->npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, 1, mixed, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
         hipError_t e = hipStreamEndCapture(st, &graph);
@@ -859,13 +894,15 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
                            S->vv_lanes > 0;
         if (plain) {
             S->steps_since_update++;
-            S->pair_count += (int64_t)S->host_counters[0];
+            S->md_step = s;
+            S->pair_count += active_pairs(S, 0);
             if (int rc = launch_step_graph(S)) return rc;
             S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
             if (S->step_timing)
                 if (int rc = step_event()) return rc;
             continue;
         }
+        S->md_step = s;
         if (int rc = forcefield(S, false)) return rc;
         SysPtrs p = S->ptrs();
         if (S->any_thermo && S->therm_dirty) {
@@ -882,6 +919,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         if (S->step_timing)
             if (int rc = step_event()) return rc;
     }
+    S->md_step = 0;
     HIPCHK(hipGetLastError());
     std::vector<int32_t> failed(S->njobs);
     HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -998,8 +1036,10 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.vv_ent = d_vv_ent;
     p.gen_ent = d_gen_ent;
     p.counters = d_counters;
+    p.gsol = d_gsol;
     p.pc_short = d_pc_short;
     p.pc_long = d_pc_long;
+    p.job_pairs = d_job_pairs;
     p.vv_cap = d_vv_cap;
     p.gen_cap = d_gen_cap;
     p.prev_vv = d_prev_vv;

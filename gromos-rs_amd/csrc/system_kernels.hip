@@ -316,6 +316,18 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
             p.pc_long[k] = 0;
             if (FILL && (nvv > vv_end - vv_base || ngen > gen_end - gen_base)) atomicOr(p.overflow, 1);
         }
+        // solvent groups with general-list entries (solute partners, long pairs)
+        // for the mixed solvent pass, in any order: each is evaluated on its own
+        const bool gs = own && ck - cg_off >= nsol && gen_pos > gen_base;
+        const uint64_t gm = __ballot(gs);
+        if (gm) {
+            const int32_t t0 = (int32_t)__builtin_ctzll(gm);
+            int32_t base = 0;
+            if (lane == t0)
+                base = (int32_t)atomicAdd((unsigned long long*)p.counters + 4, (unsigned long long)__popcll(gm));
+            base = bcast(base, t0);
+            if (gs) p.gsol[base + (int32_t)__popcll(gm & below)] = k;
+        }
     }
     // per-cell pair counts; totals come from k_totals (one atomic per block
     // there: thousands of blocks adding to the same address serialise at the L2)
@@ -357,6 +369,22 @@ __global__ void __launch_bounds__(BLOCK) k_totals(SysPtrs p, int32_t ncg) {
         unsigned long long t = (unsigned long long)lds[16 * 4 + threadIdx.x];
         if (t) atomicAdd((unsigned long long*)p.counters + threadIdx.x, t);
     }
+}
+
+// per-job pair interactions of this build (short, long): block j sums job j's
+// sorted groups, which are contiguous from cg_off, so a step can count only
+// the jobs that advance in it
+__global__ void __launch_bounds__(BLOCK) k_job_pairs(SysPtrs p) {
+    __shared__ double lds[17 * 2];
+    const JobDev& J = p.jobs[blockIdx.x];
+    const int32_t k0 = J.cg_off, k1 = J.cg_off + J.n_cg;
+    double v[2] = {0.0, 0.0};   // exact: each sum stays below 2^53
+    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += BLOCK) {
+        v[0] += (double)p.pc_short[k];
+        v[1] += (double)p.pc_long[k];
+    }
+    block_sum<2>(v, lds);
+    if (threadIdx.x < 2) p.job_pairs[2 * blockIdx.x + threadIdx.x] = (int64_t)lds[16 * 2 + threadIdx.x];
 }
 
 // capacities for a single-pass fill: the previous build's count of the same
@@ -418,11 +446,14 @@ __device__ __forceinline__ void add_virial(double (&v)[ROW], d3 r, double fx, do
     v[8] += r.z * fx; v[9] += r.z * fy; v[10] += r.z * fz;
 }
 
-__device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int slot, int nblocks) {
+__device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int slot, int nblocks, int32_t blk) {
     __shared__ double lds[17 * ROW];
     block_sum<ROW>(v, lds);
     if (threadIdx.x < ROW)
-        rows[((int64_t)slot * nblocks + blockIdx.x) * ROW + threadIdx.x] = lds[16 * ROW + threadIdx.x];
+        rows[((int64_t)slot * nblocks + blk) * ROW + threadIdx.x] = lds[16 * ROW + threadIdx.x];
+}
+__device__ __forceinline__ void write_row(double (&v)[ROW], double* rows, int slot, int nblocks) {
+    write_row(v, rows, slot, nblocks, (int32_t)blockIdx.x);
 }
 
 // ---- mixed precision: FP32 pair arithmetic, FP64 per-lane partial sums,
@@ -483,6 +514,10 @@ __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, floa
 // record (vv_e, 11 doubles) that k_reduce sums per job.  NCHM (lanes per
 // molecule: 4, 8 or 16) is chosen per batch shape by timing (autotune).
 
+// A job held past its stop step in the current md call (it leaves at the next
+// batch change; the integrator keeps its state): its pair kernels skip it.
+__device__ __forceinline__ bool held(const SysPtrs& p, int32_t job) { return *p.step >= p.jobs[job].stop; }
+
 // Wave -> groups of the solvent pair kernels: the batch's sorted groups in
 // chunks of VV_CHUNK that never straddle two jobs (vv_chunk = first group, end,
 // job); a wave holds WAVE / NCHM consecutive groups of one chunk, so the job's
@@ -518,6 +553,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
     bool valid = false;
     int chunk = 0;
     if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
+    if (held(p, job)) return;
     double f[APS][3];
     double en[11];
 #pragma unroll
@@ -638,6 +674,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     bool valid = false;
     int chunk = 0;
     if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
+    if (held(p, job)) return;
     long long fi[APS][3];
     long long ei[11];
 #pragma unroll
@@ -1302,7 +1339,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen(SysPtrs p, int32_t npad, int32_t 
     int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t o = a < npad ? p.s2o[a] : -1;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (o >= 0) {
+    if (o >= 0 && !held(p, p.atom_job[o])) {
         const JobDev& J = p.jobs[p.atom_job[o]];
         if (o - J.atom_off >= J.n_solute_atoms) gen_atom64<LONG, true>(p, a, o, 0, 1, virial, v, fx, fy, fz);
     }
@@ -1474,7 +1511,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t o = a < npad ? p.s2o[a] : -1;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (o >= 0) {
+    if (o >= 0 && !held(p, p.atom_job[o])) {
         const JobDev& J = p.jobs[p.atom_job[o]];
         if (o - J.atom_off >= J.n_solute_atoms) gen_atom32<LONG, VIR, true>(p, a, o, 0, 1, v, fx, fy, fz);
     }
@@ -1493,6 +1530,41 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
     write_row(v, p.rows, LONG ? SLOT_LONG : SLOT_GEN, p.nrow_blocks);
 }
 
+// Solvent atoms' short general pairs in mixed precision, only for the solvent
+// groups listed by the pair search (gsol): GSV_SLOT lanes per group (one group
+// per slot, GSV_LANES lanes per atom taking every GSV_LANES-th entry), FP64 lane
+// sums combined by a fixed butterfly and added to the atom's fixed-point force.
+// A solvent group's short general entries are its solute partners (solvent-
+// solvent short pairs are all in the solvent list), and the solute atom comes
+// first in the topology, so those pairs' energies and virial belong to the
+// solute atom's block (k_gen_sol): these lanes add none.
+template <int GSV_LANES>
+__global__ void __launch_bounds__(BLOCK) k_gen_sv(SysPtrs p, int32_t ngsol) {
+    constexpr int GSV_SLOT = 4 * GSV_LANES;   // solvent molecules have at most 4 atoms
+    const int32_t w = ((int32_t)blockIdx.x * BLOCK + (int32_t)threadIdx.x) / GSV_SLOT;
+    const int32_t r = (int32_t)threadIdx.x % GSV_SLOT, i = r / GSV_LANES, c = r % GSV_LANES;
+    double v[ROW] = {0};
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    int32_t a = -1;
+    if (w < ngsol) {
+        const int32_t k = p.gsol[w];
+        const int32_t sk = p.s_cg_start[k];
+        if (i < p.s_cg_n[k] && !held(p, p.atom_job[p.s2o[sk]])) a = sk + i;
+    }
+    if (a >= 0) gen_atom32<false, false, true>(p, a, p.s2o[a], c, GSV_LANES, v, fx, fy, fz);
+#pragma unroll
+    for (int m = 1; m < GSV_LANES; m <<= 1) {
+        fx += __shfl_xor(fx, m, WAVE);
+        fy += __shfl_xor(fy, m, WAVE);
+        fz += __shfl_xor(fz, m, WAVE);
+    }
+    if (c == 0 && a >= 0) {
+        p.fi_s[3 * a] += to_fx(fx);
+        p.fi_s[3 * a + 1] += to_fx(fy);
+        p.fi_s[3 * a + 2] += to_fx(fz);
+    }
+}
+
 // Solute atoms: one block (four waves) per atom (global solute index w, original
 // atom sol_atom[w]), the 256 lanes taking every 256th entry; forces and energies
 // summed over each wave with a fixed xor butterfly and over the waves in order.
@@ -1507,6 +1579,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
     double v[ROW] = {0};
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int32_t o = p.sol_atom[w];
+    if (held(p, p.atom_job[o])) return;   // block-uniform, before any barrier
     const int32_t a = p.o2s[o];
     if (MIXED) gen_atom32<LONG, VIR, false>(p, a, o, threadIdx.x, BLOCK, v, fx, fy, fz);
     else gen_atom64<LONG, false>(p, a, o, threadIdx.x, BLOCK, virial, v, fx, fy, fz);
@@ -1568,12 +1641,15 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
 // solv_energy = 0 (md steps): the solvent molecules' excluded RF energy is taken
 // by the integrator's group lanes (k_integrate), and solvent atoms have nothing
 // else here (no force, no 1-4, never perturbed).
-__global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad, int32_t solv_energy) {
+__global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad, int32_t solv_energy,
+                                                  const int32_t* blocks) {
     double v[ROW] = {0};
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t blk = blocks ? blocks[blockIdx.x] : (int32_t)blockIdx.x;
+    int32_t i = blk * BLOCK + (int32_t)threadIdx.x;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int32_t jb = i < npad ? p.atom_job[i] : -1;
     if (jb >= 0 && !solv_energy && i - p.jobs[jb].atom_off >= p.jobs[jb].n_solute_atoms) jb = -1;
+    if (jb >= 0 && held(p, jb)) jb = -1;
     if (jb >= 0) {
         const JobDev& J = p.jobs[jb];
         const double* x = p.x_cur;
@@ -1662,7 +1738,7 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad, int32
         p.f_x[3 * i + 1] = fy;
         p.f_x[3 * i + 2] = fz;
     }
-    write_row(v, p.rows, SLOT_EXCL, p.nrow_blocks);
+    write_row(v, p.rows, SLOT_EXCL, p.nrow_blocks, blk);
 }
 
 // total force on each original atom
@@ -1979,7 +2055,8 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         if (s == SLOT_LONG && !with_long) continue;
         const int32_t per = BLOCK;
         int32_t b0 = J.atom_off / per;
-        int32_t b1 = (J.atom_off + J.n_atoms + per - 1) / per;
+        // md steps write the excluded-pair rows of the solute atoms' blocks only
+        int32_t b1 = (J.atom_off + (s == SLOT_EXCL && advance ? J.n_solute_atoms : J.n_atoms) + per - 1) / per;
         for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
             const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
 #pragma unroll
@@ -2091,6 +2168,9 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStre
     unsigned nb = (unsigned)std::min<int64_t>(256, (ncg + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_totals, dim3(nb), dim3(BLOCK), 0, s, p, ncg);
 }
+void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_job_pairs, dim3(njobs), dim3(BLOCK), 0, s, p);
+}
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
 }
@@ -2141,7 +2221,8 @@ void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t
         default: launch_vv_aps<3, false>(p, ncg, virial, mixed, neutral, lanes, e0, e1, s); break;
     }
 }
-void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s) {
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, int32_t ngsol,
+                hipStream_t s) {
     // solvent atoms a lane each, then solute atoms a wave each (after: the long
     // pass of k_gen writes zeros for the solute atoms)
     const int32_t nsol = p.nsol;
@@ -2150,12 +2231,15 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
         if (lng) {
             hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
             if (nsol) hipLaunchKernelGGL((k_gen_sol<true, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
-        } else if (virial) {
-            hipLaunchKernelGGL((k_gen_mx<false, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-            if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
         } else {
-            hipLaunchKernelGGL((k_gen_mx<false, false>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-            if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, false>), gs, dim3(BLOCK), 0, s, p, nsol, 0);
+            // the short pass of the solvent atoms: only the listed groups
+            // two lanes per atom: 46.6 us per step on the 64-job FEP mix, against
+            // 55.4 (one lane), 53.4 (four) and 132.5 (sixteen)
+            constexpr int32_t GPB = BLOCK / (4 * 2);   // groups per block
+            if (ngsol > 0)
+                hipLaunchKernelGGL(k_gen_sv<2>, dim3((unsigned)((ngsol + GPB - 1) / GPB)), dim3(BLOCK), 0, s, p, ngsol);
+            if (nsol && virial) hipLaunchKernelGGL((k_gen_sol<false, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+            else if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, false>), gs, dim3(BLOCK), 0, s, p, nsol, 0);
         }
         return;
     }
@@ -2167,8 +2251,11 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
         if (nsol) hipLaunchKernelGGL((k_gen_sol<false, false, false>), gs, dim3(BLOCK), 0, s, p, nsol, virial);
     }
 }
-void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, hipStream_t s) {
-    hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad, solv_energy);
+void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, const int32_t* blocks, int32_t nblocks,
+                   hipStream_t s) {
+    if (!blocks) hipLaunchKernelGGL(k_excl14, grid_for(npad), dim3(BLOCK), 0, s, p, npad, solv_energy, blocks);
+    else if (nblocks > 0)
+        hipLaunchKernelGGL(k_excl14, dim3((unsigned)nblocks), dim3(BLOCK), 0, s, p, npad, solv_energy, blocks);
 }
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed,
                      hipStream_t s) {

@@ -58,9 +58,11 @@ struct SysPtrs {
     uint32_t* vv_ent;
     uint32_t* gen_ent;
     uint64_t* counters;        // [0] short pairs per step, [1] long pairs per rebuild,
-                               // [2] vv entries, [3] gen entries
+                               // [2] vv entries, [3] gen entries, [4] groups in gsol
+    int32_t* gsol;             // sorted solvent groups with general-list entries (any order)
     int64_t* pc_short;         // per sorted group: canonical short / long pair interactions
     int64_t* pc_long;
+    int64_t* job_pairs;        // per job: short, long pair interactions of this build
     int64_t* vv_cap;           // single-pass fill: entries reserved per sorted group
     int64_t* gen_cap;
     int32_t* prev_vv;          // previous build's counts per original group
@@ -106,6 +108,7 @@ void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
+void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
@@ -115,10 +118,14 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, hipStream_t s);
+// ngsol: solvent groups listed in gsol by the last pair search (mixed short pass)
+void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, int32_t ngsol,
+                hipStream_t s);
 // solv_energy: also the solvent molecules' excluded RF energy (single points;
 // md steps take it in the integrator)
-void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, hipStream_t s);
+// blocks (md steps, solv_energy 0): the nblocks atom blocks holding solute atoms
+void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, const int32_t* blocks, int32_t nblocks,
+                   hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group

@@ -19,10 +19,13 @@ gen = importlib.import_module("gromos-rs_amd.generators")
 pytestmark = pytest.mark.gpu
 
 
-def alone(s, nsteps, prec):
+def alone(s, nsteps, prec, pairs=None):
     ds = rt.DeviceSystem(s, precision=prec)
     ds.pairlist_update()
+    p0 = ds.pair_count
     e = ds.md(nsteps)
+    if pairs is not None:
+        pairs.append(ds.pair_count - p0)
     x, v = ds.get_state()
     ds.close()
     return e, x, v
@@ -42,7 +45,8 @@ def job_specs():
 @pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
 def test_session_jobs_match_alone(prec):
     specs = job_specs()
-    ref = [alone(s, n, prec) for s, n, _ in specs]
+    pairs = []
+    ref = [alone(s, n, prec, pairs) for s, n, _ in specs]
     sess = rt.Session(precision=prec, nsnb=5)
     ids = [None] * len(specs)
     chunks = [3, 4, 6, 1, 7, 2, 5, 8, 3, 11, 9, 4]
@@ -59,6 +63,9 @@ def test_session_jobs_match_alone(prec):
         assert k < 200
     info = sess.info()
     assert info["layouts"] >= 3
+    # a job that reached its step count waits for the next batch change without
+    # being evaluated: the session counts exactly the jobs' own pair interactions
+    assert info["pair_count"] == sum(pairs), (info["pair_count"], pairs)
     for i, (s, n, _) in enumerate(specs):
         st, done = sess.query(ids[i])
         assert st == rt.DONE and done == n

@@ -113,6 +113,11 @@ struct gromos_amd_system {
     std::vector<int32_t> job_n_atoms;
     int64_t bytes = 0;
     hipStream_t stream = nullptr;
+    // the mixed md step's solute pass (k_gen_sol, k_excl14) runs on `side`,
+    // concurrently with the solvent pair kernel on `stream` (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool no_side = false;   // GROMOS_AMD_SIDE_STREAM=0: everything on `stream`
 
     // device arrays
     gamd::JobDev* d_jobs = nullptr;

@@ -226,6 +226,11 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
 int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs, int32_t precision) {
     if (imgs.empty()) return fail(GROMOS_AMD_EINVAL, "layout: no jobs");
     if (!S->stream) HIPCHK(hipStreamCreate(&S->stream));
+    if (!S->side) {
+        HIPCHK(hipStreamCreate(&S->side));
+        HIPCHK(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming));
+    }
     HIPCHK(hipStreamSynchronize(S->stream));   // host vectors below feed async copies
     int njobs = (int)imgs.size();
     const JobImage& I0 = *imgs[0];
@@ -641,11 +646,14 @@ int build_pairlist(gromos_amd_system* S) {
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
         launch_nbl(p, ncg, S->ncell, true, st);
-        launch_job_pairs(p, S->njobs, st);
+        if (S->njobs > 1) {
+            HIPCHK(hipMemsetAsync(S->d_job_pairs, 0, 2 * (size_t)S->njobs * sizeof(int64_t), st));
+            launch_job_pairs(p, S->njobs, st);
+            HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, st));
+        }
         HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
-                              hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (overflow) {
             S->refills++;
@@ -653,9 +661,12 @@ int build_pairlist(gromos_amd_system* S) {
         }
     } else {
         launch_nbl(p, ncg, S->ncell, false, st);
-        launch_job_pairs(p, S->njobs, st);
-        HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
-                              hipMemcpyDeviceToHost, st));
+        if (S->njobs > 1) {
+            HIPCHK(hipMemsetAsync(S->d_job_pairs, 0, 2 * (size_t)S->njobs * sizeof(int64_t), st));
+            launch_job_pairs(p, S->njobs, st);
+            HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, st));
+        }
     }
     if (need_exact) {
         if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
@@ -677,6 +688,10 @@ int build_pairlist(gromos_amd_system* S) {
     }
     launch_caps(p, ncg, 2, st);
     HIPCHK(hipGetLastError());
+    if (S->njobs == 1) {   // one job: its counts are the totals
+        S->host_job_pairs[0] = (int64_t)S->host_counters[0];
+        S->host_job_pairs[1] = (int64_t)S->host_counters[1];
+    }
     S->prev_tot_vv = (int64_t)S->host_counters[2];
     S->prev_tot_gen = (int64_t)S->host_counters[3];
     S->have_prev = true;
@@ -807,17 +822,30 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     const int32_t aps = solvent_aps(S);
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
+    const int32_t ngsol = (int32_t)S->host_counters[4];
+    // mixed md step: the solute atoms' pass (general pairs, then excluded / 1-4
+    // terms) is independent of the solvent pair kernel (separate outputs, inputs
+    // ready at the fork) and latency bound, so it overlaps it on the side stream
+    const bool split = mixed && !assemble && S->nsol > 0 && !S->no_side;
+    if (split) {
+        HIPCHK(hipEventRecord(S->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
+        launch_gen(p, npad, false, S->do_virial, mixed, ngsol, S->side, GEN_SOLUTE);
+        launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, S->side);
+        HIPCHK(hipEventRecord(S->ev_join, S->side));
+    }
     launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
-    launch_gen(p, npad, false, S->do_virial, mixed, (int32_t)S->host_counters[4], st);
+    launch_gen(p, npad, false, S->do_virial, mixed, ngsol, st, split ? GEN_SOLVENT : GEN_ALL);
     S->pair_count += active_pairs(S, 0);
     if (S->twin && (rebuild || !S->long_computed)) {
-        launch_gen(p, npad, true, 1, mixed, (int32_t)S->host_counters[4], st);
+        launch_gen(p, npad, true, 1, mixed, ngsol, st);
         S->long_computed = true;
         S->pair_count += active_pairs(S, 1);
     }
     if (assemble) launch_excl14(p, npad, 1, nullptr, 0, st);
-    else launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
+    else if (!split) launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
     if (assemble) launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
+    if (split) HIPCHK(hipStreamWaitEvent(st, S->ev_join, 0));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -886,6 +914,8 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     // runtime it measured 1.5 % slower than separate launches (water_32768)
     const char* genv = std::getenv("GROMOS_AMD_GRAPHS");
     S->use_graphs = genv && std::atoi(genv) == 1;
+    const char* senv = std::getenv("GROMOS_AMD_SIDE_STREAM");
+    S->no_side = senv && std::atoi(senv) == 0;
     for (int32_t s = 0; s < nsteps; ++s) {
         // a plain step (no rebuild now or next, sorted copies current, no pending
         // thermostat rescan, no per-kernel timing events): one captured graph
@@ -966,6 +996,10 @@ gromos_amd_system::~gromos_amd_system() {
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : step_ev) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamSynchronize(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -371,20 +371,24 @@ __global__ void __launch_bounds__(BLOCK) k_totals(SysPtrs p, int32_t ncg) {
     }
 }
 
-// per-job pair interactions of this build (short, long): block j sums job j's
-// sorted groups, which are contiguous from cg_off, so a step can count only
-// the jobs that advance in it
+// per-job pair interactions of this build (short, long; job_pairs cleared
+// before): blocks (x, j) sum slices of job j's sorted groups, which are
+// contiguous from cg_off, so a step can count only the jobs that advance in it
+constexpr int JOB_PAIR_SLICES = 16;
 __global__ void __launch_bounds__(BLOCK) k_job_pairs(SysPtrs p) {
     __shared__ double lds[17 * 2];
-    const JobDev& J = p.jobs[blockIdx.x];
-    const int32_t k0 = J.cg_off, k1 = J.cg_off + J.n_cg;
+    const JobDev& J = p.jobs[blockIdx.y];
+    const int32_t k1 = J.cg_off + J.n_cg;
     double v[2] = {0.0, 0.0};   // exact: each sum stays below 2^53
-    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += BLOCK) {
+    for (int32_t k = J.cg_off + (int32_t)(blockIdx.x * BLOCK + threadIdx.x); k < k1; k += JOB_PAIR_SLICES * BLOCK) {
         v[0] += (double)p.pc_short[k];
         v[1] += (double)p.pc_long[k];
     }
     block_sum<2>(v, lds);
-    if (threadIdx.x < 2) p.job_pairs[2 * blockIdx.x + threadIdx.x] = (int64_t)lds[16 * 2 + threadIdx.x];
+    if (threadIdx.x < 2) {
+        const unsigned long long t = (unsigned long long)lds[16 * 2 + threadIdx.x];
+        if (t) atomicAdd((unsigned long long*)p.job_pairs + 2 * blockIdx.y + threadIdx.x, t);
+    }
 }
 
 // capacities for a single-pass fill: the previous build's count of the same
@@ -862,7 +866,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
             for (int i = 0; i < APS; ++i)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) p.fi_s[3 * (sk + i) + c] = fi[i][c];
-        } else {
+        } else if (p.s_cg_orig[k] - p.jobs[job].cg_off >= p.jobs[job].n_solute_cg) {
+            // a solvent group without partners; solute atoms' forces are assigned
+            // by k_gen_sol (which may run concurrently with this kernel)
             for (int32_t i = 0; i < 3 * nk; ++i) p.fi_s[3 * sk + i] = 0;
         }
 #pragma unroll
@@ -1077,14 +1083,16 @@ __device__ __forceinline__ void pert_share32(const SysPtrs& p, const JobDev& J, 
     const float a_c6 = (float)la[0], a_c12 = (float)la[1], b_c6 = (float)lb[0], b_c12 = (float)lb[1];
     const float a_q = r0.x * qj, b_q = r0.y * qj, alpha_lj = r0.z, alpha_crf = r0.w;
     const float r4 = r2 * r2, r6 = r4 * r2;
-    const float a_c126 = fabsf(a_c6) > 1e-10f ? a_c12 / a_c6 : 0.0f;
-    const float b_c126 = fabsf(b_c6) > 1e-10f ? b_c12 / b_c6 : 0.0f;
+    // hardware reciprocal and reciprocal square root (about 1 ulp; this share
+    // is FP32 arithmetic throughout)
+    const float a_c126 = fabsf(a_c6) > 1e-10f ? a_c12 * __builtin_amdgcn_rcpf(a_c6) : 0.0f;
+    const float b_c126 = fabsf(b_c6) > 1e-10f ? b_c12 * __builtin_amdgcn_rcpf(b_c6) : 0.0f;
     const float a_d2 = r2 + alpha_crf * (float)lp.b_crfs_lambda2;
     const float b_d2 = r2 + alpha_crf * (float)lp.a_crfs_lambda2;
-    const float a_di = 1.0f / sqrtf(a_d2), b_di = 1.0f / sqrtf(b_d2);
-    const float a_d3i = a_di / a_d2, b_d3i = b_di / b_d2;
-    const float a_d6i = 1.0f / (r6 + alpha_lj * (float)lp.b_ljs_lambda2 * a_c126);
-    const float b_d6i = 1.0f / (r6 + alpha_lj * (float)lp.a_ljs_lambda2 * b_c126);
+    const float a_di = __builtin_amdgcn_rsqf(a_d2), b_di = __builtin_amdgcn_rsqf(b_d2);
+    const float a_d3i = a_di * a_di * a_di, b_d3i = b_di * b_di * b_di;
+    const float a_d6i = __builtin_amdgcn_rcpf(r6 + alpha_lj * (float)lp.b_ljs_lambda2 * a_c126);
+    const float b_d6i = __builtin_amdgcn_rcpf(r6 + alpha_lj * (float)lp.a_ljs_lambda2 * b_c126);
     const float aln = (float)lp.a_lj_lambda_n, bln = (float)lp.b_lj_lambda_n;
     float f = ((float)lp.a_crf_lambda_n * a_q * (a_d3i + r1.x) + (float)lp.b_crf_lambda_n * b_q * (b_d3i + r1.y)) *
               (float)J.fpepsi;
@@ -1533,11 +1541,14 @@ __global__ void __launch_bounds__(BLOCK) k_gen_mx(SysPtrs p, int32_t npad) {
 // Solvent atoms' short general pairs in mixed precision, only for the solvent
 // groups listed by the pair search (gsol): GSV_SLOT lanes per group (one group
 // per slot, GSV_LANES lanes per atom taking every GSV_LANES-th entry), FP64 lane
-// sums combined by a fixed butterfly and added to the atom's fixed-point force.
-// A solvent group's short general entries are its solute partners (solvent-
-// solvent short pairs are all in the solvent list), and the solute atom comes
-// first in the topology, so those pairs' energies and virial belong to the
-// solute atom's block (k_gen_sol): these lanes add none.
+// sums combined by a fixed butterfly and added to the atom's fixed-point force
+// after the solvent pair kernel has written it.  A solvent group's short
+// general entries are its solute partners (solvent-solvent short pairs are all
+// in the solvent list), and the solute atom comes first in the topology, so
+// those pairs' energies and virial belong to the solute atom's block
+// (k_gen_sol): these lanes add none.  (Run beside the solvent pair kernel into
+// a per-group buffer added afterwards, it measured slower: 2.25e11 against
+// 2.33e11 pairs/s on the 64-job FEP mix.)
 template <int GSV_LANES>
 __global__ void __launch_bounds__(BLOCK) k_gen_sv(SysPtrs p, int32_t ngsol) {
     constexpr int GSV_SLOT = 4 * GSV_LANES;   // solvent molecules have at most 4 atoms
@@ -1565,15 +1576,16 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sv(SysPtrs p, int32_t ngsol) {
     }
 }
 
-// Solute atoms: one block (four waves) per atom (global solute index w, original
-// atom sol_atom[w]), the 256 lanes taking every 256th entry; forces and energies
+// Solute atoms: one block of NT lanes per atom (global solute index w, original
+// atom sol_atom[w]), lane t taking every NT-th entry from t; forces and energies
 // summed over each wave with a fixed xor butterfly and over the waves in order.
 // Energies go to the per-solute-atom record sol_e[(LONG ? nsol : 0) + w] that
-// k_reduce adds per job.
-template <bool LONG, bool MIXED, bool VIR>
-__global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int32_t virial) {
+// k_reduce adds per job.  (Eight waves per atom measured slower than four on
+// the 64-job FEP mix: 60 against 46 us per step.)
+template <bool LONG, bool MIXED, bool VIR, int NT>
+__global__ void __launch_bounds__(NT) k_gen_sol(SysPtrs p, int32_t nsol, int32_t virial) {
     constexpr int NV = ROW + 3;
-    __shared__ double lds[BLOCK / WAVE][NV];
+    __shared__ double lds[NT / WAVE][NV];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const int32_t w = blockIdx.x;
     double v[ROW] = {0};
@@ -1581,8 +1593,8 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
     const int32_t o = p.sol_atom[w];
     if (held(p, p.atom_job[o])) return;   // block-uniform, before any barrier
     const int32_t a = p.o2s[o];
-    if (MIXED) gen_atom32<LONG, VIR, false>(p, a, o, threadIdx.x, BLOCK, v, fx, fy, fz);
-    else gen_atom64<LONG, false>(p, a, o, threadIdx.x, BLOCK, virial, v, fx, fy, fz);
+    if (MIXED) gen_atom32<LONG, VIR, false>(p, a, o, threadIdx.x, NT, v, fx, fy, fz);
+    else gen_atom64<LONG, false>(p, a, o, threadIdx.x, NT, virial, v, fx, fy, fz);
 #pragma unroll
     for (int m = 1; m < WAVE; m <<= 1) {
         fx += __shfl_xor(fx, m, WAVE);
@@ -1603,7 +1615,7 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
         double t[NV];
 #pragma unroll
         for (int c = 0; c < NV; ++c) t[c] = lds[0][c];
-        for (int q = 1; q < BLOCK / WAVE; ++q)
+        for (int q = 1; q < NT / WAVE; ++q)
 #pragma unroll
             for (int c = 0; c < NV; ++c) t[c] += lds[q][c];
         if (MIXED) {
@@ -1613,10 +1625,11 @@ __global__ void __launch_bounds__(BLOCK) k_gen_sol(SysPtrs p, int32_t nsol, int3
                 p.fi_long_s[3 * a + 1] = iy;
                 p.fi_long_s[3 * a + 2] = iz;
             } else {
-                // one block per atom: nothing else writes this atom meanwhile
-                p.fi_s[3 * a] += ix;
-                p.fi_s[3 * a + 1] += iy;
-                p.fi_s[3 * a + 2] += iz;
+                // the atom's whole short-range pair force (the solvent pair kernel
+                // leaves solute atoms alone): assigned, one block per atom
+                p.fi_s[3 * a] = ix;
+                p.fi_s[3 * a + 1] = iy;
+                p.fi_s[3 * a + 2] = iz;
             }
         } else {
             if (LONG) {
@@ -2169,7 +2182,7 @@ void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStre
     hipLaunchKernelGGL(k_totals, dim3(nb), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_job_pairs, dim3(njobs), dim3(BLOCK), 0, s, p);
+    hipLaunchKernelGGL(k_job_pairs, dim3(JOB_PAIR_SLICES, njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, grid_for(npad), dim3(BLOCK), 0, s, p, npad, mixed);
@@ -2222,7 +2235,8 @@ void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t
     }
 }
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, int32_t ngsol,
-                hipStream_t s) {
+                hipStream_t s, int32_t part) {
+    const bool solv = part != GEN_SOLUTE, solu = part != GEN_SOLVENT;
     // solvent atoms a lane each, then solute atoms a wave each (after: the long
     // pass of k_gen writes zeros for the solute atoms)
     const int32_t nsol = p.nsol;
@@ -2230,25 +2244,27 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
     if (mixed) {
         if (lng) {
             hipLaunchKernelGGL((k_gen_mx<true, true>), grid_for(npad), dim3(BLOCK), 0, s, p, npad);
-            if (nsol) hipLaunchKernelGGL((k_gen_sol<true, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+            if (nsol) hipLaunchKernelGGL((k_gen_sol<true, true, true, BLOCK>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
         } else {
             // the short pass of the solvent atoms: only the listed groups
             // two lanes per atom: 46.6 us per step on the 64-job FEP mix, against
             // 55.4 (one lane), 53.4 (four) and 132.5 (sixteen)
             constexpr int32_t GPB = BLOCK / (4 * 2);   // groups per block
-            if (ngsol > 0)
+            if (solv && ngsol > 0)
                 hipLaunchKernelGGL(k_gen_sv<2>, dim3((unsigned)((ngsol + GPB - 1) / GPB)), dim3(BLOCK), 0, s, p, ngsol);
-            if (nsol && virial) hipLaunchKernelGGL((k_gen_sol<false, true, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
-            else if (nsol) hipLaunchKernelGGL((k_gen_sol<false, true, false>), gs, dim3(BLOCK), 0, s, p, nsol, 0);
+            if (solu && nsol && virial)
+                hipLaunchKernelGGL((k_gen_sol<false, true, true, BLOCK>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+            else if (solu && nsol)
+                hipLaunchKernelGGL((k_gen_sol<false, true, false, BLOCK>), gs, dim3(BLOCK), 0, s, p, nsol, 0);
         }
         return;
     }
     if (lng) {
         hipLaunchKernelGGL(k_gen<true>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
-        if (nsol) hipLaunchKernelGGL((k_gen_sol<true, false, true>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
+        if (nsol) hipLaunchKernelGGL((k_gen_sol<true, false, true, BLOCK>), gs, dim3(BLOCK), 0, s, p, nsol, 1);
     } else {
         hipLaunchKernelGGL(k_gen<false>, grid_for(npad), dim3(BLOCK), 0, s, p, npad, virial);
-        if (nsol) hipLaunchKernelGGL((k_gen_sol<false, false, false>), gs, dim3(BLOCK), 0, s, p, nsol, virial);
+        if (nsol) hipLaunchKernelGGL((k_gen_sol<false, false, false, BLOCK>), gs, dim3(BLOCK), 0, s, p, nsol, virial);
     }
 }
 void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, const int32_t* blocks, int32_t nblocks,

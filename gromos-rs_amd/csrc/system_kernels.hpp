@@ -108,6 +108,7 @@ void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
+// per-job pair counts into job_pairs (cleared by the caller)
 void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
@@ -118,9 +119,11 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// ngsol: solvent groups listed in gsol by the last pair search (mixed short pass)
+// ngsol: solvent groups listed in gsol by the last pair search (mixed short pass);
+// part: GEN_ALL, or (mixed short pass) only the solvent or only the solute atoms
+constexpr int32_t GEN_ALL = 0, GEN_SOLVENT = 1, GEN_SOLUTE = 2;
 void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_t mixed, int32_t ngsol,
-                hipStream_t s);
+                hipStream_t s, int32_t part = GEN_ALL);
 // solv_energy: also the solvent molecules' excluded RF energy (single points;
 // md steps take it in the integrator)
 // blocks (md steps, solv_energy 0): the nblocks atom blocks holding solute atoms

@@ -448,6 +448,29 @@ def test_step_graphs_match_separate_launches(prec, monkeypatch):
         assert np.array_equal(out["1"][k], out["0"][k])
 
 
+def test_side_stream_matches_single_stream(monkeypatch):
+    """The mixed md step's solute pass on the side stream (concurrent with the
+    solvent pair kernel) gives the same bits as the whole step on one stream,
+    for one FEP job and for a session batch of two."""
+    a = gen.fep_job(3000, 8, 0.5, seed=7)
+    b = gen.fep_job(1500, 5, 0.25, seed=8)
+    out = {}
+    for side in ("1", "0"):
+        monkeypatch.setenv("GROMOS_AMD_SIDE_STREAM", side)
+        ds = rt.DeviceSystem(a, precision=rt.MIXED)
+        ds.pairlist_update()
+        e = np.concatenate([ds.md(11), ds.md(6)])
+        x, v = ds.get_state()
+        ds.close()
+        sess = rt.Session(precision=rt.MIXED, nsnb=5)
+        ja, jb = sess.admit(a, 12), sess.admit(b, 9)
+        sess.step(12)
+        out[side] = (e, x, v, sess.energies(ja), sess.energies(jb), *sess.fetch(ja), *sess.fetch(jb))
+        sess.close()
+    for k in range(len(out["1"])):
+        assert np.array_equal(out["1"][k], out["0"][k]), k
+
+
 def test_autotuner_picks_a_variant():
     s = gen.spc_water_box(3000, seed=13)
     ds = rt.DeviceSystem(s, precision=rt.MIXED)

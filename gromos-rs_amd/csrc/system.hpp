@@ -68,7 +68,7 @@ constexpr int REC = 14;
 // parts per job of the solvent-solvent energy and E_kin records (two-stage
 // reduction); PART_W columns: e_lj, e_crf, virial[9], E_kin, E_kin_new
 constexpr int VV_CHUNK = 16;   // groups per chunk of the solvent pair kernels (one job each)
-constexpr int VV_PARTS = 32;
+constexpr int VV_PARTS = 128;
 constexpr int PART_W = 13;
 
 }  // namespace gamd

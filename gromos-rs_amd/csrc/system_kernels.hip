@@ -2279,15 +2279,19 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
 }
 void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,
                       int32_t mixed, hipStream_t s) {
-    if (mixed) hipLaunchKernelGGL(k_integrate<true>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, with_long, write_xq);
-    else hipLaunchKernelGGL(k_integrate<false>, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, with_long, write_xq);
+    // one wave per block: a lane per group gives few waves (32,768 waters: 512),
+    // spread over all CUs this way rather than 128 blocks on half of them
+    const dim3 g((unsigned)((ncg + WAVE - 1) / WAVE));
+    if (mixed) hipLaunchKernelGGL(k_integrate<true>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
+    else hipLaunchKernelGGL(k_integrate<false>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
 }
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_therm, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_reduce(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t with_kin, int32_t advance,
                    int32_t njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(BLOCK), 0, s, p, p.vv_virial, with_kin);
+    // one wave per part: the parts of a small job hold few groups
+    hipLaunchKernelGGL(k_vv_parts, dim3(VV_PARTS, njobs), dim3(WAVE), 0, s, p, p.vv_virial, with_kin);
     hipLaunchKernelGGL(k_reduce, dim3(njobs), dim3(BLOCK), 0, s, p, npad, with_long, with_kin, advance, njobs);
 }
 void launch_materialize(const SysPtrs& p, int32_t ncg, bool fill, unsigned long long* counts, const int64_t* caps,

@@ -133,6 +133,12 @@ struct gromos_amd_system {
     int32_t *d_s2o = nullptr, *d_s_atom_cg = nullptr, *d_s_type = nullptr;
     int64_t *d_vv_cnt = nullptr, *d_gen_cnt = nullptr, *d_vv_off = nullptr, *d_gen_off = nullptr;
     uint32_t *d_vv_ent = nullptr, *d_gen_ent = nullptr;
+    // control block of a pair-list build, read back in one copy: [0, 5) counters,
+    // [5] overflow flag, [6, 6 + 2 njobs) per-job pair counts; zeroed on the device
+    // at the start of each build
+    uint64_t* d_ctl = nullptr;
+    uint64_t* h_ctl = nullptr;   // pinned host copy
+    size_t h_ctl_n = 0;
     uint64_t* d_counters = nullptr;
     int32_t* d_gsol = nullptr;
     int64_t *d_pc_short = nullptr, *d_pc_long = nullptr;

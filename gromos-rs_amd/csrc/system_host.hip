@@ -545,12 +545,12 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_s_atom_cg, npa, true)) || (rc = S->reserve(S->d_o2s, npa, true)) ||
         (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_s_prec, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
-        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_counters, 5)) ||
+        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_ctl, 6 + 2 * (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_gsol, nc, true)) ||
         (rc = S->reserve(S->d_pc_short, nc, true)) || (rc = S->reserve(S->d_pc_long, nc, true)) ||
         (rc = S->reserve(S->d_xq, npa, true)) || (rc = S->reserve(S->d_vv_cap, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cap, nc + 1, true)) || (rc = S->reserve(S->d_prev_vv, nc, true)) ||
-        (rc = S->reserve(S->d_prev_gen, nc, true)) || (rc = S->reserve(S->d_overflow, 1)) ||
+        (rc = S->reserve(S->d_prev_gen, nc, true)) ||
         (rc = S->reserve(S->d_f_s, n3, true)) || (rc = S->reserve(S->d_f_long_s, n3, true)) ||
         (rc = S->reserve(S->d_f_x, n3, true)) ||
         (rc = S->reserve(S->d_xf, npa, true)) || (rc = S->reserve(S->d_fi_s, n3, true)) ||
@@ -561,9 +561,26 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
-        (rc = S->reserve(S->d_step_done, 1)) || (rc = S->reserve(S->d_job_pairs, 2 * (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_step_done, 1)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
+    S->d_counters = S->d_ctl;
+    S->d_overflow = (int32_t*)(S->d_ctl + 5);
+    S->d_job_pairs = (int64_t*)(S->d_ctl + 6);
+    if (S->h_ctl_n < 6 + 2 * (size_t)njobs) {
+        if (S->h_ctl) HIPCHK(hipHostFree(S->h_ctl));
+        S->h_ctl = nullptr;
+        S->h_ctl_n = 0;
+        HIPCHK(hipHostMalloc((void**)&S->h_ctl, (6 + 2 * (size_t)njobs) * sizeof(uint64_t), hipHostMallocDefault));
+        S->h_ctl_n = 6 + 2 * (size_t)njobs;
+    }
+    // set once per layout, kept by every build: padding atoms' sorted slots map to
+    // no atom; cell counts start at zero (each build's k_sorted_cg clears them
+    // for the next); sorted copies of padding atoms are zero
+    HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, npa * sizeof(int32_t), st));
+    HIPCHK(hipMemsetAsync(S->d_cell_count, 0, ncl * sizeof(int32_t), st));
+    HIPCHK(hipMemsetAsync(S->d_xq, 0, npa * sizeof(double4), st));
+    HIPCHK(hipMemsetAsync(S->d_xf, 0, npa * sizeof(float4), st));
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step_done, 0, sizeof(uint32_t), st));
@@ -620,22 +637,27 @@ int build_pairlist(gromos_amd_system* S) {
     hipStream_t st = S->stream;
     SysPtrs p = S->ptrs();
     int32_t ncg = S->ncg;
-    launch_cg_ref(p, ncg, st);
-    HIPCHK(hipMemsetAsync(S->d_cell_count, 0, ((size_t)S->ncell + 1) * sizeof(int32_t), st));
-    launch_cell_hist(p, ncg, st);
+    const size_t nctl = 6 + 2 * (size_t)S->njobs;
+    // reference points, cell keys and counts (the control block cleared here)
+    launch_cg_ref(p, ncg, (int32_t)nctl, st);
     size_t tb = S->temp_bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_cell_count, S->d_cell_start, S->ncell + 1, st));
     tb = S->temp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(S->d_temp, tb, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
                                               ncg, 0, S->cell_bits, st));
-    launch_sorted_cg(p, ncg, st);
+    launch_sorted_cg(p, ncg, S->ncell + 1, st);
     tb = S->temp_bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
-    HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, (size_t)S->npad * sizeof(int32_t), st));
-    launch_sorted_atoms(p, ncg, st);
-    HIPCHK(hipMemsetAsync(S->d_counters, 0, 5 * sizeof(uint64_t), st));
-    HIPCHK(hipMemsetAsync(S->d_overflow, 0, sizeof(int32_t), st));
-    int32_t overflow = 0;
+    // sorted maps and the sorted position copies (the gather of this order)
+    launch_sorted_atoms(p, ncg, S->precision == GROMOS_AMD_MIXED ? 1 : 0, st);
+    // one readback of the control block; after it the host knows the counts
+    auto read_ctl = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int c = 0; c < 5; ++c) S->host_counters[c] = S->h_ctl[c];
+        for (size_t b = 0; b < 2 * (size_t)S->njobs; ++b) S->host_job_pairs[b] = (int64_t)S->h_ctl[6 + b];
+        return 0;
+    };
     bool need_exact = !S->have_prev;
     if (S->have_prev) {
         launch_caps(p, ncg, 0, st);
@@ -646,52 +668,37 @@ int build_pairlist(gromos_amd_system* S) {
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
         launch_nbl(p, ncg, S->ncell, true, st);
-        if (S->njobs > 1) {
-            HIPCHK(hipMemsetAsync(S->d_job_pairs, 0, 2 * (size_t)S->njobs * sizeof(int64_t), st));
-            launch_job_pairs(p, S->njobs, st);
-            HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
-                                  hipMemcpyDeviceToHost, st));
-        }
-        HIPCHK(hipMemcpyAsync(&overflow, S->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        if (overflow) {
+        if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
+        if (int rc = read_ctl()) return rc;
+        if ((int32_t)S->h_ctl[5]) {   // the overflow flag (low word)
             S->refills++;
             need_exact = true;   // the fill pass left exact counts in vv_cnt / gen_cnt
         }
     } else {
         launch_nbl(p, ncg, S->ncell, false, st);
-        if (S->njobs > 1) {
-            HIPCHK(hipMemsetAsync(S->d_job_pairs, 0, 2 * (size_t)S->njobs * sizeof(int64_t), st));
-            launch_job_pairs(p, S->njobs, st);
-            HIPCHK(hipMemcpyAsync(S->host_job_pairs.data(), S->d_job_pairs, 2 * (size_t)S->njobs * sizeof(int64_t),
-                                  hipMemcpyDeviceToHost, st));
-        }
+        if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
     }
     if (need_exact) {
         if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
         int64_t tot[2];
         HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(S->host_counters, S->d_counters, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        if (int rc = read_ctl()) return rc;
         if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
         if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
         p = S->ptrs();
         launch_caps(p, ncg, 1, st);
-        uint64_t keep[5];
-        std::memcpy(keep, S->host_counters, sizeof(keep));
-        // the refill lists the solvent groups with general partners again
+        // the refill lists the solvent groups with general partners again (the
+        // host keeps the counts it read)
         HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
         launch_nbl(p, ncg, S->ncell, true, st);
-        std::memcpy(S->host_counters, keep, sizeof(keep));
     }
-    launch_caps(p, ncg, 2, st);
     HIPCHK(hipGetLastError());
     if (S->njobs == 1) {   // one job: its counts are the totals
         S->host_job_pairs[0] = (int64_t)S->host_counters[0];
         S->host_job_pairs[1] = (int64_t)S->host_counters[1];
     }
+    S->xq_valid = true;   // k_sorted_atoms wrote the sorted copies of the current positions
     S->prev_tot_vv = (int64_t)S->host_counters[2];
     S->prev_tot_gen = (int64_t)S->host_counters[3];
     S->have_prev = true;
@@ -810,9 +817,9 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     SysPtrs p = S->ptrs();
     int32_t npad = S->npad;
     const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
-    // sorted position copies: written by the previous step's integrator unless
-    // the list was rebuilt (new order) or the state was set from outside
-    if (rebuild || !S->xq_valid) launch_gather(p, npad, mixed, st);
+    // sorted position copies: written by the previous step's integrator or by the
+    // pair-list build (new order), unless the state was set from outside
+    if (!S->xq_valid) launch_gather(p, npad, mixed, st);
     S->xq_valid = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (S->timing) {
@@ -990,6 +997,7 @@ int collect_step_times(gromos_amd_system* S) {
 // ---------------------------------------------------------------------------
 gromos_amd_system::~gromos_amd_system() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (h_ctl) (void)hipHostFree(h_ctl);
     for (auto& g : graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& kv : bufs)

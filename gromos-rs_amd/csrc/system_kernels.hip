@@ -35,9 +35,13 @@ static inline dim3 xcd_grid(int64_t nblocks) {
 // Reference point of each charge group: COG for solute groups (left-to-right sum
 // from zero, then division, topology.rs:408-416), first atom for solvent
 // (pairlist.rs:304-307, 359-367).  Cell key from the wrapped point.
-__global__ void k_cg_ref(SysPtrs p, int32_t ncg) {
+// Also clears the build's control block (nctl words: counters, overflow flag,
+// per-job pair counts) and counts the groups per cell (cell_count is zero here:
+// cleared at layout and by the previous build's k_sorted_cg).
+__global__ void k_cg_ref(SysPtrs p, int32_t ncg, int32_t nctl) {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncg) return;
+    for (int32_t w = c; w < nctl; w += ncg) p.counters[w] = 0ull;
     const JobDev& J = p.jobs[p.cg_job[c]];
     int32_t s = p.cg_start[c], e = s + p.cg_n[c];
     const double* x = p.x_cur;
@@ -72,20 +76,19 @@ __global__ void k_cg_ref(SysPtrs p, int32_t ncg) {
             cc[a] = k < 0 ? 0 : (k >= J.ncell[a] ? J.ncell[a] - 1 : k);
         }
     }
-    p.key_in[c] = (uint32_t)(J.cell_off + cc[0] + J.ncell[0] * (cc[1] + J.ncell[1] * cc[2]));
+    const uint32_t key = (uint32_t)(J.cell_off + cc[0] + J.ncell[0] * (cc[1] + J.ncell[1] * cc[2]));
+    p.key_in[c] = key;
     p.val_in[c] = (uint32_t)c;
+    atomicAdd(p.cell_count + key, 1);
 }
 
-__global__ void k_cell_hist(SysPtrs p, int32_t ncg) {
-    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncg) return;
-    atomicAdd(p.cell_count + p.key_in[c], 1);
-}
-
-// sorted position k -> original group, atom count, sorted copy of the reference point
-__global__ void k_sorted_cg(SysPtrs p, int32_t ncg) {
+// sorted position k -> original group, atom count, sorted copy of the reference
+// point; clears cell_count (ncl words, already scanned into cell_start) for the
+// next build
+__global__ void k_sorted_cg(SysPtrs p, int32_t ncg, int32_t ncl) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncg) return;
+    for (int32_t w = k; w < ncl; w += ncg) p.cell_count[w] = 0;
     int32_t c = (int32_t)p.val_out[k];
     p.s_cg_orig[k] = c;
     p.s_cg_n[k] = p.cg_n[c];
@@ -96,8 +99,10 @@ __global__ void k_sorted_cg(SysPtrs p, int32_t ncg) {
 
 // Sorted atom layout: job j's groups fill [atom_off, atom_off + n_atoms) in
 // sorted order.  s_cg_start comes from an exclusive scan of s_cg_n, shifted per
-// job to the job's aligned offset.
-__global__ void k_sorted_atoms(SysPtrs p, int32_t ncg) {
+// job to the job's aligned offset.  Also writes the sorted position copies xq
+// (and, mixed, xf: FP32 offset from the group's first atom, difference taken in
+// FP64) of the current positions, as k_gather does.
+__global__ void k_sorted_atoms(SysPtrs p, int32_t ncg, int32_t mixed) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncg) return;
     int32_t c = p.s_cg_orig[k];
@@ -113,6 +118,12 @@ __global__ void k_sorted_atoms(SysPtrs p, int32_t ncg) {
         p.s_atom_cg[start + a] = k;
         p.s_type[start + a] = p.iac[o + a];
         p.s_prec[start + a] = (J.has_pert && p.pflag[o + a]) ? J.sol_off + (o + a - J.atom_off) : -1;
+        const double4 v = make_double4(p.x_cur[3 * (o + a)], p.x_cur[3 * (o + a) + 1], p.x_cur[3 * (o + a) + 2],
+                                       p.q[o + a]);
+        p.xq[start + a] = v;
+        if (mixed)
+            p.xf[start + a] = make_float4((float)(v.x - p.x_cur[3 * o]), (float)(v.y - p.x_cur[3 * o + 1]),
+                                          (float)(v.z - p.x_cur[3 * o + 2]), (float)v.w);
     }
 }
 
@@ -312,6 +323,8 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
             const int32_t nvv = vv_pos - vv_base, ngen = gen_pos - gen_base;
             p.vv_cnt[k] = nvv;
             p.gen_cnt[k] = ngen;
+            p.prev_vv[ck] = nvv;   // the next build's slot sizes (read by k_caps before its search)
+            p.prev_gen[ck] = ngen;
             p.pc_short[k] = 0;
             p.pc_long[k] = 0;
             if (FILL && (nvv > vv_end - vv_base || ngen > gen_end - gen_base)) atomicOr(p.overflow, 1);
@@ -402,16 +415,7 @@ __global__ void k_caps(SysPtrs p, int32_t ncg) {
     p.gen_cap[k] = g + (g >> 3) + 16;
 }
 
-// remember this build's counts per original group; capacities for an exact
-// refill are the counts themselves
-__global__ void k_keep_counts(SysPtrs p, int32_t ncg) {
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncg) return;
-    int32_t c = p.s_cg_orig[k];
-    p.prev_vv[c] = (int32_t)p.vv_cnt[k];
-    p.prev_gen[c] = (int32_t)p.gen_cnt[k];
-}
-
+// capacities for an exact refill are the counts themselves
 __global__ void k_caps_from_counts(SysPtrs p, int32_t ncg) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncg) return;
@@ -2156,22 +2160,18 @@ __global__ void k_materialize(SysPtrs p, int32_t ncg, unsigned long long* counts
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
-void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_ref, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+void launch_cg_ref(const SysPtrs& p, int32_t ncg, int32_t nctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_ref, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, nctl);
 }
-void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_hist, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+void launch_sorted_cg(const SysPtrs& p, int32_t ncg, int32_t ncl, hipStream_t s) {
+    hipLaunchKernelGGL(k_sorted_cg, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, ncl);
 }
-void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s) {
-    hipLaunchKernelGGL(k_sorted_cg, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
-}
-void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s) {
-    hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream_t s) {
+    hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, mixed);
 }
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
     if (mode == 0) hipLaunchKernelGGL(k_caps, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
-    else if (mode == 1) hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
-    else hipLaunchKernelGGL(k_keep_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    else hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s) {
     // one block per cell

@@ -103,14 +103,16 @@ struct SysPtrs {
     int32_t* o2s;              // original atom -> sorted atom
 };
 
-void launch_cg_ref(const SysPtrs& p, int32_t ncg, hipStream_t s);
-void launch_cell_hist(const SysPtrs& p, int32_t ncg, hipStream_t s);
-void launch_sorted_cg(const SysPtrs& p, int32_t ncg, hipStream_t s);
-void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, hipStream_t s);
+// nctl: words of the build's control block (counters) that k_cg_ref clears
+void launch_cg_ref(const SysPtrs& p, int32_t ncg, int32_t nctl, hipStream_t s);
+// ncl: cell_count words to clear for the next build
+void launch_sorted_cg(const SysPtrs& p, int32_t ncg, int32_t ncl, hipStream_t s);
+// mixed: also the FP32 offset copies xf
+void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
 // per-job pair counts into job_pairs (cleared by the caller)
 void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s);
-// mode 0: capacities from the previous counts; 1: capacities = counts; 2: keep counts
+// mode 0: capacities from the previous counts (kept by k_nbl); 1: capacities = counts
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
 // form: VV_LJ00 (LJ on the first-first atom pair only) | VV_NEUTRAL (neutral

@@ -290,6 +290,12 @@ extern "C" void gromos_amd_system_destroy(gromos_amd_system* S) { delete S; }
 extern "C" int64_t gromos_amd_system_device_bytes(const gromos_amd_system* S) { return S ? S->bytes : 0; }
 extern "C" int32_t gromos_amd_system_vv_lanes(const gromos_amd_system* S) { return S ? S->vv_lanes : 0; }
 extern "C" int64_t gromos_amd_system_graph_launches(const gromos_amd_system* S) { return S ? S->graph_launches : 0; }
+extern "C" int gromos_amd_system_rebuild_stats(const gromos_amd_system* S, int64_t* rebuilds, int64_t* refills) {
+    if (!S || !rebuilds || !refills) return fail(GROMOS_AMD_EINVAL, "rebuild_stats: null argument");
+    *rebuilds = S->rebuilds;
+    *refills = S->refills;
+    return 0;
+}
 extern "C" int64_t gromos_amd_system_pair_count(const gromos_amd_system* S) { return S ? S->pair_count : 0; }
 
 extern "C" int gromos_amd_system_set_state(gromos_amd_system* S, const double* pos, const double* vel) {

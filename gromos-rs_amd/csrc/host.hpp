@@ -139,6 +139,8 @@ struct gromos_amd_system {
     uint64_t* d_ctl = nullptr;
     uint64_t* h_ctl = nullptr;   // pinned host copy
     size_t h_ctl_n = 0;
+    hipEvent_t ctl_ev = nullptr;   // recorded after the control block's readback
+    bool ctl_pending = false;
     uint64_t* d_counters = nullptr;
     int32_t* d_gsol = nullptr;
     int64_t *d_pc_short = nullptr, *d_pc_long = nullptr;
@@ -280,7 +282,12 @@ namespace gamd {
 // and velocities are zeroed, the pair list is marked stale.
 int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs, int32_t precision);
 int upload_jobs(gromos_amd_system* S);
-int build_pairlist(gromos_amd_system* S);
+// deferred: leave the single-pass fill's readback in flight (ctl_pending) for
+// finish_pairlist, so that the step's first pair kernel is enqueued meanwhile
+int build_pairlist(gromos_amd_system* S, bool deferred = false);
+// waits for a pending readback; *refilled: the lists were refilled with exact
+// slots (kernels enqueued on them since the build must run again)
+int finish_pairlist(gromos_amd_system* S, bool* refilled);
 int forcefield(gromos_amd_system* S, bool assemble);
 // nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when
 // given.  A SETTLE failure is an error unless `failed` is given: it then gets

@@ -230,6 +230,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         HIPCHK(hipStreamCreate(&S->side));
         HIPCHK(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&S->ctl_ev, hipEventDisableTiming));
     }
     HIPCHK(hipStreamSynchronize(S->stream));   // host vectors below feed async copies
     int njobs = (int)imgs.size();
@@ -607,6 +608,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->xq_valid = false;
     S->built = false;
     S->have_prev = false;
+    S->ctl_pending = false;
     S->long_computed = false;
     S->steps_since_update = 0;
     return 0;
@@ -633,7 +635,55 @@ static int scan_offsets(gromos_amd_system* S, const int64_t* cnt_vv, const int64
     return 0;
 }
 
-int build_pairlist(gromos_amd_system* S) {
+// the control block's readback (enqueued after the search) -> host counters
+static int parse_ctl(gromos_amd_system* S) {
+    for (int c = 0; c < 5; ++c) S->host_counters[c] = S->h_ctl[c];
+    for (size_t b = 0; b < 2 * (size_t)S->njobs; ++b) S->host_job_pairs[b] = (int64_t)S->h_ctl[6 + b];
+    return 0;
+}
+
+static int read_ctl(gromos_amd_system* S) {
+    const size_t nctl = 6 + 2 * (size_t)S->njobs;
+    HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    return parse_ctl(S);
+}
+
+// lists with exact slots: the counts of the last search pass scanned into
+// offsets, buffers sized to the totals, a second fill pass
+static int refill_exact(gromos_amd_system* S) {
+    hipStream_t st = S->stream;
+    const int32_t ncg = S->ncg;
+    if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
+    int64_t tot[2];
+    HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (int rc = read_ctl(S)) return rc;
+    if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
+    if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
+    SysPtrs p = S->ptrs();
+    launch_caps(p, ncg, 1, st);
+    // the refill lists the solvent groups with general partners again (the host
+    // keeps the counts it read)
+    HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
+    launch_nbl(p, ncg, S->ncell, true, st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// bookkeeping once the host holds this build's counts
+static void build_done(gromos_amd_system* S) {
+    if (S->njobs == 1) {   // one job: its counts are the totals
+        S->host_job_pairs[0] = (int64_t)S->host_counters[0];
+        S->host_job_pairs[1] = (int64_t)S->host_counters[1];
+    }
+    S->prev_tot_vv = (int64_t)S->host_counters[2];
+    S->prev_tot_gen = (int64_t)S->host_counters[3];
+}
+
+// deferred: the single-pass fill's readback is left in flight (S->ctl_pending);
+// finish_pairlist waits for it, and refills if a group outgrew its slot.
+int build_pairlist(gromos_amd_system* S, bool deferred) {
     hipStream_t st = S->stream;
     SysPtrs p = S->ptrs();
     int32_t ncg = S->ncg;
@@ -650,17 +700,15 @@ int build_pairlist(gromos_amd_system* S) {
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
     // sorted maps and the sorted position copies (the gather of this order)
     launch_sorted_atoms(p, ncg, S->precision == GROMOS_AMD_MIXED ? 1 : 0, st);
-    // one readback of the control block; after it the host knows the counts
-    auto read_ctl = [&]() -> int {
-        HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int c = 0; c < 5; ++c) S->host_counters[c] = S->h_ctl[c];
-        for (size_t b = 0; b < 2 * (size_t)S->njobs; ++b) S->host_job_pairs[b] = (int64_t)S->h_ctl[6 + b];
-        return 0;
-    };
-    bool need_exact = !S->have_prev;
+    S->xq_valid = true;
+    S->built = true;
+    S->rebuilds++;
+    S->steps_since_update = 0;
     if (S->have_prev) {
-        launch_caps(p, ncg, 0, st);
+        // GROMOS_AMD_TIGHT_SLOTS=1 (tests): slots exactly the previous counts, so
+        // that the overflow and refill path runs at most builds
+        const char* tenv = std::getenv("GROMOS_AMD_TIGHT_SLOTS");
+        launch_caps(p, ncg, tenv && std::atoi(tenv) == 1 ? 2 : 0, st);
         if (int rc = scan_offsets(S, S->d_vv_cap, S->d_gen_cap)) return rc;
         int64_t bv = S->prev_tot_vv + S->prev_tot_vv / 8 + 16 * (int64_t)ncg + 64;
         int64_t bg = S->prev_tot_gen + S->prev_tot_gen / 8 + 16 * (int64_t)ncg + 64;
@@ -669,42 +717,37 @@ int build_pairlist(gromos_amd_system* S) {
         p = S->ptrs();
         launch_nbl(p, ncg, S->ncell, true, st);
         if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
-        if (int rc = read_ctl()) return rc;
-        if ((int32_t)S->h_ctl[5]) {   // the overflow flag (low word)
-            S->refills++;
-            need_exact = true;   // the fill pass left exact counts in vv_cnt / gen_cnt
-        }
-    } else {
-        launch_nbl(p, ncg, S->ncell, false, st);
-        if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
+        HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(S->ctl_ev, st));
+        HIPCHK(hipGetLastError());
+        S->ctl_pending = true;
+        if (deferred) return 0;
+        bool refilled = false;
+        return finish_pairlist(S, &refilled);
     }
-    if (need_exact) {
-        if (int rc = scan_offsets(S, S->d_vv_cnt, S->d_gen_cnt)) return rc;
-        int64_t tot[2];
-        HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        if (int rc = read_ctl()) return rc;
-        if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
-        if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
-        p = S->ptrs();
-        launch_caps(p, ncg, 1, st);
-        // the refill lists the solvent groups with general partners again (the
-        // host keeps the counts it read)
-        HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
-        launch_nbl(p, ncg, S->ncell, true, st);
-    }
-    HIPCHK(hipGetLastError());
-    if (S->njobs == 1) {   // one job: its counts are the totals
-        S->host_job_pairs[0] = (int64_t)S->host_counters[0];
-        S->host_job_pairs[1] = (int64_t)S->host_counters[1];
-    }
-    S->xq_valid = true;   // k_sorted_atoms wrote the sorted copies of the current positions
-    S->prev_tot_vv = (int64_t)S->host_counters[2];
-    S->prev_tot_gen = (int64_t)S->host_counters[3];
+    // first build: count pass, then exact slots
+    launch_nbl(p, ncg, S->ncell, false, st);
+    if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
+    if (int rc = refill_exact(S)) return rc;
+    build_done(S);
     S->have_prev = true;
-    S->built = true;
-    S->rebuilds++;
-    S->steps_since_update = 0;
+    return 0;
+}
+
+int finish_pairlist(gromos_amd_system* S, bool* refilled) {
+    *refilled = false;
+    if (!S->ctl_pending) return 0;
+    S->ctl_pending = false;
+    HIPCHK(hipEventSynchronize(S->ctl_ev));
+    parse_ctl(S);
+    if ((int32_t)S->h_ctl[5]) {   // overflow flag (low word): the fill pass left exact counts
+        S->refills++;
+        // kernels on the side stream may still read the lists the refill replaces
+        if (S->side) HIPCHK(hipStreamSynchronize(S->side));
+        if (int rc = refill_exact(S)) return rc;
+        *refilled = true;
+    }
+    build_done(S);
     return 0;
 }
 
@@ -811,7 +854,9 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     }
     bool rebuild = S->steps_since_update >= S->nsnb;   // pairlist.rs:105-107
     if (rebuild) {
-        if (int rc = build_pairlist(S)) return rc;
+        // the counts' readback stays in flight while the first pair kernels are
+        // enqueued (finish_pairlist below)
+        if (int rc = build_pairlist(S, true)) return rc;
     }
     S->steps_since_update++;
     SysPtrs p = S->ptrs();
@@ -829,19 +874,32 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     const int32_t aps = solvent_aps(S);
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
-    const int32_t ngsol = (int32_t)S->host_counters[4];
     // mixed md step: the solute atoms' pass (general pairs, then excluded / 1-4
     // terms) is independent of the solvent pair kernel (separate outputs, inputs
     // ready at the fork) and latency bound, so it overlaps it on the side stream
     const bool split = mixed && !assemble && S->nsol > 0 && !S->no_side;
-    if (split) {
-        HIPCHK(hipEventRecord(S->ev_fork, st));
-        HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
-        launch_gen(p, npad, false, S->do_virial, mixed, ngsol, S->side, GEN_SOLUTE);
-        launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, S->side);
-        HIPCHK(hipEventRecord(S->ev_join, S->side));
+    auto launch_first = [&](hipEvent_t a, hipEvent_t b) -> int {
+        if (split) {
+            HIPCHK(hipEventRecord(S->ev_fork, st));
+            HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
+            launch_gen(p, npad, false, S->do_virial, mixed, 0, S->side, GEN_SOLUTE);
+            launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, S->side);
+            HIPCHK(hipEventRecord(S->ev_join, S->side));
+        }
+        launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, a, b);
+        return 0;
+    };
+    if (int rc = launch_first(e0, e1)) return rc;
+    bool refilled = false;
+    if (int rc = finish_pairlist(S, &refilled)) return rc;
+    if (refilled) {
+        // a group outgrew its slot: the kernels above ran on truncated lists;
+        // they assign their outputs, so running them again on the refilled lists
+        // replaces everything they wrote
+        p = S->ptrs();
+        if (int rc = launch_first(nullptr, nullptr)) return rc;
     }
-    launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, e0, e1);
+    const int32_t ngsol = (int32_t)S->host_counters[4];
     launch_gen(p, npad, false, S->do_virial, mixed, ngsol, st, split ? GEN_SOLVENT : GEN_ALL);
     S->pair_count += active_pairs(S, 0);
     if (S->twin && (rebuild || !S->long_computed)) {
@@ -1007,6 +1065,7 @@ gromos_amd_system::~gromos_amd_system() {
     if (side) (void)hipStreamSynchronize(side);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ctl_ev) (void)hipEventDestroy(ctl_ev);
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
 }

@@ -406,13 +406,14 @@ __global__ void __launch_bounds__(BLOCK) k_job_pairs(SysPtrs p) {
 
 // capacities for a single-pass fill: the previous build's count of the same
 // (original) group plus 1/8 and 16 entries
-__global__ void k_caps(SysPtrs p, int32_t ncg) {
+// tight (a test hook): exactly the previous counts, so that most builds refill
+__global__ void k_caps(SysPtrs p, int32_t ncg, int32_t tight) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncg) return;
     int32_t c = p.s_cg_orig[k];
     int64_t v = p.prev_vv[c], g = p.prev_gen[c];
-    p.vv_cap[k] = v + (v >> 3) + 16;
-    p.gen_cap[k] = g + (g >> 3) + 16;
+    p.vv_cap[k] = tight ? v : v + (v >> 3) + 16;
+    p.gen_cap[k] = tight ? g : g + (g >> 3) + 16;
 }
 
 // capacities for an exact refill are the counts themselves
@@ -571,7 +572,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
     int32_t e0 = 0, e1 = 0, sk = 0;
     if (valid) {
         e0 = (int32_t)p.vv_off[k];
-        e1 = e0 + (int32_t)p.vv_cnt[k];
+        e1 = e0 + (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);   // entries written (a pending refill)
         sk = p.s_cg_start[k];
     }
     if (e1 > e0) {
@@ -692,7 +693,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     int32_t e0 = 0, e1 = 0, sk = 0;
     if (valid) {
         e0 = (int32_t)p.vv_off[k];
-        e1 = e0 + (int32_t)p.vv_cnt[k];
+        e1 = e0 + (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);   // entries written (a pending refill)
         sk = p.s_cg_start[k];
     }
     if (e1 > e0) {
@@ -1264,7 +1265,8 @@ __device__ __forceinline__ void gen_atom64(const SysPtrs& p, int32_t a, int32_t 
     const bool po = !SOLV && jp && p.pflag[o];   // solvent atoms are never perturbed
     PertAtom PA{};
     if (po) PA = pert_atom(p, J, o);
-    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
+    // entries written: a group that outgrew its slot has only cap of them until the refill
+    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)min(p.gen_cnt[k], p.gen_cap[k]);
     for (int32_t e = e0 + lane0; e < e1; e += stride) {
         uint32_t ent = p.gen_ent[e];
         bool lng = (ent & ENT_LONG) != 0;
@@ -1398,7 +1400,8 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
     const bool po = !SOLV && jp && p.pflag[o];
     PertAtom PA{};
     if (po) PA = pert_atom(p, J, o);
-    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)p.gen_cnt[k];
+    // entries written: a group that outgrew its slot has only cap of them until the refill
+    int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)min(p.gen_cnt[k], p.gen_cap[k]);
     for (int32_t e = e0 + lane0; e < e1; e += stride) {
         uint32_t ent = p.gen_ent[e];
         bool lng = (ent & ENT_LONG) != 0;
@@ -2170,7 +2173,7 @@ void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream
     hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, mixed);
 }
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
-    if (mode == 0) hipLaunchKernelGGL(k_caps, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    if (mode == 0 || mode == 2) hipLaunchKernelGGL(k_caps, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, mode == 2 ? 1 : 0);
     else hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s) {

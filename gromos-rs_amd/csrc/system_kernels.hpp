@@ -112,7 +112,8 @@ void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
 // per-job pair counts into job_pairs (cleared by the caller)
 void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s);
-// mode 0: capacities from the previous counts (kept by k_nbl); 1: capacities = counts
+// mode 0: capacities from the previous counts (kept by k_nbl) with slack; 1:
+// capacities = counts; 2: the previous counts without slack (test hook)
 void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s);
 void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s);
 // form: VV_LJ00 (LJ on the first-first atom pair only) | VV_NEUTRAL (neutral

@@ -130,6 +130,8 @@ def _declare(L):
     L.gromos_amd_system_vv_lanes.restype = I32
     L.gromos_amd_system_graph_launches.argtypes = [P]
     L.gromos_amd_system_graph_launches.restype = I64
+    L.gromos_amd_system_rebuild_stats.argtypes = [P, P, P]
+    L.gromos_amd_system_rebuild_stats.restype = C.c_int
     L.gromos_amd_session_create.argtypes = [C.POINTER(SessionParams), C.POINTER(P)]
     L.gromos_amd_session_destroy.argtypes = [P]
     L.gromos_amd_job_footprint.argtypes = [C.POINTER(Topology), C.POINTER(Params)]
@@ -311,6 +313,13 @@ class DeviceSystem:
     def graph_launches(self):
         """md steps run as a replayed graph so far"""
         return lib().gromos_amd_system_graph_launches(self.h)
+
+    @property
+    def rebuild_stats(self):
+        """(pair-list builds, builds that refilled with exact slots)"""
+        a, b = I64(), I64()
+        check(lib().gromos_amd_system_rebuild_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def kernel_timing(self, enable=-1):
         ms, n = D(), I64()

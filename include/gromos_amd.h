@@ -247,6 +247,12 @@ int32_t gromos_amd_system_vv_lanes(const gromos_amd_system *sys);
  * per-kernel timing) replay one captured graph each; by default every step is
  * launched kernel by kernel (faster on ROCm 7.2).  No counterpart in the reference. */
 int64_t gromos_amd_system_graph_launches(const gromos_amd_system *sys);
+/* Pair-list builds so far, and how many of them refilled their lists with exact
+ * slots (a charge group outgrew the slot sized from its previous count).
+ * GROMOS_AMD_TIGHT_SLOTS=1 in the environment sizes slots at exactly the
+ * previous counts (a test hook: most builds then refill).  No counterpart in the
+ * reference. */
+int gromos_amd_system_rebuild_stats(const gromos_amd_system *sys, int64_t *rebuilds, int64_t *refills);
 
 /* ------------------------------------------------------------------ */
 /* 3. sessions: continuous batching of independent jobs on one device  */

@@ -471,6 +471,27 @@ def test_side_stream_matches_single_stream(monkeypatch):
         assert np.array_equal(out["1"][k], out["0"][k]), k
 
 
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_refilled_lists_match(prec, monkeypatch):
+    """Slots sized at exactly the previous counts make most pair-list builds
+    overflow and refill, including builds inside md steps, whose first pair
+    kernels were already enqueued on the truncated lists and run again: the
+    results equal those of the default slots bit for bit."""
+    s = gen.fep_job(2500, 6, 0.4, seed=12)
+    out = {}
+    for tight in ("1", "0"):
+        monkeypatch.setenv("GROMOS_AMD_TIGHT_SLOTS", tight)
+        ds = rt.DeviceSystem(s, precision=prec)
+        ds.pairlist_update()
+        e = np.concatenate([ds.md(12), ds.md(14)])
+        x, v = ds.get_state()
+        out[tight] = (e, x, v, ds.rebuild_stats)
+        ds.close()
+    assert out["1"][3][1] >= 3 and out["0"][3][1] == 0, (out["1"][3], out["0"][3])
+    for k in range(3):
+        assert np.array_equal(out["1"][k], out["0"][k]), k
+
+
 def test_autotuner_picks_a_variant():
     s = gen.spc_water_box(3000, seed=13)
     ds = rt.DeviceSystem(s, precision=rt.MIXED)

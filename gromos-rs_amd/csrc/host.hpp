@@ -237,15 +237,17 @@ struct gromos_amd_system {
             p = (T*)b.ptr;
             return 0;
         }
+        size_t grown = 0;   // a buffer that outgrows its capacity at least doubles
         if (b.ptr) {
             HIPCHK(hipStreamSynchronize(stream));
             HIPCHK(hipFree(b.ptr));
             bytes -= (int64_t)b.bytes;
+            grown = 2 * b.bytes;
             b.ptr = nullptr;
             b.bytes = 0;
             p = nullptr;
         }
-        size_t cap = slack ? need + need / 4 + 4096 : need;
+        size_t cap = slack ? std::max(need + need / 4 + 4096, grown) : need;
         if (hipMalloc(&b.ptr, cap) != hipSuccess) {
             b.ptr = nullptr;
             return gamd::fail(GROMOS_AMD_ENOMEM, "hipMalloc of %zu bytes failed", cap);

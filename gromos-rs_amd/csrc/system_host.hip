@@ -592,14 +592,12 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipMemsetAsync(S->d_gen_cnt, 0, (nc + 1) * sizeof(int64_t), st));
     HIPCHK(hipMemsetAsync(S->d_vv_cap, 0, (nc + 1) * sizeof(int64_t), st));
     HIPCHK(hipMemsetAsync(S->d_gen_cap, 0, (nc + 1) * sizeof(int64_t), st));
-    // scratch for the library sort / scans
-    size_t t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
-                                              S->ncg, 0, S->cell_bits, st));
+    // scratch for the library scans
+    size_t t2 = 0, t3 = 0, t4 = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, S->d_cell_count, S->d_cell_start, S->ncell + 1, st));
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, S->d_s_cg_n, S->d_s_cg_scan, S->ncg + 1, st));
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, S->d_vv_cnt, S->d_vv_off, S->ncg + 1, st));
-    size_t tb = std::max(std::max(t1, t2), std::max(t3, t4));
+    size_t tb = std::max(t2, std::max(t3, t4));
     if ((rc = S->reserve(S->d_temp, tb, true))) return rc;
     S->temp_bytes = S->bufs[(void*)&S->d_temp].bytes;
     HIPCHK(hipStreamSynchronize(st));
@@ -692,10 +690,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
     launch_cg_ref(p, ncg, (int32_t)nctl, st);
     size_t tb = S->temp_bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_cell_count, S->d_cell_start, S->ncell + 1, st));
-    tb = S->temp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(S->d_temp, tb, S->d_key_in, S->d_key_out, S->d_val_in, S->d_val_out,
-                                              ncg, 0, S->cell_bits, st));
-    launch_sorted_cg(p, ncg, S->ncell + 1, st);
+    launch_sort_cg(p, ncg, S->ncell, st);
     tb = S->temp_bytes;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(S->d_temp, tb, S->d_s_cg_n, S->d_s_cg_scan, ncg + 1, st));
     // sorted maps and the sorted position copies (the gather of this order)

@@ -37,7 +37,7 @@ static inline dim3 xcd_grid(int64_t nblocks) {
 // (pairlist.rs:304-307, 359-367).  Cell key from the wrapped point.
 // Also clears the build's control block (nctl words: counters, overflow flag,
 // per-job pair counts) and counts the groups per cell (cell_count is zero here:
-// cleared at layout and by the previous build's k_sorted_cg).
+// cleared at layout and counted back to zero by the previous build's k_scatter).
 __global__ void k_cg_ref(SysPtrs p, int32_t ncg, int32_t nctl) {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncg) return;
@@ -78,23 +78,49 @@ __global__ void k_cg_ref(SysPtrs p, int32_t ncg, int32_t nctl) {
     }
     const uint32_t key = (uint32_t)(J.cell_off + cc[0] + J.ncell[0] * (cc[1] + J.ncell[1] * cc[2]));
     p.key_in[c] = key;
-    p.val_in[c] = (uint32_t)c;
     atomicAdd(p.cell_count + key, 1);
 }
 
-// sorted position k -> original group, atom count, sorted copy of the reference
-// point; clears cell_count (ncl words, already scanned into cell_start) for the
-// next build
-__global__ void k_sorted_cg(SysPtrs p, int32_t ncg, int32_t ncl) {
-    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncg) return;
-    for (int32_t w = k; w < ncl; w += ncg) p.cell_count[w] = 0;
-    int32_t c = (int32_t)p.val_out[k];
-    p.s_cg_orig[k] = c;
-    p.s_cg_n[k] = p.cg_n[c];
-    p.s_ref[3 * k] = p.ref[3 * c];
-    p.s_ref[3 * k + 1] = p.ref[3 * c + 1];
-    p.s_ref[3 * k + 2] = p.ref[3 * c + 2];
+// Counting sort of the groups by cell key, stable (ascending original index
+// within a cell, so a job's order does not depend on the batch).  k_scatter
+// drops each group into its cell's range at an arbitrary position (counting
+// cell_count back to zero for the next build); k_cell_order then ranks each
+// cell's members by original index, a wave per cell, and writes the sorted
+// group arrays: original group, atom count, reference point.
+__global__ void k_scatter(SysPtrs p, int32_t ncg) {
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncg) return;
+    const uint32_t key = p.key_in[c];
+    const int32_t r = atomicSub(p.cell_count + key, 1) - 1;
+    p.val_out[p.cell_start[key] + r] = (uint32_t)c;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_order(SysPtrs p, int32_t ncell) {
+    const int32_t cell = (int32_t)blockIdx.x * (BLOCK / WAVE) + (int32_t)(threadIdx.x / WAVE);
+    if (cell >= ncell) return;   // wave-uniform
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int32_t s = p.cell_start[cell], n = p.cell_start[cell + 1] - s;
+    auto put = [&](int32_t k, int32_t c) {
+        p.s_cg_orig[k] = c;
+        p.s_cg_n[k] = p.cg_n[c];
+        p.s_ref[3 * k] = p.ref[3 * c];
+        p.s_ref[3 * k + 1] = p.ref[3 * c + 1];
+        p.s_ref[3 * k + 2] = p.ref[3 * c + 2];
+    };
+    if (n <= WAVE) {
+        const int32_t v = lane < n ? (int32_t)p.val_out[s + lane] : INT32_MAX;
+        int32_t rank = 0;
+        for (int32_t j = 0; j < n; ++j) rank += __shfl(v, j, WAVE) < v ? 1 : 0;
+        if (lane < n) put(s + rank, v);
+    } else {
+        // large cells (few cells per axis, or no periodic box): quadratic ranks
+        for (int32_t i = lane; i < n; i += WAVE) {
+            const int32_t v = (int32_t)p.val_out[s + i];
+            int32_t rank = 0;
+            for (int32_t j = 0; j < n; ++j) rank += (int32_t)p.val_out[s + j] < v ? 1 : 0;
+            put(s + rank, v);
+        }
+    }
 }
 
 // Sorted atom layout: job j's groups fill [atom_off, atom_off + n_atoms) in
@@ -2166,8 +2192,10 @@ static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) 
 void launch_cg_ref(const SysPtrs& p, int32_t ncg, int32_t nctl, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_ref, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, nctl);
 }
-void launch_sorted_cg(const SysPtrs& p, int32_t ncg, int32_t ncl, hipStream_t s) {
-    hipLaunchKernelGGL(k_sorted_cg, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, ncl);
+void launch_sort_cg(const SysPtrs& p, int32_t ncg, int32_t ncell, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
+    constexpr int32_t WPB = BLOCK / WAVE;
+    hipLaunchKernelGGL(k_cell_order, dim3((unsigned)((ncell + WPB - 1) / WPB)), dim3(BLOCK), 0, s, p, ncell);
 }
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream_t s) {
     hipLaunchKernelGGL(k_sorted_atoms, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, mixed);

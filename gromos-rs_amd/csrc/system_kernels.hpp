@@ -105,8 +105,9 @@ struct SysPtrs {
 
 // nctl: words of the build's control block (counters) that k_cg_ref clears
 void launch_cg_ref(const SysPtrs& p, int32_t ncg, int32_t nctl, hipStream_t s);
-// ncl: cell_count words to clear for the next build
-void launch_sorted_cg(const SysPtrs& p, int32_t ncg, int32_t ncl, hipStream_t s);
+// stable counting sort of the groups by cell (cell_start scanned from cell_count)
+// into s_cg_orig, s_cg_n, s_ref
+void launch_sort_cg(const SysPtrs& p, int32_t ncg, int32_t ncell, hipStream_t s);
 // mixed: also the FP32 offset copies xf
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream_t s);
 void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);

@@ -146,10 +146,15 @@ def force_error(rt, s):
         out[prec] = ds.forcefield_apply()
         ds.close()
     f64, fmx = out[rt.FP64]["forces"], out[rt.MIXED]["forces"]
-    d = np.linalg.norm(fmx - f64, axis=1) / np.maximum(np.linalg.norm(f64, axis=1), 1.0)
+    dn = np.linalg.norm(fmx - f64, axis=1)
+    d = dn / np.maximum(np.linalg.norm(f64, axis=1), 1.0)
+    rms = float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))
     e64 = out[rt.FP64]["e_lj"] + out[rt.FP64]["e_crf"]
     emx = out[rt.MIXED]["e_lj"] + out[rt.MIXED]["e_crf"]
+    # max / median of |dF_i| / max(|F_i|, 1 kJ/mol/nm) per atom (the stated measure), and
+    # for reference the largest |dF_i| against the RMS force of the system
     return dict(max=float(d.max()), median=float(np.median(d)), f_floor=1.0,
+                max_abs_over_rms_force=float(dn.max() / rms), rms_force=rms,
                 energy_rel=float(abs(emx - e64) / abs(e64)))
 
 

@@ -82,10 +82,12 @@ struct PinnedArena {
     int reset(size_t bytes, size_t parts) {
         const size_t need = bytes + (parts + 1) * 256;
         if (need > cap) {
+            // an arena that grows at least doubles (pinning is slow; a session's
+            // batches grow job by job)
+            const size_t c = cap ? std::max(2 * need, 2 * cap) : need + need / 4;
             if (base) HIPCHK(hipHostFree(base));
             base = nullptr;
             cap = 0;
-            const size_t c = need + need / 4;
             if (hipHostMalloc((void**)&base, c, hipHostMallocDefault) != hipSuccess) {
                 base = nullptr;
                 return ::gamd::fail(GROMOS_AMD_ENOMEM, "hipHostMalloc of %zu bytes failed", c);
@@ -237,7 +239,7 @@ struct gromos_amd_system {
             p = (T*)b.ptr;
             return 0;
         }
-        size_t grown = 0;   // a buffer that outgrows its capacity at least doubles
+        size_t grown = 0;   // a buffer that outgrows its capacity: twice the need, at least double
         if (b.ptr) {
             HIPCHK(hipStreamSynchronize(stream));
             HIPCHK(hipFree(b.ptr));
@@ -247,7 +249,7 @@ struct gromos_amd_system {
             b.bytes = 0;
             p = nullptr;
         }
-        size_t cap = slack ? std::max(need + need / 4 + 4096, grown) : need;
+        size_t cap = slack ? (grown ? std::max(2 * need, grown) : need + need / 4 + 4096) : need;
         if (hipMalloc(&b.ptr, cap) != hipSuccess) {
             b.ptr = nullptr;
             return gamd::fail(GROMOS_AMD_ENOMEM, "hipMalloc of %zu bytes failed", cap);

@@ -1428,13 +1428,30 @@ __device__ __forceinline__ void gen_atom32(const SysPtrs& p, int32_t a, int32_t 
     if (po) PA = pert_atom(p, J, o);
     // entries written: a group that outgrew its slot has only cap of them until the refill
     int32_t e0 = (int32_t)p.gen_off[k], e1 = e0 + (int32_t)min(p.gen_cnt[k], p.gen_cap[k]);
-    for (int32_t e = e0 + lane0; e < e1; e += stride) {
-        uint32_t ent = p.gen_ent[e];
+    // software pipeline over this lane's entries: the entry word two ahead and the
+    // partner range one ahead are loaded while the current entry is evaluated
+    // (the loop is a chain of dependent loads otherwise)
+    int32_t e = e0 + lane0;
+    uint32_t ent1 = e < e1 ? p.gen_ent[e] : 0u;
+    uint32_t ent2 = e + stride < e1 ? p.gen_ent[e + stride] : 0u;
+    int32_t sm1 = 0, nm1 = 0;
+    if (e < e1) {
+        sm1 = p.s_cg_start[ent1 & ENT_IDX];
+        nm1 = p.s_cg_n[ent1 & ENT_IDX];
+    }
+    for (; e < e1; e += stride) {
+        const uint32_t ent = ent1;
+        const int32_t sm = sm1, nm = nm1;
+        ent1 = ent2;
+        if (e + stride < e1) {
+            sm1 = p.s_cg_start[ent1 & ENT_IDX];
+            nm1 = p.s_cg_n[ent1 & ENT_IDX];
+        }
+        if (e + 2 * stride < e1) ent2 = p.gen_ent[e + 2 * stride];
         bool lng = (ent & ENT_LONG) != 0;
         int32_t m = (int32_t)(ent & ENT_IDX);
         float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
         bool pchk = !SOLV && jp && p.cg_pert[p.s_cg_orig[canon != 0.f ? k : m]];
-        int32_t sm = p.s_cg_start[m], nm = p.s_cg_n[m];
         if (lng != LONG) {
             // cached long pairs: the correction is taken every step at the current
             // positions (forcefield.rs:796-830), each atom its own share (FP64)

@@ -323,10 +323,18 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
         if (s->clock % nsnb == 0)
             if (int rc = boundary(s)) return rc;
         // one md call up to the next admission boundary (a job that finishes on the
-        // way is held by its stop step), or to the next NSNB boundary while jobs
-        // wait in the queue (an idle batch takes them in at once)
+        // way is held by its stop step); while jobs wait in the queue, no further
+        // than the first NSNB boundary at which every resident job has finished
+        // (an idle batch takes them in at once)
         const int32_t ai = s->prm.admit_interval;
-        const int32_t span = s->queue.empty() ? ai - (int32_t)(s->clock % ai) : nsnb - (int32_t)(s->clock % nsnb);
+        int32_t span = ai - (int32_t)(s->clock % ai);
+        if (!s->queue.empty()) {
+            int64_t idle_in = 0;   // steps until no resident job runs
+            for (const SJob* j : s->batch)
+                if (j->state == GROMOS_AMD_JOB_RUNNING && !j->evict) idle_in = std::max<int64_t>(idle_in, j->nsteps - j->done);
+            const int64_t to_idle = ((s->clock + idle_in + nsnb - 1) / nsnb) * nsnb - s->clock;
+            span = (int32_t)std::min<int64_t>(span, std::max<int64_t>(to_idle, nsnb - s->clock % nsnb));
+        }
         int32_t k = std::min<int32_t>(nsteps, span);
         gromos_amd_system* S = s->S.get();
         if (S && running_count(s) > 0) {

@@ -193,6 +193,11 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
     __shared__ double sx[NBL_PIECE], sy[NBL_PIECE], sz[NBL_PIECE];
     __shared__ int32_t sm[NBL_PIECE], scm[NBL_PIECE], sst[NBL_PIECE], snn[NBL_PIECE];
     __shared__ int32_t run_m0[NBL_RUNS], run_off[NBL_RUNS + 1];
+    // each wave's own groups (reference point, original index), read back with a
+    // uniform address in the candidate loop: a broadcast through LDS instead of
+    // lane reads on the vector ALU
+    __shared__ double s_kx[NBL_WAVES][WAVE], s_ky[NBL_WAVES][WAVE], s_kz[NBL_WAVES][WAVE];
+    __shared__ int32_t s_ck[NBL_WAVES][WAVE];
     const int32_t c = xcd_block(ncell);
     if (c < 0) return;
     const int32_t s0 = p.cell_start[c], s1 = p.cell_start[c + 1];
@@ -263,6 +268,10 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
         const int32_t ck = p.s_cg_orig[k], nk = p.s_cg_n[k];
         const uint64_t own_mask = __ballot(own);
         const uint64_t solk_mask = __ballot(own && ck - cg_off < nsol);
+        s_kx[wv][lane] = kx;   // read after the barrier that opens the first piece
+        s_ky[wv][lane] = ky;
+        s_kz[wv][lane] = kz;
+        s_ck[wv][lane] = ck;
         int32_t vv_base = 0, gen_base = 0, vv_end = 0, gen_end = 0;
         if (FILL && own) {
             vv_base = (int32_t)p.vv_off[k];
@@ -307,8 +316,8 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
                     const int32_t t = (int32_t)__builtin_ctzll(rest);
                     rest &= rest - 1;
                     const int32_t i = rb + wv + NBL_WAVES * t;
-                    const d3 rk{bcast(kx, t), bcast(ky, t), bcast(kz, t)};
-                    const int32_t ck_t = bcast(ck, t);
+                    const d3 rk{s_kx[wv][t], s_ky[wv][t], s_kz[wv][t]};
+                    const int32_t ck_t = s_ck[wv][t];
                     const bool sol_k = (solk_mask >> t) & 1ull;
                     const d3 r = nim(box, rk, rm);
                     const double d2 = r.x * r.x + r.y * r.y + r.z * r.z;

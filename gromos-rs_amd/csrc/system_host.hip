@@ -576,8 +576,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         S->h_ctl_n = 6 + 2 * (size_t)njobs;
     }
     // set once per layout, kept by every build: padding atoms' sorted slots map to
-    // no atom; cell counts start at zero (each build's k_sorted_cg clears them
-    // for the next); sorted copies of padding atoms are zero
+    // no atom; cell counts start at zero (each build's k_scatter counts them back
+    // to zero for the next); sorted copies of padding atoms are zero
     HIPCHK(hipMemsetAsync(S->d_s2o, 0xFF, npa * sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_cell_count, 0, ncl * sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_xq, 0, npa * sizeof(double4), st));

@@ -1,13 +1,15 @@
-// Kernels of the device-resident path (plain FP64 form).
+// Kernels of the device-resident path (plain FP64 and mixed-precision forms).
 //
-//  pair list      k_cg_ref / k_cell_hist / k_sorted_cg / k_sorted_atoms /
-//                 k_nbl<FILL>        StandardPairlistAlgorithm semantics
+//  pair list      k_cg_ref / k_scatter / k_cell_order / k_sorted_atoms / k_caps /
+//                 k_nbl<FILL> / k_totals / k_job_pairs
+//                                    StandardPairlistAlgorithm semantics
 //                                    (pairlist.rs:183-442) on a cell grid
-//  forces         k_gather, k_vv<APS>, k_gen<LONG>, k_excl14, k_assemble
+//  forces         k_gather, k_vv / k_vv_mx, k_gen / k_gen_sv / k_gen_mx / k_gen_sol,
+//                 k_excl14, k_assemble
 //                                    (innerloops.rs:105-309, excluded.rs:25-206,
 //                                     forcefield.rs:420-756)
-//  step           k_leapfrog, k_settle, k_ekin, k_reduce
-//                                    (leap_frog.rs:37-148, constraints.rs:815-1036,
+//  step           k_therm, k_integrate (leap-frog, SETTLE, E_kin), k_vv_parts,
+//                 k_reduce           (leap_frog.rs:37-148, constraints.rs:815-1036,
 //                                     temperature.rs:63-87)
 #include "system_kernels.hpp"
 

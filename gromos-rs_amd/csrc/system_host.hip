@@ -382,7 +382,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         for (int32_t a = 0; a < im.J.n_solute_atoms; ++a) sol_atom[o.sol + a] = A + a;
     };
     {
-        const int nt = (int)std::min<int64_t>(std::min(njobs, 8), std::max<int64_t>(1, npad / 65536));
+        // a thread per 32k atoms, at most one per job and at most 16 (or the cores)
+        const int64_t hw = std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)std::thread::hardware_concurrency()));
+        const int nt = (int)std::min<int64_t>(std::min<int64_t>(njobs, hw), std::max<int64_t>(1, npad / 32768));
         if (nt <= 1) {
             for (int j = 0; j < njobs; ++j) fill(j);
         } else {

@@ -173,6 +173,29 @@ def test_resident_pairlist_and_forces_fixture(name):
     ds.close()
 
 
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_vacuum_single_cell(prec):
+    """No periodic box: every charge group lies in the job's one cell, more of
+    them than a wave holds (the cell sort's large-cell branch); the pair set is
+    the oracle's and the forces match (FP64 at the parity tolerance, mixed at
+    the mixed-precision one)."""
+    s = gen.spc_water_box(150, seed=21)
+    s.boundary = 0
+    o = oracle.OracleSystem(s)
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    assert_same_lists(ds.pairlist(), oracle.pairlist(o))
+    g = ds.forcefield_apply()
+    r = oracle.single_point(o)
+    ds.close()
+    if prec == rt.FP64:
+        assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+        assert force_err(g["forces"], r["forces"]) < F_TOL
+    else:
+        assert rel(g["e_lj"] + g["e_crf"], r["e_lj"] + r["e_crf"]) < 1e-5
+        assert force_err(g["forces"], r["forces"]) < 1e-4
+
+
 def test_water216_cutoff_beyond_half_box():
     s = gen.water_216()
     o = oracle.OracleSystem(s)

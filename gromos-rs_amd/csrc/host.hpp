@@ -84,13 +84,18 @@ struct PinnedArena {
         if (need > cap) {
             // an arena that grows at least doubles (pinning is slow; a session's
             // batches grow job by job)
-            const size_t c = cap ? std::max(2 * need, 2 * cap) : need + need / 4;
+            size_t c = cap ? std::max(2 * need, 2 * cap) : need + need / 4;
             if (base) HIPCHK(hipHostFree(base));
             base = nullptr;
             cap = 0;
             if (hipHostMalloc((void**)&base, c, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
                 base = nullptr;
-                return ::gamd::fail(GROMOS_AMD_ENOMEM, "hipHostMalloc of %zu bytes failed", c);
+                c = need + need / 4;   // the plain slack before giving up
+                if (hipHostMalloc((void**)&base, c, hipHostMallocDefault) != hipSuccess) {
+                    base = nullptr;
+                    return ::gamd::fail(GROMOS_AMD_ENOMEM, "hipHostMalloc of %zu bytes failed", c);
+                }
             }
             cap = c;
         }
@@ -249,10 +254,18 @@ struct gromos_amd_system {
             b.bytes = 0;
             p = nullptr;
         }
-        size_t cap = slack ? (grown ? std::max(2 * need, grown) : need + need / 4 + 4096) : need;
+        const size_t base = slack ? need + need / 4 + 4096 : need;
+        size_t cap = slack && grown ? std::max(2 * need, grown) : base;
         if (hipMalloc(&b.ptr, cap) != hipSuccess) {
+            // the doubled growth may not fit next to the rest of a full session:
+            // fall back to the plain slack before giving up
+            (void)hipGetLastError();
             b.ptr = nullptr;
-            return gamd::fail(GROMOS_AMD_ENOMEM, "hipMalloc of %zu bytes failed", cap);
+            if (cap == base || hipMalloc(&b.ptr, base) != hipSuccess) {
+                b.ptr = nullptr;
+                return gamd::fail(GROMOS_AMD_ENOMEM, "hipMalloc of %zu bytes failed", base);
+            }
+            cap = base;
         }
         b.bytes = cap;
         bytes += (int64_t)cap;

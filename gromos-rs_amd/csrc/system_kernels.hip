@@ -269,6 +269,7 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
         const double kx = p.s_ref[3 * k], ky = p.s_ref[3 * k + 1], kz = p.s_ref[3 * k + 2];
         const int32_t ck = p.s_cg_orig[k], nk = p.s_cg_n[k];
         const uint64_t own_mask = __ballot(own);
+        const int32_t n_own = (int32_t)__popcll(own_mask);   // a prefix of the lanes
         const uint64_t solk_mask = __ballot(own && ck - cg_off < nsol);
         s_kx[wv][lane] = kx;   // read after the barrier that opens the first piece
         s_ky[wv][lane] = ky;
@@ -313,10 +314,8 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
                 const bool sol_m = cm - cg_off < nsol;
                 const uint32_t vv_ent = (uint32_t)sst[qq];
                 const int32_t n_m = snn[qq];
-                uint64_t rest = own_mask;
-                while (rest) {
-                    const int32_t t = (int32_t)__builtin_ctzll(rest);
-                    rest &= rest - 1;
+                // the wave's own groups are lanes 0 .. n_own - 1 (gi = wv + 4 lane < nr)
+                for (int32_t t = 0; t < n_own; ++t) {
                     const int32_t i = rb + wv + NBL_WAVES * t;
                     const d3 rk{s_kx[wv][t], s_ky[wv][t], s_kz[wv][t]};
                     const int32_t ck_t = s_ck[wv][t];

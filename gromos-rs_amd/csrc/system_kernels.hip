@@ -772,6 +772,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
             for (int c = 0; c < 11; ++c) en[c] = 0.0;
             float msx = 0.f, msy = 0.f, msz = 0.f;   // NEUT: sum of partner dipoles
             float mcx = 0.f, mcy = 0.f, mcz = 0.f;   //       over canonical entries
+            float els = 0.f, ecs = 0.f;              // canonical entries' energies
             const int32_t g1 = min(g0 + VV_GROUP, n);
             for (int32_t e = e0 + g0; e < e0 + g1; ++e) {
                 uint32_t ent = p.vv_ent[e];
@@ -859,13 +860,15 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
                     f[i][1] += (double)gys;
                     f[i][2] += (double)gzs;
                 }
-                en[0] += (double)(canon * el);
-                en[1] += (double)(canon * ec);
+                els = fmaf(canon, el, els);
+                ecs = fmaf(canon, ec, ecs);
                 if (VIR) {
 #pragma unroll
                     for (int c = 0; c < 9; ++c) en[2 + c] += (double)(canon * w[c]);
                 }
             }
+            en[0] += (double)els;
+            en[1] += (double)ecs;
             if (NEUT) {
                 const double ms[3] = {(double)msx, (double)msy, (double)msz};
                 const double mc[3] = {(double)mcx, (double)mcy, (double)mcz};

@@ -780,9 +780,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
                 const float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
                 float sx, sy, sz;
                 shift32(box, pk0, xyz(p.xq[sm]), sx, sy, sz);
+                // a first atom's offset from itself is exactly zero (every writer
+                // of xf subtracts the same FP64 value), so atom 0's slot is not
+                // read and its zero offsets are left out of the arithmetic below
                 float4 db[APS];
+                db[0] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                for (int j = 0; j < APS; ++j) db[j] = p.xf[sm + j];
+                for (int j = 1; j < APS; ++j) db[j] = p.xf[sm + j];
                 if (NEUT) {
                     float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
@@ -802,11 +806,11 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
                 float w[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int i = 0; i < APS; ++i) {
-                    float bx = sx + da[i].x, by = sy + da[i].y, bz = sz + da[i].z;
+                    const float bx = i ? sx + da[i].x : sx, by = i ? sy + da[i].y : sy, bz = i ? sz + da[i].z : sz;
                     float gxs = 0.f, gys = 0.f, gzs = 0.f;
 #pragma unroll
                     for (int j = 0; j < APS; ++j) {
-                        float rx = bx - db[j].x, ry = by - db[j].y, rz = bz - db[j].z;
+                        const float rx = j ? bx - db[j].x : bx, ry = j ? by - db[j].y : by, rz = j ? bz - db[j].z : bz;
                         float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
                         bool ok = !(r2 < 1e-10f);
                         // hardware reciprocal square root (1 ulp) refined by one Newton step

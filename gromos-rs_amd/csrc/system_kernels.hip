@@ -190,6 +190,28 @@ __device__ __forceinline__ double bcast(double v, int32_t t) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// nearest_image for the search, the same vector as nim(): rint (one instruction)
+// equals the reference's half-away rounding except at exact halves (where
+// x - rint(x), exact, is +-0.5), which take round() in a branch that the wave
+// skips unless one of its lanes has one
+__device__ __forceinline__ d3 nim_search(const Box& b, d3 ri, d3 rj) {
+    d3 r{ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
+    if (b.kind) {
+        const double ux = r.x * b.invL[0], uy = r.y * b.invL[1], uz = r.z * b.invL[2];
+        double nx = rint(ux), ny = rint(uy), nz = rint(uz);
+        const bool tie = fabs(ux - nx) == 0.5 || fabs(uy - ny) == 0.5 || fabs(uz - nz) == 0.5;
+        if (__builtin_expect(tie, 0)) {
+            nx = round(ux);
+            ny = round(uy);
+            nz = round(uz);
+        }
+        r.x = r.x - b.L[0] * nx;
+        r.y = r.y - b.L[1] * ny;
+        r.z = r.z - b.L[2] * nz;
+    }
+    return r;
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
     __shared__ double sx[NBL_PIECE], sy[NBL_PIECE], sz[NBL_PIECE];
@@ -320,7 +342,7 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
                     const d3 rk{s_kx[wv][t], s_ky[wv][t], s_kz[wv][t]};
                     const int32_t ck_t = s_ck[wv][t];
                     const bool sol_k = (solk_mask >> t) & 1ull;
-                    const d3 r = nim(box, rk, rm);
+                    const d3 r = nim_search(box, rk, rm);
                     const double d2 = r.x * r.x + r.y * r.y + r.z * r.z;
                     const bool in = valid && m != i && !(d2 > c2l);
                     const bool lng = d2 > c2s;

@@ -2090,6 +2090,24 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
 }
 
 
+// One thread's rows of a part, the flags fixed so that the unrolled body has no
+// branch and several rows' loads are in flight at once (the additions keep
+// their order)
+template <bool VIR, bool KIN>
+__device__ __forceinline__ void vv_part_rows(const SysPtrs& p, int64_t k0, int64_t k1, double (&v)[PART_W]) {
+#pragma unroll 4
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        const double* r = p.vv_e + 11 * k;
+#pragma unroll
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) v[c] += r[c];
+        if (KIN) {
+            v[1] += p.cg_ek[3 * k + 2];   // solvent excluded RF energy (md steps)
+            v[11] += p.cg_ek[3 * k];
+            v[12] += p.cg_ek[3 * k + 1];
+        }
+    }
+}
+
 // Per-group solvent-solvent records summed in VV_PARTS contiguous parts per job
 // (grid VV_PARTS x njobs), so that the per-job reduction reads few values.
 // Per job and part: columns 0-10 from the solvent-solvent records (sorted
@@ -2100,15 +2118,12 @@ __global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_viri
     int64_t n = J.n_cg;
     int64_t k0 = J.cg_off + n * blockIdx.x / VV_PARTS, k1 = J.cg_off + n * (blockIdx.x + 1) / VV_PARTS;
     double v[PART_W] = {0};
-    const int nc = with_virial ? 11 : 2;
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const double* r = p.vv_e + 11 * k;
-        for (int c = 0; c < nc; ++c) v[c] += r[c];
-        if (with_kin) {
-            v[1] += p.cg_ek[3 * k + 2];   // solvent excluded RF energy (md steps)
-            v[11] += p.cg_ek[3 * k];
-            v[12] += p.cg_ek[3 * k + 1];
-        }
+    if (with_virial) {
+        if (with_kin) vv_part_rows<true, true>(p, k0, k1, v);
+        else vv_part_rows<true, false>(p, k0, k1, v);
+    } else {
+        if (with_kin) vv_part_rows<false, true>(p, k0, k1, v);
+        else vv_part_rows<false, false>(p, k0, k1, v);
     }
     block_sum<PART_W>(v, lds);
     if (threadIdx.x < PART_W)
@@ -2159,6 +2174,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         int32_t b0 = J.atom_off / per;
         // md steps write the excluded-pair rows of the solute atoms' blocks only
         int32_t b1 = (J.atom_off + (s == SLOT_EXCL && advance ? J.n_solute_atoms : J.n_atoms) + per - 1) / per;
+#pragma unroll 2
         for (int32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
             const double* r = p.rows + ((int64_t)s * stride + b) * ROW;
 #pragma unroll

@@ -196,6 +196,28 @@ def test_vacuum_single_cell(prec):
         assert force_err(g["forces"], r["forces"]) < 1e-4
 
 
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_search_exact_half_box_ties(prec):
+    """An exact lattice (64 waters, L = 2 nm, first atoms on multiples of 0.5 nm,
+    cutoff 1.4 nm > L/2): many first-atom separations are exactly L/2 along an
+    axis, where x / L = +-0.5 and the search's nearest image takes its
+    half-away branch (the reference's f64::round).  The pair sets are the
+    oracle's in both precisions and the FP64 forces match.  (The mixed force
+    kernels' shift rounds such ties to even, the other image of an exactly
+    equidistant pair, so their forces are not compared on this input.)"""
+    s = gen.spc_water_box(64, box=2.0, jitter=0.0, seed=5, rcutp=1.4, rcutl=1.4, name="lattice_64")
+    o = oracle.OracleSystem(s)
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    assert_same_lists(ds.pairlist(), oracle.pairlist(o))
+    if prec == rt.FP64:
+        g = ds.forcefield_apply()
+        r = oracle.single_point(o)
+        assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+        assert force_err(g["forces"], r["forces"]) < F_TOL
+    ds.close()
+
+
 def test_water216_cutoff_beyond_half_box():
     s = gen.water_216()
     o = oracle.OracleSystem(s)

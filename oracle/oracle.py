@@ -323,6 +323,69 @@ def perturbed_interaction(r, a_c6, a_c12, b_c6, b_c12, a_q, b_q, alj, acrf, lp, 
     return out
 
 
+def _corr(n):
+    f = np.zeros(3 * n)
+    pc = PertCorr(f.ctypes.data_as(C.POINTER(D)), 0.0, 0.0, 0.0)
+    return f, pc
+
+
+def _corr_out(f, pc):
+    return dict(forces=f.reshape(-1, 3).copy(), de_lj=pc.de_lj, de_crf=pc.de_crf, dhdl=pc.dhdl)
+
+
+def _crf_of(s):
+    return crf_new(s.rcutl, 1.0, s.epsrf, s.kappa)
+
+
+def pert_pairlist_correction(osys, pairs, lp, pos=None):
+    """perturbed_pairlist_correction (perturbed.rs:309-386) over one atom-pair list."""
+    s = osys.s
+    k = osys._keep
+    x = c64(s.pos if pos is None else pos)
+    pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1)
+    f, pc = _corr(s.n_atoms)
+    crf = _crf_of(s)
+    lib().or_perturbed_pairlist_correction(ptr(x), ptr(pr), len(pr) // 2, C.byref(osys.pert), ptr(k["charge"]),
+                                           ptr(k["iac"]), ptr(k["lj"]), s.n_types, C.byref(crf), C.byref(lp),
+                                           C.byref(osys.pbc), s.fpepsi, s.n_atoms, C.byref(pc))
+    return _corr_out(f, pc)
+
+
+def pert_self_correction(osys, lp):
+    """perturbed_self_energy_correction (perturbed.rs:395-425): (delta E_crf, dH/dlambda)."""
+    s = osys.s
+    crf = _crf_of(s)
+    de, dh = D(), D()
+    lib().or_perturbed_self_energy_correction(C.byref(osys.topo), C.byref(osys.pert), C.byref(crf), C.byref(lp),
+                                              s.fpepsi, C.byref(de), C.byref(dh))
+    return de.value, dh.value
+
+
+def _pass(name, osys, lp, pos):
+    s = osys.s
+    x = c64(s.pos if pos is None else pos)
+    f, pc = _corr(s.n_atoms)
+    crf = _crf_of(s)
+    getattr(lib(), name)(C.byref(osys.topo), ptr(x), C.byref(osys.pert), C.byref(crf), C.byref(lp),
+                         C.byref(osys.pbc), s.fpepsi, C.byref(pc))
+    return _corr_out(f, pc)
+
+
+def pert_excluded_correction(osys, lp, pos=None):
+    """perturbed_excluded_correction (perturbed.rs:431-511)."""
+    return _pass("or_perturbed_excluded_correction", osys, lp, pos)
+
+
+def pert_one_four_correction(osys, lp, pos=None):
+    """perturbed_one_four_correction (perturbed.rs:517-605)."""
+    return _pass("or_perturbed_one_four_correction", osys, lp, pos)
+
+
+def pert_atom_pair_correction(osys, lp, pos=None):
+    """perturbed_atom_pair_correction (perturbed.rs:624-707)."""
+    return _pass("or_perturbed_atom_pair_correction", osys, lp, pos)
+
+
 class ForceField:
     """Forcefield::apply restated (nonbonded only)."""
 

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 
 #include "explicit.hpp"
@@ -256,6 +257,134 @@ extern "C" int gromos_amd_perturbed_pairlist_correction(const double* pos, const
     a.forces = forces;
     a.out = out;
     HIPCHK(launch_pert_pairs(a, (hipStream_t)stream));
+    return 0;
+}
+
+// common part of the standalone correction passes: the per-atom perturbation
+// arrays, CRF, lambda, box and outputs
+static int pert_common(PertArgs& a, const gromos_amd_pert* pert, const double* charges,
+                       const gromos_amd_crf* crf, const gromos_amd_lambda* lp, int32_t boundary,
+                       const double* box, double fpe, double* forces, double* out, bool need_forces) {
+    if (int rc = check_box(boundary, box)) return rc;
+    if (!pert || !pert->flag || !pert->a_q || !pert->b_q || !pert->alpha_lj || !pert->alpha_crf || !charges ||
+        !crf || !lp || !out || (need_forces && !forces))
+        return fail(GROMOS_AMD_EINVAL, "null argument");
+    a.flag = pert->flag;
+    a.a_iac = pert->a_iac;
+    a.b_iac = pert->b_iac;
+    a.a_q = pert->a_q;
+    a.b_q = pert->b_q;
+    a.alpha_lj = pert->alpha_lj;
+    a.alpha_crf = pert->alpha_crf;
+    a.q = charges;
+    a.crf = to_crf(crf);
+    std::memcpy(&a.lp, lp, sizeof(gromos_amd_lambda));
+    a.box = make_box(boundary, box);
+    a.fpepsi = fpe;
+    a.forces = forces;
+    a.out = out;
+    return 0;
+}
+
+extern "C" int gromos_amd_perturbed_self_energy_correction(const gromos_amd_pert* pert, int32_t n_atoms,
+                                                           const gromos_amd_crf* crf, const gromos_amd_lambda* lp,
+                                                           double fpe, double* out, void* stream) {
+    static const double no_box[3] = {0.0, 0.0, 0.0};
+    static const double no_charge = 0.0;  // the self term reads only the A / B charges
+    PertArgs a{};
+    if (int rc = pert_common(a, pert, &no_charge, crf, lp, 0, no_box, fpe, nullptr, out, false)) return rc;
+    if (n_atoms < 0) return fail(GROMOS_AMD_EINVAL, "n_atoms < 0");
+    a.n_atoms = n_atoms;
+    HIPCHK(launch_pert_self(a, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int gromos_amd_perturbed_excluded_correction(const int32_t* excl_off, const int32_t* excl,
+                                                        int32_t n_atoms, int32_t n_solute, const double* pos,
+                                                        const gromos_amd_pert* pert, const double* charges,
+                                                        const gromos_amd_crf* crf, const gromos_amd_lambda* lp,
+                                                        int32_t boundary, const double box[3], double fpe,
+                                                        double* forces, double* out, void* stream) {
+    PertArgs a{};
+    if (int rc = pert_common(a, pert, charges, crf, lp, boundary, box, fpe, forces, out, true)) return rc;
+    if (!pos || !excl_off || n_atoms < 0 || n_solute < 0 || n_solute > n_atoms)
+        return fail(GROMOS_AMD_EINVAL, "bad exclusion arguments (n_atoms %d, n_solute %d)", n_atoms, n_solute);
+    a.pos = pos;
+    a.off = excl_off;
+    a.idx = excl;
+    a.n_atoms = n_atoms;
+    a.n_solute = n_solute;
+    HIPCHK(launch_pert_excluded(a, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int gromos_amd_perturbed_one_four_correction(const int32_t* o14_off, const int32_t* o14,
+                                                        int32_t n_atoms, const double* pos,
+                                                        const gromos_amd_pert* pert, const double* charges,
+                                                        const uint32_t* iac, const double* lj, int32_t n_types,
+                                                        const gromos_amd_crf* crf, const gromos_amd_lambda* lp,
+                                                        int32_t boundary, const double box[3], double fpe,
+                                                        double* forces, double* out, void* stream) {
+    PertArgs a{};
+    if (int rc = pert_common(a, pert, charges, crf, lp, boundary, box, fpe, forces, out, true)) return rc;
+    if (!pos || !o14_off || !iac || !lj || !pert->a_iac || !pert->b_iac || n_atoms < 0 || n_types < 1)
+        return fail(GROMOS_AMD_EINVAL, "bad 1-4 arguments");
+    a.pos = pos;
+    a.off = o14_off;
+    a.idx = o14;
+    a.n_atoms = n_atoms;
+    a.iac = iac;
+    a.lj = lj;
+    a.n_types = n_types;
+    HIPCHK(launch_pert_one_four(a, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int gromos_amd_perturbed_atom_pair_correction(const int32_t* atom_pairs, int64_t n_atom_pairs,
+                                                         const int32_t* o14_off, const int32_t* o14,
+                                                         int32_t n_atoms, const double* pos, const double* charges,
+                                                         const uint32_t* iac, const double* lj, int32_t n_types,
+                                                         const gromos_amd_crf* crf, const gromos_amd_lambda* lp,
+                                                         int32_t boundary, const double box[3], double fpe,
+                                                         double* forces, double* out, void* stream) {
+    if (int rc = check_box(boundary, box)) return rc;
+    if (n_atom_pairs < 0 || (n_atom_pairs > 0 && !atom_pairs) || !o14_off || !pos || !charges || !iac || !lj ||
+        !crf || !lp || !forces || !out || n_atoms < 0 || n_types < 1)
+        return fail(GROMOS_AMD_EINVAL, "null or bad argument");
+    PertArgs a{};
+    a.atom_pairs = atom_pairs;
+    a.n = n_atom_pairs;
+    a.off = o14_off;
+    a.idx = o14;
+    a.n_atoms = n_atoms;
+    a.pos = pos;
+    a.q = charges;
+    a.iac = iac;
+    a.lj = lj;
+    a.n_types = n_types;
+    a.crf = to_crf(crf);
+    std::memcpy(&a.lp, lp, sizeof(gromos_amd_lambda));
+    a.box = make_box(boundary, box);
+    a.fpepsi = fpe;
+    a.forces = forces;
+    a.out = out;
+    HIPCHK(launch_pert_atom_pairs(a, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int gromos_amd_perturbed_lj_crf_interaction(const double* r, const double* prm, int64_t n,
+                                                       const gromos_amd_lambda* lp, const gromos_amd_crf* crf,
+                                                       double fpe, double* out, void* stream) {
+    if (n < 0 || (n > 0 && (!r || !prm || !out)) || !lp || !crf) return fail(GROMOS_AMD_EINVAL, "null or bad argument");
+    SoftPairArgs a{};
+    a.r = r;
+    a.prm = prm;
+    a.n = n;
+    std::memcpy(&a.lp, lp, sizeof(gromos_amd_lambda));
+    a.crf = to_crf(crf);
+    a.fpepsi = fpe;
+    a.out = out;
+    HIPCHK(launch_soft_pairs(a, (hipStream_t)stream));
     return 0;
 }
 

@@ -178,6 +178,103 @@ __device__ __forceinline__ void pert_lj_crf64(double x, double y, double z, doub
                      soft_rf(alpha_crf, lp, crf), fpepsi, o);
 }
 
+// One pair's contribution to a perturbed correction pass: the scalar force
+// delta along r (force delta = r * f), the energy deltas and dH/dlambda.
+struct PertDelta {
+    double f, de_lj, de_crf, dhdl;
+};
+
+// perturbed_self_energy_correction, one perturbed atom (perturbed.rs:406-422):
+// the RF self term of the soft-core kernel at r = 0 minus the state-A one.
+__device__ __forceinline__ void pert_self64(double a_q, double b_q, const Lambda& lp, const Crf& crf,
+                                            double fpepsi, double& de, double& dhdl) {
+    double qa2 = a_q * a_q, qb2 = b_q * b_q;
+    double e_rf = (-(lp.a_crf_lambda_n * qa2) - lp.b_crf_lambda_n * qb2 + qa2) * crf.crf_cut * fpepsi;
+    double de_rf = (double)lp.lambda_exp * (lp.a_crf_lambda_n_1 * qa2 - lp.b_crf_lambda_n_1 * qb2) * crf.crf_cut *
+                   fpepsi;
+    de = 0.5 * e_rf;
+    dhdl = 0.5 * de_rf;
+}
+
+// perturbed_excluded_correction, one excluded pair at squared distance r2
+// (perturbed.rs:445-510): soft RF cutoff scaled by sqrt((rc^2 / cut2soft)^3),
+// no 1/r term; a_q, b_q are the pair's charge products, acrf its alpha_crf.
+__device__ __forceinline__ PertDelta pert_excl64(double r2, double a_q, double b_q, double acrf, const Lambda& lp,
+                                                 const Crf& crf, double fpepsi) {
+    double cut2 = crf.cutoff_sq;
+    double a_cut2soft = cut2 + acrf * lp.b_crfs_lambda2;
+    double b_cut2soft = cut2 + acrf * lp.a_crfs_lambda2;
+    double ra = cut2 / a_cut2soft, rb = cut2 / b_cut2soft;
+    double a_crf_2cut3i = crf.crf_2cut3i * sqrt(ra * (ra * ra));
+    double b_crf_2cut3i = crf.crf_2cut3i * sqrt(rb * (rb * rb));
+    double a_crf_cut3i = 2.0 * a_crf_2cut3i, b_crf_cut3i = 2.0 * b_crf_2cut3i;
+    double a_crf_pert = 3.0 * a_crf_2cut3i / a_cut2soft, b_crf_pert = 3.0 * b_crf_2cut3i / b_cut2soft;
+    double a_e_crf = a_q * (-a_crf_2cut3i * r2 - crf.crf_cut);
+    double b_e_crf = b_q * (-b_crf_2cut3i * r2 - crf.crf_cut);
+    double e_crf_p = (lp.a_crf_lambda_n * a_e_crf + lp.b_crf_lambda_n * b_e_crf) * fpepsi;
+    double f_p = (lp.a_crf_lambda_n * a_q * a_crf_cut3i + lp.b_crf_lambda_n * b_q * b_crf_cut3i) * fpepsi;
+    double e_crf_a = a_q * fpepsi * (-crf.crf_2cut3i * r2 - crf.crf_cut);
+    double f_a = a_q * fpepsi * crf.crf_cut3i;
+    double n = (double)lp.lambda_exp;
+    double dcrf = ((lp.a_crf_lambda_n * a_q * lp.b_crfs_lambda * a_crf_pert -
+                    lp.b_crf_lambda_n * b_q * lp.a_crfs_lambda * b_crf_pert) * r2 * acrf +
+                   n * (lp.b_crf_lambda_n_1 * b_e_crf - lp.a_crf_lambda_n_1 * a_e_crf)) * fpepsi;
+    return PertDelta{f_p - f_a, 0.0, e_crf_p - e_crf_a, dcrf};
+}
+
+// perturbed_one_four_correction, one 1-4 pair with r2 >= 1e-10
+// (perturbed.rs:565-604): soft-core kernel on the cs6/cs12 entries of both
+// states minus the state-A 1-4 term (coulomb scaling 1).
+__device__ __forceinline__ PertDelta pert_14_64(double x, double y, double z, double r2, const double* la,
+                                                const double* lb, double a_q, double b_q, double alj, double acrf,
+                                                const Lambda& lp, const Crf& crf, double fpepsi) {
+    double inv_r2 = 1.0 / r2;
+    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    double inv_r = sqrt(inv_r2);
+    double e_lj_a = (la[3] * inv_r6 - la[2]) * inv_r6;
+    double f_lj_a = (12.0 * la[3] * inv_r6 - 6.0 * la[2]) * inv_r6 * inv_r2;
+    double e_crf_a = a_q * fpepsi * (inv_r - crf.crf_2cut3i * r2 - crf.crf_cut);
+    double f_crf_a = a_q * fpepsi * (inv_r * inv_r2 + crf.crf_cut3i);
+    double o[5];
+    pert_lj_crf64(x, y, z, la[2], la[3], lb[2], lb[3], a_q, b_q, alj, acrf, lp, crf, fpepsi, o);
+    return PertDelta{o[0] - (f_lj_a + f_crf_a), o[1] - e_lj_a, o[2] - e_crf_a, o[3] + o[4]};
+}
+
+// perturbed_atom_pair_correction, one PERTATOMPAIR entry with r2 >= 1e-10
+// (perturbed.rs:654-706).  lj = the atoms' regular LJ entry, qp = q_i q_j
+// fpepsi, at / bt = interaction type in A / B (0 = c6/c12, 1 = cs6/cs12,
+// -1 = absent in B).  The regular 1-4 term is the state the pair leaves.
+__device__ __forceinline__ PertDelta pert_atom_pair64(double r2, const double* lj, double qp, int32_t at, int32_t bt,
+                                                      const Lambda& lp, const Crf& crf) {
+    double inv_r2 = 1.0 / r2;
+    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+    double inv_r = sqrt(inv_r2);
+    double e_lj_reg = (lj[3] * inv_r6 - lj[2]) * inv_r6;
+    double f_lj_reg = (12.0 * lj[3] * inv_r6 - 6.0 * lj[2]) * inv_r6 * inv_r2;
+    double e_crf_reg = qp * (inv_r - crf.crf_2cut3i * r2 - crf.crf_cut);
+    double f_crf_reg = qp * (inv_r * inv_r2 + crf.crf_cut3i);
+    double c6a = at == 0 ? lj[0] : lj[2], c12a = at == 0 ? lj[1] : lj[3];
+    double e_lj_a = (c12a * inv_r6 - c6a) * inv_r6;
+    double f_lj_a = (12.0 * c12a * inv_r6 - 6.0 * c6a) * inv_r6 * inv_r2;
+    double e_lj_b = 0.0, f_lj_b = 0.0, e_crf_b = 0.0, f_crf_b = 0.0;
+    if (bt >= 0) {
+        double c6b = bt == 0 ? lj[0] : lj[2], c12b = bt == 0 ? lj[1] : lj[3];
+        e_lj_b = (c12b * inv_r6 - c6b) * inv_r6;
+        f_lj_b = (12.0 * c12b * inv_r6 - 6.0 * c6b) * inv_r6 * inv_r2;
+        e_crf_b = e_crf_reg;
+        f_crf_b = f_crf_reg;
+    }
+    double n = (double)lp.lambda_exp;
+    double e_lj_p = lp.a_lj_lambda_n * e_lj_a + lp.b_lj_lambda_n * e_lj_b;
+    double e_crf_p = lp.a_crf_lambda_n * e_crf_reg + lp.b_crf_lambda_n * e_crf_b;
+    double f_lj_p = lp.a_lj_lambda_n * f_lj_a + lp.b_lj_lambda_n * f_lj_b;
+    double f_crf_p = lp.a_crf_lambda_n * f_crf_reg + lp.b_crf_lambda_n * f_crf_b;
+    double dlj = n * (lp.b_lj_lambda_n_1 * e_lj_b - lp.a_lj_lambda_n_1 * e_lj_a);
+    double dcrf = n * (lp.b_crf_lambda_n_1 * e_crf_b - lp.a_crf_lambda_n_1 * e_crf_reg);
+    return PertDelta{(f_lj_p - f_lj_reg) + (f_crf_p - f_crf_reg), e_lj_p - e_lj_reg, e_crf_p - e_crf_reg,
+                     dlj + dcrf};
+}
+
 // ---- wave / block reductions (deterministic order) ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

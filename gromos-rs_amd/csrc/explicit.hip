@@ -224,6 +224,148 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_pert_pairs(PertArgs a) {
     if (threadIdx.x < 3) atomicAdd(a.out + threadIdx.x, lds[16 * 3 + threadIdx.x]);
 }
 
+// dual-topology parameters of the pair (i, j), either atom perturbed
+// (perturbed.rs:458-470, 531-562): a perturbed atom takes its A / B type and
+// charge, an unperturbed one its regular ones in both states; alpha is the
+// perturbed atom's, or the mean of both when both are perturbed.
+struct PairAB {
+    const double* la;
+    const double* lb;
+    double a_q, b_q, alj, acrf;
+};
+__device__ __forceinline__ PairAB pair_ab(const PertArgs& a, int64_t i, int64_t j, bool pi, bool pj) {
+    PairAB q{nullptr, nullptr, 0.0, 0.0, 0.0, 0.0};
+    if (a.lj) {  // the excluded pass has no LJ part (and no type arrays)
+        int a_iac_i = pi ? a.a_iac[i] : (int)a.iac[i], b_iac_i = pi ? a.b_iac[i] : (int)a.iac[i];
+        int a_iac_j = pj ? a.a_iac[j] : (int)a.iac[j], b_iac_j = pj ? a.b_iac[j] : (int)a.iac[j];
+        q.la = a.lj + 4 * ((int64_t)a_iac_i * a.n_types + a_iac_j);
+        q.lb = a.lj + 4 * ((int64_t)b_iac_i * a.n_types + b_iac_j);
+    }
+    q.a_q = (pi ? a.a_q[i] : a.q[i]) * (pj ? a.a_q[j] : a.q[j]);
+    q.b_q = (pi ? a.b_q[i] : a.q[i]) * (pj ? a.b_q[j] : a.q[j]);
+    if (pi && pj) {
+        q.alj = (a.alpha_lj[i] + a.alpha_lj[j]) * 0.5;
+        q.acrf = (a.alpha_crf[i] + a.alpha_crf[j]) * 0.5;
+    } else {
+        int64_t w = pi ? i : j;
+        q.alj = a.alpha_lj[w];
+        q.acrf = a.alpha_crf[w];
+    }
+    return q;
+}
+
+__device__ __forceinline__ void flush3(double (&v)[3], double* out, int nv) {
+    __shared__ double lds[17 * 3];
+    block_sum<3>(v, lds);
+    if ((int)threadIdx.x < nv) atomicAdd(out + threadIdx.x, lds[16 * 3 + threadIdx.x]);
+}
+
+// perturbed_self_energy_correction (perturbed.rs:395-425): one lane per atom
+__global__ void __launch_bounds__(EXPL_BLOCK) k_pert_self(PertArgs a) {
+    double v[3] = {0, 0, 0};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_atoms && a.flag[i]) pert_self64(a.a_q[i], a.b_q[i], a.lp, a.crf, a.fpepsi, v[0], v[1]);
+    flush3(v, a.out, 2);
+}
+
+// perturbed_excluded_correction (perturbed.rs:431-511): one lane per solute
+// atom i, its excluded partners j > i with at least one of the two perturbed
+__global__ void __launch_bounds__(EXPL_BLOCK) k_pert_excluded(PertArgs a) {
+    double v[3] = {0, 0, 0};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_solute) {
+        const bool pi = a.flag[i] != 0;
+        d3 xi = ld3(a.pos, i);
+        for (int32_t k = a.off[i]; k < a.off[i + 1]; ++k) {
+            int64_t j = a.idx[k];
+            if (j <= i) continue;
+            const bool pj = a.flag[j] != 0;
+            if (!pi && !pj) continue;
+            PairAB q = pair_ab(a, i, j, pi, pj);
+            d3 r = nim(a.box, xi, ld3(a.pos, j));
+            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            PertDelta d = pert_excl64(r2, q.a_q, q.b_q, q.acrf, a.lp, a.crf, a.fpepsi);
+            double fx = r.x * d.f, fy = r.y * d.f, fz = r.z * d.f;
+            atomic_add3(a.forces, i, fx, fy, fz);
+            atomic_add3(a.forces, j, -fx, -fy, -fz);
+            v[1] += d.de_crf;
+            v[2] += d.dhdl;
+        }
+    }
+    flush3(v, a.out, 3);
+}
+
+// perturbed_one_four_correction (perturbed.rs:517-605): one lane per atom i,
+// its 1-4 partners j > i with at least one of the two perturbed
+__global__ void __launch_bounds__(EXPL_BLOCK) k_pert_one_four(PertArgs a) {
+    double v[3] = {0, 0, 0};
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_atoms) {
+        const bool pi = a.flag[i] != 0;
+        d3 xi = ld3(a.pos, i);
+        for (int32_t k = a.off[i]; k < a.off[i + 1]; ++k) {
+            int64_t j = a.idx[k];
+            if (j <= i) continue;
+            const bool pj = a.flag[j] != 0;
+            if (!pi && !pj) continue;
+            PairAB q = pair_ab(a, i, j, pi, pj);
+            d3 r = nim(a.box, xi, ld3(a.pos, j));
+            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            if (r2 < 1e-10) continue;
+            PertDelta d = pert_14_64(r.x, r.y, r.z, r2, q.la, q.lb, q.a_q, q.b_q, q.alj, q.acrf, a.lp, a.crf,
+                                     a.fpepsi);
+            double fx = r.x * d.f, fy = r.y * d.f, fz = r.z * d.f;
+            atomic_add3(a.forces, i, fx, fy, fz);
+            atomic_add3(a.forces, j, -fx, -fy, -fz);
+            v[0] += d.de_lj;
+            v[1] += d.de_crf;
+            v[2] += d.dhdl;
+        }
+    }
+    flush3(v, a.out, 3);
+}
+
+// perturbed_atom_pair_correction (perturbed.rs:624-707): one lane per
+// PERTATOMPAIR entry; entries that are not 1-4 pairs are skipped
+__global__ void __launch_bounds__(EXPL_BLOCK) k_pert_atom_pairs(PertArgs a) {
+    double v[3] = {0, 0, 0};
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.n) {
+        const int32_t* ap = a.atom_pairs + 4 * k;
+        int64_t i = ap[0], j = ap[1];
+        bool in14 = false;
+        for (int32_t t = a.off[i]; t < a.off[i + 1]; ++t) in14 |= a.idx[t] == j;
+        for (int32_t t = a.off[j]; t < a.off[j + 1]; ++t) in14 |= a.idx[t] == i;
+        if (in14) {
+            d3 r = nim(a.box, ld3(a.pos, i), ld3(a.pos, j));
+            double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            if (r2 >= 1e-10) {
+                const double* lj = a.lj + 4 * ((int64_t)a.iac[i] * a.n_types + a.iac[j]);
+                PertDelta d = pert_atom_pair64(r2, lj, a.q[i] * a.q[j] * a.fpepsi, ap[2], ap[3], a.lp, a.crf);
+                double fx = r.x * d.f, fy = r.y * d.f, fz = r.z * d.f;
+                atomic_add3(a.forces, i, fx, fy, fz);
+                atomic_add3(a.forces, j, -fx, -fy, -fz);
+                v[0] = d.de_lj;
+                v[1] = d.de_crf;
+                v[2] = d.dhdl;
+            }
+        }
+    }
+    flush3(v, a.out, 3);
+}
+
+// perturbed_lj_crf_interaction (perturbed.rs:124-266) on independent pairs
+__global__ void __launch_bounds__(EXPL_BLOCK) k_soft_pairs(SoftPairArgs a) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const double* r = a.r + 3 * k;
+    const double* q = a.prm + 8 * k;
+    double o[5];
+    pert_lj_crf64(r[0], r[1], r[2], q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], a.lp, a.crf, a.fpepsi, o);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a.out[5 * k + c] = o[c];
+}
+
 static int blocks_for(int64_t n) { return (int)((n + EXPL_BLOCK - 1) / EXPL_BLOCK); }
 
 hipError_t launch_plain(const ExplArgs& a, hipStream_t s) {
@@ -254,6 +396,32 @@ hipError_t launch_one_four(const ExclArgs& a, hipStream_t s) {
 hipError_t launch_pert_pairs(const PertArgs& a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pert_pairs, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pert_self(const PertArgs& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pert_self, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_pert_excluded(const PertArgs& a, hipStream_t s) {
+    if (a.n_solute <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pert_excluded, dim3(blocks_for(a.n_solute)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_pert_one_four(const PertArgs& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pert_one_four, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_pert_atom_pairs(const PertArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pert_atom_pairs, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_soft_pairs(const SoftPairArgs& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_soft_pairs, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

@@ -61,7 +61,24 @@ struct PertArgs {
     Box box;
     double fpepsi;
     double* forces;
-    double* out;  // delta_e_lj, delta_e_crf, dhdl
+    double* out;  // delta_e_lj, delta_e_crf, dhdl (self pass: delta_e, dhdl)
+    // excluded / 1-4 passes: CSR over atoms; atom-pair pass: entries and 1-4 CSR
+    const int32_t* off;
+    const int32_t* idx;
+    int32_t n_atoms;
+    int32_t n_solute;
+    const int32_t* atom_pairs;  // i, j, a_type, b_type per entry
+};
+
+// perturbed_lj_crf_interaction over n independent pairs
+struct SoftPairArgs {
+    const double* r;    // 3 per pair
+    const double* prm;  // a_c6, a_c12, b_c6, b_c12, a_q, b_q, alpha_lj, alpha_crf per pair
+    int64_t n;
+    Lambda lp;
+    Crf crf;
+    double fpepsi;
+    double* out;        // f, e_lj, e_crf, de_lj, de_crf per pair
 };
 
 hipError_t launch_plain(const ExplArgs& a, hipStream_t s);
@@ -70,5 +87,10 @@ hipError_t launch_solvent(const ExplArgs& a, hipStream_t s);
 hipError_t launch_rf_excluded(const ExclArgs& a, hipStream_t s);
 hipError_t launch_one_four(const ExclArgs& a, hipStream_t s);
 hipError_t launch_pert_pairs(const PertArgs& a, hipStream_t s);
+hipError_t launch_pert_self(const PertArgs& a, hipStream_t s);
+hipError_t launch_pert_excluded(const PertArgs& a, hipStream_t s);
+hipError_t launch_pert_one_four(const PertArgs& a, hipStream_t s);
+hipError_t launch_pert_atom_pairs(const PertArgs& a, hipStream_t s);
+hipError_t launch_soft_pairs(const SoftPairArgs& a, hipStream_t s);
 
 }  // namespace gamd

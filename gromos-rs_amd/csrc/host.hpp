@@ -50,9 +50,16 @@ struct JobImage {
     std::vector<int32_t> pa_iac, pb_iac, papairs;
     std::vector<double> pa_q, pb_q, palj, pacrf;
     int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
+    int64_t list_entries = 0;       // full neighbour-list entries of the job (estimate)
 };
 
+// The pair kernels index the neighbour lists with 32-bit offsets: a batch's
+// list entries (either list) must stay below this.
+constexpr int64_t MAX_LIST_ENTRIES = INT32_MAX;
+
 int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int job, JobImage& out);
+// GROMOS_AMD_ENOMEM unless both list sizes fit the kernels' 32-bit offsets
+int check_list_size(int64_t vv_entries, int64_t gen_entries);
 
 }  // namespace gamd
 

@@ -13,6 +13,7 @@
 // steps replicas one after another on the CPU (remd.rs:160-201); here all jobs
 // step together in one set of launches.
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <deque>
 #include <map>
@@ -129,6 +130,7 @@ struct gromos_amd_session {
     std::map<int64_t, std::unique_ptr<SJob>> jobs;
     std::deque<SJob*> queue;
     int64_t next_id = 1, reserved = 0, clock = 0, layouts = 0, pair_count = 0;
+    int64_t reserved_entries = 0;   // estimated neighbour-list entries of queued and resident jobs
     double layout_ms = 0.0;
     double concat_ms_gone = 0.0;   // concatenation time of batches already released
     DevScratch stage, map, small;
@@ -148,9 +150,18 @@ static int running_count(const gromos_amd_session* s) {
     return n;
 }
 
+// a job's admission reservations end
+static void release(gromos_amd_session* s, SJob* j) {
+    s->reserved -= j->img.footprint;
+    s->reserved_entries -= j->img.list_entries;
+}
+
 // Change the resident batch: jobs done or evicted leave (done jobs' final
 // state is copied to the host), queued jobs join.  Continuing jobs' positions
-// and velocities move to their new offsets on the device.
+// and velocities move to their new offsets on the device.  If the new layout
+// fails (device memory), the continuing jobs stop as FAILED with their last
+// state copied to the host, the queued jobs stay queued, and the device batch
+// is released; the error is returned.
 static int relayout(gromos_amd_session* s) {
     auto t0 = std::chrono::steady_clock::now();
     gromos_amd_system* S = s->S.get();
@@ -183,9 +194,10 @@ static int relayout(gromos_amd_session* s) {
     }
     for (SJob* j : leave) {
         j->slot = -1;
-        s->reserved -= j->img.footprint;
+        release(s, j);
         if (j->evict) s->jobs.erase(j->id);
     }
+    std::deque<SJob*> queued = s->queue;
     s->queue.clear();
     s->batch.clear();
     if (nb.empty()) {
@@ -201,8 +213,41 @@ static int relayout(gromos_amd_session* s) {
         S->step_timing = s->step_timing;
     }
     std::vector<const JobImage*> imgs;
-    for (SJob* j : nb) imgs.push_back(&j->img);
-    if (int rc = layout_batch(S, imgs, s->prm.precision)) return rc;
+    int64_t batch_atoms = 0;
+    for (SJob* j : nb) {
+        imgs.push_back(&j->img);
+        batch_atoms += j->img.n;
+    }
+    // GROMOS_AMD_LAYOUT_ATOM_LIMIT (a test hook): a layout of more atoms fails
+    // as if device memory ran out, before anything is allocated
+    const char* lim = std::getenv("GROMOS_AMD_LAYOUT_ATOM_LIMIT");
+    const bool injected = lim && *lim && batch_atoms > std::atoll(lim);
+    if (int rc = injected ? fail(GROMOS_AMD_ENOMEM, "layout of %lld atoms over GROMOS_AMD_LAYOUT_ATOM_LIMIT",
+                                 (long long)batch_atoms)
+                          : layout_batch(S, imgs, s->prm.precision)) {
+        const std::string why = gromos_amd_last_error();
+        // the continuing jobs' state is in the session's staging copy
+        const double* sx = keep.empty() ? nullptr : (const double*)s->stage.ptr;
+        for (size_t k = 0; k < keep.size(); ++k) {
+            SJob* j = keep[k];
+            size_t n3 = (size_t)j->img.n * 3, off = (size_t)old_off[k] * 3;
+            j->pos.resize(n3);
+            j->vel.resize(n3);
+            if (hipMemcpy(j->pos.data(), sx + off, n3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(j->vel.data(), sx + (size_t)old_npad * 3 + off, n3 * sizeof(double),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                (void)hipGetLastError();
+            j->slot = -1;
+            j->state = GROMOS_AMD_JOB_FAILED;
+            j->error = "batch layout failed: " + why;
+            release(s, j);
+        }
+        s->queue = queued;
+        if (S) s->concat_ms_gone += S->concat_ms;
+        s->S.reset();
+        s->layouts++;
+        return fail(rc, "%s", why.c_str());
+    }
     hipStream_t st = S->stream;
     if (!keep.empty()) {
         std::vector<int32_t> mp(S->npad, -1);
@@ -300,6 +345,13 @@ extern "C" int gromos_amd_job_admit(gromos_amd_session* s, const gromos_amd_topo
     if (s->reserved + j->img.footprint > s->budget)
         return fail(GROMOS_AMD_EBUSY, "job_admit: %lld B reserved of %lld B, job needs %lld B", (long long)s->reserved,
                     (long long)s->budget, (long long)j->img.footprint);
+    // the batch's neighbour lists are indexed with 32-bit offsets
+    if (j->img.list_entries >= MAX_LIST_ENTRIES)
+        return fail(GROMOS_AMD_ENOMEM, "job_admit: about %lld neighbour-list entries exceed the 32-bit list offsets",
+                    (long long)j->img.list_entries);
+    if (s->reserved_entries + j->img.list_entries >= MAX_LIST_ENTRIES)
+        return fail(GROMOS_AMD_EBUSY, "job_admit: about %lld neighbour-list entries reserved, job needs %lld (limit %lld)",
+                    (long long)s->reserved_entries, (long long)j->img.list_entries, (long long)MAX_LIST_ENTRIES);
     size_t n3 = (size_t)t->n_atoms * 3;
     j->pos.assign(pos, pos + n3);
     if (vel) j->vel.assign(vel, vel + n3);
@@ -308,7 +360,10 @@ extern "C" int gromos_amd_job_admit(gromos_amd_session* s, const gromos_amd_topo
     j->id = s->next_id++;
     if (nsteps == 0) j->state = GROMOS_AMD_JOB_DONE;
     else s->queue.push_back(j.get());
-    if (nsteps > 0) s->reserved += j->img.footprint;
+    if (nsteps > 0) {
+        s->reserved += j->img.footprint;
+        s->reserved_entries += j->img.list_entries;
+    }
     *job_id = j->id;
     s->jobs[j->id] = std::move(j);
     return 0;
@@ -447,7 +502,7 @@ extern "C" int gromos_amd_job_evict(gromos_amd_session* s, int64_t id) {
     for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
         if (*it == j) {
             s->queue.erase(it);
-            s->reserved -= j->img.footprint;
+            release(s, j);
             break;
         }
     s->jobs.erase(id);

@@ -203,9 +203,11 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
             for (int c = 0; c < 4; ++c) im.papairs.push_back(e[c]);
         }
     }
-    // footprint: per-atom and per-group arrays plus the full neighbour lists
+    // footprint: per-atom and per-group arrays, the full neighbour lists
     // (partners within the list cutoff at this job's group density, both
-    // directions, 4 bytes each, 25 % reserve)
+    // directions, 4 bytes each, with the buffers' growth policy: 25 % slack,
+    // doubled when outgrown) and the energy records of md calls of up to 1,000
+    // steps (doubled likewise)
     int64_t npad = align_up(n, BLOCK);
     double ncg = T.n_cg;
     double partners = ncg;
@@ -215,8 +217,10 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
         partners = std::min(ncg, 4.18879 * rr * rr * rr * ncg / vol);
     }
     int64_t cells = (int64_t)J.ncell[0] * J.ncell[1] * J.ncell[2];
+    im.list_entries = (int64_t)(ncg * partners * 1.25);
     im.footprint = npad * BYTES_PER_ATOM + (int64_t)T.n_cg * BYTES_PER_CG + 8 * cells +
-                   (int64_t)(ncg * partners * 4.0 * 1.25) + (int64_t)(im.excl.size() + im.o14.size() * 2) * 4 + 4096;
+                   (int64_t)(ncg * partners * 4.0 * 2.5) + (int64_t)(im.excl.size() + im.o14.size() * 2) * 4 +
+                   (int64_t)(2.5 * 1000 * REC * sizeof(double)) + 4096;
     return 0;
 }
 
@@ -649,6 +653,15 @@ static int read_ctl(gromos_amd_system* S) {
     return parse_ctl(S);
 }
 
+// the pair kernels hold list offsets in 32 bits
+int check_list_size(int64_t vv, int64_t gen) {
+    if (vv >= MAX_LIST_ENTRIES || gen >= MAX_LIST_ENTRIES)
+        return fail(GROMOS_AMD_ENOMEM,
+                    "neighbour lists of %lld + %lld entries exceed the 32-bit list offsets (%lld); run fewer or "
+                    "smaller jobs per batch", (long long)vv, (long long)gen, (long long)MAX_LIST_ENTRIES);
+    return 0;
+}
+
 // lists with exact slots: the counts of the last search pass scanned into
 // offsets, buffers sized to the totals, a second fill pass
 static int refill_exact(gromos_amd_system* S) {
@@ -659,6 +672,7 @@ static int refill_exact(gromos_amd_system* S) {
     HIPCHK(hipMemcpyAsync(&tot[0], S->d_vv_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&tot[1], S->d_gen_off + ncg, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     if (int rc = read_ctl(S)) return rc;
+    if (int rc = check_list_size(tot[0], tot[1])) return rc;
     if (int rc = S->reserve(S->d_vv_ent, (size_t)tot[0], true)) return rc;
     if (int rc = S->reserve(S->d_gen_ent, (size_t)tot[1], true)) return rc;
     SysPtrs p = S->ptrs();
@@ -709,6 +723,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
         if (int rc = scan_offsets(S, S->d_vv_cap, S->d_gen_cap)) return rc;
         int64_t bv = S->prev_tot_vv + S->prev_tot_vv / 8 + 16 * (int64_t)ncg + 64;
         int64_t bg = S->prev_tot_gen + S->prev_tot_gen / 8 + 16 * (int64_t)ncg + 64;
+        if (int rc = check_list_size(bv, bg)) return rc;
         if (int rc = S->reserve(S->d_vv_ent, (size_t)bv, true)) return rc;
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();

@@ -1176,32 +1176,14 @@ __device__ __forceinline__ void pert_share32(const SysPtrs& p, const JobDev& J, 
 // perturbed_excluded_correction for one excluded pair i < j (perturbed.rs:445-510)
 __device__ __forceinline__ void pert_excl_delta(const SysPtrs& p, const JobDev& J, int32_t i, int32_t j, bool first,
                                                 double (&v)[ROW], double& fx, double& fy, double& fz) {
-    const Lambda& lp = J.lp;
     PertPair q = pert_params(p, J, i, j);
     d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
     double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
-    double cut2 = J.crf.cutoff_sq;
-    double a_cut2soft = cut2 + q.acrf * lp.b_crfs_lambda2;
-    double b_cut2soft = cut2 + q.acrf * lp.a_crfs_lambda2;
-    double ra = cut2 / a_cut2soft, rb = cut2 / b_cut2soft;
-    double a_crf_2cut3i = J.crf.crf_2cut3i * sqrt(ra * (ra * ra));
-    double b_crf_2cut3i = J.crf.crf_2cut3i * sqrt(rb * (rb * rb));
-    double a_crf_cut3i = 2.0 * a_crf_2cut3i, b_crf_cut3i = 2.0 * b_crf_2cut3i;
-    double a_crf_pert = 3.0 * a_crf_2cut3i / a_cut2soft, b_crf_pert = 3.0 * b_crf_2cut3i / b_cut2soft;
-    double a_e_crf = q.a_q * (-a_crf_2cut3i * r2 - J.crf.crf_cut);
-    double b_e_crf = q.b_q * (-b_crf_2cut3i * r2 - J.crf.crf_cut);
-    double e_crf_p = (lp.a_crf_lambda_n * a_e_crf + lp.b_crf_lambda_n * b_e_crf) * J.fpepsi;
-    double f_p = (lp.a_crf_lambda_n * q.a_q * a_crf_cut3i + lp.b_crf_lambda_n * q.b_q * b_crf_cut3i) * J.fpepsi;
-    double e_crf_a = q.a_q * J.fpepsi * (-J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-    double f_a = q.a_q * J.fpepsi * J.crf.crf_cut3i;
-    double n = (double)lp.lambda_exp;
-    double dcrf = ((lp.a_crf_lambda_n * q.a_q * lp.b_crfs_lambda * a_crf_pert -
-                    lp.b_crf_lambda_n * q.b_q * lp.a_crfs_lambda * b_crf_pert) * r2 * q.acrf +
-                   n * (lp.b_crf_lambda_n_1 * b_e_crf - lp.a_crf_lambda_n_1 * a_e_crf)) * J.fpepsi;
-    own_side(first, r, f_p - f_a, fx, fy, fz);
+    PertDelta d = pert_excl64(r2, q.a_q, q.b_q, q.acrf, J.lp, J.crf, J.fpepsi);
+    own_side(first, r, d.f, fx, fy, fz);
     if (first) {
-        v[1] += e_crf_p - e_crf_a;
-        v[11] += dcrf;
+        v[1] += d.de_crf;
+        v[11] += d.dhdl;
     }
 }
 
@@ -1212,21 +1194,12 @@ __device__ __forceinline__ void pert_14_delta(const SysPtrs& p, const JobDev& J,
     d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
     double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
     if (r2 < 1e-10) return;
-    double inv_r2 = 1.0 / r2;
-    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-    double inv_r = sqrt(inv_r2);
-    double e_lj_a = (q.la[3] * inv_r6 - q.la[2]) * inv_r6;
-    double f_lj_a = (12.0 * q.la[3] * inv_r6 - 6.0 * q.la[2]) * inv_r6 * inv_r2;
-    double e_crf_a = q.a_q * J.fpepsi * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-    double f_crf_a = q.a_q * J.fpepsi * (inv_r * inv_r2 + J.crf.crf_cut3i);
-    double o[5];
-    pert_lj_crf64(r.x, r.y, r.z, q.la[2], q.la[3], q.lb[2], q.lb[3], q.a_q, q.b_q, q.alj, q.acrf, J.lp, J.crf,
-                  J.fpepsi, o);
-    own_side(first, r, o[0] - (f_lj_a + f_crf_a), fx, fy, fz);
+    PertDelta d = pert_14_64(r.x, r.y, r.z, r2, q.la, q.lb, q.a_q, q.b_q, q.alj, q.acrf, J.lp, J.crf, J.fpepsi);
+    own_side(first, r, d.f, fx, fy, fz);
     if (first) {
-        v[0] += o[1] - e_lj_a;
-        v[1] += o[2] - e_crf_a;
-        v[11] += o[3] + o[4];
+        v[0] += d.de_lj;
+        v[1] += d.de_crf;
+        v[11] += d.dhdl;
     }
 }
 
@@ -1242,38 +1215,12 @@ __device__ __forceinline__ void pert_atom_pair_delta(const SysPtrs& p, const Job
     d3 r = nim(J.box, ld3(p.x_cur, i), ld3(p.x_cur, j));
     double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
     if (r2 < 1e-10) return;
-    const Lambda& lp = J.lp;
-    double inv_r2 = 1.0 / r2;
-    double inv_r6 = inv_r2 * inv_r2 * inv_r2;
-    double inv_r = sqrt(inv_r2);
     const double* lj = p.lj + J.lj_off + 4 * ((int64_t)p.iac[i] * J.n_types + p.iac[j]);
-    double e_lj_reg = (lj[3] * inv_r6 - lj[2]) * inv_r6;
-    double f_lj_reg = (12.0 * lj[3] * inv_r6 - 6.0 * lj[2]) * inv_r6 * inv_r2;
-    double qp = p.q[i] * p.q[j] * J.fpepsi;
-    double e_crf_reg = qp * (inv_r - J.crf.crf_2cut3i * r2 - J.crf.crf_cut);
-    double f_crf_reg = qp * (inv_r * inv_r2 + J.crf.crf_cut3i);
-    auto ljp = [&](int32_t ty, double& e, double& f) {
-        double c6 = ty == 0 ? lj[0] : lj[2], c12 = ty == 0 ? lj[1] : lj[3];
-        e = (c12 * inv_r6 - c6) * inv_r6;
-        f = (12.0 * c12 * inv_r6 - 6.0 * c6) * inv_r6 * inv_r2;
-    };
-    double e_lj_a, f_lj_a, e_lj_b = 0.0, f_lj_b = 0.0, e_crf_b = 0.0, f_crf_b = 0.0;
-    ljp(at, e_lj_a, f_lj_a);
-    if (bt >= 0) {
-        ljp(bt, e_lj_b, f_lj_b);
-        e_crf_b = e_crf_reg;
-        f_crf_b = f_crf_reg;
-    }
-    double n = (double)lp.lambda_exp;
-    double e_lj_p = lp.a_lj_lambda_n * e_lj_a + lp.b_lj_lambda_n * e_lj_b;
-    double f_lj_p = lp.a_lj_lambda_n * f_lj_a + lp.b_lj_lambda_n * f_lj_b;
-    double f_crf_p = lp.a_crf_lambda_n * f_crf_reg + lp.b_crf_lambda_n * f_crf_b;
-    double dlj = n * (lp.b_lj_lambda_n_1 * e_lj_b - lp.a_lj_lambda_n_1 * e_lj_a);
-    double dcrf = n * (lp.b_crf_lambda_n_1 * e_crf_b - lp.a_crf_lambda_n_1 * e_crf_reg);
-    own_side(first, r, (f_lj_p - f_lj_reg) + (f_crf_p - f_crf_reg), fx, fy, fz);
+    PertDelta d = pert_atom_pair64(r2, lj, p.q[i] * p.q[j] * J.fpepsi, at, bt, J.lp, J.crf);
+    own_side(first, r, d.f, fx, fy, fz);
     if (first) {
-        v[0] += e_lj_p - e_lj_reg;
-        v[11] += dlj + dcrf;
+        v[0] += d.de_lj;
+        v[11] += d.dhdl;
     }
 }
 
@@ -1763,13 +1710,10 @@ __global__ void __launch_bounds__(BLOCK) k_excl14(SysPtrs p, int32_t npad, int32
         if (sol_i) v[1] += -0.5 * qi * qi * J.fpepsi * J.crf.crf_cut;
         if (pi_) {
             // perturbed_self_energy_correction (perturbed.rs:395-425)
-            const Lambda& lp = J.lp;
-            double qa2 = p.pa_q[i] * p.pa_q[i], qb2 = p.pb_q[i] * p.pb_q[i];
-            double e_rf = (-(lp.a_crf_lambda_n * qa2) - lp.b_crf_lambda_n * qb2 + qa2) * J.crf.crf_cut * J.fpepsi;
-            double de_rf = (double)lp.lambda_exp * (lp.a_crf_lambda_n_1 * qa2 - lp.b_crf_lambda_n_1 * qb2) *
-                           J.crf.crf_cut * J.fpepsi;
-            v[1] += 0.5 * e_rf;
-            v[11] += 0.5 * de_rf;
+            double de, dh;
+            pert_self64(p.pa_q[i], p.pb_q[i], J.lp, J.crf, J.fpepsi, de, dh);
+            v[1] += de;
+            v[11] += dh;
         }
         for (int32_t k = p.excl_off[i]; k < p.excl_off[i + 1]; ++k) {
             int32_t j = p.excl[k];

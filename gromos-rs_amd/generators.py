@@ -212,11 +212,8 @@ def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5,
     lj, nt = _mixed_table(c6s, c12s)
     t_o, t_h = n_types, n_types + 1
     iac_chain = rng.integers(0, n_types, n_chain).astype(np.int32)
-    # water on a lattice, sites too close to the chain removed
-    side = int(np.ceil((n_waters * 1.25) ** (1 / 3)))
-    g = (np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * (L / side)
-    g += rng.uniform(-0.02, 0.02, g.shape)
-    keep = np.ones(len(g), bool)
+    # water on a lattice, sites too close to the chain removed; the lattice is
+    # refined until at least n_waters sites remain
     cw = chain - L * np.floor(chain / L)
     cell = 0.6
     nc = max(1, int(L // cell))
@@ -224,19 +221,27 @@ def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5,
     buckets = {}
     for idx, c in enumerate(map(tuple, ci)):
         buckets.setdefault(c, []).append(idx)
-    gi = np.floor(g / (L / nc)).astype(int) % nc
-    for s_idx in range(len(g)):
-        cx, cy, cz = gi[s_idx]
-        near = []
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    near += buckets.get(((cx + dx) % nc, (cy + dy) % nc, (cz + dz) % nc), [])
-        if near:
-            d = cw[near] - g[s_idx]
-            d -= L * np.round(d / L)
-            if ((d * d).sum(1) < 0.28 ** 2).any():
-                keep[s_idx] = False
+    side = int(np.ceil((n_waters * 1.25) ** (1 / 3)))
+    while True:
+        g = (np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * (L / side)
+        g += rng.uniform(-0.02, 0.02, g.shape)
+        keep = np.ones(len(g), bool)
+        gi = np.floor(g / (L / nc)).astype(int) % nc
+        for s_idx in range(len(g)):
+            cx, cy, cz = gi[s_idx]
+            near = []
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        near += buckets.get(((cx + dx) % nc, (cy + dy) % nc, (cz + dz) % nc), [])
+            if near:
+                d = cw[near] - g[s_idx]
+                d -= L * np.round(d / L)
+                if ((d * d).sum(1) < 0.28 ** 2).any():
+                    keep[s_idx] = False
+        if keep.sum() >= n_waters:
+            break
+        side += 1
     g = g[keep]
     if len(g) > n_waters:
         g = g[np.sort(rng.choice(len(g), n_waters, replace=False))]

@@ -2,7 +2,7 @@
 
 Same names, argument meaning and accumulate semantics as the Rust functions
 (crates/gromos-forces/src/nonbonded/innerloops.rs:318-733, excluded.rs:25-206,
-perturbed.rs:309-386, params.rs), with device tensors (torch, ROCm) in place of
+perturbed.rs:124-707, params.rs), with device tensors (torch, ROCm) in place of
 slices:
 
     storage = ForceStorage(n_atoms)            # zeros on the GPU
@@ -184,15 +184,84 @@ class PertNBCorrection:
         return float(self.out[2])
 
 
-def perturbed_pairlist_correction(pos, pairlist, pert, charges, iac, lj, crf, lp, bc, four_pi_eps_i, out):
-    """perturbed.rs:309-386; pert: system.Perturbation (host arrays, copied)"""
+def _pert(pert):
+    """PertAtomInfo per atom as device arrays (system.Perturbation, host arrays,
+    copied); returns the C struct and the tensors it points into."""
     torch = _torch()
     keep = [torch.as_tensor(np.asarray(pert.flag, np.uint8), device="cuda"),
             _u32(pert.a_iac), _u32(pert.b_iac), _f64(pert.a_q), _f64(pert.b_q), _f64(pert.alpha_lj),
             _f64(pert.alpha_crf)]
-    ps = rt.Pert(*[rt.ptr(t) for t in keep])
+    return rt.Pert(*[rt.ptr(t) for t in keep]), keep
+
+
+def perturbed_pairlist_correction(pos, pairlist, pert, charges, iac, lj, crf, lp, bc, four_pi_eps_i, out):
+    """perturbed.rs:309-386; pert: system.Perturbation (host arrays, copied)"""
+    torch = _torch()
+    ps, keep = _pert(pert)
     x, pl, q, ia = _f64(pos), _u32(pairlist), _f64(charges), _u32(iac)
     rt.check(rt.lib().gromos_amd_perturbed_pairlist_correction(
         rt.ptr(x), rt.ptr(pl), pl.numel() // 2, rt.C.byref(ps), rt.ptr(q), rt.ptr(ia), rt.ptr(lj.data), lj.n_types,
         crf, lp, bc.kind, _box(bc), four_pi_eps_i, rt.ptr(out.forces), rt.ptr(out.out), _stream()))
     torch.cuda.current_stream().synchronize()
+
+
+def perturbed_self_energy_correction(pert, crf, lp, four_pi_eps_i):
+    """perturbed.rs:395-425: returns (delta_e_crf, dhdl)."""
+    torch = _torch()
+    ps, keep = _pert(pert)
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    rt.check(rt.lib().gromos_amd_perturbed_self_energy_correction(
+        rt.C.byref(ps), len(pert.flag), crf, lp, four_pi_eps_i, rt.ptr(out), _stream()))
+    v = out.cpu().numpy()
+    return float(v[0]), float(v[1])
+
+
+def perturbed_excluded_correction(excl_off, excl, pos, pert, charges, crf, lp, bc, n_solute, four_pi_eps_i, out):
+    """perturbed.rs:431-511; exclusions as CSR (excl_off, excl) over all atoms."""
+    torch = _torch()
+    ps, keep = _pert(pert)
+    off, idx, x, q = _u32(excl_off), _u32(excl), _f64(pos), _f64(charges)
+    rt.check(rt.lib().gromos_amd_perturbed_excluded_correction(
+        rt.ptr(off), rt.ptr(idx), x.shape[0], int(n_solute), rt.ptr(x), rt.C.byref(ps), rt.ptr(q), crf, lp, bc.kind,
+        _box(bc), four_pi_eps_i, rt.ptr(out.forces), rt.ptr(out.out), _stream()))
+    torch.cuda.current_stream().synchronize()
+
+
+def perturbed_one_four_correction(o14_off, o14, pos, pert, charges, iac, lj, crf, lp, bc, four_pi_eps_i, out):
+    """perturbed.rs:517-605; one_four_pairs as CSR (o14_off, o14)."""
+    torch = _torch()
+    ps, keep = _pert(pert)
+    off, idx, x, q, ia = _u32(o14_off), _u32(o14), _f64(pos), _f64(charges), _u32(iac)
+    rt.check(rt.lib().gromos_amd_perturbed_one_four_correction(
+        rt.ptr(off), rt.ptr(idx), x.shape[0], rt.ptr(x), rt.C.byref(ps), rt.ptr(q), rt.ptr(ia), rt.ptr(lj.data),
+        lj.n_types, crf, lp, bc.kind, _box(bc), four_pi_eps_i, rt.ptr(out.forces), rt.ptr(out.out), _stream()))
+    torch.cuda.current_stream().synchronize()
+
+
+def perturbed_atom_pair_correction(atom_pairs, o14_off, o14, pos, charges, iac, lj, crf, lp, bc, four_pi_eps_i,
+                                   out):
+    """perturbed.rs:624-707; atom_pairs (K, 4) int32: i, j, a_type, b_type (-1 = absent in B)."""
+    torch = _torch()
+    ap = _u32(np.asarray(atom_pairs, np.int32).reshape(-1, 4))
+    off, idx, x, q, ia = _u32(o14_off), _u32(o14), _f64(pos), _f64(charges), _u32(iac)
+    rt.check(rt.lib().gromos_amd_perturbed_atom_pair_correction(
+        rt.ptr(ap), ap.shape[0], rt.ptr(off), rt.ptr(idx), x.shape[0], rt.ptr(x), rt.ptr(q), rt.ptr(ia),
+        rt.ptr(lj.data), lj.n_types, crf, lp, bc.kind, _box(bc), four_pi_eps_i, rt.ptr(out.forces),
+        rt.ptr(out.out), _stream()))
+    torch.cuda.current_stream().synchronize()
+
+
+def perturbed_lj_crf_interaction(r, a_c6, a_c12, b_c6, b_c12, a_q, b_q, alpha_lj, alpha_crf, lp, crf,
+                                 four_pi_eps_i):
+    """perturbed.rs:124-266 on n pairs at once: r (n, 3), the other parameters
+    scalars or (n,) arrays; returns (n, 5): force / r, e_lj, e_crf, de_lj, de_crf."""
+    torch = _torch()
+    rr = np.asarray(r, np.float64).reshape(-1, 3)
+    n = rr.shape[0]
+    prm = np.stack([np.broadcast_to(np.asarray(v, np.float64), (n,))
+                    for v in (a_c6, a_c12, b_c6, b_c12, a_q, b_q, alpha_lj, alpha_crf)], axis=1)
+    dr, dp = _f64(rr), _f64(prm)
+    out = torch.empty((n, 5), dtype=torch.float64, device="cuda")
+    rt.check(rt.lib().gromos_amd_perturbed_lj_crf_interaction(rt.ptr(dr), rt.ptr(dp), n, lp, crf, four_pi_eps_i,
+                                                              rt.ptr(out), _stream()))
+    return out.cpu().numpy()

@@ -113,6 +113,15 @@ def _declare(L):
                                                        P, P, D, P]
     L.gromos_amd_perturbed_pairlist_correction.argtypes = [P, P, I64, C.POINTER(Pert), P, P, P, I32,
                                                            C.POINTER(Crf), C.POINTER(Lambda), I32, P, D, P, P, P]
+    L.gromos_amd_perturbed_self_energy_correction.argtypes = [C.POINTER(Pert), I32, C.POINTER(Crf),
+                                                              C.POINTER(Lambda), D, P, P]
+    L.gromos_amd_perturbed_excluded_correction.argtypes = [P, P, I32, I32, P, C.POINTER(Pert), P, C.POINTER(Crf),
+                                                           C.POINTER(Lambda), I32, P, D, P, P, P]
+    L.gromos_amd_perturbed_one_four_correction.argtypes = [P, P, I32, P, C.POINTER(Pert), P, P, P, I32,
+                                                           C.POINTER(Crf), C.POINTER(Lambda), I32, P, D, P, P, P]
+    L.gromos_amd_perturbed_atom_pair_correction.argtypes = [P, I64, P, P, I32, P, P, P, P, I32, C.POINTER(Crf),
+                                                            C.POINTER(Lambda), I32, P, D, P, P, P]
+    L.gromos_amd_perturbed_lj_crf_interaction.argtypes = [P, P, I64, C.POINTER(Lambda), C.POINTER(Crf), D, P, P]
     L.gromos_amd_system_create.argtypes = [C.POINTER(Topology), C.POINTER(Params), I32, C.POINTER(P)]
     L.gromos_amd_system_destroy.argtypes = [P]
     L.gromos_amd_system_set_state.argtypes = [P, P, P]

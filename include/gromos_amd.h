@@ -150,6 +150,62 @@ int gromos_amd_perturbed_pairlist_correction(const double *pos, const uint32_t *
                                              const double box[3], double four_pi_eps_i,
                                              double *forces, double *out_energies, void *stream);
 
+/* perturbed_self_energy_correction (perturbed.rs:395-425).  Perturbed atoms
+ * are those with pert->flag set among n_atoms (the term reads their A / B
+ * charges).  out: 2 doubles accumulated (delta_e_crf, dhdl). */
+int gromos_amd_perturbed_self_energy_correction(const gromos_amd_pert *pert, int32_t n_atoms,
+                                                const gromos_amd_crf *crf,
+                                                const gromos_amd_lambda *lp, double four_pi_eps_i,
+                                                double *out, void *stream);
+
+/* perturbed_excluded_correction (perturbed.rs:431-511): solute atoms'
+ * excluded pairs j > i with at least one perturbed atom; excl_off / excl as in
+ * gromos_amd_rf_excluded_interactions.  out_energies: 3 doubles accumulated
+ * (delta_e_lj = 0, delta_e_crf, dhdl); forces accumulate the delta. */
+int gromos_amd_perturbed_excluded_correction(const int32_t *excl_off, const int32_t *excl,
+                                             int32_t n_atoms, int32_t num_solute_atoms,
+                                             const double *pos, const gromos_amd_pert *pert,
+                                             const double *charges, const gromos_amd_crf *crf,
+                                             const gromos_amd_lambda *lp, int32_t boundary,
+                                             const double box[3], double four_pi_eps_i,
+                                             double *forces, double *out_energies, void *stream);
+
+/* perturbed_one_four_correction (perturbed.rs:517-605): 1-4 pairs j > i with
+ * at least one perturbed atom, soft-core on the cs6/cs12 entries; o14_off /
+ * o14 as in gromos_amd_one_four_interaction_loop.  out_energies: 3 doubles
+ * accumulated (delta_e_lj, delta_e_crf, dhdl). */
+int gromos_amd_perturbed_one_four_correction(const int32_t *o14_off, const int32_t *o14,
+                                             int32_t n_atoms, const double *pos,
+                                             const gromos_amd_pert *pert, const double *charges,
+                                             const uint32_t *iac, const double *lj, int32_t n_types,
+                                             const gromos_amd_crf *crf, const gromos_amd_lambda *lp,
+                                             int32_t boundary, const double box[3],
+                                             double four_pi_eps_i, double *forces,
+                                             double *out_energies, void *stream);
+
+/* perturbed_atom_pair_correction (perturbed.rs:624-707).  atom_pairs: 4 int32
+ * per PERTATOMPAIR entry (i, j, a_type, b_type; types 0 = c6/c12, 1 =
+ * cs6/cs12, b_type -1 = absent in B, PerturbedAtomPair); entries that are not
+ * 1-4 pairs (o14_off / o14) are skipped.  out_energies: 3 doubles accumulated
+ * (delta_e_lj, delta_e_crf, dhdl). */
+int gromos_amd_perturbed_atom_pair_correction(const int32_t *atom_pairs, int64_t n_atom_pairs,
+                                              const int32_t *o14_off, const int32_t *o14,
+                                              int32_t n_atoms, const double *pos,
+                                              const double *charges, const uint32_t *iac,
+                                              const double *lj, int32_t n_types,
+                                              const gromos_amd_crf *crf,
+                                              const gromos_amd_lambda *lp, int32_t boundary,
+                                              const double box[3], double four_pi_eps_i,
+                                              double *forces, double *out_energies, void *stream);
+
+/* perturbed_lj_crf_interaction (perturbed.rs:124-266) on n independent pairs
+ * (device arrays): r = 3n distance vectors; prm = 8n values (a_c6, a_c12,
+ * b_c6, b_c12, a_q, b_q, alpha_lj, alpha_crf) with a_q / b_q the pair's charge
+ * products; out = 5n values assigned (force / r, e_lj, e_crf, de_lj, de_crf). */
+int gromos_amd_perturbed_lj_crf_interaction(const double *r, const double *prm, int64_t n,
+                                            const gromos_amd_lambda *lp, const gromos_amd_crf *crf,
+                                            double four_pi_eps_i, double *out, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* 2. device-resident systems                                          */
 /* ------------------------------------------------------------------ */

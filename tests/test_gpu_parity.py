@@ -279,6 +279,53 @@ def test_water32768_full_size():
     assert np.abs(g["forces"].sum(0)).max() < 1e-6
 
 
+def test_hetero_60000_full_size():
+    """The heterogeneous single point at its full size: 6,000-atom random-walk
+    solute in charge groups of 1-4 atoms, 1-2 / 1-3 exclusions, 1-4 pairs, 18,000
+    SPC waters, L = 8.4 nm, rc 1.4 nm (60,000 atoms): exact four-list pair set
+    (exclusions included) against the oracle's standard list, FP64 forces and
+    energies at the parity tolerance, and the mixed path at its tolerance."""
+    s = gen.hetero_system()
+    assert s.n_atoms == 60000
+    o = oracle.OracleSystem(s)
+    ol = oracle.pairlist(o)
+    r = oracle.single_point(o, parallel=True)
+    for prec in (rt.FP64, rt.MIXED):
+        ds = rt.DeviceSystem(s, precision=prec)
+        ds.pairlist_update()
+        assert_same_lists(ds.pairlist(), ol)
+        g = ds.forcefield_apply()
+        ds.close()
+        if prec == rt.FP64:
+            assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+            assert force_err(g["forces"], r["forces"]) < F_TOL
+        else:
+            mixed_energy_ok(g, r)
+            assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+            assert force_err_median(g["forces"], r["forces"]) < MX_F_MED
+
+
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_remd_replica_single_point(prec):
+    """One replica of the exchange batch: 4,000 SPC waters (12,000 atoms,
+    L = 4.93 nm), rc 1.2 nm: exact pair set and forces against the oracle."""
+    s = gen.water_replica(seed=100, temperature=280.0, n_waters=4000, rc=1.2)
+    o = oracle.OracleSystem(s)
+    ds = rt.DeviceSystem(s, precision=prec)
+    ds.pairlist_update()
+    assert_same_lists(ds.pairlist(), oracle.pairlist(o))
+    g = ds.forcefield_apply()
+    ds.close()
+    r = oracle.single_point(o, parallel=True)
+    if prec == rt.FP64:
+        assert rel(g["e_lj"], r["e_lj"]) < E_TOL and rel(g["e_crf"], r["e_crf"]) < E_TOL
+        assert force_err(g["forces"], r["forces"]) < F_TOL
+    else:
+        mixed_energy_ok(g, r)
+        assert force_err(g["forces"], r["forces"]) < MX_F_TOL
+        assert force_err_median(g["forces"], r["forces"]) < MX_F_MED
+
+
 # --------------------------------------------------------------------------
 # mixed precision (FP32 pair math, int64 fixed-point forces, FP64 sums)
 # tolerance (per atom |dF_i| / max(|F_i|, 1 kJ/mol/nm)): median <= 2e-6, max
@@ -412,6 +459,15 @@ def _fep_case(s, prec=None):
 def test_resident_fep_job(lam):
     s = gen.fep_job(3000, 12, lam, seed=21)
     _fep_case(s)
+
+
+@pytest.mark.parametrize("prec", [None, rt.MIXED])
+def test_resident_fep_job_cutoff_beyond_half_box(prec):
+    """The smallest jobs of the FEP mix: 1,000 atoms, L = 2.15 nm < 2 rc."""
+    s = gen.fep_job(1000, 9, 0.7, seed=4)
+    assert s.box[0] < 2 * s.rcutl
+    g, r = _fep_case(s, prec)
+    assert g["dhdl"] != 0.0
 
 
 def test_resident_fep_bonded_solute_all_passes():

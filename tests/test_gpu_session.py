@@ -165,3 +165,109 @@ def test_replica_exchange_run():
         e = sess.energies(j)
         assert np.all(np.isfinite(e)) and e.shape == (30, 5)
     sess.close()
+
+
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_remd_batch_128_replicas_match_alone(prec):
+    """The replica-exchange batch at its full shape: 128 replicas of 4,000 SPC
+    waters (12,000 atoms, rc 1.2 nm) at T_k = 280 (450/280)^(k/127) K in one
+    session; sampled replicas are bit-identical to the same replica run alone."""
+    n, steps = 128, 12
+    temps = [280.0 * (450.0 / 280.0) ** (k / (n - 1)) for k in range(n)]
+    systems = [gen.water_replica(seed=100 + k, temperature=temps[k], n_waters=4000, rc=1.2) for k in range(n)]
+    sess = rt.Session(precision=prec, nsnb=5)
+    jobs = [sess.admit(s, steps) for s in systems]
+    sess.step(steps)
+    assert sess.info()["resident_jobs"] == n or all(sess.query(j)[0] == rt.DONE for j in jobs)
+    for k in (0, 61, 127):
+        st, done = sess.query(jobs[k])
+        assert st == rt.DONE and done == steps
+        e0, x0, v0 = alone(systems[k], steps, prec)
+        assert np.array_equal(sess.energies(jobs[k]), e0), k
+        x, v = sess.fetch(jobs[k])
+        assert np.array_equal(x, x0) and np.array_equal(v, v0), k
+    sess.close()
+
+
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_failed_job_is_isolated(prec):
+    """A job whose SETTLE solve fails (one hydrogen moved 1 nm from its oxygen:
+    sin(angle) > 1 in the analytical solve, constraints.rs:896-951) is marked
+    FAILED with the reason; the jobs batched with it finish and are
+    bit-identical to the same jobs run alone."""
+    good = [gen.spc_water_box(600, seed=31), gen.fep_job(1500, 5, 0.3, seed=32)]
+    bad = gen.spc_water_box(500, seed=33)
+    bad.pos[bad.n_solute_atoms + 3 * 40 + 1] += np.array([1.0, 0.0, 0.0])   # the oracle fails at step 1 here
+    steps = 15
+    sess = rt.Session(precision=prec, nsnb=5)
+    jg = [sess.admit(s, steps) for s in good]
+    jb = sess.admit(bad, steps)
+    sess.step(steps)
+    st, done = sess.query(jb)
+    assert st == rt.FAILED and done < steps
+    assert "SETTLE" in rt.lib().gromos_amd_last_error().decode()
+    for j, s in zip(jg, good):
+        assert sess.query(j) == (rt.DONE, steps)
+        e0, x0, v0 = alone(s, steps, prec)
+        assert np.array_equal(sess.energies(j), e0)
+        x, v = sess.fetch(j)
+        assert np.array_equal(x, x0) and np.array_equal(v, v0)
+    # the failed job leaves at the next boundary; the session goes on
+    j2 = sess.admit(gen.spc_water_box(300, seed=34), 5)
+    sess.step(5)
+    assert sess.query(j2) == (rt.DONE, 5)
+    sess.close()
+
+
+def test_layout_failure_leaves_session_consistent(monkeypatch):
+    """A batch layout that runs out of device memory (forced from the host: a
+    layout of more atoms than GROMOS_AMD_LAYOUT_ATOM_LIMIT fails before any
+    allocation): session_step returns the error, the continuing job stops as
+    FAILED with its last state fetchable, the queued job stays queued with its
+    reservation, and once memory allows it runs and matches the same job alone."""
+    a = gen.spc_water_box(600, seed=41)
+    b = gen.spc_water_box(500, seed=42)
+    sess = rt.Session(precision=rt.FP64, nsnb=5)
+    ja = sess.admit(a, 20)
+    sess.step(5)
+    x5, v5 = sess.fetch(ja)
+    jb = sess.admit(b, 10)
+    reserved = sess.info()["reserved_bytes"]
+    monkeypatch.setenv("GROMOS_AMD_LAYOUT_ATOM_LIMIT", str(a.n_atoms + b.n_atoms - 1))
+    with pytest.raises(rt.GromosAmdError):
+        sess.step(5)
+    st, done = sess.query(ja)
+    assert st == rt.FAILED and done == 5
+    assert "layout" in rt.lib().gromos_amd_last_error().decode()
+    x, v = sess.fetch(ja)
+    assert np.array_equal(x, x5) and np.array_equal(v, v5)
+    assert sess.query(jb) == (rt.QUEUED, 0)
+    info = sess.info()
+    assert info["resident_jobs"] == 0 and info["reserved_bytes"] == reserved - rt.Session.footprint(a)
+    monkeypatch.delenv("GROMOS_AMD_LAYOUT_ATOM_LIMIT")
+    sess.step(10)
+    assert sess.query(jb) == (rt.DONE, 10)
+    e0, x0, _ = alone(b, 10, rt.FP64)
+    assert np.array_equal(sess.energies(jb), e0)
+    assert np.array_equal(sess.fetch(jb)[0], x0)
+    sess.close()
+
+
+def test_admission_refuses_lists_beyond_32bit_offsets():
+    """A job whose neighbour lists would not fit the kernels' 32-bit list
+    offsets (a vacuum system of 50,000 single-atom charge groups: every pair is
+    a candidate, about 3e9 entries) is refused at admission, before anything
+    reaches the device."""
+    n = 50000
+    rng = np.random.default_rng(1)
+    s = gamd.system.System(n_atoms=n, n_solute_atoms=n, cg_start=np.arange(n + 1, dtype=np.int32), n_solute_cg=n,
+                           charge=np.zeros(n), iac=np.zeros(n, np.int32), mass=np.ones(n), n_types=1,
+                           lj=np.array([1e-3, 1e-6, 1e-3, 1e-6]), excl_off=np.zeros(n + 1, np.int32),
+                           excl=np.zeros(0, np.int32), o14_off=np.zeros(n + 1, np.int32), o14=np.zeros(0, np.int32),
+                           atoms_per_solvent=0, pos=rng.uniform(0, 20, (n, 3)), vel=np.zeros((n, 3)),
+                           box=np.zeros(3), boundary=0).validate()
+    sess = rt.Session(precision=rt.FP64, nsnb=5)
+    with pytest.raises(rt.GromosAmdError, match="32-bit"):
+        sess.admit(s, 10)
+    assert sess.info()["reserved_bytes"] == 0
+    sess.close()

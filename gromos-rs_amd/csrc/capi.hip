@@ -92,6 +92,23 @@ extern "C" int gromos_amd_lambda_from(double lj, double ljs, double crfl, double
 // ---------------------------------------------------------------------------
 // 1. pair-list kernels
 // ---------------------------------------------------------------------------
+
+// The per-block partial sums of one call (expl_blocks(n) rows of 11 doubles),
+// from the device's stream-ordered memory pool, released on the same stream
+// after the fold has read them.
+struct PartRows {
+    double* p = nullptr;
+    hipStream_t st = nullptr;
+    int alloc(int64_t n, hipStream_t s) {
+        st = s;
+        if (n <= 0) return 0;
+        HIPCHK(hipMallocAsync((void**)&p, (size_t)expl_blocks(n) * EXPL_ROW * sizeof(double), st));
+        return 0;
+    }
+    ~PartRows() {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+};
 static int expl_common(ExplArgs& a, const double* pos, const double* q, const uint32_t* iac,
                        const double* lj, int32_t nt, const gromos_amd_crf* crf, int32_t boundary,
                        const double* box, double fpe, int32_t do_virial, double* forces,
@@ -126,7 +143,30 @@ extern "C" int gromos_amd_lj_crf_innerloop(const double* pos, const double* char
     if (n_pairs < 0 || (n_pairs > 0 && !pairs)) return fail(GROMOS_AMD_EINVAL, "bad pair list");
     a.pairs = pairs;
     a.n = n_pairs;
+    PartRows part;
+    if (int rc = part.alloc(a.n, (hipStream_t)stream)) return rc;
+    a.part = part.p;
     HIPCHK(launch_plain(a, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int gromos_amd_lj_crf_innerloop_noalloc(const double* pos, const double* charges, const uint32_t* iac,
+                                                   const uint32_t* pairs, int64_t n_pairs, const double* lj,
+                                                   int32_t n_types, const gromos_amd_crf* crf, int32_t boundary,
+                                                   const double box[3], double fpe, int32_t do_virial,
+                                                   double* forces, double* energies, double* virial,
+                                                   void* stream) {
+    ExplArgs a{};
+    if (int rc = expl_common(a, pos, charges, iac, lj, n_types, crf, boundary, box, fpe, do_virial, forces,
+                             energies, virial))
+        return rc;
+    if (n_pairs < 0 || (n_pairs > 0 && !pairs)) return fail(GROMOS_AMD_EINVAL, "bad pair list");
+    if (n_pairs > NOALLOC_MAX_PAIRS)
+        return fail(GROMOS_AMD_EINVAL, "lj_crf_innerloop_noalloc: %lld pairs exceed the limit of %lld per call",
+                    (long long)n_pairs, (long long)NOALLOC_MAX_PAIRS);
+    a.pairs = pairs;
+    a.n = n_pairs;
+    HIPCHK(launch_plain_noalloc(a, (hipStream_t)stream));
     return 0;
 }
 
@@ -146,6 +186,9 @@ extern "C" int gromos_amd_lj_crf_innerloop_cg_grouped(const double* pos, const d
     a.pairs = pairs;
     a.groups = groups;
     a.n = n_groups;
+    PartRows part;
+    if (int rc = part.alloc(a.n, (hipStream_t)stream)) return rc;
+    a.part = part.p;
     HIPCHK(launch_cg_grouped(a, (hipStream_t)stream));
     return 0;
 }
@@ -165,6 +208,9 @@ extern "C" int gromos_amd_solvent_innerloop(const double* pos, const double* cha
     a.pairs = pairs;
     a.n = n_pairs;
     a.aps = aps;
+    PartRows part;
+    if (int rc = part.alloc(a.n, (hipStream_t)stream)) return rc;
+    a.part = part.p;
     HIPCHK(launch_solvent(a, (hipStream_t)stream));
     return 0;
 }
@@ -191,6 +237,9 @@ extern "C" int gromos_amd_rf_excluded_interactions(const double* charges, const 
     a.forces = forces;
     a.energies = energies;
     a.virial = virial;
+    PartRows part;
+    if (int rc = part.alloc(n_atoms, (hipStream_t)stream)) return rc;
+    a.part = part.p;
     HIPCHK(launch_rf_excluded(a, (hipStream_t)stream));
     return 0;
 }
@@ -220,6 +269,9 @@ extern "C" int gromos_amd_one_four_interaction_loop(const int32_t* o14_off, cons
     a.forces = forces;
     a.energies = energies;
     a.virial = virial;
+    PartRows part;
+    if (int rc = part.alloc(n_atoms, (hipStream_t)stream)) return rc;
+    a.part = part.p;
     HIPCHK(launch_one_four(a, (hipStream_t)stream));
     return 0;
 }

@@ -3,14 +3,16 @@
 // innerloops.rs:318-733, excluded.rs:25-206, perturbed.rs:309-386).
 //
 // One lane per listed pair (or per group / sentinel / atom).  Forces go to the
-// caller's f64 buffer with hardware FP64 atomics; energies and the virial are
-// summed per block in a fixed order and added once per block.
+// caller's f64 buffer with hardware FP64 atomics.  Energies and the virial are
+// summed per block in a fixed order into a scratch row per block, and the rows
+// are folded in a fixed order by one more block (k_fold), so they do not
+// depend on scheduling.
+#include <mutex>
+
 #include "device_common.hpp"
 #include "explicit.hpp"
 
 namespace gamd {
-
-constexpr int EXPL_BLOCK = 256;
 
 __device__ __forceinline__ void atomic_add3(double* f, int64_t i, double x, double y, double z) {
     atomicAdd(f + 3 * i, x);
@@ -18,13 +20,26 @@ __device__ __forceinline__ void atomic_add3(double* f, int64_t i, double x, doub
     atomicAdd(f + 3 * i + 2, z);
 }
 
-// sums v[0..1] into energies and v[2..10] into virial (if non-null)
-__device__ __forceinline__ void flush_block(double (&v)[11], double* energies, double* virial) {
-    __shared__ double lds[17 * 11];
-    block_sum<11>(v, lds);
-    if (threadIdx.x < 2) atomicAdd(energies + threadIdx.x, lds[16 * 11 + threadIdx.x]);
-    if (virial && threadIdx.x >= 2 && threadIdx.x < 11)
-        atomicAdd(virial + threadIdx.x - 2, lds[16 * 11 + threadIdx.x]);
+// the block's sums of v (e_lj, e_crf, virial[9]) into its scratch row
+__device__ __forceinline__ void flush_block(double (&v)[EXPL_ROW], double* part) {
+    __shared__ double lds[17 * EXPL_ROW];
+    block_sum<EXPL_ROW>(v, lds);
+    if (threadIdx.x < EXPL_ROW) part[EXPL_ROW * (int64_t)blockIdx.x + threadIdx.x] = lds[16 * EXPL_ROW + threadIdx.x];
+}
+
+// one block: the nrows scratch rows summed in a fixed order (lane t takes rows
+// t, t + 256, ..., then the block sum) and added to energies / virial
+__global__ void __launch_bounds__(EXPL_BLOCK) k_fold(const double* part, int64_t nrows, double* energies,
+                                                     double* virial) {
+    __shared__ double lds[17 * EXPL_ROW];
+    double v[EXPL_ROW] = {0};
+    for (int64_t r = threadIdx.x; r < nrows; r += EXPL_BLOCK)
+#pragma unroll
+        for (int c = 0; c < EXPL_ROW; ++c) v[c] += part[EXPL_ROW * r + c];
+    block_sum<EXPL_ROW>(v, lds);
+    if (threadIdx.x < 2) atomicAdd(energies + threadIdx.x, lds[16 * EXPL_ROW + threadIdx.x]);
+    if (virial && threadIdx.x >= 2 && threadIdx.x < EXPL_ROW)
+        atomicAdd(virial + threadIdx.x - 2, lds[16 * EXPL_ROW + threadIdx.x]);
 }
 
 __device__ __forceinline__ void acc_pair(double (&v)[11], d3 r, double fm, double elj, double ecrf,
@@ -56,7 +71,7 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_plain(ExplArgs a) {
         lj_crf64(r.x, r.y, r.z, p[0], p[1], qp, a.crf, fm, elj, ecrf);
         acc_pair(v, r, fm, elj, ecrf, a.virial != nullptr, a.forces, i, j);
     }
-    flush_block(v, a.energies, a.virial);
+    flush_block(v, a.part);
 }
 
 // process_pairs_cg_grouped (innerloops.rs:105-162): one lane per group
@@ -79,7 +94,7 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_cg_grouped(ExplArgs a) {
             acc_pair(v, r, fm, elj, ecrf, a.virial != nullptr, a.forces, i, j);
         }
     }
-    flush_block(v, a.energies, a.virial);
+    flush_block(v, a.part);
 }
 
 // process_solvent_pairs (innerloops.rs:218-309): tables from the first pair's molecule
@@ -108,7 +123,7 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_solvent(ExplArgs a) {
             }
         }
     }
-    flush_block(v, a.energies, a.virial);
+    flush_block(v, a.part);
 }
 
 // rf_excluded_interactions (excluded.rs:25-112): one lane per atom i, pairs j > i
@@ -145,7 +160,7 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_rf_excluded(ExclArgs a) {
             }
         }
     }
-    flush_block(v, a.energies, a.virial);
+    flush_block(v, a.part);
 }
 
 // one_four_interaction_loop (excluded.rs:131-206)
@@ -180,7 +195,7 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_one_four(ExclArgs a) {
                 for (int y = 0; y < 3; ++y) v[2 + 3 * x + y] += rr[x] * ff[y];
         }
     }
-    flush_block(v, a.energies, a.virial);
+    flush_block(v, a.part);
 }
 
 // perturbed_pairlist_correction (perturbed.rs:309-386)
@@ -366,32 +381,61 @@ __global__ void __launch_bounds__(EXPL_BLOCK) k_soft_pairs(SoftPairArgs a) {
     for (int c = 0; c < 5; ++c) a.out[5 * k + c] = o[c];
 }
 
-static int blocks_for(int64_t n) { return (int)((n + EXPL_BLOCK - 1) / EXPL_BLOCK); }
+static int blocks_for(int64_t n) { return (int)expl_blocks(n); }
 
-hipError_t launch_plain(const ExplArgs& a, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plain, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
+// kernel K over n lanes with its partial rows in part, then the fold
+template <class Args>
+static hipError_t launch_folded(void (*K)(Args), const Args& a, int64_t n, double* part, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(K, dim3(blocks_for(n)), dim3(EXPL_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(EXPL_BLOCK), 0, s, (const double*)part, expl_blocks(n), a.energies,
+                       a.virial);
     return hipGetLastError();
 }
+
+hipError_t launch_plain(const ExplArgs& a, hipStream_t s) { return launch_folded(k_plain, a, a.n, a.part, s); }
 hipError_t launch_cg_grouped(const ExplArgs& a, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cg_grouped, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
-    return hipGetLastError();
+    return launch_folded(k_cg_grouped, a, a.n, a.part, s);
 }
-hipError_t launch_solvent(const ExplArgs& a, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_solvent, dim3(blocks_for(a.n)), dim3(EXPL_BLOCK), 0, s, a);
-    return hipGetLastError();
-}
+hipError_t launch_solvent(const ExplArgs& a, hipStream_t s) { return launch_folded(k_solvent, a, a.n, a.part, s); }
 hipError_t launch_rf_excluded(const ExclArgs& a, hipStream_t s) {
-    if (a.n_atoms <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rf_excluded, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
-    return hipGetLastError();
+    return launch_folded(k_rf_excluded, a, (int64_t)a.n_atoms, a.part, s);
 }
 hipError_t launch_one_four(const ExclArgs& a, hipStream_t s) {
-    if (a.n_atoms <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_one_four, dim3(blocks_for(a.n_atoms)), dim3(EXPL_BLOCK), 0, s, a);
-    return hipGetLastError();
+    return launch_folded(k_one_four, a, (int64_t)a.n_atoms, a.part, s);
+}
+
+// The no-allocation variant's partial rows: one static array per device,
+// reached from host code only through the address hipGetSymbolAddress returns
+// for the current device (the array's name in host code is not a device
+// address).  Every call on a device shares it: calls that overlap there race.
+static __device__ __align__(256) double scratch[NOALLOC_BLOCKS * EXPL_ROW];
+constexpr int MAX_DEVICES = 64;
+static double* g_scratch[MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static hipError_t scratch_ptr(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = (double*)p;
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_plain_noalloc(ExplArgs a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    if (a.n > NOALLOC_MAX_PAIRS) return hipErrorInvalidValue;   // the caller reports the limit
+    hipError_t e = scratch_ptr(&a.part);
+    if (e != hipSuccess) return e;
+    return launch_folded(k_plain, a, a.n, a.part, s);
 }
 hipError_t launch_pert_pairs(const PertArgs& a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;

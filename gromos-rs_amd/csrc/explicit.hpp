@@ -20,6 +20,7 @@ struct ExplArgs {
     double* forces;
     double* energies;
     double* virial;          // null: no virial
+    double* part;            // scratch: 11 partial sums per block (blocks_for(n) rows)
 };
 
 struct ExclArgs {
@@ -39,7 +40,17 @@ struct ExclArgs {
     double* forces;
     double* energies;
     double* virial;
+    double* part;            // scratch: 11 partial sums per block
 };
+
+constexpr int EXPL_BLOCK = 256;
+constexpr int EXPL_ROW = 11;   // e_lj, e_crf, virial[9]
+inline int64_t expl_blocks(int64_t n) { return (n + EXPL_BLOCK - 1) / EXPL_BLOCK; }
+// Largest pair count of one gromos_amd_lj_crf_innerloop_noalloc call: its
+// partial sums live in a static device array of NOALLOC_BLOCKS rows
+// (65,536 x 11 doubles, 5.8 MB per device).
+constexpr int64_t NOALLOC_BLOCKS = 65536;
+constexpr int64_t NOALLOC_MAX_PAIRS = NOALLOC_BLOCKS * EXPL_BLOCK;
 
 struct PertArgs {
     const double* pos;
@@ -82,6 +93,9 @@ struct SoftPairArgs {
 };
 
 hipError_t launch_plain(const ExplArgs& a, hipStream_t s);
+// launch_plain with the partial sums in the library's static device array
+// instead of a.part (no allocation; calls that overlap on one device race)
+hipError_t launch_plain_noalloc(ExplArgs a, hipStream_t s);
 hipError_t launch_cg_grouped(const ExplArgs& a, hipStream_t s);
 hipError_t launch_solvent(const ExplArgs& a, hipStream_t s);
 hipError_t launch_rf_excluded(const ExclArgs& a, hipStream_t s);

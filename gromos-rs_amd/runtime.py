@@ -104,6 +104,8 @@ def _declare(L):
     L.gromos_amd_lambda_from.argtypes = [D] * 8 + [I32, C.POINTER(Lambda)]
     base = [P, P, P]
     L.gromos_amd_lj_crf_innerloop.argtypes = base + [P, I64, P, I32, C.POINTER(Crf), I32, P, D, I32, P, P, P, P]
+    L.gromos_amd_lj_crf_innerloop_noalloc.argtypes = base + [P, I64, P, I32, C.POINTER(Crf), I32, P, D, I32, P, P, P,
+                                                             P]
     L.gromos_amd_lj_crf_innerloop_cg_grouped.argtypes = base + [P, P, I64, P, I32, C.POINTER(Crf), I32, P, D,
                                                                 I32, P, P, P, P]
     L.gromos_amd_solvent_innerloop.argtypes = base + [P, I64, P, I32, C.POINTER(Crf), I32, P, I32, D, I32, P, P,

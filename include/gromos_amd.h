@@ -69,12 +69,33 @@ int gromos_amd_crf_new(double cutoff, double epsilon, double rf_epsilon, double 
 
 /* lj_crf_innerloop[_novirial] (innerloops.rs:318-368): per-pair nearest image.
  * pairs: 2*n_pairs u32 (i, j).  energies: 2 doubles (e_lj, e_crf).
- * virial may be NULL when do_virial == 0. */
+ * virial may be NULL when do_virial == 0.  Energies and virial are summed in a
+ * fixed order through per-block partial sums (a scratch buffer from the
+ * device's stream-ordered pool, hipMallocAsync on `stream`), so they are the
+ * same bits on every call. */
 int gromos_amd_lj_crf_innerloop(const double *pos, const double *charges, const uint32_t *iac,
                                 const uint32_t *pairs, int64_t n_pairs, const double *lj,
                                 int32_t n_types, const gromos_amd_crf *crf, int32_t boundary,
                                 const double box[3], double four_pi_eps_i, int32_t do_virial,
                                 double *forces, double *energies, double *virial, void *stream);
+
+/* gromos_amd_lj_crf_innerloop without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-block partial sums live in a static device array of the library
+ * (65,536 blocks x 11 doubles, 5.8 MB per device, held until the process
+ * ends), so n_pairs is limited to 65,536 * 256 = 16,777,216 per call
+ * (GROMOS_AMD_EINVAL beyond).  Every call on one device shares that array:
+ * calls that overlap on a device (other streams, other threads) race.
+ * Results equal gromos_amd_lj_crf_innerloop's: energies and virial bit for
+ * bit, forces to FP64 atomic-order rounding. */
+int gromos_amd_lj_crf_innerloop_noalloc(const double *pos, const double *charges,
+                                        const uint32_t *iac, const uint32_t *pairs,
+                                        int64_t n_pairs, const double *lj, int32_t n_types,
+                                        const gromos_amd_crf *crf, int32_t boundary,
+                                        const double box[3], double four_pi_eps_i,
+                                        int32_t do_virial, double *forces, double *energies,
+                                        double *virial, void *stream);
 
 /* lj_crf_innerloop_cg_grouped[_novirial] (innerloops.rs:374-428).
  * groups: 4*n_groups u32 (ref_atom_i, ref_atom_j, start, end) (CGPairGroup, params.rs:145-154). */

@@ -1,14 +1,20 @@
-"""Benchmark: device-resident MD of 32,768 SPC waters (98,304 atoms) on one GPU.
+"""Benchmark of the nonbonded hot path on one GPU.
 
-One step = Forcefield::apply (GPU charge-group pair list rebuilt every 5
-steps, LJ + reaction-field pair loops, RF excluded terms) + leap-frog +
-SETTLE + E_kin, the sequence of gromos-rs md (md.rs:1143-1286).
+Headline: the 64-job FEP lambda-window serving mix (1,000-30,000-atom SPC
+water jobs with 3-30 soft-core atoms, seeded Poisson arrivals, 2,000 md steps
+each) through the continuous-batching server in mixed precision: aggregate pair
+interactions/s, ns/day and p50/p99 step latency, against the same jobs run one
+at a time on the plain FP64 GPU path.  One md step = Forcefield::apply (GPU
+charge-group pair list every 5 steps, LJ + reaction-field pair loops, RF
+excluded terms, soft-core corrections) + leap-frog + SETTLE + E_kin, the
+sequence of gromos-rs md (md.rs:1143-1286).  Pairs are counted as the
+reference evaluates them (n_i * n_j per charge-group pair, 9 per water pair).
 
-Prints one JSON line: pair interactions/s (pairs counted as the reference
-evaluates them: n_i * n_j per charge-group pair, so 9 per water pair), ns/day,
-p50/p99 step latency, the roofline of the dominant kernel (its launches timed
-with HIP events on the library's stream) and the CPU baseline (the oracle's
-OpenMP restatement of the rayon loops, on a bounded sample of the same system).
+Beside it: the 32,768-water box in both precisions, the 128-replica exchange
+run, force-error and drift comparisons of the mixed path, the roofline of the
+dominant kernel (its launches timed with HIP events on the batch's stream) and
+the CPU baseline (the oracle's OpenMP restatement of the rayon loops on a
+bounded sample, all threads of the host share and one thread).
 """
 import argparse
 import importlib
@@ -22,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 60          # algorithmic count per pair interaction (r, r2, 1/r2, r^-6,
                             # E_LJ, F_LJ, 1/r, q, E_CRF, F_CRF, F, accumulate, energies, virial)
+VIRIAL_FLOP = 18            # the 3x3 virial's share of it (9 mul + 9 add)
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix are the same figure)
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 (packed; this build keeps FP32 un-packed: 78.6 effective)
 PEAK_HBM_GBS = 8000.0
@@ -32,20 +39,22 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=15, help="timed oracle MD steps of the CPU baseline (about 10-15 s)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=2,
+                    help="timed oracle MD steps of the single-core CPU baseline (about 10 s)")
+    ap.add_argument("--only", choices=["fep-mixed", "water"], default=None,
+                    help="run one part only (for profiling): the FEP mix's batched mixed replay, or the water box")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
     ap.add_argument("--fep-steps", type=int, default=2000, help="md steps per job of the 64-job FEP mix")
     ap.add_argument("--fep-alone-steps", type=int, default=200,
                     help="steps per job timed for the one-job-at-a-time plain FP64 comparison")
-    ap.add_argument("--remd-steps", type=int, default=1000, help="steps of the 128-replica exchange run")
+    ap.add_argument("--remd-steps", type=int, default=10000, help="steps of the 128-replica exchange run")
     ap.add_argument("--fep-admit-interval", type=int, default=200,
                     help="steps between batch layout changes of the FEP server (a multiple of 5); the mixed "
                          "server also runs at 50 for the latency side of the trade-off")
-    ap.add_argument("--no-fep", action="store_true")
     ap.add_argument("--no-remd", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=2000, help="NVE steps of the equilibrated drift comparison")
     ap.add_argument("--drift-replicas", type=int, default=8,
@@ -66,32 +75,43 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": n * 4, "read": n * 4, "written": n * 4}))
 
 
-def cpu_baseline(s, nsteps):
-    """The oracle (CPU restatement of the reference algorithm) on the same system:
-    initial pair list + one warm step untimed, then `nsteps` timed steps
-    (a pair-list rebuild every 5 steps, as on the GPU)."""
+def cpu_baseline(s, nsteps, nsteps_1core):
+    """The oracle (CPU restatement of the reference algorithm) on the same
+    32,768-water system: initial pair list + one warm step untimed, then
+    `nsteps` timed steps (a pair-list rebuild every 5 steps, as on the GPU) on
+    the host's thread share, and `nsteps_1core` on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    out = {}
+    for key, nthr, n in (("all", threads, nsteps), ("one", 1, nsteps_1core)):
+        if n <= 0:
+            continue
+        oracle.set_threads(nthr)
+        o = oracle.OracleSystem(s)
+        md = oracle.MD(o, use_settle=True, parallel=True)
+        md.run(1)
+        p0 = md.pair_count
+        t0 = time.perf_counter()
+        md.run(n)
+        dt = time.perf_counter() - t0
+        pairs = md.pair_count - p0
+        out[key] = dict(value=pairs / dt, unit="pair interactions/s", cores=nthr, kind="port",
+                        sample=f"{n} MD steps of the same 32,768-water system (steps 2-{n + 1}, the O(N^2) standard pair "
+                               f"list rebuilt every 5 steps), OpenMP fold/reduce restatement of the rayon loops on "
+                               f"{nthr} thread(s); {dt:.2f} s wall, {dt / n * 1e3:.1f} ms/step")
     oracle.set_threads(threads)
-    o = oracle.OracleSystem(s)
-    md = oracle.MD(o, use_settle=True, parallel=True)
-    md.run(1)
-    p0 = md.pair_count
-    t0 = time.perf_counter()
-    md.run(nsteps)
-    dt = time.perf_counter() - t0
-    pairs = md.pair_count - p0
-    return dict(value=pairs / dt, unit="pair interactions/s", cores=threads, kind="port",
-                sample=f"{nsteps} MD steps of the same 32,768-water system (steps 1-{nsteps}, the O(N^2) "
-                       f"standard pair list rebuilt every 5 steps), OpenMP fold/reduce restatement of the rayon loops; "
-                       f"{dt:.2f} s wall, {dt / nsteps * 1e3:.1f} ms/step")
+    res = dict(out.get("all", out.get("one")))
+    res["cpu_model"] = cpu_model()
+    if "one" in out and "all" in out:
+        res["single_core"] = out["one"]
+    return res
 
 
-def traffic_from_profile(kernel, precision):
+def traffic_from_profile(kernel, tag):
     """HBM bytes per dispatch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE scaled by
     a copy of known size), read from the committed summary of the same kernel."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_water32768_{precision}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}.md")
     try:
         for line in open(path):
             if line.startswith("machine-readable: traffic_bytes="):
@@ -116,6 +136,7 @@ def run_path(rt, s, prec, steps, warmup, latency_steps):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms, launches = ds.kernel_timing(enable=0)
+    kern_pairs = ds.kernel_pairs
     pairs = ds.pair_count - p0
     lat = []
     for _ in range(latency_steps):
@@ -123,7 +144,6 @@ def run_path(rt, s, prec, steps, warmup, latency_steps):
         ds.md(1)
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat) if lat else np.array([float("nan")])
-    vv_pairs = ds_vv_pairs(ds, s)
     ds.close()
     tot = e[:, 2] + e[:, 3]
     t_ps = np.arange(len(tot)) * s.dt
@@ -131,7 +151,7 @@ def run_path(rt, s, prec, steps, warmup, latency_steps):
     return dict(drift=drift, pairs_per_s=pairs / wall, ms_per_step=wall / steps * 1e3, wall=wall,
                 ns_per_day=steps * s.dt * 1e-3 * 86400.0 / wall,
                 p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
-                kern_ms=kern_ms / max(launches, 1), launches=launches, vv_pairs=vv_pairs,
+                kern_ms=kern_ms / max(launches, 1), launches=launches, kern_pairs=kern_pairs,
                 pairs_per_step=pairs / steps, e_first=float(e[0, 2] + e[0, 3]), e_last=float(e[-1, 2] + e[-1, 3]))
 
 
@@ -158,6 +178,21 @@ def force_error(rt, s):
                 energy_rel=float(abs(emx - e64) / abs(e64)))
 
 
+def progress(msg):
+    """a line on stderr as each part of the run ends (stdout carries only the result)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -171,78 +206,112 @@ def main():
     assert torch.cuda.is_available(), "no GPU visible"
     gen = importlib.import_module("gromos-rs_amd.generators")
     rt = importlib.import_module("gromos-rs_amd.runtime")
+    if a.only == "fep-mixed":
+        # one batched mixed-precision replay of the FEP mix, nothing else (for profiling)
+        fep = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps, a.fep_admit_interval, only_mixed=True)
+        print(json.dumps({"fep_mix": fep}), flush=True)
+        return
+    if a.only == "water":
+        # the mixed path on the water box only (for profiling)
+        print(json.dumps({"water_32768_mixed": run_path(rt, gen.water_32768(), rt.MIXED, a.steps, a.warmup, 0)}),
+              flush=True)
+        return
     s = gen.water_32768()
-    paths = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
-    res = {}
-    for name in paths:
-        prec = rt.FP64 if name == "fp64" else rt.MIXED
-        res[name] = run_path(rt, s, prec, a.steps, a.warmup, a.latency_steps)
-    head = "mixed" if "mixed" in res else "fp64"
-    h = res[head]
-    kernel = "k_vv_mx" if head == "mixed" else "k_vv"
-    flop = h["vv_pairs"] * FLOP_PER_PAIR
-    achieved = flop / (h["kern_ms"] * 1e-3) / 1e12
-    peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
-    traffic, traffic_src = traffic_from_profile(kernel, head)
+    water = water_bench(rt, gen, s, a)
+    progress("water_32768 done")
+    fep = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps, a.fep_admit_interval, drift_steps=a.drift_steps)
+    bm = fep["batched_mixed"]
+    kms, klaunch, kpairs = bm["kernel_timing"]
+    flop_per_pair = FLOP_PER_PAIR - VIRIAL_FLOP      # the md steps' solvent launches take no virial
+    achieved = kpairs * flop_per_pair / (kms * 1e-3) / 1e12 if kms > 0 else None
+    traffic, traffic_src = traffic_from_profile("k_vv_mx", "fep_mix_64jobs")
     out = {
         "metric": "pair interactions/s + aggregate ns/day; p50/p99 step latency vs plain FP64 GPU",
-        "value": h["pairs_per_s"],
+        "value": bm["pairs_per_s"],
         "unit": "pair interactions/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": bm["session_steps"],
         "warmup": a.warmup,
-        "ms_per_step": h["ms_per_step"],
+        "ms_per_step": bm["wall_s"] / bm["session_steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+int64" if head == "mixed" else "f64",
+        "dtype": "f32+int64",
         "data": "synthetic",
-        "config": {"workload": f"water_32768: 32,768 SPC waters (98,304 atoms), cubic box "
-                               f"{s.box[0]:.4f} nm (0.997 g/cm3), single-range cutoff 1.4 nm, RF eps_rf 61, "
-                               f"pair list every 5 steps, SETTLE, dt 2 fs, device-resident MD",
-                   "atoms": s.n_atoms, "precision": head},
-        "ns_per_day": h["ns_per_day"],
-        "step_latency_ms": {"p50": h["p50"], "p99": h["p99"]},
-        "pairs_per_step": h["pairs_per_step"],
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "kernel": f"{kernel} (solvent-solvent pair kernel)", "avg_launch_ms": h["kern_ms"],
-                     "launches": h["launches"], "flop_per_launch": flop, "traffic_source": traffic_src,
-                     "note": "vector-ALU bound pair kernel (no dense contraction, no MFMA); peak = MI355X "
-                             + ("FP32 157.3 TFLOP/s (packed; 78.6 without packed FP32)" if head == "mixed"
-                                else "FP64 78.6 TFLOP/s")},
+        "config": {"workload": "fep_mix: 64 FEP lambda-window jobs of SPC water, 1,000-30,000 atoms (log-uniform), "
+                               "3-30 perturbed atoms each (soft-core alpha_LJ 0.5, alpha_CRF 0.5 nm^2), lambda from 21 "
+                               "windows, seeded Poisson arrivals, rc 1.4 nm, NSNB 5, 2,000 md steps per job at 2 fs, "
+                               "continuous batching on one GPU, mixed precision",
+                   "jobs": fep["jobs"], "atoms_total": fep["atoms"], "steps_per_job": fep["steps_per_job"],
+                   "precision": "mixed", "steps_meaning": "session steps (one md step of every resident job)"},
+        "ns_per_day": bm["ns_per_day"],
+        "step_latency_ms": {"p50": bm["step_ms_p50"], "p99": bm["step_ms_p99"]},
+        "speedup_vs_plain_fp64_gpu_one_at_a_time": fep["speedup_batched_mixed_vs_one_at_a_time_fp64"],
+        "plain_fp64_gpu_one_at_a_time": fep["one_at_a_time_fp64"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": traffic,
+                     "kernel": "k_vv_mx (solvent-solvent pair kernel, mixed precision, no virial)",
+                     "avg_launch_ms": kms / max(klaunch, 1), "launches": klaunch,
+                     "flop_per_launch": kpairs * flop_per_pair / max(klaunch, 1),
+                     "flop_per_pair": flop_per_pair, "traffic_source": traffic_src,
+                     "note": "FP32 vector-ALU bound pair kernel (no dense contraction, no MFMA); its launches timed "
+                             "with hipEvents on the batch's stream over the timed replay; work = the advancing jobs' "
+                             "solvent-solvent pair interactions x 42 FLOP (60 minus the 18 virial FLOP); peak = MI355X "
+                             "FP32 157.3 TFLOP/s (packed; this build keeps FP32 un-packed: 78.6 reachable)"},
+        "fep_mix": fep,
+        "water_32768": water,
     }
-    if "fp64" in res and head != "fp64":
-        p = res["fp64"]
-        out["plain_fp64_gpu"] = {"pairs_per_s": p["pairs_per_s"], "ms_per_step": p["ms_per_step"],
-                                 "ns_per_day": p["ns_per_day"], "step_latency_ms": {"p50": p["p50"], "p99": p["p99"]},
-                                 "vv_kernel_ms": p["kern_ms"],
-                                 "vv_kernel_tflops": p["vv_pairs"] * FLOP_PER_PAIR / (p["kern_ms"] * 1e-3) / 1e12}
-        out["speedup_vs_plain_fp64_gpu"] = h["pairs_per_s"] / p["pairs_per_s"]
-        # total-energy drift (least-squares slope over the timed NVE run from
-        # the lattice start, same start for both precisions)
-        out["energy_drift_kj_mol_ps"] = {"fp64": p["drift"], "mixed": h["drift"],
-                                         "rel_diff": abs(h["drift"] - p["drift"]) / abs(p["drift"]),
-                                         "steps": a.steps, "atoms": s.n_atoms, "start": "lattice"}
-        if a.drift_steps > 0:
-            out["energy_drift_equilibrated"] = drift_bench(rt, gen, s, a.drift_steps)
-        if a.drift_replicas > 0:
-            out["energy_drift_ensemble"] = drift_ensemble_bench(rt, gen, a.drift_steps, a.drift_replicas)
-        out["mixed_force_error_vs_fp64"] = force_error(rt, s)
-    out["energy_first_last"] = [h["e_first"], h["e_last"]]
-    if not a.no_fep:
-        out["fep_mix"] = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps, a.fep_admit_interval)
     if not a.no_remd:
         out["remd"] = remd_bench(rt, gen, a.remd_steps)
+        progress("remd done")
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(s, a.cpu_steps, a.cpu_steps_1core)
+        progress("cpu baseline done")
     print(json.dumps(out), flush=True)
 
 
-def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
+def water_bench(rt, gen, s, a):
+    """32,768 SPC waters: device-resident MD in both precisions, the mixed path's
+    solvent kernel timed per launch, force error and drift comparisons."""
+    res = {name: run_path(rt, s, rt.FP64 if name == "fp64" else rt.MIXED, a.steps, a.warmup, a.latency_steps)
+           for name in ("fp64", "mixed")}
+    h, p = res["mixed"], res["fp64"]
+    flop_per_pair = FLOP_PER_PAIR - VIRIAL_FLOP
+    flop = h["kern_pairs"] / max(h["launches"], 1) * flop_per_pair
+    achieved = flop / (h["kern_ms"] * 1e-3) / 1e12
+    traffic, traffic_src = traffic_from_profile("k_vv_mx", "water32768_mixed")
+    out = {"workload": f"water_32768: 32,768 SPC waters (98,304 atoms), cubic box {s.box[0]:.4f} nm (0.997 g/cm3), "
+                       "single-range cutoff 1.4 nm, RF eps_rf 61, pair list every 5 steps, SETTLE, dt 2 fs, "
+                       "device-resident MD",
+           "steps": a.steps, "warmup": a.warmup,
+           "mixed": {"pairs_per_s": h["pairs_per_s"], "ms_per_step": h["ms_per_step"], "ns_per_day": h["ns_per_day"],
+                     "step_latency_ms": {"p50": h["p50"], "p99": h["p99"]}, "pairs_per_step": h["pairs_per_step"]},
+           "plain_fp64_gpu": {"pairs_per_s": p["pairs_per_s"], "ms_per_step": p["ms_per_step"],
+                              "ns_per_day": p["ns_per_day"], "step_latency_ms": {"p50": p["p50"], "p99": p["p99"]},
+                              "vv_kernel_ms": p["kern_ms"],
+                              "vv_kernel_tflops": p["kern_pairs"] / max(p["launches"], 1) * flop_per_pair /
+                              (p["kern_ms"] * 1e-3) / 1e12},
+           "speedup_mixed_vs_plain_fp64_gpu": h["pairs_per_s"] / p["pairs_per_s"],
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": "k_vv_mx",
+                        "avg_launch_ms": h["kern_ms"], "launches": h["launches"], "flop_per_launch": flop,
+                        "flop_per_pair": flop_per_pair, "traffic_source": traffic_src},
+           "energy_drift_kj_mol_ps": {"fp64": p["drift"], "mixed": h["drift"],
+                                      "rel_diff": abs(h["drift"] - p["drift"]) / abs(p["drift"]),
+                                      "steps": a.steps, "start": "lattice"},
+           "mixed_force_error_vs_fp64": force_error(rt, s)}
+    progress("water_32768 timed runs done")
+    if a.drift_replicas > 0:
+        out["energy_drift_ensemble"] = drift_ensemble_bench(rt, gen, a.drift_steps, a.drift_replicas)
+    return out
+
+
+def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200, only_mixed=False, drift_steps=2000):
     """The 64-job FEP lambda-window mix: seeded Poisson arrivals, continuous
     batching (mixed precision, then plain FP64), against the same jobs run one at
-    a time on the plain FP64 single-job path."""
+    a time on the plain FP64 single-job path; a wall-clock serving run (arrivals
+    while the GPU works); the mixed path's force error on every job and its NVE
+    drift against FP64 on a subset."""
     import numpy as np
     srv = importlib.import_module("gromos-rs_amd.server")
     mix = gen.fep_mix()
@@ -250,18 +319,24 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
     # synthetic lattice waters relax first (untimed), then run as NVT windows:
     # Berendsen 300 K, tau 0.1 ps, dof 3N - 3 per rigid water
     systems = srv.equilibrate(systems, [300.0] * len(systems))
-    for s in systems:
-        s.thermostat = (300.0, 0.1, 3 * s.n_atoms - 3 * s.nsm)
+    nve = [q.copy() for q in systems]
+    for q in systems:
+        q.thermostat = (300.0, 0.1, 3 * q.n_atoms - 3 * q.nsm)
     sched = [dict(key=k, system=systems[k], nsteps=steps, arrival=m["arrival"]) for k, m in enumerate(mix)]
-    out = dict(jobs=len(mix), atoms=int(sum(s.n_atoms for s in systems)), steps_per_job=steps,
+    out = dict(jobs=len(mix), atoms=int(sum(q.n_atoms for q in systems)), steps_per_job=steps,
                arrival_rate_per_step=0.05, seed=3, rc=1.4, nsnb=5, thermostat="Berendsen 300 K, tau 0.1 ps",
                admit_interval=admit_interval,
                arrivals_last_step=int(max(m["arrival"] for m in mix)))
-    runs = [("batched_mixed", rt.MIXED, admit_interval), ("batched_fp64", rt.FP64, admit_interval)]
-    if admit_interval != 50:
-        runs.append(("batched_mixed_admit50", rt.MIXED, 50))
+    runs = [("batched_mixed", rt.MIXED, admit_interval)]
+    if not only_mixed:
+        runs.append(("batched_fp64", rt.FP64, admit_interval))
+        if admit_interval != 50:
+            runs.append(("batched_mixed_admit50", rt.MIXED, 50))
     for name, prec, ai in runs:
         server = srv.Server(precision=prec, nsnb=5, admit_interval=ai)
+        timed = name == "batched_mixed"
+        if timed:
+            server.sess.kernel_timing(enable=1)
         r = srv.replay(server, sched)
         sim_ns = len(mix) * steps * 0.002 * 1e-3
         out[name] = dict(pairs_per_s=r["pairs"] / r["wall"], wall_s=r["wall"], ns_per_day=sim_ns * 86400.0 / r["wall"],
@@ -272,16 +347,37 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
                          admit_interval=ai, layouts=r["layouts"], layout_ms_total=r["layout_ms"],
                          layout_concat_ms_total=r["layout_concat_ms"], session_steps=r["clock"],
                          failed_jobs=sum(1 for v in server.results.values() if v["failed"]))
+        if timed:
+            out[name]["kernel_timing"] = server.sess.kernel_timing(enable=0)
         server.sess.close()
+        progress(f"fep_mix {name} done ({r['wall']:.2f} s)")
+    if only_mixed:
+        return out
+    # wall-clock serving: a client thread submits each job at its arrival time
+    # (arrival step x the batched mixed path's mean step time) while the server
+    # thread steps the resident batch
+    sps = out["batched_mixed"]["wall_s"] / out["batched_mixed"]["session_steps"]
+    ts = srv.ThreadedServer(precision=rt.MIXED, nsnb=5, admit_interval=admit_interval)
+    rr = srv.replay_realtime(ts, sched, sps)
+    progress("fep_mix realtime done")
+    out["realtime_mixed"] = dict(pairs_per_s=rr["pairs"] / rr["wall"], wall_s=rr["wall"],
+                                 seconds_per_arrival_step=sps, layouts=rr["layouts"],
+                                 job_latency_s_p50=float(np.percentile(rr["job_latency_s"], 50)),
+                                 job_latency_s_p99=float(np.percentile(rr["job_latency_s"], 99)),
+                                 admit_wait_s_p50=float(np.percentile(rr["admit_wait_s"], 50)),
+                                 admit_wait_s_p99=float(np.percentile(rr["admit_wait_s"], 99)),
+                                 failed_jobs=sum(1 for v in ts.results.values() if v["failed"]),
+                                 mode="arrivals in wall-clock time from a client thread; the server thread owns the "
+                                      "session and admits at its 200-step boundaries")
     # one at a time, plain FP64 (rate over alone_steps per job, pair-list cadence included)
     pairs, wall = 0, 0.0
     soft, per_step = 0, 0.0
     import torch
-    for s in systems:
-        ds = rt.DeviceSystem(s, precision=rt.FP64)
+    for q in systems:
+        ds = rt.DeviceSystem(q, precision=rt.FP64)
         ds.pairlist_update()
         # soft-core pairs (first atom perturbed) of the starting pair list, untimed
-        pf = np.asarray(s.pert.flag) != 0
+        pf = np.asarray(q.pert.flag) != 0
         pl = ds.pairlist()
         soft += sum(int(pf[pl[k][:, 0]].sum()) for k in ("solute_short", "solute_long") if len(pl[k]))
         p0 = ds.pair_count
@@ -293,6 +389,7 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
         per_step += (ds.pair_count - p0) / alone_steps
         ds.close()
     out["one_at_a_time_fp64"] = dict(pairs_per_s=pairs / wall, steps_per_job_timed=alone_steps, wall_s=wall)
+    progress("fep_mix one at a time done")
     # soft-core pairs reported separately (they are part of the pair counts above)
     out["softcore_pairs"] = dict(per_step_all_jobs=soft, fraction_of_pairs=soft / per_step,
                                  source="starting pair lists of the 64 jobs (pairs whose first atom is perturbed)")
@@ -301,7 +398,71 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200):
     if "batched_mixed_admit50" in out:
         out["speedup_batched_mixed_admit50_vs_one_at_a_time_fp64"] = (out["batched_mixed_admit50"]["pairs_per_s"] /
                                                                       (pairs / wall))
+    out["mixed_force_error_vs_fp64"] = mix_force_error(rt, systems)
+    progress("fep_mix force error done")
+    if drift_steps > 0:
+        out["nve_drift_subset"] = nve_drift(rt, nve[::8], drift_steps, "every 8th job of the mix (8 jobs), "
+                                            "equilibrated start, thermostat off")
     return out
+
+
+def mix_force_error(rt, systems, start="each job's state after 300 untimed Berendsen steps"):
+    """Mixed vs plain FP64 GPU forces (the plain path matches the CPU oracle to
+    1e-10 in the parity tests) on each system: per atom |dF_i| / max(|F_i|,
+    1 kJ/mol/nm), over all atoms of all systems."""
+    import numpy as np
+    errs, pert_errs, job_max = [], [np.zeros(0)], []
+    for q in systems:
+        out = {}
+        for prec in (rt.FP64, rt.MIXED):
+            ds = rt.DeviceSystem(q, precision=prec)
+            ds.pairlist_update()
+            out[prec] = ds.forcefield_apply()["forces"]
+            ds.close()
+        d = np.linalg.norm(out[rt.MIXED] - out[rt.FP64], axis=1) / np.maximum(np.linalg.norm(out[rt.FP64], axis=1), 1.0)
+        errs.append(d)
+        if q.pert is not None:
+            pert_errs.append(d[np.asarray(q.pert.flag) != 0])
+        job_max.append(float(d.max()))
+    e = np.concatenate(errs)
+    pe = np.concatenate(pert_errs)
+    return dict(max=float(e.max()), median=float(np.median(e)), p99_9=float(np.percentile(e, 99.9)),
+                atoms=int(e.size), systems=len(systems), systems_over_1e_5=int(sum(m > 1e-5 for m in job_max)),
+                perturbed_atoms_max=float(pe.max()) if pe.size else None, f_floor=1.0, start=start)
+
+
+def nve_drift(rt, systems, steps, what):
+    """NVE total-energy slope (least squares over `steps` steps) of each system in
+    FP64 and in mixed precision from the same start, all batched in one session
+    per precision; ensemble means with standard errors and the paired
+    difference per system."""
+    import numpy as np
+    slopes = {}
+    for name, prec in (("fp64", rt.FP64), ("mixed", rt.MIXED)):
+        sess = rt.Session(precision=prec, nsnb=5)
+        jobs = []
+        for q in systems:
+            q = q.copy()
+            q.thermostat = None
+            jobs.append(sess.admit(q, steps))
+        sess.step(steps)
+        sl = []
+        for j, q in zip(jobs, systems):
+            st, done = sess.query(j)
+            if st != rt.DONE or done != steps:
+                raise rt.GromosAmdError(f"drift job {j} ended in state {st} after {done} steps")
+            e = sess.energies(j, 0, steps)
+            sl.append(float(np.polyfit(np.arange(steps) * q.dt, e[:, 2] + e[:, 3], 1)[0]))
+        sess.close()
+        slopes[name] = np.array(sl)
+    f, m = slopes["fp64"], slopes["mixed"]
+    sem = lambda v: float(v.std(ddof=1) / np.sqrt(len(v))) if len(v) > 1 else float("nan")
+    paired = (m - f) / np.abs(f)
+    return {"systems": what, "n": len(systems), "steps": steps,
+            "fp64_mean": float(f.mean()), "fp64_sem": sem(f), "mixed_mean": float(m.mean()), "mixed_sem": sem(m),
+            "rel_diff_of_means": float(abs(m.mean() - f.mean()) / abs(f.mean())),
+            "paired_rel_diff_mean": float(paired.mean()), "paired_rel_diff_sem": sem(paired),
+            "fp64": f.tolist(), "mixed": m.tolist()}
 
 
 def remd_bench(rt, gen, steps):
@@ -315,6 +476,7 @@ def remd_bench(rt, gen, steps):
     systems = [gen.water_replica(seed=100 + k, temperature=temps[k], n_waters=4000, rc=1.2) for k in range(n)]
     srv = importlib.import_module("gromos-rs_amd.server")
     systems = srv.equilibrate(systems, temps)          # untimed relaxation of the lattice starts
+    progress("remd equilibrated")
     sess = rt.Session(precision=rt.MIXED, nsnb=5)
     rx = remd.ReplicaExchange(sess, systems, temps, exchange_interval=500, scheme="odd_even", seed=7,
                               total_steps=steps)
@@ -326,86 +488,50 @@ def remd_bench(rt, gen, steps):
     i1 = sess.info()
     ms = sess.step_times(enable=0)
     seg = min(500, steps)
-    drift, failed = [], 0
-    for j in rx.jobs:
+    drift, failed, final = [], 0, []
+    for j, q in zip(rx.jobs, systems):
         st, done = sess.query(j)
         failed += st == rt.FAILED
-        if done >= seg:
-            e = sess.energies(j, 0, seg)
-            tot = e[:, 2] + e[:, 3]
-            drift.append((tot[-1] - tot[0]) / (seg * 0.002))   # kJ/mol/ps over the first NVE segment
+        e = sess.energies(j, 0, done)
+        tot = e[:, 2] + e[:, 3]
+        # slope of E_tot within each NVE segment (exchanges rescale velocities
+        # between segments), averaged per replica
+        sl = [np.polyfit(np.arange(seg) * 0.002, tot[a:a + seg], 1)[0] for a in range(0, done - seg + 1, seg)]
+        drift.append(float(np.mean(sl)) if sl else float("nan"))
+        x, v = sess.fetch(j)
+        f = q.copy()
+        f.pos, f.vel = x, v
+        final.append(f)
     sess.close()
+    drift = np.array(drift)
+    ferr = mix_force_error(rt, final, start="every replica's final state")
     return dict(replicas=n, atoms_per_replica=systems[0].n_atoms, box_nm=float(systems[0].box[0]), rc=1.2,
                 steps=steps, exchange_interval=500, pairs_per_s=(i1["pair_count"] - i0["pair_count"]) / wall,
                 ns_per_day=n * steps * 0.002 * 1e-3 * 86400.0 / wall, wall_s=wall,
                 step_ms_p50=float(np.percentile(ms, 50)), step_ms_p99=float(np.percentile(ms, 99)),
                 exchange_attempts=rx.attempts, exchange_accepted=rx.accepted,
                 failed_replicas=int(failed),
-                nve_drift_kj_mol_ps_median_abs=float(np.median(np.abs(drift))) if drift else None,
+                nve_drift_kj_mol_ps_per_replica=dict(median=float(np.median(drift)), min=float(drift.min()),
+                                                     max=float(drift.max()),
+                                                     median_per_atom=float(np.median(drift)) / systems[0].n_atoms,
+                                                     note="mean over the replica's 500-step NVE segments of the "
+                                                          "least-squares E_tot slope"),
+                mixed_force_error_vs_fp64=ferr,
                 precision="mixed", start="lattice waters relaxed 300 steps under Berendsen tau 0.01 ps (untimed)")
-
-
-def drift_bench(rt, gen, s, steps):
-    """NVE total-energy drift of the same system after 300 untimed Berendsen
-    steps (tau 0.01 ps, 300 K) in FP64 and mixed precision from the same state:
-    least-squares slope of E_pot + E_kin over `steps` steps."""
-    import numpy as np
-    srv = importlib.import_module("gromos-rs_amd.server")
-    eq = srv.equilibrate([s], [300.0])[0]
-    out = {"steps": steps, "atoms": eq.n_atoms, "start": "lattice relaxed 300 steps under Berendsen tau 0.01 ps"}
-    for name, prec in (("fp64", rt.FP64), ("mixed", rt.MIXED)):
-        ds = rt.DeviceSystem(eq, precision=prec)
-        ds.pairlist_update()
-        e = ds.md(steps)
-        ds.close()
-        tot = e[:, 2] + e[:, 3]
-        out[name] = float(np.polyfit(np.arange(steps) * eq.dt, tot, 1)[0])
-        out[name + "_per_atom"] = out[name] / eq.n_atoms
-    out["rel_diff"] = abs(out["mixed"] - out["fp64"]) / abs(out["fp64"])
-    return out
 
 
 def drift_ensemble_bench(rt, gen, steps, replicas):
     """Drift over an ensemble: `replicas` copies of the 32,768-water system
     (lattice seeds 2, 3, ...) relaxed together (300 Berendsen steps, tau 0.01 ps,
-    300 K), then NVE for `steps` steps, all replicas batched in one session, in
-    FP64 and in mixed precision from the same states.  A single trajectory's
-    slope carries sampling noise (trajectories from nearby states separate
-    within a few hundred steps), so the comparison is the ensemble mean with its
-    standard error, and per replica the paired difference mixed - FP64."""
-    import numpy as np
+    300 K), then NVE for `steps` steps in FP64 and in mixed precision from the
+    same states.  A single trajectory's slope carries sampling noise
+    (trajectories from nearby states separate within a few hundred steps), so
+    the comparison is the ensemble mean with its standard error, and per
+    replica the paired difference mixed - FP64."""
     srv = importlib.import_module("gromos-rs_amd.server")
     systems = [gen.water_32768(seed=2 + k) for k in range(replicas)]
     eq = srv.equilibrate(systems, [300.0] * replicas)
-    slopes = {}
-    for name, prec in (("fp64", rt.FP64), ("mixed", rt.MIXED)):
-        sess = rt.Session(precision=prec, nsnb=5)
-        jobs = [sess.admit(q, steps) for q in eq]
-        sess.step(steps)
-        sl = []
-        for j, q in zip(jobs, eq):
-            st, done = sess.query(j)
-            if st != rt.DONE or done != steps:
-                raise rt.GromosAmdError(f"drift replica {j} ended in state {st} after {done} steps")
-            e = sess.energies(j, 0, steps)
-            sl.append(float(np.polyfit(np.arange(steps) * q.dt, e[:, 2] + e[:, 3], 1)[0]))
-        sess.close()
-        slopes[name] = np.array(sl)
-    f, m = slopes["fp64"], slopes["mixed"]
-    sem = lambda v: float(v.std(ddof=1) / np.sqrt(len(v))) if len(v) > 1 else float("nan")
-    paired = (m - f) / np.abs(f)
-    return {"replicas": replicas, "steps": steps, "atoms_per_replica": eq[0].n_atoms,
-            "start": "lattice seeds 2..%d relaxed 300 steps under Berendsen tau 0.01 ps" % (1 + replicas),
-            "fp64_mean": float(f.mean()), "fp64_sem": sem(f), "mixed_mean": float(m.mean()), "mixed_sem": sem(m),
-            "rel_diff_of_means": float(abs(m.mean() - f.mean()) / abs(f.mean())),
-            "paired_rel_diff_mean": float(paired.mean()), "paired_rel_diff_sem": sem(paired),
-            "fp64": f.tolist(), "mixed": m.tolist()}
-
-
-def ds_vv_pairs(ds, s):
-    """atom-pair interactions of the solvent-solvent short list per step (9 per sentinel)"""
-    pl = ds.pairlist()
-    return len(pl["solvent_short"]) * s.atoms_per_solvent ** 2
+    return nve_drift(rt, eq, steps, "lattice seeds 2..%d relaxed 300 steps under Berendsen tau 0.01 ps" % (1 + replicas))
 
 
 if __name__ == "__main__":

@@ -614,8 +614,11 @@ extern "C" int gromos_amd_system_kernel_timing(gromos_amd_system* S, int32_t ena
         if (on && !S->timing) {
             S->timed_ms = 0.0;
             S->timed_launches = 0;
+            S->timed_pairs = 0;
         }
         S->timing = on;
     }
     return 0;
 }
+
+extern "C" int64_t gromos_amd_system_kernel_pairs(const gromos_amd_system* S) { return S ? S->timed_pairs : -1; }

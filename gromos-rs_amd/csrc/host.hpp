@@ -228,6 +228,7 @@ struct gromos_amd_system {
     size_t ev_used = 0;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
+    int64_t timed_pairs = 0;    // pair interactions the timed launches evaluated
     // per-step durations (hipEvents between steps) while enabled
     bool step_timing = false;
     std::vector<hipEvent_t> step_ev;

@@ -135,6 +135,10 @@ struct gromos_amd_session {
     double concat_ms_gone = 0.0;   // concatenation time of batches already released
     DevScratch stage, map, small;
     bool step_timing = false;
+    // per-launch timing of the solvent pair kernel, kept across batch changes
+    bool kernel_timing = false;
+    double kern_ms_gone = 0.0;
+    int64_t kern_launches_gone = 0, kern_pairs_gone = 0;
     std::vector<double> step_ms;
 };
 
@@ -148,6 +152,15 @@ static int running_count(const gromos_amd_session* s) {
     for (const SJob* j : s->batch)
         if (j->state == GROMOS_AMD_JOB_RUNNING && !j->evict) ++n;
     return n;
+}
+
+// a batch system about to be released: its counters move to the session's
+static void retire_timing(gromos_amd_session* s, gromos_amd_system* S) {
+    s->concat_ms_gone += S->concat_ms;
+    if (S->collect_timing() != 0) (void)hipGetLastError();
+    s->kern_ms_gone += S->timed_ms;
+    s->kern_launches_gone += S->timed_launches;
+    s->kern_pairs_gone += S->timed_pairs;
 }
 
 // a job's admission reservations end
@@ -201,7 +214,7 @@ static int relayout(gromos_amd_session* s) {
     s->queue.clear();
     s->batch.clear();
     if (nb.empty()) {
-        if (S) s->concat_ms_gone += S->concat_ms;
+        if (S) retire_timing(s, S);
         s->S.reset();
         s->layouts++;
         s->layout_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -211,6 +224,7 @@ static int relayout(gromos_amd_session* s) {
         s->S = std::make_unique<gromos_amd_system>();
         S = s->S.get();
         S->step_timing = s->step_timing;
+        S->timing = s->kernel_timing;
     }
     std::vector<const JobImage*> imgs;
     int64_t batch_atoms = 0;
@@ -243,7 +257,7 @@ static int relayout(gromos_amd_session* s) {
             release(s, j);
         }
         s->queue = queued;
-        if (S) s->concat_ms_gone += S->concat_ms;
+        if (S) retire_timing(s, S);
         s->S.reset();
         s->layouts++;
         return fail(rc, "%s", why.c_str());
@@ -615,6 +629,34 @@ extern "C" int gromos_amd_session_step_times(gromos_amd_session* s, int32_t enab
         s->step_timing = enable != 0;
         if (s->step_timing) s->step_ms.clear();
         if (s->S) s->S->step_timing = s->step_timing;
+    }
+    return 0;
+}
+
+extern "C" int gromos_amd_session_kernel_timing(gromos_amd_session* s, int32_t enable, double* total_ms,
+                                                int64_t* launches, int64_t* pairs) {
+    if (!s) return fail(GROMOS_AMD_EINVAL, "session_kernel_timing: null session");
+    if (int rc = set_device(s)) return rc;
+    gromos_amd_system* S = s->S.get();
+    if (S)
+        if (int rc = S->collect_timing()) return rc;
+    if (total_ms) *total_ms = s->kern_ms_gone + (S ? S->timed_ms : 0.0);
+    if (launches) *launches = s->kern_launches_gone + (S ? S->timed_launches : 0);
+    if (pairs) *pairs = s->kern_pairs_gone + (S ? S->timed_pairs : 0);
+    if (enable >= 0) {
+        const bool on = enable != 0;
+        if (on && !s->kernel_timing) {
+            s->kern_ms_gone = 0.0;
+            s->kern_launches_gone = 0;
+            s->kern_pairs_gone = 0;
+            if (S) {
+                S->timed_ms = 0.0;
+                S->timed_launches = 0;
+                S->timed_pairs = 0;
+            }
+        }
+        s->kernel_timing = on;
+        if (S) S->timing = on;
     }
     return 0;
 }

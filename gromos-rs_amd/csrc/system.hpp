@@ -64,6 +64,9 @@ enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 
 
 // per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]
 constexpr int REC = 14;
+// words per job in the pair-list build's control block: short and long pair
+// interactions, solvent-solvent list entries
+constexpr int JOB_CTL = 3;
 
 // parts per job of the solvent-solvent energy and E_kin records (two-stage
 // reduction); PART_W columns: e_lj, e_crf, virial[9], E_kin, E_kin_new

@@ -249,7 +249,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         if (im->P.nsnb != I0.P.nsnb) return fail(GROMOS_AMD_EINVAL, "jobs of one batch share the rebuild cadence");
     S->precision = precision;
     S->njobs = njobs;
-    S->host_job_pairs.assign(2 * (size_t)njobs, 0);
+    S->host_job_pairs.assign(JOB_CTL * (size_t)njobs, 0);
     S->jobs.clear();
     S->job_n_atoms.clear();
     const auto t_concat = std::chrono::steady_clock::now();
@@ -552,7 +552,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_s_atom_cg, npa, true)) || (rc = S->reserve(S->d_o2s, npa, true)) ||
         (rc = S->reserve(S->d_s_type, npa, true)) || (rc = S->reserve(S->d_s_prec, npa, true)) || (rc = S->reserve(S->d_vv_cnt, nc + 1, true)) ||
         (rc = S->reserve(S->d_gen_cnt, nc + 1, true)) || (rc = S->reserve(S->d_vv_off, nc + 1, true)) ||
-        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_ctl, 6 + 2 * (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_gen_off, nc + 1, true)) || (rc = S->reserve(S->d_ctl, 6 + JOB_CTL * (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_gsol, nc, true)) ||
         (rc = S->reserve(S->d_pc_short, nc, true)) || (rc = S->reserve(S->d_pc_long, nc, true)) ||
         (rc = S->reserve(S->d_xq, npa, true)) || (rc = S->reserve(S->d_vv_cap, nc + 1, true)) ||
@@ -574,12 +574,12 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->d_counters = S->d_ctl;
     S->d_overflow = (int32_t*)(S->d_ctl + 5);
     S->d_job_pairs = (int64_t*)(S->d_ctl + 6);
-    if (S->h_ctl_n < 6 + 2 * (size_t)njobs) {
+    if (S->h_ctl_n < 6 + JOB_CTL * (size_t)njobs) {
         if (S->h_ctl) HIPCHK(hipHostFree(S->h_ctl));
         S->h_ctl = nullptr;
         S->h_ctl_n = 0;
-        HIPCHK(hipHostMalloc((void**)&S->h_ctl, (6 + 2 * (size_t)njobs) * sizeof(uint64_t), hipHostMallocDefault));
-        S->h_ctl_n = 6 + 2 * (size_t)njobs;
+        HIPCHK(hipHostMalloc((void**)&S->h_ctl, (6 + JOB_CTL * (size_t)njobs) * sizeof(uint64_t), hipHostMallocDefault));
+        S->h_ctl_n = 6 + JOB_CTL * (size_t)njobs;
     }
     // set once per layout, kept by every build: padding atoms' sorted slots map to
     // no atom; cell counts start at zero (each build's k_scatter counts them back
@@ -642,12 +642,12 @@ static int scan_offsets(gromos_amd_system* S, const int64_t* cnt_vv, const int64
 // the control block's readback (enqueued after the search) -> host counters
 static int parse_ctl(gromos_amd_system* S) {
     for (int c = 0; c < 5; ++c) S->host_counters[c] = S->h_ctl[c];
-    for (size_t b = 0; b < 2 * (size_t)S->njobs; ++b) S->host_job_pairs[b] = (int64_t)S->h_ctl[6 + b];
+    for (size_t b = 0; b < JOB_CTL * (size_t)S->njobs; ++b) S->host_job_pairs[b] = (int64_t)S->h_ctl[6 + b];
     return 0;
 }
 
 static int read_ctl(gromos_amd_system* S) {
-    const size_t nctl = 6 + 2 * (size_t)S->njobs;
+    const size_t nctl = 6 + JOB_CTL * (size_t)S->njobs;
     HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
     return parse_ctl(S);
@@ -690,6 +690,7 @@ static void build_done(gromos_amd_system* S) {
     if (S->njobs == 1) {   // one job: its counts are the totals
         S->host_job_pairs[0] = (int64_t)S->host_counters[0];
         S->host_job_pairs[1] = (int64_t)S->host_counters[1];
+        S->host_job_pairs[2] = (int64_t)S->host_counters[2];
     }
     S->prev_tot_vv = (int64_t)S->host_counters[2];
     S->prev_tot_gen = (int64_t)S->host_counters[3];
@@ -701,7 +702,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
     hipStream_t st = S->stream;
     SysPtrs p = S->ptrs();
     int32_t ncg = S->ncg;
-    const size_t nctl = 6 + 2 * (size_t)S->njobs;
+    const size_t nctl = 6 + JOB_CTL * (size_t)S->njobs;
     // reference points, cell keys and counts (the control block cleared here)
     launch_cg_ref(p, ncg, (int32_t)nctl, st);
     size_t tb = S->temp_bytes;
@@ -852,7 +853,7 @@ static int64_t active_pairs(const gromos_amd_system* S, int which) {
     if (all) return (int64_t)S->host_counters[which];
     int64_t n = 0;
     for (int32_t b = 0; b < S->njobs; ++b)
-        if (S->md_step < S->jobs[b].stop) n += S->host_job_pairs[2 * (size_t)b + which];
+        if (S->md_step < S->jobs[b].stop) n += S->host_job_pairs[JOB_CTL * (size_t)b + which];
     return n;
 }
 
@@ -911,6 +912,9 @@ int forcefield(gromos_amd_system* S, bool assemble) {
         p = S->ptrs();
         if (int rc = launch_first(nullptr, nullptr)) return rc;
     }
+    // the timed solvent pair launch's work: the advancing jobs' solvent-solvent
+    // molecule pairs (each listed from both sides) times the atom pairs of one
+    if (e0) S->timed_pairs += active_pairs(S, 2) / 2 * (int64_t)aps * aps;
     const int32_t ngsol = (int32_t)S->host_counters[4];
     launch_gen(p, npad, false, S->do_virial, mixed, ngsol, st, split ? GEN_SOLVENT : GEN_ALL);
     S->pair_count += active_pairs(S, 0);

@@ -442,23 +442,25 @@ __global__ void __launch_bounds__(BLOCK) k_totals(SysPtrs p, int32_t ncg) {
     }
 }
 
-// per-job pair interactions of this build (short, long; job_pairs cleared
-// before): blocks (x, j) sum slices of job j's sorted groups, which are
-// contiguous from cg_off, so a step can count only the jobs that advance in it
+// per-job counts of this build (pair interactions short, long; solvent-solvent
+// list entries; job_pairs cleared before): blocks (x, j) sum slices of job j's
+// sorted groups, which are contiguous from cg_off, so a step can count only the
+// jobs that advance in it
 constexpr int JOB_PAIR_SLICES = 16;
 __global__ void __launch_bounds__(BLOCK) k_job_pairs(SysPtrs p) {
-    __shared__ double lds[17 * 2];
+    __shared__ double lds[17 * JOB_CTL];
     const JobDev& J = p.jobs[blockIdx.y];
     const int32_t k1 = J.cg_off + J.n_cg;
-    double v[2] = {0.0, 0.0};   // exact: each sum stays below 2^53
+    double v[JOB_CTL] = {0.0, 0.0, 0.0};   // exact: each sum stays below 2^53
     for (int32_t k = J.cg_off + (int32_t)(blockIdx.x * BLOCK + threadIdx.x); k < k1; k += JOB_PAIR_SLICES * BLOCK) {
         v[0] += (double)p.pc_short[k];
         v[1] += (double)p.pc_long[k];
+        v[2] += (double)p.vv_cnt[k];
     }
-    block_sum<2>(v, lds);
-    if (threadIdx.x < 2) {
-        const unsigned long long t = (unsigned long long)lds[16 * 2 + threadIdx.x];
-        if (t) atomicAdd((unsigned long long*)p.job_pairs + 2 * blockIdx.y + threadIdx.x, t);
+    block_sum<JOB_CTL>(v, lds);
+    if (threadIdx.x < JOB_CTL) {
+        const unsigned long long t = (unsigned long long)lds[16 * JOB_CTL + threadIdx.x];
+        if (t) atomicAdd((unsigned long long*)p.job_pairs + JOB_CTL * blockIdx.y + threadIdx.x, t);
     }
 }
 

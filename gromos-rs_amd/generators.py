@@ -158,26 +158,31 @@ def _periodic_min_dist2(a, b, L):
 
 def _random_walk(rng, n, bond=0.15, start=None, min_dist=0.22, L=None):
     """Self-avoiding-ish chain: successive bonds at >= 70 degrees turning
-    freedom, new atoms at least min_dist from all but their two predecessors."""
+    freedom, new atoms at least min_dist from all but their two predecessors
+    (where 200 tries find no such place, the try farthest from the others)."""
     pos = np.zeros((n, 3))
     pos[0] = start if start is not None else rng.uniform(0, 1, 3)
     prev_dir = None
     for i in range(1, n):
+        best, best_d2, best_dir = None, -1.0, None
         for _ in range(200):
             d = rng.normal(size=3)
             d /= np.linalg.norm(d)
             if prev_dir is not None and np.dot(d, prev_dir) > 0.34:   # angle to previous bond > ~110 deg apart
                 continue
             cand = pos[i - 1] + bond * d
+            d2 = np.inf
             if i > 2:
                 dd = pos[: i - 2] - cand
                 if L is not None:
                     dd -= L * np.round(dd / L)
-                if (dd * dd).sum(1).min() < min_dist * min_dist:
-                    continue
-            break
-        pos[i] = cand
-        prev_dir = -d
+                d2 = (dd * dd).sum(1).min()
+            if d2 > best_d2:
+                best, best_d2, best_dir = cand, d2, d
+            if d2 >= min_dist * min_dist:
+                break
+        pos[i] = best
+        prev_dir = -best_dir
     return pos
 
 

@@ -158,6 +158,9 @@ def _declare(L):
     L.gromos_amd_jobs_swap_configurations.argtypes = [P, I64, I64]
     L.gromos_amd_session_get_info.argtypes = [P, C.POINTER(SessionInfo)]
     L.gromos_amd_session_step_times.argtypes = [P, I32, P, I64, C.POINTER(I64)]
+    L.gromos_amd_session_kernel_timing.argtypes = [P, I32, C.POINTER(D), C.POINTER(I64), C.POINTER(I64)]
+    L.gromos_amd_system_kernel_pairs.argtypes = [P]
+    L.gromos_amd_system_kernel_pairs.restype = I64
 
 
 def check(rc):
@@ -337,6 +340,11 @@ class DeviceSystem:
         check(lib().gromos_amd_system_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    @property
+    def kernel_pairs(self):
+        """pair interactions the timed solvent-kernel launches evaluated"""
+        return lib().gromos_amd_system_kernel_pairs(self.h)
+
 
 EBUSY = -5
 QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
@@ -448,3 +456,9 @@ class Session:
         ms = np.zeros(n.value)
         check(lib().gromos_amd_session_step_times(self.h, int(enable), ptr(ms), n.value, C.byref(n)))
         return ms
+
+    def kernel_timing(self, enable=-1):
+        """(total ms, launches, pair interactions) of the timed solvent-kernel launches"""
+        ms, n, pr = D(), I64(), I64()
+        check(lib().gromos_amd_session_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n), C.byref(pr)))
+        return ms.value, n.value, pr.value

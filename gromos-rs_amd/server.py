@@ -43,6 +43,7 @@ class Server:
             self.waiting.popleft()
             w["job"] = jid
             w["clock_admit"] = self.clock
+            w["t_admit"] = time.perf_counter()
             self.running[jid] = w
 
     def _collect(self):
@@ -99,6 +100,101 @@ def replay(server, schedule, chunk=None):
                 layout_ms=info["layout_ms"] - info0["layout_ms"],
                 layout_concat_ms=info["layout_concat_ms"] - info0["layout_concat_ms"], step_ms=step_ms, job_latency_s=lat,
                 job_latency_steps=lat_steps, clock=info["clock"])
+
+
+class ThreadedServer:
+    """The server loop on a worker thread that owns the session (one host thread
+    per session); clients call submit() from any thread at any time.  While the
+    worker is inside a session step (the GPU runs the resident batch; the
+    library call releases the interpreter lock), new requests queue up and are
+    admitted at the next admission boundary as memory allows, so arrivals
+    overlap the work in flight.  Results carry wall-clock submit, admit and
+    completion times."""
+
+    def __init__(self, precision=rt.MIXED, nsnb=5, admit_interval=0, hbm_budget=0, kernel_timing=False):
+        import queue
+        import threading
+        self._q = queue.Queue()
+        self._stop = threading.Event()
+        self._err = None
+        self.results = {}
+        self.layouts = 0
+        self.pairs = 0
+        self.kernel = (0.0, 0, 0)
+        self._args = (precision, nsnb, admit_interval, hbm_budget, kernel_timing)
+        self._ready = threading.Event()
+        self._t = threading.Thread(target=self._run, daemon=True)
+        self._t.start()
+        self._ready.wait()
+
+    def submit(self, key, system, nsteps):
+        self._q.put(dict(key=key, system=system, nsteps=int(nsteps), t_submit=time.perf_counter()))
+
+    def close(self, expected):
+        """wait until `expected` results are in, stop the worker, re-raise its error"""
+        while len(self.results) < expected and self._t.is_alive():
+            time.sleep(0.002)
+        self._stop.set()
+        self._t.join()
+        if self._err is not None:
+            raise self._err
+
+    def _run(self):
+        precision, nsnb, admit_interval, hbm_budget, kernel_timing = self._args
+        try:
+            srv = Server(precision=precision, nsnb=nsnb, admit_interval=admit_interval, hbm_budget=hbm_budget)
+        except Exception as e:   # noqa: BLE001 - handed to the caller
+            self._err = e
+            self._ready.set()
+            return
+        info0 = srv.sess.info()
+        if kernel_timing:
+            srv.sess.kernel_timing(enable=1)
+        self._ready.set()
+        try:
+            while not self._stop.is_set():
+                while True:
+                    try:
+                        w = self._q.get_nowait()
+                    except Exception:   # queue.Empty
+                        break
+                    srv.waiting.append(dict(key=w["key"], system=w["system"], nsteps=w["nsteps"],
+                                            t_submit=w["t_submit"], clock_submit=srv.clock))
+                if srv.idle():
+                    time.sleep(0.0005)
+                    continue
+                srv.step(srv.admit_interval - srv.clock % srv.admit_interval)
+                for k, r in srv.results.items():
+                    if k not in self.results:
+                        self.results[k] = r
+        except Exception as e:   # noqa: BLE001 - handed to the caller
+            self._err = e
+        finally:
+            info = srv.sess.info()
+            self.layouts = info["layouts"] - info0["layouts"]
+            self.pairs = info["pair_count"] - info0["pair_count"]
+            if kernel_timing:
+                self.kernel = srv.sess.kernel_timing(enable=0)
+            srv.sess.close()
+
+
+def replay_realtime(server, schedule, seconds_per_step):
+    """Submit `schedule` (dicts with key, system, nsteps, arrival in md steps)
+    to a ThreadedServer in wall-clock time, arrival k at arrival *
+    seconds_per_step seconds, from this (client) thread, while the server
+    works; returns wall seconds until the last result and per-job latencies."""
+    pending = sorted(schedule, key=lambda d: d["arrival"])
+    t0 = time.perf_counter()
+    for d in pending:
+        dt = t0 + d["arrival"] * seconds_per_step - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        server.submit(d["key"], d["system"], d["nsteps"])
+    server.close(len(pending))
+    wall = max(r["t_done"] for r in server.results.values()) - t0
+    lat = np.array([r["t_done"] - r["t_submit"] for r in server.results.values()])
+    queued = np.array([r["t_admit"] - r["t_submit"] for r in server.results.values()])
+    return dict(wall=wall, pairs=server.pairs, layouts=server.layouts, job_latency_s=lat, admit_wait_s=queued)
 
 
 def equilibrate(systems, temperatures, steps=300, tau=0.01, precision=rt.MIXED, nsnb=5):

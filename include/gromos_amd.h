@@ -313,6 +313,10 @@ int64_t gromos_amd_system_pair_count(const gromos_amd_system *sys);
  * launch while enabled.  total_ms / launches since enable. */
 int gromos_amd_system_kernel_timing(gromos_amd_system *sys, int32_t enable, double *total_ms,
                                     int64_t *launches);
+/* Pair interactions the timed launches of the dominant kernel evaluated since
+ * timing was enabled (the solvent-solvent pairs of the jobs that advanced:
+ * the unit of that kernel's roofline).  No counterpart in the reference. */
+int64_t gromos_amd_system_kernel_pairs(const gromos_amd_system *sys);
 /* HBM bytes held by the system's device buffers. */
 int64_t gromos_amd_system_device_bytes(const gromos_amd_system *sys);
 /* Lanes per solvent molecule the autotuner chose for the pair kernel (4, 8, 16;
@@ -401,6 +405,13 @@ int gromos_amd_session_get_info(gromos_amd_session *s, gromos_amd_session_info *
 /* Device time of each session step (hipEvents) while enabled: enable = 1 clears
  * and starts, 0 stops, -1 leaves it; copies up to cap values, *n = how many. */
 int gromos_amd_session_step_times(gromos_amd_session *s, int32_t enable, double *ms, int64_t cap, int64_t *n);
+/* gromos_amd_system_kernel_timing over the session's batches (kept across
+ * layout changes): hipEvents around each launch of the solvent pair kernel on
+ * the batch's stream while enabled; total_ms, launches and the pair
+ * interactions those launches evaluated since enable (any output may be NULL;
+ * enable = 1 clears and starts, 0 stops, -1 leaves it). */
+int gromos_amd_session_kernel_timing(gromos_amd_session *s, int32_t enable, double *total_ms,
+                                     int64_t *launches, int64_t *pairs);
 
 #ifdef __cplusplus
 }

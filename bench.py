@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=15, help="timed oracle MD steps of the CPU baseline (about 10-15 s)")
-    ap.add_argument("--cpu-steps-1core", type=int, default=2,
-                    help="timed oracle MD steps of the single-core CPU baseline (about 10 s)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=5,
+                    help="timed oracle MD steps of the single-core CPU baseline (one pair-list cycle, about 10 s)")
     ap.add_argument("--only", choices=["fep-mixed", "water"], default=None,
                     help="run one part only (for profiling): the FEP mix's batched mixed replay, or the water box")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -318,7 +318,9 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200, only_mixed=Fa
     systems = [gen.fep_job(m["n_atoms"], m["n_pert"], m["lam"], m["seed"]) for m in mix]
     # synthetic lattice waters relax first (untimed), then run as NVT windows:
     # Berendsen 300 K, tau 0.1 ps, dof 3N - 3 per rigid water
+    progress("fep_mix jobs generated")
     systems = srv.equilibrate(systems, [300.0] * len(systems))
+    progress("fep_mix jobs equilibrated")
     nve = [q.copy() for q in systems]
     for q in systems:
         q.thermostat = (300.0, 0.1, 3 * q.n_atoms - 3 * q.nsm)

@@ -68,3 +68,13 @@ def test_job_footprint_host_side():
     bad.iac[0] = bad.n_types       # type out of range
     with pytest.raises(rt.GromosAmdError, match="type out of range"):
         rt.Session.footprint(bad)
+
+
+def test_rust_binding_declares_every_entry_point():
+    """The Rust extern block a maintainer would add binds every C entry point."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text[text.index('#[link(name = "gromos_amd")]'):]
+    block = block[:block.index("\n}\n")]
+    bound = set(re.findall(r"pub fn (gromos_amd_[a-z0-9_]+)\(", block))
+    missing = [s for s in declared_symbols() if s not in bound]
+    assert not missing, missing

@@ -403,8 +403,8 @@ def fep_mix_bench(rt, gen, steps, alone_steps, admit_interval=200, only_mixed=Fa
     out["mixed_force_error_vs_fp64"] = mix_force_error(rt, systems)
     progress("fep_mix force error done")
     if drift_steps > 0:
-        out["nve_drift_subset"] = nve_drift(rt, nve[::8], drift_steps, "every 8th job of the mix (8 jobs), "
-                                            "equilibrated start, thermostat off")
+        out["nve_drift"] = nve_drift(rt, nve, drift_steps, "all 64 jobs of the mix, equilibrated start, "
+                                     "thermostat off, batched")
     return out
 
 

@@ -24,25 +24,63 @@ __device__ __forceinline__ d3 ld3(const double* p, int64_t i) {
     return d3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
 }
 
-// Boundary: kind 0 = vacuum, 1 = rectangular.  invL = 1.0 / L computed once on
-// the host (DVec3::recip, math.rs:49-55).
+// Boundary: kind 0 = vacuum, 1 = rectangular, 2 = triclinic.  invL = 1.0 / L
+// computed once on the host (DVec3::recip, math.rs:49-55).  Triclinic: the
+// lower-triangular box vectors a, b, c (math.rs:75-92); L holds the diagonal.
 struct Box {
     double L[3];
     double invL[3];
     int32_t kind;
     int32_t pad;
+    double a[3], b[3], c[3];
 };
 
-__device__ __forceinline__ double nim1(double d, double L, double invL, int kind) {
-    return kind ? d - L * round(d * invL) : d;
+// Triclinic::nearest_image (math.rs:105-141): z, then y, then x reduction loops
+// in the reference's operation order.  Each loop is bounded (a lattice vector
+// per pass: positions a few boxes apart need a few passes) so that non-finite
+// coordinates cannot keep a lane in it.
+constexpr int TRI_MAX_PASSES = 1024;
+__device__ __forceinline__ d3 nim_tri(const Box& B, d3 n) {
+    const double* a = B.a;
+    const double* b = B.b;
+    const double* c = B.c;
+    int k = 0;
+    while (n.z >= 0.5 * c[2] && k++ < TRI_MAX_PASSES) { n.x -= c[0]; n.y -= c[1]; n.z -= c[2]; }
+    k = 0;
+    while (n.z <= -0.5 * c[2] && k++ < TRI_MAX_PASSES) { n.x += c[0]; n.y += c[1]; n.z += c[2]; }
+    double ny = n.y - (c[1] / c[2]) * n.z;
+    k = 0;
+    while (ny >= 0.5 * b[1] && k++ < TRI_MAX_PASSES) {
+        n.x -= b[0]; n.y -= b[1]; n.z -= b[2];
+        ny = n.y - (c[1] / c[2]) * n.z;
+    }
+    k = 0;
+    while (ny <= -0.5 * b[1] && k++ < TRI_MAX_PASSES) {
+        n.x += b[0]; n.y += b[1]; n.z += b[2];
+        ny = n.y - (c[1] / c[2]) * n.z;
+    }
+    double nx = n.x - b[0] * (n.y - c[1] * n.z / c[2]) / b[1] - c[0] * n.z / c[2];
+    k = 0;
+    while (nx >= 0.5 * a[0] && k++ < TRI_MAX_PASSES) {
+        n.x -= a[0]; n.y -= a[1]; n.z -= a[2];
+        nx = n.x - b[0] * (n.y - c[1] * n.z / c[2]) / b[1] - c[0] * n.z / c[2];
+    }
+    k = 0;
+    while (nx <= -0.5 * a[0] && k++ < TRI_MAX_PASSES) {
+        n.x += a[0]; n.y += a[1]; n.z += a[2];
+        nx = n.x - b[0] * (n.y - c[1] * n.z / c[2]) / b[1] - c[0] * n.z / c[2];
+    }
+    return n;
 }
 
 __device__ __forceinline__ d3 nim(const Box& b, d3 ri, d3 rj) {
     d3 r{ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
-    if (b.kind) {
+    if (b.kind == 1) {
         r.x = r.x - b.L[0] * round(r.x * b.invL[0]);
         r.y = r.y - b.L[1] * round(r.y * b.invL[1]);
         r.z = r.z - b.L[2] * round(r.z * b.invL[2]);
+    } else if (b.kind == 2) {
+        r = nim_tri(b, r);
     }
     return r;
 }

@@ -15,9 +15,21 @@ using namespace gamd;
 
 namespace gamd {
 
+// box: 3 edge lengths (rectangular), or the vectors a, b, c (9 values, triclinic)
 Box make_box(int32_t boundary, const double* box) {
     Box b{};
-    b.kind = boundary == 1 ? 1 : 0;
+    b.kind = boundary == 1 || boundary == 2 ? boundary : 0;
+    if (b.kind == 2) {
+        for (int a = 0; a < 3; ++a) {
+            b.a[a] = box[a];
+            b.b[a] = box[3 + a];
+            b.c[a] = box[6 + a];
+        }
+        b.L[0] = b.a[0];
+        b.L[1] = b.b[1];
+        b.L[2] = b.c[2];
+        return b;
+    }
     for (int a = 0; a < 3; ++a) {
         b.L[a] = b.kind ? box[a] : 0.0;
         b.invL[a] = b.kind ? 1.0 / box[a] : 0.0;   // DVec3::recip
@@ -26,14 +38,29 @@ Box make_box(int32_t boundary, const double* box) {
 }
 
 int check_box(int32_t boundary, const double* box) {
-    if (boundary != 0 && boundary != 1) return fail(GROMOS_AMD_EINVAL, "boundary must be 0 (vacuum) or 1 (rectangular)");
+    if (boundary < 0 || boundary > 2)
+        return fail(GROMOS_AMD_EINVAL, "boundary must be 0 (vacuum), 1 (rectangular) or 2 (triclinic)");
     if (boundary == 1) {
         if (!box) return fail(GROMOS_AMD_EINVAL, "rectangular boundary needs a box");
         for (int a = 0; a < 3; ++a)
             if (!(box[a] > 0.0)) return fail(GROMOS_AMD_EINVAL, "box edge %d must be > 0", a);
     }
+    if (boundary == 2) {
+        // the GROMOS lower-triangular convention a = (ax, 0, 0), b = (bx, by, 0),
+        // c = (cx, cy, cz) that Triclinic::nearest_image requires (math.rs:97-100)
+        if (!box) return fail(GROMOS_AMD_EINVAL, "triclinic boundary needs the box vectors a, b, c");
+        if (box[1] != 0.0 || box[2] != 0.0 || box[5] != 0.0)
+            return fail(GROMOS_AMD_EINVAL, "triclinic box vectors must be lower triangular (a = (ax,0,0), b = (bx,by,0))");
+        if (!(box[0] > 0.0) || !(box[4] > 0.0) || !(box[8] > 0.0))
+            return fail(GROMOS_AMD_EINVAL, "triclinic box diagonal ax, by, cz must be > 0");
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(box[k])) return fail(GROMOS_AMD_EINVAL, "triclinic box vectors must be finite");
+    }
     return 0;
 }
+
+// the box the C ABI's params carry for each boundary kind
+static const double* params_box(const gromos_amd_params& P) { return P.boundary == 2 ? P.box_matrix : P.box; }
 
 // cells per axis: the largest count whose edge still exceeds half the list
 // cutoff (with a relative margin so that rounding in the wrapped coordinates can
@@ -54,7 +81,7 @@ constexpr int64_t BYTES_PER_ATOM = 2 * 2 * 24 + 7 * 24 + 32 + 16 + 8 + 4 + 8 + 8
 constexpr int64_t BYTES_PER_CG = 2 * 24 + 16 + 16 + 16 + 4 * 8 + 2 * 8 + 2 * 4 + 8;
 
 int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, JobImage& im) {
-    if (int rc = check_box(P.boundary, P.box)) return rc;
+    if (int rc = check_box(P.boundary, params_box(P))) return rc;
     int32_t n = T.n_atoms;
     if (n <= 0 || T.n_cg <= 0 || !T.cg_start || !T.charge || !T.iac || !T.mass || !T.lj || T.n_types <= 0 ||
         !T.excl_off || !T.o14_off)
@@ -82,7 +109,7 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     im.n_types = T.n_types;
     im.solvent = nsolv_atoms > 0;
     JobDev& J = im.J;
-    J.box = make_box(P.boundary, P.box);
+    J.box = make_box(P.boundary, params_box(P));
     gromos_amd_crf c;
     gromos_amd_crf_new(P.rcutl, 1.0, P.epsrf, P.kappa, &c);   // md.rs:326-329, 815
     J.crf = to_crf(&c);
@@ -106,13 +133,16 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.t_ref = P.t_ref;
     J.tau_t = P.tau_t;
     J.dof_t = P.dof_t;
-    // use_cg_grouped_solute (forcefield.rs:271-283)
+    // use_cg_grouped_solute (forcefield.rs:271-283; triclinic: off)
     if (J.box.kind == 0) J.use_cg_grouped = 1;
+    else if (J.box.kind == 2) J.use_cg_grouped = 0;
     else {
         double hm = std::min(std::min(0.5 * P.box[0], 0.5 * P.box[1]), 0.5 * P.box[2]);
         J.use_cg_grouped = hm > P.rcutl + P.skin + 0.3;
     }
-    for (int a = 0; a < 3; ++a) J.ncell[a] = J.box.kind ? cells_for(P.box[a], l) : 1;
+    // vacuum and triclinic jobs: one cell, every group a candidate of every other
+    // (the reference's O(N^2) standard search for both, pairlist.rs:521-532)
+    for (int a = 0; a < 3; ++a) J.ncell[a] = J.box.kind == 1 ? cells_for(P.box[a], l) : 1;
     // solvent template tables from the first solvent molecule (innerloops.rs:241-255)
     if (nsolv_atoms > 0) {
         int32_t f = T.n_solute_atoms;
@@ -212,7 +242,7 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     double ncg = T.n_cg;
     double partners = ncg;
     if (J.box.kind) {
-        double vol = P.box[0] * P.box[1] * P.box[2];
+        double vol = J.box.L[0] * J.box.L[1] * J.box.L[2];   // a lower-triangular box: the diagonal's product
         double rr = l + 0.3;
         partners = std::min(ncg, 4.18879 * rr * rr * rr * ncg / vol);
     }

@@ -67,8 +67,8 @@ __global__ void k_cg_ref(SysPtrs p, int32_t ncg, int32_t nctl) {
     p.ref[3 * c] = rx;
     p.ref[3 * c + 1] = ry;
     p.ref[3 * c + 2] = rz;
-    int32_t cc[3] = {0, 0, 0};
-    if (J.box.kind) {
+    int32_t cc[3] = {0, 0, 0};   // vacuum and triclinic jobs: one cell
+    if (J.box.kind == 1) {
         double r[3] = {rx, ry, rz};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -196,7 +196,8 @@ __device__ __forceinline__ double bcast(double v, int32_t t) {
 // skips unless one of its lanes has one
 __device__ __forceinline__ d3 nim_search(const Box& b, d3 ri, d3 rj) {
     d3 r{ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
-    if (b.kind) {
+    if (b.kind == 2) return nim_tri(b, r);
+    if (b.kind == 1) {
         const double ux = r.x * b.invL[0], uy = r.y * b.invL[1], uz = r.z * b.invL[2];
         double nx = rint(ux), ny = rint(uy), nz = rint(uz);
         const bool tie = fabs(ux - nx) == 0.5 || fabs(uy - ny) == 0.5 || fabs(uz - nz) == 0.5;
@@ -558,7 +559,12 @@ __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6,
 // the reference's half-away rounding at exactly L/2
 __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, float& y, float& z) {
     double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
-    if (b.kind) {
+    if (b.kind == 2) {
+        const d3 r = nim_tri(b, d3{dx, dy, dz});
+        dx = r.x;
+        dy = r.y;
+        dz = r.z;
+    } else if (b.kind == 1) {
         dx = dx - b.L[0] * rint(dx * b.invL[0]);
         dy = dy - b.L[1] * rint(dy * b.invL[1]);
         dz = dz - b.L[2] * rint(dz * b.invL[2]);

@@ -109,6 +109,39 @@ def spc_water_box(n_waters, box=None, density=0.997, jitter=0.02, seed=0, temper
                   name=name).validate()
 
 
+def truncoct_box(L):
+    """The truncated-octahedron box of cube edge L as lower-triangular vectors
+    a, b, c (columns; crates/gromos-core/src/math.rs truncoct_triclinic_box)."""
+    d = 0.5 * math.sqrt(3.0) * L
+    a = [d, 0.0, 0.0]
+    b = [d / 3.0, 2.0 * math.sqrt(2.0) / 3.0 * d, 0.0]
+    c = [-d / 3.0, math.sqrt(2.0) / 3.0 * d, math.sqrt(6.0) / 3.0 * d]
+    return np.array([a, b, c]).T
+
+
+def water_truncoct(n_side, density=0.997, jitter=0.02, seed=0, temperature=300.0, rc=1.2, nsnb=5, name=None):
+    """n_side^3 SPC waters in a truncated-octahedron box (triclinic boundary)
+    sized for the density: a lattice n_side^3 in fractional coordinates of the
+    box vectors, random rigid orientations, Maxwell-Boltzmann velocities."""
+    rng = np.random.default_rng(seed)
+    n_waters = n_side ** 3
+    vol = n_waters * 18.01528 / (density * 602.214076)          # nm^3
+    L = (2.0 * vol) ** (1.0 / 3.0)                              # the octahedron is half the cube
+    M = truncoct_box(L)
+    f = (np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) / n_side
+    o = f @ M.T + rng.uniform(-jitter, jitter, size=(n_waters, 3))
+    s = spc_water_box(n_waters, box=L, jitter=0.0, seed=seed, temperature=0.0, rcutp=rc, rcutl=rc, nsnb=nsnb,
+                      name=name or f"truncoct_{n_waters}")
+    R = random_rotations(rng, n_waters)
+    pos = (o[:, None, :] + np.einsum("nij,aj->nai", R, water_template())).reshape(-1, 3)
+    s.pos = pos
+    s.vel = rigid_mb_velocities(rng, pos, s.mass, temperature) if temperature > 0 else np.zeros_like(pos)
+    s.boundary = 2
+    s.box_matrix = M
+    s.box = np.diag(M).copy()
+    return s.validate()
+
+
 def water_216(seed=1):
     """216 SPC on a 6^3 lattice, L = 1.8621 nm, twin range 0.8 / 1.4 (cutoff > L/2)."""
     return spc_water_box(216, box=1.8621, seed=seed, rcutp=0.8, rcutl=1.4, nsnb=5, name="water_216")

@@ -43,6 +43,22 @@ class Rectangular:
         self.box = tuple(float(b) for b in box)
 
 
+class Triclinic:
+    """Triclinic (math.rs:75-141): box_matrix columns a, b, c, lower triangular."""
+    kind = 2
+
+    def __init__(self, box_matrix):
+        m = np.asarray(box_matrix, float)
+        self.box = tuple(float(v) for v in m.T.reshape(-1))
+
+
+def boundary_of(system):
+    """the boundary object of a system.System"""
+    if system.boundary == 2:
+        return Triclinic(system.box_matrix)
+    return Rectangular(system.box) if system.boundary else Vacuum()
+
+
 class LJParamMatrix:
     """Flat [ti * n + tj] x (c6, c12, cs6, cs12) table (params.rs:26-58) on the device."""
 
@@ -100,7 +116,7 @@ def _f64(t):
 
 
 def _box(bc):
-    return (rt.D * 3)(*bc.box)
+    return (rt.D * len(bc.box))(*bc.box)
 
 
 def _innerloop(positions, charges, iac, pairlist, lj_params, crf, periodicity, four_pi_eps_i, storage,

@@ -69,7 +69,7 @@ class Params(C.Structure):
     _fields_ = [("boundary", I32), ("box", D * 3), ("rcutp", D), ("rcutl", D), ("skin", D),
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
                 ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
-                ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D)]
+                ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9)]
 
 
 def built():
@@ -238,6 +238,9 @@ def params_struct(s, use_settle=True, do_virial=False):
     p.boundary = s.boundary
     for a in range(3):
         p.box[a] = float(s.box[a])
+    if s.boundary == 2:
+        for k, v in enumerate(s.box_values()):
+            p.box_matrix[k] = float(v)
     p.rcutp, p.rcutl, p.skin = s.rcutp, s.rcutl, s.skin
     p.nsnb = s.nsnb
     p.epsrf, p.kappa, p.four_pi_eps_i, p.dt = s.epsrf, s.kappa, s.fpepsi, s.dt

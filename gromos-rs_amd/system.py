@@ -73,8 +73,8 @@ class System:
     atoms_per_solvent: int
     pos: np.ndarray             # f64 [n, 3]
     vel: np.ndarray             # f64 [n, 3]
-    box: np.ndarray             # f64 [3]; zeros for vacuum
-    boundary: int = 1           # 0 vacuum, 1 rectangular
+    box: np.ndarray             # f64 [3]; zeros for vacuum; the diagonal for triclinic
+    boundary: int = 1           # 0 vacuum, 1 rectangular, 2 triclinic
     # run parameters (IMD PAIRLIST / NONBONDED / STEP blocks, crates/gromos-io/src/imd.rs:53-176)
     rcutp: float = 1.4
     rcutl: float = 1.4
@@ -90,6 +90,16 @@ class System:
     pert: Optional[Perturbation] = None
     thermostat: Optional[tuple] = None   # Berendsen (T_ref K, tau ps, degrees of freedom)
     name: str = "system"
+    # triclinic box (boundary 2): f64 [3, 3], columns a, b, c in the lower-triangular
+    # GROMOS convention a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz)
+    # (crates/gromos-core/src/math.rs:75-92)
+    box_matrix: Optional[np.ndarray] = None
+
+    def box_values(self):
+        """the box as the C ABI takes it: 3 lengths, or a, b, c (9 values) for triclinic"""
+        if self.boundary == 2:
+            return np.ascontiguousarray(np.asarray(self.box_matrix, float).T.reshape(-1))
+        return np.ascontiguousarray(self.box, dtype=float)
 
     @property
     def n_cg(self):
@@ -116,6 +126,11 @@ class System:
         assert len(self.excl_off) == n + 1 and len(self.o14_off) == n + 1
         if self.boundary == 1:
             assert np.all(self.box > 0)
+        if self.boundary == 2:
+            m = np.asarray(self.box_matrix, float)
+            assert m.shape == (3, 3) and m[1, 0] == 0 and m[2, 0] == 0 and m[2, 1] == 0
+            assert m[0, 0] > 0 and m[1, 1] > 0 and m[2, 2] > 0
+            assert np.allclose(self.box, np.diag(m))
         if self.pert is not None:
             p = self.pert
             assert p.a_iac.max(initial=0) < self.n_types and p.b_iac.max(initial=0) < self.n_types

@@ -24,7 +24,9 @@
  *     (innerloops.rs:199-207);
  *   - the LJ table is flat [ti * n_types + tj] x (c6, c12, cs6, cs12)
  *     (LJParamMatrix / LJParameters, params.rs:6-58); type indices are 0-based;
- *   - boundary: 0 = vacuum, 1 = rectangular (math.rs:20-74).
+ *   - boundary: 0 = vacuum, 1 = rectangular (math.rs:20-74), 2 = triclinic
+ *     (math.rs:75-141); a `box` argument holds 3 edge lengths, or for
+ *     boundary 2 the box vectors a, b, c (9 doubles, lower triangular).
  * A system handle must be used from one host thread at a time.
  */
 #ifndef GROMOS_AMD_H
@@ -275,6 +277,10 @@ typedef struct {
      * degrees of freedom (md.rs:1164-1172: 3N - constraints - NDFMIN) */
     int32_t thermostat;
     double t_ref, tau_t, dof_t;
+    /* boundary 2 (triclinic, Triclinic in math.rs:75-141): the box vectors
+     * a, b, c (9 doubles, lower triangular: a = (ax,0,0), b = (bx,by,0));
+     * `box` is not read then */
+    double box_matrix[9];
 } gromos_amd_params;
 
 enum gromos_amd_precision {

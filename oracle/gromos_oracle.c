@@ -16,12 +16,42 @@
 /* ------------------------------------------------------------------ */
 /* boundary conditions: crates/gromos-core/src/math.rs:20-74           */
 /* ------------------------------------------------------------------ */
-void or_pbc_init(or_pbc *p, int kind, const double L[3]) {
+void or_pbc_init(or_pbc *p, int kind, const double *box) {
     p->kind = kind;
     for (int a = 0; a < 3; ++a) {
-        p->L[a] = L ? L[a] : 0.0;
+        p->a[a] = p->b[a] = p->c[a] = 0.0;
+        if (kind == 2) {
+            p->a[a] = box[a];
+            p->b[a] = box[3 + a];
+            p->c[a] = box[6 + a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (kind == 2) p->L[a] = a == 0 ? p->a[0] : (a == 1 ? p->b[1] : p->c[2]);
+        else p->L[a] = box ? box[a] : 0.0;
         p->invL[a] = (kind == 1) ? 1.0 / p->L[a] : 0.0;   /* DVec3::recip */
         p->half[a] = p->L[a] * 0.5;
+    }
+}
+
+/* Triclinic::nearest_image (math.rs:105-141): z, then y, then x reduction
+ * loops over the lower-triangular box vectors, in the reference's operation
+ * order. */
+static void nim_triclinic(const or_pbc *p, double *n) {
+    const double *a = p->a, *b = p->b, *c = p->c;
+    while (n[2] >= 0.5 * c[2]) { n[0] -= c[0]; n[1] -= c[1]; n[2] -= c[2]; }
+    while (n[2] <= -0.5 * c[2]) { n[0] += c[0]; n[1] += c[1]; n[2] += c[2]; }
+    double ny = n[1] - (c[1] / c[2]) * n[2];
+    while (ny >= 0.5 * b[1]) { n[0] -= b[0]; n[1] -= b[1]; n[2] -= b[2]; ny = n[1] - (c[1] / c[2]) * n[2]; }
+    while (ny <= -0.5 * b[1]) { n[0] += b[0]; n[1] += b[1]; n[2] += b[2]; ny = n[1] - (c[1] / c[2]) * n[2]; }
+    double nx = n[0] - b[0] * (n[1] - c[1] * n[2] / c[2]) / b[1] - c[0] * n[2] / c[2];
+    while (nx >= 0.5 * a[0]) {
+        n[0] -= a[0]; n[1] -= a[1]; n[2] -= a[2];
+        nx = n[0] - b[0] * (n[1] - c[1] * n[2] / c[2]) / b[1] - c[0] * n[2] / c[2];
+    }
+    while (nx <= -0.5 * a[0]) {
+        n[0] += a[0]; n[1] += a[1]; n[2] += a[2];
+        nx = n[0] - b[0] * (n[1] - c[1] * n[2] / c[2]) / b[1] - c[0] * n[2] / c[2];
     }
 }
 
@@ -33,6 +63,7 @@ void or_nearest_image(const or_pbc *p, const double *ri, const double *rj, doubl
         if (p->kind == 1) d = d - p->L[a] * round(d * p->invL[a]);
         r[a] = d;
     }
+    if (p->kind == 2) nim_triclinic(p, r);
 }
 
 static inline double len2(const double r[3]) { return r[0] * r[0] + r[1] * r[1] + r[2] * r[2]; }
@@ -926,6 +957,7 @@ or_ff *or_ff_create(const or_topo *t, const or_pbc *pbc, const or_ff_params *prm
     /* use_cg_grouped_solute (forcefield.rs:271-283) */
     int cg_avail = t->n_cg > 0;
     if (pbc->kind == 0) ff->use_cg_grouped = cg_avail;
+    else if (pbc->kind == 2) ff->use_cg_grouped = 0;   /* "be conservative for triclinic" */
     else {
         double hm = pbc->half[0];
         if (pbc->half[1] < hm) hm = pbc->half[1];

@@ -32,13 +32,15 @@ extern "C" {
 
 /* ---- boundary conditions (crates/gromos-core/src/math.rs:20-74) ---- */
 typedef struct {
-    int kind;          /* 0 = vacuum, 1 = rectangular */
-    double L[3];       /* box edge lengths */
+    int kind;          /* 0 = vacuum, 1 = rectangular, 2 = triclinic */
+    double L[3];       /* box edge lengths (triclinic: the diagonal ax, by, cz) */
     double invL[3];    /* 1.0 / L, computed once (math.rs:49-55) */
     double half[3];
+    double a[3], b[3], c[3];   /* triclinic box vectors, lower triangular (math.rs:75-92) */
 } or_pbc;
 
-void or_pbc_init(or_pbc *p, int kind, const double L[3]);
+/* box: 3 edge lengths, or for kind 2 the vectors a, b, c (9 values) */
+void or_pbc_init(or_pbc *p, int kind, const double *box);
 void or_nearest_image(const or_pbc *p, const double *ri, const double *rj, double *r);
 
 /* ---- CRF constants (crates/gromos-forces/src/nonbonded/params.rs:60-97) ---- */

@@ -38,7 +38,8 @@ I64 = C.c_int64
 
 
 class Pbc(C.Structure):
-    _fields_ = [("kind", C.c_int), ("L", D * 3), ("invL", D * 3), ("half", D * 3)]
+    _fields_ = [("kind", C.c_int), ("L", D * 3), ("invL", D * 3), ("half", D * 3), ("a", D * 3), ("b", D * 3),
+                ("c", D * 3)]
 
 
 class Crf(C.Structure):
@@ -179,7 +180,7 @@ class OracleSystem:
                          ptr(k["cg_start"]), ptr(k["charge"]), ptr(k["iac"]), ptr(k["mass"]),
                          s.n_types, ptr(k["lj"]), ptr(k["excl_off"]), ptr(k["excl"]),
                          ptr(k["o14_off"]), ptr(k["o14"]), s.atoms_per_solvent)
-        self.pbc = make_pbc(s.boundary, s.box)
+        self.pbc = make_pbc(s.boundary, s.box_values())
         self.pert = None
         if s.pert is not None:
             p = s.pert

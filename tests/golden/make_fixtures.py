@@ -47,6 +47,7 @@ SYSTEMS = [
     ("nacl_1water_settle", None, None, None, None, "all"),
     ("aladip_solvated", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
     ("water_216_box", None, None, None, None, "first"),
+    ("aladip_trunc_oct", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
     ("water_1000_spc_gridcell", None, None, None, None, "first"),
     ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
      "../shared/ch4_spc_dummy.ptp", "first"),
@@ -308,6 +309,11 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         for a in pert["atoms"]:
             a["a_iac"], a["b_iac"] = remap[a["a_iac"]], remap[a["b_iac"]]
     ntb = imd["ntb"]
+    box_matrix = None
+    if ntb == -1:
+        (a, b3, c), pos, vel = truncoct_triclinic(box[0], pos, vel)
+        box_matrix = a + b3 + c            # columns a, b, c
+        box = [a[0], b3[1], c[2]]          # the diagonal
     fx = dict(
         name=name, source=os.path.join(FIXDIR, name),
         n_atoms=n, n_solute_atoms=ns, n_cg=len(cg_start) - 1, n_solute_cg=n_solute_cg,
@@ -315,7 +321,8 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         charge=charge, iac=[remap[x] for x in iac], mass=mass,
         n_types=nt, lj=lj, excl=excl, o14=o14,
         pos=[x for p in pos for x in p], vel=[x for v in vel for x in v] if vel else None,
-        box=box if ntb != 0 else [0.0, 0.0, 0.0], ntb=1 if ntb != 0 else 0,
+        box=box if ntb != 0 else [0.0, 0.0, 0.0], ntb=2 if ntb == -1 else (1 if ntb != 0 else 0),
+        box_matrix=box_matrix,
         fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
         algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
         dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"],
@@ -332,10 +339,70 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
     return fx
 
 
+def truncoct_triclinic(L, pos, vel):
+    """NTB = -1 (crates/gromos-md/src/bin/md.rs:530-545): the cube edge L of the
+    BOX block becomes the lower-triangular truncated-octahedron box
+    (crates/gromos-core/src/math.rs truncoct_triclinic_box, forward), and every
+    position and velocity is rotated into that frame by the transpose of the
+    rotation whose columns are (1, 1, 1)/sqrt3, (-2, 1, 1)/sqrt6, (0, -1, 1)/sqrt2
+    (truncoct_triclinic_rotmat(true)); glam's Mat3 * Vec3 sums the columns
+    times x, y, z left to right.  Returns the box vectors a, b, c and the
+    rotated vectors."""
+    import math
+    d = 0.5 * math.sqrt(3.0) * L
+    a = [d, 0.0, 0.0]
+    b = [d / 3.0, 2.0 * math.sqrt(2.0) / 3.0 * d, 0.0]
+    c = [-d / 3.0, math.sqrt(2.0) / 3.0 * d, math.sqrt(6.0) / 3.0 * d]
+    sq3i = 1.0 / math.sqrt(3.0)
+    sq2i = 1.0 / math.sqrt(2.0)
+    # columns of the applied matrix (the rows of the rotation)
+    mx = [sq3i, -2.0 * sq2i * sq3i, 0.0]
+    my = [sq3i, sq3i * sq2i, -sq2i]
+    mz = [sq3i, sq2i * sq3i, sq2i]
+
+    def rot(v):
+        return [mx[k] * v[0] + my[k] * v[1] + mz[k] * v[2] for k in range(3)]
+
+    return (a, b, c), [rot(v) for v in pos], ([rot(v) for v in vel] if vel else vel)
+
+
+# input files copied as data for the product's GROMOS readers (name: files
+# relative to the fixture directory)
+INPUT_SETS = {
+    "water_10_box": ["water_10_box.topo", "water_10_box.conf", "water_10_box.in"],
+    "nacl_water_box": ["nacl_water_box.topo", "nacl_water_box.conf", "nacl_water_box.in"],
+    "nacl_1water_settle": ["nacl_1water_settle.topo", "nacl_1water_settle.conf", "nacl_1water_settle.in"],
+    "nacl_3water_cutoff": ["nacl_3water_cutoff.topo", "nacl_3water_cutoff.conf", "nacl_3water_cutoff.in"],
+    "water_3_box_twinrange": ["water_3_box_twinrange.topo", "water_3_box_twinrange.conf",
+                              "water_3_box_twinrange.in"],
+    "aladip_solvated": ["../shared/aladip.topo", "../shared/aladip.conf", "aladip_solvated.in"],
+    "aladip_trunc_oct": ["../shared/aladip.topo", "../shared/aladip.conf", "aladip_trunc_oct.in"],
+    "ch4_water_fep": ["../shared/ch4_spc.top", "../shared/ch4_spc.cnf", "ch4_water_fep.in",
+                      "../shared/ch4_spc_dummy.ptp"],
+}
+
+
+def copy_inputs(ref, here):
+    """tests/golden/gromos/<name>/ <- the fixture's input files (data)"""
+    import shutil
+    for name, files in INPUT_SETS.items():
+        dst = os.path.join(here, "gromos", name)
+        os.makedirs(dst, exist_ok=True)
+        for f in files:
+            shutil.copyfile(os.path.join(ref, FIXDIR, name, f), os.path.join(dst, os.path.basename(f)))
+        print(f"inputs of {name}: {', '.join(os.path.basename(f) for f in files)}")
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    if len(sys.argv) > 2 and sys.argv[2] == "--inputs":
+        copy_inputs(ref, os.path.dirname(os.path.abspath(__file__)))
+        return
+    only = set(sys.argv[2:])   # names to (re)build; all when empty
     here = os.path.dirname(os.path.abspath(__file__))
     for spec in SYSTEMS:
+        if only and spec[0] not in only:
+            continue
         fx = build(ref, *spec)
         out = os.path.join(here, spec[0] + ".json")
         with open(out, "w") as fh:

@@ -54,5 +54,6 @@ def load(name):
         vel=vel, box=np.array(d["box"], float), boundary=d["ntb"], rcutp=d["rcutp"],
         rcutl=d["rcutl"], nsnb=d["nsnb"], algorithm=d["algorithm"], epsrf=d["epsrf"],
         kappa=d["appak"], fpepsi=d["fpepsi"], dt=d["dt"], dist_oh=d_oh, dist_hh=d_hh, pert=pert,
-        name=name)
+        name=name,
+        box_matrix=(np.array(d["box_matrix"], float).reshape(3, 3).T if d.get("box_matrix") else None))
     return s.validate(), d

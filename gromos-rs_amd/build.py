@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgromos_amd.so")
-SOURCES = ["explicit.hip", "system_kernels.hip", "system_host.hip", "capi.hip", "session.hip"]
+SOURCES = ["explicit.hip", "bonded.hip", "system_kernels.hip", "system_host.hip", "capi.hip", "session.hip"]
 ARCH = os.environ.get("GROMOS_AMD_ARCH", "gfx950")
 
 

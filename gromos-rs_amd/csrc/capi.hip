@@ -523,6 +523,7 @@ extern "C" int gromos_amd_system_forcefield_apply(gromos_amd_system* S, double* 
     SysPtrs p = S->ptrs();
     launch_reduce(p, S->npad, S->twin ? 1 : 0, 0, 0, S->njobs, S->stream);
     HIPCHK(hipGetLastError());
+    S->last_rec_row = 0;
     std::vector<double> rec((size_t)S->njobs * REC);
     HIPCHK(hipMemcpyAsync(rec.data(), S->d_records, rec.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
     size_t hoff = 0;
@@ -544,20 +545,38 @@ extern "C" int gromos_amd_system_forcefield_apply(gromos_amd_system* S, double* 
     return S->collect_timing();
 }
 
-extern "C" int gromos_amd_system_md(gromos_amd_system* S, int32_t nsteps, double* energies) {
+extern "C" int gromos_amd_system_covalent_energies(gromos_amd_system* S, double* out) {
+    if (!S || !out) return fail(GROMOS_AMD_EINVAL, "covalent_energies: bad arguments");
+    if (S->last_rec_row < 0) return fail(GROMOS_AMD_EINVAL, "covalent_energies: no force evaluation yet");
+    std::vector<double> rec((size_t)S->njobs * REC);
+    HIPCHK(hipMemcpyAsync(rec.data(), S->d_records + (size_t)S->last_rec_row * S->njobs * REC,
+                          rec.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    for (int j = 0; j < S->njobs; ++j)
+        for (int k = 0; k < 4; ++k) out[4 * j + k] = rec[(size_t)j * REC + 14 + k];
+    return 0;
+}
+
+extern "C" int gromos_amd_system_md_covalent(gromos_amd_system* S, int32_t nsteps, double* energies,
+                                             double* covalent) {
     if (!S || nsteps < 0) return fail(GROMOS_AMD_EINVAL, "md: bad arguments");
     if (nsteps == 0) return 0;
     std::vector<double> rec;
-    if (int rc = md_steps(S, nsteps, energies ? &rec : nullptr)) return rc;
-    if (energies) {
-        for (int32_t s = 0; s < nsteps; ++s)
-            for (int j = 0; j < S->njobs; ++j) {
-                const double* r = rec.data() + ((size_t)s * S->njobs + j) * REC;
-                double* e = energies + ((size_t)s * S->njobs + j) * 5;
-                for (int c = 0; c < 5; ++c) e[c] = r[c];
-            }
-    }
+    if (int rc = md_steps(S, nsteps, (energies || covalent) ? &rec : nullptr)) return rc;
+    for (int32_t s = 0; s < nsteps && (energies || covalent); ++s)
+        for (int j = 0; j < S->njobs; ++j) {
+            const size_t row = (size_t)s * S->njobs + j;
+            const double* r = rec.data() + row * REC;
+            if (energies)
+                for (int c = 0; c < 5; ++c) energies[row * 5 + c] = r[c];
+            if (covalent)
+                for (int c = 0; c < 4; ++c) covalent[row * 4 + c] = r[14 + c];
+        }
     return 0;
+}
+
+extern "C" int gromos_amd_system_md(gromos_amd_system* S, int32_t nsteps, double* energies) {
+    return gromos_amd_system_md_covalent(S, nsteps, energies, nullptr);
 }
 
 extern "C" int gromos_amd_system_pairlist_get(gromos_amd_system* S, int64_t n_out[4], uint32_t* ss, uint32_t* sl,

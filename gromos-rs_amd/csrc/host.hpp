@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/gromos_amd.h"
+#include "bonded.hpp"
 #include "system_kernels.hpp"
 
 namespace gamd {
@@ -30,6 +31,13 @@ Box make_box(int32_t boundary, const double* box);
 int check_box(int32_t boundary, const double* box);
 inline Crf to_crf(const gromos_amd_crf* c) { return Crf{c->crf_cut, c->crf_2cut3i, c->crf_cut3i, c->cutoff_sq}; }
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+struct BondedTerms {
+    std::vector<int4> atoms;
+    std::vector<double4> prm;
+    std::vector<int32_t> kind;
+    int32_t n[4] = {0, 0, 0, 0};
+};
 
 // One job's topology and run parameters, validated and flattened once (local
 // atom / group indices); a batch layout concatenates images with offsets.
@@ -49,6 +57,7 @@ struct JobImage {
     std::vector<uint8_t> pflag;
     std::vector<int32_t> pa_iac, pb_iac, papairs;
     std::vector<double> pa_q, pb_q, palj, pacrf;
+    BondedTerms bonded;             // covalent terms of the solute
     int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
     int64_t list_entries = 0;       // full neighbour-list entries of the job (estimate)
 };
@@ -58,6 +67,12 @@ struct JobImage {
 constexpr int64_t MAX_LIST_ENTRIES = INT32_MAX;
 
 int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int job, JobImage& out);
+
+// validated covalent terms of one topology, kind-major (bonds, angles,
+// dihedrals, impropers), local atom indices; n: terms kept per kind
+int bonded_terms(const gromos_amd_bonded& b, int32_t n_atoms, int job, BondedTerms& out);
+// per-atom CSR of the terms' contributions (entries term << 2 | slot, term order)
+void bonded_csr(const BondedTerms& t, int32_t natoms, std::vector<int32_t>& aoff, std::vector<int32_t>& aref);
 // GROMOS_AMD_ENOMEM unless both list sizes fit the kernels' 32-bit offsets
 int check_list_size(int64_t vv_entries, int64_t gen_entries);
 
@@ -207,6 +222,14 @@ struct gromos_amd_system {
     int32_t *d_pap_off = nullptr, *d_pap = nullptr;
     double *d_pa_q = nullptr, *d_pb_q = nullptr, *d_palj = nullptr, *d_pacrf = nullptr;
     int32_t n_papairs = 0;
+    // covalent terms of the batch (bonded.hpp); gathered atoms = the solute atoms
+    int4* d_bd_atoms = nullptr;
+    double4* d_bd_prm = nullptr;
+    int32_t* d_bd_kind = nullptr;
+    double *d_bd_tf = nullptr, *d_bd_tev = nullptr, *d_bd_fb = nullptr;
+    int32_t *d_bd_aoff = nullptr, *d_bd_aref = nullptr;
+    int32_t nbd = 0;
+    int32_t last_rec_row = -1;   // records row of the last force evaluation
     char* d_temp = nullptr;
     size_t temp_bytes = 0;
     int cell_bits = 1;

@@ -55,6 +55,8 @@ struct JobDev {
     int32_t settle;                        // SETTLE the solvent molecules
     int32_t thermo;                        // Berendsen weak coupling (berendsen_thermostat.rs)
     double t_ref, tau_t, dof_t;
+    int32_t bd_off;                        // first covalent term (batch order)
+    int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };
 
 // energy partial rows: per kernel slot and block, 12 doubles
@@ -62,8 +64,9 @@ struct JobDev {
 constexpr int ROW = 12;
 enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 4, NSLOT = 5 };
 
-// per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9]
-constexpr int REC = 14;
+// per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9],
+// e_bond, e_angle, e_dihedral, e_improper
+constexpr int REC = 18;
 // words per job in the pair-list build's control block: short and long pair
 // interactions, solvent-solvent list entries
 constexpr int JOB_CTL = 3;

@@ -233,6 +233,18 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
             for (int c = 0; c < 4; ++c) im.papairs.push_back(e[c]);
         }
     }
+    if (T.bonded) {
+        if (int rc = bonded_terms(*T.bonded, n, j, im.bonded)) return rc;
+        // the gathered atoms of a batch are its solute atoms
+        for (const int4& a : im.bonded.atoms)
+            for (int32_t v : {a.x, a.y, a.z, a.w})
+                if (v >= T.n_solute_atoms)
+                    return fail(GROMOS_AMD_EINVAL, "job %d: covalent terms must join solute atoms", j);
+        if (im.bonded.atoms.size() >= (size_t)(INT32_MAX >> 3))
+            return fail(GROMOS_AMD_EINVAL, "job %d: too many covalent terms", j);
+    }
+    J.bd_off = 0;
+    for (int k = 0; k < 4; ++k) J.bd_n[k] = im.bonded.n[k];
     // footprint: per-atom and per-group arrays, the full neighbour lists
     // (partners within the list cutoff at this job's group density, both
     // directions, 4 bytes each, with the buffers' growth policy: 25 % slack,
@@ -250,7 +262,8 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     im.list_entries = (int64_t)(ncg * partners * 1.25);
     im.footprint = npad * BYTES_PER_ATOM + (int64_t)T.n_cg * BYTES_PER_CG + 8 * cells +
                    (int64_t)(ncg * partners * 4.0 * 2.5) + (int64_t)(im.excl.size() + im.o14.size() * 2) * 4 +
-                   (int64_t)(2.5 * 1000 * REC * sizeof(double)) + 4096;
+                   (int64_t)(2.5 * 1000 * REC * sizeof(double)) + 4096 +
+                   (int64_t)im.bonded.atoms.size() * (16 + 32 + 4 + 8 * (12 + BD_EV) + 4 * 4) * 5 / 4;
     return 0;
 }
 
@@ -285,7 +298,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     const auto t_concat = std::chrono::steady_clock::now();
     // pass 1 (serial): each job's offsets in every concatenated array
     struct Slice {
-        int64_t atom, cg, lj, excl, o14, o14s, molf, pap, sol;
+        int64_t atom, cg, lj, excl, o14, o14s, molf, pap, sol, bd;
     };
     std::vector<Slice> sl(njobs + 1);
     int64_t cell_off = 0;
@@ -304,7 +317,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         n.molf = o.molf + (im.P.use_settle ? (int64_t)im.mol_first.size() : 0);
         n.pap = o.pap + (int64_t)im.papairs.size();
         n.sol = o.sol + im.J.n_solute_atoms;
+        n.bd = o.bd + (int64_t)im.bonded.atoms.size();
         JobDev J = im.J;
+        J.bd_off = (int32_t)o.bd;
         J.atom_off = (int32_t)o.atom;
         J.cg_off = (int32_t)o.cg;
         J.lj_off = (int32_t)o.lj;
@@ -324,13 +339,15 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     const int64_t npad = T.atom, ncg = T.cg;
     if (npad >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many atoms");
     if (ncg >= (int64_t)ENT_IDX) return fail(GROMOS_AMD_EINVAL, "too many charge groups");
-    if (T.excl >= INT32_MAX || T.o14 >= INT32_MAX || T.o14s >= INT32_MAX || cell_off >= INT32_MAX)
+    if (T.excl >= INT32_MAX || T.o14 >= INT32_MAX || T.o14s >= INT32_MAX || cell_off >= INT32_MAX ||
+        T.bd >= (INT32_MAX >> 3))
         return fail(GROMOS_AMD_EINVAL, "batch too large");
     // the arrays live in the batch's pinned staging arena (all written below)
     const size_t np1 = (size_t)npad + 1, pn = any_pert ? (size_t)npad : 0;
     const size_t n_pap = (size_t)T.pap / 2;   // two index entries per PERTATOMPAIR entry at most
     const size_t bytes = 4 * (3 * (size_t)ncg + 2 * (size_t)npad + 3 * np1 + T.excl + T.o14 + T.o14s + T.molf +
                               T.pap + T.sol) +
+                         (size_t)T.bd * (16 + 32 + 4) +
                          8 * (3 * (size_t)npad + T.lj) + pn * (1 + 4 * 2 + 8 * 4) +
                          (any_pert ? (size_t)ncg + 4 * (np1 + n_pap) : 0);
     if (int rc = S->host.reset(bytes, 40)) return rc;
@@ -341,6 +358,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     auto o14s_off = S->host.take<int32_t>(np1), o14s = S->host.take<int32_t>(T.o14s);
     auto mol_first = S->host.take<int32_t>(T.molf), papairs = S->host.take<int32_t>(T.pap);
     auto sol_atom = S->host.take<int32_t>(T.sol);
+    auto bd_atoms = S->host.take<int4>(T.bd);
+    auto bd_prm = S->host.take<double4>(T.bd);
+    auto bd_kind = S->host.take<int32_t>(T.bd);
     auto q = S->host.take<double>(npad), mass = S->host.take<double>(npad), inv_mass = S->host.take<double>(npad);
     auto lj = S->host.take<double>(T.lj);
     auto pflag = S->host.take<uint8_t>(pn), cg_pert = S->host.take<uint8_t>(any_pert ? ncg : 0);
@@ -414,6 +434,13 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         }
         std::copy(im.lj.begin(), im.lj.end(), lj.data() + o.lj);
         for (int32_t a = 0; a < im.J.n_solute_atoms; ++a) sol_atom[o.sol + a] = A + a;
+        for (size_t k = 0; k < im.bonded.atoms.size(); ++k) {
+            const int4 e = im.bonded.atoms[k];
+            bd_atoms[o.bd + k] = make_int4(e.x < 0 ? -1 : A + e.x, e.y < 0 ? -1 : A + e.y, e.z < 0 ? -1 : A + e.z,
+                                           e.w < 0 ? -1 : A + e.w);
+            bd_prm[o.bd + k] = im.bonded.prm[k];
+            bd_kind[o.bd + k] = im.bonded.kind[k];
+        }
     };
     {
         // a thread per 32k atoms, at most one per job and at most 16 (or the cores)
@@ -520,6 +547,34 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         }
     }
 
+    // covalent terms: per solute atom (sol_atom order) its contributions in term order
+    S->nbd = (int32_t)T.bd;
+    PinnedArena::Span<int32_t> bd_aoff, bd_aref;
+    if (T.bd > 0) {
+        bd_aoff = S->host.take<int32_t>((size_t)T.sol + 1);
+        std::fill(bd_aoff.begin(), bd_aoff.end(), 0);
+        // global atom -> solute slot of its job
+        auto slot = [&](int32_t a) -> int32_t {
+            const int32_t j = atom_job[a];
+            return S->jobs[j].sol_off + (a - S->jobs[j].atom_off);
+        };
+        int64_t nent = 0;
+        for (int64_t k = 0; k < T.bd; ++k)
+            for (int32_t v : {bd_atoms[k].x, bd_atoms[k].y, bd_atoms[k].z, bd_atoms[k].w})
+                if (v >= 0) {
+                    bd_aoff[slot(v) + 1]++;
+                    nent++;
+                }
+        for (int64_t w = 0; w < T.sol; ++w) bd_aoff[w + 1] += bd_aoff[w];
+        bd_aref = S->host.take<int32_t>((size_t)nent);
+        if (S->host.used > S->host.cap) return fail(GROMOS_AMD_EHIP, "layout staging overflow (internal)");
+        std::vector<int32_t> at(bd_aoff.begin(), bd_aoff.end() - 1);
+        for (int64_t k = 0; k < T.bd; ++k) {
+            const int32_t* ap = &bd_atoms[k].x;
+            for (int s2 = 0; s2 < 4; ++s2)
+                if (ap[s2] >= 0) bd_aref[at[slot(ap[s2])]++] = (int32_t)((k << 2) | s2);
+        }
+    }
     S->concat_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_concat).count();
     hipStream_t st = S->stream;
     int rc = 0;
@@ -553,6 +608,16 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_pap, pap);
     rc = rc ? rc : S->upload(S->d_vv_chunk, vv_chunk);
     rc = rc ? rc : S->upload(S->d_prec, prec);
+    if (T.bd > 0) {
+        rc = rc ? rc : S->upload(S->d_bd_atoms, bd_atoms);
+        rc = rc ? rc : S->upload(S->d_bd_prm, bd_prm);
+        rc = rc ? rc : S->upload(S->d_bd_kind, bd_kind);
+        rc = rc ? rc : S->upload(S->d_bd_aoff, bd_aoff);
+        rc = rc ? rc : S->upload(S->d_bd_aref, bd_aref);
+        rc = rc ? rc : S->reserve(S->d_bd_tf, (size_t)T.bd * 12, true);
+        rc = rc ? rc : S->reserve(S->d_bd_tev, (size_t)T.bd * BD_EV, true);
+        rc = rc ? rc : S->reserve(S->d_bd_fb, (size_t)T.sol * 3, true);
+    }
     {
         // the atom blocks of BLOCK atoms that hold solute atoms: in md steps the
         // excluded / 1-4 / self terms of solvent atoms are taken elsewhere
@@ -927,6 +992,7 @@ int forcefield(gromos_amd_system* S, bool assemble) {
             HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
             launch_gen(p, npad, false, S->do_virial, mixed, 0, S->side, GEN_SOLUTE);
             launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, S->side);
+            launch_bonded(p.bd, p.x_cur, S->side);
             HIPCHK(hipEventRecord(S->ev_join, S->side));
         }
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, a, b);
@@ -955,7 +1021,12 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     }
     if (assemble) launch_excl14(p, npad, 1, nullptr, 0, st);
     else if (!split) launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
-    if (assemble) launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
+    // covalent terms (forcefield.rs:386-400): per-atom sums in bd.fb, added last
+    if (!split) launch_bonded(p.bd, p.x_cur, st);
+    if (assemble) {
+        launch_assemble(p, npad, S->twin ? 1 : 0, mixed, st);
+        launch_bonded_add(p.bd, p.f, st);
+    }
     if (split) HIPCHK(hipStreamWaitEvent(st, S->ev_join, 0));
     HIPCHK(hipGetLastError());
     return 0;
@@ -983,6 +1054,7 @@ static int launch_step_graph(gromos_amd_system* S) {
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st);
         launch_gen(p, S->npad, false, S->do_virial, mixed, (int32_t)S->host_counters[4], st);
         launch_excl14(p, S->npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
+        launch_bonded(p.bd, p.x_cur, st);
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, 1, mixed, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
         hipError_t e = hipStreamEndCapture(st, &graph);
@@ -1061,6 +1133,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
             if (int rc = step_event()) return rc;
     }
     S->md_step = 0;
+    S->last_rec_row = nsteps - 1;
     HIPCHK(hipGetLastError());
     std::vector<int32_t> failed(S->njobs);
     HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1229,5 +1302,19 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.pap_off = d_pap_off;
     p.pap = d_pap;
     p.o2s = d_o2s;
+    p.bd = gamd::BondedDev{};
+    if (nbd > 0) {
+        p.bd.atoms = d_bd_atoms;
+        p.bd.prm = d_bd_prm;
+        p.bd.kind = d_bd_kind;
+        p.bd.tf = d_bd_tf;
+        p.bd.tev = d_bd_tev;
+        p.bd.aoff = d_bd_aoff;
+        p.bd.aref = d_bd_aref;
+        p.bd.amap = d_sol_atom;
+        p.bd.fb = d_bd_fb;
+        p.bd.nterms = nbd;
+        p.bd.natoms = nsol;
+    }
     return p;
 }

@@ -1985,6 +1985,11 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
             return;
         }
         v3 f = total_force<MIXED>(p, i, with_long, solute_group);
+        if (solute_group && p.bd.fb) {
+            // covalent + nonbonded (forcefield.rs:966)
+            const double* fb = p.bd.fb + 3 * (int64_t)(J.sol_off + (i - J.atom_off));
+            f = mk(fb[0] + f.x, fb[1] + f.y, fb[2] + f.z);
+        }
         double im = p.inv_mass[i];
         if (J.thermo) vn = mk((vo.x + f.x * im * dt) * lam, (vo.y + f.y * im * dt) * lam, (vo.z + f.z * im * dt) * lam);
         else vn = mk(vo.x + f.x * im * dt, vo.y + f.y * im * dt, vo.z + f.z * im * dt);
@@ -2089,7 +2094,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_parts(SysPtrs p, int32_t with_viri
 // block to finish advances); otherwise a single point (row 0, counter untouched)
 __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32_t with_long,
                                                   int32_t with_kin, int32_t advance, int32_t njobs) {
-    constexpr int W = ROW + 2;   // + dH/dlambda, E_kin_new
+    constexpr int W = ROW + 2 + 4;   // + dH/dlambda, E_kin_new, the four covalent energies
     __shared__ double lds[17 * W];
     const int32_t step = advance ? *p.step : 0;
     int32_t jb = blockIdx.x;
@@ -2134,16 +2139,35 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
             v[12] += r[11];
         }
     }
+    // covalent terms: energy per kind, virial with the nonbonded one
+    if (p.bd.tev) {
+        int32_t t0 = J.bd_off;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t t1 = t0 + J.bd_n[k];
+            for (int32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                const double* ev = p.bd.tev + BD_EV * (int64_t)t;
+                v[14 + k] += ev[0];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) v[2 + c] += ev[1 + c];
+            }
+            t0 = t1;
+        }
+    }
     block_sum<W>(v, lds);
     if (threadIdx.x == 0) {
         double* rec = p.records + ((int64_t)step * njobs + jb) * REC;
         const double* t = lds + 16 * W;
+        // update_potential_total (configuration.rs:197-207): bond_total carries
+        // every covalent term (forcefield.rs:970-971), then LJ, then CRF
+        const double eb = ((t[14] + t[15]) + t[16]) + t[17];
         rec[0] = t[0];
         rec[1] = t[1];
-        rec[2] = t[0] + t[1];
+        rec[2] = (eb + t[0]) + t[1];
         rec[3] = t[11];
         rec[4] = t[12];
         for (int c = 0; c < 9; ++c) rec[5 + c] = t[2 + c];
+        for (int k = 0; k < 4; ++k) rec[14 + k] = t[14 + k];
         // the next step's Berendsen scale from this step's E_kin_new: what the
         // reference reads after TemperatureCalculation stored it
         if (with_kin && J.thermo) p.therm_scale[jb] = berendsen_scale(J, t[13]);

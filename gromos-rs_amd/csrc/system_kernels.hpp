@@ -1,6 +1,7 @@
 // Device pointer bundle and launchers of the device-resident kernels.
 #pragma once
 #include <hip/hip_ext.h>
+#include "bonded.hpp"
 #include "system.hpp"
 
 namespace gamd {
@@ -101,6 +102,8 @@ struct SysPtrs {
     const int32_t* pap_off;    // npad + 1: entries of each atom
     const int32_t* pap;        // entry index | bit 31 when the atom is the entry's i
     int32_t* o2s;              // original atom -> sorted atom
+    BondedDev bd;              // covalent terms; bd.fb per solute atom (sol_atom order),
+                               // nullptr when the batch has none
 };
 
 // nctl: words of the build's control block (counters) that k_cg_ref clears

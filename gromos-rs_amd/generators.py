@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from .system import FOUR_PI_EPS_I, K_BOLTZMANN, Perturbation, System, csr_from_lists
+from .system import FOUR_PI_EPS_I, K_BOLTZMANN, Bonded, Perturbation, System, csr_from_lists
 
 SPC_C6 = 2.6173456e-3
 SPC_C12 = 2.634129e-6
@@ -219,13 +219,52 @@ def _random_walk(rng, n, bond=0.15, start=None, min_dist=0.22, L=None):
     return pos
 
 
+def _dihedral_angle(a, b, c, d):
+    """signed angle between the planes (a, b, c) and (b, c, d), as the improper
+    term measures it (improper.rs:27-47)"""
+    rij, rkj, rkl = a - b, c - b, c - d
+    m, n = np.cross(rij, rkj), np.cross(rkj, rkl)
+    cs = np.clip((m * n).sum(1) / (np.linalg.norm(m, axis=1) * np.linalg.norm(n, axis=1)), -1.0, 1.0)
+    z = np.arccos(cs)
+    return np.where((rij * n).sum(1) < 0.0, -z, z)
+
+
+def chain_terms(rng, x, off=0):
+    """Covalent terms along a chain at positions x (first atom index off): a
+    bond per neighbour pair, an angle per triple, a proper dihedral per quadruple
+    and an improper on every fifth quadruple.  Bonds, angles and impropers get a
+    type each, whose equilibrium value is the present one with a little noise
+    (r0 within 1 %, theta0 within 2 degrees, q0 within 0.05 rad) and GROMOS-like
+    constants (quartic 1.0-1.4e7 kJ/mol/nm^4, cosine 400-600 kJ/mol, improper
+    100-300 kJ/mol/rad^2); dihedrals draw from seven types, multiplicities 0-6,
+    cos(delta) = +-1, K 1-6 kJ/mol."""
+    n = len(x)
+    i = np.arange(n, dtype=np.int32)
+    r = np.linalg.norm(x[1:] - x[:-1], axis=1)
+    bt = np.stack([rng.uniform(1.0e7, 1.4e7, n - 1), np.zeros(n - 1), r * rng.uniform(0.99, 1.01, n - 1)], 1)
+    u, v = x[:-2] - x[1:-1], x[2:] - x[1:-1]
+    th = np.arccos(np.clip((u * v).sum(1) / (np.linalg.norm(u, axis=1) * np.linalg.norm(v, axis=1)), -1, 1))
+    at = np.stack([rng.uniform(400, 600, n - 2), np.zeros(n - 2), th + np.radians(rng.uniform(-2, 2, n - 2))], 1)
+    dt = np.stack([rng.uniform(1.0, 6.0, 7), rng.choice([-1.0, 1.0], 7), np.arange(7, dtype=float)], 1)
+    q = np.arange(0, n - 3, 5)
+    zeta = _dihedral_angle(x[q], x[q + 1], x[q + 2], x[q + 3])
+    it = np.stack([rng.uniform(100, 300, len(q)), zeta + rng.uniform(-0.05, 0.05, len(q))], 1)
+    bonds = np.stack([i[:-1] + off, i[1:] + off, np.arange(n - 1)], 1)
+    angles = np.stack([i[:-2] + off, i[1:-1] + off, i[2:] + off, np.arange(n - 2)], 1)
+    dih = np.stack([i[:-3] + off, i[1:-2] + off, i[2:-1] + off, i[3:] + off, rng.integers(0, 7, n - 3)], 1)
+    imp = np.stack([q + off, q + 1 + off, q + 2 + off, q + 3 + off, np.arange(len(q))], 1)
+    return Bonded.of(bonds, bt, angles, at, imp, it, dih, dt)
+
+
 def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5, n_types=20, pert_every=0,
-                  lam=0.5, name="hetero"):
+                  lam=0.5, name="hetero", bonded=False):
     """Heterogeneous single-point system: a random-walk solute chain (bond 0.15 nm)
     in charge groups of 1-4 atoms with N(0, 0.3^2) charges shifted to zero per
     group, 20 LJ types, 1-2 / 1-3 exclusions, 1-4 pairs with the scaled table,
     plus SPC water.  pert_every > 0 perturbs every pert_every-th chain atom
-    (types and charges of state B drawn the same way)."""
+    (types and charges of state B drawn the same way).  bonded: the chain keeps
+    its covalent terms (chain_terms) and stays whole (the covalent terms take
+    raw coordinate differences); otherwise each group is moved into the box."""
     rng = np.random.default_rng(seed)
     L = float(box)
     chain = _random_walk(rng, n_chain, start=rng.uniform(0, L, 3), L=L)
@@ -238,9 +277,10 @@ def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5,
         sizes.append(k)
         left -= k
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
-    for c in range(len(sizes)):
-        a, b = starts[c], starts[c + 1]
-        chain[a:b] -= L * np.floor(chain[a] / L)
+    if not bonded:
+        for c in range(len(sizes)):
+            a, b = starts[c], starts[c + 1]
+            chain[a:b] -= L * np.floor(chain[a] / L)
     q = np.zeros(n_chain)
     for c in range(len(sizes)):
         a, b = starts[c], starts[c + 1]
@@ -324,7 +364,8 @@ def hetero_system(n_chain=6000, n_waters=18000, box=8.4, seed=5, rc=1.4, nsnb=5,
                   iac=iac, mass=mass, n_types=nt, lj=lj, excl_off=excl_off, excl=excl_idx, o14_off=o14_off,
                   o14=o14_idx, atoms_per_solvent=3, pos=pos, vel=vel, box=np.array([L, L, L]), boundary=1,
                   rcutp=rc, rcutl=rc, nsnb=nsnb, epsrf=61.0, kappa=0.0, fpepsi=FOUR_PI_EPS_I, dt=0.002,
-                  dist_oh=D_OH, dist_hh=D_HH, pert=pert, name=name).validate()
+                  dist_oh=D_OH, dist_hh=D_HH, pert=pert, name=name,
+                  bonded=chain_terms(rng, chain) if bonded else None).validate()
 
 
 def fep_job(n_atoms, n_pert, lam, seed, rc=1.4, nsnb=5, alpha_lj=0.5, alpha_crf=0.5, n_types=20,

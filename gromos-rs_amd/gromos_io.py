@@ -40,7 +40,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .system import FOUR_PI_EPS_I, K_BOLTZMANN, Perturbation, System, csr_from_lists
+from .system import FOUR_PI_EPS_I, K_BOLTZMANN, Bonded, Perturbation, System, csr_from_lists
 
 
 class GromosFormatError(ValueError):
@@ -304,6 +304,7 @@ class GromosInput:
     epsrf: float = 61.0
     ntc: int = 1
     ntcs: str = "shake"                    # solvent constraints algorithm
+    ntf: Tuple[int, int, int, int] = (1, 1, 1, 1)   # FORCE: bonds, angles, impropers, dihedrals on
     baths: List[Tuple[float, float]] = field(default_factory=list)   # (TEMP0, TAU)
     coupling: str = ""
     ntg: int = 0                           # perturbation on
@@ -346,6 +347,9 @@ def read_imd(path):
         d = _data(blocks["NONBONDED"])
         p = d[1].split()
         m.appak, m.rcrf, m.epsrf = float(p[0]), float(p[1]), float(p[2])
+    if "FORCE" in blocks:
+        # md.rs:331-335: NTF(1..4) switch bonds, angles, impropers, dihedrals
+        m.ntf = tuple(int(x) for x in _data(blocks["FORCE"])[0].split()[:4])
     if "CONSTRAINT" in blocks:
         d = _data(blocks["CONSTRAINT"])
         m.ntc = int(d[0].split()[0])
@@ -515,12 +519,23 @@ def build_system(topo, conf, imd, ptp=None, name="gromos"):
             pert.alpha_crf[k] = a["acrf"] * imd.alphc
         pert.lam, pert.nlam = imd.rlam, imd.nlam
         pert.atom_pairs = np.array(pert_pairs, np.int32).reshape(-1, 4)
+    bonded = None
+    top = topo
+    if top.bonds or top.angles or top.impropers or top.dihedrals:
+        # type tables as the reference stores them: theta0 and Q0 in radians
+        # (topology.rs:505-513, :699-703), cos(PD) (:661-664)
+        bonded = Bonded.of(top.bonds, [list(t) for t in top.bond_types], top.angles,
+                           [[ct, cht, t0 * (math.pi / 180.0)] for ct, cht, t0 in top.angle_types],
+                           top.impropers, [[cq * 180.0 * 180.0 / (math.pi * math.pi), q0 * math.pi / 180.0]
+                                            for cq, q0 in top.improper_types],
+                           top.dihedrals, [[cp, math.cos(pd * math.pi / 180.0), npd]
+                                           for cp, pd, npd in top.dihedral_types], imd.ntf)
     s = System(n_atoms=n, n_solute_atoms=ns, cg_start=np.array(cg, np.int32), n_solute_cg=n_solute_cg,
                charge=charge, iac=iac, mass=mass, n_types=nt, lj=lj.reshape(-1), excl_off=excl_off, excl=excl,
                o14_off=o14_off, o14=o14, atoms_per_solvent=aps, pos=pos, vel=vel, box=box, boundary=boundary,
                rcutp=imd.rcutp, rcutl=imd.rcutl, skin=0.0, nsnb=imd.nsnb, algorithm=imd.algorithm,
                epsrf=imd.epsrf, kappa=imd.appak, fpepsi=topo.fpepsi, dt=imd.dt, dist_oh=d_oh, dist_hh=d_hh,
-               pert=pert, name=name, box_matrix=box_matrix)
+               pert=pert, name=name, box_matrix=box_matrix, bonded=bonded)
     return s.validate()
 
 
@@ -627,16 +642,16 @@ def write_configuration(path, system, pos, vel, title="final configuration", ntb
 # an md run from GROMOS files on the device
 # ---------------------------------------------------------------------------
 def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, nsteps=None):
-    """gromos-rs md for the systems this library covers (nonbonded forces, SETTLE
-    waters, optional Berendsen bath and perturbation): reads the files, runs
-    NSTLIM (or nsteps) md steps on the device, and writes out_prefix.tre (every
-    step, or every NTWE), out_prefix.trf (the step-0 forces) and out_prefix.cnf.
+    """gromos-rs md for the systems this library covers (nonbonded forces, the
+    solute's covalent terms, SETTLE waters, optional Berendsen bath and
+    perturbation of the nonbonded terms): reads the files, runs NSTLIM (or
+    nsteps) md steps on the device, and writes out_prefix.tre (every step, or
+    every NTWE), out_prefix.trf (the step-0 forces) and out_prefix.cnf.
     Returns the System and the per-step energies (e_lj, e_crf, e_pot, e_kin, dhdl)."""
     from . import runtime as rt
     s, top, inp = load(topology, configuration, imd, ptp)
-    if top.bonds or top.angles or top.impropers or top.dihedrals:
-        raise GromosFormatError("bonded terms are not computed on the device here; this run covers systems "
-                                "without them")
+    if inp.ntc > 1 and s.bonded is not None and len(s.bonded.bonds):
+        raise GromosFormatError("solute bond constraints (NTC > 1) are not provided")
     precision = rt.FP64 if precision is None else precision
     n = inp.nstlim if nsteps is None else int(nsteps)
     if inp.baths and inp.baths[0][1] > 0.0:
@@ -652,7 +667,7 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
     fw.close()
     ds = rt.DeviceSystem(s, precision=precision)
     ds.pairlist_update()
-    e = ds.md(n) if n > 0 else np.zeros((0, 5))
+    e, cov = ds.md(n, covalent=True) if n > 0 else (np.zeros((0, 5)), np.zeros((0, 4)))
     x, v = ds.get_state()
     ds.close()
     ew = EnergyTrajectoryWriter(out_prefix + ".tre")
@@ -661,7 +676,8 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
     for k in range(0, len(e), every):
         ew.write(time=inp.t0 + k * s.dt, kinetic=e[k, 3], potential=e[k, 2], total=e[k, 2] + e[k, 3],
                  temperature=2.0 * e[k, 3] / (dof * K_BOLTZMANN) if dof > 0 else 0.0,
-                 volume=float(np.prod(s.box)) if s.boundary else 0.0, lj=e[k, 0], coul_real=e[k, 1])
+                 volume=float(np.prod(s.box)) if s.boundary else 0.0, lj=e[k, 0], coul_real=e[k, 1],
+                 bond=cov[k, 0], angle=cov[k, 1], dihedral=cov[k, 2], improper=cov[k, 3])
     ew.close()
     out_pos, out_vel = x, v
     if inp.ntb == -1:

@@ -45,6 +45,15 @@ class Pert(C.Structure):
                 ("alpha_crf", P)]
 
 
+class Bonded(C.Structure):
+    _fields_ = [("n_atoms", I32),
+                ("n_bonds", I32), ("n_bond_types", I32), ("bonds", P), ("bond_types", P),
+                ("n_angles", I32), ("n_angle_types", I32), ("angles", P), ("angle_types", P),
+                ("n_impropers", I32), ("n_improper_types", I32), ("impropers", P), ("improper_types", P),
+                ("n_dihedrals", I32), ("n_dihedral_types", I32), ("dihedrals", P), ("dihedral_types", P),
+                ("ntf", I32 * 4)]
+
+
 class Topology(C.Structure):
     _fields_ = [("n_atoms", I32), ("n_solute_atoms", I32), ("n_cg", I32), ("n_solute_cg", I32),
                 ("cg_start", P), ("charge", P), ("iac", P), ("mass", P), ("n_types", I32), ("lj", P),
@@ -52,7 +61,7 @@ class Topology(C.Structure):
                 ("dist_oh", D), ("dist_hh", D),
                 ("pert_flag", P), ("pert_a_iac", P), ("pert_b_iac", P), ("pert_a_q", P), ("pert_b_q", P),
                 ("pert_alpha_lj", P), ("pert_alpha_crf", P), ("n_pert_atom_pairs", I32),
-                ("pert_atom_pairs", P)]
+                ("pert_atom_pairs", P), ("bonded", C.POINTER(Bonded))]
 
 
 class SessionParams(C.Structure):
@@ -131,6 +140,12 @@ def _declare(L):
     L.gromos_amd_system_pairlist_update.argtypes = [P]
     L.gromos_amd_system_forcefield_apply.argtypes = [P, P, P, P]
     L.gromos_amd_system_md.argtypes = [P, I32, P]
+    L.gromos_amd_system_covalent_energies.argtypes = [P, P]
+    L.gromos_amd_system_md_covalent.argtypes = [P, I32, P, P]
+    L.gromos_amd_bonded_plan_create.argtypes = [C.POINTER(Bonded), C.POINTER(P)]
+    L.gromos_amd_bonded_plan_destroy.argtypes = [P]
+    L.gromos_amd_bonded_plan_terms.argtypes = [P, P]
+    L.gromos_amd_bonded_forces.argtypes = [P, P, P, P, P]
     L.gromos_amd_system_pairlist_get.argtypes = [P, P, P, P, P, P]
     L.gromos_amd_system_pair_count.argtypes = [P]
     L.gromos_amd_system_pair_count.restype = I64
@@ -195,6 +210,9 @@ def lambda_from(lam, n=1, ljs=None, crfs=None, deriv=1.0):
     return lp
 
 
+COVALENT = ("e_bond", "e_angle", "e_dihedral", "e_improper")
+
+
 class _Keep:
     """Holds numpy arrays alive while C structs point into them."""
 
@@ -230,7 +248,28 @@ def topology_struct(s, keep):
         ap = np.asarray(p.atom_pairs, np.int32).reshape(-1)
         t.n_pert_atom_pairs = len(ap) // 4
         t.pert_atom_pairs = keep(ap, np.int32)
+    b = getattr(s, "bonded", None)
+    if b is not None and b.n_terms > 0:
+        bs = bonded_struct(b, s.n_atoms, keep)
+        keep.arrays.append(bs)
+        t.bonded = C.pointer(bs)
     return t
+
+
+def bonded_struct(b, n_atoms, keep):
+    """gromos_amd_bonded over a system.Bonded (arrays kept alive by keep)."""
+    bs = Bonded()
+    bs.n_atoms = int(n_atoms)
+    for nm, w, tw in (("bond", 3, 3), ("angle", 4, 3), ("improper", 5, 2), ("dihedral", 5, 3)):
+        idx = np.asarray(getattr(b, nm + "s"), np.int32).reshape(-1, w)
+        ty = np.asarray(getattr(b, nm + "_types"), np.float64).reshape(-1, tw)
+        setattr(bs, f"n_{nm}s", len(idx))
+        setattr(bs, f"n_{nm}_types", len(ty))
+        setattr(bs, nm + "s", keep(idx, np.int32))
+        setattr(bs, nm + "_types", keep(ty, np.float64))
+    for k in range(4):
+        bs.ntf[k] = int(b.ntf[k])
+    return bs
 
 
 def params_struct(s, use_settle=True, do_virial=False):
@@ -298,12 +337,26 @@ class DeviceSystem:
         e = np.zeros(3)
         vir = np.zeros(9)
         check(lib().gromos_amd_system_forcefield_apply(self.h, ptr(f), ptr(e), ptr(vir)))
-        return dict(forces=f, e_lj=e[0], e_crf=e[1], dhdl=e[2], virial=vir.reshape(3, 3))
+        out = dict(forces=f, e_lj=e[0], e_crf=e[1], dhdl=e[2], virial=vir.reshape(3, 3))
+        out.update(zip(COVALENT, self.covalent_energies()))
+        return out
 
-    def md(self, nsteps):
-        e = np.zeros((nsteps, 5))
-        check(lib().gromos_amd_system_md(self.h, int(nsteps), ptr(e)))
+    def covalent_energies(self):
+        """e_bond, e_angle, e_dihedral, e_improper of the last force evaluation."""
+        e = np.zeros(4)
+        check(lib().gromos_amd_system_covalent_energies(self.h, ptr(e)))
         return e
+
+    def md(self, nsteps, covalent=False):
+        """(nsteps, 5) e_lj, e_crf, e_pot, e_kin, dhdl; with covalent=True also
+        (nsteps, 4) e_bond, e_angle, e_dihedral, e_improper."""
+        e = np.zeros((nsteps, 5))
+        if not covalent:
+            check(lib().gromos_amd_system_md(self.h, int(nsteps), ptr(e)))
+            return e
+        c = np.zeros((nsteps, 4))
+        check(lib().gromos_amd_system_md_covalent(self.h, int(nsteps), ptr(e), ptr(c)))
+        return e, c
 
     def pairlist(self):
         n = np.zeros(4, np.int64)

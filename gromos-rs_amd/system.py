@@ -56,6 +56,40 @@ class Perturbation:
 
 
 @dataclass
+class Bonded:
+    """Covalent terms of the solute (crates/gromos-forces/src/bonded): index
+    arrays with a 0-based type per term, and the type tables as the reference
+    stores them (crates/gromos-io/src/topology.rs:373-710)."""
+    bonds: np.ndarray            # i32 [nb, 3]  i, j, type
+    bond_types: np.ndarray       # f64 [tb, 3]  k_quartic (CB), k_harmonic (CHB), r0
+    angles: np.ndarray           # i32 [na, 4]  i, j, k, type
+    angle_types: np.ndarray      # f64 [ta, 3]  k_cosine (CT), k_harmonic (CHT), theta0 (rad)
+    impropers: np.ndarray        # i32 [ni, 5]  i, j, k, l, type
+    improper_types: np.ndarray   # f64 [ti, 2]  k (CQ), q0 (rad)
+    dihedrals: np.ndarray        # i32 [nd, 5]  i, j, k, l, type
+    dihedral_types: np.ndarray   # f64 [td, 3]  k (CP), cos(PD), multiplicity
+    # FORCE block NTF(1..4): bonds, angles, impropers, dihedrals switched on (md.rs:331-335)
+    ntf: tuple = (1, 1, 1, 1)
+
+    @staticmethod
+    def of(bonds, bond_types, angles, angle_types, impropers, improper_types, dihedrals, dihedral_types,
+           ntf=(1, 1, 1, 1)):
+        def ia(a, w):
+            return np.asarray(a, np.int32).reshape(-1, w)
+
+        def fa(a, w):
+            return np.asarray(a, float).reshape(-1, w)
+
+        return Bonded(ia(bonds, 3), fa(bond_types, 3), ia(angles, 4), fa(angle_types, 3), ia(impropers, 5),
+                      fa(improper_types, 2), ia(dihedrals, 5), fa(dihedral_types, 3),
+                      tuple(int(bool(x)) for x in ntf))
+
+    @property
+    def n_terms(self):
+        return len(self.bonds) + len(self.angles) + len(self.impropers) + len(self.dihedrals)
+
+
+@dataclass
 class System:
     n_atoms: int
     n_solute_atoms: int
@@ -94,6 +128,7 @@ class System:
     # GROMOS convention a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz)
     # (crates/gromos-core/src/math.rs:75-92)
     box_matrix: Optional[np.ndarray] = None
+    bonded: Optional[Bonded] = None      # covalent terms of the solute (none: nonbonded only)
 
     def box_values(self):
         """the box as the C ABI takes it: 3 lengths, or a, b, c (9 values) for triclinic"""
@@ -134,6 +169,12 @@ class System:
         if self.pert is not None:
             p = self.pert
             assert p.a_iac.max(initial=0) < self.n_types and p.b_iac.max(initial=0) < self.n_types
+        if self.bonded is not None:
+            b = self.bonded
+            for idx, w in ((b.bonds, 2), (b.angles, 3), (b.impropers, 4), (b.dihedrals, 4)):
+                if len(idx):
+                    assert idx[:, :w].min() >= 0 and idx[:, :w].max() < self.n_solute_atoms
+                    assert idx[:, w].min() >= 0
         return self
 
 

@@ -229,6 +229,49 @@ int gromos_amd_perturbed_lj_crf_interaction(const double *r, const double *prm, 
                                             const gromos_amd_lambda *lp, const gromos_amd_crf *crf,
                                             double four_pi_eps_i, double *out, void *stream);
 
+/* Covalent terms of the solute (crates/gromos-forces/src/bonded): each index
+ * row holds the atoms and a 0-based type; type tables as the reference stores
+ * them (crates/gromos-io/src/topology.rs:373-710, units converted):
+ *   bonds      i j t       bond_types     k_quartic k_harmonic r0        (3 per type)
+ *   angles     i j k t     angle_types    k_cosine k_harmonic theta0/rad (3)
+ *   impropers  i j k l t   improper_types k (kJ/mol/rad^2) q0/rad      (2)
+ *   dihedrals  i j k l t   dihedral_types k cos(delta) m (integer 0..6)  (3)
+ * A term whose type is beyond its table, or a dihedral of another
+ * multiplicity, is skipped as the reference skips it.  ntf: the FORCE block's
+ * NTF(1..4) switches (md.rs:331-335): bonds, angles, impropers, dihedrals. */
+typedef struct {
+    int32_t n_atoms;                /* the standalone plan: atoms of the position array */
+    int32_t n_bonds, n_bond_types;
+    const int32_t *bonds;
+    const double *bond_types;
+    int32_t n_angles, n_angle_types;
+    const int32_t *angles;
+    const double *angle_types;
+    int32_t n_impropers, n_improper_types;
+    const int32_t *impropers;
+    const double *improper_types;
+    int32_t n_dihedrals, n_dihedral_types;
+    const int32_t *dihedrals;
+    const double *dihedral_types;
+    int32_t ntf[4];
+} gromos_amd_bonded;
+
+/* calculate_bonded_forces_ntf with quartic bonds (bonded/mod.rs:141-185,
+ * forcefield.rs:386-400) as a plan: the terms are validated and laid out on the
+ * device once (host inputs), then applied to device positions any number of
+ * times.  Covalent forces only (3 n_atoms, assigned) and out[13] (device):
+ * e_bond, e_angle, e_dihedral, e_improper, virial[9] (row a, column b).  FP64;
+ * an atom's force is summed in the reference's order (term order within a kind,
+ * kinds added bonds, angles, dihedrals, impropers).  The plan holds per-term
+ * scratch: calls that overlap on one plan race. */
+typedef struct gromos_amd_bonded_plan gromos_amd_bonded_plan;
+int gromos_amd_bonded_plan_create(const gromos_amd_bonded *terms, gromos_amd_bonded_plan **out);
+void gromos_amd_bonded_plan_destroy(gromos_amd_bonded_plan *plan);
+/* terms kept per kind: bonds, angles, dihedrals, impropers */
+int gromos_amd_bonded_plan_terms(const gromos_amd_bonded_plan *plan, int32_t counts[4]);
+int gromos_amd_bonded_forces(const gromos_amd_bonded_plan *plan, const double *pos, double *forces,
+                             double *out, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* 2. device-resident systems                                          */
 /* ------------------------------------------------------------------ */
@@ -256,6 +299,10 @@ typedef struct {
     const double *pert_a_q, *pert_b_q, *pert_alpha_lj, *pert_alpha_crf;
     int32_t n_pert_atom_pairs;
     const int32_t *pert_atom_pairs; /* i, j, a_type, b_type (-1 = absent in B) */
+    /* optional covalent terms of the solute atoms (NULL = none; n_atoms not read):
+     * every force evaluation adds them, FP64 in both precisions, and e_pot
+     * includes their energies (configuration.rs:197-207) */
+    const gromos_amd_bonded *bonded;
 } gromos_amd_topology;
 
 /* Run parameters (IMD PAIRLIST / NONBONDED / STEP / PERTURBATION blocks). */
@@ -302,10 +349,17 @@ int gromos_amd_system_pairlist_update(gromos_amd_system *sys);
  * Host outputs, any may be NULL: forces (3n), energies (e_lj, e_crf, dhdl), virial (9). */
 int gromos_amd_system_forcefield_apply(gromos_amd_system *sys, double *forces, double *energies,
                                        double *virial);
+/* Covalent energies of the last force evaluation (forcefield_apply, or the
+ * last step of md): 4 per job, e_bond, e_angle, e_dihedral, e_improper. */
+int gromos_amd_system_covalent_energies(gromos_amd_system *sys, double *out);
 /* nsteps of Forcefield -> LeapFrogVelocity -> LeapFrogPosition -> SETTLE ->
  * TemperatureCalculation (md.rs:1143-1286).  energies (optional, host): per step
  * 5 doubles (e_lj, e_crf, e_pot, e_kin, dhdl). */
 int gromos_amd_system_md(gromos_amd_system *sys, int32_t nsteps, double *energies);
+/* md with the covalent energies of every step beside (covalent: 4 per step and
+ * job, e_bond, e_angle, e_dihedral, e_improper; either output may be NULL):
+ * the .tre bond / angle / improper / dihedral columns (md.rs energy output). */
+int gromos_amd_system_md_covalent(gromos_amd_system *sys, int32_t nsteps, double *energies, double *covalent);
 /* Pair lists materialized as the reference's four atom-pair lists (test mode).
  * Pass NULL buffers to query counts (n_out[4]: solute_short, solute_long,
  * solvent_short, solvent_long); then call again with buffers of 2*n u32. */

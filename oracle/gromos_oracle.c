@@ -921,6 +921,224 @@ void or_perturbed_atom_pair_correction(const or_topo *t, const double *pos, cons
 }
 
 /* ------------------------------------------------------------------ */
+/* covalent terms (crates/gromos-forces/src/bonded), no periodic image */
+/* (the reference takes pos(i) - pos(j) directly)                      */
+/* ------------------------------------------------------------------ */
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void sub3(const double *a, const double *b, double *r) { for (int k = 0; k < 3; ++k) r[k] = a[k] - b[k]; }
+static inline void cross3(const double *a, const double *b, double *r) {   /* glam DVec3::cross */
+    r[0] = a[1] * b[2] - b[1] * a[2];
+    r[1] = a[2] * b[0] - b[2] * a[0];
+    r[2] = a[0] * b[1] - b[0] * a[1];
+}
+/* glam DVec3::length: sqrt of the dot product */
+static inline double length3(const double *a) { return sqrt(dot3(a, a)); }
+
+/* calculate_bond_forces_quartic (bonds.rs:12-50) */
+static double bonds_quartic(const or_bonded *b, const double *pos, double *f, double *vir) {
+    double e = 0.0;
+    for (int k = 0; k < b->nb; ++k) {
+        int i = b->bonds[3 * k], j = b->bonds[3 * k + 1], t = b->bonds[3 * k + 2];
+        if (t >= b->nbt) continue;
+        const double *p = b->bond_types + 3 * t;
+        double r[3];
+        sub3(pos + 3 * i, pos + 3 * j, r);
+        double r2 = dot3(r, r);
+        double r0_2 = p[2] * p[2];
+        double dr2 = r2 - r0_2;
+        e += 0.25 * p[0] * dr2 * dr2;
+        double fs = -p[0] * dr2, fv[3];
+        for (int a = 0; a < 3; ++a) { fv[a] = r[a] * fs; f[3 * i + a] += fv[a]; f[3 * j + a] -= fv[a]; }
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) vir[3 * a + c] += r[a] * fv[c];
+    }
+    return e;
+}
+
+/* calculate_angle_forces, cosine-harmonic (angles.rs:12-75) */
+static double angles_cosine(const or_bonded *b, const double *pos, double *f, double *vir) {
+    double e = 0.0;
+    for (int q = 0; q < b->na; ++q) {
+        int i = b->angles[4 * q], j = b->angles[4 * q + 1], k = b->angles[4 * q + 2], t = b->angles[4 * q + 3];
+        if (t >= b->nat) continue;
+        const double *p = b->angle_types + 3 * t;
+        double rij[3], rkj[3];
+        sub3(pos + 3 * i, pos + 3 * j, rij);
+        sub3(pos + 3 * k, pos + 3 * j, rkj);
+        double dij = length3(rij), dkj = length3(rkj);
+        if (dij < 1e-10 || dkj < 1e-10) continue;
+        double cost = dot3(rij, rkj) / (dij * dkj);
+        if (cost < -1.0) cost = -1.0;
+        if (cost > 1.0) cost = 1.0;
+        double cos0 = cos(p[2]);
+        double d_cos = cost - cos0;
+        e += 0.5 * p[0] * d_cos * d_cos;
+        double df = -p[0] * d_cos;
+        double fi[3], fk[3];
+        for (int a = 0; a < 3; ++a) {
+            fi[a] = (rkj[a] * (1.0 / dkj) - rij[a] * (cost / dij)) * (df / dij);
+            fk[a] = (rij[a] * (1.0 / dij) - rkj[a] * (cost / dkj)) * (df / dkj);
+        }
+        for (int a = 0; a < 3; ++a) {
+            double fj = -(fi[a] + fk[a]);
+            f[3 * i + a] += fi[a];
+            f[3 * j + a] += fj;
+            f[3 * k + a] += fk[a];
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) vir[3 * a + c] += rij[a] * fi[c] + rkj[a] * fk[c];
+    }
+    return e;
+}
+
+/* powi for the Chebyshev terms: squaring as LLVM expands a constant power */
+static inline double pw(double x, int n) { return powi_(x, n); }
+
+/* calculate_dihedral_forces (dihedrals.rs:19-130) */
+static double dihedrals_cheb(const or_bonded *b, const double *pos, double *f, double *vir) {
+    double e = 0.0;
+    for (int q = 0; q < b->nd; ++q) {
+        const int32_t *d = b->dihedrals + 5 * q;
+        int i = d[0], j = d[1], k = d[2], l = d[3], t = d[4];
+        if (t >= b->ndt) continue;
+        const double *p = b->dihedral_types + 3 * t;
+        double K = p[0], cospd = p[1];
+        int m = (int)p[2];
+        double rij[3], rkj[3], rkl[3];
+        sub3(pos + 3 * i, pos + 3 * j, rij);
+        sub3(pos + 3 * k, pos + 3 * j, rkj);
+        sub3(pos + 3 * k, pos + 3 * l, rkl);
+        double dkj2 = dot3(rkj, rkj);
+        if (dkj2 < 1e-10) continue;
+        double frim = dot3(rij, rkj) / dkj2, frln = dot3(rkl, rkj) / dkj2;
+        double rim[3], rln[3];
+        for (int a = 0; a < 3; ++a) {
+            rim[a] = rij[a] - rkj[a] * frim;
+            rln[a] = rkj[a] * frln - rkl[a];
+        }
+        double dim = length3(rim), dln = length3(rln);
+        if (dim < 1e-10 || dln < 1e-10) continue;
+        double c = dot3(rim, rln) / (dim * dln);
+        if (c < -1.0) c = -1.0;
+        if (c > 1.0) c = 1.0;
+        double cm, dcm;
+        switch (m) {
+            case 0: cm = 0.0; dcm = 0.0; break;
+            case 1: cm = c; dcm = 1.0; break;
+            case 2: cm = 2.0 * c * c - 1.0; dcm = 4.0 * c; break;
+            case 3: cm = 4.0 * pw(c, 3) - 3.0 * c; dcm = 12.0 * c * c - 3.0; break;
+            case 4: cm = 8.0 * pw(c, 4) - 8.0 * c * c + 1.0; dcm = 32.0 * pw(c, 3) - 16.0 * c; break;
+            case 5: cm = 16.0 * pw(c, 5) - 20.0 * pw(c, 3) + 5.0 * c; dcm = 80.0 * pw(c, 4) - 60.0 * c * c + 5.0; break;
+            case 6:
+                cm = 32.0 * pw(c, 6) - 48.0 * pw(c, 4) + 18.0 * c * c - 1.0;
+                dcm = 192.0 * pw(c, 5) - 192.0 * pw(c, 3) + 36.0 * c;
+                break;
+            default: continue;   /* unsupported multiplicity: the term is skipped */
+        }
+        e += K * (1.0 + cospd * cm);
+        double ki = -K * cospd * dcm / dim, kl = -K * cospd * dcm / dln;
+        double kj1 = frim - 1.0, kj2 = frln;
+        double fi[3], fl[3], fj[3], fk[3], rlj[3];
+        for (int a = 0; a < 3; ++a) {
+            fi[a] = rln[a] * (ki / dln) - rim[a] * (ki * c / dim);
+            fl[a] = rim[a] * (kl / dim) - rln[a] * (kl * c / dln);
+            fj[a] = fi[a] * kj1 - fl[a] * kj2;
+            fk[a] = -(fi[a] + fj[a] + fl[a]);
+            f[3 * i + a] += fi[a];
+            f[3 * j + a] += fj[a];
+            f[3 * k + a] += fk[a];
+            f[3 * l + a] += fl[a];
+        }
+        sub3(pos + 3 * l, pos + 3 * j, rlj);
+        for (int a = 0; a < 3; ++a)
+            for (int bb = 0; bb < 3; ++bb) vir[3 * a + bb] += rij[a] * fi[bb] + rkj[a] * fk[bb] + rlj[a] * fl[bb];
+    }
+    return e;
+}
+
+/* calculate_improper_dihedral_forces (improper.rs:12-105) */
+static double impropers_harm(const or_bonded *b, const double *pos, double *f, double *vir) {
+    const double PI = 3.14159265358979323846;
+    double e = 0.0;
+    for (int q = 0; q < b->ni; ++q) {
+        const int32_t *d = b->impropers + 5 * q;
+        int i = d[0], j = d[1], k = d[2], l = d[3], t = d[4];
+        if (t >= b->nit) continue;
+        const double *p = b->improper_types + 2 * t;
+        double K = p[0], q0 = p[1];
+        double rkj[3], rij[3], rkl[3], rmj[3], rnk[3];
+        sub3(pos + 3 * k, pos + 3 * j, rkj);
+        sub3(pos + 3 * i, pos + 3 * j, rij);
+        sub3(pos + 3 * k, pos + 3 * l, rkl);
+        cross3(rij, rkj, rmj);
+        cross3(rkj, rkl, rnk);
+        double dkj2 = dot3(rkj, rkj), dmj2 = dot3(rmj, rmj), dnk2 = dot3(rnk, rnk);
+        double dkj = sqrt(dkj2), dmj = sqrt(dmj2), dnk = sqrt(dnk2);
+        if (dmj < 1e-10 || dnk < 1e-10) continue;
+        double acs = dot3(rmj, rnk) / (dmj * dnk);
+        if (acs < -1.0) acs = -1.0;
+        if (acs > 1.0) acs = 1.0;
+        double zeta = acos(acs);
+        if (dot3(rij, rnk) < 0.0) zeta = -zeta;
+        double za = zeta;
+        while (za < (q0 - PI)) za += 2.0 * PI;
+        while (za > (q0 + PI)) za -= 2.0 * PI;
+        double dz = za - q0;
+        e += 0.5 * K * dz * dz;
+        double ki = -K * dz * dkj, kl = -ki;
+        if (dmj2 < 1.0e-10 * dkj2) ki = 0.0; else ki /= dmj2;
+        if (dnk2 < 1.0e-10 * dkj2) kl = 0.0; else kl /= dnk2;
+        double kj1 = dot3(rij, rkj) / dkj2 - 1.0, kj2 = dot3(rkl, rkj) / dkj2;
+        double fi[3], fl[3], fj[3], fk[3], rlj[3];
+        for (int a = 0; a < 3; ++a) {
+            fi[a] = rmj[a] * ki;
+            fl[a] = rnk[a] * kl;
+            fj[a] = fi[a] * kj1 - fl[a] * kj2;
+            fk[a] = -(fi[a] + fj[a] + fl[a]);
+            f[3 * i + a] += fi[a];
+            f[3 * j + a] += fj[a];
+            f[3 * k + a] += fk[a];
+            f[3 * l + a] += fl[a];
+        }
+        sub3(pos + 3 * l, pos + 3 * j, rlj);
+        for (int a = 0; a < 3; ++a)
+            for (int bb = 0; bb < 3; ++bb) vir[3 * a + bb] += rij[a] * fi[bb] + rkj[a] * fk[bb] + rlj[a] * fl[bb];
+    }
+    return e;
+}
+
+void or_bonded_forces(const or_bonded *b, int n, const double *pos, double *f, double vir[9], or_bonded_e *e) {
+    size_t nb = sizeof(double) * 3 * (size_t)n;
+    double *part = (double *)malloc(nb);
+    double pv[9];
+    memset(f, 0, nb);
+    for (int c = 0; c < 9; ++c) vir[c] = 0.0;
+    double en[4];
+    /* NTF gates (md.rs:331-335): bonds ntf[0], angles ntf[1], impropers ntf[2], dihedrals ntf[3] */
+    const int on[4] = {b->ntf[0], b->ntf[1], b->ntf[3], b->ntf[2]};
+    for (int kind = 0; kind < 4; ++kind) {
+        en[kind] = 0.0;
+        if (!on[kind]) continue;
+        memset(part, 0, nb);
+        for (int c = 0; c < 9; ++c) pv[c] = 0.0;
+        switch (kind) {
+            case 0: en[0] = bonds_quartic(b, pos, part, pv); break;
+            case 1: en[1] = angles_cosine(b, pos, part, pv); break;
+            case 2: en[2] = dihedrals_cheb(b, pos, part, pv); break;
+            default: en[3] = impropers_harm(b, pos, part, pv); break;
+        }
+        /* ForceEnergy::add (bonded/mod.rs:54-62) */
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) f[k] += part[k];
+        for (int c = 0; c < 9; ++c) vir[c] += pv[c];
+    }
+    free(part);
+    e->e_bond = en[0];
+    e->e_angle = en[1];
+    e->e_dihedral = en[2];
+    e->e_improper = en[3];
+}
+
+/* ------------------------------------------------------------------ */
 /* Forcefield::apply, nonbonded part (forcefield.rs:226-1060)          */
 /* ------------------------------------------------------------------ */
 struct or_ff {
@@ -939,6 +1157,8 @@ struct or_ff {
     int cached;
     or_storage nb, lr;
     int64_t last_pairs;
+    double *bf;                  /* covalent forces of the last apply */
+    or_bonded_e be;
 };
 
 or_ff *or_ff_create(const or_topo *t, const or_pbc *pbc, const or_ff_params *prm, const or_pert *pt) {
@@ -954,6 +1174,7 @@ or_ff *or_ff_create(const or_topo *t, const or_pbc *pbc, const or_ff_params *prm
     }
     or_storage_init(&ff->nb, t->n_atoms);
     or_storage_init(&ff->lr, t->n_atoms);
+    if (t->bonded) ff->bf = (double *)calloc(3 * (size_t)(t->n_atoms > 0 ? t->n_atoms : 1), sizeof(double));
     /* use_cg_grouped_solute (forcefield.rs:271-283) */
     int cg_avail = t->n_cg > 0;
     if (pbc->kind == 0) ff->use_cg_grouped = cg_avail;
@@ -973,6 +1194,7 @@ void or_ff_destroy(or_ff *ff) {
     or_storage_free(&ff->nb);
     or_storage_free(&ff->lr);
     free(ff->groups);
+    free(ff->bf);
     free(ff);
 }
 
@@ -986,6 +1208,7 @@ static void ff_update_pairlist(or_ff *ff, const double *pos) {
 
 void or_ff_initial_pairlist(or_ff *ff, const double *pos) { ff_update_pairlist(ff, pos); }
 const or_pairlist *or_ff_pairlist(const or_ff *ff) { return &ff->pl; }
+void or_ff_last_bonded(const or_ff *ff, or_bonded_e *e) { *e = ff->be; }
 int or_ff_used_cg_grouped(const or_ff *ff) { return ff->use_cg_grouped; }
 int64_t or_ff_last_pair_count(const or_ff *ff) { return ff->last_pairs; }
 
@@ -1132,10 +1355,21 @@ int or_ff_apply(or_ff *ff, const double *pos, double *forces, double *e_lj, doub
         }
         free(c.f);
     }
-    if (forces) memcpy(forces, nb->f, sizeof(double) * 3 * (size_t)n);
+    /* assembly (forcefield.rs:953-1000): force = bonded + nonbonded, virial nonbonded + bonded */
+    if (t->bonded) {
+        double bvir[9];
+        or_bonded_forces(t->bonded, n, pos, ff->bf, bvir, &ff->be);
+        if (forces)
+            for (int64_t k = 0; k < 3 * (int64_t)n; ++k) forces[k] = ff->bf[k] + nb->f[k];
+        if (vir)
+            for (int c = 0; c < 9; ++c) vir[c] = nb->vir[c] + bvir[c];
+    } else {
+        memset(&ff->be, 0, sizeof ff->be);
+        if (forces) memcpy(forces, nb->f, sizeof(double) * 3 * (size_t)n);
+        if (vir) memcpy(vir, nb->vir, sizeof(double) * 9);
+    }
     if (e_lj) *e_lj = nb->e_lj;
     if (e_crf) *e_crf = nb->e_crf;
-    if (vir) memcpy(vir, nb->vir, sizeof(double) * 9);
     if (dhdl) *dhdl = dh;
     ff->last_pairs = npairs;
     return 0;
@@ -1367,7 +1601,11 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
         if (energies) {
             energies[5 * s + 0] = elj;
             energies[5 * s + 1] = ecrf;
-            energies[5 * s + 2] = elj + ecrf;
+            /* update_potential_total (configuration.rs:197-207): bond_total holds every
+             * covalent term; the other covalent totals stay zero */
+            const or_bonded_e *be = &md->ff->be;
+            double eb = ((be->e_bond + be->e_angle) + be->e_dihedral) + be->e_improper;
+            energies[5 * s + 2] = (eb + elj) + ecrf;
             energies[5 * s + 3] = ek;
             energies[5 * s + 4] = dh;
         }

@@ -57,6 +57,25 @@ void or_crf_new(double cutoff, double epsilon, double rf_epsilon, double rf_kapp
 void or_lj_crf_interaction(const double r[3], double c6, double c12, double q_prod,
                            const or_crf *crf, double *f, double *e_lj, double *e_crf);
 
+/* ---- covalent terms of the solute (crates/gromos-forces/src/bonded) ---- */
+typedef struct {
+    int32_t nb;  const int32_t *bonds;     const double *bond_types;      /* i j t; kq kh r0 */
+    int32_t na;  const int32_t *angles;    const double *angle_types;     /* i j k t; kc kh theta0 */
+    int32_t ni;  const int32_t *impropers; const double *improper_types;  /* i j k l t; k q0 */
+    int32_t nd;  const int32_t *dihedrals; const double *dihedral_types;  /* i j k l t; k cospd m */
+    int32_t nbt, nat, nit, ndt;            /* type table sizes (terms of a larger type are skipped) */
+    int32_t ntf[4];                        /* FORCE block NTF(1..4): bond, angle, improper, dihedral on */
+} or_bonded;
+
+/* e: bond, angle, dihedral, improper energy */
+typedef struct { double e_bond, e_angle, e_dihedral, e_improper; } or_bonded_e;
+
+/* calculate_bonded_forces_ntf with quartic bonds (bonded/mod.rs:141-185): each
+ * kind into its own zeroed buffer, summed per atom in the order bonds, angles,
+ * dihedrals, impropers; f (3n) and vir (9, row a col b) are ASSIGNED */
+void or_bonded_forces(const or_bonded *b, int n_atoms, const double *pos, double *f, double vir[9],
+                      or_bonded_e *e);
+
 /* ---- topology essentials ---- */
 typedef struct {
     int32_t n_atoms;
@@ -74,6 +93,7 @@ typedef struct {
     const int32_t *o14_off;      /* n_atoms + 1; per-atom 1-4 partners as read (INE14) */
     const int32_t *o14;
     int32_t atoms_per_solvent;
+    const or_bonded *bonded;     /* NULL: no covalent terms */
 } or_topo;
 
 /* ---- growable u32 pair list ---- */
@@ -224,6 +244,8 @@ void or_ff_initial_pairlist(or_ff *ff, const double *pos);
 int or_ff_apply(or_ff *ff, const double *pos, double *forces, double *e_lj, double *e_crf,
                 double *vir, double *dhdl);
 const or_pairlist *or_ff_pairlist(const or_ff *ff);
+/* the covalent energies of the last or_ff_apply (zero without covalent terms) */
+void or_ff_last_bonded(const or_ff *ff, or_bonded_e *e);
 int or_ff_used_cg_grouped(const or_ff *ff);
 /* number of pair interactions evaluated by the last apply (for throughput accounting) */
 int64_t or_ff_last_pair_count(const or_ff *ff);

@@ -46,11 +46,22 @@ class Crf(C.Structure):
     _fields_ = [("crf_cut", D), ("crf_2cut3i", D), ("crf_cut3i", D), ("cutoff_sq", D)]
 
 
+class Bonded(C.Structure):
+    _fields_ = [("nb", I32), ("bonds", P), ("bond_types", P), ("na", I32), ("angles", P), ("angle_types", P),
+                ("ni", I32), ("impropers", P), ("improper_types", P), ("nd", I32), ("dihedrals", P),
+                ("dihedral_types", P), ("nbt", I32), ("nat", I32), ("nit", I32), ("ndt", I32),
+                ("ntf", I32 * 4)]
+
+
+class BondedE(C.Structure):
+    _fields_ = [("e_bond", D), ("e_angle", D), ("e_dihedral", D), ("e_improper", D)]
+
+
 class Topo(C.Structure):
     _fields_ = [("n_atoms", I32), ("n_solute_atoms", I32), ("n_cg", I32), ("n_solute_cg", I32),
                 ("cg_start", P), ("charge", P), ("iac", P), ("mass", P), ("n_types", I32),
                 ("lj", P), ("excl_off", P), ("excl", P), ("o14_off", P), ("o14", P),
-                ("atoms_per_solvent", I32)]
+                ("atoms_per_solvent", I32), ("bonded", C.POINTER(Bonded))]
 
 
 class PList(C.Structure):
@@ -130,6 +141,8 @@ def _declare(L):
     L.or_ff_initial_pairlist.argtypes = [P, P]
     L.or_ff_apply.argtypes = [P, P, P, P, P, P, P]
     L.or_ff_pairlist.argtypes = [P]
+    L.or_ff_last_bonded.argtypes = [P, C.POINTER(BondedE)]
+    L.or_bonded_forces.argtypes = [C.POINTER(Bonded), C.c_int, P, P, P, C.POINTER(BondedE)]
     L.or_ff_pairlist.restype = C.POINTER(PairList)
     L.or_ff_used_cg_grouped.argtypes = [P]
     L.or_ff_last_pair_count.argtypes = [P]
@@ -179,7 +192,11 @@ class OracleSystem:
         self.topo = Topo(s.n_atoms, s.n_solute_atoms, len(s.cg_start) - 1, s.n_solute_cg,
                          ptr(k["cg_start"]), ptr(k["charge"]), ptr(k["iac"]), ptr(k["mass"]),
                          s.n_types, ptr(k["lj"]), ptr(k["excl_off"]), ptr(k["excl"]),
-                         ptr(k["o14_off"]), ptr(k["o14"]), s.atoms_per_solvent)
+                         ptr(k["o14_off"]), ptr(k["o14"]), s.atoms_per_solvent, None)
+        self.n = s.n_atoms
+        self.bonded = _bonded_struct(getattr(s, "bonded", None), k)
+        if self.bonded is not None:
+            self.topo.bonded = C.pointer(self.bonded)
         self.pbc = make_pbc(s.boundary, s.box_values())
         self.pert = None
         if s.pert is not None:
@@ -387,8 +404,48 @@ def pert_atom_pair_correction(osys, lp, pos=None):
     return _pass("or_perturbed_atom_pair_correction", osys, lp, pos)
 
 
+def _bonded_struct(b, keep):
+    """or_bonded over a system.Bonded (arrays kept in the dict keep); None
+    without terms"""
+    if b is None or b.n_terms == 0:
+        return None
+    for nm in ("bonds", "angles", "impropers", "dihedrals"):
+        keep[nm] = c32(getattr(b, nm))
+    for nm in ("bond_types", "angle_types", "improper_types", "dihedral_types"):
+        keep[nm] = c64(getattr(b, nm))
+    ntf = (I32 * 4)(*[int(x) for x in b.ntf])
+    return Bonded(len(b.bonds), ptr(keep["bonds"]), ptr(keep["bond_types"]),
+                  len(b.angles), ptr(keep["angles"]), ptr(keep["angle_types"]),
+                  len(b.impropers), ptr(keep["impropers"]), ptr(keep["improper_types"]),
+                  len(b.dihedrals), ptr(keep["dihedrals"]), ptr(keep["dihedral_types"]),
+                  len(b.bond_types), len(b.angle_types), len(b.improper_types), len(b.dihedral_types), ntf)
+
+
+class BondedOnly:
+    """Covalent terms without a system (positions x give the atom count)."""
+
+    def __init__(self, x, bonded):
+        self._keep = {}
+        self.n = len(x)
+        self.bonded = _bonded_struct(bonded, self._keep)
+
+
+def bonded_forces(osys, pos=None):
+    """calculate_bonded_forces_ntf (bonded/mod.rs:141-185) alone: forces, virial
+    and the four covalent energies (osys: OracleSystem or BondedOnly)."""
+    n = osys.n
+    x = c64(osys.s.pos if pos is None else pos)
+    f, vir, be = np.zeros((n, 3)), np.zeros(9), BondedE()
+    if osys.bonded is None:
+        return dict(forces=f, virial=vir.reshape(3, 3), e_bond=0.0, e_angle=0.0, e_dihedral=0.0, e_improper=0.0)
+    lib().or_bonded_forces(C.byref(osys.bonded), n, ptr(x), ptr(f), ptr(vir), C.byref(be))
+    return dict(forces=f, virial=vir.reshape(3, 3), e_bond=be.e_bond, e_angle=be.e_angle,
+                e_dihedral=be.e_dihedral, e_improper=be.e_improper)
+
+
 class ForceField:
-    """Forcefield::apply restated (nonbonded only)."""
+    """Forcefield::apply restated: nonbonded, plus the covalent terms when the
+    system has them."""
 
     def __init__(self, osys, parallel=False, virial=True):
         self.osys = osys
@@ -408,8 +465,11 @@ class ForceField:
         vir = np.zeros(9)
         lib().or_ff_apply(self.h, ptr(x), ptr(f), C.byref(vals[0]), C.byref(vals[1]), ptr(vir),
                           C.byref(vals[2]))
+        be = BondedE()
+        lib().or_ff_last_bonded(self.h, C.byref(be))
         return dict(forces=f, e_lj=vals[0].value, e_crf=vals[1].value, virial=vir.reshape(3, 3),
-                    dhdl=vals[2].value, pairs=lib().or_ff_last_pair_count(self.h))
+                    dhdl=vals[2].value, pairs=lib().or_ff_last_pair_count(self.h),
+                    e_bond=be.e_bond, e_angle=be.e_angle, e_dihedral=be.e_dihedral, e_improper=be.e_improper)
 
     def pairlist(self):
         pl = lib().or_ff_pairlist(self.h).contents

@@ -48,6 +48,9 @@ SYSTEMS = [
     ("aladip_solvated", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
     ("water_216_box", None, None, None, None, "first"),
     ("aladip_trunc_oct", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
+    ("aladip_vacuum", "../shared/aladip.topo", None, None, None, "all"),
+    ("butane_vacuum", None, None, None, None, "all"),
+    ("benzene_vacuum", None, None, None, None, "all"),
     ("water_1000_spc_gridcell", None, None, None, None, "first"),
     ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
      "../shared/ch4_spc_dummy.ptp", "first"),
@@ -113,6 +116,40 @@ def parse_topology(path):
             p = l.split()
             angles.append((int(p[0]) - 1, int(p[1]) - 1, int(p[2]) - 1))
     t["bonds"], t["angles"] = bonds, angles
+    # bonded terms with their types (1-based type in the file), H blocks first
+    # (topology.rs:157-180); type tables converted as the reference stores them
+    # (topology.rs:499-513 theta0 in radians, :661-664 cos(PD), :699-703 Q0 in radians)
+    import math
+
+    def entries(keys, natoms):
+        out = []
+        for key in keys:
+            tok = [x for l in data_lines(b.get(key, [])) for x in l.split()]
+            if not tok:
+                continue
+            n = int(tok[0])
+            for k in range(n):
+                e = tok[1 + k * (natoms + 1): 1 + (k + 1) * (natoms + 1)]
+                out.append([int(x) - 1 for x in e])
+        return out
+
+    def table(key, ncol):
+        d = data_lines(b.get(key, []))
+        if not d:
+            return []
+        return [[float(x) for x in l.split()[:ncol]] for l in d[1:1 + int(d[0].split()[0])]]
+
+    t["bond_terms"] = entries(("BONDH", "BOND"), 2)
+    t["angle_terms"] = entries(("BONDANGLEH", "BONDANGLE"), 3)
+    t["improper_terms"] = entries(("IMPDIHEDRALH", "IMPDIHEDRAL"), 4)
+    t["dihedral_terms"] = entries(("DIHEDRALH", "DIHEDRAL"), 4)
+    t["bond_types"] = [[cb, chb, b0] for cb, chb, b0 in table("BONDSTRETCHTYPE", 3)]
+    t["angle_types"] = [[ct, cht, t0 * (math.pi / 180.0)] for ct, cht, t0 in table("BONDANGLEBENDTYPE", 3)]
+    # topology.rs:702-705: CQ kJ/(mol deg^2) -> kJ/(mol rad^2), Q0 deg -> rad
+    t["improper_types"] = [[cq * 180.0 * 180.0 / (math.pi * math.pi), q0 * math.pi / 180.0]
+                           for cq, q0 in table("IMPDIHEDRALTYPE", 2)]
+    t["dihedral_types"] = [[cp, math.cos(pd * math.pi / 180.0), int(np_)]
+                           for cp, pd, np_ in table("TORSDIHEDRALTYPE", 3)]
     lj = {}
     key = "LJPARAMETERS" if "LJPARAMETERS" in b else "CGPARAMETERS"
     for l in data_lines(b.get(key, []))[1:]:
@@ -173,6 +210,8 @@ def parse_imd(path):
     nb = data_lines(b["NONBONDED"])
     p = nb[1].split()
     d["appak"], d["rcrf"], d["epsrf"] = float(p[0]), float(p[1]), float(p[2])
+    frc = data_lines(b.get("FORCE", []))
+    d["ntf"] = [int(x) for x in frc[0].split()[:4]] if frc else [1, 1, 1, 1]
     cons = data_lines(b.get("CONSTRAINT", []))
     d["ntcs"] = cons[3].split()[0] if len(cons) > 3 else "shake"
     if "PERTURBATION" in b:
@@ -225,7 +264,8 @@ def energies(path):
                 break
             if s and not s.startswith("#"):
                 vals.append(float(s.split()[0]))
-        frames.append(dict(step=step, e_tot=vals[0], e_kin=vals[1], e_pot=vals[2],
+        frames.append(dict(step=step, e_tot=vals[0], e_kin=vals[1], e_pot=vals[2], e_covalent=vals[3],
+                           e_bond=vals[4], e_angle=vals[5], e_improper=vals[6], e_dihedral=vals[7],
                            e_nonbonded=vals[9], e_lj=vals[10], e_crf=vals[11]))
     return frames
 
@@ -328,6 +368,9 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"],
         solvent_constr=t["solvent_constr"], pert=pert,
         has_bonded_terms=bool(t["bonds"] or t["angles"]),
+        bonded=dict(bonds=t["bond_terms"], angles=t["angle_terms"], impropers=t["improper_terms"],
+                    dihedrals=t["dihedral_terms"], bond_types=t["bond_types"], angle_types=t["angle_types"],
+                    improper_types=t["improper_types"], dihedral_types=t["dihedral_types"], ntf=imd["ntf"]),
     )
     ex_dir = os.path.join(d, "expected")
     fx["expected_energies"] = energies(os.path.join(ex_dir, "energies.tre"))
@@ -379,6 +422,9 @@ INPUT_SETS = {
     "aladip_trunc_oct": ["../shared/aladip.topo", "../shared/aladip.conf", "aladip_trunc_oct.in"],
     "ch4_water_fep": ["../shared/ch4_spc.top", "../shared/ch4_spc.cnf", "ch4_water_fep.in",
                       "../shared/ch4_spc_dummy.ptp"],
+    "aladip_vacuum": ["../shared/aladip.topo", "aladip_vacuum.conf", "aladip_vacuum.in"],
+    "butane_vacuum": ["butane_vacuum.topo", "butane_vacuum.conf", "butane_vacuum.in"],
+    "benzene_vacuum": ["benzene_vacuum.topo", "benzene_vacuum.conf", "benzene_vacuum.in"],
 }
 
 

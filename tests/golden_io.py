@@ -20,6 +20,15 @@ def load_json(name):
         return json.load(fh)
 
 
+def _bonded(d):
+    b = d.get("bonded")
+    if not b or not (b["bonds"] or b["angles"] or b["impropers"] or b["dihedrals"]):
+        return None
+    return gamd.system.Bonded.of(b["bonds"], b["bond_types"], b["angles"], b["angle_types"], b["impropers"],
+                                 b["improper_types"], b["dihedrals"], b["dihedral_types"],
+                                 b.get("ntf", (1, 1, 1, 1)))
+
+
 def load(name):
     d = load_json(name)
     n = d["n_atoms"]
@@ -55,5 +64,6 @@ def load(name):
         rcutl=d["rcutl"], nsnb=d["nsnb"], algorithm=d["algorithm"], epsrf=d["epsrf"],
         kappa=d["appak"], fpepsi=d["fpepsi"], dt=d["dt"], dist_oh=d_oh, dist_hh=d_hh, pert=pert,
         name=name,
-        box_matrix=(np.array(d["box_matrix"], float).reshape(3, 3).T if d.get("box_matrix") else None))
+        box_matrix=(np.array(d["box_matrix"], float).reshape(3, 3).T if d.get("box_matrix") else None),
+        bonded=_bonded(d))
     return s.validate(), d

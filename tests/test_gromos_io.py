@@ -41,6 +41,12 @@ def test_reader_builds_the_fixture_system(name):
         assert getattr(s, k) == getattr(g, k), k
     if g.box_matrix is not None:
         assert np.array_equal(s.box_matrix, g.box_matrix)
+    assert (s.bonded is None) == (g.bonded is None)
+    if g.bonded is not None:
+        for k in ("bonds", "bond_types", "angles", "angle_types", "impropers", "improper_types", "dihedrals",
+                  "dihedral_types"):
+            assert np.array_equal(getattr(s.bonded, k), getattr(g.bonded, k)), k
+        assert tuple(s.bonded.ntf) == tuple(g.bonded.ntf)
     if g.pert is not None:
         for k in ("flag", "a_iac", "b_iac", "a_q", "b_q", "alpha_lj", "alpha_crf"):
             assert np.array_equal(getattr(s.pert, k), getattr(g.pert, k)), k
@@ -54,7 +60,7 @@ def test_system_from_files_reproduces_gromosxx_energies(name):
     _, d = golden_io.load(name)
     r = oracle.single_point(oracle.OracleSystem(s))
     e0 = d["expected_energies"][0]
-    for k in ("e_lj", "e_crf"):
+    for k in ("e_lj", "e_crf", "e_bond", "e_angle", "e_improper", "e_dihedral"):
         if e0[k] == 0.0:
             assert abs(r[k]) < 1e-12
         else:
@@ -70,6 +76,13 @@ def test_topology_bonded_blocks():
     assert all(0 <= b[2] < nb for b in t.bonds) and all(0 <= a[3] < na for a in t.angles)
     assert all(0 <= i < t.n_solute for b in t.bonds for i in b[:2])
     assert all(len(x) == 3 for x in t.bond_types) and all(x[2] > 0 for x in t.bond_types)
+
+
+def test_imd_force_switches():
+    _, _, imd, _ = files("benzene_vacuum")
+    assert gio.read_imd(imd).ntf == (0, 0, 0, 0)
+    _, _, imd, _ = files("aladip_vacuum")
+    assert gio.read_imd(imd).ntf == (1, 1, 1, 1)
 
 
 def test_imd_fields():

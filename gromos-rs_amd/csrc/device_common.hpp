@@ -33,6 +33,7 @@ struct Box {
     int32_t kind;
     int32_t pad;
     double a[3], b[3], c[3];
+    double tr[3];   // triclinic: c[1] / c[2], b[0] / b[1], c[0] / c[2]
 };
 
 // Triclinic::nearest_image (math.rs:105-141): z, then y, then x reduction loops
@@ -73,6 +74,30 @@ __device__ __forceinline__ d3 nim_tri(const Box& B, d3 n) {
     return n;
 }
 
+// The same reduction in closed form for the force kernels: the number of
+// lattice vectors each loop of nim_tri takes is the rounded ratio of the
+// reduced coordinate to the diagonal entry (a[1] = a[2] = b[2] = 0, so the z, y
+// and x steps are independent in that order).  In exact arithmetic the vector
+// is nim_tri's (ties at exact halves aside); in floating point it differs from
+// the loops' repeated subtraction by an ulp or so, with far fewer registers
+// and no data-dependent loops.  The pair search keeps the loops (its pair set
+// is the reference's to the bit).
+__device__ __forceinline__ d3 nim_tri_fast(const Box& B, d3 n) {
+    const double kz = rint(n.z * B.invL[2]);
+    n.x -= kz * B.c[0];
+    n.y -= kz * B.c[1];
+    n.z -= kz * B.c[2];
+    const double ky = rint((n.y - B.tr[0] * n.z) * B.invL[1]);
+    n.x -= ky * B.b[0];
+    n.y -= ky * B.b[1];
+    n.z -= ky * B.b[2];
+    const double kx = rint((n.x - B.tr[1] * (n.y - B.tr[0] * n.z) - B.tr[2] * n.z) * B.invL[0]);
+    n.x -= kx * B.a[0];
+    n.y -= kx * B.a[1];
+    n.z -= kx * B.a[2];
+    return n;
+}
+
 __device__ __forceinline__ d3 nim(const Box& b, d3 ri, d3 rj) {
     d3 r{ri.x - rj.x, ri.y - rj.y, ri.z - rj.z};
     if (b.kind == 1) {
@@ -80,9 +105,15 @@ __device__ __forceinline__ d3 nim(const Box& b, d3 ri, d3 rj) {
         r.y = r.y - b.L[1] * round(r.y * b.invL[1]);
         r.z = r.z - b.L[2] * round(r.z * b.invL[2]);
     } else if (b.kind == 2) {
-        r = nim_tri(b, r);
+        r = nim_tri_fast(b, r);
     }
     return r;
+}
+
+// nearest image with the reference's triclinic loops (the pair search)
+__device__ __forceinline__ d3 nim_exact(const Box& b, d3 ri, d3 rj) {
+    if (b.kind == 2) return nim_tri(b, d3{ri.x - rj.x, ri.y - rj.y, ri.z - rj.z});
+    return nim(b, ri, rj);
 }
 
 struct Crf {

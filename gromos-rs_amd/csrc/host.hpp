@@ -229,6 +229,8 @@ struct gromos_amd_system {
     double *d_bd_tf = nullptr, *d_bd_tev = nullptr, *d_bd_fb = nullptr;
     int32_t *d_bd_aoff = nullptr, *d_bd_aref = nullptr;
     int32_t nbd = 0;
+    int32_t* d_vv_near = nullptr;   // mixed: near partners at the front of each solvent list
+    double near_r = gamd::VV_NEAR_R;   // their radius (nm)
     int32_t last_rec_row = -1;   // records row of the last force evaluation
     char* d_temp = nullptr;
     size_t temp_bytes = 0;

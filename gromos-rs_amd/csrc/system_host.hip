@@ -28,6 +28,10 @@ Box make_box(int32_t boundary, const double* box) {
         b.L[0] = b.a[0];
         b.L[1] = b.b[1];
         b.L[2] = b.c[2];
+        for (int a = 0; a < 3; ++a) b.invL[a] = 1.0 / b.L[a];
+        b.tr[0] = b.c[1] / b.c[2];
+        b.tr[1] = b.b[0] / b.b[1];
+        b.tr[2] = b.c[0] / b.c[2];
         return b;
     }
     for (int a = 0; a < 3; ++a) {
@@ -470,6 +474,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->do_virial = do_virial;
     S->twin = twin;
     S->any_thermo = any_thermo;
+    S->near_r = VV_NEAR_R;
+    if (const char* env = std::getenv("GROMOS_AMD_VV_NEAR_R")) S->near_r = std::max(0.0, std::atof(env));
     // LJ only between the solvent molecules' first atoms (SPC-like waters)
     S->vv_lj00 = 1;
     for (const JobDev& J : S->jobs)
@@ -663,6 +669,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
+        (rc = S->reserve(S->d_vv_near, nc, true)) ||
         (rc = S->reserve(S->d_step_done, 1)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
@@ -684,6 +691,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipMemsetAsync(S->d_xq, 0, npa * sizeof(double4), st));
     HIPCHK(hipMemsetAsync(S->d_xf, 0, npa * sizeof(float4), st));
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_vv_near, 0, nc * sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step_done, 0, sizeof(uint32_t), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -776,6 +784,7 @@ static int refill_exact(gromos_amd_system* S) {
     // keeps the counts it read)
     HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
     launch_nbl(p, ncg, S->ncell, true, st);
+    if (S->precision == GROMOS_AMD_MIXED) launch_vv_near(p, ncg, S->near_r, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -824,6 +833,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
         launch_nbl(p, ncg, S->ncell, true, st);
+        if (S->precision == GROMOS_AMD_MIXED) launch_vv_near(p, ncg, S->near_r, st);
         if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
         HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(S->ctl_ev, st));
@@ -1302,6 +1312,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.pap_off = d_pap_off;
     p.pap = d_pap;
     p.o2s = d_o2s;
+    p.vv_near = d_vv_near;
     p.bd = gamd::BondedDev{};
     if (nbd > 0) {
         p.bd.atoms = d_bd_atoms;

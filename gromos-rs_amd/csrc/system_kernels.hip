@@ -157,7 +157,7 @@ __global__ void k_sorted_atoms(SysPtrs p, int32_t ncg, int32_t mixed) {
 
 // canonical squared distance of two reference points: nearest_image(lower, higher)
 __device__ __forceinline__ double ref_d2(const Box& b, d3 lo, d3 hi) {
-    d3 r = nim(b, lo, hi);
+    d3 r = nim_exact(b, lo, hi);
     return r.x * r.x + r.y * r.y + r.z * r.z;
 }
 
@@ -560,7 +560,7 @@ __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6,
 __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, float& y, float& z) {
     double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     if (b.kind == 2) {
-        const d3 r = nim_tri(b, d3{dx, dy, dz});
+        const d3 r = nim_tri_fast(b, d3{dx, dy, dz});
         dx = r.x;
         dy = r.y;
         dz = r.z;
@@ -741,6 +741,10 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
 // exact arithmetic, and in floating point it drops the cancelling constant
 // terms.  Atom pairs closer than 1e-5 nm (never two molecules) keep zero 1/r
 // terms but, unlike the reference, their RF polynomial is not dropped.
+//
+// Near partners: a molecule's list starts with the partners whose first atoms
+// were closer than VV_NEAR_R at the list build (k_vv_near puts them first and
+// counts them); this kernel evaluates the rest, k_vv_nr the near ones in FP64.
 template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
 // four waves per SIMD (at most 128 vector registers; a few values spill to
 // scratch outside the pair loop): 8 % faster than three waves with no spills
@@ -756,13 +760,15 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     for (int i = 0; i < APS; ++i) fi[i][0] = fi[i][1] = fi[i][2] = 0;
 #pragma unroll
     for (int c = 0; c < 11; ++c) ei[c] = 0;
-    int32_t e0 = 0, e1 = 0, sk = 0;
+    int32_t e0 = 0, e1 = 0, sk = 0, nn = 0;
     if (valid) {
         e0 = (int32_t)p.vv_off[k];
         e1 = e0 + (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);   // entries written (a pending refill)
         sk = p.s_cg_start[k];
+        nn = min(p.vv_near[k], e1 - e0);
     }
-    if (e1 > e0) {
+    const int32_t ef = e0 + nn;   // the far partners follow the near ones
+    if (e1 > ef) {
         const JobDev& J = p.jobs[job];
         const Box box = J.box;
         const float c2 = (float)J.crf.crf_2cut3i, cc = (float)J.crf.crf_cut, c3 = (float)J.crf.crf_cut3i;
@@ -789,7 +795,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
             for (int i = 0; i < APS; ++i) kf[i] = -(double)da[i].w * J.fpepsi * J.crf.crf_cut3i;
         }
         d3 pk0 = xyz(p.xq[sk]);
-        const int32_t n = e1 - e0;
+        const int32_t n = e1 - ef;
         // Groups of VV_GROUP consecutive entries are summed in FP64 and then
         // added as fixed point: the result does not depend on how groups are
         // spread over the lanes, so every lane count gives the same bits.
@@ -804,7 +810,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
             float mcx = 0.f, mcy = 0.f, mcz = 0.f;   //       over canonical entries
             float els = 0.f, ecs = 0.f;              // canonical entries' energies
             const int32_t g1 = min(g0 + VV_GROUP, n);
-            for (int32_t e = e0 + g0; e < e0 + g1; ++e) {
+            for (int32_t e = ef + g0; e < ef + g1; ++e) {
                 uint32_t ent = p.vv_ent[e];
                 int32_t sm = (int32_t)(ent & ENT_IDX);
                 const float canon = (ent & ENT_CANON) ? 1.0f : 0.0f;
@@ -951,6 +957,118 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
         }
 #pragma unroll
         for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = (double)ei[c] * EX_INV;
+    }
+}
+
+// Near solvent partners of the mixed path (the first VV_NEAR_R neighbours listed
+// at the start of each list, k_vv_near) in FP64 with k_vv's per-atom-pair form,
+// after k_vv_mx on the same stream: FP32 rounding of the large, mutually
+// cancelling Coulomb terms of close molecule pairs is what limits the mixed
+// force error, and these pairs are ~3 % of a 1.4 nm list.  Each entry's sums
+// become fixed point on their own (so the bits do not depend on the lane
+// count); the group's first lane adds them to the forces and energies k_vv_mx
+// assigned.  A kernel of its own: inside k_vv_mx the FP64 registers spill.
+template <int APS, bool LJ00, bool VIR, int NCHM>
+__global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
+    int32_t k = 0, job = 0;
+    bool valid = false;
+    int chunk = 0;
+    if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
+    if (held(p, job)) return;
+    long long fi[APS][3];
+    long long ei[11];
+#pragma unroll
+    for (int i = 0; i < APS; ++i) fi[i][0] = fi[i][1] = fi[i][2] = 0;
+#pragma unroll
+    for (int c = 0; c < 11; ++c) ei[c] = 0;
+    int32_t e0 = 0, nn = 0, sk = 0;
+    if (valid) {
+        e0 = (int32_t)p.vv_off[k];
+        nn = min(p.vv_near[k], (int32_t)min(p.vv_cnt[k], p.vv_cap[k]));
+        sk = p.s_cg_start[k];
+    }
+    if (nn > 0) {
+        const JobDev& J = p.jobs[job];
+        const Box box = J.box;
+        const double c2 = J.crf.crf_2cut3i, cc = J.crf.crf_cut, c3 = J.crf.crf_cut3i;
+        d3 pa[APS];
+#pragma unroll
+        for (int i = 0; i < APS; ++i) pa[i] = xyz(p.xq[sk + i]);
+        for (int32_t e = e0 + chunk; e < e0 + nn; e += NCHM) {
+            const uint32_t ent = p.vv_ent[e];
+            const int32_t sm = (int32_t)(ent & ENT_IDX);
+            const bool canon = (ent & ENT_CANON) != 0;
+            d3 pb[APS];
+#pragma unroll
+            for (int j = 0; j < APS; ++j) pb[j] = xyz(p.xq[sm + j]);
+            const d3 rf = nim(box, pa[0], pb[0]);
+            const double tx = rf.x - pa[0].x + pb[0].x, ty = rf.y - pa[0].y + pb[0].y, tz = rf.z - pa[0].z + pb[0].z;
+            double en[11];
+#pragma unroll
+            for (int c = 0; c < 11; ++c) en[c] = 0.0;
+#pragma unroll
+            for (int i = 0; i < APS; ++i) {
+                const double xi = pa[i].x + tx, yi = pa[i].y + ty, zi = pa[i].z + tz;
+                double gxs = 0.0, gys = 0.0, gzs = 0.0;
+#pragma unroll
+                for (int j = 0; j < APS; ++j) {
+                    const double rx = xi - pb[j].x, ry = yi - pb[j].y, rz = zi - pb[j].z;
+                    const double r2 = rx * rx + ry * ry + rz * rz;
+                    const bool ok = !(r2 < 1e-10);
+                    // 1/r: the FP32 hardware estimate (1 ulp) and one Newton step in
+                    // FP64 (relative error ~1e-14)
+                    const double y0 = (double)__builtin_amdgcn_rsqf((float)r2);
+                    const double inv_r = y0 * (1.5 - (0.5 * r2 * y0) * y0);
+                    const double inv_r2 = inv_r * inv_r;
+                    const double qq = J.sv_qq[i * 4 + j];
+                    const double ecrf = qq * (inv_r - c2 * r2 - cc);
+                    double fm = qq * (inv_r * inv_r2 + c3);
+                    double elj = 0.0;
+                    if (has_lj<APS, LJ00>(i, j)) {
+                        const double c6 = J.sv_c6[i * 4 + j], c12 = J.sv_c12[i * 4 + j];
+                        const double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+                        elj = (c12 * inv_r6 - c6) * inv_r6;
+                        fm = (12.0 * c12 * inv_r6 - 6.0 * c6) * inv_r6 * inv_r2 + fm;
+                    }
+                    fm = ok ? fm : 0.0;
+                    const double gx = rx * fm, gy = ry * fm, gz = rz * fm;
+                    gxs += gx;
+                    gys += gy;
+                    gzs += gz;
+                    en[0] += ok ? elj : 0.0;
+                    en[1] += ok ? ecrf : 0.0;
+                    if (VIR) {
+                        en[2] += rx * gx; en[3] += rx * gy; en[4] += rx * gz;
+                        en[5] += ry * gx; en[6] += ry * gy; en[7] += ry * gz;
+                        en[8] += rz * gx; en[9] += rz * gy; en[10] += rz * gz;
+                    }
+                }
+                fi[i][0] += to_fx(gxs);
+                fi[i][1] += to_fx(gys);
+                fi[i][2] += to_fx(gzs);
+            }
+            if (canon) {
+#pragma unroll
+                for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += (long long)rint(en[c] * EX_SCALE);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < NCHM; m <<= 1) {
+#pragma unroll
+        for (int i = 0; i < APS; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fi[i][c] += __shfl_xor(fi[i][c], m, WAVE);
+#pragma unroll
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += __shfl_xor(ei[c], m, WAVE);
+    }
+    if (chunk == 0 && valid && nn > 0) {
+#pragma unroll
+        for (int i = 0; i < APS; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p.fi_s[3 * (sk + i) + c] += fi[i][c];
+#pragma unroll
+        for (int c = 0; c < (VIR ? 11 : 2); ++c) p.vv_e[11 * (int64_t)k + c] += (double)ei[c] * EX_INV;
     }
 }
 
@@ -2268,6 +2386,65 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 }
 // start / stop events (optional) are recorded by the dispatch itself
 // (hipExtLaunchKernelGGL), so timing adds no packets between kernels
+// Stable split of every solvent group's list: partners whose first atoms lie
+// within near_r of the group's first atom (nearest image, positions of the
+// build) first, then the rest.  A wave per group, the list held in registers
+// (NEAR_REG entries a lane, loads issued together); longer lists stay unsplit.
+constexpr int NEAR_REG = 16;
+__global__ void __launch_bounds__(BLOCK) k_vv_near(SysPtrs p, int32_t ncg, double near_r2) {
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int32_t k = blockIdx.x * (BLOCK / WAVE) + w;
+    if (k >= ncg) return;
+    const int32_t e0 = (int32_t)p.vv_off[k];
+    const int32_t n = (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);
+    if (n <= 0 || n > NEAR_REG * WAVE) {
+        if (lane == 0) p.vv_near[k] = 0;
+        return;
+    }
+    const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
+    const d3 a = xyz(p.xq[p.s_cg_start[k]]);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t ent[NEAR_REG];
+    unsigned long long mn[NEAR_REG], mf[NEAR_REG];
+#pragma unroll
+    for (int c = 0; c < NEAR_REG; ++c) {
+        const int32_t e = c * WAVE + lane;
+        ent[c] = e < n ? p.vv_ent[e0 + e] : 0u;
+    }
+    int32_t nnear = 0;
+#pragma unroll
+    for (int c = 0; c < NEAR_REG; ++c) {
+        const int32_t e = c * WAVE + lane;
+        bool nr = false;
+        if (e < n) {
+            const d3 r = nim(J.box, a, xyz(p.xq[ent[c] & ENT_IDX]));
+            nr = r.x * r.x + r.y * r.y + r.z * r.z < near_r2;
+        }
+        mn[c] = __ballot(nr);
+        mf[c] = __ballot(e < n && !nr);
+        nnear += __popcll(mn[c]);
+    }
+    int32_t nb = 0, fb = 0;
+#pragma unroll
+    for (int c = 0; c < NEAR_REG; ++c) {
+        const int32_t e = c * WAVE + lane;
+        if (e < n) {
+            const bool nr = (mn[c] >> lane) & 1ull;
+            const int32_t pos = nr ? nb + __popcll(mn[c] & below) : nnear + fb + __popcll(mf[c] & below);
+            p.vv_ent[e0 + pos] = ent[c];
+        }
+        nb += __popcll(mn[c]);
+        fb += __popcll(mf[c]);
+    }
+    if (lane == 0) p.vv_near[k] = nnear;
+}
+
+void launch_vv_near(const SysPtrs& p, int32_t ncg, double near_r, hipStream_t s) {
+    constexpr int32_t GPB = BLOCK / WAVE;
+    if (ncg > 0)
+        hipLaunchKernelGGL(k_vv_near, dim3((unsigned)((ncg + GPB - 1) / GPB)), dim3(BLOCK), 0, s, p, ncg, near_r * near_r);
+}
+
 template <int APS, bool LJ00, int L>
 static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
                         hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
@@ -2280,17 +2457,32 @@ static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t m
         if (e0 || e1) hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);       \
         else hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                               \
     } while (0)
+    // the mixed path: k_vv_mx, then k_vv_nr (4 lanes per molecule: its few
+    // entries); timing events bracket both (start on the first, stop on the second)
+    const int32_t nw4 = p.nvv_chunk * (VV_CHUNK / (WAVE / 4));
+    dim3 g4 = xcd_grid((nw4 + BLOCK / WAVE - 1) / (BLOCK / WAVE));
+#define GAMD_VV_MX(KER, NR)                                                                          \
+    do {                                                                                             \
+        if (e0 || e1) {                                                                              \
+            hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, nullptr, 0, p, nw);                 \
+            hipExtLaunchKernelGGL(NR, g4, dim3(BLOCK), 0, s, nullptr, e1, 0, p, nw4);                \
+        } else {                                                                                     \
+            hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                                    \
+            hipLaunchKernelGGL(NR, g4, dim3(BLOCK), 0, s, p, nw4);                                   \
+        }                                                                                            \
+    } while (0)
     if (mixed && neutral) {
-        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, true>));
-        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, true>));
+        if (virial) GAMD_VV_MX((k_vv_mx<APS, LJ00, true, L, true>), (k_vv_nr<APS, LJ00, true, 4>));
+        else GAMD_VV_MX((k_vv_mx<APS, LJ00, false, L, true>), (k_vv_nr<APS, LJ00, false, 4>));
     } else if (mixed) {
-        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, false>));
-        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, false>));
+        if (virial) GAMD_VV_MX((k_vv_mx<APS, LJ00, true, L, false>), (k_vv_nr<APS, LJ00, true, 4>));
+        else GAMD_VV_MX((k_vv_mx<APS, LJ00, false, L, false>), (k_vv_nr<APS, LJ00, false, 4>));
     } else {
         if (virial) GAMD_VV_LAUNCH((k_vv<APS, LJ00, true, L>));
         else GAMD_VV_LAUNCH((k_vv<APS, LJ00, false, L>));
     }
 #undef GAMD_VV_LAUNCH
+#undef GAMD_VV_MX
 }
 template <int APS, bool LJ00>
 static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,

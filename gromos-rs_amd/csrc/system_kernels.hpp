@@ -102,6 +102,7 @@ struct SysPtrs {
     const int32_t* pap_off;    // npad + 1: entries of each atom
     const int32_t* pap;        // entry index | bit 31 when the atom is the entry's i
     int32_t* o2s;              // original atom -> sorted atom
+    int32_t* vv_near;          // mixed: per sorted group, near partners at the start of its list
     BondedDev bd;              // covalent terms; bd.fb per solute atom (sol_atom order),
                                // nullptr when the batch has none
 };
@@ -124,8 +125,14 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 // solvent molecules: the mixed kernel sums the RF polynomial per molecule pair);
 // lanes: lanes per solvent molecule (4, 8 or 16)
 constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
+// mixed: k_vv_mx then k_vv_nr (the near partners in FP64); timing events, when
+// given, bracket both
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// mixed precision, after every list fill: each solvent group's partners whose
+// first atoms lie within near_r (nm) moved to the front of its list (stable)
+// and counted into vv_near
+void launch_vv_near(const SysPtrs& p, int32_t ncg, double near_r, hipStream_t s);
 // ngsol: solvent groups listed in gsol by the last pair search (mixed short pass);
 // part: GEN_ALL, or (mixed short pass) only the solvent or only the solute atoms
 constexpr int32_t GEN_ALL = 0, GEN_SOLVENT = 1, GEN_SOLUTE = 2;

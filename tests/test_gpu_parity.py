@@ -371,6 +371,29 @@ def test_mixed_fixture_forces(name):
     mixed_energy_ok(g, r)
 
 
+def test_mixed_fep_mix_jobs_meet_the_force_bar():
+    """Jobs of the 64-job FEP mix after 300 relaxation steps (the state the
+    mix's force error is quoted on): the mixed path's per-atom force error
+    against plain FP64 stays within 1e-5 (near solvent partners in FP64,
+    the rest FP32)."""
+    srv = importlib.import_module("gromos-rs_amd.server")
+    mix = gen.fep_mix()
+    pick = [0, 5, 17, 33, 48, 63]
+    systems = [gen.fep_job(mix[k]["n_atoms"], mix[k]["n_pert"], mix[k]["lam"], mix[k]["seed"]) for k in pick]
+    systems = srv.equilibrate(systems, [300.0] * len(systems))
+    worst = 0.0
+    for q in systems:
+        out = {}
+        for prec in (rt.FP64, rt.MIXED):
+            ds = rt.DeviceSystem(q, precision=prec)
+            ds.pairlist_update()
+            out[prec] = ds.forcefield_apply()
+            ds.close()
+        worst = max(worst, force_err(out[rt.MIXED]["forces"], out[rt.FP64]["forces"]))
+        mixed_energy_ok(out[rt.MIXED], out[rt.FP64])
+    assert worst <= 1e-5, worst
+
+
 def test_mixed_water32768_error_and_determinism():
     s = gen.water_32768()
     r = oracle.single_point(oracle.OracleSystem(s), parallel=True)

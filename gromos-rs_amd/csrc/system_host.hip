@@ -765,6 +765,12 @@ int check_list_size(int64_t vv, int64_t gen) {
     return 0;
 }
 
+// the search's near radius: mixed precision evaluates the near solvent
+// partners in FP64; the FP64 path lists every partner as near (one ascending run)
+static double near_r2(const gromos_amd_system* S) {
+    return S->precision == GROMOS_AMD_MIXED ? S->near_r * S->near_r : 1e300;
+}
+
 // lists with exact slots: the counts of the last search pass scanned into
 // offsets, buffers sized to the totals, a second fill pass
 static int refill_exact(gromos_amd_system* S) {
@@ -783,8 +789,7 @@ static int refill_exact(gromos_amd_system* S) {
     // the refill lists the solvent groups with general partners again (the host
     // keeps the counts it read)
     HIPCHK(hipMemsetAsync(S->d_counters + 4, 0, sizeof(uint64_t), st));
-    launch_nbl(p, ncg, S->ncell, true, st);
-    if (S->precision == GROMOS_AMD_MIXED) launch_vv_near(p, ncg, S->near_r, st);
+    launch_nbl(p, ncg, S->ncell, true, near_r2(S), st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -832,8 +837,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
         if (int rc = S->reserve(S->d_vv_ent, (size_t)bv, true)) return rc;
         if (int rc = S->reserve(S->d_gen_ent, (size_t)bg, true)) return rc;
         p = S->ptrs();
-        launch_nbl(p, ncg, S->ncell, true, st);
-        if (S->precision == GROMOS_AMD_MIXED) launch_vv_near(p, ncg, S->near_r, st);
+        launch_nbl(p, ncg, S->ncell, true, near_r2(S), st);
         if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
         HIPCHK(hipMemcpyAsync(S->h_ctl, S->d_ctl, nctl * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(S->ctl_ev, st));
@@ -844,7 +848,7 @@ int build_pairlist(gromos_amd_system* S, bool deferred) {
         return finish_pairlist(S, &refilled);
     }
     // first build: count pass, then exact slots
-    launch_nbl(p, ncg, S->ncell, false, st);
+    launch_nbl(p, ncg, S->ncell, false, near_r2(S), st);
     if (S->njobs > 1) launch_job_pairs(p, S->njobs, st);
     if (int rc = refill_exact(S)) return rc;
     build_done(S);

@@ -214,7 +214,7 @@ __device__ __forceinline__ d3 nim_search(const Box& b, d3 ri, d3 rj) {
 }
 
 template <bool FILL>
-__global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
+__global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell, double near_r2) {
     __shared__ double sx[NBL_PIECE], sy[NBL_PIECE], sz[NBL_PIECE];
     __shared__ int32_t sm[NBL_PIECE], scm[NBL_PIECE], sst[NBL_PIECE], snn[NBL_PIECE];
     __shared__ int32_t run_m0[NBL_RUNS], run_off[NBL_RUNS + 1];
@@ -305,7 +305,9 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
             vv_end = vv_base + (int32_t)p.vv_cap[k];
             gen_end = gen_base + (int32_t)p.gen_cap[k];
         }
-        int32_t vv_pos = vv_base, gen_pos = gen_base;
+        // solvent entries: near partners (d2 < near_r2) from the slot's start
+        // upwards, the others from its end downwards
+        int32_t vv_nn = 0, vv_nf = 0, gen_pos = gen_base;
         for (int32_t P = 0; P < total; P += NBL_PIECE) {
             const int32_t plen = min(NBL_PIECE, total - P);
             __syncthreads();   // the previous piece is consumed
@@ -352,11 +354,23 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
                     const bool is_gen = in && !is_vv;
                     const uint64_t bv = __ballot(is_vv), bg = __ballot(is_gen);
                     if (bv) {
-                        if (FILL) {
-                            const int32_t pv = bcast(vv_pos, t) + (int32_t)__popcll(bv & below);
-                            if (is_vv && pv < bcast(vv_end, t)) p.vv_ent[pv] = vv_ent | (canon ? ENT_CANON : 0u);
+                        const bool nr = is_vv && d2 < near_r2;
+                        const uint64_t bn = __ballot(nr), bf = bv & ~bn;
+                        if (FILL && is_vv) {
+                            const int32_t base = bcast(vv_base, t), cap = bcast(vv_end, t) - base;
+                            const uint32_t e = vv_ent | (canon ? ENT_CANON : 0u);
+                            if (nr) {
+                                const int32_t x = bcast(vv_nn, t) + (int32_t)__popcll(bn & below);
+                                if (x < cap) p.vv_ent[base + x] = e;
+                            } else {
+                                const int32_t x = bcast(vv_nf, t) + (int32_t)__popcll(bf & below);
+                                if (x < cap) p.vv_ent[base + cap - 1 - x] = e;
+                            }
                         }
-                        if (lane == t) vv_pos += (int32_t)__popcll(bv);
+                        if (lane == t) {
+                            vv_nn += (int32_t)__popcll(bn);
+                            vv_nf += (int32_t)__popcll(bf);
+                        }
                         pairs_short += (uint64_t)__popcll(bv & __ballot(canon)) * (uint64_t)(aps * aps);
                     }
                     if (bg) {
@@ -379,8 +393,9 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell) {
             }
         }
         if (own) {
-            const int32_t nvv = vv_pos - vv_base, ngen = gen_pos - gen_base;
+            const int32_t nvv = vv_nn + vv_nf, ngen = gen_pos - gen_base;
             p.vv_cnt[k] = nvv;
+            p.vv_near[k] = vv_nn;
             p.gen_cnt[k] = ngen;
             p.prev_vv[ck] = nvv;   // the next build's slot sizes (read by k_caps before its search)
             p.prev_gen[ck] = ngen;
@@ -742,9 +757,9 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
 // terms.  Atom pairs closer than 1e-5 nm (never two molecules) keep zero 1/r
 // terms but, unlike the reference, their RF polynomial is not dropped.
 //
-// Near partners: a molecule's list starts with the partners whose first atoms
-// were closer than VV_NEAR_R at the list build (k_vv_near puts them first and
-// counts them); this kernel evaluates the rest, k_vv_nr the near ones in FP64.
+// Near partners: the search writes the partners whose first atoms are closer
+// than VV_NEAR_R at the slot's start (upwards) and the others at its end
+// (downwards); this kernel evaluates the latter, k_vv_nr the near ones in FP64.
 template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
 // four waves per SIMD (at most 128 vector registers; a few values spill to
 // scratch outside the pair loop): 8 % faster than three waves with no spills
@@ -760,14 +775,17 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     for (int i = 0; i < APS; ++i) fi[i][0] = fi[i][1] = fi[i][2] = 0;
 #pragma unroll
     for (int c = 0; c < 11; ++c) ei[c] = 0;
-    int32_t e0 = 0, e1 = 0, sk = 0, nn = 0;
+    // the far partners: the last min(count - near, cap) entries of the slot (a
+    // pending refill may have dropped some)
+    int32_t e1 = 0, ef = 0, sk = 0;
+    int64_t cnt = 0;
     if (valid) {
-        e0 = (int32_t)p.vv_off[k];
-        e1 = e0 + (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);   // entries written (a pending refill)
+        const int64_t cap = p.vv_cap[k];
+        cnt = p.vv_cnt[k];
+        e1 = (int32_t)(p.vv_off[k] + cap);
+        ef = e1 - (int32_t)min(cnt - (int64_t)p.vv_near[k], cap);
         sk = p.s_cg_start[k];
-        nn = min(p.vv_near[k], e1 - e0);
     }
-    const int32_t ef = e0 + nn;   // the far partners follow the near ones
     if (e1 > ef) {
         const JobDev& J = p.jobs[job];
         const Box box = J.box;
@@ -945,7 +963,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     }
     if (chunk == 0 && valid) {
         int32_t nk = p.s_cg_n[k];
-        if (e1 > e0) {
+        if (cnt > 0) {
 #pragma unroll
             for (int i = 0; i < APS; ++i)
 #pragma unroll
@@ -960,8 +978,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
     }
 }
 
-// Near solvent partners of the mixed path (the first VV_NEAR_R neighbours listed
-// at the start of each list, k_vv_near) in FP64 with k_vv's per-atom-pair form,
+// Near solvent partners of the mixed path (the neighbours within VV_NEAR_R,
+// listed at the start of each slot by the search) in FP64 with k_vv's per-atom-pair form,
 // after k_vv_mx on the same stream: FP32 rounding of the large, mutually
 // cancelling Coulomb terms of close molecule pairs is what limits the mixed
 // force error, and these pairs are ~3 % of a 1.4 nm list.  Each entry's sums
@@ -984,7 +1002,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
     int32_t e0 = 0, nn = 0, sk = 0;
     if (valid) {
         e0 = (int32_t)p.vv_off[k];
-        nn = min(p.vv_near[k], (int32_t)min(p.vv_cnt[k], p.vv_cap[k]));
+        nn = (int32_t)min((int64_t)p.vv_near[k], p.vv_cap[k]);   // the slot's first entries
         sk = p.s_cg_start[k];
     }
     if (nn > 0) {
@@ -2328,8 +2346,10 @@ __global__ void k_materialize(SysPtrs p, int32_t ncg, unsigned long long* counts
                 if (!excluded_or_14(p, i, j)) emit(0, i, j);
             }
     }
-    for (int32_t e = (int32_t)p.vv_off[k]; e < (int32_t)(p.vv_off[k] + p.vv_cnt[k]); ++e) {
-        uint32_t ent = p.vv_ent[e];
+    // solvent entries: near ones at the slot's start, the others at its end
+    const int64_t vo = p.vv_off[k], vc = p.vv_cap[k], vn = p.vv_near[k];
+    for (int64_t x = 0; x < p.vv_cnt[k]; ++x) {
+        const uint32_t ent = p.vv_ent[x < vn ? vo + x : vo + vc - 1 - (x - vn)];
         if (!(ent & ENT_CANON)) continue;
         int32_t m = (int32_t)(ent & ENT_IDX);
         emit(2, p.s2o[sk], p.s2o[m]);   // solvent entries hold the partner's first sorted atom
@@ -2370,11 +2390,11 @@ void launch_caps(const SysPtrs& p, int32_t ncg, int32_t mode, hipStream_t s) {
     if (mode == 0 || mode == 2) hipLaunchKernelGGL(k_caps, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg, mode == 2 ? 1 : 0);
     else hipLaunchKernelGGL(k_caps_from_counts, grid_for(ncg), dim3(BLOCK), 0, s, p, ncg);
 }
-void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s) {
+void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, double near_r2, hipStream_t s) {
     // one block per cell
     dim3 g = xcd_grid(ncell);
-    if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncell);
-    else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncell);
+    if (fill) hipLaunchKernelGGL(k_nbl<true>, g, dim3(BLOCK), 0, s, p, ncell, near_r2);
+    else hipLaunchKernelGGL(k_nbl<false>, g, dim3(BLOCK), 0, s, p, ncell, near_r2);
     unsigned nb = (unsigned)std::min<int64_t>(256, (ncg + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_totals, dim3(nb), dim3(BLOCK), 0, s, p, ncg);
 }
@@ -2386,64 +2406,6 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 }
 // start / stop events (optional) are recorded by the dispatch itself
 // (hipExtLaunchKernelGGL), so timing adds no packets between kernels
-// Stable split of every solvent group's list: partners whose first atoms lie
-// within near_r of the group's first atom (nearest image, positions of the
-// build) first, then the rest.  A wave per group, the list held in registers
-// (NEAR_REG entries a lane, loads issued together); longer lists stay unsplit.
-constexpr int NEAR_REG = 16;
-__global__ void __launch_bounds__(BLOCK) k_vv_near(SysPtrs p, int32_t ncg, double near_r2) {
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-    const int32_t k = blockIdx.x * (BLOCK / WAVE) + w;
-    if (k >= ncg) return;
-    const int32_t e0 = (int32_t)p.vv_off[k];
-    const int32_t n = (int32_t)min(p.vv_cnt[k], p.vv_cap[k]);
-    if (n <= 0 || n > NEAR_REG * WAVE) {
-        if (lane == 0) p.vv_near[k] = 0;
-        return;
-    }
-    const JobDev& J = p.jobs[p.cg_job[p.s_cg_orig[k]]];
-    const d3 a = xyz(p.xq[p.s_cg_start[k]]);
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t ent[NEAR_REG];
-    unsigned long long mn[NEAR_REG], mf[NEAR_REG];
-#pragma unroll
-    for (int c = 0; c < NEAR_REG; ++c) {
-        const int32_t e = c * WAVE + lane;
-        ent[c] = e < n ? p.vv_ent[e0 + e] : 0u;
-    }
-    int32_t nnear = 0;
-#pragma unroll
-    for (int c = 0; c < NEAR_REG; ++c) {
-        const int32_t e = c * WAVE + lane;
-        bool nr = false;
-        if (e < n) {
-            const d3 r = nim(J.box, a, xyz(p.xq[ent[c] & ENT_IDX]));
-            nr = r.x * r.x + r.y * r.y + r.z * r.z < near_r2;
-        }
-        mn[c] = __ballot(nr);
-        mf[c] = __ballot(e < n && !nr);
-        nnear += __popcll(mn[c]);
-    }
-    int32_t nb = 0, fb = 0;
-#pragma unroll
-    for (int c = 0; c < NEAR_REG; ++c) {
-        const int32_t e = c * WAVE + lane;
-        if (e < n) {
-            const bool nr = (mn[c] >> lane) & 1ull;
-            const int32_t pos = nr ? nb + __popcll(mn[c] & below) : nnear + fb + __popcll(mf[c] & below);
-            p.vv_ent[e0 + pos] = ent[c];
-        }
-        nb += __popcll(mn[c]);
-        fb += __popcll(mf[c]);
-    }
-    if (lane == 0) p.vv_near[k] = nnear;
-}
-
-void launch_vv_near(const SysPtrs& p, int32_t ncg, double near_r, hipStream_t s) {
-    constexpr int32_t GPB = BLOCK / WAVE;
-    if (ncg > 0)
-        hipLaunchKernelGGL(k_vv_near, dim3((unsigned)((ncg + GPB - 1) / GPB)), dim3(BLOCK), 0, s, p, ncg, near_r * near_r);
-}
 
 template <int APS, bool LJ00, int L>
 static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,

@@ -102,7 +102,7 @@ struct SysPtrs {
     const int32_t* pap_off;    // npad + 1: entries of each atom
     const int32_t* pap;        // entry index | bit 31 when the atom is the entry's i
     int32_t* o2s;              // original atom -> sorted atom
-    int32_t* vv_near;          // mixed: per sorted group, near partners at the start of its list
+    int32_t* vv_near;          // per sorted group: solvent partners at the start of its slot (near ones)
     BondedDev bd;              // covalent terms; bd.fb per solute atom (sol_atom order),
                                // nullptr when the batch has none
 };
@@ -114,7 +114,9 @@ void launch_cg_ref(const SysPtrs& p, int32_t ncg, int32_t nctl, hipStream_t s);
 void launch_sort_cg(const SysPtrs& p, int32_t ncg, int32_t ncell, hipStream_t s);
 // mixed: also the FP32 offset copies xf
 void launch_sorted_atoms(const SysPtrs& p, int32_t ncg, int32_t mixed, hipStream_t s);
-void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, hipStream_t s);
+// near_r2: solvent partners closer than this (squared, nm^2) go to the start of
+// their group's slot, the others to its end (vv_near counts the former)
+void launch_nbl(const SysPtrs& p, int32_t ncg, int32_t ncell, bool fill, double near_r2, hipStream_t s);
 // per-job pair counts into job_pairs (cleared by the caller)
 void launch_job_pairs(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // mode 0: capacities from the previous counts (kept by k_nbl) with slack; 1:
@@ -129,10 +131,7 @@ constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
 // given, bracket both
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// mixed precision, after every list fill: each solvent group's partners whose
-// first atoms lie within near_r (nm) moved to the front of its list (stable)
-// and counted into vv_near
-void launch_vv_near(const SysPtrs& p, int32_t ncg, double near_r, hipStream_t s);
+
 // ngsol: solvent groups listed in gsol by the last pair search (mixed short pass);
 // part: GEN_ALL, or (mixed short pass) only the solvent or only the solute atoms
 constexpr int32_t GEN_ALL = 0, GEN_SOLVENT = 1, GEN_SOLUTE = 2;

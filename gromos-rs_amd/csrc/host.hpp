@@ -212,6 +212,7 @@ struct gromos_amd_system {
     int32_t vv_lanes = 0;  // lanes per solvent molecule in the pair kernel (0: not tuned yet)
     int64_t tuned_key = -1;
     int32_t* d_settle_fail = nullptr;
+    int32_t* d_force_fail = nullptr;
     double* d_therm_scale = nullptr;
     bool any_thermo = false;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
@@ -340,9 +341,11 @@ int build_pairlist(gromos_amd_system* S, bool deferred = false);
 int finish_pairlist(gromos_amd_system* S, bool* refilled);
 int forcefield(gromos_amd_system* S, bool assemble);
 // nsteps of the md sequence; records (nsteps x njobs x REC) copied to rec when
-// given.  A SETTLE failure is an error unless `failed` is given: it then gets
-// each job's first failing step (INT32_MAX: none).
-int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed = nullptr);
+// given.  A failing job (SETTLE, or a non-finite / out-of-range force) is an
+// error unless `failed` is given: it then gets each job's first failing step
+// (INT32_MAX: none), and `force` (if given) whether that was a force failure.
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed = nullptr,
+             std::vector<uint8_t>* force = nullptr);
 int collect_step_times(gromos_amd_system* S);
 
 // device helpers for the session

@@ -423,7 +423,8 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
                 if (int rc = upload_jobs(S)) return rc;
             int64_t pc0 = S->pair_count;
             std::vector<int32_t> failed;
-            if (int rc = md_steps(S, k, &rec, &failed)) return rc;
+            std::vector<uint8_t> fkind;
+            if (int rc = md_steps(S, k, &rec, &failed, &fkind)) return rc;
             s->pair_count += S->pair_count - pc0;
             if (s->step_timing) {
                 s->step_ms.insert(s->step_ms.end(), S->step_ms.begin(), S->step_ms.end());
@@ -436,8 +437,12 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
                     adv[b] = failed[b];
                     j->state = GROMOS_AMD_JOB_FAILED;
                     char msg[160];
-                    snprintf(msg, sizeof(msg), "SETTLE failed: sin(angle) > 1.0 in analytical solve (step %lld)",
-                             (long long)(j->done + failed[b]));
+                    if (fkind[b])
+                        snprintf(msg, sizeof(msg), "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm) (step %lld)",
+                                 FORCE_LIMIT, (long long)(j->done + failed[b]));
+                    else
+                        snprintf(msg, sizeof(msg), "SETTLE failed: sin(angle) > 1.0 in analytical solve (step %lld)",
+                                 (long long)(j->done + failed[b]));
                     j->error = msg;
                 }
                 for (int64_t st = 0; st < adv[b]; ++st) {

@@ -64,6 +64,10 @@ struct JobDev {
 constexpr int ROW = 12;
 enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 4, NSLOT = 5 };
 
+// a force component at or beyond this (kJ mol^-1 nm^-1), or a non-finite one,
+// marks its job as diverged (the fixed-point conversions saturate at twice it)
+constexpr double FORCE_LIMIT = 33554432.0;   // 2^25
+
 // per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9],
 // e_bond, e_angle, e_dihedral, e_improper
 constexpr int REC = 18;
@@ -75,12 +79,12 @@ constexpr int JOB_CTL = 3;
 // reduction); PART_W columns: e_lj, e_crf, virial[9], E_kin, E_kin_new
 constexpr int VV_CHUNK = 16;   // groups per chunk of the solvent pair kernels (one job each)
 // mixed precision: solvent molecule pairs whose first atoms are closer than
-// VV_NEAR_R (nm) at the list build are evaluated in FP64 (0.65 nm: the first
-// two hydration shells, ~10 % of a 1.4 nm list; the largest FP32 force errors
-// of the 64-job FEP mix fall from 2.3e-5 to 8.7e-6 of the atom's force for 8 %
-// of the step time).  GROMOS_AMD_VV_NEAR_R in the environment sets another
-// radius (0: all FP32).
-constexpr double VV_NEAR_R = 0.65;
+// VV_NEAR_R (nm) at the list build are evaluated in FP64 (0.6 nm: the first
+// two hydration shells, ~8 % of a 1.4 nm list; the largest FP32 force error
+// over the 506,566 atoms of the 64-job FEP mix falls from 2.3e-5 to 6.7e-6 of
+// the atom's force for ~6 % of the step time).  GROMOS_AMD_VV_NEAR_R in the
+// environment sets another radius (0: all FP32).
+constexpr double VV_NEAR_R = 0.6;
 constexpr int VV_PARTS = 128;
 constexpr int PART_W = 13;
 

@@ -665,6 +665,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_fi_long_s, n3, true)) ||
         (rc = S->reserve(S->d_rows, (size_t)NSLOT * S->nrow_blocks * ROW, true)) ||
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_force_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -1088,13 +1089,15 @@ static int launch_step_graph(gromos_amd_system* S) {
     return 0;
 }
 
-int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed_out) {
+int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std::vector<int32_t>* failed_out,
+             std::vector<uint8_t>* force_out) {
     if (nsteps <= 0) return 0;
     int64_t need = (int64_t)nsteps * S->njobs * REC;
     if (int rc = S->reserve(S->d_records, (size_t)need, true)) return rc;
     hipStream_t st = S->stream;
     std::vector<int32_t> big(S->njobs, INT32_MAX);
     HIPCHK(hipMemcpyAsync(S->d_settle_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S->d_force_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     size_t ev0 = S->step_ev.size();
     auto step_event = [&]() -> int {
         hipEvent_t e;
@@ -1149,8 +1152,9 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     S->md_step = 0;
     S->last_rec_row = nsteps - 1;
     HIPCHK(hipGetLastError());
-    std::vector<int32_t> failed(S->njobs);
+    std::vector<int32_t> failed(S->njobs), ffail(S->njobs);
     HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ffail.data(), S->d_force_fail, ffail.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (rec) {
         rec->resize((size_t)need);
         HIPCHK(hipMemcpyAsync(rec->data(), S->d_records, (size_t)need * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1160,13 +1164,26 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         if (int rc = collect_step_times(S)) return rc;
     }
     if (int rc = S->collect_timing()) return rc;
+    // a force failure at step s leaves that step's forces undefined: the job
+    // stops there like a SETTLE failure
+    std::vector<uint8_t> kind(S->njobs, 0);
+    for (int j = 0; j < S->njobs; ++j)
+        if (ffail[j] <= failed[j] && ffail[j] != INT32_MAX) {
+            failed[j] = ffail[j];
+            kind[j] = 1;
+        }
     if (failed_out) {
         *failed_out = failed;
+        if (force_out) *force_out = kind;
         return 0;
     }
     for (int j = 0; j < S->njobs; ++j)
-        if (failed[j] != INT32_MAX)
+        if (failed[j] != INT32_MAX) {
+            if (kind[j])
+                return fail(GROMOS_AMD_EFORCE, "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm; job %d, step %d)",
+                            FORCE_LIMIT, j, failed[j]);
             return fail(GROMOS_AMD_ESETTLE, "SETTLE failed: sin(angle) > 1.0 in analytical solve (job %d, step %d)", j, failed[j]);
+        }
     return 0;
 }
 
@@ -1303,6 +1320,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.vv_virial = do_virial;
     p.records = d_records;
     p.settle_fail = d_settle_fail;
+    p.force_fail = d_force_fail;
     p.therm_scale = d_therm_scale;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;

@@ -549,7 +549,14 @@ constexpr double EX_SCALE = 4294967296.0;        // 2^32 per kJ mol^-1 (energies
 constexpr double EX_INV = 1.0 / 4294967296.0;
 constexpr int VV_GROUP = 8;                      // entries per fixed-point group
 
-__device__ __forceinline__ long long to_fx(double v) { return (long long)rint(v * FX_SCALE); }
+// Conversions saturate (a NaN goes to the negative cap): a diverged job's
+// forces stay defined sums that k_integrate recognises (FORCE_BAD) and turns
+// into a failed job.  At most 2^12 conversions add up per value, so the int64
+// sums cannot wrap.
+constexpr double FX_CAP = 2.0 * FORCE_LIMIT * FX_SCALE;   // 2^50: |F| of 2^26 kJ mol^-1 nm^-1 per term
+constexpr double EX_CAP = 18014398509481984.0;            // 2^54: |E| of 2^22 kJ mol^-1 per term
+__device__ __forceinline__ long long to_fx(double v) { return (long long)rint(fmin(fmax(v * FX_SCALE, -FX_CAP), FX_CAP)); }
+__device__ __forceinline__ long long to_ex(double v) { return (long long)rint(fmin(fmax(v * EX_SCALE, -EX_CAP), EX_CAP)); }
 
 __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6, float c12, float qp, float c2,
                                          float cc, float c3, float& f, float& elj, float& ecrf) {
@@ -570,8 +577,7 @@ __device__ __forceinline__ void lj_crf32(float rx, float ry, float rz, float c6,
 }
 
 // nearest-image vector between two group first atoms in FP64, returned as
-// float; ties of the image rounding go to even (rint), which only differs from
-// the reference's half-away rounding at exactly L/2
+// float, with the reference's image choice (ties included)
 __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, float& y, float& z) {
     double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     if (b.kind == 2) {
@@ -580,9 +586,18 @@ __device__ __forceinline__ void shift32(const Box& b, d3 A, d3 B, float& x, floa
         dy = r.y;
         dz = r.z;
     } else if (b.kind == 1) {
-        dx = dx - b.L[0] * rint(dx * b.invL[0]);
-        dy = dy - b.L[1] * rint(dy * b.invL[1]);
-        dz = dz - b.L[2] * rint(dz * b.invL[2]);
+        // rint, and the reference's half-away rounding (f64::round) where the
+        // ratio is an exact half: a branch the wave skips unless a lane has one
+        const double ux = dx * b.invL[0], uy = dy * b.invL[1], uz = dz * b.invL[2];
+        double nx = rint(ux), ny = rint(uy), nz = rint(uz);
+        if (__builtin_expect(fabs(ux - nx) == 0.5 || fabs(uy - ny) == 0.5 || fabs(uz - nz) == 0.5, 0)) {
+            nx = round(ux);
+            ny = round(uy);
+            nz = round(uz);
+        }
+        dx = dx - b.L[0] * nx;
+        dy = dy - b.L[1] * ny;
+        dz = dz - b.L[2] * nz;
     }
     x = (float)dx;
     y = (float)dy;
@@ -949,7 +964,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
 #pragma unroll
                 for (int c = 0; c < 3; ++c) fi[i][c] += to_fx(f[i][c]);
 #pragma unroll
-            for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += (long long)rint(en[c] * EX_SCALE);
+            for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += to_ex(en[c]);
         }
     }
 #pragma unroll
@@ -982,10 +997,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
 // listed at the start of each slot by the search) in FP64 with k_vv's per-atom-pair form,
 // after k_vv_mx on the same stream: FP32 rounding of the large, mutually
 // cancelling Coulomb terms of close molecule pairs is what limits the mixed
-// force error, and these pairs are ~3 % of a 1.4 nm list.  Each entry's sums
-// become fixed point on their own (so the bits do not depend on the lane
-// count); the group's first lane adds them to the forces and energies k_vv_mx
-// assigned.  A kernel of its own: inside k_vv_mx the FP64 registers spill.
+// force error, and these pairs are ~10 % of a 1.4 nm list.  A fixed 4 lanes per
+// molecule; each lane's FP64 sums become fixed point, and the group's first lane
+// adds them to the forces and energies k_vv_mx assigned.  A kernel of its own:
+// inside k_vv_mx the FP64 registers spill.
 template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
     int32_t k = 0, job = 0;
@@ -993,12 +1008,12 @@ __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
     int chunk = 0;
     if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
     if (held(p, job)) return;
-    long long fi[APS][3];
-    long long ei[11];
+    double f[APS][3];
+    double en[11];
 #pragma unroll
-    for (int i = 0; i < APS; ++i) fi[i][0] = fi[i][1] = fi[i][2] = 0;
+    for (int i = 0; i < APS; ++i) f[i][0] = f[i][1] = f[i][2] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 11; ++c) ei[c] = 0;
+    for (int c = 0; c < 11; ++c) en[c] = 0.0;
     int32_t e0 = 0, nn = 0, sk = 0;
     if (valid) {
         e0 = (int32_t)p.vv_off[k];
@@ -1015,19 +1030,15 @@ __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
         for (int32_t e = e0 + chunk; e < e0 + nn; e += NCHM) {
             const uint32_t ent = p.vv_ent[e];
             const int32_t sm = (int32_t)(ent & ENT_IDX);
-            const bool canon = (ent & ENT_CANON) != 0;
+            const double canon = (ent & ENT_CANON) ? 1.0 : 0.0;
             d3 pb[APS];
 #pragma unroll
             for (int j = 0; j < APS; ++j) pb[j] = xyz(p.xq[sm + j]);
             const d3 rf = nim(box, pa[0], pb[0]);
             const double tx = rf.x - pa[0].x + pb[0].x, ty = rf.y - pa[0].y + pb[0].y, tz = rf.z - pa[0].z + pb[0].z;
-            double en[11];
-#pragma unroll
-            for (int c = 0; c < 11; ++c) en[c] = 0.0;
 #pragma unroll
             for (int i = 0; i < APS; ++i) {
                 const double xi = pa[i].x + tx, yi = pa[i].y + ty, zi = pa[i].z + tz;
-                double gxs = 0.0, gys = 0.0, gzs = 0.0;
 #pragma unroll
                 for (int j = 0; j < APS; ++j) {
                     const double rx = xi - pb[j].x, ry = yi - pb[j].y, rz = zi - pb[j].z;
@@ -1050,27 +1061,31 @@ __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
                     }
                     fm = ok ? fm : 0.0;
                     const double gx = rx * fm, gy = ry * fm, gz = rz * fm;
-                    gxs += gx;
-                    gys += gy;
-                    gzs += gz;
-                    en[0] += ok ? elj : 0.0;
-                    en[1] += ok ? ecrf : 0.0;
+                    f[i][0] += gx;
+                    f[i][1] += gy;
+                    f[i][2] += gz;
+                    en[0] += ok ? canon * elj : 0.0;
+                    en[1] += ok ? canon * ecrf : 0.0;
                     if (VIR) {
-                        en[2] += rx * gx; en[3] += rx * gy; en[4] += rx * gz;
-                        en[5] += ry * gx; en[6] += ry * gy; en[7] += ry * gz;
-                        en[8] += rz * gx; en[9] += rz * gy; en[10] += rz * gz;
+                        const double cx = canon * rx, cy = canon * ry, cz = canon * rz;
+                        en[2] += cx * gx; en[3] += cx * gy; en[4] += cx * gz;
+                        en[5] += cy * gx; en[6] += cy * gy; en[7] += cy * gz;
+                        en[8] += cz * gx; en[9] += cz * gy; en[10] += cz * gz;
                     }
                 }
-                fi[i][0] += to_fx(gxs);
-                fi[i][1] += to_fx(gys);
-                fi[i][2] += to_fx(gzs);
-            }
-            if (canon) {
-#pragma unroll
-                for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += (long long)rint(en[c] * EX_SCALE);
             }
         }
     }
+    // each lane's FP64 sums as fixed point (the lane count is fixed at NCHM, so
+    // a list gives the same bits whatever batch it is in), then the lanes' sum
+    long long fi[APS][3];
+    long long ei[11];
+#pragma unroll
+    for (int i = 0; i < APS; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fi[i][c] = to_fx(f[i][c]);
+#pragma unroll
+    for (int c = 0; c < 11; ++c) ei[c] = to_ex(en[c]);
 #pragma unroll
     for (int m = 1; m < NCHM; m <<= 1) {
 #pragma unroll
@@ -2126,6 +2141,9 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
             const double* fb = p.bd.fb + 3 * (int64_t)(J.sol_off + (i - J.atom_off));
             f = mk(fb[0] + f.x, fb[1] + f.y, fb[2] + f.z);
         }
+        // a non-finite or absurd force (a diverged job): the job fails at this step
+        if (!(fabs(f.x) < FORCE_LIMIT && fabs(f.y) < FORCE_LIMIT && fabs(f.z) < FORCE_LIMIT))
+            atomicMin(p.force_fail + jb, step);
         double im = p.inv_mass[i];
         if (J.thermo) vn = mk((vo.x + f.x * im * dt) * lam, (vo.y + f.y * im * dt) * lam, (vo.z + f.z * im * dt) * lam);
         else vn = mk(vo.x + f.x * im * dt, vo.y + f.y * im * dt, vo.z + f.z * im * dt);

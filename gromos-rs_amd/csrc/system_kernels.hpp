@@ -88,6 +88,7 @@ struct SysPtrs {
     int32_t vv_virial;         // vv_e carries the virial
     double* records;
     int32_t* settle_fail;      // per job: first failing step
+    int32_t* force_fail;       // per job: first step with a non-finite or out-of-range force
     double* therm_scale;       // per job: Berendsen velocity scale of the current step
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;

@@ -45,7 +45,8 @@ enum gromos_amd_status {
     GROMOS_AMD_ENOMEM = -3,     /* device allocation failed / over budget */
     GROMOS_AMD_ESETTLE = -4,    /* SETTLE failed (constraints.rs:896-951) */
     GROMOS_AMD_EBUSY = -5,      /* no admission (server full) */
-    GROMOS_AMD_ENOTFOUND = -6
+    GROMOS_AMD_ENOTFOUND = -6,
+    GROMOS_AMD_EFORCE = -7      /* a non-finite or out-of-range force (|F| >= 2^25 kJ/mol/nm): the job diverged */
 };
 
 /* CRFParameters (params.rs:60-68). */
@@ -407,7 +408,7 @@ enum gromos_amd_job_state {
     GROMOS_AMD_JOB_QUEUED = 0,   /* admitted, waiting for the next layout boundary */
     GROMOS_AMD_JOB_RUNNING = 1,
     GROMOS_AMD_JOB_DONE = 2,     /* all requested steps done */
-    GROMOS_AMD_JOB_FAILED = 3    /* stopped (SETTLE failure); last_error() after job_query says why */
+    GROMOS_AMD_JOB_FAILED = 3    /* stopped (SETTLE failure or a diverged force); last_error() after job_query says why */
 };
 
 typedef struct {

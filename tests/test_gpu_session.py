@@ -219,6 +219,40 @@ def test_failed_job_is_isolated(prec):
     sess.close()
 
 
+@pytest.mark.parametrize("prec", [rt.FP64, rt.MIXED])
+def test_diverged_force_fails_only_its_job(prec):
+    """A job whose forces blow up (one water moved onto another, first atoms
+    1e-4 nm apart: the O-O repulsion is ~1e47 kJ/mol/nm, infinite in FP32) is
+    marked FAILED for a diverged force at its first step (the fixed-point
+    conversions saturate instead of wrapping); the jobs batched with it finish
+    bit-identical to the same jobs run alone."""
+    good = [gen.spc_water_box(600, seed=51), gen.fep_job(1500, 5, 0.3, seed=52)]
+    bad = gen.spc_water_box(500, seed=53)
+    a, b = bad.n_solute_atoms + 3 * 10, bad.n_solute_atoms + 3 * 11
+    bad.pos[b:b + 3] += bad.pos[a] + np.array([1e-4, 0.0, 0.0]) - bad.pos[b]
+    steps = 10
+    sess = rt.Session(precision=prec, nsnb=5)
+    jg = [sess.admit(s, steps) for s in good]
+    jb = sess.admit(bad, steps)
+    sess.step(steps)
+    st, done = sess.query(jb)
+    assert st == rt.FAILED and done == 0
+    assert "force" in rt.lib().gromos_amd_last_error().decode()
+    for j, s in zip(jg, good):
+        assert sess.query(j) == (rt.DONE, steps)
+        e0, x0, v0 = alone(s, steps, prec)
+        assert np.array_equal(sess.energies(j), e0)
+        x, v = sess.fetch(j)
+        assert np.array_equal(x, x0) and np.array_equal(v, v0)
+    sess.close()
+    # the single-system entry reports the same as an error
+    ds = rt.DeviceSystem(bad, precision=prec)
+    ds.pairlist_update()
+    with pytest.raises(rt.GromosAmdError, match="force"):
+        ds.md(3)
+    ds.close()
+
+
 def test_layout_failure_leaves_session_consistent(monkeypatch):
     """A batch layout that runs out of device memory (forced from the host: a
     layout of more atoms than GROMOS_AMD_LAYOUT_ATOM_LIMIT fails before any

@@ -231,6 +231,8 @@ struct gromos_amd_system {
     int32_t *d_bd_aoff = nullptr, *d_bd_aref = nullptr;
     int32_t nbd = 0;
     int32_t* d_vv_near = nullptr;   // mixed: near partners at the front of each solvent list
+    long long* d_fi_near = nullptr;  // mixed: the near pairs' forces (k_vv_nr)
+    double* d_vv_en = nullptr;       // mixed: the near pairs' per-group records
     double near_r = gamd::VV_NEAR_R;   // their radius (nm)
     int32_t last_rec_row = -1;   // records row of the last force evaluation
     char* d_temp = nullptr;
@@ -252,6 +254,8 @@ struct gromos_amd_system {
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    std::vector<hipEvent_t> ev_near_pool;   // the near-pair kernel's launches (mixed)
+    size_t ev_near_used = 0;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
     int64_t timed_pairs = 0;    // pair interactions the timed launches evaluated
@@ -324,6 +328,7 @@ struct gromos_amd_system {
 
     gamd::SysPtrs ptrs() const;
     hipEvent_t next_event();
+    hipEvent_t next_near_event();
     int collect_timing();
 };
 

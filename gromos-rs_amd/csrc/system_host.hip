@@ -670,7 +670,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
-        (rc = S->reserve(S->d_vv_near, nc, true)) ||
+        (rc = S->reserve(S->d_vv_near, nc, true)) || (rc = S->reserve(S->d_fi_near, n3, true)) ||
+        (rc = S->reserve(S->d_vv_en, 11 * nc, true)) ||
         (rc = S->reserve(S->d_step_done, 1)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
         return rc;
@@ -693,6 +694,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipMemsetAsync(S->d_xf, 0, npa * sizeof(float4), st));
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_vv_near, 0, nc * sizeof(int32_t), st));
+    HIPCHK(hipMemsetAsync(S->d_fi_near, 0, n3 * sizeof(long long), st));   // padding slots stay zero
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step_done, 0, sizeof(uint32_t), st));
     HIPCHK(hipMemsetAsync(S->d_rows, 0, (size_t)NSLOT * S->nrow_blocks * ROW * sizeof(double), st));
@@ -990,30 +992,38 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     if (!S->xq_valid) launch_gather(p, npad, mixed, st);
     S->xq_valid = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t n0 = nullptr, n1 = nullptr;
     if (S->timing) {
         e0 = S->next_event();
         e1 = S->next_event();
+        if (mixed) {
+            n0 = S->next_near_event();
+            n1 = S->next_near_event();
+        }
     }
     const int32_t aps = solvent_aps(S);
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
-    // mixed md step: the solute atoms' pass (general pairs, then excluded / 1-4
-    // terms) is independent of the solvent pair kernel (separate outputs, inputs
-    // ready at the fork) and latency bound, so it overlaps it on the side stream
+    // mixed md step: the near solvent pairs (FP64) and the solute atoms' pass
+    // (general pairs, then excluded / 1-4 terms, covalent terms) are independent
+    // of the far solvent pair kernel (separate outputs, inputs ready at the fork),
+    // so they overlap it on the side stream
     const bool split = mixed && !assemble && S->nsol > 0 && !S->no_side;
-    auto launch_first = [&](hipEvent_t a, hipEvent_t b) -> int {
+    auto launch_first = [&](hipEvent_t a, hipEvent_t b, hipEvent_t na, hipEvent_t nb) -> int {
         if (split) {
             HIPCHK(hipEventRecord(S->ev_fork, st));
             HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
+            launch_vv_near(p, aps, form, S->do_virial, S->side, na, nb);
             launch_gen(p, npad, false, S->do_virial, mixed, 0, S->side, GEN_SOLUTE);
             launch_excl14(p, npad, 0, S->d_excl_blocks, S->n_excl_blocks, S->side);
             launch_bonded(p.bd, p.x_cur, S->side);
             HIPCHK(hipEventRecord(S->ev_join, S->side));
         }
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st, a, b);
+        if (mixed && !split) launch_vv_near(p, aps, form, S->do_virial, st, na, nb);
         return 0;
     };
-    if (int rc = launch_first(e0, e1)) return rc;
+    if (int rc = launch_first(e0, e1, n0, n1)) return rc;
     bool refilled = false;
     if (int rc = finish_pairlist(S, &refilled)) return rc;
     if (refilled) {
@@ -1021,7 +1031,7 @@ int forcefield(gromos_amd_system* S, bool assemble) {
         // they assign their outputs, so running them again on the refilled lists
         // replaces everything they wrote
         p = S->ptrs();
-        if (int rc = launch_first(nullptr, nullptr)) return rc;
+        if (int rc = launch_first(nullptr, nullptr, nullptr, nullptr)) return rc;
     }
     // the timed solvent pair launch's work: the advancing jobs' solvent-solvent
     // molecule pairs (each listed from both sides) times the atom pairs of one
@@ -1067,6 +1077,7 @@ static int launch_step_graph(gromos_amd_system* S) {
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         launch_vv(p, S->ncg, aps, form, S->do_virial, mixed, S->vv_lanes, st);
+        if (mixed) launch_vv_near(p, aps, form, S->do_virial, st);
         launch_gen(p, S->npad, false, S->do_virial, mixed, (int32_t)S->host_counters[4], st);
         launch_excl14(p, S->npad, 0, S->d_excl_blocks, S->n_excl_blocks, st);
         launch_bonded(p.bd, p.x_cur, st);
@@ -1211,6 +1222,7 @@ gromos_amd_system::~gromos_amd_system() {
     for (auto& kv : bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_near_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : step_ev) (void)hipEventDestroy(e);
     if (side) (void)hipStreamSynchronize(side);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -1229,16 +1241,34 @@ hipEvent_t gromos_amd_system::next_event() {
     return ev_pool[ev_used++];
 }
 
+hipEvent_t gromos_amd_system::next_near_event() {
+    if (ev_near_used == ev_near_pool.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        ev_near_pool.push_back(e);
+    }
+    return ev_near_pool[ev_near_used++];
+}
+
+// the solvent pair launches' kernel time: k_vv (FP64) or k_vv_mx plus the near
+// pairs' k_vv_nr (mixed), counted as one launch per step
 int gromos_amd_system::collect_timing() {
-    if (ev_used == 0) return 0;
+    if (ev_used == 0 && ev_near_used == 0) return 0;
     HIPCHK(hipStreamSynchronize(stream));
+    if (side) HIPCHK(hipStreamSynchronize(side));
     for (size_t k = 0; k + 1 < ev_used; k += 2) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
         timed_ms += ms;
         timed_launches++;
     }
+    for (size_t k = 0; k + 1 < ev_near_used; k += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev_near_pool[k], ev_near_pool[k + 1]));
+        timed_ms += ms;
+    }
     ev_used = 0;
+    ev_near_used = 0;
     return 0;
 }
 
@@ -1306,6 +1336,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.f_x = d_f_x;
     p.rows = d_rows;
     p.vv_e = d_vv_e;
+    p.fi_near = d_fi_near;
+    p.vv_en = precision == GROMOS_AMD_MIXED ? d_vv_en : nullptr;
     p.vv_part = d_vv_part;
     p.cg_ek = d_cg_ek;
     p.sol_atom = d_sol_atom;

@@ -995,12 +995,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
 
 // Near solvent partners of the mixed path (the neighbours within VV_NEAR_R,
 // listed at the start of each slot by the search) in FP64 with k_vv's per-atom-pair form,
-// after k_vv_mx on the same stream: FP32 rounding of the large, mutually
+// with outputs of its own: FP32 rounding of the large, mutually
 // cancelling Coulomb terms of close molecule pairs is what limits the mixed
-// force error, and these pairs are ~10 % of a 1.4 nm list.  A fixed 4 lanes per
+// force error, and these pairs are ~8 % of a 1.4 nm list.  A fixed 4 lanes per
 // molecule; each lane's FP64 sums become fixed point, and the group's first lane
-// adds them to the forces and energies k_vv_mx assigned.  A kernel of its own:
-// inside k_vv_mx the FP64 registers spill.
+// writes them to fi_near / vv_en, which the integrator and the energy reduction
+// add to k_vv_mx's outputs.  A kernel of its own (inside k_vv_mx the FP64
+// registers spill), on the side stream beside k_vv_mx in md steps.
 template <int APS, bool LJ00, bool VIR, int NCHM>
 __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
     int32_t k = 0, job = 0;
@@ -1095,13 +1096,20 @@ __global__ void __launch_bounds__(BLOCK) k_vv_nr(SysPtrs p, int32_t nwaves) {
 #pragma unroll
         for (int c = 0; c < (VIR ? 11 : 2); ++c) ei[c] += __shfl_xor(ei[c], m, WAVE);
     }
-    if (chunk == 0 && valid && nn > 0) {
+    // assigned for every group (zeros without near partners), so that this kernel
+    // shares no output with k_vv_mx and may run beside it
+    if (chunk == 0 && valid) {
+        const int32_t nk = p.s_cg_n[k];
+        if (nn > 0) {
 #pragma unroll
-        for (int i = 0; i < APS; ++i)
+            for (int i = 0; i < APS; ++i)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) p.fi_s[3 * (sk + i) + c] += fi[i][c];
+                for (int c = 0; c < 3; ++c) p.fi_near[3 * (sk + i) + c] = fi[i][c];
+        } else {
+            for (int32_t i = 0; i < 3 * nk; ++i) p.fi_near[3 * sk + i] = 0;
+        }
 #pragma unroll
-        for (int c = 0; c < (VIR ? 11 : 2); ++c) p.vv_e[11 * (int64_t)k + c] += (double)ei[c] * EX_INV;
+        for (int c = 0; c < 11; ++c) p.vv_en[11 * (int64_t)k + c] = (VIR || c < 2) ? (double)ei[c] * EX_INV : 0.0;
     }
 }
 
@@ -1956,7 +1964,7 @@ __global__ void k_assemble(SysPtrs p, int32_t npad, int32_t with_long, int32_t m
     for (int c = 0; c < 3; ++c) {
         double f;
         if (mixed) {
-            long long t = p.fi_s[3 * a + c];
+            long long t = p.fi_s[3 * a + c] + p.fi_near[3 * a + c];
             if (with_long) t += p.fi_long_s[3 * a + c];
             f = (double)t * FX_INV;
         } else {
@@ -2102,7 +2110,7 @@ __device__ __forceinline__ v3 total_force(const SysPtrs& p, int32_t i, int32_t w
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         if (MIXED) {
-            long long t = p.fi_s[3 * a + c];
+            long long t = p.fi_s[3 * a + c] + p.fi_near[3 * a + c];
             if (with_long) t += p.fi_long_s[3 * a + c];
             f[c] = (double)t * FX_INV;
         } else {
@@ -2211,6 +2219,11 @@ __device__ __forceinline__ void vv_part_rows(const SysPtrs& p, int64_t k0, int64
         const double* r = p.vv_e + 11 * k;
 #pragma unroll
         for (int c = 0; c < (VIR ? 11 : 2); ++c) v[c] += r[c];
+        if (p.vv_en) {   // mixed: the near solvent pairs' record
+            const double* rn = p.vv_en + 11 * k;
+#pragma unroll
+            for (int c = 0; c < (VIR ? 11 : 2); ++c) v[c] += rn[c];
+        }
         if (KIN) {
             v[1] += p.cg_ek[3 * k + 2];   // solvent excluded RF energy (md steps)
             v[11] += p.cg_ek[3 * k];
@@ -2437,32 +2450,17 @@ static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t m
         if (e0 || e1) hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);       \
         else hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                               \
     } while (0)
-    // the mixed path: k_vv_mx, then k_vv_nr (4 lanes per molecule: its few
-    // entries); timing events bracket both (start on the first, stop on the second)
-    const int32_t nw4 = p.nvv_chunk * (VV_CHUNK / (WAVE / 4));
-    dim3 g4 = xcd_grid((nw4 + BLOCK / WAVE - 1) / (BLOCK / WAVE));
-#define GAMD_VV_MX(KER, NR)                                                                          \
-    do {                                                                                             \
-        if (e0 || e1) {                                                                              \
-            hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, nullptr, 0, p, nw);                 \
-            hipExtLaunchKernelGGL(NR, g4, dim3(BLOCK), 0, s, nullptr, e1, 0, p, nw4);                \
-        } else {                                                                                     \
-            hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                                    \
-            hipLaunchKernelGGL(NR, g4, dim3(BLOCK), 0, s, p, nw4);                                   \
-        }                                                                                            \
-    } while (0)
     if (mixed && neutral) {
-        if (virial) GAMD_VV_MX((k_vv_mx<APS, LJ00, true, L, true>), (k_vv_nr<APS, LJ00, true, 4>));
-        else GAMD_VV_MX((k_vv_mx<APS, LJ00, false, L, true>), (k_vv_nr<APS, LJ00, false, 4>));
+        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, true>));
+        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, true>));
     } else if (mixed) {
-        if (virial) GAMD_VV_MX((k_vv_mx<APS, LJ00, true, L, false>), (k_vv_nr<APS, LJ00, true, 4>));
-        else GAMD_VV_MX((k_vv_mx<APS, LJ00, false, L, false>), (k_vv_nr<APS, LJ00, false, 4>));
+        if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, false>));
+        else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, false>));
     } else {
         if (virial) GAMD_VV_LAUNCH((k_vv<APS, LJ00, true, L>));
         else GAMD_VV_LAUNCH((k_vv<APS, LJ00, false, L>));
     }
 #undef GAMD_VV_LAUNCH
-#undef GAMD_VV_MX
 }
 template <int APS, bool LJ00>
 static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,
@@ -2470,6 +2468,31 @@ static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t
     if (lanes == 4) launch_vv_l<APS, LJ00, 4>(p, ncg, virial, mixed, neutral, e0, e1, s);
     else if (lanes == 16) launch_vv_l<APS, LJ00, 16>(p, ncg, virial, mixed, neutral, e0, e1, s);
     else launch_vv_l<APS, LJ00, 8>(p, ncg, virial, mixed, neutral, e0, e1, s);
+}
+template <int APS, bool LJ00>
+static void launch_vv_near_aps(const SysPtrs& p, int32_t virial, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+    const int32_t nw = p.nvv_chunk * (VV_CHUNK / (WAVE / 4));   // 4 lanes per molecule
+    dim3 g = xcd_grid((nw + BLOCK / WAVE - 1) / (BLOCK / WAVE));
+    if (e0 || e1) {
+        if (virial) hipExtLaunchKernelGGL((k_vv_nr<APS, LJ00, true, 4>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+        else hipExtLaunchKernelGGL((k_vv_nr<APS, LJ00, false, 4>), g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);
+    } else {
+        if (virial) hipLaunchKernelGGL((k_vv_nr<APS, LJ00, true, 4>), g, dim3(BLOCK), 0, s, p, nw);
+        else hipLaunchKernelGGL((k_vv_nr<APS, LJ00, false, 4>), g, dim3(BLOCK), 0, s, p, nw);
+    }
+}
+void launch_vv_near(const SysPtrs& p, int32_t aps, int32_t form, int32_t virial, hipStream_t s, hipEvent_t e0,
+                    hipEvent_t e1) {
+    switch (aps * 2 + ((form & VV_LJ00) ? 1 : 0)) {
+        case 2: launch_vv_near_aps<1, false>(p, virial, e0, e1, s); break;
+        case 3: launch_vv_near_aps<1, true>(p, virial, e0, e1, s); break;
+        case 4: launch_vv_near_aps<2, false>(p, virial, e0, e1, s); break;
+        case 5: launch_vv_near_aps<2, true>(p, virial, e0, e1, s); break;
+        case 8: launch_vv_near_aps<4, false>(p, virial, e0, e1, s); break;
+        case 9: launch_vv_near_aps<4, true>(p, virial, e0, e1, s); break;
+        case 7: launch_vv_near_aps<3, true>(p, virial, e0, e1, s); break;
+        default: launch_vv_near_aps<3, false>(p, virial, e0, e1, s); break;
+    }
 }
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {

@@ -104,6 +104,8 @@ struct SysPtrs {
     const int32_t* pap;        // entry index | bit 31 when the atom is the entry's i
     int32_t* o2s;              // original atom -> sorted atom
     int32_t* vv_near;          // per sorted group: solvent partners at the start of its slot (near ones)
+    long long* fi_near;        // mixed: the near solvent pairs' fixed-point forces (sorted), k_vv_nr
+    double* vv_en;             // mixed: their per-group energy / virial record (11), k_vv_nr; else null
     BondedDev bd;              // covalent terms; bd.fb per solute atom (sol_atom order),
                                // nullptr when the batch has none
 };
@@ -128,10 +130,12 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 // solvent molecules: the mixed kernel sums the RF polynomial per molecule pair);
 // lanes: lanes per solvent molecule (4, 8 or 16)
 constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
-// mixed: k_vv_mx then k_vv_nr (the near partners in FP64); timing events, when
-// given, bracket both
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// mixed: the near solvent partners in FP64 (k_vv_nr) into fi_near / vv_en;
+// independent of launch_vv's outputs
+void launch_vv_near(const SysPtrs& p, int32_t aps, int32_t form, int32_t virial, hipStream_t s,
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // ngsol: solvent groups listed in gsol by the last pair search (mixed short pass);
 // part: GEN_ALL, or (mixed short pass) only the solvent or only the solute atoms

@@ -410,7 +410,10 @@ def test_mixed_water32768_error_and_determinism():
     g = out[0]
     assert force_err(g["forces"], r["forces"]) < MX_F_TOL
     assert force_err_median(g["forces"], r["forces"]) < MX_F_MED
-    assert rel(g["e_lj"], r["e_lj"]) < MX_E_TOL and rel(g["e_crf"], r["e_crf"]) < MX_E_TOL
+    # the lattice's CRF energy is a small difference of large pair sums: each
+    # component is held against the system's nonbonded energy scale, as in the
+    # other mixed-path tests
+    mixed_energy_ok(g, r)
     # fixed order everywhere: bit-identical run to run
     assert np.array_equal(out[0]["forces"], out[1]["forces"])
     assert out[0]["e_lj"] == out[1]["e_lj"] and out[0]["e_crf"] == out[1]["e_crf"]

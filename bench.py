@@ -109,14 +109,18 @@ def cpu_baseline(s, nsteps, nsteps_1core):
 
 
 def traffic_from_profile(kernel, tag):
-    """HBM bytes per dispatch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE scaled by
-    a copy of known size), read from the committed summary of the same kernel."""
+    """HBM bytes per step of the solvent pair kernels measured by rocprofv3
+    (FETCH_SIZE, WRITE_SIZE scaled by a copy of known size), read from the
+    committed summary: the far-pair kernel's dispatch plus, where the summary
+    has it, the near-pair kernel's."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}.md")
     try:
         for line in open(path):
-            if line.startswith("machine-readable: traffic_bytes="):
-                return float(line.split("=", 1)[1]), os.path.relpath(path, ROOT)
-    except OSError:
+            if line.startswith("machine-readable:"):
+                kv = dict(t.split("=", 1) for t in line.split(":", 1)[1].split() if "=" in t)
+                total = float(kv["traffic_bytes"]) + float(kv.get("traffic_bytes_near", 0.0))
+                return total, os.path.relpath(path, ROOT)
+    except (OSError, KeyError, ValueError):
         pass
     return None, None
 
@@ -250,13 +254,16 @@ def main():
         "plain_fp64_gpu_one_at_a_time": fep["one_at_a_time_fp64"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": traffic,
-                     "kernel": "k_vv_mx (solvent-solvent pair kernel, mixed precision, no virial)",
+                     "kernel": "k_vv_mx + k_vv_nr (solvent-solvent pair kernels of one md step: far pairs in "
+                               "FP32, near pairs in FP64; mixed precision, no virial)",
                      "avg_launch_ms": kms / max(klaunch, 1), "launches": klaunch,
                      "flop_per_launch": kpairs * flop_per_pair / max(klaunch, 1),
                      "flop_per_pair": flop_per_pair, "traffic_source": traffic_src,
-                     "note": "FP32 vector-ALU bound pair kernel (no dense contraction, no MFMA); its launches timed "
-                             "with hipEvents on the batch's stream over the timed replay; work = the advancing jobs' "
-                             "solvent-solvent pair interactions x 42 FLOP (60 minus the 18 virial FLOP); peak = MI355X "
+                     "note": "vector-ALU bound pair kernels (no dense contraction, no MFMA); each step's far-pair "
+                             "launch timed with hipEvents on the batch's stream and its near-pair launch on the side "
+                             "stream beside it, the two durations summed (they overlap in time, so this understates "
+                             "the rate); work = the advancing jobs' solvent-solvent pair interactions x 42 FLOP (60 "
+                             "minus the 18 virial FLOP); traffic = both kernels' HBM bytes per step; peak = MI355X "
                              "FP32 157.3 TFLOP/s (packed; this build keeps FP32 un-packed: 78.6 reachable)"},
         "fep_mix": fep,
         "water_32768": water,
@@ -293,7 +300,8 @@ def water_bench(rt, gen, s, a):
                               (p["kern_ms"] * 1e-3) / 1e12},
            "speedup_mixed_vs_plain_fp64_gpu": h["pairs_per_s"] / p["pairs_per_s"],
            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": "k_vv_mx",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                        "kernel": "k_vv_mx + k_vv_nr (one system: run one after the other on one stream)",
                         "avg_launch_ms": h["kern_ms"], "launches": h["launches"], "flop_per_launch": flop,
                         "flop_per_pair": flop_per_pair, "traffic_source": traffic_src},
            "energy_drift_kj_mol_ps": {"fp64": p["drift"], "mixed": h["drift"],

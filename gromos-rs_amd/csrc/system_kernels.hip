@@ -356,16 +356,15 @@ __global__ void __launch_bounds__(BLOCK) k_nbl(SysPtrs p, int32_t ncell, double 
                     if (bv) {
                         const bool nr = is_vv && d2 < near_r2;
                         const uint64_t bn = __ballot(nr), bf = bv & ~bn;
-                        if (FILL && is_vv) {
+                        if (FILL) {
+                            // group t's slot and cursors (wave-uniform), then one store:
+                            // the near lanes' rank from the start, the far lanes' from the end
                             const int32_t base = bcast(vv_base, t), cap = bcast(vv_end, t) - base;
-                            const uint32_t e = vv_ent | (canon ? ENT_CANON : 0u);
-                            if (nr) {
-                                const int32_t x = bcast(vv_nn, t) + (int32_t)__popcll(bn & below);
-                                if (x < cap) p.vv_ent[base + x] = e;
-                            } else {
-                                const int32_t x = bcast(vv_nf, t) + (int32_t)__popcll(bf & below);
-                                if (x < cap) p.vv_ent[base + cap - 1 - x] = e;
-                            }
+                            const int32_t nn0 = bcast(vv_nn, t), nf0 = bcast(vv_nf, t);
+                            const int32_t x = nr ? nn0 + (int32_t)__popcll(bn & below)
+                                                 : nf0 + (int32_t)__popcll(bf & below);
+                            if (is_vv && x < cap)
+                                p.vv_ent[nr ? base + x : base + cap - 1 - x] = vv_ent | (canon ? ENT_CANON : 0u);
                         }
                         if (lane == t) {
                             vv_nn += (int32_t)__popcll(bn);

@@ -260,11 +260,11 @@ def main():
                      "flop_per_launch": kpairs * flop_per_pair / max(klaunch, 1),
                      "flop_per_pair": flop_per_pair, "traffic_source": traffic_src,
                      "note": "vector-ALU bound pair kernels (no dense contraction, no MFMA); each step's far-pair "
-                             "launch timed with hipEvents on the batch's stream and its near-pair launch on the side "
-                             "stream beside it, the two durations summed (they overlap in time, so this understates "
-                             "the rate); work = the advancing jobs' solvent-solvent pair interactions x 42 FLOP (60 "
-                             "minus the 18 virial FLOP); traffic = both kernels' HBM bytes per step; peak = MI355X "
-                             "FP32 157.3 TFLOP/s (packed; this build keeps FP32 un-packed: 78.6 reachable)"},
+                             "launch (batch's stream) and near-pair launch (side stream, beside it) timed with HIP "
+                             "events as one span, from the first start to the last end, over the timed replay; "
+                             "work = the advancing jobs' solvent-solvent pair interactions x 42 FLOP (60 minus the "
+                             "18 virial FLOP); traffic = both kernels' HBM bytes per step; peak = MI355X FP32 157.3 "
+                             "TFLOP/s (packed; this build keeps FP32 un-packed: 78.6 reachable)"},
         "fep_mix": fep,
         "water_32768": water,
     }

@@ -254,8 +254,13 @@ struct gromos_amd_system {
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<hipEvent_t> ev_near_pool;   // the near-pair kernel's launches (mixed)
-    size_t ev_near_used = 0;
+    // one timed solvent pair launch: the far (or FP64) kernel between a0 and a1;
+    // mixed: the near kernel between n0 and n1; f, where set, is recorded on the
+    // batch's stream before the fork that puts the near kernel on the side stream
+    struct TimedLaunch {
+        hipEvent_t f, a0, a1, n0, n1;
+    };
+    std::vector<TimedLaunch> timed;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
     int64_t timed_pairs = 0;    // pair interactions the timed launches evaluated
@@ -328,7 +333,6 @@ struct gromos_amd_system {
 
     gamd::SysPtrs ptrs() const;
     hipEvent_t next_event();
-    hipEvent_t next_near_event();
     int collect_timing();
 };
 

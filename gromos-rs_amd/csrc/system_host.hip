@@ -991,16 +991,16 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     // pair-list build (new order), unless the state was set from outside
     if (!S->xq_valid) launch_gather(p, npad, mixed, st);
     S->xq_valid = false;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEvent_t n0 = nullptr, n1 = nullptr;
+    gromos_amd_system::TimedLaunch tl{};
     if (S->timing) {
-        e0 = S->next_event();
-        e1 = S->next_event();
+        tl.a0 = S->next_event();
+        tl.a1 = S->next_event();
         if (mixed) {
-            n0 = S->next_near_event();
-            n1 = S->next_near_event();
+            tl.n0 = S->next_event();
+            tl.n1 = S->next_event();
         }
     }
+    hipEvent_t e0 = tl.a0, e1 = tl.a1, n0 = tl.n0, n1 = tl.n1;
     const int32_t aps = solvent_aps(S);
     if (int rc = tune_vv(S, p, aps, mixed)) return rc;
     const int32_t form = (S->vv_lj00 ? VV_LJ00 : 0) | (S->vv_neutral ? VV_NEUTRAL : 0);
@@ -1011,6 +1011,10 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     const bool split = mixed && !assemble && S->nsol > 0 && !S->no_side;
     auto launch_first = [&](hipEvent_t a, hipEvent_t b, hipEvent_t na, hipEvent_t nb) -> int {
         if (split) {
+            if (na) {
+                tl.f = S->next_event();
+                HIPCHK(hipEventRecord(tl.f, st));
+            }
             HIPCHK(hipEventRecord(S->ev_fork, st));
             HIPCHK(hipStreamWaitEvent(S->side, S->ev_fork, 0));
             launch_vv_near(p, aps, form, S->do_virial, S->side, na, nb);
@@ -1035,7 +1039,10 @@ int forcefield(gromos_amd_system* S, bool assemble) {
     }
     // the timed solvent pair launch's work: the advancing jobs' solvent-solvent
     // molecule pairs (each listed from both sides) times the atom pairs of one
-    if (e0) S->timed_pairs += active_pairs(S, 2) / 2 * (int64_t)aps * aps;
+    if (e0) {
+        S->timed.push_back(tl);
+        S->timed_pairs += active_pairs(S, 2) / 2 * (int64_t)aps * aps;
+    }
     const int32_t ngsol = (int32_t)S->host_counters[4];
     launch_gen(p, npad, false, S->do_virial, mixed, ngsol, st, split ? GEN_SOLVENT : GEN_ALL);
     S->pair_count += active_pairs(S, 0);
@@ -1222,7 +1229,6 @@ gromos_amd_system::~gromos_amd_system() {
     for (auto& kv : bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ev_near_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : step_ev) (void)hipEventDestroy(e);
     if (side) (void)hipStreamSynchronize(side);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -1241,34 +1247,40 @@ hipEvent_t gromos_amd_system::next_event() {
     return ev_pool[ev_used++];
 }
 
-hipEvent_t gromos_amd_system::next_near_event() {
-    if (ev_near_used == ev_near_pool.size()) {
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        ev_near_pool.push_back(e);
-    }
-    return ev_near_pool[ev_near_used++];
-}
-
-// the solvent pair launches' kernel time: k_vv (FP64) or k_vv_mx plus the near
-// pairs' k_vv_nr (mixed), counted as one launch per step
+// the solvent pair work of each timed step: k_vv (FP64) or k_vv_mx followed by
+// the near pairs' k_vv_nr on one stream (the two durations summed), or, where
+// k_vv_nr runs on the side stream beside k_vv_mx, the span from the first of the
+// two starting to the last finishing (overlapping durations are not added)
 int gromos_amd_system::collect_timing() {
-    if (ev_used == 0 && ev_near_used == 0) return 0;
+    if (timed.empty()) {
+        ev_used = 0;
+        return 0;
+    }
     HIPCHK(hipStreamSynchronize(stream));
     if (side) HIPCHK(hipStreamSynchronize(side));
-    for (size_t k = 0; k + 1 < ev_used; k += 2) {
+    auto el = [](hipEvent_t a, hipEvent_t b, float& ms) { return hipEventElapsedTime(&ms, a, b); };
+    for (const TimedLaunch& t : timed) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
+        if (t.f) {
+            float a0 = 0.f, a1 = 0.f, n0 = 0.f, n1 = 0.f;
+            HIPCHK(el(t.f, t.a0, a0));
+            HIPCHK(el(t.f, t.a1, a1));
+            HIPCHK(el(t.f, t.n0, n0));
+            HIPCHK(el(t.f, t.n1, n1));
+            ms = std::max(a1, n1) - std::min(a0, n0);
+        } else {
+            HIPCHK(el(t.a0, t.a1, ms));
+            if (t.n0) {
+                float mn = 0.f;
+                HIPCHK(el(t.n0, t.n1, mn));
+                ms += mn;
+            }
+        }
         timed_ms += ms;
         timed_launches++;
     }
-    for (size_t k = 0; k + 1 < ev_near_used; k += 2) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, ev_near_pool[k], ev_near_pool[k + 1]));
-        timed_ms += ms;
-    }
+    timed.clear();
     ev_used = 0;
-    ev_near_used = 0;
     return 0;
 }
 

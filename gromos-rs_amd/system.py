@@ -122,7 +122,8 @@ class System:
     dist_oh: float = 0.1
     dist_hh: float = 0.1632990
     pert: Optional[Perturbation] = None
-    thermostat: Optional[tuple] = None   # Berendsen (T_ref K, tau ps, degrees of freedom)
+    thermostat: Optional[tuple] = None   # one bath: (T_ref K, tau ps, degrees of freedom)
+    thermostat_nhc: int = 0              # 0: Berendsen; N >= 1: Nose-Hoover chain of length N
     name: str = "system"
     # triclinic box (boundary 2): f64 [3, 3], columns a, b, c in the lower-triangular
     # GROMOS convention a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz)

@@ -6,6 +6,7 @@
  */
 #include "gromos_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1502,8 +1503,10 @@ struct or_md {
     or_ff *ff;
     double dt;
     int use_settle;
-    int thermo;                 /* Berendsen weak coupling, one bath over all atoms */
+    int thermo;                 /* one bath over all atoms: 1 Berendsen, 2 Nose-Hoover (chain) */
     double t_ref, tau_t, dof_t;
+    int nhc;                    /* Nose-Hoover chain length */
+    double zeta[OR_NHC_MAX];    /* the chain's state, zero at start */
     double *pos[2], *vel[2], *force[2];
     double *inv_mass;
     int cur;
@@ -1555,6 +1558,58 @@ void or_md_set_berendsen(or_md *md, double t_ref, double tau, double dof) {
     md->dof_t = dof;
 }
 
+/* NoseHooverThermostat::new_single_bath / new_chain_bath
+ * (nosehoover_thermostat.rs:64-113): the chain starts at zero */
+int or_md_set_nosehoover(or_md *md, double t_ref, double tau, double dof, int nhc) {
+    if (nhc < 1 || nhc > OR_NHC_MAX) return -1;
+    md->thermo = 2;
+    md->t_ref = t_ref;
+    md->tau_t = tau;
+    md->dof_t = dof;
+    md->nhc = nhc;
+    for (int i = 0; i < OR_NHC_MAX; ++i) md->zeta[i] = 0.0;
+    return 0;
+}
+
+/* NoseHooverThermostat::apply (nosehoover_thermostat.rs:116-227), one bath:
+ * advances the chain with E_kin_new (the velocities entering the step) and
+ * returns the velocity scale; 1 where the reference leaves the velocities
+ * untouched (no coupling, or |scale - 1| < 1e-15) */
+double or_nosehoover_step(double *zeta, int nhc, double ekin, double t_ref, double tau, double dof, double dt) {
+    if (tau < 0.0) return 1.0;
+    double free_temp = dof > 0.0 ? 2.0 * ekin / (dof * OR_KB) : 0.0;
+    if (free_temp < DBL_EPSILON) free_temp = t_ref;
+    double scale;
+    if (nhc == 1) {
+        zeta[0] += dt / (tau * tau) * (free_temp / t_ref - 1.0);
+        scale = 1.0 - zeta[0] * dt;
+    } else {
+        /* tail to head; tau_vec[0] = tau^2, tau_vec[i > 0] = tau^2 / dof */
+        const double tau2 = tau * tau;
+#define OR_TV(i) ((i) == 0 ? tau2 : tau2 / dof)
+        {
+            const double z = zeta[nhc - 2];
+            zeta[nhc - 1] += (OR_TV(nhc - 2) * z * z - 1.0 / dof) / OR_TV(nhc - 1) * dt;
+        }
+        for (int i = nhc - 2; i >= 1; --i) {
+            const double zp = zeta[i - 1], zc = zeta[i], zn = zeta[i + 1];
+            zeta[i] += ((OR_TV(i - 1) * zp * zp - 1.0 / dof) / OR_TV(i) - zc * zn) * dt;
+        }
+        {
+            const double z0 = zeta[0], z1 = zeta[1];
+            zeta[0] += ((free_temp / t_ref - 1.0) / OR_TV(0) - z1 * z0) * dt;
+        }
+#undef OR_TV
+        scale = 1.0 - zeta[0] * dt;
+    }
+    if (fabs(scale - 1.0) < 1e-15) return 1.0;
+    return scale;
+}
+
+void or_md_get_nosehoover(const or_md *md, double *zeta) {
+    for (int i = 0; i < md->nhc; ++i) zeta[i] = md->zeta[i];
+}
+
 /* BerendsenThermostat::apply (berendsen_thermostat.rs:61-100): the scale from
  * E_kin_new = 0.5 sum m |v|^2 of the velocities entering the step (stored by the
  * previous TemperatureCalculation, temperature.rs:76-85, or by its init) */
@@ -1580,8 +1635,18 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
         md->pairs += or_ff_last_pair_count(md->ff);
         /* exchange_state: the current buffer becomes old */
         if (md->thermo) {
-            /* LeapFrogVelocity, Berendsen scaling, LeapFrogPosition */
-            double lam = or_berendsen_scale(n, md->t->mass, md->vel[c], md->t_ref, md->tau_t, md->dof_t, md->dt);
+            /* LeapFrogVelocity, thermostat scaling, LeapFrogPosition (md.rs:1143-1206) */
+            double lam;
+            if (md->thermo == 1) {
+                lam = or_berendsen_scale(n, md->t->mass, md->vel[c], md->t_ref, md->tau_t, md->dof_t, md->dt);
+            } else {
+                double ek_new = 0.0;
+                for (int i = 0; i < n; ++i) {
+                    const double *v = md->vel[c] + 3 * i;
+                    ek_new += 0.5 * md->t->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                }
+                lam = or_nosehoover_step(md->zeta, md->nhc, ek_new, md->t_ref, md->tau_t, md->dof_t, md->dt);
+            }
             for (int i = 0; i < n; ++i)
                 for (int a = 0; a < 3; ++a) {
                     double acc = md->force[c][3 * i + a] * md->inv_mass[i];

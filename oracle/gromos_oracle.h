@@ -277,6 +277,12 @@ int or_md_run(or_md *md, int nsteps, double *energies);
 /* Berendsen weak coupling (berendsen_thermostat.rs), one bath over all atoms */
 #define OR_KB 0.00831441   /* kJ/(mol K), units.rs:47 */
 void or_md_set_berendsen(or_md *md, double t_ref, double tau, double dof);
+/* Nose-Hoover chain (nosehoover_thermostat.rs), one bath over all atoms, chain
+ * length 1..OR_NHC_MAX; returns -1 for another length */
+#define OR_NHC_MAX 16
+int or_md_set_nosehoover(or_md *md, double t_ref, double tau, double dof, int nhc);
+double or_nosehoover_step(double *zeta, int nhc, double ekin, double t_ref, double tau, double dof, double dt);
+void or_md_get_nosehoover(const or_md *md, double *zeta);
 double or_berendsen_scale(int n, const double *mass, const double *vel, double t_ref, double tau, double dof,
                           double dt);
 void or_md_get(const or_md *md, double *pos, double *vel, double *force);

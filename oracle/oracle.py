@@ -157,6 +157,11 @@ def _declare(L):
     L.or_md_destroy.argtypes = [P]
     L.or_md_set_settle_geometry.argtypes = [P, D, D]
     L.or_md_set_berendsen.argtypes = [P, D, D, D]
+    L.or_md_set_nosehoover.argtypes = [P, D, D, D, C.c_int]
+    L.or_md_set_nosehoover.restype = C.c_int
+    L.or_nosehoover_step.argtypes = [P, C.c_int, D, D, D, D, D]
+    L.or_nosehoover_step.restype = D
+    L.or_md_get_nosehoover.argtypes = [P, P]
     L.or_berendsen_scale.argtypes = [C.c_int, P, P, D, D, D, D]
     L.or_berendsen_scale.restype = D
     L.or_md_run.argtypes = [P, C.c_int, P]
@@ -506,9 +511,16 @@ class MD:
                                     C.byref(osys.pert) if osys.pert is not None else None, s.dt,
                                     int(use_settle), ptr(self.x0), ptr(self.v0))
         lib().or_md_set_settle_geometry(self.h, s.dist_oh, s.dist_hh)
+        self.nhc = 0
         if getattr(s, "thermostat", None) is not None:
             t_ref, tau, dof = s.thermostat
-            lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
+            nhc = int(getattr(s, "thermostat_nhc", 0))
+            if nhc > 0:
+                if lib().or_md_set_nosehoover(self.h, t_ref, tau, dof, nhc) != 0:
+                    raise ValueError(f"Nose-Hoover chain length {nhc} out of range")
+                self.nhc = nhc
+            else:
+                lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
         self.n = s.n_atoms
 
     def run(self, nsteps):
@@ -523,6 +535,12 @@ class MD:
         lib().or_md_get(self.h, ptr(x), ptr(v), ptr(f))
         return x, v, f
 
+    def zeta(self):
+        """the Nose-Hoover chain's state"""
+        z = np.zeros(max(self.nhc, 1))
+        lib().or_md_get_nosehoover(self.h, ptr(z))
+        return z[:self.nhc]
+
     @property
     def pair_count(self):
         return lib().or_md_pair_count(self.h)
@@ -532,6 +550,13 @@ class MD:
             lib().or_md_destroy(self.h)
         except Exception:
             pass
+
+
+def nosehoover_step(zeta, ekin, t_ref, tau, dof, dt):
+    """one NoseHooverThermostat::apply on the chain state `zeta` (updated in
+    place, float64 array); returns the velocity scale"""
+    assert zeta.dtype == np.float64 and zeta.flags.c_contiguous
+    return lib().or_nosehoover_step(zeta.ctypes.data_as(C.c_void_p), len(zeta), ekin, t_ref, tau, dof, dt)
 
 
 def berendsen_scale(mass, vel, t_ref, tau, dof, dt):

@@ -52,6 +52,12 @@ SYSTEMS = [
     ("butane_vacuum", None, None, None, None, "all"),
     ("benzene_vacuum", None, None, None, None, "all"),
     ("water_1000_spc_gridcell", None, None, None, None, "first"),
+    ("water_216_nvt", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf", None, None,
+     "all"),
+    ("water_216_nvt_nosehoover", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+     None, None, "all"),
+    ("water_216_nvt_nhc_chain", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+     None, None, "all"),
     ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
      "../shared/ch4_spc_dummy.ptp", "first"),
 ]
@@ -203,6 +209,18 @@ def parse_imd(path):
     d["nstlim"], d["dt"] = int(st[0]), float(st[2])
     bc = data_lines(b["BOUNDCOND"])[0].split()
     d["ntb"] = int(bc[0])
+    d["ndfmin"] = int(bc[1]) if len(bc) > 1 else 0
+    # MULTIBATH (one bath): weak-coupling -> Berendsen, nose-hoover -> a chain of
+    # one, nose-hoover-chains N -> a chain of N
+    mb = data_lines(b.get("MULTIBATH", []))
+    d["thermostat"] = None
+    if mb:
+        alg = mb[0].split()
+        tt = mb[2].split()
+        kind = {"weak-coupling": "berendsen", "0": "berendsen", "nose-hoover": "nosehoover", "1": "nosehoover",
+                "nose-hoover-chains": "nosehoover"}.get(alg[0], alg[0])
+        nhc = int(alg[1]) if alg[0] == "nose-hoover-chains" and len(alg) > 1 else 1
+        d["thermostat"] = dict(kind=kind, nhc=nhc, t_ref=float(tt[0]), tau=float(tt[1]), n_baths=int(mb[1].split()[0]))
     pl = data_lines(b["PAIRLIST"])[0].split()
     alg = pl[0]
     d["algorithm"] = {"standard": 0, "0": 0, "grid": 1, "1": 1, "grid_cell": 2, "2": 2}.get(alg, 0)
@@ -365,7 +383,7 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         box_matrix=box_matrix,
         fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
         algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
-        dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"],
+        dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"], ndfmin=imd["ndfmin"], thermostat=imd["thermostat"],
         solvent_constr=t["solvent_constr"], pert=pert,
         has_bonded_terms=bool(t["bonds"] or t["angles"]),
         bonded=dict(bonds=t["bond_terms"], angles=t["angle_terms"], impropers=t["improper_terms"],

@@ -214,6 +214,8 @@ struct gromos_amd_system {
     int32_t* d_settle_fail = nullptr;
     int32_t* d_force_fail = nullptr;
     double* d_therm_scale = nullptr;
+    double* d_nh_zeta = nullptr;   // per job: the Nose-Hoover chain (GROMOS_AMD_NHC_MAX each)
+    double* d_nh_pend = nullptr;   // per job: the chain after the next step's update
     bool any_thermo = false;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
     bool xq_valid = false;     // xq / xf hold the current positions in the current sorted order

@@ -112,6 +112,7 @@ struct SJob {
     int64_t id = 0;
     JobImage img;
     std::vector<double> pos, vel;   // host state while not resident
+    std::vector<double> zeta;       // its Nose-Hoover chain across a layout (empty: at rest)
     int64_t nsteps = 0, done = 0;
     int32_t state = GROMOS_AMD_JOB_QUEUED;
     int32_t slot = -1;              // index in the resident batch, -1 if not resident
@@ -196,6 +197,14 @@ static int relayout(gromos_amd_session* s) {
             HIPCHK(hipMemcpyAsync(j->vel.data(), S->d_v[S->cur] + off, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
         }
         if (!keep.empty()) {
+            // the continuing jobs' Nose-Hoover chains (layout_batch starts every
+            // chain at zero; they are put back below)
+            std::vector<double> z((size_t)GROMOS_AMD_NHC_MAX * S->njobs);
+            HIPCHK(hipMemcpyAsync(z.data(), S->d_nh_zeta, z.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (SJob* j : keep)
+                j->zeta.assign(z.begin() + (size_t)GROMOS_AMD_NHC_MAX * j->slot,
+                               z.begin() + (size_t)GROMOS_AMD_NHC_MAX * (j->slot + 1));
             size_t n3 = (size_t)old_npad * 3;
             if (int rc = s->stage.reserve(2 * n3 * sizeof(double), st)) return rc;
             double* sx = (double*)s->stage.ptr;
@@ -277,6 +286,10 @@ static int relayout(gromos_amd_session* s) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));   // mp feeds an async copy
     }
+    for (size_t k = 0; k < keep.size(); ++k)
+        if (!keep[k]->zeta.empty())
+            HIPCHK(hipMemcpyAsync(S->d_nh_zeta + (size_t)GROMOS_AMD_NHC_MAX * k, keep[k]->zeta.data(),
+                                  GROMOS_AMD_NHC_MAX * sizeof(double), hipMemcpyHostToDevice, st));
     for (size_t k = keep.size(); k < nb.size(); ++k) {
         SJob* j = nb[k];
         size_t n3 = (size_t)j->img.n * 3, off = (size_t)S->jobs[k].atom_off * 3;

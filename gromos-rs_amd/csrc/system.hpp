@@ -53,8 +53,9 @@ struct JobDev {
     double m_o, m_h, dist_oh, dist_hh;     // SETTLE template
     int32_t sol_off;                       // first index of the job's solute atoms in sol_atom
     int32_t settle;                        // SETTLE the solvent molecules
-    int32_t thermo;                        // Berendsen weak coupling (berendsen_thermostat.rs)
+    int32_t thermo;                        // 1 Berendsen (berendsen_thermostat.rs), 2 Nose-Hoover chain
     double t_ref, tau_t, dof_t;
+    int32_t nhc;                           // Nose-Hoover chain length (nosehoover_thermostat.rs)
     int32_t bd_off;                        // first covalent term (batch order)
     int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };
@@ -67,6 +68,7 @@ enum Slot { SLOT_VV = 0, SLOT_GEN = 1, SLOT_LONG = 2, SLOT_EXCL = 3, SLOT_KIN = 
 // a force component at or beyond this (kJ mol^-1 nm^-1), or a non-finite one,
 // marks its job as diverged (the fixed-point conversions saturate at twice it)
 constexpr double FORCE_LIMIT = 33554432.0;   // 2^25
+constexpr int NHC_MAX = 16;                   // Nose-Hoover chain slots per job (GROMOS_AMD_NHC_MAX)
 
 // per step and job record: e_lj, e_crf, e_pot, e_kin, dhdl, vir[9],
 // e_bond, e_angle, e_dihedral, e_improper

@@ -132,8 +132,13 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.dist_oh = T.dist_oh;
     J.dist_hh = T.dist_hh;
     J.stop = INT32_MAX;
-    if (P.thermostat != 0 && P.thermostat != 1) return fail(GROMOS_AMD_EINVAL, "job %d: thermostat must be 0 or 1", j);
+    static_assert(NHC_MAX == GROMOS_AMD_NHC_MAX, "chain slots per job");
+    if (P.thermostat < 0 || P.thermostat > 2) return fail(GROMOS_AMD_EINVAL, "job %d: thermostat must be 0, 1 or 2", j);
+    if (P.thermostat == 2 && (P.nhc < 1 || P.nhc > GROMOS_AMD_NHC_MAX))
+        return fail(GROMOS_AMD_EINVAL, "job %d: Nose-Hoover chain length %d outside 1..%d", j, P.nhc,
+                    GROMOS_AMD_NHC_MAX);
     J.thermo = P.thermostat;
+    J.nhc = P.thermostat == 2 ? P.nhc : 0;
     J.t_ref = P.t_ref;
     J.tau_t = P.tau_t;
     J.dof_t = P.dof_t;
@@ -667,6 +672,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_settle_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_force_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_nh_zeta, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
+        (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
@@ -694,6 +701,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipMemsetAsync(S->d_xf, 0, npa * sizeof(float4), st));
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_vv_near, 0, nc * sizeof(int32_t), st));
+    // Nose-Hoover chains start at zero (nosehoover_thermostat.rs:64-113); a
+    // session puts continuing jobs' chains back after the layout
+    HIPCHK(hipMemsetAsync(S->d_nh_zeta, 0, (size_t)GROMOS_AMD_NHC_MAX * njobs * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_fi_near, 0, n3 * sizeof(long long), st));   // padding slots stay zero
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step_done, 0, sizeof(uint32_t), st));
@@ -1366,6 +1376,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.settle_fail = d_settle_fail;
     p.force_fail = d_force_fail;
     p.therm_scale = d_therm_scale;
+    p.nh_zeta = d_nh_zeta;
+    p.nh_pend = d_nh_pend;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -1995,6 +1995,44 @@ __device__ __forceinline__ double berendsen_scale(const JobDev& J, double ekin) 
     return sqrt(1.0 + J.dt / J.tau_t * (J.t_ref / free_temp - 1.0));
 }
 
+// NoseHooverThermostat::apply (nosehoover_thermostat.rs:116-227) for one job:
+// the chain advanced from its last committed state `z` with E_kin_new into
+// `pend` (k_integrate commits it when it applies the scale), and the velocity
+// scale, 1 where the reference leaves the velocities untouched
+__device__ __forceinline__ double nosehoover_scale(const JobDev& J, const double* z, double* pend, double ekin) {
+    const int n = J.nhc;
+    for (int i = 0; i < n; ++i) pend[i] = z[i];
+    if (J.tau_t < 0.0) return 1.0;   // no coupling
+    const double kB = 0.00831441;    // units.rs:47
+    const double dt = J.dt, tau = J.tau_t, dof = J.dof_t, t0 = J.t_ref;
+    double free_temp = dof > 0.0 ? 2.0 * ekin / (dof * kB) : 0.0;
+    if (free_temp < 2.220446049250313e-16) free_temp = t0;   // f64::EPSILON
+    if (n == 1) {
+        pend[0] = z[0] + dt / (tau * tau) * (free_temp / t0 - 1.0);
+    } else {
+        // tail to head; tau_vec[0] = tau^2, tau_vec[i > 0] = tau^2 / dof
+        const double tau2 = tau * tau, tv1 = tau2 / dof;
+        {
+            const double zz = z[n - 2];
+            pend[n - 1] = z[n - 1] + ((n - 2 == 0 ? tau2 : tv1) * zz * zz - 1.0 / dof) / tv1 * dt;
+        }
+        for (int i = n - 2; i >= 1; --i) {
+            const double zp = z[i - 1];
+            pend[i] = z[i] + (((i - 1 == 0 ? tau2 : tv1) * zp * zp - 1.0 / dof) / tv1 - z[i] * pend[i + 1]) * dt;
+        }
+        pend[0] = z[0] + ((free_temp / t0 - 1.0) / tau2 - pend[1] * z[0]) * dt;
+    }
+    const double scale = 1.0 - pend[0] * dt;
+    return fabs(scale - 1.0) < 1e-15 ? 1.0 : scale;
+}
+
+__device__ __forceinline__ double bath_scale(const SysPtrs& p, const JobDev& J, int32_t jb, double ekin) {
+    if (J.thermo == 2)
+        return nosehoover_scale(J, p.nh_zeta + (int64_t)NHC_MAX * jb,
+                                p.nh_pend + (int64_t)NHC_MAX * jb, ekin);
+    return berendsen_scale(J, ekin);
+}
+
 struct v3 {
     double x, y, z;
 };
@@ -2080,10 +2118,13 @@ __device__ __forceinline__ bool settle_one(const JobDev& J, const v3 (&po)[3], v
     return true;
 }
 
-// Berendsen scale of each coupled job from E_kin_new = 0.5 sum m |v|^2 of the
-// velocities entering the step (what the previous TemperatureCalculation
-// stored, temperature.rs:76-85; berendsen_thermostat.rs:66-100).  k_reduce sets
-// it after every step; k_therm, block per job, after velocities changed outside.
+// The bath's scale of each coupled job (Berendsen, berendsen_thermostat.rs:66-100,
+// or Nose-Hoover, nosehoover_thermostat.rs:116-227) from E_kin_new = 0.5 sum m
+// |v|^2 of the velocities entering the step (what the previous
+// TemperatureCalculation stored, temperature.rs:76-85).  k_reduce sets it after
+// every step; k_therm, block per job, after velocities changed outside (a
+// Nose-Hoover chain advances from its committed state either way, so it moves
+// once per step).
 __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     __shared__ double lds[17];
     const JobDev& J = p.jobs[blockIdx.x];
@@ -2098,11 +2139,11 @@ __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     }
     double t[1] = {s};
     block_sum<1>(t, lds);
-    if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = berendsen_scale(J, lds[16]);
+    if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = bath_scale(p, J, blockIdx.x, lds[16]);
 }
 
 // One md step's update of one original charge group (thread per group): the
-// total force (k_assemble), LeapFrogVelocity with the Berendsen scale and
+// total force (k_assemble), LeapFrogVelocity with the bath's scale and
 // LeapFrogPosition (k_leapfrog), SETTLE for rigid waters (k_settle), the E_kin
 // terms per group (k_ekin), and - when the next step keeps the pair list - the
 // sorted copies xq / xf it reads (k_gather).  A frozen job keeps its state.
@@ -2139,6 +2180,12 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
     const bool solute_group = c - J.cg_off < J.n_solute_cg;
     const double dt = J.dt;
     const double lam = J.thermo ? p.therm_scale[jb] : 1.0;
+    if (J.thermo == 2 && !frozen && c == J.cg_off) {
+        // this step applies the chain its scale came from: commit it
+        const double* pd = p.nh_pend + (int64_t)NHC_MAX * jb;
+        double* zd = p.nh_zeta + (int64_t)NHC_MAX * jb;
+        for (int i = 0; i < J.nhc; ++i) zd[i] = pd[i];
+    }
     double ek = 0.0, ekn = 0.0;
     auto step_atom = [&](int32_t i, v3& xo, v3& xn, v3& vo, v3& vn) {
         xo = ldv(p.x_cur, i);
@@ -2342,7 +2389,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(SysPtrs p, int32_t npad, int32
         for (int k = 0; k < 4; ++k) rec[14 + k] = t[14 + k];
         // the next step's Berendsen scale from this step's E_kin_new: what the
         // reference reads after TemperatureCalculation stored it
-        if (with_kin && J.thermo) p.therm_scale[jb] = berendsen_scale(J, t[13]);
+        if (with_kin && J.thermo) p.therm_scale[jb] = bath_scale(p, J, jb, t[13]);
         if (advance) {
             // every block read the counter at its start: the last one to finish
             // advances it for the next step's kernels and rearms the count

@@ -89,7 +89,9 @@ struct SysPtrs {
     double* records;
     int32_t* settle_fail;      // per job: first failing step
     int32_t* force_fail;       // per job: first step with a non-finite or out-of-range force
-    double* therm_scale;       // per job: Berendsen velocity scale of the current step
+    double* therm_scale;       // per job: the bath's velocity scale of the current step
+    double* nh_zeta;           // per job: Nose-Hoover chain (GROMOS_AMD_NHC_MAX), as of the last step
+    double* nh_pend;           // per job: the chain the current step's scale came from
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;

@@ -306,7 +306,8 @@ class GromosInput:
     ntcs: str = "shake"                    # solvent constraints algorithm
     ntf: Tuple[int, int, int, int] = (1, 1, 1, 1)   # FORCE: bonds, angles, impropers, dihedrals on
     baths: List[Tuple[float, float]] = field(default_factory=list)   # (TEMP0, TAU)
-    coupling: str = ""
+    coupling: str = ""                     # weak-coupling, nose-hoover, nose-hoover-chains
+    nhc: int = 1                           # nose-hoover-chains: the chain length
     ntg: int = 0                           # perturbation on
     rlam: float = 0.0
     nlam: int = 1
@@ -359,7 +360,10 @@ def read_imd(path):
         d = _data(blocks["MULTIBATH"])
         k = 0
         if not d[0].split()[0].lstrip("-").isdigit():
-            m.coupling = d[0].split()[0].lower()
+            w = d[0].split()
+            m.coupling = w[0].lower()
+            if m.coupling == "nose-hoover-chains" and len(w) > 1:
+                m.nhc = int(w[1])
             k = 1
         nb = int(d[k].split()[0])
         vals = " ".join(d[k + 1:]).split()
@@ -643,7 +647,7 @@ def write_configuration(path, system, pos, vel, title="final configuration", ntb
 # ---------------------------------------------------------------------------
 def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, nsteps=None):
     """gromos-rs md for the systems this library covers (nonbonded forces, the
-    solute's covalent terms, SETTLE waters, optional Berendsen bath and
+    solute's covalent terms, SETTLE waters, an optional Berendsen or Nose-Hoover bath and
     perturbation of the nonbonded terms): reads the files, runs NSTLIM (or
     nsteps) md steps on the device, and writes out_prefix.tre (every step, or
     every NTWE), out_prefix.trf (the step-0 forces) and out_prefix.cnf.
@@ -655,8 +659,14 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
     precision = rt.FP64 if precision is None else precision
     n = inp.nstlim if nsteps is None else int(nsteps)
     if inp.baths and inp.baths[0][1] > 0.0:
+        # md.rs:1143-1206: weak-coupling -> Berendsen, nose-hoover -> one zeta,
+        # nose-hoover-chains N -> a chain of N; one bath over all atoms
         dof = 3 * s.n_atoms - 3 * s.nsm - inp.ndfmin
         s.thermostat = (inp.baths[0][0], inp.baths[0][1], dof)
+        if inp.coupling.startswith("nose-hoover"):
+            s.thermostat_nhc = inp.nhc if inp.coupling == "nose-hoover-chains" else 1
+        elif inp.coupling not in ("", "weak-coupling", "berendsen"):
+            raise GromosFormatError(f"MULTIBATH coupling {inp.coupling!r} is not provided")
     # step-0 forces (a separate single point: the md run keeps its own rebuild cadence)
     ds = rt.DeviceSystem(s, precision=precision)
     ds.pairlist_update()

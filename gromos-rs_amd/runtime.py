@@ -78,7 +78,8 @@ class Params(C.Structure):
     _fields_ = [("boundary", I32), ("box", D * 3), ("rcutp", D), ("rcutl", D), ("skin", D),
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
                 ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
-                ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9)]
+                ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9),
+                ("nhc", I32)]
 
 
 def built():
@@ -288,7 +289,9 @@ def params_struct(s, use_settle=True, do_virial=False):
     p.nlam = s.pert.nlam if s.pert is not None else 1
     th = getattr(s, "thermostat", None)
     if th is not None:
-        p.thermostat = 1
+        nhc = int(getattr(s, "thermostat_nhc", 0))
+        p.thermostat = 2 if nhc > 0 else 1     # Nose-Hoover chain / Berendsen
+        p.nhc = nhc
         p.t_ref, p.tau_t, p.dof_t = (float(x) for x in th)
     return p
 

@@ -319,17 +319,22 @@ typedef struct {
     int32_t do_virial;
     double lambda;
     int32_t nlam;
-    /* Berendsen weak coupling over the job's atoms (berendsen_thermostat.rs):
-     * 0 = none (NVE), 1 = Berendsen with reference temperature t_ref (K),
-     * coupling time tau_t (ps; 0 = instantaneous, < 0 = no coupling) and dof_t
-     * degrees of freedom (md.rs:1164-1172: 3N - constraints - NDFMIN) */
+    /* one bath over the job's atoms: 0 = none (NVE), 1 = Berendsen weak
+     * coupling (berendsen_thermostat.rs), 2 = Nose-Hoover chain of length nhc
+     * (nosehoover_thermostat.rs; nhc 1 is the single Nose-Hoover thermostat);
+     * reference temperature t_ref (K), coupling time tau_t (ps; < 0 = no
+     * coupling, Berendsen: 0 = instantaneous) and dof_t degrees of freedom
+     * (md.rs:1152-1172: 3N - constraints - NDFMIN) */
     int32_t thermostat;
     double t_ref, tau_t, dof_t;
     /* boundary 2 (triclinic, Triclinic in math.rs:75-141): the box vectors
      * a, b, c (9 doubles, lower triangular: a = (ax,0,0), b = (bx,by,0));
      * `box` is not read then */
     double box_matrix[9];
+    int32_t nhc;                    /* thermostat 2: chain length, 1 .. GROMOS_AMD_NHC_MAX */
 } gromos_amd_params;
+
+#define GROMOS_AMD_NHC_MAX 16
 
 enum gromos_amd_precision {
     GROMOS_AMD_FP64 = 0,   /* plain: FP64 throughout, reference-exact pair list */

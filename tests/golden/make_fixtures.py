@@ -443,6 +443,10 @@ INPUT_SETS = {
     "aladip_vacuum": ["../shared/aladip.topo", "aladip_vacuum.conf", "aladip_vacuum.in"],
     "butane_vacuum": ["butane_vacuum.topo", "butane_vacuum.conf", "butane_vacuum.in"],
     "benzene_vacuum": ["benzene_vacuum.topo", "benzene_vacuum.conf", "benzene_vacuum.in"],
+    "water_216_nvt_nosehoover": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+                                 "water_216_nvt_nosehoover.in"],
+    "water_216_nvt_nhc_chain": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+                                "water_216_nvt_nhc_chain.in"],
 }
 
 

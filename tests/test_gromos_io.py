@@ -93,6 +93,12 @@ def test_imd_fields():
     _, _, imd, _ = files("aladip_trunc_oct")
     m = gio.read_imd(imd)
     assert m.ntb == -1 and m.nstlim == 100
+    _, _, imd, _ = files("water_216_nvt_nhc_chain")
+    m = gio.read_imd(imd)
+    assert m.coupling == "nose-hoover-chains" and m.nhc == 3 and m.baths == [(300.0, 0.1)]
+    _, _, imd, _ = files("water_216_nvt_nosehoover")
+    m = gio.read_imd(imd)
+    assert m.coupling == "nose-hoover" and m.baths == [(300.0, 0.1)] and m.nstlim == 20
 
 
 def test_energy_trajectory_round_trip(tmp_path):

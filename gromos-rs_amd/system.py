@@ -124,6 +124,11 @@ class System:
     pert: Optional[Perturbation] = None
     thermostat: Optional[tuple] = None   # one bath: (T_ref K, tau ps, degrees of freedom)
     thermostat_nhc: int = 0              # 0: Berendsen; N >= 1: Nose-Hoover chain of length N
+    # centre-of-mass motion removal (remove_com_motion.rs): NTICOM 1 / 2 removes the
+    # translation / translation and rotation at step 0; NSCM > 0 the translation
+    # every NSCM steps, NSCM < 0 translation and (vacuum only) rotation every |NSCM|
+    nticom: int = 0
+    nscm: int = 0
     name: str = "system"
     # triclinic box (boundary 2): f64 [3, 3], columns a, b, c in the lower-triangular
     # GROMOS convention a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz)

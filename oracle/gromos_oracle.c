@@ -1507,6 +1507,8 @@ struct or_md {
     double t_ref, tau_t, dof_t;
     int nhc;                    /* Nose-Hoover chain length */
     double zeta[OR_NHC_MAX];    /* the chain's state, zero at start */
+    int nticom, nscm;           /* centre-of-mass motion removal (INITIALISE NTICOM, COMTRANSROT NSCM) */
+    int64_t step;               /* md steps done (SimulationState::step) */
     double *pos[2], *vel[2], *force[2];
     double *inv_mass;
     int cur;
@@ -1610,15 +1612,117 @@ void or_md_get_nosehoover(const or_md *md, double *zeta) {
     for (int i = 0; i < md->nhc; ++i) zeta[i] = md->zeta[i];
 }
 
+/* RemoveCOMMotion::new (remove_com_motion.rs:62-85): NTICOM >= 1 removes the
+ * translation at step 0, >= 2 the rotation too; NSCM > 0 removes the
+ * translation every NSCM steps, NSCM < 0 translation and rotation every |NSCM| */
+void or_md_set_com(or_md *md, int nticom, int nscm) {
+    md->nticom = nticom;
+    md->nscm = nscm;
+}
+
+/* remove_com_translation (remove_com_motion.rs:90-113): v -= sum m v / sum m */
+static void or_com_translation(int n, const double *mass, double *vel) {
+    double m_tot = 0.0, cv[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        m_tot += mass[i];
+        for (int a = 0; a < 3; ++a) cv[a] += mass[i] * vel[3 * i + a];
+    }
+    for (int a = 0; a < 3; ++a) cv[a] /= m_tot;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) vel[3 * i + a] -= cv[a];
+}
+
+/* remove_com_rotation (remove_com_motion.rs:129-240): positions of a rectangular
+ * box put into (-L/2, L/2] (put_into_box_rect, remove_com_motion.rs:115-123),
+ * taken back half a step (r - dt/2 v) about their centre of mass; the angular
+ * velocity omega = I^-1 L (adjugate over the determinant, zero below 1e-12)
+ * and v -= omega x r */
+static void or_com_rotation(int n, const double *mass, const double *pos, double *vel, const or_pbc *pbc,
+                            double dt) {
+    const int wrap = pbc->kind == 1;
+    double m_tot = 0.0, cv[3] = {0.0, 0.0, 0.0}, cr[3] = {0.0, 0.0, 0.0};
+#define OR_WRAP(i, a) (wrap ? pos[3 * (i) + (a)] - round(pos[3 * (i) + (a)] / pbc->L[a]) * pbc->L[a] : pos[3 * (i) + (a)])
+    for (int i = 0; i < n; ++i) {
+        const double m = mass[i];
+        m_tot += m;
+        for (int a = 0; a < 3; ++a) {
+            cv[a] += m * vel[3 * i + a];
+            cr[a] += m * OR_WRAP(i, a) - 0.5 * m * vel[3 * i + a] * dt;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        cv[a] /= m_tot;
+        cr[a] /= m_tot;
+    }
+    double L[3] = {0.0, 0.0, 0.0};
+    double i00 = 0.0, i11 = 0.0, i22 = 0.0, i01 = 0.0, i02 = 0.0, i12 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double m = mass[i];
+        double r[3], u[3];
+        for (int a = 0; a < 3; ++a) {
+            r[a] = OR_WRAP(i, a) - 0.5 * dt * vel[3 * i + a] - cr[a];
+            u[a] = vel[3 * i + a] - cv[a];
+        }
+        L[0] += m * (r[1] * u[2] - r[2] * u[1]);
+        L[1] += m * (r[2] * u[0] - r[0] * u[2]);
+        L[2] += m * (r[0] * u[1] - r[1] * u[0]);
+        i00 += m * (r[1] * r[1] + r[2] * r[2]);
+        i11 += m * (r[0] * r[0] + r[2] * r[2]);
+        i22 += m * (r[0] * r[0] + r[1] * r[1]);
+        i01 -= m * r[0] * r[1];
+        i02 -= m * r[0] * r[2];
+        i12 -= m * r[1] * r[2];
+    }
+    const double den = -i02 * i02 * i11 + 2.0 * i01 * i02 * i12 - i00 * i12 * i12 - i01 * i01 * i22 + i00 * i11 * i22;
+    const double a00 = -i12 * i12 + i11 * i22, a01 = i02 * i12 - i01 * i22, a02 = -i02 * i11 + i01 * i12;
+    const double a11 = -i02 * i02 + i00 * i22, a12 = i01 * i02 - i00 * i12, a22 = -i01 * i01 + i00 * i11;
+    double w[3] = {0.0, 0.0, 0.0};
+    if (!(den < 1e-12)) {
+        w[0] = (a00 * L[0] + a01 * L[1] + a02 * L[2]) / den;
+        w[1] = (a01 * L[0] + a11 * L[1] + a12 * L[2]) / den;
+        w[2] = (a02 * L[0] + a12 * L[1] + a22 * L[2]) / den;
+    }
+    for (int i = 0; i < n; ++i) {
+        double r[3];
+        for (int a = 0; a < 3; ++a) r[a] = OR_WRAP(i, a) - 0.5 * dt * vel[3 * i + a] - cr[a];
+        vel[3 * i] -= w[1] * r[2] - w[2] * r[1];
+        vel[3 * i + 1] -= w[2] * r[0] - w[0] * r[2];
+        vel[3 * i + 2] -= w[0] * r[1] - w[1] * r[0];
+    }
+#undef OR_WRAP
+}
+
+/* RemoveCOMMotion::apply (remove_com_motion.rs:246-313) at md step `step` */
+void or_remove_com(int n, const double *mass, const double *pos, double *vel, const or_pbc *pbc, double dt,
+                   int nticom, int nscm, int64_t step) {
+    int trans, rot;
+    const int64_t skip = nscm > 0 ? nscm : -(int64_t)nscm;
+    if (step == 0) {
+        trans = nticom >= 1;
+        rot = nticom >= 2;
+    } else if (skip > 0 && step % skip == 0) {
+        trans = 1;
+        rot = nscm < 0 && pbc->kind == 0;   /* periodic boxes: no periodic rotation removal */
+    } else {
+        return;
+    }
+    if (trans) or_com_translation(n, mass, vel);
+    if (rot) or_com_rotation(n, mass, pos, vel, pbc, dt);
+}
+
 /* BerendsenThermostat::apply (berendsen_thermostat.rs:61-100): the scale from
  * E_kin_new = 0.5 sum m |v|^2 of the velocities entering the step (stored by the
  * previous TemperatureCalculation, temperature.rs:76-85, or by its init) */
 double or_berendsen_scale(int n, const double *mass, const double *vel, double t_ref, double tau, double dof,
                           double dt) {
-    if (tau < 0.0) return 1.0;
     double ekin = 0.0;
     for (int i = 0; i < n; ++i)
         ekin += 0.5 * mass[i] * (vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] + vel[3 * i + 2] * vel[3 * i + 2]);
+    return or_berendsen_scale_ekin(ekin, t_ref, tau, dof, dt);
+}
+
+double or_berendsen_scale_ekin(double ekin, double t_ref, double tau, double dof, double dt) {
+    if (tau < 0.0) return 1.0;
     double free_temp = dof > 0.0 ? 2.0 * ekin / (dof * OR_KB) : 0.0;
     if (free_temp < 1e-10) free_temp = t_ref;
     if (free_temp < 1e-10) return 1.0;
@@ -1631,22 +1735,26 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
     for (int s = 0; s < nsteps; ++s) {
         int c = md->cur, o = 1 - c;
         double elj, ecrf, dh;
+        /* the bath reads E_kin_new stored by the previous step, before this
+         * step's centre-of-mass removal (md.rs:1095-1097 runs that first) */
+        double ek_bath = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double *v = md->vel[c] + 3 * i;
+            ek_bath += 0.5 * md->t->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        if (md->nticom >= 1 || md->nscm != 0)
+            or_remove_com(n, md->t->mass, md->pos[c], md->vel[c], &md->ff->pbc, md->dt, md->nticom, md->nscm,
+                          md->step);
         or_ff_apply(md->ff, md->pos[c], md->force[c], &elj, &ecrf, NULL, &dh);
         md->pairs += or_ff_last_pair_count(md->ff);
         /* exchange_state: the current buffer becomes old */
         if (md->thermo) {
             /* LeapFrogVelocity, thermostat scaling, LeapFrogPosition (md.rs:1143-1206) */
             double lam;
-            if (md->thermo == 1) {
-                lam = or_berendsen_scale(n, md->t->mass, md->vel[c], md->t_ref, md->tau_t, md->dof_t, md->dt);
-            } else {
-                double ek_new = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    const double *v = md->vel[c] + 3 * i;
-                    ek_new += 0.5 * md->t->mass[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                }
-                lam = or_nosehoover_step(md->zeta, md->nhc, ek_new, md->t_ref, md->tau_t, md->dof_t, md->dt);
-            }
+            if (md->thermo == 1)
+                lam = or_berendsen_scale_ekin(ek_bath, md->t_ref, md->tau_t, md->dof_t, md->dt);
+            else
+                lam = or_nosehoover_step(md->zeta, md->nhc, ek_bath, md->t_ref, md->tau_t, md->dof_t, md->dt);
             for (int i = 0; i < n; ++i)
                 for (int a = 0; a < 3; ++a) {
                     double acc = md->force[c][3 * i + a] * md->inv_mass[i];
@@ -1675,6 +1783,7 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
             energies[5 * s + 4] = dh;
         }
         md->cur = o;
+        md->step++;
     }
     return 0;
 }

@@ -283,6 +283,11 @@ void or_md_set_berendsen(or_md *md, double t_ref, double tau, double dof);
 int or_md_set_nosehoover(or_md *md, double t_ref, double tau, double dof, int nhc);
 double or_nosehoover_step(double *zeta, int nhc, double ekin, double t_ref, double tau, double dof, double dt);
 void or_md_get_nosehoover(const or_md *md, double *zeta);
+double or_berendsen_scale_ekin(double ekin, double t_ref, double tau, double dof, double dt);
+/* centre-of-mass motion removal (remove_com_motion.rs) at the start of each md step */
+void or_md_set_com(or_md *md, int nticom, int nscm);
+void or_remove_com(int n, const double *mass, const double *pos, double *vel, const or_pbc *pbc, double dt,
+                   int nticom, int nscm, int64_t step);
 double or_berendsen_scale(int n, const double *mass, const double *vel, double t_ref, double tau, double dof,
                           double dt);
 void or_md_get(const or_md *md, double *pos, double *vel, double *force);

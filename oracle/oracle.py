@@ -162,6 +162,8 @@ def _declare(L):
     L.or_nosehoover_step.argtypes = [P, C.c_int, D, D, D, D, D]
     L.or_nosehoover_step.restype = D
     L.or_md_get_nosehoover.argtypes = [P, P]
+    L.or_md_set_com.argtypes = [P, C.c_int, C.c_int]
+    L.or_remove_com.argtypes = [C.c_int, P, P, P, C.POINTER(Pbc), D, C.c_int, C.c_int, C.c_int64]
     L.or_berendsen_scale.argtypes = [C.c_int, P, P, D, D, D, D]
     L.or_berendsen_scale.restype = D
     L.or_md_run.argtypes = [P, C.c_int, P]
@@ -521,6 +523,8 @@ class MD:
                 self.nhc = nhc
             else:
                 lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
+        if getattr(s, "nticom", 0) or getattr(s, "nscm", 0):
+            lib().or_md_set_com(self.h, int(s.nticom), int(s.nscm))
         self.n = s.n_atoms
 
     def run(self, nsteps):
@@ -550,6 +554,16 @@ class MD:
             lib().or_md_destroy(self.h)
         except Exception:
             pass
+
+
+def remove_com(osys, pos, vel, nticom, nscm, step):
+    """RemoveCOMMotion::apply at md step `step` on copies of pos / vel; returns
+    the new velocities"""
+    x, v = np.array(pos, np.float64), np.array(vel, np.float64)
+    m = c64(osys.s.mass)
+    lib().or_remove_com(len(m), ptr(m), ptr(x), ptr(v), C.byref(osys.pbc), osys.s.dt, int(nticom), int(nscm),
+                        int(step))
+    return v.reshape(-1, 3)
 
 
 def nosehoover_step(zeta, ekin, t_ref, tau, dof, dt):

@@ -58,6 +58,10 @@ SYSTEMS = [
      None, None, "all"),
     ("water_216_nvt_nhc_chain", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
      None, None, "all"),
+    ("water_216_box_com", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+     None, None, "all"),
+    ("water_216_box_com_rot", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+     None, None, "all"),
     ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
      "../shared/ch4_spc_dummy.ptp", "first"),
 ]
@@ -210,6 +214,12 @@ def parse_imd(path):
     bc = data_lines(b["BOUNDCOND"])[0].split()
     d["ntb"] = int(bc[0])
     d["ndfmin"] = int(bc[1]) if len(bc) > 1 else 0
+    # centre-of-mass motion removal: INITIALISE NTICOM (third value of its second
+    # row), COMTRANSROT NSCM
+    ini = data_lines(b.get("INITIALISE", []))
+    d["nticom"] = int(ini[1].split()[2]) if len(ini) > 1 else 0
+    ctr = data_lines(b.get("COMTRANSROT", []))
+    d["nscm"] = int(ctr[0].split()[0]) if ctr else 0
     # MULTIBATH (one bath): weak-coupling -> Berendsen, nose-hoover -> a chain of
     # one, nose-hoover-chains N -> a chain of N
     mb = data_lines(b.get("MULTIBATH", []))
@@ -384,6 +394,7 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
         algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
         dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"], ndfmin=imd["ndfmin"], thermostat=imd["thermostat"],
+        nticom=imd["nticom"], nscm=imd["nscm"],
         solvent_constr=t["solvent_constr"], pert=pert,
         has_bonded_terms=bool(t["bonds"] or t["angles"]),
         bonded=dict(bonds=t["bond_terms"], angles=t["angle_terms"], impropers=t["improper_terms"],
@@ -447,6 +458,8 @@ INPUT_SETS = {
                                  "water_216_nvt_nosehoover.in"],
     "water_216_nvt_nhc_chain": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
                                 "water_216_nvt_nhc_chain.in"],
+    "water_216_box_com_rot": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+                              "water_216_box_com_rot.in"],
 }
 
 

@@ -217,6 +217,9 @@ struct gromos_amd_system {
     double* d_nh_zeta = nullptr;   // per job: the Nose-Hoover chain (GROMOS_AMD_NHC_MAX each)
     double* d_nh_pend = nullptr;   // per job: the chain after the next step's update
     bool any_thermo = false;
+    bool any_com = false;                  // a job removes centre-of-mass motion
+    std::vector<int64_t> job_step0;        // per job: md steps done before the current call
+    int64_t* d_job_step0 = nullptr;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
     bool xq_valid = false;     // xq / xf hold the current positions in the current sorted order
     double* d_cg_ek = nullptr;

@@ -434,6 +434,7 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
             }
             if (changed)
                 if (int rc = upload_jobs(S)) return rc;
+            for (size_t b = 0; b < s->batch.size(); ++b) S->job_step0[b] = s->batch[b]->done;
             int64_t pc0 = S->pair_count;
             std::vector<int32_t> failed;
             std::vector<uint8_t> fkind;

@@ -56,6 +56,7 @@ struct JobDev {
     int32_t thermo;                        // 1 Berendsen (berendsen_thermostat.rs), 2 Nose-Hoover chain
     double t_ref, tau_t, dof_t;
     int32_t nhc;                           // Nose-Hoover chain length (nosehoover_thermostat.rs)
+    int32_t nticom, nscm;                  // centre-of-mass motion removal (remove_com_motion.rs)
     int32_t bd_off;                        // first covalent term (batch order)
     int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };

@@ -139,6 +139,9 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
                     GROMOS_AMD_NHC_MAX);
     J.thermo = P.thermostat;
     J.nhc = P.thermostat == 2 ? P.nhc : 0;
+    if (P.nticom < 0 || P.nticom > 2) return fail(GROMOS_AMD_EINVAL, "job %d: nticom must be 0, 1 or 2", j);
+    J.nticom = P.nticom;
+    J.nscm = P.nscm;
     J.t_ref = P.t_ref;
     J.tau_t = P.tau_t;
     J.dof_t = P.dof_t;
@@ -311,7 +314,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     };
     std::vector<Slice> sl(njobs + 1);
     int64_t cell_off = 0;
-    bool any_pert = false, twin = false, any_thermo = false;
+    bool any_pert = false, twin = false, any_thermo = false, any_com = false;
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -339,6 +342,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         any_pert = any_pert || im.has_pert;
         twin = twin || J.twin;
         any_thermo = any_thermo || J.thermo;
+        any_com = any_com || J.nticom >= 1 || J.nscm != 0;
         use_settle |= im.P.use_settle && !im.mol_first.empty();
         do_virial |= im.P.do_virial;
         S->jobs.push_back(J);
@@ -479,6 +483,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->do_virial = do_virial;
     S->twin = twin;
     S->any_thermo = any_thermo;
+    S->any_com = any_com;
+    S->job_step0.assign((size_t)njobs, 0);
     S->near_r = VV_NEAR_R;
     if (const char* env = std::getenv("GROMOS_AMD_VV_NEAR_R")) S->near_r = std::max(0.0, std::atof(env));
     // LJ only between the solvent molecules' first atoms (SPC-like waters)
@@ -673,6 +679,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_force_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_nh_zeta, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
+        (rc = S->reserve(S->d_job_step0, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -1138,6 +1145,9 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         if (int rc = step_event()) return rc;
     const int32_t mixed = S->precision == GROMOS_AMD_MIXED;
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));   // never inside a captured graph
+    if (S->any_com)
+        HIPCHK(hipMemcpyAsync(S->d_job_step0, S->job_step0.data(), S->job_step0.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice, st));
     // graph replay of the plain steps is opt-in (GROMOS_AMD_GRAPHS=1): on this
     // runtime it measured 1.5 % slower than separate launches (water_32768)
     const char* genv = std::getenv("GROMOS_AMD_GRAPHS");
@@ -1149,6 +1159,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // thermostat rescan, no per-kernel timing events): one captured graph
         const bool plain = S->use_graphs && S->built && S->steps_since_update + 1 < S->nsnb && S->xq_valid &&
                            !(S->any_thermo && S->therm_dirty) && !S->timing && !(S->twin && !S->long_computed) &&
+                           !S->any_com &&
                            S->vv_lanes > 0;
         if (plain) {
             S->steps_since_update++;
@@ -1167,6 +1178,9 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
             launch_therm(p, S->njobs, st);
             S->therm_dirty = false;
         }
+        // centre-of-mass motion removal (first in the reference's step, md.rs:1095-1097):
+        // on the velocities entering the step, after the bath read their E_kin
+        if (S->any_com) launch_com(p, S->njobs, st);
         // the integrator leaves the sorted copies for the next step unless that
         // step rebuilds the list (pairlist.rs:105-107)
         const int32_t write_xq = S->steps_since_update < S->nsnb ? 1 : 0;
@@ -1179,6 +1193,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     }
     S->md_step = 0;
     S->last_rec_row = nsteps - 1;
+    for (int32_t b = 0; b < S->njobs; ++b) S->job_step0[b] += std::min<int64_t>(nsteps, S->jobs[b].stop);
     HIPCHK(hipGetLastError());
     std::vector<int32_t> failed(S->njobs), ffail(S->njobs);
     HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1378,6 +1393,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.therm_scale = d_therm_scale;
     p.nh_zeta = d_nh_zeta;
     p.nh_pend = d_nh_pend;
+    p.job_step0 = d_job_step0;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

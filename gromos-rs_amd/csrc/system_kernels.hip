@@ -2142,6 +2142,115 @@ __global__ void __launch_bounds__(BLOCK) k_therm(SysPtrs p) {
     if (threadIdx.x == 0) p.therm_scale[blockIdx.x] = bath_scale(p, J, blockIdx.x, lds[16]);
 }
 
+// RemoveCOMMotion::apply (remove_com_motion.rs:246-313), first in the md step:
+// a block per job, at the job's step 0 (NTICOM) and every |NSCM| steps, on the
+// velocities entering the step.  Translation: v -= sum m v / sum m
+// (remove_com_motion.rs:90-113).  Rotation (remove_com_motion.rs:129-240): the
+// positions, a rectangular box's put into (-L/2, L/2] and taken back half a
+// step, about their centre of mass; omega = I^-1 L by the adjugate (zero when
+// the determinant is below 1e-12); v -= omega x r.
+__global__ void __launch_bounds__(BLOCK) k_com(SysPtrs p) {
+    __shared__ double lds[17 * 9];
+    const int32_t jb = blockIdx.x;
+    const JobDev& J = p.jobs[jb];
+    if (J.nticom < 1 && J.nscm == 0) return;
+    const int32_t step = *p.step;
+    if (step >= J.stop) return;
+    const int64_t js = p.job_step0[jb] + step;
+    const int64_t skip = J.nscm > 0 ? (int64_t)J.nscm : -(int64_t)J.nscm;
+    bool trans, rot;
+    if (js == 0) {
+        trans = J.nticom >= 1;
+        rot = J.nticom >= 2;
+    } else if (skip > 0 && js % skip == 0) {
+        trans = true;
+        rot = J.nscm < 0 && J.box.kind == 0;   // periodic boxes: no periodic rotation removal
+    } else {
+        return;
+    }
+    const int32_t a0 = J.atom_off, a1 = J.atom_off + J.n_atoms;
+    // the removal changes the step's entering velocities in place (the
+    // reference's conf.current().vel), before k_integrate reads them
+    double* v = const_cast<double*>(p.v_cur);
+    const double* x = p.x_cur;
+    if (trans) {
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int32_t a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+            const double m = p.mass[a];
+            t[0] += m;
+            for (int c = 0; c < 3; ++c) t[1 + c] += m * v[3 * (int64_t)a + c];
+        }
+        block_sum<4>(t, lds);
+        const double cv[3] = {lds[65] / lds[64], lds[66] / lds[64], lds[67] / lds[64]};
+        __syncthreads();
+        for (int32_t a = a0 + threadIdx.x; a < a1; a += blockDim.x)
+            for (int c = 0; c < 3; ++c) v[3 * (int64_t)a + c] -= cv[c];
+    }
+    if (!rot) return;
+    const bool wrap = J.box.kind == 1;
+    const double dt = J.dt;
+    auto wx = [&](int32_t a, int c) {
+        const double q = x[3 * (int64_t)a + c];
+        return wrap ? q - round(q / J.box.L[c]) * J.box.L[c] : q;
+    };
+    double t[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int32_t a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+        const double m = p.mass[a];
+        t[0] += m;
+        for (int c = 0; c < 3; ++c) {
+            const double vc = v[3 * (int64_t)a + c];
+            t[1 + c] += m * vc;
+            t[4 + c] += m * wx(a, c) - 0.5 * m * vc * dt;
+        }
+    }
+    block_sum<7>(t, lds);
+    double cv[3], cr[3];
+    for (int c = 0; c < 3; ++c) {
+        cv[c] = lds[112 + 1 + c] / lds[112];
+        cr[c] = lds[112 + 4 + c] / lds[112];
+    }
+    __syncthreads();
+    double u[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // L, I00 I11 I22 I01 I02 I12
+    for (int32_t a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+        const double m = p.mass[a];
+        double r[3], w[3];
+        for (int c = 0; c < 3; ++c) {
+            const double vc = v[3 * (int64_t)a + c];
+            r[c] = wx(a, c) - 0.5 * dt * vc - cr[c];
+            w[c] = vc - cv[c];
+        }
+        u[0] += m * (r[1] * w[2] - r[2] * w[1]);
+        u[1] += m * (r[2] * w[0] - r[0] * w[2]);
+        u[2] += m * (r[0] * w[1] - r[1] * w[0]);
+        u[3] += m * (r[1] * r[1] + r[2] * r[2]);
+        u[4] += m * (r[0] * r[0] + r[2] * r[2]);
+        u[5] += m * (r[0] * r[0] + r[1] * r[1]);
+        u[6] -= m * r[0] * r[1];
+        u[7] -= m * r[0] * r[2];
+        u[8] -= m * r[1] * r[2];
+    }
+    block_sum<9>(u, lds);
+    const double* q = lds + 144;
+    const double i00 = q[3], i11 = q[4], i22 = q[5], i01 = q[6], i02 = q[7], i12 = q[8];
+    const double den = -i02 * i02 * i11 + 2.0 * i01 * i02 * i12 - i00 * i12 * i12 - i01 * i01 * i22 + i00 * i11 * i22;
+    const double b00 = -i12 * i12 + i11 * i22, b01 = i02 * i12 - i01 * i22, b02 = -i02 * i11 + i01 * i12;
+    const double b11 = -i02 * i02 + i00 * i22, b12 = i01 * i02 - i00 * i12, b22 = -i01 * i01 + i00 * i11;
+    double om[3] = {0.0, 0.0, 0.0};
+    if (!(den < 1e-12)) {
+        om[0] = (b00 * q[0] + b01 * q[1] + b02 * q[2]) / den;
+        om[1] = (b01 * q[0] + b11 * q[1] + b12 * q[2]) / den;
+        om[2] = (b02 * q[0] + b12 * q[1] + b22 * q[2]) / den;
+    }
+    for (int32_t a = a0 + threadIdx.x; a < a1; a += blockDim.x) {
+        double r[3];
+        double* va = v + 3 * (int64_t)a;
+        for (int c = 0; c < 3; ++c) r[c] = wx(a, c) - 0.5 * dt * va[c] - cr[c];
+        va[0] -= om[1] * r[2] - om[2] * r[1];
+        va[1] -= om[2] * r[0] - om[0] * r[2];
+        va[2] -= om[0] * r[1] - om[1] * r[0];
+    }
+}
+
 // One md step's update of one original charge group (thread per group): the
 // total force (k_assemble), LeapFrogVelocity with the bath's scale and
 // LeapFrogPosition (k_leapfrog), SETTLE for rigid waters (k_settle), the E_kin
@@ -2610,6 +2719,9 @@ void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t 
     const dim3 g((unsigned)((ncg + WAVE - 1) / WAVE));
     if (mixed) hipLaunchKernelGGL(k_integrate<true>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
     else hipLaunchKernelGGL(k_integrate<false>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
+}
+void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_com, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_therm, dim3(njobs), dim3(BLOCK), 0, s, p);

@@ -92,6 +92,7 @@ struct SysPtrs {
     double* therm_scale;       // per job: the bath's velocity scale of the current step
     double* nh_zeta;           // per job: Nose-Hoover chain (GROMOS_AMD_NHC_MAX), as of the last step
     double* nh_pend;           // per job: the chain the current step's scale came from
+    const int64_t* job_step0;  // per job: its md steps done before the current call
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;
@@ -150,6 +151,7 @@ void launch_gen(const SysPtrs& p, int32_t npad, bool lng, int32_t virial, int32_
 void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, const int32_t* blocks, int32_t nblocks,
                    hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
+void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
 void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,

@@ -318,6 +318,7 @@ class GromosInput:
     ntwv: int = 0
     ntwf: int = 0
     nscm: int = 0
+    nticom: int = 0                        # INITIALISE: centre-of-mass motion removal at step 0
 
 
 _ALG = {"standard": 0, "0": 0, "grid": 1, "1": 1, "grid_cell": 2, "2": 2}
@@ -380,6 +381,10 @@ def read_imd(path):
             m.ntwv, m.ntwf, m.ntwe = v[2], v[3], v[4]
     if "COMTRANSROT" in blocks:
         m.nscm = int(_tokens(blocks["COMTRANSROT"])[0])
+    if "INITIALISE" in blocks:
+        v = _tokens(blocks["INITIALISE"])
+        if len(v) > 6:
+            m.nticom = int(v[6])     # NTIVEL NTISHK NTINHT NTINHB NTISHI NTIRTC NTICOM
     return m
 
 
@@ -667,6 +672,7 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
             s.thermostat_nhc = inp.nhc if inp.coupling == "nose-hoover-chains" else 1
         elif inp.coupling not in ("", "weak-coupling", "berendsen"):
             raise GromosFormatError(f"MULTIBATH coupling {inp.coupling!r} is not provided")
+    s.nticom, s.nscm = inp.nticom, inp.nscm      # md.rs:1095-1097
     # step-0 forces (a separate single point: the md run keeps its own rebuild cadence)
     ds = rt.DeviceSystem(s, precision=precision)
     ds.pairlist_update()

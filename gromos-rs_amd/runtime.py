@@ -79,7 +79,7 @@ class Params(C.Structure):
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
                 ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
                 ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9),
-                ("nhc", I32)]
+                ("nhc", I32), ("nticom", I32), ("nscm", I32)]
 
 
 def built():
@@ -293,6 +293,7 @@ def params_struct(s, use_settle=True, do_virial=False):
         p.thermostat = 2 if nhc > 0 else 1     # Nose-Hoover chain / Berendsen
         p.nhc = nhc
         p.t_ref, p.tau_t, p.dof_t = (float(x) for x in th)
+    p.nticom, p.nscm = int(getattr(s, "nticom", 0)), int(getattr(s, "nscm", 0))
     return p
 
 

@@ -332,6 +332,11 @@ typedef struct {
      * `box` is not read then */
     double box_matrix[9];
     int32_t nhc;                    /* thermostat 2: chain length, 1 .. GROMOS_AMD_NHC_MAX */
+    /* centre-of-mass motion removal at the start of md steps (remove_com_motion.rs):
+     * nticom 1 removes the translation at the job's step 0, 2 also the rotation;
+     * nscm > 0 removes the translation every nscm steps, nscm < 0 the
+     * translation and, without a periodic box, the rotation every |nscm| steps */
+    int32_t nticom, nscm;
 } gromos_amd_params;
 
 #define GROMOS_AMD_NHC_MAX 16

@@ -129,6 +129,13 @@ class System:
     # every NSCM steps, NSCM < 0 translation and (vacuum only) rotation every |NSCM|
     nticom: int = 0
     nscm: int = 0
+    # pressure coupling (berendsen_barostat.rs, pressure_calculation.rs), isotropic,
+    # rectangular boxes: (P0 kJ mol^-1 nm^-3, compressibility, tau_p ps, virial
+    # 0 none / 1 atomic / 2 molecular)
+    barostat: Optional[tuple] = None
+    # last atom of each solute pressure group (topology PRESSUREGROUPS); every
+    # solvent molecule is a group of its own
+    pressure_groups: Optional[np.ndarray] = None
     name: str = "system"
     # triclinic box (boundary 2): f64 [3, 3], columns a, b, c in the lower-triangular
     # GROMOS convention a = (ax, 0, 0), b = (bx, by, 0), c = (cx, cy, cz)
@@ -155,6 +162,14 @@ class System:
     def copy(self):
         import copy
         return copy.deepcopy(self)
+
+    def pressure_group_last(self):
+        """the last atom of every pressure group: the solute's groups, then each
+        solvent molecule (core topology.rs:1242-1263)"""
+        sol = [] if self.pressure_groups is None else [int(x) for x in self.pressure_groups]
+        aps = self.atoms_per_solvent
+        solv = list(range(self.n_solute_atoms + aps - 1, self.n_atoms, aps)) if aps > 0 else []
+        return np.array(sol + solv, np.int32)
 
     def validate(self):
         n = self.n_atoms

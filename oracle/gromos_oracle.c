@@ -1508,6 +1508,12 @@ struct or_md {
     int nhc;                    /* Nose-Hoover chain length */
     double zeta[OR_NHC_MAX];    /* the chain's state, zero at start */
     int nticom, nscm;           /* centre-of-mass motion removal (INITIALISE NTICOM, COMTRANSROT NSCM) */
+    int baro;                   /* Berendsen barostat, isotropic */
+    double p_ref, comp, tau_p;
+    int vtype;                  /* virial: 0 none, 1 atomic, 2 molecular */
+    int n_pg;                   /* pressure groups: atoms [pg_start[g], pg_start[g + 1]) */
+    int32_t *pg_start;
+    double pressure;            /* the last step's isotropic pressure */
     int64_t step;               /* md steps done (SimulationState::step) */
     double *pos[2], *vel[2], *force[2];
     double *inv_mass;
@@ -1547,6 +1553,7 @@ void or_md_set_settle_geometry(or_md *md, double d_oh, double d_hh) { md->d_oh =
 
 void or_md_destroy(or_md *md) {
     if (!md) return;
+    free(md->pg_start);
     or_ff_destroy(md->ff);
     for (int k = 0; k < 2; ++k) { free(md->pos[k]); free(md->vel[k]); free(md->force[k]); }
     free(md->inv_mass);
@@ -1610,6 +1617,107 @@ double or_nosehoover_step(double *zeta, int nhc, double ekin, double t_ref, doub
 
 void or_md_get_nosehoover(const or_md *md, double *zeta) {
     for (int i = 0; i < md->nhc; ++i) zeta[i] = md->zeta[i];
+}
+
+/* The periodic box changed (the barostat): Forcefield::apply's per-step box
+ * refresh (forcefield.rs:243-265: Rectangular::new, half box and reciprocal)
+ * and its CG-grouped predicate (forcefield.rs:267-283), rectangular boxes */
+void or_ff_set_box(or_ff *ff, const double L[3]) {
+    for (int a = 0; a < 3; ++a) {
+        ff->pbc.L[a] = L[a];
+        ff->pbc.invL[a] = 1.0 / L[a];
+        ff->pbc.half[a] = L[a] * 0.5;
+    }
+    double hm = ff->pbc.half[0];
+    if (ff->pbc.half[1] < hm) hm = ff->pbc.half[1];
+    if (ff->pbc.half[2] < hm) hm = ff->pbc.half[2];
+    ff->use_cg_grouped = ff->t->n_cg > 0 && (hm > ff->prm.rcutl + ff->prm.skin + 0.3);
+}
+
+/* pressure coupling (md.rs:1116-1129, 1259-1277): PressureCalculation
+ * (pressure_calculation.rs) on the atomic or molecular virial and
+ * BerendsenBarostat (berendsen_barostat.rs), isotropic, rectangular boxes;
+ * pg_last: the last atom of each pressure group (solute groups, then every
+ * solvent molecule) */
+int or_md_set_barostat(or_md *md, double p_ref, double comp, double tau_p, int vtype, int n_pg,
+                       const int32_t *pg_last) {
+    if (md->ff->pbc.kind != 1 || vtype < 0 || vtype > 2 || n_pg < 0) return -1;
+    md->baro = 1;
+    md->p_ref = p_ref;
+    md->comp = comp;
+    md->tau_p = tau_p;
+    md->vtype = vtype;
+    md->n_pg = n_pg;
+    free(md->pg_start);
+    md->pg_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_pg + 1));
+    md->pg_start[0] = 0;
+    for (int g = 0; g < n_pg; ++g) md->pg_start[g + 1] = pg_last[g] + 1;
+    md->ff->prm.virial = 1;
+    return 0;
+}
+
+double or_md_pressure(const or_md *md) { return md->pressure; }
+void or_md_box(const or_md *md, double *L) {
+    for (int a = 0; a < 3; ++a) L[a] = md->ff->pbc.L[a];
+}
+
+/* prepare_virial's kinetic energy tensor (forcefield.rs:285-296, 1064-1120):
+ * molecular, 0.5 (sum m v)(sum m v)^T / M per pressure group, or atomic,
+ * 0.5 m v v^T per atom; ke[3 a + b] */
+static void or_ke_tensor(const or_md *md, const double *vel, double *ke) {
+    const double *mass = md->t->mass;
+    for (int k = 0; k < 9; ++k) ke[k] = 0.0;
+    if (md->vtype == 2 && md->n_pg > 0) {
+        for (int g = 0; g < md->n_pg; ++g) {
+            double mv[3] = {0.0, 0.0, 0.0}, m_tot = 0.0;
+            for (int i = md->pg_start[g]; i < md->pg_start[g + 1]; ++i) {
+                m_tot += mass[i];
+                for (int a = 0; a < 3; ++a) mv[a] += mass[i] * vel[3 * i + a];
+            }
+            if (m_tot > 0.0)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) ke[3 * a + b] += 0.5 * mv[a] * mv[b] / m_tot;
+        }
+    } else {
+        for (int k = 0; k < md->t->n_atoms; ++k)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) ke[3 * a + b] += 0.5 * mass[k] * vel[3 * k + a] * vel[3 * k + b];
+    }
+}
+
+/* apply_molecular_virial_correction (forcefield.rs:1128-1200): per pressure
+ * group the centre of mass of the chain-gathered positions (each atom's image
+ * nearest the previous gathered one), then vir[b][a] -= f_i[a] r_i[b] with r_i
+ * the image of atom i nearest that centre */
+static void or_molecular_virial(const or_md *md, const double *pos, const double *force, double *vir) {
+    const double *mass = md->t->mass;
+    const or_pbc *pbc = &md->ff->pbc;
+    double corr[9] = {0.0};
+    for (int g = 0; g < md->n_pg; ++g) {
+        const int i0 = md->pg_start[g], i1 = md->pg_start[g + 1];
+        double com[3] = {0.0, 0.0, 0.0}, m_tot = 0.0, prev[3];
+        for (int a = 0; a < 3; ++a) prev[a] = pos[3 * i0 + a];
+        for (int i = i0; i < i1; ++i) {
+            const double m = mass[i];
+            double p[3], gathered[3];
+            m_tot += m;
+            or_nearest_image(pbc, pos + 3 * i, prev, p);
+            for (int a = 0; a < 3; ++a) {
+                gathered[a] = p[a] + prev[a];
+                com[a] += m * gathered[a];
+                prev[a] = gathered[a];
+            }
+        }
+        if (m_tot > 0.0)
+            for (int a = 0; a < 3; ++a) com[a] /= m_tot;
+        for (int i = i0; i < i1; ++i) {
+            double r[3];
+            or_nearest_image(pbc, pos + 3 * i, com, r);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) corr[3 * b + a] += force[3 * i + a] * r[b];
+        }
+    }
+    for (int k = 0; k < 9; ++k) vir[k] = vir[k] - corr[k];
 }
 
 /* RemoveCOMMotion::new (remove_com_motion.rs:62-85): NTICOM >= 1 removes the
@@ -1745,7 +1853,12 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
         if (md->nticom >= 1 || md->nscm != 0)
             or_remove_com(n, md->t->mass, md->pos[c], md->vel[c], &md->ff->pbc, md->dt, md->nticom, md->nscm,
                           md->step);
-        or_ff_apply(md->ff, md->pos[c], md->force[c], &elj, &ecrf, NULL, &dh);
+        /* prepare_virial: the kinetic energy tensor of the velocities entering
+         * the step, before the forces (forcefield.rs:285-296) */
+        double ke[9], vir[9];
+        if (md->baro) or_ke_tensor(md, md->vel[c], ke);
+        or_ff_apply(md->ff, md->pos[c], md->force[c], &elj, &ecrf, md->baro ? vir : NULL, &dh);
+        if (md->baro && md->vtype == 2 && md->n_pg > 0) or_molecular_virial(md, md->pos[c], md->force[c], vir);
         md->pairs += or_ff_last_pair_count(md->ff);
         /* exchange_state: the current buffer becomes old */
         if (md->thermo) {
@@ -1771,6 +1884,24 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
             if (err) return -1;
         }
         double ek = or_ekin(n, md->t->mass, md->vel[o], md->vel[c]);
+        if (md->baro) {
+            /* PressureCalculation (pressure_calculation.rs:40-80) on this step's
+             * tensors and box: P = (KE + vir / 2) * 2 / V; BerendsenBarostat
+             * (berendsen_barostat.rs:30-70): mu = (1 - comp dt / tau (P0 - P))^(1/3)
+             * scales the box and the new positions */
+            const double *L = md->ff->pbc.L;
+            const double V = L[0] * L[1] * L[2];
+            if (md->vtype != 0 && !(V < 1e-30)) {
+                const double factor = 2.0 / V;
+                double pt[3];
+                for (int a = 0; a < 3; ++a) pt[a] = (ke[4 * a] + vir[4 * a] * 0.5) * factor;
+                md->pressure = (pt[0] + pt[1] + pt[2]) / 3.0;
+            }
+            const double mu = pow(1.0 - md->comp * md->dt / md->tau_p * (md->p_ref - md->pressure), 1.0 / 3.0);
+            double Ln[3] = {L[0] * mu, L[1] * mu, L[2] * mu};
+            or_ff_set_box(md->ff, Ln);
+            for (int k = 0; k < 3 * n; ++k) md->pos[o][k] *= mu;
+        }
         if (energies) {
             energies[5 * s + 0] = elj;
             energies[5 * s + 1] = ecrf;

@@ -284,6 +284,13 @@ int or_md_set_nosehoover(or_md *md, double t_ref, double tau, double dof, int nh
 double or_nosehoover_step(double *zeta, int nhc, double ekin, double t_ref, double tau, double dof, double dt);
 void or_md_get_nosehoover(const or_md *md, double *zeta);
 double or_berendsen_scale_ekin(double ekin, double t_ref, double tau, double dof, double dt);
+/* Berendsen barostat with the pressure from the atomic (vtype 1) or molecular
+ * (vtype 2) virial, rectangular boxes; pg_last: last atom of each pressure
+ * group; returns -1 for another box or virial kind */
+int or_md_set_barostat(or_md *md, double p_ref, double comp, double tau_p, int vtype, int n_pg,
+                       const int32_t *pg_last);
+double or_md_pressure(const or_md *md);
+void or_md_box(const or_md *md, double *L);
 /* centre-of-mass motion removal (remove_com_motion.rs) at the start of each md step */
 void or_md_set_com(or_md *md, int nticom, int nscm);
 void or_remove_com(int n, const double *mass, const double *pos, double *vel, const or_pbc *pbc, double dt,

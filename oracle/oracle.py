@@ -163,6 +163,11 @@ def _declare(L):
     L.or_nosehoover_step.restype = D
     L.or_md_get_nosehoover.argtypes = [P, P]
     L.or_md_set_com.argtypes = [P, C.c_int, C.c_int]
+    L.or_md_set_barostat.argtypes = [P, D, D, D, C.c_int, C.c_int, P]
+    L.or_md_set_barostat.restype = C.c_int
+    L.or_md_pressure.argtypes = [P]
+    L.or_md_pressure.restype = D
+    L.or_md_box.argtypes = [P, P]
     L.or_remove_com.argtypes = [C.c_int, P, P, P, C.POINTER(Pbc), D, C.c_int, C.c_int, C.c_int64]
     L.or_berendsen_scale.argtypes = [C.c_int, P, P, D, D, D, D]
     L.or_berendsen_scale.restype = D
@@ -525,6 +530,11 @@ class MD:
                 lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
         if getattr(s, "nticom", 0) or getattr(s, "nscm", 0):
             lib().or_md_set_com(self.h, int(s.nticom), int(s.nscm))
+        if getattr(s, "barostat", None) is not None:
+            p_ref, comp, tau_p, vtype = s.barostat
+            self.pg = np.ascontiguousarray(s.pressure_group_last(), dtype=np.int32)
+            if lib().or_md_set_barostat(self.h, p_ref, comp, tau_p, int(vtype), len(self.pg), ptr(self.pg)) != 0:
+                raise ValueError("the barostat needs a rectangular box and virial kind 0, 1 or 2")
         self.n = s.n_atoms
 
     def run(self, nsteps):
@@ -538,6 +548,15 @@ class MD:
         x, v, f = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros((self.n, 3))
         lib().or_md_get(self.h, ptr(x), ptr(v), ptr(f))
         return x, v, f
+
+    def pressure(self):
+        """the last step's isotropic pressure (kJ mol^-1 nm^-3)"""
+        return lib().or_md_pressure(self.h)
+
+    def box(self):
+        L = np.zeros(3)
+        lib().or_md_box(self.h, ptr(L))
+        return L
 
     def zeta(self):
         """the Nose-Hoover chain's state"""

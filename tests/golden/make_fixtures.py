@@ -62,6 +62,8 @@ SYSTEMS = [
      None, None, "all"),
     ("water_216_box_com_rot", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
      None, None, "all"),
+    ("water_216_npt", "../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+     None, None, "all"),
     ("ch4_water_fep", "../shared/ch4_spc.top", "../shared/ch4_spc.cnf", None,
      "../shared/ch4_spc_dummy.ptp", "first"),
 ]
@@ -97,6 +99,10 @@ def parse_topology(path):
     t = {}
     fp = [float(x.split()[0]) for x in data_lines(b.get("PHYSICALCONSTANTS", []))]
     t["fpepsi"] = fp[0] if fp else 138.9354
+    # PRESSUREGROUPS (gromos-io topology.rs:822-851): NSVM, then the last atom
+    # (1-based) of each solute pressure group
+    pg = " ".join(data_lines(b.get("PRESSUREGROUPS", []))).split()
+    t["pressure_groups"] = [int(x) - 1 for x in pg[1:1 + int(pg[0])]] if pg else []
     # SOLUTEATOM (topology.rs:270-370)
     lines = data_lines(b.get("SOLUTEATOM", []))
     n = int(lines[0].split()[0])
@@ -214,6 +220,17 @@ def parse_imd(path):
     bc = data_lines(b["BOUNDCOND"])[0].split()
     d["ntb"] = int(bc[0])
     d["ndfmin"] = int(bc[1]) if len(bc) > 1 else 0
+    # PRESSURESCALE (md.rs:1116-1129, 1259-1277): COUPLE SCALE COMP TAUP VIRIAL,
+    # the SEMI row, then the 3x3 reference pressure (isotropic: its [0][0])
+    ps = data_lines(b.get("PRESSURESCALE", []))
+    d["pressure"] = None
+    if ps:
+        w = ps[0].split()
+        ref = [float(x) for x in " ".join(ps[2:5]).split()]
+        d["pressure"] = dict(couple=w[0], scale=w[1], comp=float(w[2]), tau_p=float(w[3]),
+                             virial={"none": 0, "atomic": 1, "molecular": 2}.get(w[4].lower(), int(w[4]) if
+                                                                                w[4].isdigit() else 0),
+                             p_ref=ref[0])
     # centre-of-mass motion removal: INITIALISE NTICOM (third value of its second
     # row), COMTRANSROT NSCM
     ini = data_lines(b.get("INITIALISE", []))
@@ -307,9 +324,14 @@ def forces(path):
 
 
 def positions(path):
+    boxes = {step: [float(x) for x in data_lines(lines)[1].split()[:3]]
+             for step, lines in parse_frames(path, "GENBOX")}
     out = []
     for step, lines in parse_frames(path, "POSITIONRED"):
-        out.append(dict(step=step, x=[float(x) for l in data_lines(lines) for x in l.split()[:3]]))
+        fr = dict(step=step, x=[float(x) for l in data_lines(lines) for x in l.split()[:3]])
+        if step in boxes:
+            fr["box"] = boxes[step]
+        out.append(fr)
     return out
 
 
@@ -394,7 +416,8 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
         algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
         dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"], ndfmin=imd["ndfmin"], thermostat=imd["thermostat"],
-        nticom=imd["nticom"], nscm=imd["nscm"],
+        nticom=imd["nticom"], nscm=imd["nscm"], pressure=imd["pressure"],
+        pressure_groups=t["pressure_groups"],
         solvent_constr=t["solvent_constr"], pert=pert,
         has_bonded_terms=bool(t["bonds"] or t["angles"]),
         bonded=dict(bonds=t["bond_terms"], angles=t["angle_terms"], impropers=t["improper_terms"],
@@ -408,6 +431,10 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
     trc = os.path.join(ex_dir, "trajectory.trc")
     if keep == "all" and os.path.exists(trc):
         fx["expected_positions"] = positions(trc)
+    if imd.get("pressure"):
+        # the box after the run (final.conf GENBOX edge lengths)
+        b, _ = blocks(os.path.join(ex_dir, "final.conf"))
+        fx["final_box"] = [float(x) for x in data_lines(b["GENBOX"])[1].split()[:3]]
     return fx
 
 
@@ -460,6 +487,8 @@ INPUT_SETS = {
                                 "water_216_nvt_nhc_chain.in"],
     "water_216_box_com_rot": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
                               "water_216_box_com_rot.in"],
+    "water_216_npt": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",
+                      "water_216_npt.in"],
 }
 
 

@@ -1,0 +1,104 @@
+"""Pinning of the oracle's pressure coupling (md.rs:1116-1129, 1259-1277): the
+kinetic energy tensor and the molecular virial correction of prepare_virial
+(forcefield.rs:285-296, 1064-1200), PressureCalculation
+(pressure_calculation.rs) and the isotropic BerendsenBarostat
+(berendsen_barostat.rs), restated in oracle/gromos_oracle.c.
+
+  * gromosXX fixture water_216_npt (216 flexible waters, one pressure group,
+    molecular virial, P0 1 bar, compressibility 4.575e-4 bar^-1, tau_p 0.5 ps,
+    Berendsen bath): every printed frame's box edge, positions, forces, E_pot
+    and E_kin over 20 steps, and the final box; the frames reject the atomic
+    virial and the run without the barostat;
+  * the mathematics: the pressure of an ideal gas (no interactions) is
+    2 E_kin / (3 V) from the atomic kinetic tensor; with one pressure group
+    its kinetic tensor is that of the total momentum; a box at the reference
+    pressure stays put, and mu^3 - 1 is linear in P0 - P."""
+import importlib
+
+import numpy as np
+import pytest
+
+import golden_io
+import oracle
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def with_npt(name="water_216_npt", vtype=None):
+    s, d = golden_io.load(name)
+    th, pr = d["thermostat"], d["pressure"]
+    s.thermostat = (th["t_ref"], th["tau"], float(3 * s.n_atoms - d["ndfmin"]))
+    s.pressure_groups = np.array(d["pressure_groups"], np.int32)
+    s.barostat = (pr["p_ref"], pr["comp"], pr["tau_p"], pr["virial"] if vtype is None else vtype)
+    return s, d
+
+
+def run(s, d, steps):
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=False)
+    frames = {e["step"]: e for e in d["expected_energies"]}
+    pos = {p["step"]: p for p in d["expected_positions"]}
+    frc = {f["step"]: f for f in d["expected_forces"]}
+    dev = dict(x=0.0, box=0.0, f=0.0, epot=0.0, ekin=0.0)
+    for k in range(steps):
+        if k in pos:
+            x, _, _ = md.state()
+            dev["x"] = max(dev["x"], np.abs(x - np.array(pos[k]["x"]).reshape(-1, 3)).max())
+            dev["box"] = max(dev["box"], np.abs(md.box() - np.array(pos[k]["box"])).max())
+        e = md.run(1)[0]
+        if k in frames:
+            _, _, f = md.state()
+            dev["f"] = max(dev["f"], np.abs(f - np.array(frc[k]["f"]).reshape(-1, 3)).max())
+            dev["epot"] = max(dev["epot"], rel(e[2], frames[k]["e_pot"]))
+            dev["ekin"] = max(dev["ekin"], rel(e[3], frames[k]["e_kin"]))
+    return md, dev
+
+
+def test_npt_md_follows_gromosxx():
+    s, d = with_npt()
+    md, dev = run(s, d, d["nstlim"])
+    assert dev["x"] <= 5.01e-10 and dev["box"] <= 5.01e-10, dev
+    assert dev["f"] <= 1e-9 and dev["epot"] < 6e-10 and dev["ekin"] < 6e-10, dev
+    assert np.abs(md.box() - np.array(d["final_box"])).max() <= 5.01e-10
+    assert md.box()[0] != d["box"][0]
+
+
+@pytest.mark.parametrize("change", ["atomic", "no barostat"])
+def test_fixture_rejects_other_pressure_coupling(change):
+    s, d = with_npt(vtype=1 if change == "atomic" else None)
+    if change == "no barostat":
+        s.barostat = None
+    _, dev = run(s, d, 11)
+    assert dev["box"] > 1e-6 or dev["x"] > 1e-6, dev
+
+
+def test_ideal_gas_pressure_and_reference_pressure_fixed_point():
+    test_oracle = importlib.import_module("test_oracle")
+    s = test_oracle._free_particles(n=20, seed=4)
+    s.boundary, s.box = 1, np.array([3.0, 3.0, 3.0])
+    kb = 0.00831441
+    ek = 0.5 * np.sum(s.mass[:, None] * s.vel ** 2)
+    p_ideal = 2.0 * ek / (3.0 * 27.0)
+    # atomic virial: P = 2 E_kin / (3 V) exactly (no forces); a box already at
+    # that reference pressure keeps its size
+    s.barostat = (p_ideal, 4.575e-4, 0.5, 1)
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=False)
+    md.run(1)
+    assert md.pressure() == pytest.approx(p_ideal, rel=1e-13)
+    assert np.abs(md.box() - 3.0).max() < 1e-14
+    # P0 away from P: mu^3 - 1 = -comp dt / tau (P0 - P)
+    s.barostat = (p_ideal + 10.0, 4.575e-4, 0.5, 1)
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=False)
+    md.run(1)
+    mu = md.box()[0] / 3.0
+    assert mu ** 3 - 1.0 == pytest.approx(-4.575e-4 * s.dt / 0.5 * 10.0, rel=1e-9)
+    # one pressure group over everything: the kinetic tensor of the total
+    # momentum, 0.5 |P|^2 / M on the trace
+    s.barostat = (0.0, 4.575e-4, 0.5, 2)
+    s.pressure_groups = np.array([s.n_atoms - 1], np.int32)
+    md = oracle.MD(oracle.OracleSystem(s), use_settle=False)
+    md.run(1)
+    ptot = (s.mass[:, None] * s.vel).sum(0)
+    p_mol = 2.0 / 27.0 * (0.5 * np.dot(ptot, ptot) / s.mass.sum()) / 3.0
+    assert md.pressure() == pytest.approx(p_mol, rel=1e-12)

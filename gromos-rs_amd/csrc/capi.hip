@@ -557,6 +557,16 @@ extern "C" int gromos_amd_system_covalent_energies(gromos_amd_system* S, double*
     return 0;
 }
 
+extern "C" int gromos_amd_system_box(gromos_amd_system* S, double* box, double* pressure) {
+    if (!S) return fail(GROMOS_AMD_EINVAL, "null system");
+    for (int32_t b = 0; b < S->njobs; ++b) {
+        if (box)
+            for (int a = 0; a < 3; ++a) box[3 * b + a] = S->jobs[b].box.L[a];
+        if (pressure) pressure[b] = S->host_pressure[b];
+    }
+    return 0;
+}
+
 extern "C" int gromos_amd_system_md_covalent(gromos_amd_system* S, int32_t nsteps, double* energies,
                                              double* covalent) {
     if (!S || nsteps < 0) return fail(GROMOS_AMD_EINVAL, "md: bad arguments");

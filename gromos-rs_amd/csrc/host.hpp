@@ -58,6 +58,7 @@ struct JobImage {
     std::vector<int32_t> pa_iac, pb_iac, papairs;
     std::vector<double> pa_q, pb_q, palj, pacrf;
     BondedTerms bonded;             // covalent terms of the solute
+    std::vector<int32_t> pg_start;  // pressure groups (job atoms), n_pg + 1; empty: none
     int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
     int64_t list_entries = 0;       // full neighbour-list entries of the job (estimate)
 };
@@ -67,6 +68,9 @@ struct JobImage {
 constexpr int64_t MAX_LIST_ENTRIES = INT32_MAX;
 
 int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int job, JobImage& out);
+// a pressure-coupled job's image takes the box it has now: edge lengths, the
+// cell grid (5 % to spare) and the CG-grouped predicate follow
+void rebox_image(JobImage& im, const Box& box);
 
 // validated covalent terms of one topology, kind-major (bonds, angles,
 // dihedrals, impropers), local atom indices; n: terms kept per kind
@@ -218,6 +222,10 @@ struct gromos_amd_system {
     double* d_nh_pend = nullptr;   // per job: the chain after the next step's update
     bool any_thermo = false;
     bool any_com = false;                  // a job removes centre-of-mass motion
+    bool any_baro = false;                 // a job couples pressure
+    double* d_press = nullptr;             // per job: kinetic tensor (9), then the last pressure
+    int32_t* d_pg_start = nullptr;         // pressure groups of the batch (atom indices)
+    int32_t* d_box_fail = nullptr;         // per job: first step whose box outgrew the cell grid
     std::vector<int64_t> job_step0;        // per job: md steps done before the current call
     int64_t* d_job_step0 = nullptr;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one
@@ -236,6 +244,8 @@ struct gromos_amd_system {
     int32_t *d_bd_aoff = nullptr, *d_bd_aref = nullptr;
     int32_t nbd = 0;
     int32_t* d_vv_near = nullptr;   // mixed: near partners at the front of each solvent list
+    std::vector<double> host_pressure;   // per job: the last md step's pressure
+    std::vector<int32_t> host_pg;        // pressure-group starts of the batch (uploaded at layout)
     long long* d_fi_near = nullptr;  // mixed: the near pairs' forces (k_vv_nr)
     double* d_vv_en = nullptr;       // mixed: the near pairs' per-group records
     double near_r = gamd::VV_NEAR_R;   // their radius (nm)

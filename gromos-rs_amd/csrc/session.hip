@@ -113,6 +113,7 @@ struct SJob {
     JobImage img;
     std::vector<double> pos, vel;   // host state while not resident
     std::vector<double> zeta;       // its Nose-Hoover chain across a layout (empty: at rest)
+    double pressure = 0.0;          // its last md step's pressure, once it left the device
     int64_t nsteps = 0, done = 0;
     int32_t state = GROMOS_AMD_JOB_QUEUED;
     int32_t slot = -1;              // index in the resident batch, -1 if not resident
@@ -190,6 +191,10 @@ static int relayout(gromos_amd_session* s) {
         hipStream_t st = S->stream;
         for (SJob* j : leave) {
             if (j->evict) continue;
+            if (j->img.J.baro) {
+                rebox_image(j->img, S->jobs[j->slot].box);
+                j->pressure = S->host_pressure[j->slot];
+            }
             size_t n3 = (size_t)j->img.n * 3, off = (size_t)S->jobs[j->slot].atom_off * 3;
             j->pos.resize(n3);
             j->vel.resize(n3);
@@ -202,9 +207,12 @@ static int relayout(gromos_amd_session* s) {
             std::vector<double> z((size_t)GROMOS_AMD_NHC_MAX * S->njobs);
             HIPCHK(hipMemcpyAsync(z.data(), S->d_nh_zeta, z.size() * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            for (SJob* j : keep)
+            for (SJob* j : keep) {
                 j->zeta.assign(z.begin() + (size_t)GROMOS_AMD_NHC_MAX * j->slot,
                                z.begin() + (size_t)GROMOS_AMD_NHC_MAX * (j->slot + 1));
+                // a barostat's box as it is now (the new layout sizes its cells from it)
+                if (j->img.J.baro) rebox_image(j->img, S->jobs[j->slot].box);
+            }
             size_t n3 = (size_t)old_npad * 3;
             if (int rc = s->stage.reserve(2 * n3 * sizeof(double), st)) return rc;
             double* sx = (double*)s->stage.ptr;
@@ -451,7 +459,10 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
                     adv[b] = failed[b];
                     j->state = GROMOS_AMD_JOB_FAILED;
                     char msg[160];
-                    if (fkind[b])
+                    if (fkind[b] == 2)
+                        snprintf(msg, sizeof(msg), "the barostat shrank the box below the pair search's cells (step %lld)",
+                                 (long long)(j->done + failed[b]));
+                    else if (fkind[b])
                         snprintf(msg, sizeof(msg), "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm) (step %lld)",
                                  FORCE_LIMIT, (long long)(j->done + failed[b]));
                     else
@@ -492,6 +503,20 @@ extern "C" int gromos_amd_job_query(gromos_amd_session* s, int64_t id, int32_t* 
     if (state) *state = j->state;
     if (steps_done) *steps_done = j->done;
     if (j->state == GROMOS_AMD_JOB_FAILED) fail(GROMOS_AMD_ESETTLE, "%s", j->error.c_str());
+    return 0;
+}
+
+extern "C" int gromos_amd_job_box(gromos_amd_session* s, int64_t id, double* box, double* pressure) {
+    if (!s) return fail(GROMOS_AMD_EINVAL, "job_box: null session");
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(GROMOS_AMD_EINVAL, "job_box: no job %lld", (long long)id);
+    const SJob* j = it->second.get();
+    const gromos_amd_system* S = s->S.get();
+    const bool resident = S && j->slot >= 0;
+    const Box& b = resident ? S->jobs[j->slot].box : j->img.J.box;
+    if (box)
+        for (int a = 0; a < 3; ++a) box[a] = b.L[a];
+    if (pressure) *pressure = resident ? S->host_pressure[j->slot] : j->pressure;
     return 0;
 }
 

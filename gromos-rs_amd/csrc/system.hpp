@@ -57,6 +57,11 @@ struct JobDev {
     double t_ref, tau_t, dof_t;
     int32_t nhc;                           // Nose-Hoover chain length (nosehoover_thermostat.rs)
     int32_t nticom, nscm;                  // centre-of-mass motion removal (remove_com_motion.rs)
+    int32_t baro, vtype;                   // Berendsen barostat; virial 0 none, 1 atomic, 2 molecular
+    double p_ref, comp, tau_p;
+    double cell_min;                       // the pair search's least cell width (half the list cutoff)
+    double cgg_limit;                      // CG-grouped loops while half the shortest edge exceeds it
+    int32_t pg_off, n_pg;                  // pressure groups [pg_start[pg_off + g], pg_start[pg_off + g + 1])
     int32_t bd_off;                        // first covalent term (batch order)
     int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };

@@ -78,6 +78,17 @@ static int32_t cells_for(double L, double rl) {
     return (int32_t)n;
 }
 
+void rebox_image(JobImage& im, const Box& box) {
+    JobDev& J = im.J;
+    J.box = box;
+    double hm = 1e300;
+    for (int a = 0; a < 3; ++a) {
+        J.ncell[a] = cells_for(box.L[a] * 0.95, 2.0 * J.cell_min);
+        hm = std::min(hm, 0.5 * box.L[a]);
+    }
+    J.use_cg_grouped = hm > J.cgg_limit;
+}
+
 // device bytes per padded atom / per charge group held by a batch (the arrays
 // of gromos_amd_system), used for admission by footprint
 constexpr int64_t BYTES_PER_ATOM = 2 * 2 * 24 + 7 * 24 + 32 + 16 + 8 + 4 + 8 + 8 + 4 + 12 + 16 + 41 +
@@ -145,6 +156,40 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.t_ref = P.t_ref;
     J.tau_t = P.tau_t;
     J.dof_t = P.dof_t;
+    if (P.barostat < 0 || P.barostat > 1) return fail(GROMOS_AMD_EINVAL, "job %d: barostat must be 0 or 1", j);
+    if (P.barostat) {
+        if (P.boundary != 1)
+            return fail(GROMOS_AMD_EINVAL, "job %d: the barostat needs a rectangular box", j);
+        if (P.virial < 0 || P.virial > 2 || !(P.tau_p > 0.0))
+            return fail(GROMOS_AMD_EINVAL, "job %d: barostat virial must be 0, 1 or 2 and tau_p > 0", j);
+    }
+    J.baro = P.barostat;
+    J.vtype = P.barostat ? P.virial : 0;
+    J.p_ref = P.p_ref;
+    J.comp = P.compressibility;
+    J.tau_p = P.tau_p;
+    J.cgg_limit = P.rcutl + P.skin + 0.3;
+    J.cell_min = 0.5 * (P.rcutl + P.skin);
+    if (J.baro && J.vtype == 2) {
+        // pressure groups: the solute's, then each solvent molecule (core topology.rs:1242-1263)
+        im.pg_start.assign(1, 0);
+        int32_t prev = -1;
+        for (int32_t g = 0; g < T.n_pressure_groups; ++g) {
+            const int32_t last = T.pressure_group_last ? T.pressure_group_last[g] : -1;
+            if (last <= prev || last >= T.n_solute_atoms)
+                return fail(GROMOS_AMD_EINVAL, "job %d: pressure group %d must end after %d and before %d", j, g,
+                            prev, T.n_solute_atoms);
+            im.pg_start.push_back(last + 1);
+            prev = last;
+        }
+        if (T.n_pressure_groups > 0 && prev != T.n_solute_atoms - 1)
+            return fail(GROMOS_AMD_EINVAL, "job %d: the pressure groups must cover the solute", j);
+        if (T.n_pressure_groups == 0 && T.n_solute_atoms > 0)
+            return fail(GROMOS_AMD_EINVAL, "job %d: a molecular virial needs the solute's pressure groups", j);
+        for (int32_t a = T.n_solute_atoms + T.atoms_per_solvent; a <= n && T.atoms_per_solvent > 0;
+             a += T.atoms_per_solvent)
+            im.pg_start.push_back(a);
+    }
     // use_cg_grouped_solute (forcefield.rs:271-283; triclinic: off)
     if (J.box.kind == 0) J.use_cg_grouped = 1;
     else if (J.box.kind == 2) J.use_cg_grouped = 0;
@@ -154,7 +199,10 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     }
     // vacuum and triclinic jobs: one cell, every group a candidate of every other
     // (the reference's O(N^2) standard search for both, pairlist.rs:521-532)
-    for (int a = 0; a < 3; ++a) J.ncell[a] = J.box.kind == 1 ? cells_for(P.box[a], l) : 1;
+    // a barostat's box: cells with 5 % to spare, so that the grid holds while the
+    // box shrinks that much (the step checks it and stops the job beyond)
+    for (int a = 0; a < 3; ++a)
+        J.ncell[a] = J.box.kind == 1 ? cells_for(P.box[a] * (J.baro ? 0.95 : 1.0), l) : 1;
     // solvent template tables from the first solvent molecule (innerloops.rs:241-255)
     if (nsolv_atoms > 0) {
         int32_t f = T.n_solute_atoms;
@@ -314,7 +362,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     };
     std::vector<Slice> sl(njobs + 1);
     int64_t cell_off = 0;
-    bool any_pert = false, twin = false, any_thermo = false, any_com = false;
+    bool any_pert = false, twin = false, any_thermo = false, any_com = false, any_baro = false;
+    S->host_pg.clear();
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -343,6 +392,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         twin = twin || J.twin;
         any_thermo = any_thermo || J.thermo;
         any_com = any_com || J.nticom >= 1 || J.nscm != 0;
+        any_baro = any_baro || J.baro;
+        J.pg_off = (int32_t)S->host_pg.size();
+        J.n_pg = im.pg_start.empty() ? 0 : (int32_t)im.pg_start.size() - 1;
+        for (int32_t x : im.pg_start) S->host_pg.push_back((int32_t)o.atom + x);
         use_settle |= im.P.use_settle && !im.mol_first.empty();
         do_virial |= im.P.do_virial;
         S->jobs.push_back(J);
@@ -480,11 +533,13 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->nrow_blocks = (int32_t)(align_up(npad * 4, BLOCK) / BLOCK);   // 4 lanes per atom in the pair kernel
     S->nsnb = std::max(1, I0.P.nsnb);
     S->use_settle = use_settle;
-    S->do_virial = do_virial;
+    S->do_virial = do_virial || any_baro;   // the pressure needs every step's virial
     S->twin = twin;
     S->any_thermo = any_thermo;
     S->any_com = any_com;
+    S->any_baro = any_baro;
     S->job_step0.assign((size_t)njobs, 0);
+    S->host_pressure.assign((size_t)njobs, 0.0);
     S->near_r = VV_NEAR_R;
     if (const char* env = std::getenv("GROMOS_AMD_VV_NEAR_R")) S->near_r = std::max(0.0, std::atof(env));
     // LJ only between the solvent molecules' first atoms (SPC-like waters)
@@ -596,6 +651,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     hipStream_t st = S->stream;
     int rc = 0;
     rc = rc ? rc : S->upload(S->d_jobs, S->jobs);
+    rc = rc ? rc : S->upload(S->d_pg_start, S->host_pg);
     rc = rc ? rc : S->upload(S->d_cg_start, cg_start);
     rc = rc ? rc : S->upload(S->d_cg_n, cg_n);
     rc = rc ? rc : S->upload(S->d_cg_job, cg_job);
@@ -680,6 +736,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_therm_scale, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_nh_zeta, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_job_step0, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_press, (size_t)10 * njobs, true)) ||
+        (rc = S->reserve(S->d_box_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -711,6 +769,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     // Nose-Hoover chains start at zero (nosehoover_thermostat.rs:64-113); a
     // session puts continuing jobs' chains back after the layout
     HIPCHK(hipMemsetAsync(S->d_nh_zeta, 0, (size_t)GROMOS_AMD_NHC_MAX * njobs * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(S->d_press, 0, (size_t)10 * njobs * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_fi_near, 0, n3 * sizeof(long long), st));   // padding slots stay zero
     HIPCHK(hipMemsetAsync(S->d_step, 0, sizeof(int32_t), st));
     HIPCHK(hipMemsetAsync(S->d_step_done, 0, sizeof(uint32_t), st));
@@ -742,6 +801,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
 }
 
 int upload_jobs(gromos_amd_system* S) {
+    // the host copies of barostat boxes follow the device after every md call
+    // (md_steps), so this upload keeps them
     HIPCHK(hipMemcpyAsync(S->d_jobs, S->jobs.data(), S->jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
     return 0;
@@ -1133,6 +1194,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     std::vector<int32_t> big(S->njobs, INT32_MAX);
     HIPCHK(hipMemcpyAsync(S->d_settle_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S->d_force_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S->d_box_fail, big.data(), big.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     size_t ev0 = S->step_ev.size();
     auto step_event = [&]() -> int {
         hipEvent_t e;
@@ -1159,7 +1221,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // thermostat rescan, no per-kernel timing events): one captured graph
         const bool plain = S->use_graphs && S->built && S->steps_since_update + 1 < S->nsnb && S->xq_valid &&
                            !(S->any_thermo && S->therm_dirty) && !S->timing && !(S->twin && !S->long_computed) &&
-                           !S->any_com &&
+                           !S->any_com && !S->any_baro &&
                            S->vv_lanes > 0;
         if (plain) {
             S->steps_since_update++;
@@ -1181,13 +1243,16 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // centre-of-mass motion removal (first in the reference's step, md.rs:1095-1097):
         // on the velocities entering the step, after the bath read their E_kin
         if (S->any_com) launch_com(p, S->njobs, st);
+        if (S->any_baro) launch_press_ke(p, S->njobs, st);
         // the integrator leaves the sorted copies for the next step unless that
         // step rebuilds the list (pairlist.rs:105-107)
         const int32_t write_xq = S->steps_since_update < S->nsnb ? 1 : 0;
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, write_xq, mixed, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
+        if (S->any_baro) launch_baro(p, S->njobs, st);
         S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
-        S->xq_valid = write_xq != 0;
+        // the barostat scaled the new positions after the integrator copied them
+        S->xq_valid = write_xq != 0 && !S->any_baro;
         if (S->step_timing)
             if (int rc = step_event()) return rc;
     }
@@ -1195,9 +1260,19 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     S->last_rec_row = nsteps - 1;
     for (int32_t b = 0; b < S->njobs; ++b) S->job_step0[b] += std::min<int64_t>(nsteps, S->jobs[b].stop);
     HIPCHK(hipGetLastError());
-    std::vector<int32_t> failed(S->njobs), ffail(S->njobs);
+    std::vector<int32_t> failed(S->njobs), ffail(S->njobs), bfail(S->njobs, INT32_MAX);
     HIPCHK(hipMemcpyAsync(failed.data(), S->d_settle_fail, failed.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(ffail.data(), S->d_force_fail, ffail.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    std::vector<JobDev> jd;
+    std::vector<double> press;
+    if (S->any_baro) {
+        // the barostat's boxes and pressures, kept on the host side too
+        jd.resize(S->jobs.size());
+        press.resize((size_t)10 * S->njobs);
+        HIPCHK(hipMemcpyAsync(bfail.data(), S->d_box_fail, bfail.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(jd.data(), S->d_jobs, jd.size() * sizeof(JobDev), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(press.data(), S->d_press, press.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     if (rec) {
         rec->resize((size_t)need);
         HIPCHK(hipMemcpyAsync(rec->data(), S->d_records, (size_t)need * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1207,14 +1282,24 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         if (int rc = collect_step_times(S)) return rc;
     }
     if (int rc = S->collect_timing()) return rc;
+    for (size_t b = 0; b < jd.size(); ++b) {
+        S->jobs[b].box = jd[b].box;
+        S->jobs[b].use_cg_grouped = jd[b].use_cg_grouped;
+        S->host_pressure[b] = press[10 * b + 9];
+    }
     // a force failure at step s leaves that step's forces undefined: the job
-    // stops there like a SETTLE failure
+    // stops there like a SETTLE failure; so does a box the cell grid no longer fits
     std::vector<uint8_t> kind(S->njobs, 0);
-    for (int j = 0; j < S->njobs; ++j)
+    for (int j = 0; j < S->njobs; ++j) {
         if (ffail[j] <= failed[j] && ffail[j] != INT32_MAX) {
             failed[j] = ffail[j];
             kind[j] = 1;
         }
+        if (bfail[j] < failed[j]) {
+            failed[j] = bfail[j];
+            kind[j] = 2;
+        }
+    }
     if (failed_out) {
         *failed_out = failed;
         if (force_out) *force_out = kind;
@@ -1222,6 +1307,9 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
     }
     for (int j = 0; j < S->njobs; ++j)
         if (failed[j] != INT32_MAX) {
+            if (kind[j] == 2)
+                return fail(GROMOS_AMD_EBOX, "the barostat shrank the box below the pair search's cells (job %d, step %d)",
+                            j, failed[j]);
             if (kind[j])
                 return fail(GROMOS_AMD_EFORCE, "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm; job %d, step %d)",
                             FORCE_LIMIT, j, failed[j]);
@@ -1394,6 +1482,9 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.nh_zeta = d_nh_zeta;
     p.nh_pend = d_nh_pend;
     p.job_step0 = d_job_step0;
+    p.press = d_press;
+    p.pg_start = d_pg_start;
+    p.box_fail = d_box_fail;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -2251,6 +2251,131 @@ __global__ void __launch_bounds__(BLOCK) k_com(SysPtrs p) {
     }
 }
 
+// prepare_virial's kinetic energy tensor (forcefield.rs:285-296, 1064-1120) of
+// each pressure-coupled job, from the velocities entering the step (after the
+// centre-of-mass removal): molecular, 0.5 (sum m v)(sum m v)^T / M per
+// pressure group, or atomic, 0.5 m v v^T per atom.  A block per job.
+__global__ void __launch_bounds__(BLOCK) k_press_ke(SysPtrs p) {
+    __shared__ double lds[17 * 9];
+    const int32_t jb = blockIdx.x;
+    const JobDev& J = p.jobs[jb];
+    if (!J.baro || J.vtype == 0) return;
+    if (*p.step >= J.stop) return;
+    const double* v = p.v_cur;
+    double ke[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (J.vtype == 2 && J.n_pg > 0) {
+        for (int32_t g = threadIdx.x; g < J.n_pg; g += blockDim.x) {
+            const int32_t a0 = p.pg_start[J.pg_off + g], a1 = p.pg_start[J.pg_off + g + 1];
+            double mv[3] = {0.0, 0.0, 0.0}, mt = 0.0;
+            for (int32_t a = a0; a < a1; ++a) {
+                const double m = p.mass[a];
+                mt += m;
+                for (int c = 0; c < 3; ++c) mv[c] += m * v[3 * (int64_t)a + c];
+            }
+            if (mt > 0.0)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) ke[3 * a + b] += 0.5 * mv[a] * mv[b] / mt;
+        }
+    } else {
+        for (int32_t i = J.atom_off + threadIdx.x; i < J.atom_off + J.n_atoms; i += blockDim.x) {
+            const double m = p.mass[i];
+            const double* vi = v + 3 * (int64_t)i;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) ke[3 * a + b] += 0.5 * m * vi[a] * vi[b];
+        }
+    }
+    block_sum<9>(ke, lds);
+    if (threadIdx.x < 9) p.press[10 * (int64_t)jb + threadIdx.x] = lds[144 + threadIdx.x];
+}
+
+// The pressure and the Berendsen barostat at the end of the md step
+// (md.rs:1259-1277), a block per pressure-coupled job, after k_reduce (whose
+// last block advanced the step counter, so this step is the counter - 1):
+//  * the molecular virial (apply_molecular_virial_correction,
+//    forcefield.rs:1128-1200): per pressure group the centre of mass of the
+//    chain-gathered positions, then vir -= f_i (x) r_i with r_i the atom's image
+//    nearest that centre (the trace: the isotropic pressure reads no more);
+//  * PressureCalculation (pressure_calculation.rs:40-80): P = tr(KE + vir / 2)
+//    * 2 / V / 3 with this step's box;
+//  * BerendsenBarostat (berendsen_barostat.rs:30-70): mu = (1 - comp dt / tau_p
+//    (P0 - P))^(1/3) scales the new positions and the box (edge lengths and
+//    their reciprocals, Rectangular::new) and refreshes the CG-grouped
+//    predicate (forcefield.rs:243-283) for the next step.  A box whose cells
+//    would fall below half the list cutoff stops the job (box_fail).
+__global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
+    __shared__ double lds[17 * 3];
+    __shared__ double s_mu;
+    const int32_t jb = blockIdx.x;
+    // the barostat changes the job's box in place, read by the next step's kernels
+    JobDev& J = const_cast<JobDev&>(p.jobs[jb]);
+    if (!J.baro) return;
+    const int32_t step = *p.step - 1;
+    if (step >= J.stop) return;
+    const Box box = J.box;
+    const double* x = p.x_cur;
+    double corr[3] = {0.0, 0.0, 0.0};
+    if (J.vtype == 2) {
+        for (int32_t g = threadIdx.x; g < J.n_pg; g += blockDim.x) {
+            const int32_t a0 = p.pg_start[J.pg_off + g], a1 = p.pg_start[J.pg_off + g + 1];
+            d3 prev = ld3(x, a0);
+            double com[3] = {0.0, 0.0, 0.0}, mt = 0.0;
+            for (int32_t a = a0; a < a1; ++a) {
+                const double m = p.mass[a];
+                mt += m;
+                const d3 r = nim_exact(box, ld3(x, a), prev);
+                const d3 gth{r.x + prev.x, r.y + prev.y, r.z + prev.z};
+                com[0] += m * gth.x;
+                com[1] += m * gth.y;
+                com[2] += m * gth.z;
+                prev = gth;
+            }
+            if (mt > 0.0)
+                for (int c = 0; c < 3; ++c) com[c] /= mt;
+            const d3 cm{com[0], com[1], com[2]};
+            for (int32_t a = a0; a < a1; ++a) {
+                const d3 r = nim_exact(box, ld3(x, a), cm);
+                const double* f = p.f + 3 * (int64_t)a;
+                corr[0] += f[0] * r.x;
+                corr[1] += f[1] * r.y;
+                corr[2] += f[2] * r.z;
+            }
+        }
+    }
+    block_sum<3>(corr, lds);
+    if (threadIdx.x == 0) {
+        double* pr = p.press + 10 * (int64_t)jb;
+        const double V = box.L[0] * box.L[1] * box.L[2];
+        double P = pr[9];   // no virial: the pressure keeps its last value (0 at start)
+        if (J.vtype != 0 && !(V < 1e-30)) {
+            const double* rec = p.records + ((int64_t)step * gridDim.x + jb) * REC;
+            const double factor = 2.0 / V;
+            double pt[3];
+            for (int a = 0; a < 3; ++a) {
+                const double vir = rec[5 + 4 * a] - lds[48 + a];
+                pt[a] = (pr[4 * a] + vir * 0.5) * factor;
+            }
+            P = (pt[0] + pt[1] + pt[2]) / 3.0;
+            pr[9] = P;
+        }
+        const double mu = pow(1.0 - J.comp * J.dt / J.tau_p * (J.p_ref - P), 1.0 / 3.0);
+        s_mu = mu;
+        double hm = 1e300;
+        for (int a = 0; a < 3; ++a) {
+            const double L = box.L[a] * mu;
+            J.box.L[a] = L;
+            J.box.invL[a] = 1.0 / L;
+            hm = fmin(hm, L * 0.5);
+            if (!(L / J.ncell[a] >= J.cell_min)) atomicMin(p.box_fail + jb, step);
+        }
+        J.use_cg_grouped = hm > J.cgg_limit;
+    }
+    __syncthreads();
+    const double mu = s_mu;
+    double* xn = p.x_new;
+    for (int64_t k = 3 * (int64_t)J.atom_off + threadIdx.x; k < 3 * (int64_t)(J.atom_off + J.n_atoms); k += blockDim.x)
+        xn[k] *= mu;
+}
+
 // One md step's update of one original charge group (thread per group): the
 // total force (k_assemble), LeapFrogVelocity with the bath's scale and
 // LeapFrogPosition (k_leapfrog), SETTLE for rigid waters (k_settle), the E_kin
@@ -2313,6 +2438,7 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
         // a non-finite or absurd force (a diverged job): the job fails at this step
         if (!(fabs(f.x) < FORCE_LIMIT && fabs(f.y) < FORCE_LIMIT && fabs(f.z) < FORCE_LIMIT))
             atomicMin(p.force_fail + jb, step);
+        if (J.vtype == 2) stv(p.f, i, f);   // the molecular virial's forces (k_baro)
         double im = p.inv_mass[i];
         if (J.thermo) vn = mk((vo.x + f.x * im * dt) * lam, (vo.y + f.y * im * dt) * lam, (vo.z + f.z * im * dt) * lam);
         else vn = mk(vo.x + f.x * im * dt, vo.y + f.y * im * dt, vo.z + f.z * im * dt);
@@ -2719,6 +2845,12 @@ void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t 
     const dim3 g((unsigned)((ncg + WAVE - 1) / WAVE));
     if (mixed) hipLaunchKernelGGL(k_integrate<true>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
     else hipLaunchKernelGGL(k_integrate<false>, g, dim3(WAVE), 0, s, p, ncg, with_long, write_xq);
+}
+void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_press_ke, dim3(njobs), dim3(BLOCK), 0, s, p);
+}
+void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_baro, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_com, dim3(njobs), dim3(BLOCK), 0, s, p);

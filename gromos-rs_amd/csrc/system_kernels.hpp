@@ -93,6 +93,9 @@ struct SysPtrs {
     double* nh_zeta;           // per job: Nose-Hoover chain (GROMOS_AMD_NHC_MAX), as of the last step
     double* nh_pend;           // per job: the chain the current step's scale came from
     const int64_t* job_step0;  // per job: its md steps done before the current call
+    double* press;             // per job: kinetic energy tensor (9), then the last pressure
+    const int32_t* pg_start;   // pressure groups (atom index starts; JobDev pg_off / n_pg)
+    int32_t* box_fail;         // per job: first step whose box outgrew the cell grid
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;
@@ -152,6 +155,8 @@ void launch_excl14(const SysPtrs& p, int32_t npad, int32_t solv_energy, const in
                    hipStream_t s);
 void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t mixed, hipStream_t s);
 void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s);
+void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s);
+void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
 void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,

@@ -319,6 +319,12 @@ class GromosInput:
     ntwf: int = 0
     nscm: int = 0
     nticom: int = 0                        # INITIALISE: centre-of-mass motion removal at step 0
+    couple: str = "off"                    # PRESSURESCALE COUPLE: off, calc, scale
+    scale: str = "iso"
+    compressibility: float = 4.575e-4
+    tau_p: float = 0.5
+    virial: int = 0                        # 0 none, 1 atomic, 2 molecular
+    p_ref: float = 0.06102                 # PRESSURESCALE reference pressure [0][0]
 
 
 _ALG = {"standard": 0, "0": 0, "grid": 1, "1": 1, "grid_cell": 2, "2": 2}
@@ -379,6 +385,16 @@ def read_imd(path):
         m.ntwx = v[0]
         if len(v) >= 7:
             m.ntwv, m.ntwf, m.ntwe = v[2], v[3], v[4]
+    if "PRESSURESCALE" in blocks:
+        d = _data(blocks["PRESSURESCALE"])
+        w = d[0].split()
+        m.couple = {"0": "off", "1": "calc", "2": "scale"}.get(w[0], w[0].lower())
+        m.scale = {"1": "iso", "2": "aniso", "3": "full-aniso", "4": "semi-aniso"}.get(w[1], w[1].lower())
+        m.compressibility, m.tau_p = float(w[2]), float(w[3])
+        m.virial = {"none": 0, "atomic": 1, "molecular": 2}.get(w[4].lower(), None)
+        if m.virial is None:
+            m.virial = int(w[4])
+        m.p_ref = float(" ".join(d[2:5]).split()[0])
     if "COMTRANSROT" in blocks:
         m.nscm = int(_tokens(blocks["COMTRANSROT"])[0])
     if "INITIALISE" in blocks:
@@ -545,6 +561,8 @@ def build_system(topo, conf, imd, ptp=None, name="gromos"):
                rcutp=imd.rcutp, rcutl=imd.rcutl, skin=0.0, nsnb=imd.nsnb, algorithm=imd.algorithm,
                epsrf=imd.epsrf, kappa=imd.appak, fpepsi=topo.fpepsi, dt=imd.dt, dist_oh=d_oh, dist_hh=d_hh,
                pert=pert, name=name, box_matrix=box_matrix, bonded=bonded)
+    if topo.pressure_groups:
+        s.pressure_groups = np.array(topo.pressure_groups, np.int32)
     return s.validate()
 
 
@@ -631,8 +649,9 @@ def read_force_trajectory(path):
     return out
 
 
-def write_configuration(path, system, pos, vel, title="final configuration", ntb=None):
-    """POSITION, VELOCITY (atom numbers and 15.9 columns) and GENBOX."""
+def write_configuration(path, system, pos, vel, title="final configuration", ntb=None, box=None):
+    """POSITION, VELOCITY (atom numbers and 15.9 columns) and GENBOX (the
+    system's box, or `box` edge lengths: a barostat's final box)."""
     with open(path, "w") as fh:
         fh.write(f"TITLE\n  {title}\nEND\nPOSITION\n")
         for i, p in enumerate(np.asarray(pos, float)):
@@ -640,7 +659,7 @@ def write_configuration(path, system, pos, vel, title="final configuration", ntb
         fh.write("END\nVELOCITY\n")
         for i, v in enumerate(np.asarray(vel, float)):
             fh.write(f"{1:5d} {'RES':<5} {'AT':<5}{i + 1:7d}{v[0]:15.9f}{v[1]:15.9f}{v[2]:15.9f}\n")
-        b = np.asarray(system.box, float)
+        b = np.asarray(system.box if box is None else box, float)
         kind = system.boundary if ntb is None else ntb
         fh.write(f"END\nGENBOX\n{kind:5d}\n{b[0]:15.9f}{b[1]:15.9f}{b[2]:15.9f}\n"
                  f"{90.0:15.9f}{90.0:15.9f}{90.0:15.9f}\n{0.0:15.9f}{0.0:15.9f}{0.0:15.9f}\n"
@@ -673,6 +692,13 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
         elif inp.coupling not in ("", "weak-coupling", "berendsen"):
             raise GromosFormatError(f"MULTIBATH coupling {inp.coupling!r} is not provided")
     s.nticom, s.nscm = inp.nticom, inp.nscm      # md.rs:1095-1097
+    if inp.couple == "scale":
+        # md.rs:1116-1129, 1259-1277: PressureCalculation + BerendsenBarostat
+        if inp.scale != "iso":
+            raise GromosFormatError(f"PRESSURESCALE scaling {inp.scale!r} is not provided (isotropic only)")
+        s.barostat = (inp.p_ref, inp.compressibility, inp.tau_p, inp.virial)
+    elif inp.couple != "off":
+        raise GromosFormatError(f"PRESSURESCALE coupling {inp.couple!r} is not provided")
     # step-0 forces (a separate single point: the md run keeps its own rebuild cadence)
     ds = rt.DeviceSystem(s, precision=precision)
     ds.pairlist_update()
@@ -683,16 +709,33 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
     fw.close()
     ds = rt.DeviceSystem(s, precision=precision)
     ds.pairlist_update()
-    e, cov = ds.md(n, covalent=True) if n > 0 else (np.zeros((0, 5)), np.zeros((0, 4)))
+    every = inp.ntwe if inp.ntwe > 0 else 1
+    volume = {}
+    pressure = {}
+    if s.barostat is None or n == 0:
+        e, cov = ds.md(n, covalent=True) if n > 0 else (np.zeros((0, 5)), np.zeros((0, 4)))
+    else:
+        # the box changes every step: each written frame's volume (its step's
+        # box) and pressure (after its step), the steps between in one call
+        parts = []
+        for k in range(0, n, every):
+            volume[k] = float(np.prod(ds.box()[0]))
+            parts.append(ds.md(1, covalent=True))
+            pressure[k] = ds.box()[1]
+            if min(every, n - k) > 1:
+                parts.append(ds.md(min(every, n - k) - 1, covalent=True))
+        e = np.concatenate([p[0] for p in parts])
+        cov = np.concatenate([p[1] for p in parts])
     x, v = ds.get_state()
+    final_box = ds.box()[0]
     ds.close()
     ew = EnergyTrajectoryWriter(out_prefix + ".tre")
-    every = inp.ntwe if inp.ntwe > 0 else 1
     dof = 3 * s.n_atoms - 3 * s.nsm - inp.ndfmin
     for k in range(0, len(e), every):
         ew.write(time=inp.t0 + k * s.dt, kinetic=e[k, 3], potential=e[k, 2], total=e[k, 2] + e[k, 3],
                  temperature=2.0 * e[k, 3] / (dof * K_BOLTZMANN) if dof > 0 else 0.0,
-                 volume=float(np.prod(s.box)) if s.boundary else 0.0, lj=e[k, 0], coul_real=e[k, 1],
+                 volume=volume.get(k, float(np.prod(s.box)) if s.boundary else 0.0),
+                 pressure=pressure.get(k, 0.0), lj=e[k, 0], coul_real=e[k, 1],
                  bond=cov[k, 0], angle=cov[k, 1], dihedral=cov[k, 2], improper=cov[k, 3])
     ew.close()
     out_pos, out_vel = x, v
@@ -700,7 +743,8 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
         # back to the truncated-octahedron frame (the rotation's inverse)
         R = _truncoct_rotation()
         out_pos, out_vel = x @ R, v @ R
-    write_configuration(out_prefix + ".cnf", s, out_pos, out_vel, ntb=inp.ntb)
+    write_configuration(out_prefix + ".cnf", s, out_pos, out_vel, ntb=inp.ntb,
+                        box=final_box if s.barostat is not None else None)
     return s, e
 
 

@@ -61,7 +61,8 @@ class Topology(C.Structure):
                 ("dist_oh", D), ("dist_hh", D),
                 ("pert_flag", P), ("pert_a_iac", P), ("pert_b_iac", P), ("pert_a_q", P), ("pert_b_q", P),
                 ("pert_alpha_lj", P), ("pert_alpha_crf", P), ("n_pert_atom_pairs", I32),
-                ("pert_atom_pairs", P), ("bonded", C.POINTER(Bonded))]
+                ("pert_atom_pairs", P), ("bonded", C.POINTER(Bonded)),
+                ("n_pressure_groups", I32), ("pressure_group_last", P)]
 
 
 class SessionParams(C.Structure):
@@ -79,7 +80,8 @@ class Params(C.Structure):
                 ("nsnb", I32), ("epsrf", D), ("kappa", D), ("four_pi_eps_i", D), ("dt", D),
                 ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
                 ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9),
-                ("nhc", I32), ("nticom", I32), ("nscm", I32)]
+                ("nhc", I32), ("nticom", I32), ("nscm", I32),
+                ("barostat", I32), ("p_ref", D), ("compressibility", D), ("tau_p", D), ("virial", I32)]
 
 
 def built():
@@ -142,6 +144,8 @@ def _declare(L):
     L.gromos_amd_system_forcefield_apply.argtypes = [P, P, P, P]
     L.gromos_amd_system_md.argtypes = [P, I32, P]
     L.gromos_amd_system_covalent_energies.argtypes = [P, P]
+    L.gromos_amd_system_box.argtypes = [P, P, P]
+    L.gromos_amd_job_box.argtypes = [P, I64, P, P]
     L.gromos_amd_system_md_covalent.argtypes = [P, I32, P, P]
     L.gromos_amd_bonded_plan_create.argtypes = [C.POINTER(Bonded), C.POINTER(P)]
     L.gromos_amd_bonded_plan_destroy.argtypes = [P]
@@ -249,6 +253,10 @@ def topology_struct(s, keep):
         ap = np.asarray(p.atom_pairs, np.int32).reshape(-1)
         t.n_pert_atom_pairs = len(ap) // 4
         t.pert_atom_pairs = keep(ap, np.int32)
+    pg = getattr(s, "pressure_groups", None)
+    if pg is not None and len(pg):
+        t.n_pressure_groups = len(pg)
+        t.pressure_group_last = keep(np.asarray(pg), np.int32)
     b = getattr(s, "bonded", None)
     if b is not None and b.n_terms > 0:
         bs = bonded_struct(b, s.n_atoms, keep)
@@ -294,6 +302,11 @@ def params_struct(s, use_settle=True, do_virial=False):
         p.nhc = nhc
         p.t_ref, p.tau_t, p.dof_t = (float(x) for x in th)
     p.nticom, p.nscm = int(getattr(s, "nticom", 0)), int(getattr(s, "nscm", 0))
+    ba = getattr(s, "barostat", None)
+    if ba is not None:
+        p.barostat = 1
+        p.p_ref, p.compressibility, p.tau_p = (float(x) for x in ba[:3])
+        p.virial = int(ba[3])
     return p
 
 
@@ -344,6 +357,12 @@ class DeviceSystem:
         out = dict(forces=f, e_lj=e[0], e_crf=e[1], dhdl=e[2], virial=vir.reshape(3, 3))
         out.update(zip(COVALENT, self.covalent_energies()))
         return out
+
+    def box(self):
+        """(box edge lengths now, the last md step's pressure in kJ mol^-1 nm^-3)"""
+        b, pr = np.zeros(3), np.zeros(1)
+        check(lib().gromos_amd_system_box(self.h, ptr(b), ptr(pr)))
+        return b, float(pr[0])
 
     def covalent_energies(self):
         """e_bond, e_angle, e_dihedral, e_improper of the last force evaluation."""
@@ -478,6 +497,12 @@ class Session:
         x, v = np.zeros((n, 3)), np.zeros((n, 3))
         check(lib().gromos_amd_job_fetch(self.h, int(job), ptr(x), ptr(v)))
         return x, v
+
+    def box(self, job):
+        """(the job's box edge lengths now, its last md step's pressure)"""
+        b, pr = np.zeros(3), np.zeros(1)
+        check(lib().gromos_amd_job_box(self.h, int(job), ptr(b), ptr(pr)))
+        return b, float(pr[0])
 
     def energies(self, job, first=0, count=None):
         if count is None:

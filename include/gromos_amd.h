@@ -46,7 +46,8 @@ enum gromos_amd_status {
     GROMOS_AMD_ESETTLE = -4,    /* SETTLE failed (constraints.rs:896-951) */
     GROMOS_AMD_EBUSY = -5,      /* no admission (server full) */
     GROMOS_AMD_ENOTFOUND = -6,
-    GROMOS_AMD_EFORCE = -7      /* a non-finite or out-of-range force (|F| >= 2^25 kJ/mol/nm): the job diverged */
+    GROMOS_AMD_EFORCE = -7,     /* a non-finite or out-of-range force (|F| >= 2^25 kJ/mol/nm): the job diverged */
+    GROMOS_AMD_EBOX = -8        /* a barostat shrank the box below the pair search's cell grid (5 % margin) */
 };
 
 /* CRFParameters (params.rs:60-68). */
@@ -304,6 +305,11 @@ typedef struct {
      * every force evaluation adds them, FP64 in both precisions, and e_pot
      * includes their energies (configuration.rs:197-207) */
     const gromos_amd_bonded *bonded;
+    /* pressure groups of the solute (topology PRESSUREGROUPS): the last atom of
+     * each, ascending, the last one n_solute_atoms - 1; every solvent molecule
+     * is a group of its own.  Read only with a molecular virial (params.virial 2) */
+    int32_t n_pressure_groups;
+    const int32_t *pressure_group_last;
 } gromos_amd_topology;
 
 /* Run parameters (IMD PAIRLIST / NONBONDED / STEP / PERTURBATION blocks). */
@@ -337,6 +343,15 @@ typedef struct {
      * nscm > 0 removes the translation every nscm steps, nscm < 0 the
      * translation and, without a periodic box, the rotation every |nscm| steps */
     int32_t nticom, nscm;
+    /* pressure coupling at the end of md steps (md.rs:1116-1129, 1259-1277;
+     * rectangular boxes): barostat 1 = isotropic Berendsen (berendsen_barostat.rs)
+     * toward p_ref (kJ mol^-1 nm^-3) with compressibility (mol nm^3 kJ^-1, the
+     * GROMOS input value) and tau_p (ps); the pressure (pressure_calculation.rs)
+     * from the virial: 1 atomic, 2 molecular (prepare_virial, forcefield.rs:285-296,
+     * 1128-1200), 0 none (the pressure stays 0) */
+    int32_t barostat;
+    double p_ref, compressibility, tau_p;
+    int32_t virial;
 } gromos_amd_params;
 
 #define GROMOS_AMD_NHC_MAX 16
@@ -363,6 +378,10 @@ int gromos_amd_system_forcefield_apply(gromos_amd_system *sys, double *forces, d
 /* Covalent energies of the last force evaluation (forcefield_apply, or the
  * last step of md): 4 per job, e_bond, e_angle, e_dihedral, e_improper. */
 int gromos_amd_system_covalent_energies(gromos_amd_system *sys, double *out);
+/* The periodic box edge lengths now (3 per job; a barostat changes them) and
+ * the isotropic pressure of each job's last md step with a virial (1 per job;
+ * kJ mol^-1 nm^-3); either output may be NULL. */
+int gromos_amd_system_box(gromos_amd_system *sys, double *box, double *pressure);
 /* nsteps of Forcefield -> LeapFrogVelocity -> LeapFrogPosition -> SETTLE ->
  * TemperatureCalculation (md.rs:1143-1286).  energies (optional, host): per step
  * 5 doubles (e_lj, e_crf, e_pot, e_kin, dhdl). */
@@ -457,6 +476,8 @@ int gromos_amd_session_step(gromos_amd_session *s, int32_t nsteps);
 int gromos_amd_job_query(gromos_amd_session *s, int64_t job, int32_t *state, int64_t *steps_done);
 /* Current positions / velocities (n_atoms * 3, host); either may be NULL. */
 int gromos_amd_job_fetch(gromos_amd_session *s, int64_t job, double *pos, double *vel);
+/* The job's box edge lengths now (3) and its last md step's pressure; either may be NULL. */
+int gromos_amd_job_box(gromos_amd_session *s, int64_t job, double *box, double *pressure);
 /* Energy records of steps [first, first + count): 5 doubles per step
  * (e_lj, e_crf, e_pot, e_kin, dhdl), as gromos_amd_system_md. */
 int gromos_amd_job_energies(gromos_amd_session *s, int64_t job, int64_t first, int64_t count, double *out);

@@ -226,6 +226,7 @@ struct gromos_amd_system {
     double* d_press = nullptr;             // per job: kinetic tensor (9), then the last pressure
     int32_t* d_pg_start = nullptr;         // pressure groups of the batch (atom indices)
     int32_t* d_box_fail = nullptr;         // per job: first step whose box outgrew the cell grid
+    double* d_cg_cvir = nullptr;           // per group: SETTLE's virial diagonal
     std::vector<int64_t> job_step0;        // per job: md steps done before the current call
     int64_t* d_job_step0 = nullptr;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one

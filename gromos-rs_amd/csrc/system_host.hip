@@ -738,6 +738,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_job_step0, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_press, (size_t)10 * njobs, true)) ||
         (rc = S->reserve(S->d_box_fail, (size_t)njobs, true)) ||
+        (rc = S->reserve(S->d_cg_cvir, any_baro ? 3 * nc : 1, true)) ||
         (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -1485,6 +1486,7 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.press = d_press;
     p.pg_start = d_pg_start;
     p.box_fail = d_box_fail;
+    p.cg_cvir = d_cg_cvir;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -2295,6 +2295,8 @@ __global__ void __launch_bounds__(BLOCK) k_press_ke(SysPtrs p) {
 //    forcefield.rs:1128-1200): per pressure group the centre of mass of the
 //    chain-gathered positions, then vir -= f_i (x) r_i with r_i the atom's image
 //    nearest that centre (the trace: the isotropic pressure reads no more);
+//  * the constraints' atomic virial (SETTLE, constraints.rs:978-1010), per
+//    group from k_integrate, added after that correction as in the reference;
 //  * PressureCalculation (pressure_calculation.rs:40-80): P = tr(KE + vir / 2)
 //    * 2 / V / 3 with this step's box;
 //  * BerendsenBarostat (berendsen_barostat.rs:30-70): mu = (1 - comp dt / tau_p
@@ -2303,7 +2305,7 @@ __global__ void __launch_bounds__(BLOCK) k_press_ke(SysPtrs p) {
 //    predicate (forcefield.rs:243-283) for the next step.  A box whose cells
 //    would fall below half the list cutoff stops the job (box_fail).
 __global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
-    __shared__ double lds[17 * 3];
+    __shared__ double lds[17 * 6];
     __shared__ double s_mu;
     const int32_t jb = blockIdx.x;
     // the barostat changes the job's box in place, read by the next step's kernels
@@ -2341,7 +2343,11 @@ __global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
             }
         }
     }
-    block_sum<3>(corr, lds);
+    // SETTLE's virial, per group from k_integrate
+    double w[6] = {corr[0], corr[1], corr[2], 0.0, 0.0, 0.0};
+    for (int32_t c = J.cg_off + threadIdx.x; c < J.cg_off + J.n_cg; c += blockDim.x)
+        for (int a = 0; a < 3; ++a) w[3 + a] += p.cg_cvir[3 * (int64_t)c + a];
+    block_sum<6>(w, lds);
     if (threadIdx.x == 0) {
         double* pr = p.press + 10 * (int64_t)jb;
         const double V = box.L[0] * box.L[1] * box.L[2];
@@ -2351,7 +2357,7 @@ __global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
             const double factor = 2.0 / V;
             double pt[3];
             for (int a = 0; a < 3; ++a) {
-                const double vir = rec[5 + 4 * a] - lds[48 + a];
+                const double vir = (rec[5 + 4 * a] - lds[96 + a]) + lds[96 + 3 + a];
                 pt[a] = (pr[4 * a] + vir * 0.5) * factor;
             }
             P = (pt[0] + pt[1] + pt[2]) / 3.0;
@@ -2460,10 +2466,23 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
                                       (float)q);
         }
     };
+    double cv[3] = {0.0, 0.0, 0.0};   // SETTLE's atomic virial diagonal (pressure-coupled jobs)
     if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
         v3 xo[3], xn[3], vo[3], vn[3];
         for (int k = 0; k < 3; ++k) step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
+        const v3 xu[3] = {xn[0], xn[1], xn[2]};
         if (!frozen && !settle_one(J, xo, xn, vn)) atomicMin(p.settle_fail + jb, step);
+        if (J.baro && !frozen) {
+            // constraints.rs:978-1010: constraint forces d m / dt^2 of the displacement,
+            // virial += x_old (x) f at the step's starting positions
+            const double dti = 1.0 / J.dt, dt2i = dti * dti;
+            const double mm[3] = {J.m_o, J.m_h, J.m_h};
+            for (int k = 0; k < 3; ++k) {
+                cv[0] += xo[k].x * ((xn[k].x - xu[k].x) * mm[k] * dt2i);
+                cv[1] += xo[k].y * ((xn[k].y - xu[k].y) * mm[k] * dt2i);
+                cv[2] += xo[k].z * ((xn[k].z - xu[k].z) * mm[k] * dt2i);
+            }
+        }
         for (int k = 0; k < 3; ++k) finish_atom(s + k, xn[k], vo[k], vn[k], xn[0]);
     } else {
         v3 x_first = mk(0.0, 0.0, 0.0);
@@ -2493,6 +2512,8 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
     p.cg_ek[3 * (int64_t)c] = ek;
     p.cg_ek[3 * (int64_t)c + 1] = ekn;
     p.cg_ek[3 * (int64_t)c + 2] = ex;
+    if (J.baro)
+        for (int a = 0; a < 3; ++a) p.cg_cvir[3 * (int64_t)c + a] = cv[a];
 }
 
 

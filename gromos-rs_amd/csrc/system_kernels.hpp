@@ -96,6 +96,7 @@ struct SysPtrs {
     double* press;             // per job: kinetic energy tensor (9), then the last pressure
     const int32_t* pg_start;   // pressure groups (atom index starts; JobDev pg_off / n_pg)
     int32_t* box_fail;         // per job: first step whose box outgrew the cell grid
+    double* cg_cvir;           // per group: SETTLE's virial diagonal (pressure-coupled jobs)
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;

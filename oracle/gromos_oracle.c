@@ -1879,8 +1879,10 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
             or_leapfrog(n, md->inv_mass, md->dt, md->pos[c], md->vel[c], md->force[c], md->pos[o], md->vel[o]);
         }
         if (md->use_settle) {
+            /* the constraint's atomic virial joins the pressure's virial after the
+             * molecular correction (constraints.rs:978-1010) */
             int err = or_settle(md->t->n_solute_atoms, n, md->m_o, md->m_h, md->d_oh, md->d_hh, md->dt,
-                                md->pos[c], md->pos[o], md->vel[o], NULL, NULL);
+                                md->pos[c], md->pos[o], md->vel[o], NULL, md->baro ? vir : NULL);
             if (err) return -1;
         }
         double ek = or_ekin(n, md->t->mass, md->vel[o], md->vel[c]);

@@ -53,6 +53,24 @@ def test_npt_md_matches_oracle(vtype):
         assert np.abs(box - np.array(d["final_box"])).max() <= 5.01e-10
 
 
+@pytest.mark.parametrize("vtype", [2, 1])
+def test_npt_settle_water_matches_oracle(vtype):
+    """rigid SETTLE waters under the barostat: SETTLE's constraint virial joins
+    the pressure (constraints.rs:978-1010); each molecule its own pressure group"""
+    w = gen.spc_water_box(400, seed=12)
+    w.barostat = (0.06102, 4.575e-4, 0.5, vtype)
+    n = 15
+    md = oracle.MD(oracle.OracleSystem(w), use_settle=True)
+    e_ref = md.run(n)
+    x_ref, _, _ = md.state()
+    e, x, v, box, p = device_md(w, n)
+    for c in range(4):
+        assert rel(e[:, c], e_ref[:, c]).max() < 1e-10, (c, rel(e[:, c], e_ref[:, c]).max())
+    assert np.abs(x - x_ref).max() < 1e-10
+    assert rel(box, md.box()).max() < 1e-13 and rel(p, md.pressure()) < 1e-9, (p, md.pressure())
+    assert box[0] != w.box[0]
+
+
 def test_npt_mixed_precision_follows_fp64():
     s, d = tn.with_npt()
     _, _, _, b64, p64 = device_md(s, 20)

@@ -133,6 +133,14 @@ class System:
     # rectangular boxes: (P0 kJ mol^-1 nm^-3, compressibility, tau_p ps, virial
     # 0 none / 1 atomic / 2 molecular)
     barostat: Optional[tuple] = None
+    # SHAKE (constraints.rs:213-570): solvent_shake constrains every solvent
+    # molecule by the template solvent_constraints ((i, j, length) within the
+    # molecule) instead of SETTLE; solute_constraints (i, j, length) are the
+    # solute's (NTC 2 / 3); tolerance NTCP0 / NTCS0
+    solvent_shake: bool = False
+    solvent_constraints: Optional[np.ndarray] = None
+    solute_constraints: Optional[np.ndarray] = None
+    shake_tolerance: float = 1e-4
     # last atom of each solute pressure group (topology PRESSUREGROUPS); every
     # solvent molecule is a group of its own
     pressure_groups: Optional[np.ndarray] = None

@@ -1508,6 +1508,10 @@ struct or_md {
     int nhc;                    /* Nose-Hoover chain length */
     double zeta[OR_NHC_MAX];    /* the chain's state, zero at start */
     int nticom, nscm;           /* centre-of-mass motion removal (INITIALISE NTICOM, COMTRANSROT NSCM) */
+    int n_shake;                /* SHAKE constraints: the solute's, then the solvent's */
+    int32_t *sh_i, *sh_j;
+    double *sh_len;
+    double sh_tol;
     int baro;                   /* Berendsen barostat, isotropic */
     double p_ref, comp, tau_p;
     int vtype;                  /* virial: 0 none, 1 atomic, 2 molecular */
@@ -1554,6 +1558,9 @@ void or_md_set_settle_geometry(or_md *md, double d_oh, double d_hh) { md->d_oh =
 void or_md_destroy(or_md *md) {
     if (!md) return;
     free(md->pg_start);
+    free(md->sh_i);
+    free(md->sh_j);
+    free(md->sh_len);
     or_ff_destroy(md->ff);
     for (int k = 0; k < 2; ++k) { free(md->pos[k]); free(md->vel[k]); free(md->force[k]); }
     free(md->inv_mass);
@@ -1720,6 +1727,106 @@ static void or_molecular_virial(const or_md *md, const double *pos, const double
     for (int k = 0; k < 9; ++k) vir[k] = vir[k] - corr[k];
 }
 
+/* shake (constraints.rs:213-389) with shake_one_constraint_full
+ * (constraints.rs:501-570): sweeps over the constraints in order (the
+ * solute's, then every solvent molecule's) until a sweep changes nothing, an
+ * atom's constraints skipped in a sweep when no constraint moved it in the
+ * previous one; each unconverged constraint (|l^2 - r^2| >= 2 tol l^2) moves
+ * both atoms along the old bond vector by lambda = (l^2 - r^2) / (2 (old . r)
+ * (1/m_i + 1/m_j)); then v = (x_new - x_old) / dt for every constrained atom.
+ * vir (optional) += old bond (x) old bond * lambda / dt^2.  Returns the sweeps
+ * done, or -1 when max_iter sweeps did not converge. */
+int or_shake(int n_atoms, const double *inv_mass, int nc, const int32_t *ci, const int32_t *cj, const double *len,
+             double tol, int max_iter, double dt, const double *pos_old, double *pos_new, double *vel_new,
+             double *vir) {
+    const double dt2 = dt * dt;
+    unsigned char *skip_now = (unsigned char *)calloc((size_t)n_atoms, 1);
+    unsigned char *skip_next = (unsigned char *)malloc((size_t)n_atoms);
+    memset(skip_next, 1, (size_t)n_atoms);
+    int converged = 0, it = 0;
+    while (it < max_iter && !converged) {
+        converged = 1;
+        ++it;
+        for (int k = 0; k < nc; ++k) {
+            const int i = ci[k], j = cj[k];
+            if (skip_now[i] && skip_now[j]) continue;
+            if (inv_mass[i] == 0.0 && inv_mass[j] == 0.0) continue;
+            const double l2 = len[k] * len[k];
+            double r[3], rr[3];
+            for (int a = 0; a < 3; ++a) {
+                r[a] = pos_new[3 * i + a] - pos_new[3 * j + a];
+                rr[a] = pos_old[3 * i + a] - pos_old[3 * j + a];
+            }
+            const double dist2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const double diff = l2 - dist2;
+            if (fabs(diff) < l2 * tol * 2.0) continue;
+            const double sp = rr[0] * r[0] + rr[1] * r[1] + rr[2] * r[2];
+            const double ims = inv_mass[i] + inv_mass[j];
+            if (ims < 1e-20) continue;
+            const double lambda = diff / (sp * 2.0 * ims);
+            if (vir) {
+                const double lo = lambda / dt2;
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) vir[3 * a + b] += rr[a] * rr[b] * lo;
+            }
+            for (int a = 0; a < 3; ++a) {
+                const double corr = rr[a] * lambda;
+                pos_new[3 * i + a] += corr * inv_mass[i];
+                pos_new[3 * j + a] -= corr * inv_mass[j];
+            }
+            skip_next[i] = 0;
+            skip_next[j] = 0;
+            converged = 0;
+        }
+        unsigned char *t = skip_now;
+        skip_now = skip_next;
+        skip_next = t;
+        memset(skip_next, 1, (size_t)n_atoms);
+    }
+    for (int k = 0; k < nc; ++k)
+        for (int s = 0; s < 2; ++s) {
+            const int a0 = s ? cj[k] : ci[k];
+            for (int a = 0; a < 3; ++a) vel_new[3 * a0 + a] = (pos_new[3 * a0 + a] - pos_old[3 * a0 + a]) * (1.0 / dt);
+        }
+    free(skip_now);
+    free(skip_next);
+    return converged ? it : -1;
+}
+
+/* SHAKE in the md step (md.rs:1208-1236): the solute constraints (NTC 2: bonds
+ * with a hydrogen, 3: all bonds; chosen by the caller) and, for NTCS shake,
+ * every solvent molecule's template constraints; tolerance NTCP0 / NTCS0 */
+int or_md_set_shake(or_md *md, int n_solute_c, const int32_t *sol_ij, const double *sol_len, int n_template,
+                    const int32_t *tmpl_ij, const double *tmpl_len, double tol) {
+    const or_topo *t = md->t;
+    const int aps = t->atoms_per_solvent;
+    const int nsm = aps > 0 ? (t->n_atoms - t->n_solute_atoms) / aps : 0;
+    const int n = n_solute_c + (n_template > 0 ? nsm * n_template : 0);
+    free(md->sh_i);
+    free(md->sh_j);
+    free(md->sh_len);
+    md->sh_i = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    md->sh_j = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    md->sh_len = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int k = 0;
+    for (int c = 0; c < n_solute_c; ++c, ++k) {
+        md->sh_i[k] = sol_ij[2 * c];
+        md->sh_j[k] = sol_ij[2 * c + 1];
+        md->sh_len[k] = sol_len[c];
+    }
+    if (n_template > 0)
+        for (int m = 0; m < nsm; ++m)
+            for (int c = 0; c < n_template; ++c, ++k) {
+                const int base = t->n_solute_atoms + m * aps;
+                md->sh_i[k] = base + tmpl_ij[2 * c];
+                md->sh_j[k] = base + tmpl_ij[2 * c + 1];
+                md->sh_len[k] = tmpl_len[c];
+            }
+    md->n_shake = k;
+    md->sh_tol = tol;
+    return 0;
+}
+
 /* RemoveCOMMotion::new (remove_com_motion.rs:62-85): NTICOM >= 1 removes the
  * translation at step 0, >= 2 the rotation too; NSCM > 0 removes the
  * translation every NSCM steps, NSCM < 0 translation and rotation every |NSCM| */
@@ -1877,6 +1984,11 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
                 for (int a = 0; a < 3; ++a) md->pos[o][3 * i + a] = md->pos[c][3 * i + a] + md->vel[o][3 * i + a] * md->dt;
         } else {
             or_leapfrog(n, md->inv_mass, md->dt, md->pos[c], md->vel[c], md->force[c], md->pos[o], md->vel[o]);
+        }
+        if (md->n_shake > 0) {
+            int it = or_shake(n, md->inv_mass, md->n_shake, md->sh_i, md->sh_j, md->sh_len, md->sh_tol, 1000, md->dt,
+                              md->pos[c], md->pos[o], md->vel[o], md->baro ? vir : NULL);
+            if (it < 0) return -1;
         }
         if (md->use_settle) {
             /* the constraint's atomic virial joins the pressure's virial after the

@@ -290,6 +290,13 @@ double or_berendsen_scale_ekin(double ekin, double t_ref, double tau, double dof
 int or_md_set_barostat(or_md *md, double p_ref, double comp, double tau_p, int vtype, int n_pg,
                        const int32_t *pg_last);
 double or_md_pressure(const or_md *md);
+/* SHAKE (constraints.rs:213-570) in the md step: solute constraints (pairs,
+ * lengths) then each solvent molecule's template constraints */
+int or_md_set_shake(or_md *md, int n_solute_c, const int32_t *sol_ij, const double *sol_len, int n_template,
+                    const int32_t *tmpl_ij, const double *tmpl_len, double tol);
+int or_shake(int n_atoms, const double *inv_mass, int nc, const int32_t *ci, const int32_t *cj, const double *len,
+             double tol, int max_iter, double dt, const double *pos_old, double *pos_new, double *vel_new,
+             double *vir);
 void or_md_box(const or_md *md, double *L);
 /* centre-of-mass motion removal (remove_com_motion.rs) at the start of each md step */
 void or_md_set_com(or_md *md, int nticom, int nscm);

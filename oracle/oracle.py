@@ -165,6 +165,9 @@ def _declare(L):
     L.or_md_set_com.argtypes = [P, C.c_int, C.c_int]
     L.or_md_set_barostat.argtypes = [P, D, D, D, C.c_int, C.c_int, P]
     L.or_md_set_barostat.restype = C.c_int
+    L.or_md_set_shake.argtypes = [P, C.c_int, P, P, C.c_int, P, P, D]
+    L.or_shake.argtypes = [C.c_int, P, C.c_int, P, P, P, D, C.c_int, D, P, P, P, P]
+    L.or_shake.restype = C.c_int
     L.or_md_pressure.argtypes = [P]
     L.or_md_pressure.restype = D
     L.or_md_box.argtypes = [P, P]
@@ -530,6 +533,15 @@ class MD:
                 lib().or_md_set_berendsen(self.h, t_ref, tau, dof)
         if getattr(s, "nticom", 0) or getattr(s, "nscm", 0):
             lib().or_md_set_com(self.h, int(s.nticom), int(s.nscm))
+        sol_c = getattr(s, "solute_constraints", None)
+        tmpl = getattr(s, "solvent_constraints", None) if getattr(s, "solvent_shake", False) else None
+        if (sol_c is not None and len(sol_c)) or tmpl is not None:
+            sc = np.zeros((0, 3)) if sol_c is None else np.asarray(sol_c, float).reshape(-1, 3)
+            tc = np.zeros((0, 3)) if tmpl is None else np.asarray(tmpl, float).reshape(-1, 3)
+            self.sh = (np.ascontiguousarray(sc[:, :2], np.int32), np.ascontiguousarray(sc[:, 2]),
+                       np.ascontiguousarray(tc[:, :2], np.int32), np.ascontiguousarray(tc[:, 2]))
+            lib().or_md_set_shake(self.h, len(sc), ptr(self.sh[0]), ptr(self.sh[1]), len(tc), ptr(self.sh[2]),
+                                  ptr(self.sh[3]), float(s.shake_tolerance))
         if getattr(s, "barostat", None) is not None:
             p_ref, comp, tau_p, vtype = s.barostat
             self.pg = np.ascontiguousarray(s.pressure_group_last(), dtype=np.int32)
@@ -541,7 +553,7 @@ class MD:
         e = np.zeros((nsteps, 5))
         rc = lib().or_md_run(self.h, nsteps, ptr(e))
         if rc != 0:
-            raise RuntimeError("SETTLE failed in the oracle")
+            raise RuntimeError("SETTLE or SHAKE failed in the oracle")
         return e
 
     def state(self):

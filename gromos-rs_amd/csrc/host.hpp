@@ -59,6 +59,11 @@ struct JobImage {
     std::vector<double> pa_q, pb_q, palj, pacrf;
     BondedTerms bonded;             // covalent terms of the solute
     std::vector<int32_t> pg_start;  // pressure groups (job atoms), n_pg + 1; empty: none
+    struct Constraint {
+        int32_t i, j;
+        double len;
+    };
+    std::vector<Constraint> shake;  // the solute's SHAKE constraints (job atoms)
     int64_t footprint = 0;          // device bytes this job adds to a batch (estimate)
     int64_t list_entries = 0;       // full neighbour-list entries of the job (estimate)
 };
@@ -227,6 +232,12 @@ struct gromos_amd_system {
     int32_t* d_pg_start = nullptr;         // pressure groups of the batch (atom indices)
     int32_t* d_box_fail = nullptr;         // per job: first step whose box outgrew the cell grid
     double* d_cg_cvir = nullptr;           // per group: SETTLE's virial diagonal
+    bool any_solute_shake = false;
+    std::vector<int32_t> host_sh_ij;       // the batch's solute constraints (atom pairs)
+    std::vector<double> host_sh_len;
+    int32_t* d_sh_ij = nullptr;
+    double* d_sh_len = nullptr;
+    uint8_t* d_sh_skip = nullptr;
     std::vector<int64_t> job_step0;        // per job: md steps done before the current call
     int64_t* d_job_step0 = nullptr;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one

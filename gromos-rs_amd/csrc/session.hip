@@ -466,7 +466,7 @@ extern "C" int gromos_amd_session_step(gromos_amd_session* s, int32_t nsteps) {
                         snprintf(msg, sizeof(msg), "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm) (step %lld)",
                                  FORCE_LIMIT, (long long)(j->done + failed[b]));
                     else
-                        snprintf(msg, sizeof(msg), "SETTLE failed: sin(angle) > 1.0 in analytical solve (step %lld)",
+                        snprintf(msg, sizeof(msg), "constraints failed: SETTLE sin(angle) > 1.0 or SHAKE unconverged after 1000 sweeps (step %lld)",
                                  (long long)(j->done + failed[b]));
                     j->error = msg;
                 }

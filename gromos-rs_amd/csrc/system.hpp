@@ -62,6 +62,12 @@ struct JobDev {
     double cell_min;                       // the pair search's least cell width (half the list cutoff)
     double cgg_limit;                      // CG-grouped loops while half the shortest edge exceeds it
     int32_t pg_off, n_pg;                  // pressure groups [pg_start[pg_off + g], pg_start[pg_off + g + 1])
+    int32_t shake_sv;                      // SHAKE the solvent molecules (template below)
+    int32_t n_sc;                          // template constraints (<= 6), atom pairs within a molecule
+    int32_t sc_ij[12];
+    double sc_len[6];
+    double sh_tol;                         // SHAKE tolerance
+    int32_t sh_off, sh_n;                  // the solute's constraints in the batch arrays
     int32_t bd_off;                        // first covalent term (batch order)
     int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };

@@ -119,6 +119,7 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     if (!(P.dt > 0.0)) return fail(GROMOS_AMD_EINVAL, "job %d: dt must be > 0", j);
     im = JobImage{};
     im.P = P;
+    if (P.solvent_shake) im.P.use_settle = 0;   // SHAKE constrains the solvent instead
     im.n = n;
     im.n_cg = T.n_cg;
     im.n_types = T.n_types;
@@ -228,14 +229,42 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
             }
     }
     J.has_pert = im.has_pert ? 1 : 0;
-    J.settle = (P.use_settle && nsolv_atoms > 0 && aps == 3) ? 1 : 0;
+    J.settle = (im.P.use_settle && nsolv_atoms > 0 && aps == 3) ? 1 : 0;
+    // SHAKE: the solvent template and the solute's constraints
+    J.sh_tol = P.shake_tolerance > 0.0 ? P.shake_tolerance : 1e-4;
+    J.shake_sv = 0;
+    J.n_sc = 0;
+    if (P.solvent_shake && nsolv_atoms > 0) {
+        const int32_t nc = T.n_solvent_constraints;
+        if (nc < 1 || nc > 6 || !T.solvent_constraint_ij || !T.solvent_constraint_len)
+            return fail(GROMOS_AMD_EINVAL, "job %d: solvent SHAKE needs 1..6 template constraints", j);
+        for (int32_t c = 0; c < nc; ++c) {
+            const int32_t a = T.solvent_constraint_ij[2 * c], b = T.solvent_constraint_ij[2 * c + 1];
+            if (a < 0 || b < 0 || a >= aps || b >= aps || a == b || !(T.solvent_constraint_len[c] > 0.0))
+                return fail(GROMOS_AMD_EINVAL, "job %d: solvent constraint %d out of range", j, c);
+            J.sc_ij[2 * c] = a;
+            J.sc_ij[2 * c + 1] = b;
+            J.sc_len[c] = T.solvent_constraint_len[c];
+        }
+        J.n_sc = nc;
+        J.shake_sv = 1;
+    }
+    im.shake.clear();
+    for (int32_t c = 0; c < T.n_solute_constraints; ++c) {
+        const int32_t a = T.solute_constraint_ij ? T.solute_constraint_ij[2 * c] : -1;
+        const int32_t b = T.solute_constraint_ij ? T.solute_constraint_ij[2 * c + 1] : -1;
+        const double l = T.solute_constraint_len ? T.solute_constraint_len[c] : 0.0;
+        if (a < 0 || b < 0 || a >= T.n_solute_atoms || b >= T.n_solute_atoms || a == b || !(l > 0.0))
+            return fail(GROMOS_AMD_EINVAL, "job %d: solute constraint %d out of range", j, c);
+        im.shake.push_back({a, b, l});
+    }
     // from_lambda(l, l, l, l, 1, 1, 1, 1, NLAM) (forcefield.rs:767-777)
     {
         gromos_amd_lambda L;
         gromos_amd_lambda_from(P.lambda, P.lambda, P.lambda, P.lambda, 1.0, 1.0, 1.0, 1.0, P.nlam, &L);
         std::memcpy(&J.lp, &L, sizeof(L));
     }
-    if (P.use_settle && nsolv_atoms > 0) {
+    if (im.P.use_settle && nsolv_atoms > 0) {
         if (aps != 3) return fail(GROMOS_AMD_EINVAL, "SETTLE needs 3-atom solvent (settle_algorithm.rs:43-45)");
         if (T.mass[T.n_solute_atoms + 1] != T.mass[T.n_solute_atoms + 2])
             return fail(GROMOS_AMD_EINVAL, "SETTLE needs equal H masses (settle_algorithm.rs:46-51)");
@@ -363,7 +392,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     std::vector<Slice> sl(njobs + 1);
     int64_t cell_off = 0;
     bool any_pert = false, twin = false, any_thermo = false, any_com = false, any_baro = false;
+    bool any_solute_shake = false;
     S->host_pg.clear();
+    S->host_sh_ij.clear();
+    S->host_sh_len.clear();
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -396,6 +428,14 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         J.pg_off = (int32_t)S->host_pg.size();
         J.n_pg = im.pg_start.empty() ? 0 : (int32_t)im.pg_start.size() - 1;
         for (int32_t x : im.pg_start) S->host_pg.push_back((int32_t)o.atom + x);
+        J.sh_off = (int32_t)S->host_sh_len.size();
+        J.sh_n = (int32_t)im.shake.size();
+        for (const auto& c : im.shake) {
+            S->host_sh_ij.push_back((int32_t)o.atom + c.i);
+            S->host_sh_ij.push_back((int32_t)o.atom + c.j);
+            S->host_sh_len.push_back(c.len);
+        }
+        any_solute_shake = any_solute_shake || J.sh_n > 0;
         use_settle |= im.P.use_settle && !im.mol_first.empty();
         do_virial |= im.P.do_virial;
         S->jobs.push_back(J);
@@ -538,6 +578,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->any_thermo = any_thermo;
     S->any_com = any_com;
     S->any_baro = any_baro;
+    S->any_solute_shake = any_solute_shake;
     S->job_step0.assign((size_t)njobs, 0);
     S->host_pressure.assign((size_t)njobs, 0.0);
     S->near_r = VV_NEAR_R;
@@ -652,6 +693,8 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     int rc = 0;
     rc = rc ? rc : S->upload(S->d_jobs, S->jobs);
     rc = rc ? rc : S->upload(S->d_pg_start, S->host_pg);
+    rc = rc ? rc : S->upload(S->d_sh_ij, S->host_sh_ij);
+    rc = rc ? rc : S->upload(S->d_sh_len, S->host_sh_len);
     rc = rc ? rc : S->upload(S->d_cg_start, cg_start);
     rc = rc ? rc : S->upload(S->d_cg_n, cg_n);
     rc = rc ? rc : S->upload(S->d_cg_job, cg_job);
@@ -739,6 +782,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_press, (size_t)10 * njobs, true)) ||
         (rc = S->reserve(S->d_box_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_cg_cvir, any_baro ? 3 * nc : 1, true)) ||
+        (rc = S->reserve(S->d_sh_skip, any_solute_shake ? 2 * (size_t)npad : 1, true)) ||
         (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -1222,7 +1266,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // thermostat rescan, no per-kernel timing events): one captured graph
         const bool plain = S->use_graphs && S->built && S->steps_since_update + 1 < S->nsnb && S->xq_valid &&
                            !(S->any_thermo && S->therm_dirty) && !S->timing && !(S->twin && !S->long_computed) &&
-                           !S->any_com && !S->any_baro &&
+                           !S->any_com && !S->any_baro && !S->any_solute_shake &&
                            S->vv_lanes > 0;
         if (plain) {
             S->steps_since_update++;
@@ -1249,11 +1293,13 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // step rebuilds the list (pairlist.rs:105-107)
         const int32_t write_xq = S->steps_since_update < S->nsnb ? 1 : 0;
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, write_xq, mixed, st);
+        if (S->any_solute_shake) launch_shake_solute(p, S->njobs, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
         if (S->any_baro) launch_baro(p, S->njobs, st);
         S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
         // the barostat scaled the new positions after the integrator copied them
-        S->xq_valid = write_xq != 0 && !S->any_baro;
+        // (and the solute's SHAKE moved positions after that copy)
+        S->xq_valid = write_xq != 0 && !S->any_baro && !S->any_solute_shake;
         if (S->step_timing)
             if (int rc = step_event()) return rc;
     }
@@ -1314,7 +1360,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
             if (kind[j])
                 return fail(GROMOS_AMD_EFORCE, "non-finite or out-of-range force (|F| >= %.3g kJ/mol/nm; job %d, step %d)",
                             FORCE_LIMIT, j, failed[j]);
-            return fail(GROMOS_AMD_ESETTLE, "SETTLE failed: sin(angle) > 1.0 in analytical solve (job %d, step %d)", j, failed[j]);
+            return fail(GROMOS_AMD_ESETTLE, "constraints failed: SETTLE sin(angle) > 1.0 or SHAKE unconverged after 1000 sweeps (job %d, step %d)", j, failed[j]);
         }
     return 0;
 }
@@ -1487,6 +1533,9 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.pg_start = d_pg_start;
     p.box_fail = d_box_fail;
     p.cg_cvir = d_cg_cvir;
+    p.sh_ij = d_sh_ij;
+    p.sh_len = d_sh_len;
+    p.sh_skip = d_sh_skip;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -2118,6 +2118,162 @@ __device__ __forceinline__ bool settle_one(const JobDev& J, const v3 (&po)[3], v
     return true;
 }
 
+// shake (constraints.rs:213-389) of one solvent molecule held in registers:
+// sweeps over the template constraints until a sweep changes nothing (an atom's
+// constraints skipped when no constraint moved it in the previous sweep); an
+// unconverged constraint (|l^2 - r^2| >= 2 tol l^2) moves both atoms along
+// the old bond vector by lambda = (l^2 - r^2) / (2 (old . r) (1/m_i + 1/m_j))
+// (shake_one_constraint_full, constraints.rs:501-570); then v = (x_new -
+// x_old) / dt for the constrained atoms.  cv += the constraint virial's
+// diagonal (old bond^2 lambda / dt^2).  False after 1000 sweeps unconverged.
+__device__ __forceinline__ v3 pick4(const v3 (&a)[4], int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
+}
+__device__ __forceinline__ void put4(v3 (&a)[4], int i, v3 v) {
+    if (i == 0) a[0] = v;
+    if (i == 1) a[1] = v;
+    if (i == 2) a[2] = v;
+    if (i == 3) a[3] = v;
+}
+__device__ bool shake_molecule(const JobDev& J, int n, const v3 (&xo)[4], v3 (&xn)[4], v3 (&vn)[4],
+                               const double (&im)[4], double (&cv)[3]) {
+    const double dt2 = J.dt * J.dt;
+    unsigned skip_now = 0u, constrained = 0u;
+    bool conv = false;
+    int it = 0;
+    for (int k = 0; k < J.n_sc; ++k) constrained |= (1u << J.sc_ij[2 * k]) | (1u << J.sc_ij[2 * k + 1]);
+    while (it < 1000 && !conv) {
+        conv = true;
+        ++it;
+        unsigned moved = 0u;
+        for (int k = 0; k < J.n_sc; ++k) {
+            const int i = J.sc_ij[2 * k], j = J.sc_ij[2 * k + 1];
+            if ((skip_now >> i & 1u) && (skip_now >> j & 1u)) continue;
+            const double imi = i == 0 ? im[0] : (i == 1 ? im[1] : (i == 2 ? im[2] : im[3]));
+            const double imj = j == 0 ? im[0] : (j == 1 ? im[1] : (j == 2 ? im[2] : im[3]));
+            if (imi == 0.0 && imj == 0.0) continue;
+            const double l2 = J.sc_len[k] * J.sc_len[k];
+            const v3 ni = pick4(xn, i), nj = pick4(xn, j), oi = pick4(xo, i), oj = pick4(xo, j);
+            const v3 r = mk(ni.x - nj.x, ni.y - nj.y, ni.z - nj.z);
+            const double diff = l2 - (r.x * r.x + r.y * r.y + r.z * r.z);
+            if (fabs(diff) < l2 * J.sh_tol * 2.0) continue;
+            const v3 rr = mk(oi.x - oj.x, oi.y - oj.y, oi.z - oj.z);
+            const double sp = rr.x * r.x + rr.y * r.y + rr.z * r.z;
+            const double ims = imi + imj;
+            if (ims < 1e-20) continue;
+            const double lambda = diff / (sp * 2.0 * ims);
+            const double lo = lambda / dt2;
+            cv[0] += rr.x * rr.x * lo;
+            cv[1] += rr.y * rr.y * lo;
+            cv[2] += rr.z * rr.z * lo;
+            const v3 cr = mk(rr.x * lambda, rr.y * lambda, rr.z * lambda);
+            put4(xn, i, mk(ni.x + cr.x * imi, ni.y + cr.y * imi, ni.z + cr.z * imi));
+            put4(xn, j, mk(nj.x - cr.x * imj, nj.y - cr.y * imj, nj.z - cr.z * imj));
+            moved |= (1u << i) | (1u << j);
+            conv = false;
+        }
+        skip_now = ~moved;
+    }
+    const double dti = 1.0 / J.dt;
+    for (int a = 0; a < n; ++a)
+        if (constrained >> a & 1u) {
+            const v3 q = pick4(xn, a), o = pick4(xo, a);
+            put4(vn, a, mk((q.x - o.x) * dti, (q.y - o.y) * dti, (q.z - o.z) * dti));
+        }
+    return conv;
+}
+
+// shake of a job's solute constraints (constraints.rs:213-389, the solute's
+// list in the caller's order), a block per job after k_integrate: one thread
+// sweeps (the constraints couple the solute's charge groups), skip flags in
+// p.sh_skip; then v = (x_new - x_old) / dt for the constrained atoms, the
+// solute groups' E_kin terms again (as k_integrate writes them) and, for a
+// pressure-coupled job, the constraint virial into the first group's row.
+__global__ void __launch_bounds__(BLOCK) k_shake_solute(SysPtrs p) {
+    const int32_t jb = blockIdx.x;
+    const JobDev& J = p.jobs[jb];
+    if (J.sh_n == 0) return;
+    const int32_t step = *p.step;
+    if (step >= J.stop) return;
+    const int32_t* ij = p.sh_ij + 2 * (int64_t)J.sh_off;
+    const double* len = p.sh_len + J.sh_off;
+    double* xn = p.x_new;
+    const double* xo = p.x_cur;
+    if (threadIdx.x == 0) {
+        uint8_t* now = p.sh_skip + 2 * (int64_t)J.atom_off;   // two flags per atom
+        for (int32_t a = 0; a < J.n_solute_atoms; ++a) {
+            now[2 * a] = 0;
+            now[2 * a + 1] = 1;
+        }
+        const double dt2 = J.dt * J.dt;
+        double cv[3] = {0.0, 0.0, 0.0};
+        bool conv = false;
+        int it = 0, cur = 0;
+        while (it < 1000 && !conv) {
+            conv = true;
+            ++it;
+            for (int32_t k = 0; k < J.sh_n; ++k) {
+                const int32_t i = ij[2 * k], j = ij[2 * k + 1];
+                const int32_t li = i - J.atom_off, lj = j - J.atom_off;
+                if (now[2 * li + cur] && now[2 * lj + cur]) continue;
+                const double imi = p.inv_mass[i], imj = p.inv_mass[j];
+                if (imi == 0.0 && imj == 0.0) continue;
+                const double l2 = len[k] * len[k];
+                const d3 ni = ld3(xn, i), nj = ld3(xn, j), oi = ld3(xo, i), oj = ld3(xo, j);
+                const double rx = ni.x - nj.x, ry = ni.y - nj.y, rz = ni.z - nj.z;
+                const double diff = l2 - (rx * rx + ry * ry + rz * rz);
+                if (fabs(diff) < l2 * J.sh_tol * 2.0) continue;
+                const double ax = oi.x - oj.x, ay = oi.y - oj.y, az = oi.z - oj.z;
+                const double sp = ax * rx + ay * ry + az * rz;
+                const double ims = imi + imj;
+                if (ims < 1e-20) continue;
+                const double lambda = diff / (sp * 2.0 * ims);
+                const double lo = lambda / dt2;
+                cv[0] += ax * ax * lo;
+                cv[1] += ay * ay * lo;
+                cv[2] += az * az * lo;
+                const double cx = ax * lambda, cy = ay * lambda, cz = az * lambda;
+                xn[3 * (int64_t)i] = ni.x + cx * imi;
+                xn[3 * (int64_t)i + 1] = ni.y + cy * imi;
+                xn[3 * (int64_t)i + 2] = ni.z + cz * imi;
+                xn[3 * (int64_t)j] = nj.x - cx * imj;
+                xn[3 * (int64_t)j + 1] = nj.y - cy * imj;
+                xn[3 * (int64_t)j + 2] = nj.z - cz * imj;
+                now[2 * li + (1 - cur)] = 0;
+                now[2 * lj + (1 - cur)] = 0;
+                conv = false;
+            }
+            // the next sweep's flags become the current ones; the others start set
+            cur = 1 - cur;
+            for (int32_t a = 0; a < J.n_solute_atoms; ++a) now[2 * a + (1 - cur)] = 1;
+        }
+        if (!conv) atomicMin(p.settle_fail + jb, step);
+        const double dti = 1.0 / J.dt;
+        double* vn = p.v_new;
+        for (int32_t k = 0; k < J.sh_n; ++k)
+            for (int s = 0; s < 2; ++s) {
+                const int64_t a = ij[2 * k + s];
+                for (int c = 0; c < 3; ++c) vn[3 * a + c] = (xn[3 * a + c] - xo[3 * a + c]) * dti;
+            }
+        if (J.baro)
+            for (int c = 0; c < 3; ++c) p.cg_cvir[3 * (int64_t)J.cg_off + c] += cv[c];
+    }
+    __syncthreads();
+    // the solute groups' E_kin terms from the constrained velocities (k_integrate's order)
+    for (int32_t c = J.cg_off + threadIdx.x; c < J.cg_off + J.n_solute_cg; c += blockDim.x) {
+        double ek = 0.0, ekn = 0.0;
+        for (int32_t i = p.cg_start[c]; i < p.cg_start[c] + p.cg_n[c]; ++i) {
+            const d3 vn = ld3(p.v_new, i), vo = ld3(p.v_cur, i);
+            const double a2 = vn.x * vn.x + vn.y * vn.y + vn.z * vn.z;
+            const double b2 = vo.x * vo.x + vo.y * vo.y + vo.z * vo.z;
+            ek += 0.5 * p.mass[i] * (a2 + b2) / 2.0;
+            ekn += 0.5 * p.mass[i] * a2;
+        }
+        p.cg_ek[3 * (int64_t)c] = ek;
+        p.cg_ek[3 * (int64_t)c + 1] = ekn;
+    }
+}
+
 // The bath's scale of each coupled job (Berendsen, berendsen_thermostat.rs:66-100,
 // or Nose-Hoover, nosehoover_thermostat.rs:116-227) from E_kin_new = 0.5 sum m
 // |v|^2 of the velocities entering the step (what the previous
@@ -2466,8 +2622,22 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
                                       (float)q);
         }
     };
-    double cv[3] = {0.0, 0.0, 0.0};   // SETTLE's atomic virial diagonal (pressure-coupled jobs)
-    if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
+    double cv[3] = {0.0, 0.0, 0.0};   // the constraints' virial diagonal (pressure-coupled jobs)
+    if (J.shake_sv && c - J.cg_off >= J.n_solute_cg) {
+        // a solvent molecule constrained by SHAKE with the template (n <= 4 atoms)
+        v3 xo[4], xn[4], vo[4], vn[4];
+        double im[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k) xo[k] = xn[k] = vo[k] = vn[k] = mk(0.0, 0.0, 0.0);
+        for (int k = 0; k < n && k < 4; ++k) {
+            step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
+            im[k] = p.inv_mass[s + k];
+        }
+        double sv[3] = {0.0, 0.0, 0.0};
+        if (!frozen && !shake_molecule(J, n, xo, xn, vn, im, sv)) atomicMin(p.settle_fail + jb, step);
+        if (J.baro && !frozen)
+            for (int a = 0; a < 3; ++a) cv[a] = sv[a];
+        for (int k = 0; k < n && k < 4; ++k) finish_atom(s + k, xn[k], vo[k], vn[k], xn[0]);
+    } else if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
         v3 xo[3], xn[3], vo[3], vn[3];
         for (int k = 0; k < 3; ++k) step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
         const v3 xu[3] = {xn[0], xn[1], xn[2]};
@@ -2869,6 +3039,9 @@ void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t 
 }
 void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_press_ke, dim3(njobs), dim3(BLOCK), 0, s, p);
+}
+void launch_shake_solute(const SysPtrs& p, int32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_shake_solute, dim3(njobs), dim3(BLOCK), 0, s, p);
 }
 void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_baro, dim3(njobs), dim3(BLOCK), 0, s, p);

@@ -97,6 +97,9 @@ struct SysPtrs {
     const int32_t* pg_start;   // pressure groups (atom index starts; JobDev pg_off / n_pg)
     int32_t* box_fail;         // per job: first step whose box outgrew the cell grid
     double* cg_cvir;           // per group: SETTLE's virial diagonal (pressure-coupled jobs)
+    const int32_t* sh_ij;      // the solute's SHAKE constraints (atom pairs; JobDev sh_off / sh_n)
+    const double* sh_len;
+    uint8_t* sh_skip;          // two SHAKE skip flags per atom (the solute's sweeps)
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;
@@ -158,6 +161,7 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
 void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s);
+void launch_shake_solute(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
 void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,

@@ -62,7 +62,9 @@ class Topology(C.Structure):
                 ("pert_flag", P), ("pert_a_iac", P), ("pert_b_iac", P), ("pert_a_q", P), ("pert_b_q", P),
                 ("pert_alpha_lj", P), ("pert_alpha_crf", P), ("n_pert_atom_pairs", I32),
                 ("pert_atom_pairs", P), ("bonded", C.POINTER(Bonded)),
-                ("n_pressure_groups", I32), ("pressure_group_last", P)]
+                ("n_pressure_groups", I32), ("pressure_group_last", P),
+                ("n_solvent_constraints", I32), ("solvent_constraint_ij", P), ("solvent_constraint_len", P),
+                ("n_solute_constraints", I32), ("solute_constraint_ij", P), ("solute_constraint_len", P)]
 
 
 class SessionParams(C.Structure):
@@ -81,7 +83,8 @@ class Params(C.Structure):
                 ("use_settle", I32), ("do_virial", I32), ("lam", D), ("nlam", I32),
                 ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9),
                 ("nhc", I32), ("nticom", I32), ("nscm", I32),
-                ("barostat", I32), ("p_ref", D), ("compressibility", D), ("tau_p", D), ("virial", I32)]
+                ("barostat", I32), ("p_ref", D), ("compressibility", D), ("tau_p", D), ("virial", I32),
+                ("solvent_shake", I32), ("shake_tolerance", D)]
 
 
 def built():
@@ -253,6 +256,18 @@ def topology_struct(s, keep):
         ap = np.asarray(p.atom_pairs, np.int32).reshape(-1)
         t.n_pert_atom_pairs = len(ap) // 4
         t.pert_atom_pairs = keep(ap, np.int32)
+    tc = getattr(s, "solvent_constraints", None)
+    if getattr(s, "solvent_shake", False) and tc is not None and len(tc):
+        tc = np.asarray(tc, float).reshape(-1, 3)
+        t.n_solvent_constraints = len(tc)
+        t.solvent_constraint_ij = keep(tc[:, :2], np.int32)
+        t.solvent_constraint_len = keep(tc[:, 2], np.float64)
+    sc = getattr(s, "solute_constraints", None)
+    if sc is not None and len(sc):
+        sc = np.asarray(sc, float).reshape(-1, 3)
+        t.n_solute_constraints = len(sc)
+        t.solute_constraint_ij = keep(sc[:, :2], np.int32)
+        t.solute_constraint_len = keep(sc[:, 2], np.float64)
     pg = getattr(s, "pressure_groups", None)
     if pg is not None and len(pg):
         t.n_pressure_groups = len(pg)
@@ -307,6 +322,8 @@ def params_struct(s, use_settle=True, do_virial=False):
         p.barostat = 1
         p.p_ref, p.compressibility, p.tau_p = (float(x) for x in ba[:3])
         p.virial = int(ba[3])
+    p.solvent_shake = int(bool(getattr(s, "solvent_shake", False)))
+    p.shake_tolerance = float(getattr(s, "shake_tolerance", 1e-4))
     return p
 
 

@@ -310,6 +310,16 @@ typedef struct {
      * is a group of its own.  Read only with a molecular virial (params.virial 2) */
     int32_t n_pressure_groups;
     const int32_t *pressure_group_last;
+    /* SHAKE constraints (constraints.rs:213-570): the solvent molecule's template
+     * (pairs of atom indices within the molecule, lengths; read with
+     * params.solvent_shake), and the solute's (pairs of solute atoms, lengths;
+     * NTC 2 / 3 as the caller selected them, in the reference's order) */
+    int32_t n_solvent_constraints;
+    const int32_t *solvent_constraint_ij;
+    const double *solvent_constraint_len;
+    int32_t n_solute_constraints;
+    const int32_t *solute_constraint_ij;
+    const double *solute_constraint_len;
 } gromos_amd_topology;
 
 /* Run parameters (IMD PAIRLIST / NONBONDED / STEP / PERTURBATION blocks). */
@@ -352,6 +362,11 @@ typedef struct {
     int32_t barostat;
     double p_ref, compressibility, tau_p;
     int32_t virial;
+    /* solvent_shake 1: the solvent molecules are constrained by SHAKE with the
+     * topology's template instead of SETTLE; shake_tolerance: NTCP0 / NTCS0 (also
+     * for the solute's constraints) */
+    int32_t solvent_shake;
+    double shake_tolerance;
 } gromos_amd_params;
 
 #define GROMOS_AMD_NHC_MAX 16

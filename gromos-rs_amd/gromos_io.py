@@ -304,6 +304,8 @@ class GromosInput:
     epsrf: float = 61.0
     ntc: int = 1
     ntcs: str = "shake"                    # solvent constraints algorithm
+    ntcp: str = "shake"                    # solute constraints algorithm
+    shake_tol: float = 1e-4                # NTCP0 when NTCP is shake
     ntf: Tuple[int, int, int, int] = (1, 1, 1, 1)   # FORCE: bonds, angles, impropers, dihedrals on
     baths: List[Tuple[float, float]] = field(default_factory=list)   # (TEMP0, TAU)
     coupling: str = ""                     # weak-coupling, nose-hoover, nose-hoover-chains
@@ -361,8 +363,18 @@ def read_imd(path):
     if "CONSTRAINT" in blocks:
         d = _data(blocks["CONSTRAINT"])
         m.ntc = int(d[0].split()[0])
+        # imd.rs:586-639: NTC; NTCP (a name or 1 shake, 2 lincs, 3 settle); NTCP0, the
+        # one SHAKE tolerance unless NTCP is lincs; NTCS (as NTCP, 0 none); NTCS0
+        # (read only for lincs)
+        names = {"1": "shake", "2": "lincs", "3": "settle", "0": "none"}
+        if len(d) > 1:
+            w = d[1].split()[0].lower()
+            m.ntcp = names.get(w, w)
+        if len(d) > 2 and m.ntcp != "lincs":
+            m.shake_tol = float(d[2].split()[0])
         if len(d) > 3:
-            m.ntcs = d[3].split()[0].lower()
+            w = d[3].split()[0].lower()
+            m.ntcs = names.get(w, w)
     if "MULTIBATH" in blocks:
         d = _data(blocks["MULTIBATH"])
         k = 0
@@ -669,17 +681,43 @@ def write_configuration(path, system, pos, vel, title="final configuration", ntb
 # ---------------------------------------------------------------------------
 # an md run from GROMOS files on the device
 # ---------------------------------------------------------------------------
+def apply_constraints(s, top, inp):
+    """the CONSTRAINT block on a System (md.rs:337-352, 1208-1236; constraints.rs:36-60):
+    NTC 2 constrains the solute bonds with a hydrogen (an atom of mass 2 or less),
+    NTC 3 all of them, each of length B0 (bonds with B0 below 1e-10 are skipped), in
+    the topology's order, by SHAKE; the solvent by SHAKE (NTCS shake: every
+    molecule by the SOLVENTCONSTR template) or SETTLE; one tolerance, NTCP0.
+    LINCS is refused."""
+    if inp.ntc > 1 and top.bonds:
+        if inp.ntcp != "shake":
+            raise GromosFormatError(f"solute constraints by {inp.ntcp!r} are not provided (SHAKE only)")
+        cons = []
+        for i, j, t in top.bonds:
+            r0 = top.bond_types[t][2]
+            if r0 < 1e-10 or (inp.ntc == 2 and s.mass[i] > 2.0 and s.mass[j] > 2.0):
+                continue
+            cons.append((i, j, r0))
+        s.solute_constraints = np.array(cons, float).reshape(-1, 3)
+    if s.nsm > 0 and inp.ntcs == "shake":
+        s.solvent_shake = True
+        s.solvent_constraints = np.array(top.solvent_constr, float).reshape(-1, 3)
+    elif s.nsm > 0 and inp.ntcs != "settle":
+        raise GromosFormatError(f"solvent constraints by {inp.ntcs!r} are not provided (SHAKE or SETTLE)")
+    s.shake_tolerance = inp.shake_tol
+    return s
+
+
 def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, nsteps=None):
     """gromos-rs md for the systems this library covers (nonbonded forces, the
-    solute's covalent terms, SETTLE waters, an optional Berendsen or Nose-Hoover bath and
+    solute's covalent terms, SHAKE or SETTLE constraints, a Berendsen or Nose-Hoover
+    bath, centre-of-mass motion removal, isotropic Berendsen pressure scaling and
     perturbation of the nonbonded terms): reads the files, runs NSTLIM (or
     nsteps) md steps on the device, and writes out_prefix.tre (every step, or
     every NTWE), out_prefix.trf (the step-0 forces) and out_prefix.cnf.
     Returns the System and the per-step energies (e_lj, e_crf, e_pot, e_kin, dhdl)."""
     from . import runtime as rt
     s, top, inp = load(topology, configuration, imd, ptp)
-    if inp.ntc > 1 and s.bonded is not None and len(s.bonded.bonds):
-        raise GromosFormatError("solute bond constraints (NTC > 1) are not provided")
+    apply_constraints(s, top, inp)
     precision = rt.FP64 if precision is None else precision
     n = inp.nstlim if nsteps is None else int(nsteps)
     if inp.baths and inp.baths[0][1] > 0.0:

@@ -4,6 +4,7 @@ molecule) against the oracle and the gromosXX frames; the solute's constraints
 (one sweep per job after the integrator) against the oracle; SHAKE jobs alone
 and in a batch."""
 import importlib
+import os
 
 import numpy as np
 import pytest
@@ -102,3 +103,47 @@ def test_shake_jobs_in_a_batch_match_alone(prec):
         x, v = sess.fetch(j)
         assert np.array_equal(x, x0) and np.array_equal(v, v0)
     sess.close()
+
+
+@pytest.mark.parametrize("name", ["water_10_box", "nacl_water_box"])
+def test_run_md_from_files_with_solvent_shake(name, tmp_path):
+    """the gromosXX input files (NTCS shake) through the readers, the device md
+    and the .tre writer: every written frame matches the gromosXX energies"""
+    gio = importlib.import_module("gromos-rs_amd.gromos_io")
+    d = os.path.join(os.path.dirname(__file__), "golden", "gromos", name)
+    fs = {f.rsplit(".", 1)[1]: os.path.join(d, f) for f in os.listdir(d)}
+    out = str(tmp_path / "run")
+    s, e = gio.run_md(fs["topo"], fs["conf"], fs["in"], out)
+    assert s.solvent_shake
+    _, fx = ts.with_shake(name)
+    tre = gio.read_energy_trajectory(out + ".tre")
+    col = {f: k for k, f in enumerate(gio.ENERTRJ_FIELDS)}
+    for k, ref in enumerate(fx["expected_energies"]):
+        assert rel(tre[k, col["kinetic"]], ref["e_kin"]) < 6e-10, k
+        assert rel(tre[k, col["potential"]], ref["e_pot"]) < 6e-10, k
+
+
+def test_run_md_from_files_with_solute_shake(tmp_path):
+    """aladip in SHAKE water with NTC 3 from the files: run_md's constraint
+    selection and device run equal the oracle's md over the fixture's bonds"""
+    gio = importlib.import_module("gromos-rs_amd.gromos_io")
+    d = os.path.join(os.path.dirname(__file__), "golden", "gromos", "aladip_solvated")
+    fs = {f.rsplit(".", 1)[1]: os.path.join(d, f) for f in os.listdir(d)}
+    text = open(fs["in"]).read().split("\n")
+    k = next(i for i, l in enumerate(text) if l.startswith("CONSTRAINT"))
+    k = next(i for i in range(k + 1, len(text)) if not text[i].lstrip().startswith("#"))
+    assert text[k].split()[0] == "1"
+    text[k] = "3"
+    imd = str(tmp_path / "ntc3.in")
+    open(imd, "w").write("\n".join(text))
+    n = 20
+    s, e = gio.run_md(fs["topo"], fs["conf"], imd, str(tmp_path / "run"), nsteps=n)
+    assert len(s.solute_constraints) == 11
+    r, dd = ts.with_shake("aladip_solvated")
+    r.solute_constraints = solute_constraints(r, dd, 3)
+    assert (s.nticom, s.nscm) == (1, 10)           # the input's centre-of-mass removal
+    r.nticom, r.nscm = s.nticom, s.nscm
+    md = oracle.MD(oracle.OracleSystem(r), use_settle=False)
+    e_ref = md.run(n)
+    for c in range(4):
+        assert rel(e[:, c], e_ref[:, c]).max() < 1e-10, (c, rel(e[:, c], e_ref[:, c]).max())

@@ -101,6 +101,33 @@ def test_imd_fields():
     assert m.coupling == "nose-hoover" and m.baths == [(300.0, 0.1)] and m.nstlim == 20
 
 
+def test_constraint_block_and_constraint_selection():
+    """CONSTRAINT: NTC, NTCP, NTCP0 (the SHAKE tolerance), NTCS by name; NTC 2
+    constrains aladip's two N-H bonds (H mass 1.008, B0 0.1 nm), NTC 3 all eleven
+    bonds, NTC 1 none; the solvent by SHAKE or SETTLE as NTCS says"""
+    _, _, imd, _ = files("ch4_water_fep")
+    m = gio.read_imd(imd)
+    assert (m.ntc, m.ntcp, m.shake_tol, m.ntcs) == (3, "shake", 1e-4, "shake")
+    _, _, imd, _ = files("nacl_1water_settle")
+    assert gio.read_imd(imd).ntcs == "settle"
+    top, conf, imd, _ = files("aladip_solvated")
+    got = {}
+    for ntc in (1, 2, 3):
+        s, t, inp = gio.load(top, conf, imd)
+        inp.ntc = ntc
+        gio.apply_constraints(s, t, inp)
+        assert s.solvent_shake and s.solvent_constraints.shape == (3, 3)
+        got[ntc] = s.solute_constraints
+    assert got[1] is None and len(got[3]) == len(t.bonds) == 11
+    assert len(got[2]) == 2 and np.all(got[2][:, 2] == 0.1)
+    assert all(min(s.mass[int(i)], s.mass[int(j)]) == 1.008 for i, j, _ in got[2])
+    # every NTC 2 constraint is also an NTC 3 one
+    assert {tuple(r) for r in got[2]} <= {tuple(r) for r in got[3]}
+    inp.ntcs = "lincs"
+    with pytest.raises(gio.GromosFormatError):
+        gio.apply_constraints(s, t, inp)
+
+
 def test_energy_trajectory_round_trip(tmp_path):
     p = str(tmp_path / "e.tre")
     w = gio.EnergyTrajectoryWriter(p)

@@ -141,6 +141,12 @@ class System:
     solvent_constraints: Optional[np.ndarray] = None
     solute_constraints: Optional[np.ndarray] = None
     shake_tolerance: float = 1e-4
+    # LINCS (constraints.rs:1168-1445): an expansion order above 0 constrains
+    # the solute_constraints (solute_lincs) or every solvent molecule by the
+    # template solvent_constraints (solvent_lincs, instead of SHAKE or SETTLE)
+    # by LINCS
+    solute_lincs: int = 0
+    solvent_lincs: int = 0
     # last atom of each solute pressure group (topology PRESSUREGROUPS); every
     # solvent molecule is a group of its own
     pressure_groups: Optional[np.ndarray] = None

@@ -1512,6 +1512,13 @@ struct or_md {
     int32_t *sh_i, *sh_j;
     double *sh_len;
     double sh_tol;
+    int lc_order_u, lc_order_v; /* LINCS expansion orders: solute (NTCP0), solvent (NTCS0) */
+    int lc_nu;                  /* the solute's LINCS constraints */
+    int32_t *lc_ij;
+    double *lc_len;
+    int lc_nt;                  /* one solvent molecule's LINCS template */
+    int32_t lc_tij[2 * OR_LINCS_TPL_MAX];
+    double lc_tlen[OR_LINCS_TPL_MAX];
     int baro;                   /* Berendsen barostat, isotropic */
     double p_ref, comp, tau_p;
     int vtype;                  /* virial: 0 none, 1 atomic, 2 molecular */
@@ -1561,6 +1568,8 @@ void or_md_destroy(or_md *md) {
     free(md->sh_i);
     free(md->sh_j);
     free(md->sh_len);
+    free(md->lc_ij);
+    free(md->lc_len);
     or_ff_destroy(md->ff);
     for (int k = 0; k < 2; ++k) { free(md->pos[k]); free(md->vel[k]); free(md->force[k]); }
     free(md->inv_mass);
@@ -1793,6 +1802,138 @@ int or_shake(int n_atoms, const double *inv_mass, int nc, const int32_t *ci, con
     return converged ? it : -1;
 }
 
+/* setup_lincs (constraints.rs:1168-1220) and lincs (constraints.rs:1241-1390),
+ * LINCS of Hess et al. (1997) as gromosXX's lincs.cc writes it: constraint k
+ * joins atoms ci[k] + off and cj[k] + off.
+ *   setup: sdiag[k] = 1 / sqrt(1/m_i + 1/m_j); constraints a < b sharing an atom
+ *   are coupled with coef = (1/m_common) sdiag[a] sdiag[b], negated when the
+ *   shared atom is the first of both or the second of both; each constraint's
+ *   coupled list ascends.
+ *   B[k] = (x_old[j] - x_old[i]) / |...| (zero below 1e-10 nm); A[k][n] =
+ *   coef[k][n] (B[k] . B[n]); rhs = sol = sdiag (B . (x[j] - x[i]) - l); order
+ *   times: rhs' = A rhs, sol += rhs'; x[i] += B sdiag sol / m_i, x[j] -= B sdiag
+ *   sol / m_j.  Then the rotational correction: p = sqrt(2 l^2 - |x[j] - x[i]|^2)
+ *   (0 when not positive), rhs = sol = sdiag (l - p), the same expansion and
+ *   update; v = (x - x_old) / dt for each constrained atom.  No virial.
+ * Returns the largest relative length error after the solve. */
+double or_lincs(int nc, const int32_t *ci, const int32_t *cj, const double *len, int off, int order,
+                const double *inv_mass, double dt, const double *pos_old, double *pos_new, double *vel_new) {
+    if (nc <= 0) return 0.0;
+    double *sdiag = (double *)malloc(sizeof(double) * (size_t)nc);
+    double *coef = (double *)calloc((size_t)nc * (size_t)nc, sizeof(double));   /* dense, symmetric */
+    unsigned char *cpl = (unsigned char *)calloc((size_t)nc * (size_t)nc, 1);
+    double *b = (double *)malloc(sizeof(double) * 3 * (size_t)nc);
+    double *rhs0 = (double *)malloc(sizeof(double) * (size_t)nc);
+    double *rhs1 = (double *)malloc(sizeof(double) * (size_t)nc);
+    double *sol = (double *)malloc(sizeof(double) * (size_t)nc);
+    double *rhs[2] = {rhs0, rhs1};
+    for (int k = 0; k < nc; ++k) sdiag[k] = 1.0 / sqrt(inv_mass[ci[k] + off] + inv_mass[cj[k] + off]);
+    for (int a = 0; a < nc; ++a)
+        for (int c = a + 1; c < nc; ++c) {
+            int common = -1;
+            if (ci[a] == ci[c]) common = ci[a];
+            else if (cj[a] == ci[c]) common = cj[a];
+            else if (ci[a] == cj[c]) common = ci[a];
+            else if (cj[a] == cj[c]) common = cj[a];
+            if (common < 0) continue;
+            double v = inv_mass[common + off] * sdiag[a] * sdiag[c];
+            if (ci[a] == ci[c] || cj[a] == cj[c]) v *= -1.0;
+            coef[(size_t)a * nc + c] = coef[(size_t)c * nc + a] = v;
+            cpl[(size_t)a * nc + c] = cpl[(size_t)c * nc + a] = 1;
+        }
+    for (int k = 0; k < nc; ++k) {
+        const int i = ci[k] + off, j = cj[k] + off;
+        double r[3];
+        for (int a = 0; a < 3; ++a) r[a] = pos_old[3 * j + a] - pos_old[3 * i + a];
+        const double rl = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (int a = 0; a < 3; ++a) b[3 * k + a] = rl < 1e-10 ? 0.0 : r[a] / rl;
+    }
+#define OR_BDOT(p, q) (b[3 * (p)] * b[3 * (q)] + b[3 * (p) + 1] * b[3 * (q) + 1] + b[3 * (p) + 2] * b[3 * (q) + 2])
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < nc; ++k) {
+            const int i = ci[k] + off, j = cj[k] + off;
+            double r[3];
+            for (int a = 0; a < 3; ++a) r[a] = pos_new[3 * j + a] - pos_new[3 * i + a];
+            if (pass == 0) {
+                const double proj = b[3 * k] * r[0] + b[3 * k + 1] * r[1] + b[3 * k + 2] * r[2];
+                rhs[0][k] = sdiag[k] * (proj - len[k]);
+            } else {
+                const double diff = 2.0 * len[k] * len[k] - (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const double p = diff > 0.0 ? sqrt(diff) : 0.0;
+                rhs[0][k] = sdiag[k] * (len[k] - p);
+            }
+            sol[k] = rhs[0][k];
+        }
+        int w = 1;
+        for (int rec = 0; rec < order; ++rec) {
+            for (int k = 0; k < nc; ++k) {
+                rhs[w][k] = 0.0;
+                for (int c = 0; c < nc; ++c)
+                    if (cpl[(size_t)k * nc + c]) rhs[w][k] += coef[(size_t)k * nc + c] * OR_BDOT(k, c) * rhs[1 - w][c];
+                sol[k] += rhs[w][k];
+            }
+            w = 1 - w;
+        }
+        for (int k = 0; k < nc; ++k) {
+            const int i = ci[k] + off, j = cj[k] + off;
+            const double corr = sdiag[k] * sol[k];
+            const double si = corr * inv_mass[i], sj = corr * inv_mass[j];
+            for (int a = 0; a < 3; ++a) {
+                pos_new[3 * i + a] += b[3 * k + a] * si;
+                pos_new[3 * j + a] -= b[3 * k + a] * sj;
+            }
+        }
+    }
+#undef OR_BDOT
+    double err = 0.0;
+    for (int k = 0; k < nc; ++k) {
+        const int i = ci[k] + off, j = cj[k] + off;
+        for (int a = 0; a < 3; ++a) {
+            vel_new[3 * i + a] = (pos_new[3 * i + a] - pos_old[3 * i + a]) * (1.0 / dt);
+            vel_new[3 * j + a] = (pos_new[3 * j + a] - pos_old[3 * j + a]) * (1.0 / dt);
+        }
+        double r[3];
+        for (int a = 0; a < 3; ++a) r[a] = pos_new[3 * j + a] - pos_new[3 * i + a];
+        const double e = fabs((sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) - len[k]) / len[k]);
+        if (e > err) err = e;
+    }
+    free(sdiag);
+    free(coef);
+    free(cpl);
+    free(b);
+    free(rhs0);
+    free(rhs1);
+    free(sol);
+    return err;
+}
+
+/* LINCS in the md step (md.rs:1244-1257, lincs_buffered constraints.rs:1397-1445),
+ * after SHAKE and SETTLE: the solute's constraints (order_u > 0), then every
+ * solvent molecule by the template (order_v > 0) */
+int or_md_set_lincs(or_md *md, int order_u, int n_solute_c, const int32_t *sol_ij, const double *sol_len,
+                    int order_v, int n_template, const int32_t *tmpl_ij, const double *tmpl_len) {
+    if (n_template > OR_LINCS_TPL_MAX) return -1;
+    free(md->lc_ij);
+    free(md->lc_len);
+    md->lc_ij = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(n_solute_c > 0 ? n_solute_c : 1));
+    md->lc_len = (double *)malloc(sizeof(double) * (size_t)(n_solute_c > 0 ? n_solute_c : 1));
+    for (int c = 0; c < n_solute_c; ++c) {
+        md->lc_ij[2 * c] = sol_ij[2 * c];
+        md->lc_ij[2 * c + 1] = sol_ij[2 * c + 1];
+        md->lc_len[c] = sol_len[c];
+    }
+    md->lc_nu = order_u > 0 ? n_solute_c : 0;
+    md->lc_nt = order_v > 0 ? n_template : 0;
+    for (int c = 0; c < md->lc_nt; ++c) {
+        md->lc_tij[2 * c] = tmpl_ij[2 * c];
+        md->lc_tij[2 * c + 1] = tmpl_ij[2 * c + 1];
+        md->lc_tlen[c] = tmpl_len[c];
+    }
+    md->lc_order_u = order_u;
+    md->lc_order_v = order_v;
+    return 0;
+}
+
 /* SHAKE in the md step (md.rs:1208-1236): the solute constraints (NTC 2: bonds
  * with a hydrogen, 3: all bonds; chosen by the caller) and, for NTCS shake,
  * every solvent molecule's template constraints; tolerance NTCP0 / NTCS0 */
@@ -1996,6 +2137,30 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
             int err = or_settle(md->t->n_solute_atoms, n, md->m_o, md->m_h, md->d_oh, md->d_hh, md->dt,
                                 md->pos[c], md->pos[o], md->vel[o], NULL, md->baro ? vir : NULL);
             if (err) return -1;
+        }
+        if (md->lc_nu > 0 || md->lc_nt > 0) {
+            /* the solute's constraints, then each solvent molecule's (split
+             * index pairs for or_lincs) */
+            int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)(md->lc_nu + md->lc_nt + 1));
+            int32_t *cj = (int32_t *)malloc(sizeof(int32_t) * (size_t)(md->lc_nu + md->lc_nt + 1));
+            for (int k = 0; k < md->lc_nu; ++k) {
+                ci[k] = md->lc_ij[2 * k];
+                cj[k] = md->lc_ij[2 * k + 1];
+            }
+            or_lincs(md->lc_nu, ci, cj, md->lc_len, 0, md->lc_order_u, md->inv_mass, md->dt, md->pos[c], md->pos[o],
+                     md->vel[o]);
+            const int aps = md->t->atoms_per_solvent;
+            const int nsm = aps > 0 ? (n - md->t->n_solute_atoms) / aps : 0;
+            for (int k = 0; k < md->lc_nt; ++k) {
+                ci[k] = md->lc_tij[2 * k];
+                cj[k] = md->lc_tij[2 * k + 1];
+            }
+            if (md->lc_nt > 0)
+                for (int m = 0; m < nsm; ++m)
+                    or_lincs(md->lc_nt, ci, cj, md->lc_tlen, md->t->n_solute_atoms + m * aps, md->lc_order_v,
+                             md->inv_mass, md->dt, md->pos[c], md->pos[o], md->vel[o]);
+            free(ci);
+            free(cj);
         }
         double ek = or_ekin(n, md->t->mass, md->vel[o], md->vel[c]);
         if (md->baro) {

@@ -297,6 +297,16 @@ int or_md_set_shake(or_md *md, int n_solute_c, const int32_t *sol_ij, const doub
 int or_shake(int n_atoms, const double *inv_mass, int nc, const int32_t *ci, const int32_t *cj, const double *len,
              double tol, int max_iter, double dt, const double *pos_old, double *pos_new, double *vel_new,
              double *vir);
+/* LINCS (constraints.rs:1168-1390) of constraints (ci[k] + off, cj[k] + off);
+ * returns the largest relative length error after it */
+#define OR_LINCS_TPL_MAX 6
+double or_lincs(int nc, const int32_t *ci, const int32_t *cj, const double *len, int off, int order,
+                const double *inv_mass, double dt, const double *pos_old, double *pos_new, double *vel_new);
+/* LINCS in the md step after SHAKE / SETTLE: the solute's constraints with
+ * expansion order order_u (0: none), then each solvent molecule's template
+ * (at most OR_LINCS_TPL_MAX constraints) with order_v (0: none) */
+int or_md_set_lincs(or_md *md, int order_u, int n_solute_c, const int32_t *sol_ij, const double *sol_len,
+                    int order_v, int n_template, const int32_t *tmpl_ij, const double *tmpl_len);
 void or_md_box(const or_md *md, double *L);
 /* centre-of-mass motion removal (remove_com_motion.rs) at the start of each md step */
 void or_md_set_com(or_md *md, int nticom, int nscm);

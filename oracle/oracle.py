@@ -168,6 +168,10 @@ def _declare(L):
     L.or_md_set_shake.argtypes = [P, C.c_int, P, P, C.c_int, P, P, D]
     L.or_shake.argtypes = [C.c_int, P, C.c_int, P, P, P, D, C.c_int, D, P, P, P, P]
     L.or_shake.restype = C.c_int
+    L.or_md_set_lincs.argtypes = [P, C.c_int, C.c_int, P, P, C.c_int, C.c_int, P, P]
+    L.or_md_set_lincs.restype = C.c_int
+    L.or_lincs.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, P, D, P, P, P]
+    L.or_lincs.restype = D
     L.or_md_pressure.argtypes = [P]
     L.or_md_pressure.restype = D
     L.or_md_box.argtypes = [P, P]
@@ -535,6 +539,19 @@ class MD:
             lib().or_md_set_com(self.h, int(s.nticom), int(s.nscm))
         sol_c = getattr(s, "solute_constraints", None)
         tmpl = getattr(s, "solvent_constraints", None) if getattr(s, "solvent_shake", False) else None
+        lu, lv = int(getattr(s, "solute_lincs", 0)), int(getattr(s, "solvent_lincs", 0))
+        if lu > 0 or lv > 0:
+            # LINCS takes the solute's constraints (order lu) and / or the solvent
+            # template (order lv) in place of SHAKE
+            lsc = np.zeros((0, 3)) if (lu == 0 or sol_c is None) else np.asarray(sol_c, float).reshape(-1, 3)
+            ltc = np.zeros((0, 3)) if lv == 0 else np.asarray(s.solvent_constraints, float).reshape(-1, 3)
+            self.lc = (np.ascontiguousarray(lsc[:, :2], np.int32), np.ascontiguousarray(lsc[:, 2]),
+                       np.ascontiguousarray(ltc[:, :2], np.int32), np.ascontiguousarray(ltc[:, 2]))
+            if lib().or_md_set_lincs(self.h, lu, len(lsc), ptr(self.lc[0]), ptr(self.lc[1]), lv, len(ltc),
+                                     ptr(self.lc[2]), ptr(self.lc[3])) != 0:
+                raise ValueError("a LINCS solvent template holds at most 6 constraints")
+            if lu > 0:
+                sol_c = None
         if (sol_c is not None and len(sol_c)) or tmpl is not None:
             sc = np.zeros((0, 3)) if sol_c is None else np.asarray(sol_c, float).reshape(-1, 3)
             tc = np.zeros((0, 3)) if tmpl is None else np.asarray(tmpl, float).reshape(-1, 3)

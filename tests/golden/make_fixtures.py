@@ -45,6 +45,8 @@ SYSTEMS = [
     ("nacl_water_box", None, None, None, None, "all"),
     ("nacl_water_box_shifted", None, None, None, None, "all"),
     ("nacl_1water_settle", None, None, None, None, "all"),
+    ("nacl_1water_lincs", None, None, None, None, "all"),
+    ("aladip_vacuum_lincs", "../shared/aladip.topo", None, None, None, "all"),
     ("aladip_solvated", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
     ("water_216_box", None, None, None, None, "first"),
     ("aladip_trunc_oct", "../shared/aladip.topo", "../shared/aladip.conf", None, None, "first"),
@@ -259,6 +261,11 @@ def parse_imd(path):
     d["ntf"] = [int(x) for x in frc[0].split()[:4]] if frc else [1, 1, 1, 1]
     cons = data_lines(b.get("CONSTRAINT", []))
     d["ntcs"] = cons[3].split()[0] if len(cons) > 3 else "shake"
+    # NTC, NTCP, NTCP0 (tolerance, or LINCS order), NTCS, NTCS0 as written
+    d["constraint"] = dict(ntc=int(cons[0].split()[0]) if cons else 1,
+                           ntcp=cons[1].split()[0] if len(cons) > 1 else "shake",
+                           ntcp0=float(cons[2].split()[0]) if len(cons) > 2 else 1e-4,
+                           ntcs0=float(cons[4].split()[0]) if len(cons) > 4 else 1e-4)
     if "PERTURBATION" in b:
         pr = data_lines(b["PERTURBATION"])
         a = pr[0].split()
@@ -415,7 +422,7 @@ def build(ref, name, topo_rel, conf_rel, imd_rel, ptp_rel, keep):
         box_matrix=box_matrix,
         fpepsi=t["fpepsi"], rcutp=imd["rcutp"], rcutl=imd["rcutl"], nsnb=imd["nsnb"],
         algorithm=imd["algorithm"], epsrf=imd["epsrf"], appak=imd["appak"],
-        dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"], ndfmin=imd["ndfmin"], thermostat=imd["thermostat"],
+        dt=imd["dt"], nstlim=imd["nstlim"], ntcs=imd["ntcs"], constraint=imd["constraint"], ndfmin=imd["ndfmin"], thermostat=imd["thermostat"],
         nticom=imd["nticom"], nscm=imd["nscm"], pressure=imd["pressure"],
         pressure_groups=t["pressure_groups"],
         solvent_constr=t["solvent_constr"], pert=pert,
@@ -479,6 +486,8 @@ INPUT_SETS = {
     "ch4_water_fep": ["../shared/ch4_spc.top", "../shared/ch4_spc.cnf", "ch4_water_fep.in",
                       "../shared/ch4_spc_dummy.ptp"],
     "aladip_vacuum": ["../shared/aladip.topo", "aladip_vacuum.conf", "aladip_vacuum.in"],
+    "nacl_1water_lincs": ["nacl_1water_lincs.topo", "nacl_1water_lincs.conf", "nacl_1water_lincs.in"],
+    "aladip_vacuum_lincs": ["../shared/aladip.topo", "aladip_vacuum_lincs.conf", "aladip_vacuum_lincs.in"],
     "butane_vacuum": ["butane_vacuum.topo", "butane_vacuum.conf", "butane_vacuum.in"],
     "benzene_vacuum": ["benzene_vacuum.topo", "benzene_vacuum.conf", "benzene_vacuum.in"],
     "water_216_nvt_nosehoover": ["../water_216_box/water_216_box.topo", "../water_216_box/water_216_box.conf",

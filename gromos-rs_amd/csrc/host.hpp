@@ -238,6 +238,14 @@ struct gromos_amd_system {
     int32_t* d_sh_ij = nullptr;
     double* d_sh_len = nullptr;
     uint8_t* d_sh_skip = nullptr;
+    bool any_solvent_pass = false;         // a job constrains its solvent by SHAKE or LINCS (after the integrator)
+    std::vector<double> host_lc_sdiag, host_lc_coef;   // LINCS of the solute constraints
+    std::vector<int32_t> host_lc_cpl, host_lc_cidx;
+    double* d_lc_sdiag = nullptr;
+    int32_t* d_lc_cpl = nullptr;
+    int32_t* d_lc_cidx = nullptr;
+    double* d_lc_coef = nullptr;
+    double* d_lc_work = nullptr;
     std::vector<int64_t> job_step0;        // per job: md steps done before the current call
     int64_t* d_job_step0 = nullptr;
     bool therm_dirty = true;   // velocities changed outside the step: rescan before the next one

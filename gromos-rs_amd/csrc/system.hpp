@@ -68,6 +68,11 @@ struct JobDev {
     double sc_len[6];
     double sh_tol;                         // SHAKE tolerance
     int32_t sh_off, sh_n;                  // the solute's constraints in the batch arrays
+    int32_t lincs_u;                       // LINCS order of the solute's constraints (0: SHAKE)
+    int32_t lincs_sv;                      // LINCS order of the solvent template (0: none)
+    int32_t lc_cpl[6];                     // template: bit c of lc_cpl[k] - constraints k, c share an atom
+    double lc_sdiag[6];                    // template: 1 / sqrt(1/m_i + 1/m_j)
+    double lc_coef[36];                    // template: coupling coefficients [k * 6 + c]
     int32_t bd_off;                        // first covalent term (batch order)
     int32_t bd_n[4];                       // terms per kind: bonds, angles, dihedrals, impropers
 };

@@ -2,12 +2,14 @@
 // pair-list rebuild cadence, Forcefield::apply and the md step sequence.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <map>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 
 #include "host.hpp"
 
@@ -95,6 +97,40 @@ constexpr int64_t BYTES_PER_ATOM = 2 * 2 * 24 + 7 * 24 + 32 + 16 + 8 + 4 + 8 + 8
                                    NSLOT * ROW * 8 * 4 / BLOCK + 16;
 constexpr int64_t BYTES_PER_CG = 2 * 24 + 16 + 16 + 16 + 4 * 8 + 2 * 8 + 2 * 4 + 8;
 
+// setup_lincs (constraints.rs:1168-1220): sd[k] = 1 / sqrt(1/m_i + 1/m_j); two
+// constraints a < b sharing an atom (checked first-first, second-first,
+// first-second, second-second) are coupled with (1/m_common) sd[a] sd[b],
+// negated when the atom is the first of both or the second of both; each
+// constraint's coupled ones ascend.  ij: atom pairs; inv_m(atom).
+template <class F>
+void lincs_setup(int32_t nc, const int32_t* ij, F inv_m, std::vector<double>& sd,
+                 std::vector<std::vector<std::pair<int32_t, double>>>& cpl) {
+    sd.assign((size_t)nc, 0.0);
+    cpl.assign((size_t)nc, {});
+    std::unordered_map<int32_t, std::vector<int32_t>> by_atom;
+    for (int32_t k = 0; k < nc; ++k) {
+        sd[k] = 1.0 / std::sqrt(inv_m(ij[2 * k]) + inv_m(ij[2 * k + 1]));
+        by_atom[ij[2 * k]].push_back(k);
+        if (ij[2 * k + 1] != ij[2 * k]) by_atom[ij[2 * k + 1]].push_back(k);
+    }
+    for (int32_t k = 0; k < nc; ++k) {
+        std::vector<int32_t> other;
+        for (int s = 0; s < 2; ++s)
+            for (int32_t c : by_atom[ij[2 * k + s]])
+                if (c != k) other.push_back(c);
+        std::sort(other.begin(), other.end());
+        other.erase(std::unique(other.begin(), other.end()), other.end());
+        for (int32_t c : other) {
+            const int32_t a = std::min(k, c), b = std::max(k, c);
+            const int32_t ai = ij[2 * a], aj = ij[2 * a + 1], bi = ij[2 * b], bj = ij[2 * b + 1];
+            const int32_t common = ai == bi ? ai : (aj == bi ? aj : (ai == bj ? ai : aj));
+            double v = inv_m(common) * sd[a] * sd[b];
+            if (ai == bi || aj == bj) v *= -1.0;
+            cpl[k].push_back({c, v});
+        }
+    }
+}
+
 int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, JobImage& im) {
     if (int rc = check_box(P.boundary, params_box(P))) return rc;
     int32_t n = T.n_atoms;
@@ -119,7 +155,11 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     if (!(P.dt > 0.0)) return fail(GROMOS_AMD_EINVAL, "job %d: dt must be > 0", j);
     im = JobImage{};
     im.P = P;
-    if (P.solvent_shake) im.P.use_settle = 0;   // SHAKE constrains the solvent instead
+    if (P.solvent_shake && P.solvent_lincs > 0)
+        return fail(GROMOS_AMD_EINVAL, "job %d: the solvent is constrained by SHAKE or by LINCS, not both", j);
+    if (P.solute_lincs < 0 || P.solvent_lincs < 0 || P.solute_lincs > 64 || P.solvent_lincs > 64)
+        return fail(GROMOS_AMD_EINVAL, "job %d: LINCS orders must be 0..64", j);
+    if (P.solvent_shake || P.solvent_lincs > 0) im.P.use_settle = 0;   // SHAKE or LINCS instead
     im.n = n;
     im.n_cg = T.n_cg;
     im.n_types = T.n_types;
@@ -234,10 +274,12 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.sh_tol = P.shake_tolerance > 0.0 ? P.shake_tolerance : 1e-4;
     J.shake_sv = 0;
     J.n_sc = 0;
-    if (P.solvent_shake && nsolv_atoms > 0) {
+    J.lincs_u = P.solute_lincs;
+    J.lincs_sv = 0;
+    if ((P.solvent_shake || P.solvent_lincs > 0) && nsolv_atoms > 0) {
         const int32_t nc = T.n_solvent_constraints;
         if (nc < 1 || nc > 6 || !T.solvent_constraint_ij || !T.solvent_constraint_len)
-            return fail(GROMOS_AMD_EINVAL, "job %d: solvent SHAKE needs 1..6 template constraints", j);
+            return fail(GROMOS_AMD_EINVAL, "job %d: solvent SHAKE / LINCS needs 1..6 template constraints", j);
         for (int32_t c = 0; c < nc; ++c) {
             const int32_t a = T.solvent_constraint_ij[2 * c], b = T.solvent_constraint_ij[2 * c + 1];
             if (a < 0 || b < 0 || a >= aps || b >= aps || a == b || !(T.solvent_constraint_len[c] > 0.0))
@@ -247,7 +289,30 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
             J.sc_len[c] = T.solvent_constraint_len[c];
         }
         J.n_sc = nc;
-        J.shake_sv = 1;
+        if (P.solvent_lincs > 0) {
+            // the template's LINCS matrix from the first molecule's masses (constraints.rs:1130-1137)
+            double inv_m[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int32_t a = 0; a < aps; ++a) {
+                const double m = T.mass[T.n_solute_atoms + a];
+                inv_m[a] = m > 0.0 ? 1.0 / m : 0.0;
+            }
+            std::vector<double> sd;
+            std::vector<std::vector<std::pair<int32_t, double>>> cpl;
+            lincs_setup(nc, J.sc_ij, [&](int32_t a) { return inv_m[a]; }, sd, cpl);
+            for (int32_t k = 0; k < 6; ++k) {
+                J.lc_sdiag[k] = k < nc ? sd[k] : 0.0;
+                J.lc_cpl[k] = 0;
+                for (int32_t c = 0; c < 6; ++c) J.lc_coef[6 * k + c] = 0.0;
+            }
+            for (int32_t k = 0; k < nc; ++k)
+                for (const auto& e : cpl[k]) {
+                    J.lc_cpl[k] |= 1 << e.first;
+                    J.lc_coef[6 * k + e.first] = e.second;
+                }
+            J.lincs_sv = P.solvent_lincs;
+        } else {
+            J.shake_sv = 1;
+        }
     }
     im.shake.clear();
     for (int32_t c = 0; c < T.n_solute_constraints; ++c) {
@@ -392,10 +457,14 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     std::vector<Slice> sl(njobs + 1);
     int64_t cell_off = 0;
     bool any_pert = false, twin = false, any_thermo = false, any_com = false, any_baro = false;
-    bool any_solute_shake = false;
+    bool any_solute_shake = false, any_solvent_pass = false, any_lincs_u = false;
     S->host_pg.clear();
     S->host_sh_ij.clear();
     S->host_sh_len.clear();
+    S->host_lc_sdiag.clear();
+    S->host_lc_cpl.clear();
+    S->host_lc_cidx.clear();
+    S->host_lc_coef.clear();
     int32_t use_settle = 0, do_virial = 0;
     for (int j = 0; j < njobs; ++j) {
         const JobImage& im = *imgs[j];
@@ -435,7 +504,37 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
             S->host_sh_ij.push_back((int32_t)o.atom + c.j);
             S->host_sh_len.push_back(c.len);
         }
+        if (J.lincs_u > 0 && J.sh_n > 0) {
+            // LINCS matrix of the job's solute constraints, rows beside sh_ij / sh_len
+            std::vector<int32_t> ij;
+            for (const auto& c : im.shake) {
+                ij.push_back(c.i);
+                ij.push_back(c.j);
+            }
+            std::vector<double> sd;
+            std::vector<std::vector<std::pair<int32_t, double>>> cpl;
+            lincs_setup(J.sh_n, ij.data(),
+                        [&](int32_t a) { return im.mass[a] > 0.0 ? 1.0 / im.mass[a] : 0.0; }, sd, cpl);
+            for (int32_t k = 0; k < J.sh_n; ++k) {
+                S->host_lc_sdiag.push_back(sd[k]);
+                S->host_lc_cpl.push_back((int32_t)S->host_lc_cidx.size());
+                S->host_lc_cpl.push_back((int32_t)cpl[k].size());
+                for (const auto& e : cpl[k]) {
+                    S->host_lc_cidx.push_back(e.first);
+                    S->host_lc_coef.push_back(e.second);
+                }
+            }
+            any_lincs_u = true;
+        } else {
+            for (int32_t k = 0; k < J.sh_n; ++k) {
+                S->host_lc_sdiag.push_back(0.0);
+                S->host_lc_cpl.push_back(0);
+                S->host_lc_cpl.push_back(0);
+            }
+            J.lincs_u = 0;
+        }
         any_solute_shake = any_solute_shake || J.sh_n > 0;
+        any_solvent_pass = any_solvent_pass || J.lincs_sv > 0 || J.shake_sv;
         use_settle |= im.P.use_settle && !im.mol_first.empty();
         do_virial |= im.P.do_virial;
         S->jobs.push_back(J);
@@ -579,6 +678,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->any_com = any_com;
     S->any_baro = any_baro;
     S->any_solute_shake = any_solute_shake;
+    S->any_solvent_pass = any_solvent_pass;
     S->job_step0.assign((size_t)njobs, 0);
     S->host_pressure.assign((size_t)njobs, 0.0);
     S->near_r = VV_NEAR_R;
@@ -695,6 +795,10 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     rc = rc ? rc : S->upload(S->d_pg_start, S->host_pg);
     rc = rc ? rc : S->upload(S->d_sh_ij, S->host_sh_ij);
     rc = rc ? rc : S->upload(S->d_sh_len, S->host_sh_len);
+    rc = rc ? rc : S->upload(S->d_lc_sdiag, S->host_lc_sdiag);
+    rc = rc ? rc : S->upload(S->d_lc_cpl, S->host_lc_cpl);
+    rc = rc ? rc : S->upload(S->d_lc_cidx, S->host_lc_cidx);
+    rc = rc ? rc : S->upload(S->d_lc_coef, S->host_lc_coef);
     rc = rc ? rc : S->upload(S->d_cg_start, cg_start);
     rc = rc ? rc : S->upload(S->d_cg_n, cg_n);
     rc = rc ? rc : S->upload(S->d_cg_job, cg_job);
@@ -783,6 +887,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_box_fail, (size_t)njobs, true)) ||
         (rc = S->reserve(S->d_cg_cvir, any_baro ? 3 * nc : 1, true)) ||
         (rc = S->reserve(S->d_sh_skip, any_solute_shake ? 2 * (size_t)npad : 1, true)) ||
+        (rc = S->reserve(S->d_lc_work, any_lincs_u ? 6 * S->host_sh_len.size() : 1, true)) ||
         (rc = S->reserve(S->d_nh_pend, (size_t)GROMOS_AMD_NHC_MAX * njobs, true)) ||
         (rc = S->reserve(S->d_vv_e, 11 * nc, true)) ||
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
@@ -1266,7 +1371,7 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         // thermostat rescan, no per-kernel timing events): one captured graph
         const bool plain = S->use_graphs && S->built && S->steps_since_update + 1 < S->nsnb && S->xq_valid &&
                            !(S->any_thermo && S->therm_dirty) && !S->timing && !(S->twin && !S->long_computed) &&
-                           !S->any_com && !S->any_baro && !S->any_solute_shake &&
+                           !S->any_com && !S->any_baro && !S->any_solute_shake && !S->any_solvent_pass &&
                            S->vv_lanes > 0;
         if (plain) {
             S->steps_since_update++;
@@ -1294,12 +1399,13 @@ int md_steps(gromos_amd_system* S, int32_t nsteps, std::vector<double>* rec, std
         const int32_t write_xq = S->steps_since_update < S->nsnb ? 1 : 0;
         launch_integrate(p, S->ncg, S->twin ? 1 : 0, write_xq, mixed, st);
         if (S->any_solute_shake) launch_shake_solute(p, S->njobs, st);
+        if (S->any_solvent_pass) launch_solvent_constraints(p, S->ncg, st);
         launch_reduce(p, S->npad, S->twin ? 1 : 0, 1, 1, S->njobs, st);
         if (S->any_baro) launch_baro(p, S->njobs, st);
         S->cur = 1 - S->cur;   // exchange_state (configuration.rs:426-428)
         // the barostat scaled the new positions after the integrator copied them
-        // (and the solute's SHAKE moved positions after that copy)
-        S->xq_valid = write_xq != 0 && !S->any_baro && !S->any_solute_shake;
+        // (and the constraints after the integrator moved positions after that copy)
+        S->xq_valid = write_xq != 0 && !S->any_baro && !S->any_solute_shake && !S->any_solvent_pass;
         if (S->step_timing)
             if (int rc = step_event()) return rc;
     }
@@ -1536,6 +1642,11 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.sh_ij = d_sh_ij;
     p.sh_len = d_sh_len;
     p.sh_skip = d_sh_skip;
+    p.lc_sdiag = d_lc_sdiag;
+    p.lc_cpl = d_lc_cpl;
+    p.lc_cidx = d_lc_cidx;
+    p.lc_coef = d_lc_coef;
+    p.lc_work = d_lc_work;
     p.pflag = d_pflag;
     p.pa_iac = d_pa_iac;
     p.pb_iac = d_pb_iac;

@@ -2183,12 +2183,87 @@ __device__ bool shake_molecule(const JobDev& J, int n, const v3 (&xo)[4], v3 (&x
     return conv;
 }
 
-// shake of a job's solute constraints (constraints.rs:213-389, the solute's
-// list in the caller's order), a block per job after k_integrate: one thread
-// sweeps (the constraints couple the solute's charge groups), skip flags in
-// p.sh_skip; then v = (x_new - x_old) / dt for the constrained atoms, the
-// solute groups' E_kin terms again (as k_integrate writes them) and, for a
-// pressure-coupled job, the constraint virial into the first group's row.
+// lincs (constraints.rs:1241-1390) of a job's solute constraints, one thread:
+// B[k] = (x_old[j] - x_old[i]) / |...| (zero below 1e-10 nm); rhs = sol =
+// sdiag (B . (x[j] - x[i]) - l), then order times rhs' = A rhs (A[k][c] =
+// coef (B[k] . B[c]) over the coupled c, ascending), sol += rhs'; x[i] += B
+// sdiag sol / m_i, x[j] -= B sdiag sol / m_j; the same again with rhs = sdiag
+// (l - sqrt(max(2 l^2 - |x[j] - x[i]|^2, 0))) (the rotational correction);
+// then v = (x - x_old) / dt for the constrained atoms.  Work rows per
+// constraint in p.lc_work: B (3), the two right-hand sides, the solution.
+__device__ void lincs_solute(const SysPtrs& p, const JobDev& J) {
+    const int32_t nc = J.sh_n;
+    const int32_t* ij = p.sh_ij + 2 * (int64_t)J.sh_off;
+    const double* len = p.sh_len + J.sh_off;
+    const double* sd = p.lc_sdiag + J.sh_off;
+    const int32_t* cp = p.lc_cpl + 2 * (int64_t)J.sh_off;
+    double* w = p.lc_work + 6 * (int64_t)J.sh_off;
+    double* xn = p.x_new;
+    const double* xo = p.x_cur;
+    for (int32_t k = 0; k < nc; ++k) {
+        const d3 oi = ld3(xo, ij[2 * k]), oj = ld3(xo, ij[2 * k + 1]);
+        const double rx = oj.x - oi.x, ry = oj.y - oi.y, rz = oj.z - oi.z;
+        const double rl = sqrt(rx * rx + ry * ry + rz * rz);
+        w[6 * k] = rl < 1e-10 ? 0.0 : rx / rl;
+        w[6 * k + 1] = rl < 1e-10 ? 0.0 : ry / rl;
+        w[6 * k + 2] = rl < 1e-10 ? 0.0 : rz / rl;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int32_t k = 0; k < nc; ++k) {
+            const d3 ni = ld3(xn, ij[2 * k]), nj = ld3(xn, ij[2 * k + 1]);
+            const double rx = nj.x - ni.x, ry = nj.y - ni.y, rz = nj.z - ni.z;
+            double rhs;
+            if (pass == 0) {
+                const double proj = w[6 * k] * rx + w[6 * k + 1] * ry + w[6 * k + 2] * rz;
+                rhs = sd[k] * (proj - len[k]);
+            } else {
+                const double diff = 2.0 * len[k] * len[k] - (rx * rx + ry * ry + rz * rz);
+                const double pp = diff > 0.0 ? sqrt(diff) : 0.0;
+                rhs = sd[k] * (len[k] - pp);
+            }
+            w[6 * k + 3] = rhs;
+            w[6 * k + 5] = rhs;
+        }
+        int ws = 1;
+        for (int rec = 0; rec < J.lincs_u; ++rec) {
+            for (int32_t k = 0; k < nc; ++k) {
+                double acc = 0.0;
+                for (int32_t e = cp[2 * k]; e < cp[2 * k] + cp[2 * k + 1]; ++e) {
+                    const int32_t c = p.lc_cidx[e];
+                    const double bd = w[6 * k] * w[6 * c] + w[6 * k + 1] * w[6 * c + 1] + w[6 * k + 2] * w[6 * c + 2];
+                    acc += p.lc_coef[e] * bd * w[6 * c + 3 + (1 - ws)];
+                }
+                w[6 * k + 3 + ws] = acc;
+                w[6 * k + 5] += acc;
+            }
+            ws = 1 - ws;
+        }
+        for (int32_t k = 0; k < nc; ++k) {
+            const int64_t i = ij[2 * k], j = ij[2 * k + 1];
+            const double corr = sd[k] * w[6 * k + 5];
+            const double si = corr * p.inv_mass[i], sj = corr * p.inv_mass[j];
+            for (int c = 0; c < 3; ++c) {
+                xn[3 * i + c] = xn[3 * i + c] + w[6 * k + c] * si;
+                xn[3 * j + c] = xn[3 * j + c] - w[6 * k + c] * sj;
+            }
+        }
+    }
+    const double dti = 1.0 / J.dt;
+    double* vn = p.v_new;
+    for (int32_t k = 0; k < nc; ++k)
+        for (int s = 0; s < 2; ++s) {
+            const int64_t a = ij[2 * k + s];
+            for (int c = 0; c < 3; ++c) vn[3 * a + c] = (xn[3 * a + c] - xo[3 * a + c]) * dti;
+        }
+}
+
+// a job's solute constraints after k_integrate, a block per job: by LINCS
+// (lincs_solute) when the job asks for it, else by shake (constraints.rs:213-389,
+// the solute's list in the caller's order): one thread sweeps (the constraints
+// couple the solute's charge groups), skip flags in p.sh_skip; then v = (x_new -
+// x_old) / dt for the constrained atoms and, for a pressure-coupled job, the
+// constraint virial into the first group's row.  Either way the solute groups'
+// E_kin terms again (as k_integrate writes them).
 __global__ void __launch_bounds__(BLOCK) k_shake_solute(SysPtrs p) {
     const int32_t jb = blockIdx.x;
     const JobDev& J = p.jobs[jb];
@@ -2199,7 +2274,9 @@ __global__ void __launch_bounds__(BLOCK) k_shake_solute(SysPtrs p) {
     const double* len = p.sh_len + J.sh_off;
     double* xn = p.x_new;
     const double* xo = p.x_cur;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && J.lincs_u > 0) {
+        lincs_solute(p, J);
+    } else if (threadIdx.x == 0) {
         uint8_t* now = p.sh_skip + 2 * (int64_t)J.atom_off;   // two flags per atom
         for (int32_t a = 0; a < J.n_solute_atoms; ++a) {
             now[2 * a] = 0;
@@ -2272,6 +2349,139 @@ __global__ void __launch_bounds__(BLOCK) k_shake_solute(SysPtrs p) {
         p.cg_ek[3 * (int64_t)c] = ek;
         p.cg_ek[3 * (int64_t)c + 1] = ekn;
     }
+}
+
+// one LINCS expansion of a solvent molecule's template (at most 6 constraints,
+// unrolled so the rows stay in registers): dst[k] = sum over the coupled c
+// (ascending) of coef[k][c] (B[k] . B[c]) src[c]; sol += dst
+__device__ __forceinline__ void lincs_expand(const JobDev& J, int nc, const v3 (&b)[6], const double (&src)[6],
+                                             double (&dst)[6], double (&sol)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (k >= nc) continue;
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (c >= nc || !(J.lc_cpl[k] >> c & 1)) continue;
+            const double bd = b[k].x * b[c].x + b[k].y * b[c].y + b[k].z * b[c].z;
+            acc += J.lc_coef[6 * k + c] * bd * src[c];
+        }
+        dst[k] = acc;
+        sol[k] += acc;
+    }
+}
+
+// lincs of one solvent molecule by the template (lincs_buffered,
+// constraints.rs:1426-1440; the same steps as lincs_solute) in registers
+__device__ void lincs_molecule(const JobDev& J, const v3 (&xo)[4], v3 (&xn)[4], const double (&im)[4],
+                               unsigned& constrained) {
+    const int nc = J.n_sc;
+    v3 b[6];
+    double r0[6], r1[6], sol[6];
+    constrained = 0u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        b[k] = mk(0.0, 0.0, 0.0);
+        r0[k] = r1[k] = sol[k] = 0.0;
+        if (k >= nc) continue;
+        const int i = J.sc_ij[2 * k], j = J.sc_ij[2 * k + 1];
+        constrained |= (1u << i) | (1u << j);
+        const v3 oi = pick4(xo, i), oj = pick4(xo, j);
+        const double rx = oj.x - oi.x, ry = oj.y - oi.y, rz = oj.z - oi.z;
+        const double rl = sqrt(rx * rx + ry * ry + rz * rz);
+        if (!(rl < 1e-10)) b[k] = mk(rx / rl, ry / rl, rz / rl);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= nc) continue;
+            const v3 ni = pick4(xn, J.sc_ij[2 * k]), nj = pick4(xn, J.sc_ij[2 * k + 1]);
+            const double rx = nj.x - ni.x, ry = nj.y - ni.y, rz = nj.z - ni.z;
+            const double l = J.sc_len[k];
+            if (pass == 0) {
+                r0[k] = J.lc_sdiag[k] * ((b[k].x * rx + b[k].y * ry + b[k].z * rz) - l);
+            } else {
+                const double diff = 2.0 * l * l - (rx * rx + ry * ry + rz * rz);
+                r0[k] = J.lc_sdiag[k] * (l - (diff > 0.0 ? sqrt(diff) : 0.0));
+            }
+            sol[k] = r0[k];
+        }
+        for (int rec = 0; rec < J.lincs_sv; rec += 2) {   // r0 -> r1, then r1 -> r0
+            lincs_expand(J, nc, b, r0, r1, sol);
+            if (rec + 1 < J.lincs_sv) lincs_expand(J, nc, b, r1, r0, sol);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k >= nc) continue;
+            const int i = J.sc_ij[2 * k], j = J.sc_ij[2 * k + 1];
+            const double corr = J.lc_sdiag[k] * sol[k];
+            const double si = corr * (i == 0 ? im[0] : (i == 1 ? im[1] : (i == 2 ? im[2] : im[3])));
+            const double sj = corr * (j == 0 ? im[0] : (j == 1 ? im[1] : (j == 2 ? im[2] : im[3])));
+            const v3 ni = pick4(xn, i);
+            put4(xn, i, mk(ni.x + b[k].x * si, ni.y + b[k].y * si, ni.z + b[k].z * si));
+            const v3 nj = pick4(xn, j);
+            put4(xn, j, mk(nj.x - b[k].x * sj, nj.y - b[k].y * sj, nj.z - b[k].z * sj));
+        }
+    }
+}
+
+// The solvent molecules constrained by SHAKE or LINCS with the template (at
+// most 4 atoms and 6 constraints), a lane per group after k_integrate, which
+// moved them unconstrained: shake_molecule (its virial into the group's row
+// for a pressure-coupled job; an unconverged molecule fails its job) or
+// lincs_molecule (no virial: the reference's LINCS has none), then the new
+// positions, the constrained atoms' velocities (x - x_old) / dt and the
+// group's E_kin terms again, in k_integrate's order.
+__global__ void __launch_bounds__(WAVE) k_solvent_constraints(SysPtrs p, int32_t ncg) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncg) return;
+    const int32_t jb = p.cg_job[c];
+    const JobDev& J = p.jobs[jb];
+    if ((J.lincs_sv == 0 && !J.shake_sv) || c - J.cg_off < J.n_solute_cg) return;
+    const int32_t step = *p.step;
+    if (step >= J.stop) return;   // a frozen job keeps its state
+    const int32_t s = p.cg_start[c], n = p.cg_n[c];
+    v3 xo[4], xn[4], vn[4];
+    double im[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xo[k] = xn[k] = vn[k] = mk(0.0, 0.0, 0.0);
+        im[k] = 0.0;
+        if (k < n) {
+            xo[k] = ldv(p.x_cur, s + k);
+            xn[k] = ldv(p.x_new, s + k);
+            vn[k] = ldv(p.v_new, s + k);
+            im[k] = p.inv_mass[s + k];
+        }
+    }
+    if (J.shake_sv) {
+        double sv[3] = {0.0, 0.0, 0.0};
+        if (!shake_molecule(J, n, xo, xn, vn, im, sv)) atomicMin(p.settle_fail + jb, step);
+        if (J.baro)
+            for (int a = 0; a < 3; ++a) p.cg_cvir[3 * (int64_t)c + a] = sv[a];
+    } else {
+        unsigned constrained = 0u;
+        lincs_molecule(J, xo, xn, im, constrained);
+        const double dti = 1.0 / J.dt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (constrained >> k & 1u)
+                vn[k] = mk((xn[k].x - xo[k].x) * dti, (xn[k].y - xo[k].y) * dti, (xn[k].z - xo[k].z) * dti);
+    }
+    double ek = 0.0, ekn = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= n) continue;
+        stv(p.x_new, s + k, xn[k]);
+        stv(p.v_new, s + k, vn[k]);
+        const v3 vo = ldv(p.v_cur, s + k);
+        const double a2 = vn[k].x * vn[k].x + vn[k].y * vn[k].y + vn[k].z * vn[k].z;
+        const double b2 = vo.x * vo.x + vo.y * vo.y + vo.z * vo.z;
+        ek += 0.5 * p.mass[s + k] * (a2 + b2) / 2.0;   // temperature.rs:76-80, k_integrate's order
+        ekn += 0.5 * p.mass[s + k] * a2;
+    }
+    p.cg_ek[3 * (int64_t)c] = ek;
+    p.cg_ek[3 * (int64_t)c + 1] = ekn;
 }
 
 // The bath's scale of each coupled job (Berendsen, berendsen_thermostat.rs:66-100,
@@ -2623,21 +2833,9 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(SysPtrs p, int32_t ncg, int
         }
     };
     double cv[3] = {0.0, 0.0, 0.0};   // the constraints' virial diagonal (pressure-coupled jobs)
-    if (J.shake_sv && c - J.cg_off >= J.n_solute_cg) {
-        // a solvent molecule constrained by SHAKE with the template (n <= 4 atoms)
-        v3 xo[4], xn[4], vo[4], vn[4];
-        double im[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < 4; ++k) xo[k] = xn[k] = vo[k] = vn[k] = mk(0.0, 0.0, 0.0);
-        for (int k = 0; k < n && k < 4; ++k) {
-            step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
-            im[k] = p.inv_mass[s + k];
-        }
-        double sv[3] = {0.0, 0.0, 0.0};
-        if (!frozen && !shake_molecule(J, n, xo, xn, vn, im, sv)) atomicMin(p.settle_fail + jb, step);
-        if (J.baro && !frozen)
-            for (int a = 0; a < 3; ++a) cv[a] = sv[a];
-        for (int k = 0; k < n && k < 4; ++k) finish_atom(s + k, xn[k], vo[k], vn[k], xn[0]);
-    } else if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
+    // (a solvent molecule constrained by SHAKE or LINCS moves unconstrained here;
+    // k_solvent_constraints follows)
+    if (J.settle && n == 3 && c - J.cg_off >= J.n_solute_cg) {
         v3 xo[3], xn[3], vo[3], vn[3];
         for (int k = 0; k < 3; ++k) step_atom(s + k, xo[k], xn[k], vo[k], vn[k]);
         const v3 xu[3] = {xn[0], xn[1], xn[2]};
@@ -3042,6 +3240,9 @@ void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s) {
 }
 void launch_shake_solute(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_shake_solute, dim3(njobs), dim3(BLOCK), 0, s, p);
+}
+void launch_solvent_constraints(const SysPtrs& p, int32_t ncg, hipStream_t s) {
+    hipLaunchKernelGGL(k_solvent_constraints, dim3((unsigned)((ncg + WAVE - 1) / WAVE)), dim3(WAVE), 0, s, p, ncg);
 }
 void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_baro, dim3(njobs), dim3(BLOCK), 0, s, p);

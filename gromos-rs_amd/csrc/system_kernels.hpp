@@ -100,6 +100,11 @@ struct SysPtrs {
     const int32_t* sh_ij;      // the solute's SHAKE constraints (atom pairs; JobDev sh_off / sh_n)
     const double* sh_len;
     uint8_t* sh_skip;          // two SHAKE skip flags per atom (the solute's sweeps)
+    const double* lc_sdiag;    // LINCS of the solute's constraints (sh_off / sh_n): sdiag,
+    const int32_t* lc_cpl;     // (start, count) of each constraint's coupled ones in
+    const int32_t* lc_cidx;    //   lc_cidx (job-local constraint index, ascending) and
+    const double* lc_coef;     //   lc_coef
+    double* lc_work;           // per constraint: direction (3), two right-hand sides, solution
     // perturbation (original order; flag 0 = not perturbed)
     const uint8_t* pflag;
     const int32_t* pa_iac;
@@ -161,7 +166,10 @@ void launch_assemble(const SysPtrs& p, int32_t npad, int32_t with_long, int32_t 
 void launch_com(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_press_ke(const SysPtrs& p, int32_t njobs, hipStream_t s);
 void launch_baro(const SysPtrs& p, int32_t njobs, hipStream_t s);
+// the solute's constraints by SHAKE or LINCS, a block per job (after the integrator)
 void launch_shake_solute(const SysPtrs& p, int32_t njobs, hipStream_t s);
+// SHAKE or LINCS of the solvent molecules, a lane per group (after the integrator)
+void launch_solvent_constraints(const SysPtrs& p, int32_t ncg, hipStream_t s);
 void launch_therm(const SysPtrs& p, int32_t njobs, hipStream_t s);
 // leap-frog + Berendsen + SETTLE + E_kin (+ the next step's sorted copies) per group
 void launch_integrate(const SysPtrs& p, int32_t ncg, int32_t with_long, int32_t write_xq,

@@ -306,6 +306,8 @@ class GromosInput:
     ntcs: str = "shake"                    # solvent constraints algorithm
     ntcp: str = "shake"                    # solute constraints algorithm
     shake_tol: float = 1e-4                # NTCP0 when NTCP is shake
+    lincs_order_solute: int = 4            # NTCP0 when NTCP is lincs
+    lincs_order_solvent: int = 4           # NTCS0 when NTCS is lincs
     ntf: Tuple[int, int, int, int] = (1, 1, 1, 1)   # FORCE: bonds, angles, impropers, dihedrals on
     baths: List[Tuple[float, float]] = field(default_factory=list)   # (TEMP0, TAU)
     coupling: str = ""                     # weak-coupling, nose-hoover, nose-hoover-chains
@@ -364,17 +366,21 @@ def read_imd(path):
         d = _data(blocks["CONSTRAINT"])
         m.ntc = int(d[0].split()[0])
         # imd.rs:586-639: NTC; NTCP (a name or 1 shake, 2 lincs, 3 settle); NTCP0, the
-        # one SHAKE tolerance unless NTCP is lincs; NTCS (as NTCP, 0 none); NTCS0
-        # (read only for lincs)
+        # one SHAKE tolerance, or the solute's LINCS order when NTCP is lincs; NTCS
+        # (as NTCP, 0 none); NTCS0, read only as the solvent's LINCS order
         names = {"1": "shake", "2": "lincs", "3": "settle", "0": "none"}
         if len(d) > 1:
             w = d[1].split()[0].lower()
             m.ntcp = names.get(w, w)
         if len(d) > 2 and m.ntcp != "lincs":
             m.shake_tol = float(d[2].split()[0])
+        elif len(d) > 2:
+            m.lincs_order_solute = int(float(d[2].split()[0]))
         if len(d) > 3:
             w = d[3].split()[0].lower()
             m.ntcs = names.get(w, w)
+        if len(d) > 4 and m.ntcs == "lincs":
+            m.lincs_order_solvent = int(float(d[4].split()[0]))
     if "MULTIBATH" in blocks:
         d = _data(blocks["MULTIBATH"])
         k = 0
@@ -685,12 +691,14 @@ def apply_constraints(s, top, inp):
     """the CONSTRAINT block on a System (md.rs:337-352, 1208-1236; constraints.rs:36-60):
     NTC 2 constrains the solute bonds with a hydrogen (an atom of mass 2 or less),
     NTC 3 all of them, each of length B0 (bonds with B0 below 1e-10 are skipped), in
-    the topology's order, by SHAKE; the solvent by SHAKE (NTCS shake: every
-    molecule by the SOLVENTCONSTR template) or SETTLE; one tolerance, NTCP0.
-    LINCS is refused."""
+    the topology's order, by SHAKE or LINCS (NTCP, order NTCP0); the solvent by
+    SHAKE or LINCS (NTCS shake / lincs, order NTCS0: every molecule by the
+    SOLVENTCONSTR template) or SETTLE; one SHAKE tolerance, NTCP0."""
     if inp.ntc > 1 and top.bonds:
-        if inp.ntcp != "shake":
-            raise GromosFormatError(f"solute constraints by {inp.ntcp!r} are not provided (SHAKE only)")
+        if inp.ntcp not in ("shake", "lincs"):
+            raise GromosFormatError(f"solute constraints by {inp.ntcp!r} are not provided (SHAKE or LINCS)")
+        if inp.ntcp == "lincs":
+            s.solute_lincs = inp.lincs_order_solute
         cons = []
         for i, j, t in top.bonds:
             r0 = top.bond_types[t][2]
@@ -698,11 +706,14 @@ def apply_constraints(s, top, inp):
                 continue
             cons.append((i, j, r0))
         s.solute_constraints = np.array(cons, float).reshape(-1, 3)
-    if s.nsm > 0 and inp.ntcs == "shake":
-        s.solvent_shake = True
+    if s.nsm > 0 and inp.ntcs in ("shake", "lincs"):
         s.solvent_constraints = np.array(top.solvent_constr, float).reshape(-1, 3)
+        if inp.ntcs == "shake":
+            s.solvent_shake = True
+        else:
+            s.solvent_lincs = inp.lincs_order_solvent
     elif s.nsm > 0 and inp.ntcs != "settle":
-        raise GromosFormatError(f"solvent constraints by {inp.ntcs!r} are not provided (SHAKE or SETTLE)")
+        raise GromosFormatError(f"solvent constraints by {inp.ntcs!r} are not provided (SHAKE, LINCS or SETTLE)")
     s.shake_tolerance = inp.shake_tol
     return s
 

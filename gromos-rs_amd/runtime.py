@@ -84,7 +84,7 @@ class Params(C.Structure):
                 ("thermostat", I32), ("t_ref", D), ("tau_t", D), ("dof_t", D), ("box_matrix", D * 9),
                 ("nhc", I32), ("nticom", I32), ("nscm", I32),
                 ("barostat", I32), ("p_ref", D), ("compressibility", D), ("tau_p", D), ("virial", I32),
-                ("solvent_shake", I32), ("shake_tolerance", D)]
+                ("solvent_shake", I32), ("shake_tolerance", D), ("solute_lincs", I32), ("solvent_lincs", I32)]
 
 
 def built():
@@ -257,7 +257,7 @@ def topology_struct(s, keep):
         t.n_pert_atom_pairs = len(ap) // 4
         t.pert_atom_pairs = keep(ap, np.int32)
     tc = getattr(s, "solvent_constraints", None)
-    if getattr(s, "solvent_shake", False) and tc is not None and len(tc):
+    if (getattr(s, "solvent_shake", False) or getattr(s, "solvent_lincs", 0) > 0) and tc is not None and len(tc):
         tc = np.asarray(tc, float).reshape(-1, 3)
         t.n_solvent_constraints = len(tc)
         t.solvent_constraint_ij = keep(tc[:, :2], np.int32)
@@ -324,6 +324,8 @@ def params_struct(s, use_settle=True, do_virial=False):
         p.virial = int(ba[3])
     p.solvent_shake = int(bool(getattr(s, "solvent_shake", False)))
     p.shake_tolerance = float(getattr(s, "shake_tolerance", 1e-4))
+    p.solute_lincs = int(getattr(s, "solute_lincs", 0))
+    p.solvent_lincs = int(getattr(s, "solvent_lincs", 0))
     return p
 
 

@@ -367,6 +367,13 @@ typedef struct {
      * for the solute's constraints) */
     int32_t solvent_shake;
     double shake_tolerance;
+    /* LINCS (constraints.rs:1168-1445, md.rs:1244-1257): solute_lincs > 0
+     * constrains the solute's constraints (topology) by LINCS of that expansion
+     * order (NTCP lincs, NTCP0) instead of SHAKE; solvent_lincs > 0 every solvent
+     * molecule by the topology's template (NTCS lincs, NTCS0; at most 4 atoms and
+     * 6 constraints) instead of SHAKE or SETTLE.  LINCS adds no constraint virial. */
+    int32_t solute_lincs;
+    int32_t solvent_lincs;
 } gromos_amd_params;
 
 #define GROMOS_AMD_NHC_MAX 16

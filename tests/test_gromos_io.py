@@ -123,9 +123,20 @@ def test_constraint_block_and_constraint_selection():
     assert all(min(s.mass[int(i)], s.mass[int(j)]) == 1.008 for i, j, _ in got[2])
     # every NTC 2 constraint is also an NTC 3 one
     assert {tuple(r) for r in got[2]} <= {tuple(r) for r in got[3]}
-    inp.ntcs = "lincs"
+    inp.ntcs = "none"
     with pytest.raises(gio.GromosFormatError):
         gio.apply_constraints(s, t, inp)
+    # LINCS: NTCP lincs with NTCP0 the solute's order; NTCS lincs with NTCS0 the solvent's
+    top, conf, imd, _ = files("aladip_vacuum_lincs")
+    s, t, inp = gio.load(top, conf, imd)
+    assert (inp.ntc, inp.ntcp, inp.lincs_order_solute, inp.ntcs) == (3, "lincs", 4, "shake")
+    gio.apply_constraints(s, t, inp)
+    assert s.solute_lincs == 4 and len(s.solute_constraints) == 11 and not s.solvent_shake
+    top, conf, imd, _ = files("nacl_1water_lincs")
+    s, t, inp = gio.load(top, conf, imd)
+    assert (inp.ntc, inp.ntcs, inp.lincs_order_solvent) == (1, "lincs", 4)
+    gio.apply_constraints(s, t, inp)
+    assert s.solvent_lincs == 4 and s.solvent_constraints.shape == (3, 3) and s.solute_constraints is None
 
 
 def test_energy_trajectory_round_trip(tmp_path):

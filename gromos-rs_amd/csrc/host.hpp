@@ -238,6 +238,8 @@ struct gromos_amd_system {
     int32_t* d_sh_ij = nullptr;
     double* d_sh_len = nullptr;
     uint8_t* d_sh_skip = nullptr;
+    unsigned* d_vv_sched = nullptr;        // the persistent far-pair kernel's counters
+    int32_t vv_persist = 0;                // its resident blocks (0: not used)
     bool any_solvent_pass = false;         // a job constrains its solvent by SHAKE or LINCS (after the integrator)
     std::vector<double> host_lc_sdiag, host_lc_coef;   // LINCS of the solute constraints
     std::vector<int32_t> host_lc_cpl, host_lc_cidx;

@@ -682,6 +682,16 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     S->job_step0.assign((size_t)njobs, 0);
     S->host_pressure.assign((size_t)njobs, 0.0);
     S->near_r = VV_NEAR_R;
+    // the far-pair kernel's persistent form (GROMOS_AMD_VV_PERSIST=1): three
+    // resident blocks of four waves per CU at its three waves per SIMD
+    if (const char* env = std::getenv("GROMOS_AMD_VV_PERSIST"); env && std::atoi(env) == 1) {
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        S->vv_persist = std::max(1, cus) * 3;
+    } else {
+        S->vv_persist = 0;
+    }
     if (const char* env = std::getenv("GROMOS_AMD_VV_NEAR_R")) S->near_r = std::max(0.0, std::atof(env));
     // LJ only between the solvent molecules' first atoms (SPC-like waters)
     S->vv_lj00 = 1;
@@ -893,6 +903,7 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
         (rc = S->reserve(S->d_vv_part, (size_t)PART_W * VV_PARTS * njobs, true)) ||
         (rc = S->reserve(S->d_cg_ek, 3 * nc, true)) || (rc = S->reserve(S->d_step, 1)) ||
         (rc = S->reserve(S->d_vv_near, nc, true)) || (rc = S->reserve(S->d_fi_near, n3, true)) ||
+        (rc = S->reserve(S->d_vv_sched, (size_t)NXCD_SCHED)) ||
         (rc = S->reserve(S->d_vv_en, 11 * nc, true)) ||
         (rc = S->reserve(S->d_step_done, 1)) ||
         (rc = S->reserve(S->d_sol_e, (size_t)2 * ROW * std::max<int32_t>(S->nsol, 1), true)))
@@ -916,6 +927,9 @@ int layout_batch(gromos_amd_system* S, const std::vector<const JobImage*>& imgs,
     HIPCHK(hipMemsetAsync(S->d_xf, 0, npa * sizeof(float4), st));
     HIPCHK(hipMemsetAsync(S->d_f_long_s, 0, n3 * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->d_vv_near, 0, nc * sizeof(int32_t), st));
+    // the persistent far-pair kernel's counters start at zero (each launch's last
+    // wave zeroes them again)
+    HIPCHK(hipMemsetAsync(S->d_vv_sched, 0, NXCD_SCHED * sizeof(unsigned), st));
     // Nose-Hoover chains start at zero (nosehoover_thermostat.rs:64-113); a
     // session puts continuing jobs' chains back after the layout
     HIPCHK(hipMemsetAsync(S->d_nh_zeta, 0, (size_t)GROMOS_AMD_NHC_MAX * njobs * sizeof(double), st));
@@ -1622,6 +1636,8 @@ SysPtrs gromos_amd_system::ptrs() const {
     p.nsol = nsol;
     p.vv_chunk = d_vv_chunk;
     p.nvv_chunk = nvv_chunk;
+    p.vv_sched = d_vv_sched;
+    p.vv_persist = vv_persist;
     p.step = d_step;
     p.prec = d_prec;
     p.s_prec = d_s_prec;

@@ -11,6 +11,8 @@
 //  step           k_therm, k_integrate (leap-frog, SETTLE, E_kin), k_vv_parts,
 //                 k_reduce           (leap_frog.rs:37-148, constraints.rs:815-1036,
 //                                     temperature.rs:63-87)
+#include <algorithm>
+
 #include "system_kernels.hpp"
 
 namespace gamd {
@@ -21,6 +23,7 @@ namespace gamd {
 // and its halo instead of the whole system.  The grid is a multiple of NXCD;
 // returns -1 for padding blocks (whole blocks exit, no block barrier is hit).
 constexpr int NXCD = 8;
+static_assert(NXCD_SCHED == NXCD + 1, "one counter per XCD and a finished-wave count");
 __device__ __forceinline__ int32_t xcd_block(int32_t nreal) {
     const int32_t per = (int32_t)gridDim.x / NXCD;
     int32_t b = ((int32_t)blockIdx.x % NXCD) * per + (int32_t)blockIdx.x / NXCD;
@@ -628,6 +631,19 @@ __device__ __forceinline__ bool held(const SysPtrs& p, int32_t job) { return *p.
 // constants are wave-uniform (scalar registers, fewer vector registers).
 // Returns false for waves past the last chunk (the whole wave leaves).
 template <int NCHM>
+__device__ __forceinline__ void vv_wave_at(const SysPtrs& p, int32_t wvi, int32_t& k, bool& valid, int& chunk,
+                                           int32_t& job) {
+    constexpr int G = WAVE / NCHM;
+    constexpr int SUB = VV_CHUNK / G;
+    const int4 ch = p.vv_chunk[wvi / SUB];
+    const int32_t first = __builtin_amdgcn_readfirstlane(ch.x), end = __builtin_amdgcn_readfirstlane(ch.y);
+    job = __builtin_amdgcn_readfirstlane(ch.z);
+    const int lane = threadIdx.x & (WAVE - 1);
+    k = first + (wvi % SUB) * G + lane / NCHM;
+    valid = k < end;
+    chunk = lane % NCHM;
+}
+template <int NCHM>
 __device__ __forceinline__ bool vv_wave(const SysPtrs& p, int32_t nwaves, int32_t& k, bool& valid, int& chunk,
                                         int32_t& job) {
     constexpr int G = WAVE / NCHM;
@@ -775,12 +791,7 @@ __global__ void __launch_bounds__(BLOCK) k_vv(SysPtrs p, int32_t nwaves) {
 // than VV_NEAR_R at the slot's start (upwards) and the others at its end
 // (downwards); this kernel evaluates the latter, k_vv_nr the near ones in FP64.
 template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
-// four waves per SIMD (at most 128 vector registers, no spills)
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) k_vv_mx(SysPtrs p, int32_t nwaves) {
-    int32_t k = 0, job = 0;
-    bool valid = false;
-    int chunk = 0;
-    if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
+__device__ __forceinline__ void vv_mx_wave(const SysPtrs& p, int32_t k, bool valid, int chunk, int32_t job) {
     if (held(p, job)) return;
     long long fi[APS][3];
     long long ei[11];
@@ -995,6 +1006,53 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4
         }
 #pragma unroll
         for (int c = 0; c < 11; ++c) p.vv_e[11 * (int64_t)k + c] = (double)ei[c] * EX_INV;
+    }
+}
+
+// four waves per SIMD (at most 128 vector registers, no spills)
+template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) k_vv_mx(SysPtrs p, int32_t nwaves) {
+    int32_t k = 0, job = 0;
+    bool valid = false;
+    int chunk = 0;
+    if (!vv_wave<NCHM>(p, nwaves, k, valid, chunk, job)) return;
+    vv_mx_wave<APS, LJ00, VIR, NCHM, NEUT>(p, k, valid, chunk, job);
+}
+
+// The same waves in a persistent grid (as many blocks as stay resident): the
+// logical waves are split over the eight XCDs as k_vv_mx's blocks are (the
+// same slab of sorted groups per L2), and each resident wave takes the next
+// logical wave of its XCD from a counter until none is left, so a wave that
+// finishes early takes more work instead of idling until its block ends.
+// Every logical wave writes only its own rows, so the order of taking changes
+// no result.  sched[0..7]: the XCDs' next logical wave, sched[8]: resident
+// waves finished; the last one out zeroes them for the next launch.  Every
+// wave leaves: the counters only grow, and a wave leaves at the first index
+// past its XCD's share.  Three waves per SIMD: with the loop around the body
+// four (128 registers) spill to scratch.
+template <int APS, bool LJ00, bool VIR, int NCHM, bool NEUT>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_vv_mx_persist(SysPtrs p, int32_t nwaves, int32_t per_xcd) {
+    const int32_t x = (int32_t)blockIdx.x % NXCD;
+    const int32_t lo = x * per_xcd, hi = min(nwaves, lo + per_xcd);
+    unsigned* sched = p.vv_sched;
+    const bool lead = (threadIdx.x & (WAVE - 1)) == 0;
+    for (;;) {
+        int32_t i = 0;
+        if (lead) i = (int32_t)atomicAdd(sched + x, 1u);
+        i = __builtin_amdgcn_readfirstlane(i);   // lane 0's: every lane is active here
+        if (lo + i >= hi) break;
+        int32_t k = 0, job = 0;
+        bool valid = false;
+        int chunk = 0;
+        vv_wave_at<NCHM>(p, lo + i, k, valid, chunk, job);
+        vv_mx_wave<APS, LJ00, VIR, NCHM, NEUT>(p, k, valid, chunk, job);
+    }
+    if (lead) {
+        __threadfence();
+        const unsigned waves = gridDim.x * (BLOCK / WAVE);
+        if (atomicAdd(sched + NXCD, 1u) == waves - 1u)
+            for (int c = 0; c <= NXCD; ++c) atomicExch(sched + c, 0u);
     }
 }
 
@@ -3126,7 +3184,25 @@ static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t m
         if (e0 || e1) hipExtLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, e0, e1, 0, p, nw);       \
         else hipLaunchKernelGGL(KER, g, dim3(BLOCK), 0, s, p, nw);                               \
     } while (0)
-    if (mixed && neutral) {
+#define GAMD_VV_PERSIST(KER)                                                                               \
+    do {                                                                                                   \
+        if (e0 || e1) hipExtLaunchKernelGGL(KER, gp, dim3(BLOCK), 0, s, e0, e1, 0, p, nw, per_xcd);       \
+        else hipLaunchKernelGGL(KER, gp, dim3(BLOCK), 0, s, p, nw, per_xcd);                               \
+    } while (0)
+    if (mixed && p.vv_persist > 0) {
+        // the persistent form: at most p.vv_persist resident blocks, a multiple of
+        // the XCD count, each XCD's share of logical waves as k_vv_mx splits them
+        const int32_t perb = (int32_t)(g.x / NXCD);
+        const int32_t per_xcd = perb * (BLOCK / WAVE);
+        const dim3 gp((unsigned)(std::max(1, std::min(perb, p.vv_persist / NXCD)) * NXCD));
+        if (neutral) {
+            if (virial) GAMD_VV_PERSIST((k_vv_mx_persist<APS, LJ00, true, L, true>));
+            else GAMD_VV_PERSIST((k_vv_mx_persist<APS, LJ00, false, L, true>));
+        } else {
+            if (virial) GAMD_VV_PERSIST((k_vv_mx_persist<APS, LJ00, true, L, false>));
+            else GAMD_VV_PERSIST((k_vv_mx_persist<APS, LJ00, false, L, false>));
+        }
+    } else if (mixed && neutral) {
         if (virial) GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, true, L, true>));
         else GAMD_VV_LAUNCH((k_vv_mx<APS, LJ00, false, L, true>));
     } else if (mixed) {
@@ -3137,6 +3213,7 @@ static void launch_vv_l(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t m
         else GAMD_VV_LAUNCH((k_vv<APS, LJ00, false, L>));
     }
 #undef GAMD_VV_LAUNCH
+#undef GAMD_VV_PERSIST
 }
 template <int APS, bool LJ00>
 static void launch_vv_aps(const SysPtrs& p, int32_t ncg, int32_t virial, int32_t mixed, int32_t neutral,

@@ -50,6 +50,8 @@ struct SysPtrs {
     int64_t* gen_cnt;
     const int4* vv_chunk;      // nvv_chunk: solvent pair kernel chunks (first sorted group, end, job, 0)
     int32_t nvv_chunk;
+    unsigned* vv_sched;        // the persistent far-pair kernel's counters (NXCD_SCHED, zero between launches)
+    int32_t vv_persist;        // resident blocks of that kernel; 0: the one-wave-per-launch-slot form
     int32_t* step;             // md step index within the current md call (device counter)
     const float4* prec;        // 2 per solute atom: (a_q, b_q, alpha_lj, alpha_crf), (a/b soft crf_cut3i, a/b iac bits)
     int32_t* s_prec;           // sorted atom -> its prec record (perturbed atoms), -1 otherwise
@@ -145,6 +147,7 @@ void launch_gather(const SysPtrs& p, int32_t npad, int32_t mixed, hipStream_t s)
 // solvent molecules: the mixed kernel sums the RF polynomial per molecule pair);
 // lanes: lanes per solvent molecule (4, 8 or 16)
 constexpr int32_t VV_LJ00 = 1, VV_NEUTRAL = 2;
+constexpr int32_t NXCD_SCHED = 9;   // the persistent far-pair kernel's counters: one per XCD, waves done
 void launch_vv(const SysPtrs& p, int32_t ncg, int32_t aps, int32_t form, int32_t virial, int32_t mixed,
                int32_t lanes, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // mixed: the near solvent partners in FP64 (k_vv_nr) into fi_near / vv_en;

@@ -43,8 +43,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=15, help="timed oracle MD steps of the CPU baseline (about 10-15 s)")
     ap.add_argument("--cpu-steps-1core", type=int, default=5,
                     help="timed oracle MD steps of the single-core CPU baseline (one pair-list cycle, about 10 s)")
-    ap.add_argument("--only", choices=["fep-mixed", "water"], default=None,
-                    help="run one part only (for profiling): the FEP mix's batched mixed replay, or the water box")
+    ap.add_argument("--only", choices=["fep-mixed", "water", "remd"], default=None,
+                    help="run one part only (for profiling): the FEP mix's batched mixed replay, the water box, "
+                         "or the replica-exchange batch (--remd-steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (HBM counter calibration) and exit")
@@ -214,6 +215,10 @@ def main():
         # one batched mixed-precision replay of the FEP mix, nothing else (for profiling)
         fep = fep_mix_bench(rt, gen, a.fep_steps, a.fep_alone_steps, a.fep_admit_interval, only_mixed=True)
         print(json.dumps({"fep_mix": fep}), flush=True)
+        return
+    if a.only == "remd":
+        # the 128-replica exchange batch only (for profiling)
+        print(json.dumps({"remd": remd_bench(rt, gen, a.remd_steps)}), flush=True)
         return
     if a.only == "water":
         # the mixed path on the water box only (for profiling)

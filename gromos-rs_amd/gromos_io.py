@@ -323,6 +323,7 @@ class GromosInput:
     ntwf: int = 0
     nscm: int = 0
     nticom: int = 0                        # INITIALISE: centre-of-mass motion removal at step 0
+    ntishk: int = 0                        # INITIALISE: SHAKE the initial positions (1) and velocities (2)
     couple: str = "off"                    # PRESSURESCALE COUPLE: off, calc, scale
     scale: str = "iso"
     compressibility: float = 4.575e-4
@@ -417,6 +418,8 @@ def read_imd(path):
         m.nscm = int(_tokens(blocks["COMTRANSROT"])[0])
     if "INITIALISE" in blocks:
         v = _tokens(blocks["INITIALISE"])
+        if len(v) > 1:
+            m.ntishk = int(v[1])
         if len(v) > 6:
             m.nticom = int(v[6])     # NTIVEL NTISHK NTINHT NTINHB NTISHI NTIRTC NTICOM
     return m
@@ -728,6 +731,9 @@ def run_md(topology, configuration, imd, out_prefix, precision=None, ptp=None, n
     Returns the System and the per-step energies (e_lj, e_crf, e_pot, e_kin, dhdl)."""
     from . import runtime as rt
     s, top, inp = load(topology, configuration, imd, ptp)
+    if inp.ntishk >= 1:
+        # md.rs:1227-1235, shake_positions / shake_velocities (constraints.rs:575-650)
+        raise GromosFormatError("INITIALISE NTISHK >= 1 (SHAKE of the initial configuration) is not provided")
     apply_constraints(s, top, inp)
     precision = rt.FP64 if precision is None else precision
     n = inp.nstlim if nsteps is None else int(nsteps)

@@ -109,7 +109,7 @@ def test_constraint_block_and_constraint_selection():
     m = gio.read_imd(imd)
     assert (m.ntc, m.ntcp, m.shake_tol, m.ntcs) == (3, "shake", 1e-4, "shake")
     _, _, imd, _ = files("nacl_1water_settle")
-    assert gio.read_imd(imd).ntcs == "settle"
+    assert gio.read_imd(imd).ntcs == "settle" and gio.read_imd(imd).ntishk == 0
     top, conf, imd, _ = files("aladip_solvated")
     got = {}
     for ntc in (1, 2, 3):

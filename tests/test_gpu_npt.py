@@ -71,6 +71,31 @@ def test_npt_settle_water_matches_oracle(vtype):
     assert box[0] != w.box[0]
 
 
+@pytest.mark.parametrize("method", ["shake", "lincs"])
+def test_npt_constrained_water_matches_oracle(method):
+    """waters constrained by SHAKE (its constraint virial joins the pressure,
+    constraints.rs:213-389) or by LINCS (no constraint virial, as the
+    reference's) under the barostat, molecular virial"""
+    w = gen.spc_water_box(400, seed=12)
+    w.barostat = (0.06102, 4.575e-4, 0.5, 2)
+    hh = 2.0 * 0.1 * np.sin(np.radians(109.47) / 2.0)
+    w.solvent_constraints = np.array([[0, 1, 0.1], [0, 2, 0.1], [1, 2, hh]])
+    if method == "shake":
+        w.solvent_shake = True
+    else:
+        w.solvent_lincs = 4
+    n = 15
+    md = oracle.MD(oracle.OracleSystem(w), use_settle=False)
+    e_ref = md.run(n)
+    x_ref, _, _ = md.state()
+    e, x, v, box, p = device_md(w, n)
+    for c in range(4):
+        assert rel(e[:, c], e_ref[:, c]).max() < 1e-10, (c, rel(e[:, c], e_ref[:, c]).max())
+    assert np.abs(x - x_ref).max() < 1e-10
+    assert rel(box, md.box()).max() < 1e-13 and rel(p, md.pressure()) < 1e-9, (p, md.pressure())
+    assert box[0] != w.box[0]
+
+
 def test_npt_mixed_precision_follows_fp64():
     s, d = tn.with_npt()
     _, _, _, b64, p64 = device_md(s, 20)

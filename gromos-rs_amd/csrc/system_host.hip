@@ -83,6 +83,7 @@ static int32_t cells_for(double L, double rl) {
 void rebox_image(JobImage& im, const Box& box) {
     JobDev& J = im.J;
     J.box = box;
+    if (box.kind == 2) return;   // one cell, CG-grouped loops off, whatever the size
     double hm = 1e300;
     for (int a = 0; a < 3; ++a) {
         J.ncell[a] = cells_for(box.L[a] * 0.95, 2.0 * J.cell_min);
@@ -199,8 +200,8 @@ int make_image(const gromos_amd_topology& T, const gromos_amd_params& P, int j, 
     J.dof_t = P.dof_t;
     if (P.barostat < 0 || P.barostat > 1) return fail(GROMOS_AMD_EINVAL, "job %d: barostat must be 0 or 1", j);
     if (P.barostat) {
-        if (P.boundary != 1)
-            return fail(GROMOS_AMD_EINVAL, "job %d: the barostat needs a rectangular box", j);
+        if (P.boundary != 1 && P.boundary != 2)
+            return fail(GROMOS_AMD_EINVAL, "job %d: the barostat needs a periodic box", j);
         if (P.virial < 0 || P.virial > 2 || !(P.tau_p > 0.0))
             return fail(GROMOS_AMD_EINVAL, "job %d: barostat virial must be 0, 1 or 2 and tau_p > 0", j);
     }

@@ -2774,7 +2774,12 @@ __global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
     block_sum<6>(w, lds);
     if (threadIdx.x == 0) {
         double* pr = p.press + 10 * (int64_t)jb;
-        const double V = box.L[0] * box.L[1] * box.L[2];
+        // BoxConfig::volume, the box matrix's determinant c . (a x b) in glam's
+        // order (configuration.rs:86-89); the rectangular product equals it
+        const double V = box.kind == 2 ? box.c[0] * (box.a[1] * box.b[2] - box.b[1] * box.a[2]) +
+                                             box.c[1] * (box.a[2] * box.b[0] - box.b[2] * box.a[0]) +
+                                             box.c[2] * (box.a[0] * box.b[1] - box.b[0] * box.a[1])
+                                       : box.L[0] * box.L[1] * box.L[2];
         double P = pr[9];   // no virial: the pressure keeps its last value (0 at start)
         if (J.vtype != 0 && !(V < 1e-30)) {
             const double* rec = p.records + ((int64_t)step * gridDim.x + jb) * REC;
@@ -2789,15 +2794,33 @@ __global__ void __launch_bounds__(BLOCK) k_baro(SysPtrs p) {
         }
         const double mu = pow(1.0 - J.comp * J.dt / J.tau_p * (J.p_ref - P), 1.0 / 3.0);
         s_mu = mu;
-        double hm = 1e300;
-        for (int a = 0; a < 3; ++a) {
-            const double L = box.L[a] * mu;
-            J.box.L[a] = L;
-            J.box.invL[a] = 1.0 / L;
-            hm = fmin(hm, L * 0.5);
-            if (!(L / J.ncell[a] >= J.cell_min)) atomicMin(p.box_fail + jb, step);
+        if (box.kind == 2) {
+            // every box vector scaled (berendsen_barostat.rs:40-48); the diagonal,
+            // its inverse and the reduction ratios follow as the box was built
+            // (one cell, CG-grouped loops off: nothing else depends on the size)
+            for (int a = 0; a < 3; ++a) {
+                J.box.a[a] = box.a[a] * mu;
+                J.box.b[a] = box.b[a] * mu;
+                J.box.c[a] = box.c[a] * mu;
+            }
+            J.box.L[0] = J.box.a[0];
+            J.box.L[1] = J.box.b[1];
+            J.box.L[2] = J.box.c[2];
+            for (int a = 0; a < 3; ++a) J.box.invL[a] = 1.0 / J.box.L[a];
+            J.box.tr[0] = J.box.c[1] / J.box.c[2];
+            J.box.tr[1] = J.box.b[0] / J.box.b[1];
+            J.box.tr[2] = J.box.c[0] / J.box.c[2];
+        } else {
+            double hm = 1e300;
+            for (int a = 0; a < 3; ++a) {
+                const double L = box.L[a] * mu;
+                J.box.L[a] = L;
+                J.box.invL[a] = 1.0 / L;
+                hm = fmin(hm, L * 0.5);
+                if (!(L / J.ncell[a] >= J.cell_min)) atomicMin(p.box_fail + jb, step);
+            }
+            J.use_cg_grouped = hm > J.cgg_limit;
         }
-        J.use_cg_grouped = hm > J.cgg_limit;
     }
     __syncthreads();
     const double mu = s_mu;

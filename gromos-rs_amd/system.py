@@ -130,7 +130,7 @@ class System:
     nticom: int = 0
     nscm: int = 0
     # pressure coupling (berendsen_barostat.rs, pressure_calculation.rs), isotropic,
-    # rectangular boxes: (P0 kJ mol^-1 nm^-3, compressibility, tau_p ps, virial
+    # rectangular or triclinic boxes: (P0 kJ mol^-1 nm^-3, compressibility, tau_p ps, virial
     # 0 none / 1 atomic / 2 molecular)
     barostat: Optional[tuple] = None
     # SHAKE (constraints.rs:213-570): solvent_shake constrains every solvent

@@ -354,7 +354,8 @@ typedef struct {
      * translation and, without a periodic box, the rotation every |nscm| steps */
     int32_t nticom, nscm;
     /* pressure coupling at the end of md steps (md.rs:1116-1129, 1259-1277;
-     * rectangular boxes): barostat 1 = isotropic Berendsen (berendsen_barostat.rs)
+     * rectangular or triclinic boxes, every box vector scaled): barostat 1 =
+     * isotropic Berendsen (berendsen_barostat.rs)
      * toward p_ref (kJ mol^-1 nm^-3) with compressibility (mol nm^3 kJ^-1, the
      * GROMOS input value) and tau_p (ps); the pressure (pressure_calculation.rs)
      * from the virial: 1 atomic, 2 molecular (prepare_virial, forcefield.rs:285-296,

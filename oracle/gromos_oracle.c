@@ -1652,12 +1652,13 @@ void or_ff_set_box(or_ff *ff, const double L[3]) {
 
 /* pressure coupling (md.rs:1116-1129, 1259-1277): PressureCalculation
  * (pressure_calculation.rs) on the atomic or molecular virial and
- * BerendsenBarostat (berendsen_barostat.rs), isotropic, rectangular boxes;
+ * BerendsenBarostat (berendsen_barostat.rs), isotropic, rectangular or
+ * triclinic boxes (every box vector scaled, the volume the determinant);
  * pg_last: the last atom of each pressure group (solute groups, then every
  * solvent molecule) */
 int or_md_set_barostat(or_md *md, double p_ref, double comp, double tau_p, int vtype, int n_pg,
                        const int32_t *pg_last) {
-    if (md->ff->pbc.kind != 1 || vtype < 0 || vtype > 2 || n_pg < 0) return -1;
+    if ((md->ff->pbc.kind != 1 && md->ff->pbc.kind != 2) || vtype < 0 || vtype > 2 || n_pg < 0) return -1;
     md->baro = 1;
     md->p_ref = p_ref;
     md->comp = comp;
@@ -2169,7 +2170,14 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
              * (berendsen_barostat.rs:30-70): mu = (1 - comp dt / tau (P0 - P))^(1/3)
              * scales the box and the new positions */
             const double *L = md->ff->pbc.L;
-            const double V = L[0] * L[1] * L[2];
+            const or_pbc *pb = &md->ff->pbc;
+            /* BoxConfig::volume, the box matrix's determinant c . (a x b) (glam's
+             * order, configuration.rs:86-89); the rectangular product equals it */
+            const double V = pb->kind == 2
+                                 ? pb->c[0] * (pb->a[1] * pb->b[2] - pb->b[1] * pb->a[2]) +
+                                       pb->c[1] * (pb->a[2] * pb->b[0] - pb->b[2] * pb->a[0]) +
+                                       pb->c[2] * (pb->a[0] * pb->b[1] - pb->b[0] * pb->a[1])
+                                 : L[0] * L[1] * L[2];
             if (md->vtype != 0 && !(V < 1e-30)) {
                 const double factor = 2.0 / V;
                 double pt[3];
@@ -2177,8 +2185,19 @@ int or_md_run(or_md *md, int nsteps, double *energies) {
                 md->pressure = (pt[0] + pt[1] + pt[2]) / 3.0;
             }
             const double mu = pow(1.0 - md->comp * md->dt / md->tau_p * (md->p_ref - md->pressure), 1.0 / 3.0);
-            double Ln[3] = {L[0] * mu, L[1] * mu, L[2] * mu};
-            or_ff_set_box(md->ff, Ln);
+            if (pb->kind == 2) {
+                /* every box vector scaled (berendsen_barostat.rs:40-48) */
+                double v9[9];
+                for (int a = 0; a < 3; ++a) {
+                    v9[a] = pb->a[a] * mu;
+                    v9[3 + a] = pb->b[a] * mu;
+                    v9[6 + a] = pb->c[a] * mu;
+                }
+                or_pbc_init(&md->ff->pbc, 2, v9);   /* CG-grouped loops stay off for triclinic */
+            } else {
+                double Ln[3] = {L[0] * mu, L[1] * mu, L[2] * mu};
+                or_ff_set_box(md->ff, Ln);
+            }
             for (int k = 0; k < 3 * n; ++k) md->pos[o][k] *= mu;
         }
         if (energies) {

@@ -563,7 +563,7 @@ class MD:
             p_ref, comp, tau_p, vtype = s.barostat
             self.pg = np.ascontiguousarray(s.pressure_group_last(), dtype=np.int32)
             if lib().or_md_set_barostat(self.h, p_ref, comp, tau_p, int(vtype), len(self.pg), ptr(self.pg)) != 0:
-                raise ValueError("the barostat needs a rectangular box and virial kind 0, 1 or 2")
+                raise ValueError("the barostat needs a periodic box and virial kind 0, 1 or 2")
         self.n = s.n_atoms
 
     def run(self, nsteps):

@@ -96,6 +96,25 @@ def test_npt_constrained_water_matches_oracle(method):
     assert box[0] != w.box[0]
 
 
+@pytest.mark.parametrize("vtype", [2, 1])
+def test_npt_truncated_octahedron_matches_oracle(vtype):
+    """a truncated-octahedron SETTLE water box under the barostat: every box
+    vector scaled (berendsen_barostat.rs:40-48), the volume the box matrix's
+    determinant (configuration.rs:86-89)"""
+    w = gen.water_truncoct(5, seed=3)
+    w.barostat = (0.06102, 4.575e-4, 0.5, vtype)
+    n = 15
+    md = oracle.MD(oracle.OracleSystem(w), use_settle=True)
+    e_ref = md.run(n)
+    x_ref, _, _ = md.state()
+    e, x, v, box, p = device_md(w, n)
+    for c in range(4):
+        assert rel(e[:, c], e_ref[:, c]).max() < 1e-10, (c, rel(e[:, c], e_ref[:, c]).max())
+    assert np.abs(x - x_ref).max() < 1e-10
+    assert rel(box, md.box()).max() < 1e-13 and rel(p, md.pressure()) < 1e-9, (p, md.pressure())
+    assert box[0] != np.asarray(w.box_matrix, float).reshape(3, 3)[0, 0]
+
+
 def test_npt_mixed_precision_follows_fp64():
     s, d = tn.with_npt()
     _, _, _, b64, p64 = device_md(s, 20)
